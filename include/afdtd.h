@@ -1,0 +1,256 @@
+/*
+ * afdtd.h — C ABI of the MI355X acoustic FDTD engine (libafdtd.so).
+ *
+ * Part 1 declares the 26 functions of the reference's uniform-grid FDTD ABI
+ * with the same names, signatures, return codes and ownership rules, so a
+ * caller that binds csrc/include/acoustic_fdtd.h (reference) can load this
+ * library instead.  Each declaration cites the reference declaration it
+ * replaces (csrc/include/acoustic_fdtd.h:line) and the definition whose
+ * behaviour it reproduces (csrc/kernels/acoustic_fdtd.cpp:line).
+ *
+ * Part 2 declares extensions under new names (fdtd_create_ex, point sources,
+ * receiver recording, energy audit, batched rooms, adjoint).  The reference
+ * has no counterpart for them; the originals keep their exact semantics.
+ *
+ * Grid conventions (reference csrc/include/acoustic_fdtd.h:65-68):
+ *   flat cell index = k + Nz*(j + Ny*i), z fastest.
+ *   Vx packed (Nx-1)*Ny*Nz, index k + Nz*(j + Ny*i)
+ *   Vy packed Nx*(Ny-1)*Nz, index k + Nz*(j + (Ny-1)*i)
+ *   Vz packed Nx*Ny*(Nz-1), index k + (Nz-1)*(j + Ny*i)
+ * (reference csrc/include/acoustic_fdtd.h:418-437).
+ *
+ * Device execution: every handle owns one HIP stream.  fdtd_step enqueues its
+ * kernels and synchronises once before returning (it must report
+ * FDTD_ERR_UNSTABLE).  Every get_* / sample_* call is stream-ordered and
+ * synchronous on return.  Handles are not thread-safe (as in the reference).
+ */
+#ifndef AFDTD_H
+#define AFDTD_H
+
+#include <stdint.h>
+
+#if defined(_WIN32)
+#  define SK_API
+#else
+#  define SK_API __attribute__((visibility("default")))
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Cell types — reference csrc/include/acoustic_fdtd.h:92-98 */
+#define FDTD_AIR    ((uint8_t)0)
+#define FDTD_WALL   ((uint8_t)1)
+#define FDTD_PLATE  ((uint8_t)2)
+#define FDTD_PML    ((uint8_t)3)
+
+/* Error codes — reference csrc/include/acoustic_fdtd.h:102-105 */
+#define FDTD_OK            0
+#define FDTD_ERR_NULL     (-1)
+#define FDTD_ERR_DIM      (-2)
+#define FDTD_ERR_UNSTABLE (-3)
+/* Extension error codes (never returned by the 26 reference functions). */
+#define FDTD_ERR_DEVICE   (-4)   /* HIP runtime error                        */
+#define FDTD_ERR_ARG      (-5)   /* unsupported option combination           */
+
+typedef struct AcousticFDTDState AcousticFDTDState;
+
+/* ═════════════════════ Part 1: the reference ABI ═══════════════════════ */
+
+/* acoustic_fdtd.h:137 / acoustic_fdtd.cpp:327-768 */
+SK_API AcousticFDTDState* fdtd_create(
+    int Nx, int Ny, int Nz, float dx, float c, float rho_air,
+    const uint8_t* cell_type, int plate_iz, const uint8_t* plate_active,
+    float plate_mass_density, float plate_stiffness_D,
+    float plate_alpha_M, float plate_beta_K, int n_pml);
+
+/* acoustic_fdtd.h:153 / acoustic_fdtd.cpp:770-794 */
+SK_API void fdtd_destroy(AcousticFDTDState* st);
+
+/* acoustic_fdtd.h:170 / acoustic_fdtd.cpp:798-830 */
+SK_API int fdtd_set_bridge_sources(AcousticFDTDState* st, int n_cells,
+                                   const int* cell_indices, const float* weights);
+
+/* acoustic_fdtd.h:184 / acoustic_fdtd.cpp:874-885 (disabled: returns -3) */
+SK_API int fdtd_inject_bridge(AcousticFDTDState* st, float signal_val,
+                              float signal_ddt, float force_scale);
+
+/* acoustic_fdtd.h:207 / acoustic_fdtd.cpp:1699-1721 */
+SK_API int fdtd_inject_bridge_drive(AcousticFDTDState* st,
+                                    const float* cell_drives, float force_scale);
+
+/* acoustic_fdtd.h:220 / acoustic_fdtd.cpp:1723-1742 */
+SK_API int fdtd_inject_plate_force(AcousticFDTDState* st, int n_plate,
+                                   const int* plate_indices, const float* weights,
+                                   float total_force_N);
+
+/* acoustic_fdtd.h:248 / acoustic_fdtd.cpp:1064-1077 */
+SK_API int fdtd_set_face_fractions(AcousticFDTDState* st, const float* vx_frac,
+                                   const float* vy_frac, const float* vz_frac);
+
+/* acoustic_fdtd.h:257 / acoustic_fdtd.cpp:1744-1760 */
+SK_API int fdtd_sample_plate_weighted(const AcousticFDTDState* st, int n_plate,
+                                      const int* plate_indices, const float* weights,
+                                      float* out_w);
+
+/* acoustic_fdtd.h:281 / acoustic_fdtd.cpp:1567-1596 */
+SK_API int fdtd_precompute_pressure_samplers(const AcousticFDTDState* st, int n_rec,
+                                             const float* rec_xyz,
+                                             int* out_idx8, float* out_wgt8);
+
+/* acoustic_fdtd.h:296 / acoustic_fdtd.cpp:1647-1661 */
+SK_API int fdtd_precompute_velocity_samplers(const AcousticFDTDState* st, int n_rec,
+                                             const float* rec_xyz,
+                                             int* out_vx_idx8, float* out_vx_wgt8,
+                                             int* out_vy_idx8, float* out_vy_wgt8,
+                                             int* out_vz_idx8, float* out_vz_wgt8);
+
+/* acoustic_fdtd.h:308 / acoustic_fdtd.cpp:1663-1676 */
+SK_API int fdtd_sample_pressure_precomputed(const AcousticFDTDState* st, int n_rec,
+                                            const int* idx8, const float* wgt8,
+                                            float* out_p);
+
+/* acoustic_fdtd.h:319 / acoustic_fdtd.cpp:1678-1697 */
+SK_API int fdtd_sample_velocity_precomputed(const AcousticFDTDState* st, int n_rec,
+                                            const int* vx_idx8, const float* vx_wgt8,
+                                            const int* vy_idx8, const float* vy_wgt8,
+                                            const int* vz_idx8, const float* vz_wgt8,
+                                            float* out_vx, float* out_vy, float* out_vz);
+
+/* acoustic_fdtd.h:339 / acoustic_fdtd.cpp:1272-1290 */
+SK_API int fdtd_step(AcousticFDTDState* st, int n_steps);
+
+/* acoustic_fdtd.h:342 / acoustic_fdtd.cpp:1294 */
+SK_API int fdtd_get_step_count(const AcousticFDTDState* st);
+
+/* acoustic_fdtd.h:345 / acoustic_fdtd.cpp:1295 */
+SK_API float fdtd_get_dt(const AcousticFDTDState* st);
+
+/* acoustic_fdtd.h:348 / acoustic_fdtd.cpp:1297-1310 */
+SK_API int fdtd_reset(AcousticFDTDState* st);
+
+/* acoustic_fdtd.h:359 / acoustic_fdtd.cpp:1314-1321 */
+SK_API int fdtd_get_pressure_field(const AcousticFDTDState* st, float* out, int out_len);
+
+/* acoustic_fdtd.h:372 / acoustic_fdtd.cpp:1323-1330 */
+SK_API int fdtd_get_plate_displacement(const AcousticFDTDState* st, float* out,
+                                       int out_len);
+
+/* acoustic_fdtd.h:394 / acoustic_fdtd.cpp:1332-1353 */
+SK_API int fdtd_sample_plate_displacement_batch(const AcousticFDTDState* st,
+                                                int n_points, const int* saddle_idx4,
+                                                const float* saddle_wgt4, float* out_w);
+
+/* acoustic_fdtd.h:411 / acoustic_fdtd.cpp:1355-1395 */
+SK_API int fdtd_sample_pressure(const AcousticFDTDState* st, int n_rec,
+                                const float* rec_xyz, float* out_p);
+
+/* acoustic_fdtd.h:440 / acoustic_fdtd.cpp:1399-1406 */
+SK_API void fdtd_get_velocity_dims(const AcousticFDTDState* st,
+                                   int* Nvx, int* Nvy, int* Nvz);
+
+/* acoustic_fdtd.h:448-456 / acoustic_fdtd.cpp:1408-1430 */
+SK_API int fdtd_get_velocity_x(const AcousticFDTDState* st, float* out, int out_len);
+SK_API int fdtd_get_velocity_y(const AcousticFDTDState* st, float* out, int out_len);
+SK_API int fdtd_get_velocity_z(const AcousticFDTDState* st, float* out, int out_len);
+
+/* acoustic_fdtd.h:475 / acoustic_fdtd.cpp:1523-1538 */
+SK_API int fdtd_sample_velocity(const AcousticFDTDState* st, int n_rec,
+                                const float* rec_xyz,
+                                float* out_vx, float* out_vy, float* out_vz);
+
+/* acoustic_fdtd.h:499 / acoustic_fdtd.cpp:1540-1563 */
+SK_API int fdtd_get_surface_emission(const AcousticFDTDState* st, int n_surf,
+                                     const float* surf_xyz, const float* surf_normals,
+                                     float* out_P, float* out_vn);
+
+/* ═════════════════════ Part 2: extensions (no reference counterpart) ════ */
+
+/* Stencil order.  4 = the reference's Fornberg 9/8, -1/24 staggered stencil
+ * with 2nd-order edge faces (acoustic_fdtd.cpp:889-890, :1107-1112).
+ * 2 = the Yee stencil of the reference's AMR order-2 path
+ * (acoustic_amr.cpp:1146-1154) on the uniform grid. */
+#define FDTD_ORDER_4 4
+#define FDTD_ORDER_2 2
+
+/* Domain-boundary model. */
+#define FDTD_BOUNDARY_RIGID_PML  0  /* reference: rigid domain edges + n_pml PML   */
+#define FDTD_BOUNDARY_IMPEDANCE  1  /* locally reacting walls, reflection refl[6]  */
+
+/* Field storage type (arithmetic is always FP32). */
+#define FDTD_STORE_F32  0
+#define FDTD_STORE_BF16 1
+#define FDTD_STORE_F16  2
+
+/* Kernel family selection. */
+#define FDTD_KERNEL_AUTO    0
+#define FDTD_KERNEL_GENERIC 1  /* separate velocity / pressure sweeps, all features */
+#define FDTD_KERNEL_FUSED   2  /* fused 2.5-D plane-marching sweep                   */
+
+typedef struct FdtdOptions {
+    int          struct_size;    /* sizeof(FdtdOptions)                          */
+    int          stencil_order;  /* FDTD_ORDER_4 (default) or FDTD_ORDER_2        */
+    float        courant;        /* <= 0: reference dt = 0.77*dx/(c*sqrt3);
+                                    > 0: dt = courant*dx/c_max                   */
+    int          boundary;       /* FDTD_BOUNDARY_*                               */
+    float        refl[6];        /* reflection coefficient per domain side
+                                    (x-, x+, y-, y+, z-, z+), impedance only       */
+    const float* rho;            /* per-cell density (N) or NULL = rho_air        */
+    const float* bulk;           /* per-cell bulk modulus (N) or NULL = rho*c^2   */
+    int          storage;        /* FDTD_STORE_*                                  */
+    int          kernel;         /* FDTD_KERNEL_*                                 */
+    int          tile_y;         /* fused-kernel tile rows (0 = autotuned)        */
+    int          seg_x;          /* fused-kernel x-segment length (0 = autotuned) */
+} FdtdOptions;
+
+SK_API void fdtd_default_options(FdtdOptions* opt);
+
+/* fdtd_create with the options above.  opt == NULL gives fdtd_create. */
+SK_API AcousticFDTDState* fdtd_create_ex(
+    int Nx, int Ny, int Nz, float dx, float c, float rho_air,
+    const uint8_t* cell_type, int plate_iz, const uint8_t* plate_active,
+    float plate_mass_density, float plate_stiffness_D,
+    float plate_alpha_M, float plate_beta_K, int n_pml,
+    const FdtdOptions* opt);
+
+/* Point pressure sources: after the pressure update of the step with global
+ * index n (0-based step_count before the step), P[cell_idx[s]] +=
+ * signals[s*n_samples + n] for n < n_samples (additive, as
+ * acoustic_amr.cpp:1227-1244).  Sources on solid cells are rejected. */
+SK_API int fdtd_set_point_sources(AcousticFDTDState* st, int n_src,
+                                  const int* cell_idx, const float* signals,
+                                  int n_samples);
+
+/* Receivers at grid coordinates, gathered on the device after every step with
+ * the 8-tap trilinear stencil of fdtd_precompute_pressure_samplers. */
+SK_API int fdtd_set_receivers(AcousticFDTDState* st, int n_rec, const float* rec_xyz);
+/* Start (or restart) recording up to max_steps steps of receiver traces. */
+SK_API int fdtd_record_receivers(AcousticFDTDState* st, int max_steps);
+/* Copy recorded traces (n_recorded x n_rec, row = step). */
+SK_API int fdtd_get_receiver_traces(const AcousticFDTDState* st, float* out,
+                                    int out_len, int* n_recorded);
+
+/* FP64 energy audit: out[0] = sum p^2/(2K) dV, out[1] = sum rho_f v^2/2 dV
+ * (velocities at the current half step). */
+SK_API int fdtd_energy(const AcousticFDTDState* st, double* out2);
+
+/* Upload a full state (P then packed Vx, Vy, Vz) — used by tests and the
+ * checkpoint/recompute adjoint. */
+SK_API int fdtd_set_state(AcousticFDTDState* st, const float* P, const float* vx,
+                          const float* vy, const float* vz);
+
+/* Introspection: which kernel family runs, dt, stencil order. */
+SK_API int fdtd_get_kernel_family(const AcousticFDTDState* st);
+SK_API const char* fdtd_last_error(void);
+
+/* Timing helper: run n_steps with HIP events around the step kernels and
+ * return the average duration of one launch of the dominant (stencil) kernel
+ * in *ms_per_launch and the wall time of the whole call in *ms_total. */
+SK_API int fdtd_step_timed(AcousticFDTDState* st, int n_steps,
+                           float* ms_total, float* ms_stencil_avg);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* AFDTD_H */

@@ -1,0 +1,13 @@
+"""Drop-in for the reference module ``_spectral_kernels`` on the FDTD path.
+
+The reference bridge imports ``from _spectral_kernels import AcousticFDTD``
+(acoustic_fdtd_bridge.py:81-86) and the FDTD_* constants
+(pybind_kernels.cpp:9213-9216); with this directory on ``sys.path`` the same
+import resolves to the MI355X engine.  Only the uniform FDTD path is provided.
+"""
+from afdtd import AcousticFDTD  # noqa: F401
+
+FDTD_AIR = 0
+FDTD_WALL = 1
+FDTD_PLATE = 2
+FDTD_PML = 3

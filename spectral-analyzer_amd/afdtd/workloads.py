@@ -1,0 +1,127 @@
+"""Synthetic workloads of the benchmark configurations (seeded, reproducible).
+
+Conventions recorded here are the ones the benchmark and the tests use:
+  * heterogeneous media: g ~ N(0,1) per cell, smoothed by a 3x3x3 box
+    filter (scipy.ndimage.uniform_filter, mode 'nearest'), renormalised to
+    unit variance; rho = rho0 exp(0.1 g_rho), K = rho0 c0^2 exp(0.1 g_K) with
+    two independent fields;
+  * sources: differentiated Gaussian, sigma = 8 steps, n0 = 48, 1 Pa, at
+    seeded interior cells (margin 8 cells);
+  * receivers: seeded interior cell centres (margin 8 cells).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from typing import List, Optional
+
+import numpy as np
+
+from .signals import differentiated_gaussian
+
+RHO0 = 1.2
+C0 = 343.0
+
+
+@dataclass
+class Workload:
+    name: str
+    Nx: int
+    Ny: int
+    Nz: int
+    dx: float
+    steps: int
+    order: int
+    courant: Optional[float]
+    boundary: str
+    refl: List[float]
+    n_pml: int = 0
+    rho: Optional[np.ndarray] = None
+    bulk: Optional[np.ndarray] = None
+    src: List[int] = field(default_factory=list)
+    rec: Optional[np.ndarray] = None
+    sigma: float = 8.0
+    n0: float = 48.0
+
+    @property
+    def cells(self) -> int:
+        return self.Nx * self.Ny * self.Nz
+
+    @property
+    def hetero(self) -> bool:
+        return self.rho is not None or self.bulk is not None
+
+    @property
+    def bytes_per_cell_update(self) -> int:
+        """Algorithmic HBM bytes per cell-update of the FP32 step: P, Vx, Vy,
+        Vz read and written once (32 B) plus one read of rho and K (8 B)
+        when the medium is heterogeneous."""
+        return 40 if self.hetero else 32
+
+    def signals(self, n: int) -> np.ndarray:
+        return np.stack([differentiated_gaussian(n, self.sigma, self.n0, 1.0)
+                         for _ in self.src]) if self.src else np.zeros((0, n), np.float32)
+
+    def solver_kwargs(self) -> dict:
+        return dict(order=self.order, courant=self.courant, boundary=self.boundary,
+                    refl=self.refl, rho=self.rho, bulk=self.bulk)
+
+
+def _smoothed_lognormal_field(rng, shape, sigma: float) -> np.ndarray:
+    from scipy.ndimage import uniform_filter
+    g = rng.standard_normal(shape, dtype=np.float32)
+    g = uniform_filter(g, size=3, mode="nearest")
+    g /= max(float(g.std()), 1e-12)
+    return np.exp(sigma * g).astype(np.float32)
+
+
+def _interior_cells(rng, Nx, Ny, Nz, n, margin=8):
+    i = rng.integers(margin, Nx - margin, n)
+    j = rng.integers(margin, Ny - margin, n)
+    k = rng.integers(margin, Nz - margin, n)
+    return i, j, k
+
+
+def config1(seed: int = 1, parity: bool = False) -> Workload:
+    """64^3, dx = 1 cm, air (rho 1.2, c 343).  As written: order 2, Courant
+    0.5, impedance walls R = 0.9.  parity=True: the reference's own scheme
+    (order 4, reference dt, rigid walls)."""
+    rng = np.random.default_rng(seed)
+    N = 64
+    i, j, k = _interior_cells(rng, N, N, N, 1)
+    ri, rj, rk = _interior_cells(rng, N, N, N, 16)
+    w = Workload("config1-64cube" + ("-parity" if parity else ""), N, N, N, 0.01, 2000,
+                 4 if parity else 2, None if parity else 0.5,
+                 "rigid" if parity else "impedance", [1.0] * 6 if parity else [0.9] * 6)
+    w.src = [int(k[0] + N * (j[0] + N * i[0]))]
+    w.rec = np.stack([ri, rj, rk], 1).astype(np.float32)
+    return w
+
+
+def config2(seed: int = 2, Nx=512, Ny=384, Nz=256, steps=4000) -> Workload:
+    """Single large room: 512x384x256, order 2, Courant 0.55, heterogeneous
+    rho and K, per-side R ~ U[0.6, 0.98], 4 sources, 64 receivers."""
+    rng = np.random.default_rng(seed)
+    shape = (Nx, Ny, Nz)
+    rho = (RHO0 * _smoothed_lognormal_field(rng, shape, 0.1)).ravel()
+    bulk = (RHO0 * C0 * C0 * _smoothed_lognormal_field(rng, shape, 0.1)).ravel()
+    refl = [float(x) for x in rng.uniform(0.6, 0.98, 6)]
+    w = Workload(f"config2-room-{Nx}x{Ny}x{Nz}-hetero-order2", Nx, Ny, Nz, 0.01, steps, 2,
+                 0.55, "impedance", refl, rho=rho, bulk=bulk)
+    i, j, k = _interior_cells(rng, Nx, Ny, Nz, 4)
+    w.src = [int(k[q] + Nz * (j[q] + Ny * i[q])) for q in range(4)]
+    ri, rj, rk = _interior_cells(rng, Nx, Ny, Nz, 64)
+    w.rec = np.stack([ri, rj, rk], 1).astype(np.float32)
+    return w
+
+
+def config2_order4_pml(seed: int = 2, Nx=512, Ny=384, Nz=256, steps=4000) -> Workload:
+    """The config-2 grid with the reference scheme: homogeneous air, order 4,
+    reference dt, PML 10 cells."""
+    rng = np.random.default_rng(seed)
+    w = Workload(f"room-{Nx}x{Ny}x{Nz}-order4-pml10", Nx, Ny, Nz, 0.01, steps, 4, None,
+                 "rigid", [1.0] * 6, n_pml=10)
+    i, j, k = _interior_cells(rng, Nx, Ny, Nz, 4, margin=16)
+    w.src = [int(k[q] + Nz * (j[q] + Ny * i[q])) for q in range(4)]
+    ri, rj, rk = _interior_cells(rng, Nx, Ny, Nz, 64, margin=16)
+    w.rec = np.stack([ri, rj, rk], 1).astype(np.float32)
+    return w

@@ -1,0 +1,341 @@
+// afdtd_aux.hip — the small kernels around the stencil: Kirchhoff plate,
+// plate-velocity injection, point sources + receiver gather, the sampled
+// stability check, API gathers, storage conversions and the FP64 energy audit.
+#include "afdtd_common.cuh"
+
+namespace {
+
+// Kirchhoff plate leapfrog over the active nodes (acoustic_fdtd.cpp:984-1053).
+// The 13-tap biharmonic uses the precomputed redirected indices
+// (acoustic_fdtd.cpp:673-719) and sums in tap order.
+__global__ void __launch_bounds__(256)
+k_plate_step(int nact, const int* __restrict__ act, const int* __restrict__ b13,
+             const int* __restrict__ above, const int* __restrict__ below,
+             const float* __restrict__ P, const float* __restrict__ w,
+             const float* __restrict__ wp, float* __restrict__ wt,
+             const float* __restrict__ ext, float a, float bhc, float vd, float bk,
+             int* status, int step_index) {
+    if (status[0]) return;
+    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nact) return;
+    const float W13[13] = {20.f, -8.f, -8.f, -8.f, -8.f, 2.f, 2.f, 2.f, 2.f, 1.f, 1.f, 1.f, 1.f};
+    const int p = act[n];
+    const int* nb = b13 + 13 * n;
+    float bh = 0.0f;
+    for (int t = 0; t < 13; ++t) bh += W13[t] * w[nb[t]];
+    float Pa = above[n] >= 0 ? P[above[n]] : 0.0f;
+    float Pb = below[n] >= 0 ? P[below[n]] : 0.0f;
+    float fb = ext[p];
+    float bhp = 0.0f;
+    for (int t = 0; t < 13; ++t) bhp += W13[t] * wp[nb[t]];
+    float vp = w[p] - wp[p];
+    float wn = w[p] + vd * vp + a * ((Pb - Pa) + fb) - bhc * bh - bk * (bh - bhp);
+    if (!isfinite(wn) || fabsf(wn) > 0.02f) {
+        // Plate guard (acoustic_fdtd.cpp:1032-1040): the step fails before the
+        // pressure update; later kernels see the flag and skip.
+        if (atomicCAS(&status[0], 0, 2) == 0) status[1] = step_index;
+        return;
+    }
+    wt[p] = wn;
+}
+
+// Plate velocity as the normal velocity of the two plate-layer Vz faces
+// (acoustic_fdtd.cpp:1158-1163).  vzb/vza are padded-layout indices or -1.
+__global__ void __launch_bounds__(256)
+k_plate_inject(int nact, const int* __restrict__ act, const int* __restrict__ vzb,
+               const int* __restrict__ vza, const float* __restrict__ w,
+               const float* __restrict__ wp, float inv_dt, float* __restrict__ Vz,
+               const int* status) {
+    if (status[0]) return;
+    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nact) return;
+    int p = act[n];
+    float v = (w[p] - wp[p]) * inv_dt;
+    if (vzb[n] >= 0) Vz[vzb[n]] = v;
+    if (vza[n] >= 0) Vz[vza[n]] = v;
+}
+
+// After the pressure update of step `step_index`: point sources (sequential,
+// additive) then the 8-tap receiver gather in the summation order of
+// fdtd_sample_pressure_precomputed (acoustic_fdtd.cpp:1672-1673).
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_post_step(T* __restrict__ P, int npsrc, const int* __restrict__ pidx,
+            const float* __restrict__ psig, int plen, int nrec,
+            const int* __restrict__ ridx, const float* __restrict__ rwgt,
+            float* __restrict__ trace, int trace_row, int step_index, const int* status) {
+    typedef Store<T> S;
+    if (status[0]) return;
+    if (threadIdx.x == 0 && step_index < plen) {
+        for (int s = 0; s < npsrc; ++s) {
+            int c = pidx[s];
+            S::st(P, c, S::ld(P, c) + psig[(size_t)s * plen + step_index]);
+        }
+    }
+    __threadfence_block();
+    __syncthreads();
+    if (trace == nullptr || trace_row < 0) return;
+    for (int r = threadIdx.x; r < nrec; r += blockDim.x) {
+        const int* id = ridx + 8 * r;
+        const float* w = rwgt + 8 * r;
+        float v = w[0] * S::ld(P, id[0]) + w[1] * S::ld(P, id[1]) + w[2] * S::ld(P, id[2])
+                + w[3] * S::ld(P, id[3]) + w[4] * S::ld(P, id[4]) + w[5] * S::ld(P, id[5])
+                + w[6] * S::ld(P, id[6]) + w[7] * S::ld(P, id[7]);
+        trace[(size_t)trace_row * nrec + r] = v;
+    }
+}
+
+// Sampled stability check (acoustic_fdtd.cpp:1245-1268): cells start,
+// start+1013, ...; non-finite or |p| > 2e3 Pa fails the step.
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_stability(const T* __restrict__ P, int N, int start, int* status, int step_index) {
+    typedef Store<T> S;
+    if (status[0]) return;
+    long m = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long q = start + m * 1013;
+    if (q >= N) return;
+    float p = S::ld(P, (int)q);
+    if (!isfinite(p) || p > 2e3f || p < -2e3f) {
+        if (atomicCAS(&status[0], 0, 3) == 0) status[1] = step_index;
+    }
+}
+
+__global__ void k_fill(float* p, size_t n, float v) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// Sequential accumulation ext[idx[i] / div] += (scale * val[i]) * (1/dx^2)
+// performed by one thread, in index order, as the reference's loops
+// (acoustic_fdtd.cpp:1716-1719, :1735-1740).  `val` already carries the
+// per-entry factor; `mask` (optional) skips inactive plate nodes.
+__global__ void k_accumulate(float* ext, const int* idx, const float* val, int n, float scale,
+                             float inv_dx2, int div, const uint8_t* mask, int nmax,
+                             const int* status) {
+    if (status[0] || threadIdx.x != 0 || blockIdx.x != 0) return;
+    for (int i = 0; i < n; ++i) {
+        int p = idx[i] / div;
+        if (p < 0 || p >= nmax) continue;
+        if (mask && !mask[p]) continue;
+        ext[p] += scale * val[i] * inv_dx2;
+    }
+}
+
+template <typename T>
+__global__ void k_gather(const T* __restrict__ F, const int* __restrict__ idx,
+                         const float* __restrict__ wgt, int n, int taps, int zero_start,
+                         float scale, float* __restrict__ out) {
+    typedef Store<T> S;
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const int* id = idx + taps * r;
+    const float* w = wgt + taps * r;
+    float v;
+    if (zero_start) {
+        v = 0.0f;
+        for (int t = 0; t < taps; ++t) v += w[t] * (S::exact ? S::ld(F, id[t]) : S::ld(F, id[t]) * scale);
+    } else {
+        v = w[0] * (S::exact ? S::ld(F, id[0]) : S::ld(F, id[0]) * scale);
+        for (int t = 1; t < taps; ++t) v = v + w[t] * (S::exact ? S::ld(F, id[t]) : S::ld(F, id[t]) * scale);
+    }
+    out[r] = v;
+}
+
+template <typename T>
+__global__ void k_to_f32(const T* __restrict__ src, float* __restrict__ dst, size_t n, float scale) {
+    typedef Store<T> S;
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = S::ld(src, (int)i) * scale;
+}
+
+template <typename T>
+__global__ void k_from_f32(const float* __restrict__ src, T* __restrict__ dst, size_t n, float scale) {
+    typedef Store<T> S;
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) S::st(dst, (int)i, src[i] * scale);
+}
+
+// FP64 energy audit: out[0] += sum p^2/(2K) dV (non-solid cells),
+// out[1] += sum rho_f v^2/2 dV (valid faces).  Block reduction in double.
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_energy(StencilParams p, StencilAux a, const T* __restrict__ P, const T* __restrict__ Vx,
+         const T* __restrict__ Vy, const T* __restrict__ Vz, double dV, double* out) {
+    typedef Store<T> S;
+    __shared__ double red[2][256];
+    const int tid = threadIdx.x;
+    const int N = p.Nx * p.Ny * p.Nz;
+    double ep = 0.0, ek = 0.0;
+    for (int c = blockIdx.x * blockDim.x + tid; c < N; c += gridDim.x * blockDim.x) {
+        int k = c % p.Nz, t = c / p.Nz, j = t % p.Ny, i = t / p.Ny;
+        if (!(p.has_solid && is_solid(a.ct[c]))) {
+            double K = p.hetero ? (double)a.K[c] : (double)p.K0;
+            double pc = (double)S::ld(P, c);
+            ep += pc * pc / (2.0 * K);
+        }
+        double rc = p.hetero ? (double)a.rho[c] : (double)p.rho_air;
+        double vs = S::exact ? 1.0 : (double)p.inv_sV;
+        if (i < p.Nx - 1) {
+            double r = p.hetero ? 0.5 * (rc + (double)a.rho[c + p.Ny * p.Nz]) : rc;
+            double v = (double)S::ld(Vx, c) * vs;
+            ek += 0.5 * r * v * v;
+        }
+        if (j < p.Ny - 1) {
+            double r = p.hetero ? 0.5 * (rc + (double)a.rho[c + p.Nz]) : rc;
+            double v = (double)S::ld(Vy, c) * vs;
+            ek += 0.5 * r * v * v;
+        }
+        if (k < p.Nz - 1) {
+            double r = p.hetero ? 0.5 * (rc + (double)a.rho[c + 1]) : rc;
+            double v = (double)S::ld(Vz, c) * vs;
+            ek += 0.5 * r * v * v;
+        }
+    }
+    red[0][tid] = ep; red[1][tid] = ek;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (tid < s) { red[0][tid] += red[0][tid + s]; red[1][tid] += red[1][tid + s]; }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        atomicAdd(&out[0], red[0][0] * dV);
+        atomicAdd(&out[1], red[1][0] * dV);
+    }
+}
+
+inline unsigned blocks_for(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
+
+}  // namespace
+
+hipError_t launch_plate_step(AcousticFDTDState* st, int step_index) {
+    if (st->nact <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_plate_step, dim3(blocks_for(st->nact, 256)), dim3(256), 0, st->stream,
+                       st->nact, st->d_act, st->d_b13, st->d_above, st->d_below,
+                       (const float*)st->P[st->cur], st->d_w[0], st->d_w[1], st->d_w[2],
+                       st->d_ext, st->plate_a, st->plate_bh, st->plate_vd, st->plate_bk,
+                       st->d_status, step_index);
+    return hipGetLastError();
+}
+
+hipError_t launch_plate_inject(AcousticFDTDState* st) {
+    if (st->nact <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_plate_inject, dim3(blocks_for(st->nact, 256)), dim3(256), 0,
+                       st->stream, st->nact, st->d_act, st->d_vzb, st->d_vza, st->d_w[0],
+                       st->d_w[1], 1.0f / st->dt, (float*)st->V[2], st->d_status);
+    return hipGetLastError();
+}
+
+hipError_t launch_post_step(AcousticFDTDState* st, int step_index, int trace_row) {
+    if (st->npsrc == 0 && (st->nrec == 0 || trace_row < 0)) return hipSuccess;
+    float* tr = (st->nrec > 0 && trace_row >= 0) ? st->d_trace : nullptr;
+    void* P = st->P[st->cur];
+#define POST(T)                                                                         \
+    hipLaunchKernelGGL(k_post_step<T>, dim3(1), dim3(256), 0, st->stream, (T*)P,        \
+                       st->npsrc, st->d_pidx, st->d_psig, st->plen, st->nrec, st->d_ridx, \
+                       st->d_rwgt, tr, trace_row, step_index, st->d_status)
+    switch (st->storage) {
+        case FDTD_STORE_BF16: POST(bf16_t); break;
+        case FDTD_STORE_F16: POST(f16_t); break;
+        default: POST(float);
+    }
+#undef POST
+    return hipGetLastError();
+}
+
+hipError_t launch_stability_check(AcousticFDTDState* st, int start, int step_index) {
+    long cnt = (st->N > start) ? ((long)(st->N - start) + 1012) / 1013 : 0;
+    if (cnt <= 0) return hipSuccess;
+    const void* P = st->P[st->cur];
+#define CHK(T)                                                                          \
+    hipLaunchKernelGGL(k_stability<T>, dim3(blocks_for(cnt, 256)), dim3(256), 0, st->stream, \
+                       (const T*)P, st->N, start, st->d_status, step_index)
+    switch (st->storage) {
+        case FDTD_STORE_BF16: CHK(bf16_t); break;
+        case FDTD_STORE_F16: CHK(f16_t); break;
+        default: CHK(float);
+    }
+#undef CHK
+    return hipGetLastError();
+}
+
+hipError_t launch_fill(float* p, size_t n, float v, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill, dim3(blocks_for(n, 256)), dim3(256), 0, s, p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_accumulate(float* ext, const int* idx, const float* val, int n, float scale,
+                              float inv_dx2, int div, const uint8_t* mask, int nmax,
+                              const int* status, hipStream_t s) {
+    hipLaunchKernelGGL(k_accumulate, dim3(1), dim3(64), 0, s, ext, idx, val, n, scale, inv_dx2,
+                       div, mask, nmax, status);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather(const void* F, int storage, float scale, const int* idx,
+                          const float* wgt, int n, int taps, int zero_start, float* out,
+                          hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+#define G(T)                                                                              \
+    hipLaunchKernelGGL(k_gather<T>, dim3(blocks_for(n, 256)), dim3(256), 0, s, (const T*)F, \
+                       idx, wgt, n, taps, zero_start, scale, out)
+    switch (storage) {
+        case FDTD_STORE_BF16: G(bf16_t); break;
+        case FDTD_STORE_F16: G(f16_t); break;
+        default: G(float);
+    }
+#undef G
+    return hipGetLastError();
+}
+
+hipError_t launch_to_f32(const void* src, int storage, float scale, float* dst, size_t n,
+                           hipStream_t s) {
+    if (n == 0) return hipSuccess;
+#define C(T)                                                                               \
+    hipLaunchKernelGGL(k_to_f32<T>, dim3(blocks_for(n, 256)), dim3(256), 0, s, (const T*)src, \
+                       dst, n, scale)
+    switch (storage) {
+        case FDTD_STORE_BF16: C(bf16_t); break;
+        case FDTD_STORE_F16: C(f16_t); break;
+        default: C(float);
+    }
+#undef C
+    return hipGetLastError();
+}
+
+hipError_t launch_from_f32(const float* src, void* dst, int storage, float scale, size_t n,
+                             hipStream_t s) {
+    if (n == 0) return hipSuccess;
+#define C(T)                                                                                \
+    hipLaunchKernelGGL(k_from_f32<T>, dim3(blocks_for(n, 256)), dim3(256), 0, s, src, (T*)dst, \
+                       n, scale)
+    switch (storage) {
+        case FDTD_STORE_BF16: C(bf16_t); break;
+        case FDTD_STORE_F16: C(f16_t); break;
+        default: C(float);
+    }
+#undef C
+    return hipGetLastError();
+}
+
+hipError_t launch_energy(AcousticFDTDState* st, double* d_out) {
+    StencilAux a;
+    a.ct = st->d_ct; a.fx = st->d_fx; a.fy = st->d_fy; a.fz = st->d_fz;
+    a.rho = st->d_rho; a.kap = st->d_kap; a.K = st->d_K;
+    a.alpha = st->d_alpha; a.pdamp = st->d_pdamp; a.status = st->d_status;
+    double dV = (double)st->dx * (double)st->dx * (double)st->dx;
+    unsigned nb = blocks_for(st->N, 256);
+    if (nb > 2048) nb = 2048;
+#define E(T)                                                                                 \
+    hipLaunchKernelGGL(k_energy<T>, dim3(nb), dim3(256), 0, st->stream, st->sp, a,           \
+                       (const T*)st->P[st->cur], (const T*)st->V[0], (const T*)st->V[1],     \
+                       (const T*)st->V[2], dV, d_out)
+    switch (st->storage) {
+        case FDTD_STORE_BF16: E(bf16_t); break;
+        case FDTD_STORE_F16: E(f16_t); break;
+        default: E(float);
+    }
+#undef E
+    return hipGetLastError();
+}

@@ -1,0 +1,69 @@
+// afdtd_common.cuh — device helpers shared by the stencil kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
+#include <stdint.h>
+
+#include "afdtd_internal.h"
+
+// Storage element types.  Arithmetic is always FP32; conversion to 16-bit
+// storage rounds to nearest even.
+typedef __hip_bfloat16 bf16_t;
+typedef _Float16 f16_t;
+
+template <typename T> struct Store;
+template <> struct Store<float> {
+    static __device__ __forceinline__ float ld(const float* p, int i) { return p[i]; }
+    static __device__ __forceinline__ void st(float* p, int i, float v) { p[i] = v; }
+    static constexpr bool exact = true;
+};
+template <> struct Store<bf16_t> {
+    static __device__ __forceinline__ float ld(const bf16_t* p, int i) {
+        return __bfloat162float(p[i]);
+    }
+    static __device__ __forceinline__ void st(bf16_t* p, int i, float v) {
+        p[i] = __float2bfloat16(v);
+    }
+    static constexpr bool exact = false;
+};
+template <> struct Store<f16_t> {
+    static __device__ __forceinline__ float ld(const f16_t* p, int i) { return (float)p[i]; }
+    static __device__ __forceinline__ void st(f16_t* p, int i, float v) { p[i] = (f16_t)v; }
+    static constexpr bool exact = false;
+};
+
+__device__ __forceinline__ int is_solid(uint8_t c) { return c == 1 || c == 2; }
+
+// PML depth of one axis position (acoustic_fdtd.cpp:381-383).
+__device__ __forceinline__ int pml_depth(int i, int n, int np) {
+    return (i < np) ? (np - i) : ((i >= n - np) ? (i - (n - np) + 1) : 0);
+}
+__device__ __forceinline__ int imax3(int a, int b, int c) {
+    int d = a > b ? a : b;
+    return d > c ? d : c;
+}
+
+// Face damping 1 - 0.5*(aL + aR)*dt (acoustic_fdtd.cpp:542).
+__device__ __forceinline__ float face_damp(const float* alpha, int dL, int dR, float dt) {
+    return 1.0f - 0.5f * (alpha[dL] + alpha[dR]) * dt;
+}
+
+// Locally reacting wall update of a boundary cell (order 2, extension):
+// P' = ((P (1 - xi)) - kap*dv) / (1 + xi), xi = sum over touching domain
+// sides of (kap/2) * g_side / sqrt(rho K).
+__device__ __forceinline__ float impedance_update(float P, float dv, float kap, float r,
+                                                  float KK, const float* g, int i, int j,
+                                                  int k, int Nx, int Ny, int Nz) {
+    float z = sqrtf(r * KK);
+    float hk = 0.5f * kap;
+    float xi = 0.0f;
+    if (i == 0) xi = xi + hk * (g[0] / z);
+    if (i == Nx - 1) xi = xi + hk * (g[1] / z);
+    if (j == 0) xi = xi + hk * (g[2] / z);
+    if (j == Ny - 1) xi = xi + hk * (g[3] / z);
+    if (k == 0) xi = xi + hk * (g[4] / z);
+    if (k == Nz - 1) xi = xi + hk * (g[5] / z);
+    float num = P * (1.0f - xi) - kap * dv;
+    return num / (1.0f + xi);
+}

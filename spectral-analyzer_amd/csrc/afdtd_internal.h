@@ -1,0 +1,149 @@
+// afdtd_internal.h — device state and kernel launchers of libafdtd.so.
+//
+// Device layout (HBM): the four fields P, Vx, Vy, Vz share ONE padded layout
+// of N = Nx*Ny*Nz elements indexed by the cell index c = k + Nz*(j + Ny*i)
+// (z fastest, as the reference's IDX3, acoustic_fdtd.cpp:24-27):
+//   Vx[c] = face (i+1/2, j, k), valid for i <= Nx-2 (slot i = Nx-1 stays 0)
+//   Vy[c] = face (i, j+1/2, k), valid for j <= Ny-2
+//   Vz[c] = face (i, j, k+1/2), valid for k <= Nz-2
+// so every stencil tap of every field is a fixed offset from c (+-1, +-Nz,
+// +-Ny*Nz) and one z-run of 64 cells is one coalesced 256-byte wavefront
+// access for all four fields.  The reference's packed staggered layouts
+// (acoustic_fdtd.h:430-437) are produced only at the API boundary, by
+// strided device-to-host copies.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+
+#include "../../include/afdtd.h"
+
+#define AFDTD_CHECK(expr)                                                   \
+    do {                                                                    \
+        hipError_t _e = (expr);                                             \
+        if (_e != hipSuccess) {                                             \
+            afdtd_set_error(std::string(#expr) + ": " + hipGetErrorString(_e)); \
+            return FDTD_ERR_DEVICE;                                         \
+        }                                                                   \
+    } while (0)
+
+void afdtd_set_error(const std::string& msg);
+
+// Per-step constants shared by every stencil kernel (passed by value).
+struct StencilParams {
+    int Nx, Ny, Nz;
+    int order;          // 2 or 4
+    int n_pml;          // PML depth (0 = none)
+    int impedance;      // locally reacting domain walls (order 2)
+    int hetero;         // per-cell rho / kappa arrays present
+    int has_solid;      // cell-type array present (walls / plate cells)
+    int has_closed;     // face-fraction arrays present (closed-face test)
+    int has_frac;       // divergence weighted by face fractions
+    float dt, dx, vel_coeff, bulk_coeff;
+    float C1, C2;
+    float gside[6];     // (1-R)/(1+R) per domain side x-,x+,y-,y+,z-,z+
+    float rho_air, K0;  // homogeneous medium
+    float sV, inv_sV;   // power-of-two velocity storage scale (16-bit storage)
+};
+
+// Auxiliary per-handle device arrays read by the stencils.
+struct StencilAux {
+    const uint8_t* ct;      // cell types (has_solid)
+    const float* fx;        // face fractions, padded layout (has_closed)
+    const float* fy;
+    const float* fz;
+    const float* rho;       // per-cell density (hetero)
+    const float* kap;       // per-cell K*dt/dx (hetero)
+    const float* K;         // per-cell bulk modulus (hetero)
+    const float* alpha;     // alpha(d), d = 0..n_pml
+    const float* pdamp;     // 1/(1+alpha(d) dt)
+    const int* status;      // nonzero: a previous step failed, skip work
+};
+
+struct AcousticFDTDState {
+    int Nx, Ny, Nz, N, Nplate, plate_iz;
+    int Nvx, Nvy, Nvz;
+    int order, boundary, n_pml, storage, family;
+    int hetero, has_solid, has_frac;
+    float dx, dt, c, rho_air;
+    StencilParams sp;
+    hipStream_t stream;
+    hipEvent_t ev0, ev1;
+
+    // Fields (element type per `storage`; bytes per element in esz).
+    int esz;
+    void* P[2];             // P[cur] = P^n
+    int cur;
+    void* V[3];
+    // Conversion staging for non-FP32 storage.
+    float* stage;
+
+    // Stencil aux arrays.
+    uint8_t* d_ct;
+    float *d_fx, *d_fy, *d_fz;
+    float *d_rho, *d_kap, *d_K;
+    float *d_alpha, *d_pdamp;
+    int* d_status;          // [0] code, [1] step index at failure
+    std::vector<uint8_t> h_ct;   // cell types after retyping (host copy)
+
+    // Kirchhoff plate.
+    int nact;
+    float *d_w[3];          // w_curr, w_prev, w_tmp rotate
+    float* d_ext;
+    int *d_act, *d_above, *d_below, *d_b13, *d_vzb, *d_vza;
+    float plate_a, plate_bh, plate_vd, plate_bk;
+    std::vector<uint8_t> h_pa;
+
+    // Bridge sources.
+    int nbsrc;
+    int* d_bidx;
+    float* d_bdrive;        // staging for drives (nbsrc) / plate forces
+    std::vector<int> h_bidx;
+    std::vector<float> h_bwgt;
+    int* d_fidx;            // plate-force staging (indices)
+    int fcap;
+
+    // Point sources.
+    int npsrc, plen;
+    int* d_pidx;
+    float* d_psig;
+
+    // Receivers.
+    int nrec;
+    int* d_ridx;
+    float* d_rwgt;
+    float* d_trace;
+    int trace_cap, trace_n;
+
+    // Scratch for API-side gathers.
+    int* d_gidx;
+    float* d_gwgt;
+    float* d_gout;
+    int gcap;
+
+    int step_count, phase;
+    int fused_tile_y, fused_seg_x;
+};
+
+// Launchers (all enqueue on st->stream; return hipError_t of the launch).
+hipError_t launch_velocity_generic(AcousticFDTDState* st);
+hipError_t launch_pressure_generic(AcousticFDTDState* st);
+hipError_t launch_fused_step(AcousticFDTDState* st);
+bool fused_supported(const AcousticFDTDState* st);
+hipError_t launch_plate_step(AcousticFDTDState* st, int step_index);
+hipError_t launch_plate_inject(AcousticFDTDState* st);
+hipError_t launch_post_step(AcousticFDTDState* st, int step_index, int trace_row);
+hipError_t launch_stability_check(AcousticFDTDState* st, int start, int step_index);
+hipError_t launch_fill(float* p, size_t n, float v, hipStream_t s);
+hipError_t launch_accumulate(float* ext, const int* idx, const float* val, int n,
+                             float scale, float inv_dx2, int div, const uint8_t* mask,
+                             int nmax, const int* status, hipStream_t s);
+hipError_t launch_gather(const void* F, int storage, float scale, const int* idx,
+                         const float* wgt, int n, int taps, int zero_start, float* out,
+                         hipStream_t s);
+hipError_t launch_to_f32(const void* src, int storage, float scale, float* dst, size_t n,
+                         hipStream_t s);
+hipError_t launch_from_f32(const float* src, void* dst, int storage, float scale, size_t n,
+                           hipStream_t s);
+hipError_t launch_energy(AcousticFDTDState* st, double* d_out);
