@@ -38,13 +38,29 @@ def make_workload(name, steps):
     raise SystemExit(f"unknown config {name}")
 
 
-def make_solver(w, storage="f32", kernel="auto"):
+def make_solver(w, storage="f32", kernel="auto", tile_y=0, seg_x=0):
     from afdtd import AcousticFDTD
     ct = np.zeros(w.cells, np.uint8)
     pa = np.zeros(w.Nx * w.Ny, np.uint8)
     return AcousticFDTD(w.Nx, w.Ny, w.Nz, w.dx, c=343.0, rho_air=1.2, cell_type=ct,
                         plate_iz=w.Nz // 2, plate_active=pa, n_pml=w.n_pml,
-                        storage=storage, kernel=kernel, **w.solver_kwargs())
+                        storage=storage, kernel=kernel, tile_y=tile_y, seg_x=seg_x,
+                        **w.solver_kwargs())
+
+
+def copy_calibration(nbytes=2 << 30, reps=20):
+    """Measured copy ceiling: the library's 16-byte-per-lane device copy of a
+    known size (also used to scale the HBM byte counters of profile runs)."""
+    import ctypes as C
+    from afdtd import lib
+    ms = C.c_float()
+    rc = lib().fdtd_copy_calibration(nbytes, reps, C.byref(ms))
+    if rc:
+        raise SystemExit(f"fdtd_copy_calibration failed: {rc}")
+    out = {"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "ms_per_copy": ms.value,
+           "GBps_read_plus_write": 2 * nbytes / (ms.value * 1e-3) / 1e9}
+    print(json.dumps(out), flush=True)
+    return out
 
 
 def cpu_baseline(w, budget_s=12.0):
@@ -82,13 +98,19 @@ def main():
     ap.add_argument("--storage", default="f32")
     ap.add_argument("--kernel", default="auto")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tile-y", type=int, default=0)
+    ap.add_argument("--seg-x", type=int, default=0)
+    ap.add_argument("--copy-calibration", action="store_true")
     a = ap.parse_args()
+    if a.copy_calibration:
+        copy_calibration()
+        return 0
     if a.gpus != 1:
         print("bench.py measures one MI355X: the multi-GPU (slab-sharded) path is a "
               "design only in this build; run with --gpus 1", file=sys.stderr)
         return 2
     w = make_workload(a.config, a.warmup + a.steps)
-    s = make_solver(w, a.storage, a.kernel)
+    s = make_solver(w, a.storage, a.kernel, a.tile_y, a.seg_x)
     total = a.warmup + a.steps
     s.set_point_sources(w.src, w.signals(total))
     s.set_receivers(w.rec)
@@ -120,7 +142,8 @@ def main():
         "config": {"workload": w.name, "cells": w.cells, "stencil_order": w.order,
                    "courant": w.courant, "boundary": w.boundary, "heterogeneous": w.hetero,
                    "sources": len(w.src), "receivers": int(len(w.rec)),
-                   "kernel": s.kernel_family, "storage": a.storage},
+                   "kernel": s.kernel_family, "storage": a.storage,
+                   "tile_y": a.tile_y or 16, "seg_x": a.seg_x or 64},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "bytes_per_cell_update": bpc,

@@ -27,6 +27,7 @@
 #ifndef AFDTD_H
 #define AFDTD_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #if defined(_WIN32)
@@ -249,6 +250,32 @@ SK_API const char* fdtd_last_error(void);
  * in *ms_per_launch and the wall time of the whole call in *ms_total. */
 SK_API int fdtd_step_timed(AcousticFDTDState* st, int n_steps,
                            float* ms_total, float* ms_stencil_avg);
+
+/* Device-bandwidth probe (no reference counterpart): time a 16-byte-per-lane
+ * device-to-device copy of nbytes over `reps` launches; *ms_per_copy is the
+ * average launch time.  Reported as the measured copy ceiling. */
+SK_API int fdtd_copy_calibration(size_t nbytes, int reps, float* ms_per_copy);
+
+/* ── Batched rooms (no reference counterpart) ─────────────────────────────
+ * n_rooms independent rooms of uneven size advanced by ONE fused launch per
+ * step (plus one source/receiver launch), the whole n-step sequence captured
+ * in a hipGraph.  dims: n_rooms x (Nx, Ny, Nz); courant: per-room Courant
+ * number (dt = courant*dx/c); refl: n_rooms x 6 reflection coefficients
+ * (impedance walls, order 2) or NULL for rigid walls. */
+typedef struct FdtdBatch FdtdBatch;
+SK_API FdtdBatch* fdtd_batch_create(int n_rooms, const int* dims, float dx, float c,
+                                    float rho_air, const float* courant, const float* refl,
+                                    int stencil_order, int storage, int tile_y, int seg_x);
+SK_API void fdtd_batch_destroy(FdtdBatch* b);
+SK_API int fdtd_batch_cells(const FdtdBatch* b, long long* total, int* n_tiles);
+SK_API int fdtd_batch_set_sources(FdtdBatch* b, const int* cell_idx, const float* signals,
+                                  int n_samples);
+SK_API int fdtd_batch_set_receivers(FdtdBatch* b, int per_room, const float* xyz);
+SK_API int fdtd_batch_record(FdtdBatch* b, int max_steps);
+SK_API int fdtd_batch_step(FdtdBatch* b, int n_steps, int use_graph, float* ms_total);
+SK_API int fdtd_batch_get_traces(const FdtdBatch* b, float* out, int len, int* n_recorded);
+SK_API int fdtd_batch_get_pressure(const FdtdBatch* b, int room, float* out, int len);
+SK_API int fdtd_batch_get_step_count(const FdtdBatch* b);
 
 #ifdef __cplusplus
 }

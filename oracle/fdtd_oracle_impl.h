@@ -31,6 +31,8 @@ typedef struct SUF(OrcState) {
     REAL *fx, *fy, *fz;     /* per-face open fractions                      */
     int has_frac;
     REAL *rho, *K, *kap;    /* per-cell density, bulk, K*dt/dx (hetero)      */
+    REAL *bu;               /* per-cell buoyancy 1/rho (hetero)              */
+    REAL dtdx;              /* dt/dx                                         */
     REAL gside[6];          /* (1-R)/(1+R) per domain side                   */
     /* Kirchhoff plate */
     REAL *w, *wp, *wt, *ext;
@@ -67,7 +69,7 @@ SK_EXPORT void SUF(orc_destroy)(OST* s)
     if (!s) return;
     free(s->P); free(s->Ps); free(s->ct); free(s->alpha); free(s->pdamp);
     free(s->vx); free(s->vy); free(s->vz); free(s->vxd); free(s->vyd); free(s->vzd);
-    free(s->fx); free(s->fy); free(s->fz); free(s->rho); free(s->K); free(s->kap);
+    free(s->fx); free(s->fy); free(s->fz); free(s->rho); free(s->K); free(s->kap); free(s->bu);
     free(s->w); free(s->wp); free(s->wt); free(s->ext); free(s->pa);
     free(s->act); free(s->above); free(s->below); free(s->b13); free(s->vzb); free(s->vza);
     free(s->bidx); free(s->bwgt); free(s->pidx); free(s->psig);
@@ -115,6 +117,7 @@ SK_EXPORT OST* SUF(orc_create)(const OrcConfig* cf, const uint8_t* cell_type,
     s->dt = (REAL)dt_f;
     s->dx = (REAL)cf->dx; s->c = (REAL)cf->c; s->rho_air = (REAL)cf->rho_air;
     s->inv_dx = (REAL)1 / s->dx;
+    s->dtdx = s->dt / s->dx;
     s->vel_coeff = s->dt / (s->rho_air * s->dx);                       /* :1089 */
     s->bulk_coeff = s->rho_air * s->c * s->c * s->dt * s->inv_dx;        /* :1176 */
     s->C1 = (REAL)9 / (REAL)8;                                          /* :889 */
@@ -227,12 +230,14 @@ SK_EXPORT OST* SUF(orc_create)(const OrcConfig* cf, const uint8_t* cell_type,
         s->rho = (REAL*)malloc(N * sizeof(REAL));
         s->K = (REAL*)malloc(N * sizeof(REAL));
         s->kap = (REAL*)malloc(N * sizeof(REAL));
-        if (!s->rho || !s->K || !s->kap) goto fail;
+        s->bu = (REAL*)malloc(N * sizeof(REAL));
+        if (!s->rho || !s->K || !s->kap || !s->bu) goto fail;
         REAL K0 = s->rho_air * s->c * s->c;
         for (int q = 0; q < N; ++q) {
             s->rho[q] = rho_in ? (REAL)rho_in[q] : s->rho_air;
             s->K[q] = bulk_in ? (REAL)bulk_in[q] : K0;
             s->kap[q] = s->K[q] * s->dt * s->inv_dx;
+            s->bu[q] = (REAL)1 / s->rho[q];
         }
     }
     for (int f = 0; f < 6; ++f) {
@@ -488,10 +493,14 @@ static REAL SUF(gradient)(const OST* s, int Lm, int L, int R, int Rp, int inner)
     return P[R] - P[L];
 }
 
+/* Velocity coefficient of the face between cells L and R: dt/(rho dx) for a
+ * homogeneous medium (acoustic_fdtd.cpp:1089); for a heterogeneous medium
+ * (extension) the face buoyancy is the mean of the cell buoyancies 1/rho,
+ * i.e. the face density is the harmonic mean of the cell densities. */
 static REAL SUF(face_coeff)(const OST* s, int L, int R)
 {
     if (!s->hetero) return s->vel_coeff;
-    return s->dt / (((REAL)0.5f * (s->rho[L] + s->rho[R])) * s->dx);
+    return ((REAL)0.5f * (s->bu[L] + s->bu[R])) * s->dtdx;
 }
 
 /* One leapfrog update — acoustic_fdtd.cpp:1081-1241 (+ extension terms). */
@@ -750,23 +759,25 @@ SK_EXPORT int SUF(orc_energy)(const OST* s, double* out)
         e0 += (double)s->P[q] * (double)s->Ps[q] / (2.0 * K);
         e2 += (double)s->P[q] * (double)s->P[q] / (2.0 * K);
     }
-#define RHO(c) (s->hetero ? (double)s->rho[c] : (double)s->rho_air)
+    /* face density = 1 / face buoyancy (the weight that makes the discrete
+     * energy of the heterogeneous Yee scheme exactly invariant) */
+#define RHOF(L, R) (s->hetero ? 2.0 / ((double)s->bu[L] + (double)s->bu[R]) : (double)s->rho_air)
     for (int i = 0; i < Nx - 1; ++i) for (int j = 0; j < Ny; ++j) for (int k = 0; k < Nz; ++k) {
         int L = SUF(idx3)(s, i, j, k), R = SUF(idx3)(s, i + 1, j, k);
         double v = (double)s->vx[k + Nz * (j + Ny * i)];
-        e1 += 0.25 * (RHO(L) + RHO(R)) * v * v;
+        e1 += 0.5 * RHOF(L, R) * v * v;
     }
     for (int i = 0; i < Nx; ++i) for (int j = 0; j < Ny - 1; ++j) for (int k = 0; k < Nz; ++k) {
         int L = SUF(idx3)(s, i, j, k), R = SUF(idx3)(s, i, j + 1, k);
         double v = (double)s->vy[k + Nz * (j + (Ny - 1) * i)];
-        e1 += 0.25 * (RHO(L) + RHO(R)) * v * v;
+        e1 += 0.5 * RHOF(L, R) * v * v;
     }
     for (int i = 0; i < Nx; ++i) for (int j = 0; j < Ny; ++j) for (int k = 0; k < Nz - 1; ++k) {
         int L = SUF(idx3)(s, i, j, k), R = SUF(idx3)(s, i, j, k + 1);
         double v = (double)s->vz[k + (Nz - 1) * (j + Ny * i)];
-        e1 += 0.25 * (RHO(L) + RHO(R)) * v * v;
+        e1 += 0.5 * RHOF(L, R) * v * v;
     }
-#undef RHO
+#undef RHOF
     out[0] = e0 * dV; out[1] = e1 * dV; out[2] = e2 * dV;
     return 0;
 }

@@ -83,6 +83,17 @@ _SIGS = {
     "fdtd_energy": (_i, [_vp, _vp]),
     "fdtd_set_state": (_i, [_vp, _vp, _vp, _vp, _vp]),
     "fdtd_last_error": (C.c_char_p, []),
+    "fdtd_batch_create": (_vp, [_i, _vp, _f, _f, _f, _vp, _vp, _i, _i, _i, _i]),
+    "fdtd_batch_destroy": (None, [_vp]),
+    "fdtd_batch_cells": (_i, [_vp, _vp, _vp]),
+    "fdtd_batch_set_sources": (_i, [_vp, _vp, _vp, _i]),
+    "fdtd_batch_set_receivers": (_i, [_vp, _i, _vp]),
+    "fdtd_batch_record": (_i, [_vp, _i]),
+    "fdtd_batch_step": (_i, [_vp, _i, _i, _vp]),
+    "fdtd_batch_get_traces": (_i, [_vp, _vp, _i, _vp]),
+    "fdtd_batch_get_pressure": (_i, [_vp, _i, _vp, _i]),
+    "fdtd_batch_get_step_count": (_i, [_vp]),
+    "fdtd_copy_calibration": (_i, [C.c_size_t, _i, _vp]),
 }
 
 

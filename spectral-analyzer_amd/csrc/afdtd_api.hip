@@ -234,10 +234,10 @@ void destroy_state(AcousticFDTDState* st) {
     if (!st) return;
     if (st->stream) (void)hipStreamSynchronize(st->stream);
     dfree(st->P[0]); dfree(st->P[1]);
-    for (int a = 0; a < 3; ++a) dfree(st->V[a]);
+    for (int a = 0; a < 3; ++a) { dfree(st->V[a]); dfree(st->V2[a]); }
     dfree(st->stage);
     dfree(st->d_ct); dfree(st->d_fx); dfree(st->d_fy); dfree(st->d_fz);
-    dfree(st->d_rho); dfree(st->d_kap); dfree(st->d_K);
+    dfree(st->d_bu); dfree(st->d_kap); dfree(st->d_xi);
     dfree(st->d_alpha); dfree(st->d_pdamp); dfree(st->d_status);
     for (int a = 0; a < 3; ++a) dfree(st->d_w[a]);
     dfree(st->d_ext); dfree(st->d_act); dfree(st->d_above); dfree(st->d_below);
@@ -391,6 +391,13 @@ AcousticFDTDState* create_impl(int Nx, int Ny, int Nz, float dx, float c, float 
     }
     sp.rho_air = rho_air;
     sp.K0 = rho_air * c * c;
+    sp.dtdx = dt / dx;
+    {
+        // Homogeneous wall terms (kap/2) * g / sqrt(rho K), one per side.
+        const float z0 = sqrtf(rho_air * sp.K0);
+        const float hk = 0.5f * sp.bulk_coeff;
+        for (int f = 0; f < 6; ++f) sp.xterm[f] = hk * (sp.gside[f] / z0);
+    }
     // FP16 stores velocities scaled by 2^round(log2(rho c)) so they sit in
     // the same range as pressures; BF16/FP32 keep the physical value.
     sp.sV = 1.0f;
@@ -424,16 +431,39 @@ AcousticFDTDState* create_impl(int Nx, int Ny, int Nz, float dx, float c, float 
         if (alloc_fracs(st) != FDTD_OK) { destroy_state(st); return nullptr; }
     }
     if (st->hetero) {
-        std::vector<float> rho(N), K(N), kap(N);
-        for (int q = 0; q < N; ++q) {
-            rho[q] = o->rho ? o->rho[q] : rho_air;
-            K[q] = o->bulk ? o->bulk[q] : sp.K0;
-            kap[q] = K[q] * dt * inv_dx;
-        }
-        TRY(dalloc(&st->d_rho, N)); TRY(dalloc(&st->d_K, N)); TRY(dalloc(&st->d_kap, N));
-        TRY(hipMemcpyAsync(st->d_rho, rho.data(), sizeof(float) * N, hipMemcpyHostToDevice, st->stream));
-        TRY(hipMemcpyAsync(st->d_K, K.data(), sizeof(float) * N, hipMemcpyHostToDevice, st->stream));
+        // Per-cell buoyancy 1/rho and kappa = K dt/dx; for impedance walls the
+        // per-cell wall sum xi = sum_sides (kappa/2) g_side / sqrt(rho K),
+        // accumulated in side order x-, x+, y-, y+, z-, z+ (edge cells only).
+        std::vector<float> bu(N), kap(N), xi;
+        const bool imp = o->boundary == FDTD_BOUNDARY_IMPEDANCE;
+        if (imp) xi.assign(N, 0.0f);
+        for (int i = 0; i < Nx; ++i)
+            for (int j = 0; j < Ny; ++j)
+                for (int k = 0; k < Nz; ++k) {
+                    const int q = k + Nz * (j + Ny * i);
+                    const float r = o->rho ? o->rho[q] : rho_air;
+                    const float K = o->bulk ? o->bulk[q] : sp.K0;
+                    bu[q] = 1.0f / r;
+                    kap[q] = K * dt * inv_dx;
+                    if (imp && (i == 0 || i == Nx - 1 || j == 0 || j == Ny - 1 || k == 0 || k == Nz - 1)) {
+                        const float z = sqrtf(r * K), hk = 0.5f * kap[q];
+                        float x = 0.0f;
+                        if (i == 0) x = x + hk * (sp.gside[0] / z);
+                        if (i == Nx - 1) x = x + hk * (sp.gside[1] / z);
+                        if (j == 0) x = x + hk * (sp.gside[2] / z);
+                        if (j == Ny - 1) x = x + hk * (sp.gside[3] / z);
+                        if (k == 0) x = x + hk * (sp.gside[4] / z);
+                        if (k == Nz - 1) x = x + hk * (sp.gside[5] / z);
+                        xi[q] = x;
+                    }
+                }
+        TRY(dalloc(&st->d_bu, N)); TRY(dalloc(&st->d_kap, N));
+        TRY(hipMemcpyAsync(st->d_bu, bu.data(), sizeof(float) * N, hipMemcpyHostToDevice, st->stream));
         TRY(hipMemcpyAsync(st->d_kap, kap.data(), sizeof(float) * N, hipMemcpyHostToDevice, st->stream));
+        if (imp) {
+            TRY(dalloc(&st->d_xi, N));
+            TRY(hipMemcpyAsync(st->d_xi, xi.data(), sizeof(float) * N, hipMemcpyHostToDevice, st->stream));
+        }
         TRY(hipStreamSynchronize(st->stream));
     }
 
@@ -500,13 +530,21 @@ AcousticFDTDState* create_impl(int Nx, int Ny, int Nz, float dx, float c, float 
             return nullptr;
         }
     }
+    if (st->family == FDTD_KERNEL_FUSED) {
+        for (int a = 0; a < 3; ++a) {
+            TRY(hipMalloc(&st->V2[a], (size_t)N * st->esz));
+            TRY(hipMemsetAsync(st->V2[a], 0, (size_t)N * st->esz, st->stream));
+        }
+        TRY(hipStreamSynchronize(st->stream));
+    }
     return st;
 }
 
 #undef TRY
 
 int apply_failure(AcousticFDTDState* st, int start_step, int n_steps, int start_cur,
-                  float* const w0[3], int start_trace, int start_phase) {
+                  float* const w0[3], void* const v0[3], void* const v20[3], int start_trace,
+                  int start_phase) {
     int status[2] = {0, 0};
     AFDTD_CHECK(hipMemcpy(status, st->d_status, sizeof(status), hipMemcpyDeviceToHost));
     if (status[0] == 0) {
@@ -523,6 +561,13 @@ int apply_failure(AcousticFDTDState* st, int start_step, int n_steps, int start_
         float* t = w[2]; w[2] = w[1]; w[1] = w[0]; w[0] = t;
     }
     for (int a = 0; a < 3; ++a) st->d_w[a] = w[a];
+    // The fused sweep swaps the velocity buffers once per completed step.
+    if (st->family == FDTD_KERNEL_FUSED) {
+        for (int a = 0; a < 3; ++a) {
+            st->V[a] = (completed & 1) ? v20[a] : v0[a];
+            st->V2[a] = (completed & 1) ? v0[a] : v20[a];
+        }
+    }
     if (st->trace_cap > 0) st->trace_n = std::min(st->trace_cap, start_trace + completed);
     int checks = 0;
     for (int s = start_step; s < start_step + completed; ++s)
@@ -571,6 +616,8 @@ int step_impl(AcousticFDTDState* st, int n_steps, float* ms_total, float* ms_ste
     const int start_step = st->step_count, start_cur = st->cur;
     const int start_trace = st->trace_n, start_phase = st->phase;
     float* w0[3] = {st->d_w[0], st->d_w[1], st->d_w[2]};
+    void* v0[3] = {st->V[0], st->V[1], st->V[2]};
+    void* v20[3] = {st->V2[0], st->V2[1], st->V2[2]};
     std::vector<hipEvent_t> evs;
     if (ms_stencil) {
         evs.resize(2 * (size_t)n_steps);
@@ -595,7 +642,8 @@ int step_impl(AcousticFDTDState* st, int n_steps, float* ms_total, float* ms_ste
         *ms_stencil = (float)(sum / n_steps);
         for (auto& e : evs) (void)hipEventDestroy(e);
     }
-    return apply_failure(st, start_step, n_steps, start_cur, w0, start_trace, start_phase);
+    return apply_failure(st, start_step, n_steps, start_cur, w0, v0, v20, start_trace,
+                         start_phase);
 }
 
 }  // namespace

@@ -170,26 +170,26 @@ k_energy(StencilParams p, StencilAux a, const T* __restrict__ P, const T* __rest
     for (int c = blockIdx.x * blockDim.x + tid; c < N; c += gridDim.x * blockDim.x) {
         int k = c % p.Nz, t = c / p.Nz, j = t % p.Ny, i = t / p.Ny;
         if (!(p.has_solid && is_solid(a.ct[c]))) {
-            double K = p.hetero ? (double)a.K[c] : (double)p.K0;
+            double K = p.hetero ? (double)a.kap[c] / ((double)p.dt / (double)p.dx) : (double)p.K0;
             double pc = (double)S::ld(P, c);
             ep += pc * pc / (2.0 * K);
         }
-        double rc = p.hetero ? (double)a.rho[c] : (double)p.rho_air;
         double vs = S::exact ? 1.0 : (double)p.inv_sV;
+        // face density = 1 / face buoyancy (exact energy weight of the scheme)
+        auto rf = [&](int L, int R) {
+            return p.hetero ? 2.0 / ((double)a.bu[L] + (double)a.bu[R]) : (double)p.rho_air;
+        };
         if (i < p.Nx - 1) {
-            double r = p.hetero ? 0.5 * (rc + (double)a.rho[c + p.Ny * p.Nz]) : rc;
             double v = (double)S::ld(Vx, c) * vs;
-            ek += 0.5 * r * v * v;
+            ek += 0.5 * rf(c, c + p.Ny * p.Nz) * v * v;
         }
         if (j < p.Ny - 1) {
-            double r = p.hetero ? 0.5 * (rc + (double)a.rho[c + p.Nz]) : rc;
             double v = (double)S::ld(Vy, c) * vs;
-            ek += 0.5 * r * v * v;
+            ek += 0.5 * rf(c, c + p.Nz) * v * v;
         }
         if (k < p.Nz - 1) {
-            double r = p.hetero ? 0.5 * (rc + (double)a.rho[c + 1]) : rc;
             double v = (double)S::ld(Vz, c) * vs;
-            ek += 0.5 * r * v * v;
+            ek += 0.5 * rf(c, c + 1) * v * v;
         }
     }
     red[0][tid] = ep; red[1][tid] = ek;
@@ -322,7 +322,7 @@ hipError_t launch_from_f32(const float* src, void* dst, int storage, float scale
 hipError_t launch_energy(AcousticFDTDState* st, double* d_out) {
     StencilAux a;
     a.ct = st->d_ct; a.fx = st->d_fx; a.fy = st->d_fy; a.fz = st->d_fz;
-    a.rho = st->d_rho; a.kap = st->d_kap; a.K = st->d_K;
+    a.bu = st->d_bu; a.kap = st->d_kap; a.xi = st->d_xi;
     a.alpha = st->d_alpha; a.pdamp = st->d_pdamp; a.status = st->d_status;
     double dV = (double)st->dx * (double)st->dx * (double)st->dx;
     unsigned nb = blocks_for(st->N, 256);
@@ -338,4 +338,42 @@ hipError_t launch_energy(AcousticFDTDState* st, double* d_out) {
     }
 #undef E
     return hipGetLastError();
+}
+
+namespace {
+__global__ void __launch_bounds__(256) k_copy16(const float4* __restrict__ a, float4* __restrict__ b,
+                                                size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        b[i] = a[i];
+}
+}  // namespace
+
+// Device-bandwidth probe: a 16-byte-per-lane copy of nbytes, timed with HIP
+// events over `reps` launches (no reference counterpart; used to report the
+// measured copy ceiling next to the stencil's roofline fraction).
+extern "C" SK_API int fdtd_copy_calibration(size_t nbytes, int reps, float* ms_per_copy) {
+    if (!ms_per_copy || reps <= 0 || nbytes < 16) return FDTD_ERR_NULL;
+    const size_t n = nbytes / 16;
+    float4 *a = nullptr, *b = nullptr;
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    AFDTD_CHECK(hipMalloc(&a, n * 16));
+    AFDTD_CHECK(hipMalloc(&b, n * 16));
+    AFDTD_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    AFDTD_CHECK(hipEventCreate(&e0));
+    AFDTD_CHECK(hipEventCreate(&e1));
+    AFDTD_CHECK(hipMemsetAsync(a, 0, n * 16, s));
+    const unsigned blocks = 256 * 32;
+    hipLaunchKernelGGL(k_copy16, dim3(blocks), dim3(256), 0, s, a, b, n);
+    AFDTD_CHECK(hipEventRecord(e0, s));
+    for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(k_copy16, dim3(blocks), dim3(256), 0, s, a, b, n);
+    AFDTD_CHECK(hipEventRecord(e1, s));
+    AFDTD_CHECK(hipStreamSynchronize(s));
+    float ms = 0.0f;
+    AFDTD_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    *ms_per_copy = ms / reps;
+    (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); (void)hipStreamDestroy(s);
+    (void)hipFree(a); (void)hipFree(b);
+    return FDTD_OK;
 }

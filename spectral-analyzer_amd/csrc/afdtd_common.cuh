@@ -50,20 +50,23 @@ __device__ __forceinline__ float face_damp(const float* alpha, int dL, int dR, f
 }
 
 // Locally reacting wall update of a boundary cell (order 2, extension):
-// P' = ((P (1 - xi)) - kap*dv) / (1 + xi), xi = sum over touching domain
-// sides of (kap/2) * g_side / sqrt(rho K).
-__device__ __forceinline__ float impedance_update(float P, float dv, float kap, float r,
-                                                  float KK, const float* g, int i, int j,
-                                                  int k, int Nx, int Ny, int Nz) {
-    float z = sqrtf(r * KK);
-    float hk = 0.5f * kap;
-    float xi = 0.0f;
-    if (i == 0) xi = xi + hk * (g[0] / z);
-    if (i == Nx - 1) xi = xi + hk * (g[1] / z);
-    if (j == 0) xi = xi + hk * (g[2] / z);
-    if (j == Ny - 1) xi = xi + hk * (g[3] / z);
-    if (k == 0) xi = xi + hk * (g[4] / z);
-    if (k == Nz - 1) xi = xi + hk * (g[5] / z);
+// P' = ((P (1 - xi)) - kap*dv) / (1 + xi), xi = sum over the cell's domain
+// sides of (kap/2) * g_side / sqrt(rho K), g = (1 - R)/(1 + R).
+__device__ __forceinline__ float impedance_update(float P, float dv, float kap, float xi) {
     float num = P * (1.0f - xi) - kap * dv;
     return num / (1.0f + xi);
+}
+
+// Sum of the homogeneous wall terms of the sides a cell touches, accumulated
+// in the fixed side order x-, x+, y-, y+, z-, z+.
+__device__ __forceinline__ float wall_xi(const float* t, int i, int j, int k, int Nx, int Ny,
+                                         int Nz) {
+    float xi = 0.0f;
+    if (i == 0) xi = xi + t[0];
+    if (i == Nx - 1) xi = xi + t[1];
+    if (j == 0) xi = xi + t[2];
+    if (j == Ny - 1) xi = xi + t[3];
+    if (k == 0) xi = xi + t[4];
+    if (k == Nz - 1) xi = xi + t[5];
+    return xi;
 }

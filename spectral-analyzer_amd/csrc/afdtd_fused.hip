@@ -1,4 +1,126 @@
-// afdtd_fused.hip — placeholder until the fused sweep lands.
-#include "afdtd_common.cuh"
-bool fused_supported(const AcousticFDTDState* st) { (void)st; return false; }
-hipError_t launch_fused_step(AcousticFDTDState* st) { (void)st; return hipErrorInvalidValue; }
+// afdtd_fused.hip — one-pass leapfrog step: 2.5-D tiles marching along x.
+//
+// A workgroup owns a (TY rows in y) x (64 columns in z) tile and marches an
+// x-segment [i0, i1) of it.  Per x-plane q it
+//   1. takes the P^n plane q+LA (LA = 3 for order 4, 1 for order 2) from
+//      registers into an LDS ring (the load was issued one plane earlier, so
+//      HBM latency hides behind one plane of arithmetic);
+//   2. updates the Vx face q+FHI of its own columns from the ring (the new
+//      Vx faces q-2..q+1 live in a per-thread register queue);
+//   3. updates Vy and Vz of plane q on the tile plus the halo faces the
+//      divergence needs, into LDS;
+//   4. updates P at plane q from the register queue and the LDS faces.
+// Each field is read once and written once per step from HBM (32 B per cell
+// update in FP32, +8 B for per-cell rho and K), apart from halo re-reads
+// that neighbouring tiles share through L2.  Halo faces are recomputed, not
+// exchanged, so the velocities are double-buffered (read V^{n-1/2} from one
+// buffer, write V^{n+1/2} to the other) and P is double-buffered as in the
+// generic path.
+//
+// The per-face and per-cell arithmetic is the reference's, in the same
+// order (acoustic_fdtd.cpp:1107-1113 gradient/velocity, :894-925 divergence,
+// :1198 pressure), so results equal the generic kernels and the oracle bit
+// for bit in FP32.  Geometry with walls, plate cells, cut-cell fractions or
+// an active plate runs on the generic kernels instead.
+//
+// Workgroup order is XCD-aware: blocks are dealt round-robin to the 8 XCDs,
+// so the block index is remapped to give each XCD a contiguous range of
+// tiles (y fastest), which keeps halo rows of neighbouring tiles in the same
+// L2.
+#include "afdtd_fused_tile.cuh"
+
+using namespace fused;
+
+namespace {
+
+struct FusedArgs {
+    StencilParams p;
+    FusedPtrs f;
+    const float* alpha;   // alpha(d)
+    const float* pdamp;   // 1/(1 + alpha(d) dt)
+    const int* status;
+    int seg;              // x-segment length
+    int nty, ntz, nseg;   // tile counts
+};
+
+template <int ORDER, typename T, bool HET, int TY, int NW>
+__global__ void __launch_bounds__(64 * NW)
+k_fused(FusedArgs a) {
+    if (*a.status) return;
+    int t = xcd_remap(blockIdx.x, gridDim.x);
+    const int ty = t % a.nty; t /= a.nty;
+    const int tz = t % a.ntz;
+    const int sg = t / a.ntz;
+    fused_tile<ORDER, T, HET, TY, NW>(a.p, a.f, a.alpha, a.pdamp, ty, tz, sg, a.seg);
+}
+
+template <int ORDER, typename T, bool HET, int TY>
+hipError_t launch_t(AcousticFDTDState* st, const FusedArgs& a0) {
+    constexpr int NW = TY / 2 > 4 ? TY / 2 : 4;     // two rows per thread
+    FusedArgs a = a0;
+    a.nty = (st->Ny + TY - 1) / TY;
+    a.ntz = (st->Nz + TZ - 1) / TZ;
+    a.nseg = (st->Nx + a.seg - 1) / a.seg;
+    const int nb = a.nty * a.ntz * a.nseg;
+    hipLaunchKernelGGL((k_fused<ORDER, T, HET, TY, NW>), dim3(nb), dim3(64, NW), 0, st->stream, a);
+    return hipGetLastError();
+}
+
+template <int ORDER, typename T, bool HET>
+hipError_t launch_ty(AcousticFDTDState* st, const FusedArgs& a, int ty) {
+    switch (ty) {
+        case 8: return launch_t<ORDER, T, HET, 8>(st, a);
+        case 32: return launch_t<ORDER, T, HET, 32>(st, a);
+        default: return launch_t<ORDER, T, HET, 16>(st, a);
+    }
+}
+
+template <int ORDER, typename T>
+hipError_t launch_het(AcousticFDTDState* st, const FusedArgs& a, int ty) {
+    return st->hetero ? launch_ty<ORDER, T, true>(st, a, ty) : launch_ty<ORDER, T, false>(st, a, ty);
+}
+
+template <int ORDER>
+hipError_t launch_store(AcousticFDTDState* st, const FusedArgs& a, int ty) {
+    switch (st->storage) {
+        case FDTD_STORE_BF16: return launch_het<ORDER, bf16_t>(st, a, ty);
+        case FDTD_STORE_F16: return launch_het<ORDER, f16_t>(st, a, ty);
+        default: return launch_het<ORDER, float>(st, a, ty);
+    }
+}
+
+}  // namespace
+
+bool fused_supported(const AcousticFDTDState* st) {
+    return !st->has_solid && !st->has_frac && st->nact == 0 && st->n_pml <= 64;
+}
+
+hipError_t launch_fused_step(AcousticFDTDState* st) {
+    if (!st->V2[0]) return hipErrorInvalidValue;
+    FusedArgs a;
+    a.p = st->sp;
+    a.f.Pin = st->P[st->cur];
+    a.f.Pout = st->P[st->cur ^ 1];
+    for (int c = 0; c < 3; ++c) {
+        a.f.Vin[c] = st->V[c];
+        a.f.Vout[c] = st->V2[c];
+    }
+    a.f.bu = st->d_bu;
+    a.f.kap = st->d_kap;
+    a.f.xi = st->d_xi;
+    a.alpha = st->d_alpha;
+    a.pdamp = st->d_pdamp;
+    a.status = st->d_status;
+    int ty = st->fused_tile_y > 0 ? st->fused_tile_y : 16;
+    a.seg = st->fused_seg_x > 0 ? st->fused_seg_x : 64;
+    if (a.seg > st->Nx) a.seg = st->Nx;
+    hipError_t e = st->order == 4 ? launch_store<4>(st, a, ty) : launch_store<2>(st, a, ty);
+    if (e == hipSuccess) {
+        for (int c = 0; c < 3; ++c) {
+            void* t = st->V[c];
+            st->V[c] = st->V2[c];
+            st->V2[c] = t;
+        }
+    }
+    return e;
+}

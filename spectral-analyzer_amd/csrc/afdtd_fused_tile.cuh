@@ -1,0 +1,425 @@
+// afdtd_fused_tile.cuh — the fused plane-marching tile update (device code),
+// shared by the single-room sweep (afdtd_fused.hip) and the batched-rooms
+// sweep (afdtd_batch.hip).  See afdtd_fused.hip for the algorithm.
+//
+// Work mapping inside a tile of TY rows x 64 columns (NW wavefronts):
+//   lane = z column, wave w owns rows w*R .. w*R+R-1 (R = TY/NW);
+//   halo face rows of Vy are one extra task each for waves 0..FLO+FHI-1, the
+//   halo face columns of Vz are one extra task for the last lanes of the
+//   block, so the halo work lands on different wavefronts;
+//   every global offset, validity flag and LDS address is computed once per
+//   workgroup, so a plane costs only loads, LDS traffic and the stencil flops.
+#pragma once
+#include "afdtd_common.cuh"
+
+namespace fused {
+
+constexpr int TZ = 64;
+
+struct FusedPtrs {
+    const void* Pin;
+    void* Pout;
+    const void* Vin[3];
+    void* Vout[3];
+    const float* bu;      // per-cell buoyancy 1/rho (hetero)
+    const float* kap;     // per-cell K dt/dx (hetero)
+    const float* xi;      // per-cell wall sums (hetero + impedance, edge cells)
+};
+
+template <int ORDER>
+struct Geo {
+    static constexpr int PH = ORDER == 4 ? 3 : 1;   // P halo (cells)
+    static constexpr int FLO = ORDER == 4 ? 2 : 1;  // faces below own range
+    static constexpr int FHI = ORDER == 4 ? 1 : 0;  // faces above own range
+    static constexpr int GHI = ORDER == 4 ? 2 : 1;  // gradient taps above face
+    static constexpr int LA = FHI + GHI;            // P-plane lookahead
+    static constexpr int NS = LA + 1;               // ring depth
+    static constexpr int NQ = FLO + FHI + 1;        // Vx queue length
+};
+
+__device__ __forceinline__ int xcd_remap(int b, int nb) {
+    const int x = b & 7, per = nb >> 3, rem = nb & 7;
+    const int base = x < rem ? x * (per + 1) : rem * (per + 1) + (x - rem) * per;
+    return base + (b >> 3);
+}
+
+// Face update shared by the three components: gradient from the four P taps
+// (2nd order at edge faces or for the order-2 stencil,
+// acoustic_fdtd.cpp:1107-1112), then (V - coef*g) (:1113).
+template <int ORDER>
+__device__ __forceinline__ float face_new(float vold, float pm, float pl, float pr, float pp,
+                                          bool inner, float coef, float C1, float C2) {
+    float g;
+    if (ORDER == 4 && inner)
+        g = C1 * (pr - pl) + C2 * (pp - pm);
+    else
+        g = pr - pl;
+    return vold - coef * g;
+}
+
+template <int ORDER, typename T, bool HET, int TY, int NW>
+__device__ __forceinline__ void fused_tile(const StencilParams& p, const FusedPtrs& fp,
+                                           const float* __restrict__ alpha_tab,
+                                           const float* __restrict__ pdamp_tab,
+                                           int ty, int tz, int sg, int seg) {
+    typedef Geo<ORDER> G;
+    typedef Store<T> S;
+    constexpr int PH = G::PH, FLO = G::FLO, FHI = G::FHI, NS = G::NS, LA = G::LA, NQ = G::NQ;
+    constexpr int R = TY / NW;                      // rows per thread
+    constexpr int PR = TY + 2 * PH, PC = TZ + 2 * PH;
+    constexpr int FR = TY + FLO + FHI;              // Vy face rows held in LDS
+    constexpr int FC = TZ + FLO + FHI;              // Vz face cols held in LDS
+    constexpr int NT = 64 * NW;
+    constexpr int HR = 2 * PH;                      // halo rows of a P plane
+    constexpr int NHRW = (HR + NW - 1) / NW;        // halo rows per wave
+    constexpr int HC = PR * 2 * PH;                 // halo-column elements
+    constexpr int NHC = (HC + NT - 1) / NT;         // halo-column elements per thread
+    constexpr int NHY = FLO + FHI;                  // halo Vy face rows (one per wave)
+    constexpr int NHZ = TY * (FLO + FHI);           // halo Vz faces (last threads)
+    static_assert(NHY <= NW, "one halo Vy row per wave");
+    static_assert(NHZ <= NT, "halo Vz faces fit the block");
+    constexpr int PLANE = PR * PC;
+
+    __shared__ float sP[NS * PLANE];
+    __shared__ float sB[HET ? NS * PLANE : 1];
+    __shared__ float sVy[FR * TZ];
+    __shared__ float sVz[TY * FC];
+    __shared__ float sAlpha[65], sPd[65];
+
+    const int Nx = p.Nx, Ny = p.Ny, Nz = p.Nz;
+    const int sx = Ny * Nz;
+    const int lane = threadIdx.x, w = threadIdx.y, tid = lane + 64 * w;
+    const int j0 = ty * TY, k0 = tz * TZ;
+    const int i0 = sg * seg, i1 = min(i0 + seg, Nx);
+    const int k = k0 + lane;
+    const bool kval = k < Nz;
+    const T* __restrict__ Pin = (const T*)fp.Pin;
+    T* __restrict__ Pout = (T*)fp.Pout;
+    const T* __restrict__ Vxi = (const T*)fp.Vin[0];
+    const T* __restrict__ Vyi = (const T*)fp.Vin[1];
+    const T* __restrict__ Vzi = (const T*)fp.Vin[2];
+    T* __restrict__ Vxo = (T*)fp.Vout[0];
+    T* __restrict__ Vyo = (T*)fp.Vout[1];
+    T* __restrict__ Vzo = (T*)fp.Vout[2];
+    const int npml = p.n_pml;
+    if (npml > 0)
+        for (int d = tid; d <= npml && d < 65; d += NT) { sAlpha[d] = alpha_tab[d]; sPd[d] = pdamp_tab[d]; }
+    const float C1 = p.C1, C2 = p.C2, vcoef = p.vel_coeff, dtdx = p.dtdx, dt = p.dt;
+    const float isv = p.inv_sV, sv = p.sV;
+
+    // ── per-thread geometry, computed once ──
+    int ownOff[R];          // in-plane offset of own cell (row w*R+r, col lane)
+    bool ownV[R];           // own cell inside the room
+    bool vyV[R], vzV[R];    // own Vy / Vz face exists (j <= Ny-2 / k <= Nz-2)
+    bool vyIn[R];           // order-4 taps for the own Vy face
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int j = j0 + w * R + r;
+        ownOff[r] = j * Nz + k;
+        ownV[r] = kval && j < Ny;
+        vyV[r] = kval && j <= Ny - 2;
+        vzV[r] = j < Ny && k <= Nz - 2;
+        vyIn[r] = j > 0 && j < Ny - 2;
+    }
+    const bool vzIn = k > 0 && k < Nz - 2;
+    // halo rows of the P plane (cols PH .. PH+63)
+    int hrOff[NHRW], hrLds[NHRW];
+    bool hrV[NHRW];
+#pragma unroll
+    for (int m = 0; m < NHRW; ++m) {
+        const int h = w + NW * m;
+        const int row = h < PH ? h : TY + h;
+        const int gj = j0 - PH + row;
+        hrLds[m] = row * PC + PH + lane;
+        hrOff[m] = gj * Nz + k;
+        hrV[m] = h < HR && gj >= 0 && gj < Ny && kval;
+        if (h >= HR) hrLds[m] = -1;
+    }
+    // halo columns of the P plane (all PR rows)
+    int hcOff[NHC], hcLds[NHC];
+    bool hcV[NHC];
+#pragma unroll
+    for (int m = 0; m < NHC; ++m) {
+        const int e = tid + NT * m;
+        const int row = e / (2 * PH), c = e - row * (2 * PH);
+        const int col = c < PH ? c : TZ + c;
+        const int gj = j0 - PH + row, gk = k0 - PH + col;
+        hcLds[m] = e < HC ? row * PC + col : -1;
+        hcOff[m] = gj * Nz + gk;
+        hcV[m] = e < HC && gj >= 0 && gj < Ny && gk >= 0 && gk < Nz;
+    }
+    // halo Vy face row handled by this wave (task A)
+    const bool tA = w < NHY;
+    const int frA = w < FLO ? w : TY + w;            // LDS face row
+    const int jA = j0 - FLO + frA;                   // global face row
+    const bool vA = tA && kval && jA >= 0 && jA <= Ny - 2;
+    const bool inA = jA > 0 && jA < Ny - 2;
+    const int offA = jA * Nz + k;
+    // halo Vz face handled by this thread (task B)
+    const int uB = tid - (NT - NHZ);
+    const bool tB = uB >= 0;
+    const int lyB = tB ? uB / (FLO + FHI) : 0;
+    const int hcB = tB ? uB - lyB * (FLO + FHI) : 0;
+    const int fcB = hcB < FLO ? hcB : TZ + hcB;      // LDS face col
+    const int jB = j0 + lyB, kB = k0 - FLO + fcB;
+    const bool vB = tB && jB < Ny && kB >= 0 && kB <= Nz - 2;
+    const bool inB = kB > 0 && kB < Nz - 2;
+    const int offB = jB * Nz + kB;
+    // PML depths (used only when npml > 0)
+    int dJ[R], dJ1[R];
+    const int dK = pml_depth(k, Nz, npml), dK1 = pml_depth(k + 1, Nz, npml);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        dJ[r] = pml_depth(j0 + w * R + r, Ny, npml);
+        dJ1[r] = pml_depth(j0 + w * R + r + 1, Ny, npml);
+    }
+    const int dJA = pml_depth(jA, Ny, npml), dJA1 = pml_depth(jA + 1, Ny, npml);
+    const int dJB = pml_depth(jB, Ny, npml), dKB = pml_depth(kB, Nz, npml),
+              dKB1 = pml_depth(kB + 1, Nz, npml);
+    // own cells on a y/z domain side (impedance walls)
+    bool edgeYZ[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int j = j0 + w * R + r;
+        edgeYZ[r] = (j == 0) | (j == Ny - 1) | (k == 0) | (k == Nz - 1);
+    }
+    const int ownLds = (PH + w * R) * PC + PH + lane;    // + r*PC
+
+    // ── staging registers (filled one plane ahead) ──
+    float stP[R + NHRW + NHC], stB[HET ? R + NHRW + NHC : 1];
+    float oVx[R], oVy[R], oVz[R], oA = 0.0f, oB = 0.0f, oK[HET ? R : 1];
+    float nVx[R], nVy[R], nVz[R], nA = 0.0f, nB = 0.0f, nK[HET ? R : 1];
+    float vq[NQ][R];
+#pragma unroll
+    for (int s = 0; s < NQ; ++s)
+#pragma unroll
+        for (int r = 0; r < R; ++r) vq[s][r] = 0.0f;
+
+    auto ldv = [&](const T* __restrict__ F, int off) {
+        float v = S::ld(F, off);
+        return S::exact ? v : v * isv;
+    };
+    auto load_plane = [&](int pl) {
+        const bool pv = pl >= 0 && pl < Nx;
+        const int base = pl * sx;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const bool v = pv && ownV[r];
+            stP[r] = v ? S::ld(Pin, base + ownOff[r]) : 0.0f;
+            if (HET) stB[r] = v ? fp.bu[base + ownOff[r]] : 0.0f;
+        }
+#pragma unroll
+        for (int m = 0; m < NHRW; ++m) {
+            const bool v = pv && hrV[m];
+            stP[R + m] = v ? S::ld(Pin, base + hrOff[m]) : 0.0f;
+            if (HET) stB[R + m] = v ? fp.bu[base + hrOff[m]] : 0.0f;
+        }
+#pragma unroll
+        for (int m = 0; m < NHC; ++m) {
+            const bool v = pv && hcV[m];
+            stP[R + NHRW + m] = v ? S::ld(Pin, base + hcOff[m]) : 0.0f;
+            if (HET) stB[R + NHRW + m] = v ? fp.bu[base + hcOff[m]] : 0.0f;
+        }
+    };
+    auto store_plane = [&](int slot) {
+        float* dP = sP + slot * PLANE;
+        float* dB = sB + (HET ? slot * PLANE : 0);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            dP[ownLds + r * PC] = stP[r];
+            if (HET) dB[ownLds + r * PC] = stB[r];
+        }
+#pragma unroll
+        for (int m = 0; m < NHRW; ++m)
+            if (hrLds[m] >= 0) {
+                dP[hrLds[m]] = stP[R + m];
+                if (HET) dB[hrLds[m]] = stB[R + m];
+            }
+#pragma unroll
+        for (int m = 0; m < NHC; ++m)
+            if (hcLds[m] >= 0) {
+                dP[hcLds[m]] = stP[R + NHRW + m];
+                if (HET) dB[hcLds[m]] = stB[R + NHRW + m];
+            }
+    };
+    // Old values of plane q (Vy, Vz, kap) and of Vx face f.
+    auto load_old = [&](int q, int f, float* ox, float* oy, float* oz, float& oa, float& ob,
+                        float* ok) {
+        const bool fv = f >= 0 && f <= Nx - 2;
+        const bool qv = q >= i0 && q < i1;
+        const int bf = f * sx, bq = q * sx;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            ox[r] = (fv && ownV[r]) ? ldv(Vxi, bf + ownOff[r]) : 0.0f;
+            oy[r] = (qv && vyV[r]) ? ldv(Vyi, bq + ownOff[r]) : 0.0f;
+            oz[r] = (qv && vzV[r]) ? ldv(Vzi, bq + ownOff[r]) : 0.0f;
+            if (HET) ok[r] = (qv && ownV[r]) ? fp.kap[bq + ownOff[r]] : 0.0f;
+        }
+        oa = (qv && vA) ? ldv(Vyi, bq + offA) : 0.0f;
+        ob = (qv && vB) ? ldv(Vzi, bq + offB) : 0.0f;
+    };
+    auto slot_of = [](int pl) { return ((pl % NS) + NS) % NS; };
+    auto damp = [&](int dL, int dR) { return face_damp(sAlpha, dL, dR, dt); };
+
+    const int q0 = i0 - FLO - FHI;
+    for (int pl = q0; pl < q0 + LA; ++pl) {
+        load_plane(pl);
+        store_plane(slot_of(pl));
+    }
+    load_plane(q0 + LA);
+    load_old(q0, q0 + FHI, oVx, oVy, oVz, oA, oB, oK);
+    __syncthreads();
+
+    for (int q = q0; q < i1; ++q) {
+        __syncthreads();                                   // ring slot / face tiles free
+        store_plane(slot_of(q + LA));
+        if (q + 1 < i1) {
+            load_plane(q + 1 + LA);
+            load_old(q + 1, q + 1 + FHI, nVx, nVy, nVz, nA, nB, nK);
+        }
+        __syncthreads();                                   // ring plane visible
+
+        // ── Vx face f = q + FHI of own cells ──
+        {
+            const int f = q + FHI;
+            const bool fv = f >= 0 && f <= Nx - 2;
+            const bool inner = f > 0 && f < Nx - 2;
+            const bool own = f >= i0 && f < i1;
+            const float* s0 = sP + slot_of(f) * PLANE;
+            const float* s1 = sP + slot_of(f + 1) * PLANE;
+            const float* sm = sP + slot_of(f - 1) * PLANE;
+            const float* s2 = sP + slot_of(f + 2) * PLANE;
+            int di0 = 0, di1 = 0;
+            if (npml > 0) { di0 = pml_depth(f, Nx, npml); di1 = pml_depth(f + 1, Nx, npml); }
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                float nv = 0.0f;
+                if (fv && ownV[r]) {
+                    const int a = ownLds + r * PC;
+                    float coef = vcoef;
+                    if (HET) {
+                        const float* b0 = sB + slot_of(f) * PLANE;
+                        const float* b1 = sB + slot_of(f + 1) * PLANE;
+                        coef = (0.5f * (b0[a] + b1[a])) * dtdx;
+                    }
+                    nv = face_new<ORDER>(oVx[r], ORDER == 4 ? sm[a] : 0.0f, s0[a], s1[a],
+                                         ORDER == 4 ? s2[a] : 0.0f, inner, coef, C1, C2);
+                    if (npml > 0) nv = nv * damp(imax3(di0, dJ[r], dK), imax3(di1, dJ[r], dK));
+                    if (own) S::st(Vxo, f * sx + ownOff[r], S::exact ? nv : nv * sv);
+                }
+#pragma unroll
+                for (int s = 0; s < NQ - 1; ++s) vq[s][r] = vq[s + 1][r];
+                vq[NQ - 1][r] = nv;
+            }
+        }
+
+        if (q >= i0) {
+            const float* sq = sP + slot_of(q) * PLANE;
+            const float* bq = sB + (HET ? slot_of(q) * PLANE : 0);
+            const int base = q * sx;
+            const int dI = pml_depth(q, Nx, npml);
+            // ── Vy faces of own rows (face j between rows j and j+1) ──
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                float nv = 0.0f;
+                if (vyV[r]) {
+                    const int a = ownLds + r * PC;
+                    const float coef = HET ? (0.5f * (bq[a] + bq[a + PC])) * dtdx : vcoef;
+                    nv = face_new<ORDER>(oVy[r], ORDER == 4 ? sq[a - PC] : 0.0f, sq[a],
+                                         sq[a + PC], ORDER == 4 ? sq[a + 2 * PC] : 0.0f,
+                                         vyIn[r], coef, C1, C2);
+                    if (npml > 0) nv = nv * damp(imax3(dI, dJ[r], dK), imax3(dI, dJ1[r], dK));
+                    S::st(Vyo, base + ownOff[r], S::exact ? nv : nv * sv);
+                }
+                sVy[(FLO + w * R + r) * TZ + lane] = nv;
+            }
+            // ── halo Vy face row (task A) ──
+            if (tA) {
+                float nv = 0.0f;
+                if (vA) {
+                    const int a = (frA - FLO + PH) * PC + PH + lane;
+                    const float coef = HET ? (0.5f * (bq[a] + bq[a + PC])) * dtdx : vcoef;
+                    nv = face_new<ORDER>(oA, ORDER == 4 ? sq[a - PC] : 0.0f, sq[a], sq[a + PC],
+                                         ORDER == 4 ? sq[a + 2 * PC] : 0.0f, inA, coef, C1, C2);
+                    if (npml > 0) nv = nv * damp(imax3(dI, dJA, dK), imax3(dI, dJA1, dK));
+                }
+                sVy[frA * TZ + lane] = nv;
+            }
+            // ── Vz faces of own rows (face k between cols k and k+1) ──
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                float nv = 0.0f;
+                if (vzV[r]) {
+                    const int a = ownLds + r * PC;
+                    const float coef = HET ? (0.5f * (bq[a] + bq[a + 1])) * dtdx : vcoef;
+                    nv = face_new<ORDER>(oVz[r], ORDER == 4 ? sq[a - 1] : 0.0f, sq[a], sq[a + 1],
+                                         ORDER == 4 ? sq[a + 2] : 0.0f, vzIn, coef, C1, C2);
+                    if (npml > 0) nv = nv * damp(imax3(dI, dJ[r], dK), imax3(dI, dJ[r], dK1));
+                    S::st(Vzo, base + ownOff[r], S::exact ? nv : nv * sv);
+                }
+                sVz[(w * R + r) * FC + FLO + lane] = nv;
+            }
+            // ── halo Vz face (task B) ──
+            if (tB) {
+                float nv = 0.0f;
+                if (vB) {
+                    const int a = (PH + lyB) * PC + fcB - FLO + PH;
+                    const float coef = HET ? (0.5f * (bq[a] + bq[a + 1])) * dtdx : vcoef;
+                    nv = face_new<ORDER>(oB, ORDER == 4 ? sq[a - 1] : 0.0f, sq[a], sq[a + 1],
+                                         ORDER == 4 ? sq[a + 2] : 0.0f, inB, coef, C1, C2);
+                    if (npml > 0) nv = nv * damp(imax3(dI, dJB, dKB), imax3(dI, dJB, dKB1));
+                }
+                sVz[lyB * FC + fcB] = nv;
+            }
+            __syncthreads();                               // faces visible
+            // ── pressure of own cells at plane q ──
+            const bool xedge = (q == 0) | (q == Nx - 1);
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                if (!ownV[r]) continue;
+                const int fy = (FLO + w * R + r) * TZ + lane;
+                const int fz = (w * R + r) * FC + FLO + lane;
+                const float xp = vq[FLO][r], xm = vq[FLO - 1][r];
+                const float yp = sVy[fy], ym = sVy[fy - TZ];
+                const float zp = sVz[fz], zm = sVz[fz - 1];
+                float dv;
+                if (ORDER == 2) {
+                    dv = (xp - xm) + (yp - ym) + (zp - zm);
+                } else {
+                    const float xpp = vq[FLO + 1][r], xmm = vq[FLO - 2][r];
+                    const float ypp = sVy[fy + TZ], ymm = sVy[fy - 2 * TZ];
+                    const float zpp = sVz[fz + 1], zmm = sVz[fz - 2];
+                    dv = C1 * ((xp - xm) + (yp - ym) + (zp - zm))
+                       + C2 * ((xpp - xmm) + (ypp - ymm) + (zpp - zmm));
+                }
+                const float pc = sq[ownLds + r * PC];
+                const float kap = HET ? oK[r] : p.bulk_coeff;
+                float out;
+                if (p.impedance && (xedge || edgeYZ[r])) {
+                    float xi;
+                    if (HET) {
+                        xi = fp.xi[base + ownOff[r]];
+                    } else {
+                        const int j = j0 + w * R + r;
+                        xi = wall_xi(p.xterm, q, j, k, Nx, Ny, Nz);
+                    }
+                    out = impedance_update(pc, dv, kap, xi);
+                } else {
+                    out = pc - kap * dv;
+                }
+                if (npml > 0) out = out * sPd[imax3(dI, dJ[r], dK)];
+                S::st(Pout, base + ownOff[r], out);
+            }
+        }
+        // rotate staged old values
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            oVx[r] = nVx[r]; oVy[r] = nVy[r]; oVz[r] = nVz[r];
+            if (HET) oK[r] = nK[r];
+        }
+        oA = nA;
+        oB = nB;
+    }
+}
+
+}  // namespace fused

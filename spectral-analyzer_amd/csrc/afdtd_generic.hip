@@ -52,7 +52,7 @@ k_velocity(StencilParams p, StencilAux a, const T* __restrict__ P, T* __restrict
     // x face (i+1/2, j, k)
     if (i < p.Nx - 1) {
         float g = face_gradient(P, c, sx, p.order, i > 0 && i < p.Nx - 2, p.C1, p.C2);
-        float coef = p.hetero ? p.dt / ((0.5f * (a.rho[c] + a.rho[c + sx])) * p.dx) : p.vel_coeff;
+        float coef = p.hetero ? (0.5f * (a.bu[c] + a.bu[c + sx])) * p.dtdx : p.vel_coeff;
         float v = S::ld(Vx, c);
         if (!S::exact) v = v * p.inv_sV;
         float nv = v - coef * g;
@@ -67,7 +67,7 @@ k_velocity(StencilParams p, StencilAux a, const T* __restrict__ P, T* __restrict
     // y face (i, j+1/2, k)
     if (j < p.Ny - 1) {
         float g = face_gradient(P, c, sy, p.order, j > 0 && j < p.Ny - 2, p.C1, p.C2);
-        float coef = p.hetero ? p.dt / ((0.5f * (a.rho[c] + a.rho[c + sy])) * p.dx) : p.vel_coeff;
+        float coef = p.hetero ? (0.5f * (a.bu[c] + a.bu[c + sy])) * p.dtdx : p.vel_coeff;
         float v = S::ld(Vy, c);
         if (!S::exact) v = v * p.inv_sV;
         float nv = v - coef * g;
@@ -82,7 +82,7 @@ k_velocity(StencilParams p, StencilAux a, const T* __restrict__ P, T* __restrict
     // z face (i, j, k+1/2)
     if (k < p.Nz - 1) {
         float g = face_gradient(P, c, 1, p.order, k > 0 && k < p.Nz - 2, p.C1, p.C2);
-        float coef = p.hetero ? p.dt / ((0.5f * (a.rho[c] + a.rho[c + 1])) * p.dx) : p.vel_coeff;
+        float coef = p.hetero ? (0.5f * (a.bu[c] + a.bu[c + 1])) * p.dtdx : p.vel_coeff;
         float v = S::ld(Vz, c);
         if (!S::exact) v = v * p.inv_sV;
         float nv = v - coef * g;
@@ -145,9 +145,8 @@ k_pressure(StencilParams p, StencilAux a, const T* __restrict__ P, T* __restrict
     float out;
     const bool edge = (i == 0) | (i == Nx - 1) | (j == 0) | (j == Ny - 1) | (k == 0) | (k == Nz - 1);
     if (p.impedance && edge) {
-        float r = p.hetero ? a.rho[c] : p.rho_air;
-        float KK = p.hetero ? a.K[c] : p.K0;
-        out = impedance_update(pc, dv, kap, r, KK, p.gside, i, j, k, Nx, Ny, Nz);
+        float xi = p.hetero ? a.xi[c] : wall_xi(p.xterm, i, j, k, Nx, Ny, Nz);
+        out = impedance_update(pc, dv, kap, xi);
     } else {
         out = pc - kap * dv;
     }
@@ -163,7 +162,7 @@ StencilAux make_aux(const AcousticFDTDState* st) {
     StencilAux a;
     a.ct = st->d_ct;
     a.fx = st->d_fx; a.fy = st->d_fy; a.fz = st->d_fz;
-    a.rho = st->d_rho; a.kap = st->d_kap; a.K = st->d_K;
+    a.bu = st->d_bu; a.kap = st->d_kap; a.xi = st->d_xi;
     a.alpha = st->d_alpha; a.pdamp = st->d_pdamp;
     a.status = st->d_status;
     return a;

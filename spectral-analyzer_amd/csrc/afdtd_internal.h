@@ -43,7 +43,9 @@ struct StencilParams {
     float dt, dx, vel_coeff, bulk_coeff;
     float C1, C2;
     float gside[6];     // (1-R)/(1+R) per domain side x-,x+,y-,y+,z-,z+
+    float xterm[6];     // homogeneous wall terms (kap/2) * g_side / sqrt(rho K)
     float rho_air, K0;  // homogeneous medium
+    float dtdx;         // dt/dx (heterogeneous face coefficient)
     float sV, inv_sV;   // power-of-two velocity storage scale (16-bit storage)
 };
 
@@ -53,9 +55,9 @@ struct StencilAux {
     const float* fx;        // face fractions, padded layout (has_closed)
     const float* fy;
     const float* fz;
-    const float* rho;       // per-cell density (hetero)
+    const float* bu;        // per-cell buoyancy 1/rho (hetero)
     const float* kap;       // per-cell K*dt/dx (hetero)
-    const float* K;         // per-cell bulk modulus (hetero)
+    const float* xi;        // per-cell wall term sum (hetero + impedance; edges only)
     const float* alpha;     // alpha(d), d = 0..n_pml
     const float* pdamp;     // 1/(1+alpha(d) dt)
     const int* status;      // nonzero: a previous step failed, skip work
@@ -75,14 +77,15 @@ struct AcousticFDTDState {
     int esz;
     void* P[2];             // P[cur] = P^n
     int cur;
-    void* V[3];
+    void* V[3];             // V^{n-1/2} (current)
+    void* V2[3];            // second velocity buffer of the fused sweep
     // Conversion staging for non-FP32 storage.
     float* stage;
 
     // Stencil aux arrays.
     uint8_t* d_ct;
     float *d_fx, *d_fy, *d_fz;
-    float *d_rho, *d_kap, *d_K;
+    float *d_bu, *d_kap, *d_xi;
     float *d_alpha, *d_pdamp;
     int* d_status;          // [0] code, [1] step index at failure
     std::vector<uint8_t> h_ct;   // cell types after retyping (host copy)

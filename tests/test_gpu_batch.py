@@ -1,0 +1,50 @@
+"""Batched rooms: every room of one graph-captured batch sweep equals the
+oracle run of that room alone, bit for bit (FP32)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _rooms(n, seed, lo=20, hi=45):
+    rng = np.random.default_rng(seed)
+    dims = rng.integers(lo, hi + 1, size=(n, 3))
+    courant = rng.uniform(0.4, 0.55, n).astype(np.float32)
+    refl = np.repeat(rng.uniform(0.6, 0.98, (n, 1)), 6, axis=1).astype(np.float32)
+    return rng, dims, courant, refl
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_batch_rooms_match_oracle(oracle_mod, graph):
+    from afdtd import RoomBatch, differentiated_gaussian
+    n, steps = 5, 60
+    rng, dims, courant, refl = _rooms(n, 31)
+    b = RoomBatch(dims, 0.01, courant, refl, c=343.0, rho_air=1.2, order=2, tile_y=8, seg_x=16)
+    src = []
+    rec = np.zeros((n, 8, 3), np.float32)
+    for r, (Nx, Ny, Nz) in enumerate(dims):
+        i, j, k = (rng.integers(4, d - 4) for d in (Nx, Ny, Nz))
+        src.append(int(k + Nz * (j + Ny * i)))
+        rec[r] = rng.uniform(3, np.array([Nx, Ny, Nz]) - 4, size=(8, 3))
+    sig = np.stack([differentiated_gaussian(steps, 6.0, 20.0, 1.0)] * n)
+    b.set_sources(src, sig)
+    b.set_receivers(rec)
+    b.record(steps)
+    b.step(steps // 2, graph=graph)
+    b.step(steps - steps // 2, graph=graph)
+    tr = b.get_traces()
+    assert tr.shape == (steps, n, 8)
+    for r, (Nx, Ny, Nz) in enumerate(dims):
+        o = oracle_mod.OracleFDTD(int(Nx), int(Ny), int(Nz), 0.01, c=343.0, rho_air=1.2,
+                                  n_pml=0, order=2, courant=float(courant[r]), boundary=1,
+                                  refl=refl[r], precision="f32")
+        o.set_point_sources([src[r]], sig[r:r + 1])
+        o.set_receivers(rec[r])
+        o.record_receivers(steps)
+        o.step(steps)
+        to = o.get_receiver_traces().astype(np.float32)
+        assert np.array_equal(tr[:, r, :].view(np.uint32), to.view(np.uint32)), f"room {r}"
+        Pg = b.get_pressure(r)
+        Po = o.get_pressure_field()
+        assert np.array_equal(Pg.view(np.uint32), Po.view(np.uint32)), f"room {r} field"
+        assert np.abs(to).max() > 0

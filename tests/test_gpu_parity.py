@@ -245,3 +245,50 @@ def test_reset_and_errors(oracle_mod):
     assert L.fdtd_inject_bridge(g._h, 1.0, 0.0, 1.0) == -3
     with pytest.raises(RuntimeError, match="disabled"):
         g.inject_bridge(1.0, 0.0)
+
+
+@pytest.mark.parametrize("order,hetero,imped,n_pml", [(4, False, False, 0), (4, True, False, 5),
+                                                      (2, True, True, 0), (2, False, False, 3)])
+@pytest.mark.parametrize("tile_y,seg_x", [(16, 64), (8, 5), (32, 13)])
+def test_fused_sweep_bitwise(oracle_mod, order, hetero, imped, n_pml, tile_y, seg_x):
+    """The fused plane-marching sweep equals the oracle bit for bit on ragged
+    grids (Ny, Nz not multiples of the tile, Nx not a multiple of the segment)."""
+    Nx, Ny, Nz = 23, 37, 70
+    kw = dict(order=order, n_pml=n_pml, kernel="fused", tile_y=tile_y, seg_x=seg_x)
+    if order == 2:
+        kw["courant"] = 0.5
+    if imped:
+        kw.update(boundary="impedance", refl=[0.6, 0.7, 0.8, 0.85, 0.9, 0.95])
+    if hetero:
+        rng = np.random.default_rng(12)
+        kw["rho"] = (1.2 * np.exp(0.1 * rng.standard_normal(Nx * Ny * Nz))).astype(np.float32)
+        kw["bulk"] = (1.2 * 343.0 ** 2 * np.exp(0.1 * rng.standard_normal(Nx * Ny * Nz))).astype(np.float32)
+    okw = {k: v for k, v in kw.items() if k not in ("tile_y", "seg_x")}
+    g, o = _pair(oracle_mod, Nx, Ny, Nz, **okw)
+    g.close()
+    from afdtd import AcousticFDTD
+    ct = np.zeros(Nx * Ny * Nz, np.uint8)
+    pa = np.zeros(Nx * Ny, np.uint8)
+    g = AcousticFDTD(Nx, Ny, Nz, 0.01, c=343.0, rho_air=1.21, cell_type=ct, plate_iz=Nz // 2,
+                     plate_active=pa, **kw)
+    assert g.kernel_family == "fused"
+    _random_state(o, g, seed=13)
+    for chunk in (1, 2, 9):
+        g.step(chunk); o.step(chunk)
+        _assert_fields_equal(g, o, f"fused o{order} het={hetero} imp={imped} ty={tile_y} seg={seg_x} n={g.step_count}")
+
+
+def test_fused_unstable_restores_buffers(oracle_mod):
+    """A failing stability check stops the fused sweep at step 256 with the
+    same fields as the oracle (velocity double-buffer parity restored)."""
+    g, o = _pair(oracle_mod, 20, 20, 20, n_pml=0, kernel="fused")
+    assert g.kernel_family == "fused"
+    sig = np.full((1, 401), 1e6, np.float32)
+    src = [10 + 20 * (10 + 20 * 10)]
+    for m in (g, o):
+        m.set_point_sources(src, sig)
+    assert o.step(401) == -3
+    with pytest.raises(RuntimeError, match="FDTD_ERR_UNSTABLE"):
+        g.step(401)
+    assert g.step_count == o.step_count == 256
+    _assert_fields_equal(g, o, "fused unstable")

@@ -305,6 +305,9 @@ def test_two_layer_interface_reflection(oracle_mod):
 
 
 def test_heterogeneous_constant_fields_reduce_to_homogeneous(oracle_mod):
+    """Constant rho, K through the per-cell path agree with the homogeneous
+    path to float rounding (the face coefficient (1/rho)*(dt/dx) differs from
+    dt/(rho dx) by at most an ulp)."""
     N = 12
     rng = np.random.default_rng(6)
     P0 = rng.standard_normal(N ** 3).astype(np.float32)
@@ -315,7 +318,8 @@ def test_heterogeneous_constant_fields_reduce_to_homogeneous(oracle_mod):
     for m in (a, b):
         m.set_fields(P=P0); m.step(20)
     for u, v in zip(a.get_fields(), b.get_fields()):
-        assert np.array_equal(u.view(np.uint32), v.view(np.uint32))
+        err = np.linalg.norm(u.astype(np.float64) - v) / np.linalg.norm(v)
+        assert err < 1e-5, err
 
 
 def test_trilinear_samplers_exact_on_affine_fields(oracle_mod):
