@@ -125,3 +125,21 @@ def config2_order4_pml(seed: int = 2, Nx=512, Ny=384, Nz=256, steps=4000) -> Wor
     ri, rj, rk = _interior_cells(rng, Nx, Ny, Nz, 64, margin=16)
     w.rec = np.stack([ri, rj, rk], 1).astype(np.float32)
     return w
+
+
+def config4(seed: int = 4, Nx=1024, Ny=1024, Nz=512, steps=3000, full=(1024, 1024, 512)) -> Workload:
+    """Precision workload: homogeneous air, order 2 at Courant 0.5, walls with
+    R = 0.95 on every side, 2 sources and 128 receivers.  Source and receiver
+    positions are drawn as fractions of the full 1024x1024x512 room and mapped
+    proportionally, so a smaller grid is the crop-and-rescale of the same
+    seeded set-up (same dx; convention used for the FP64 comparison)."""
+    rng = np.random.default_rng(seed)
+    fs = rng.uniform(0.2, 0.8, size=(2, 3))
+    fr = rng.uniform(0.1, 0.9, size=(128, 3))
+    w = Workload(f"precision-room-{Nx}x{Ny}x{Nz}-order2", Nx, Ny, Nz, 0.01, steps, 2, 0.5,
+                 "impedance", [0.95] * 6)
+    dims = np.array([Nx, Ny, Nz])
+    si = np.minimum((fs * dims).astype(int), dims - 1)
+    w.src = [int(k + Nz * (j + Ny * i)) for i, j, k in si]
+    w.rec = (fr * (dims - 1)).astype(np.float32)
+    return w

@@ -237,7 +237,7 @@ SK_API FdtdBatch* fdtd_batch_create(int n_rooms, const int* dims, float dx, floa
     b->storage = storage;
     b->esz = storage == FDTD_STORE_F32 ? 4 : 2;
     b->tile_y = tile_y > 0 ? tile_y : 16;
-    b->seg = seg_x > 0 ? seg_x : 64;
+    b->seg = seg_x > 0 ? seg_x : 32;
     b->boundary = refl ? FDTD_BOUNDARY_IMPEDANCE : FDTD_BOUNDARY_RIGID_PML;
     const int TYv = b->tile_y;
     StencilParams& sp = b->common;

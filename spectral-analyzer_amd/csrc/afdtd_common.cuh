@@ -16,6 +16,10 @@ template <typename T> struct Store;
 template <> struct Store<float> {
     static __device__ __forceinline__ float ld(const float* p, int i) { return p[i]; }
     static __device__ __forceinline__ void st(float* p, int i, float v) { p[i] = v; }
+    // streaming store: written once per step, not re-read by this sweep
+    static __device__ __forceinline__ void st_nt(float* p, int i, float v) {
+        __builtin_nontemporal_store(v, p + i);
+    }
     static constexpr bool exact = true;
 };
 template <> struct Store<bf16_t> {
@@ -25,11 +29,13 @@ template <> struct Store<bf16_t> {
     static __device__ __forceinline__ void st(bf16_t* p, int i, float v) {
         p[i] = __float2bfloat16(v);
     }
+    static __device__ __forceinline__ void st_nt(bf16_t* p, int i, float v) { st(p, i, v); }
     static constexpr bool exact = false;
 };
 template <> struct Store<f16_t> {
     static __device__ __forceinline__ float ld(const f16_t* p, int i) { return (float)p[i]; }
     static __device__ __forceinline__ void st(f16_t* p, int i, float v) { p[i] = (f16_t)v; }
+    static __device__ __forceinline__ void st_nt(f16_t* p, int i, float v) { st(p, i, v); }
     static constexpr bool exact = false;
 };
 

@@ -112,7 +112,7 @@ hipError_t launch_fused_step(AcousticFDTDState* st) {
     a.pdamp = st->d_pdamp;
     a.status = st->d_status;
     int ty = st->fused_tile_y > 0 ? st->fused_tile_y : 16;
-    a.seg = st->fused_seg_x > 0 ? st->fused_seg_x : 64;
+    a.seg = st->fused_seg_x > 0 ? st->fused_seg_x : 32;
     if (a.seg > st->Nx) a.seg = st->Nx;
     hipError_t e = st->order == 4 ? launch_store<4>(st, a, ty) : launch_store<2>(st, a, ty);
     if (e == hipSuccess) {

@@ -33,7 +33,8 @@ struct Geo {
     static constexpr int FHI = ORDER == 4 ? 1 : 0;  // faces above own range
     static constexpr int GHI = ORDER == 4 ? 2 : 1;  // gradient taps above face
     static constexpr int LA = FHI + GHI;            // P-plane lookahead
-    static constexpr int NS = LA + 1;               // ring depth
+    static constexpr int NS = LA + 2;               // ring depth (+1: the slot being
+                                                    // filled was last read two planes ago)
     static constexpr int NQ = FLO + FHI + 1;        // Vx queue length
 };
 
@@ -271,7 +272,9 @@ __device__ __forceinline__ void fused_tile(const StencilParams& p, const FusedPt
     __syncthreads();
 
     for (int q = q0; q < i1; ++q) {
-        __syncthreads();                                   // ring slot / face tiles free
+        // The slot of plane q+LA was last read at plane q-2 (ring depth LA+2)
+        // and the face tiles at plane q-1 before the previous iteration's
+        // last barrier, so no barrier is needed before refilling them.
         store_plane(slot_of(q + LA));
         if (q + 1 < i1) {
             load_plane(q + 1 + LA);
@@ -305,7 +308,7 @@ __device__ __forceinline__ void fused_tile(const StencilParams& p, const FusedPt
                     nv = face_new<ORDER>(oVx[r], ORDER == 4 ? sm[a] : 0.0f, s0[a], s1[a],
                                          ORDER == 4 ? s2[a] : 0.0f, inner, coef, C1, C2);
                     if (npml > 0) nv = nv * damp(imax3(di0, dJ[r], dK), imax3(di1, dJ[r], dK));
-                    if (own) S::st(Vxo, f * sx + ownOff[r], S::exact ? nv : nv * sv);
+                    if (own) S::st_nt(Vxo, f * sx + ownOff[r], S::exact ? nv : nv * sv);
                 }
 #pragma unroll
                 for (int s = 0; s < NQ - 1; ++s) vq[s][r] = vq[s + 1][r];
@@ -329,7 +332,7 @@ __device__ __forceinline__ void fused_tile(const StencilParams& p, const FusedPt
                                          sq[a + PC], ORDER == 4 ? sq[a + 2 * PC] : 0.0f,
                                          vyIn[r], coef, C1, C2);
                     if (npml > 0) nv = nv * damp(imax3(dI, dJ[r], dK), imax3(dI, dJ1[r], dK));
-                    S::st(Vyo, base + ownOff[r], S::exact ? nv : nv * sv);
+                    S::st_nt(Vyo, base + ownOff[r], S::exact ? nv : nv * sv);
                 }
                 sVy[(FLO + w * R + r) * TZ + lane] = nv;
             }
@@ -355,7 +358,7 @@ __device__ __forceinline__ void fused_tile(const StencilParams& p, const FusedPt
                     nv = face_new<ORDER>(oVz[r], ORDER == 4 ? sq[a - 1] : 0.0f, sq[a], sq[a + 1],
                                          ORDER == 4 ? sq[a + 2] : 0.0f, vzIn, coef, C1, C2);
                     if (npml > 0) nv = nv * damp(imax3(dI, dJ[r], dK), imax3(dI, dJ[r], dK1));
-                    S::st(Vzo, base + ownOff[r], S::exact ? nv : nv * sv);
+                    S::st_nt(Vzo, base + ownOff[r], S::exact ? nv : nv * sv);
                 }
                 sVz[(w * R + r) * FC + FLO + lane] = nv;
             }
@@ -408,7 +411,7 @@ __device__ __forceinline__ void fused_tile(const StencilParams& p, const FusedPt
                     out = pc - kap * dv;
                 }
                 if (npml > 0) out = out * sPd[imax3(dI, dJ[r], dK)];
-                S::st(Pout, base + ownOff[r], out);
+                S::st_nt(Pout, base + ownOff[r], out);
             }
         }
         // rotate staged old values
