@@ -277,6 +277,27 @@ SK_API int fdtd_batch_get_traces(const FdtdBatch* b, float* out, int len, int* n
 SK_API int fdtd_batch_get_pressure(const FdtdBatch* b, int room, float* out, int len);
 SK_API int fdtd_batch_get_step_count(const FdtdBatch* b);
 
+/* ── Adjoint (no reference counterpart) ──────────────────────────────────
+ * Gradients of receiver traces of the order-2 heterogeneous step with
+ * impedance walls with respect to per-cell m = log rho, l = log K and the
+ * per-boundary-cell admittance sum eta.  precision_bytes 4 or 8 selects the
+ * REAL type of every array; parameters, traces and gradients are DEVICE
+ * arrays (e.g. torch tensors), sources/receivers are host arrays.  The
+ * forward stores (P, V) checkpoints every checkpoint_every steps; the
+ * backward recomputes each segment and runs the time-reversed adjoint. */
+typedef struct FdtdAdjoint FdtdAdjoint;
+SK_API FdtdAdjoint* fdtd_adjoint_create(int Nx, int Ny, int Nz, double dtdx,
+                                        int precision_bytes, int checkpoint_every);
+SK_API void fdtd_adjoint_destroy(FdtdAdjoint* a);
+SK_API int fdtd_adjoint_set_sources(FdtdAdjoint* a, int n_src, const int* idx,
+                                    const void* signals, int n_samples);
+SK_API int fdtd_adjoint_set_receivers(FdtdAdjoint* a, int n_rec, const int* idx8,
+                                      const void* wgt8);
+SK_API int fdtd_adjoint_forward(FdtdAdjoint* a, const void* m, const void* l,
+                                const void* eta, int n_steps, void* traces);
+SK_API int fdtd_adjoint_backward(FdtdAdjoint* a, const void* grad_traces, void* grad_m,
+                                 void* grad_l, void* grad_eta);
+
 #ifdef __cplusplus
 }
 #endif

@@ -94,6 +94,12 @@ _SIGS = {
     "fdtd_batch_get_pressure": (_i, [_vp, _i, _vp, _i]),
     "fdtd_batch_get_step_count": (_i, [_vp]),
     "fdtd_copy_calibration": (_i, [C.c_size_t, _i, _vp]),
+    "fdtd_adjoint_create": (_vp, [_i, _i, _i, C.c_double, _i, _i]),
+    "fdtd_adjoint_destroy": (None, [_vp]),
+    "fdtd_adjoint_set_sources": (_i, [_vp, _i, _vp, _vp, _i]),
+    "fdtd_adjoint_set_receivers": (_i, [_vp, _i, _vp, _vp]),
+    "fdtd_adjoint_forward": (_i, [_vp, _vp, _vp, _vp, _i, _vp]),
+    "fdtd_adjoint_backward": (_i, [_vp, _vp, _vp, _vp, _vp]),
 }
 
 
