@@ -111,6 +111,119 @@ def measured_traffic(workload, kernel, storage):
     return None
 
 
+def bench_batch(a):
+    """Config 3: one graph-captured batched sweep over 256 rooms of uneven
+    size against the same rooms stepped one handle at a time."""
+    from afdtd import RoomBatch, AcousticFDTD, workloads as W
+    w = W.config3(steps=a.steps)
+    dims, steps = w["dims"], w["steps"]
+    cells = int(np.prod(dims, axis=1).sum())
+    b = RoomBatch(dims, 0.01, w["courant"], w["refl"], c=343.0, rho_air=1.2, order=2,
+                  tile_y=a.tile_y, seg_x=a.seg_x)
+    b.set_sources(w["src"], w["signals"])
+    b.set_receivers(w["rec"])
+    b.record(steps + 10)
+    b.step(10)                                       # warm-up (graph path)
+    t0 = time.perf_counter()
+    ms_dev = b.step(steps, graph=True)
+    wall_b = time.perf_counter() - t0
+    tr_b = b.get_traces()
+    del b
+    # the same rooms, one plain handle each (per-room launches)
+    t_rooms, check = 0.0, []
+    for r, (Nx, Ny, Nz) in enumerate(dims):
+        Nx, Ny, Nz = int(Nx), int(Ny), int(Nz)
+        s = AcousticFDTD(Nx, Ny, Nz, 0.01, c=343.0, rho_air=1.2,
+                         cell_type=np.zeros(Nx * Ny * Nz, np.uint8), plate_iz=Nz // 2,
+                         plate_active=np.zeros(Nx * Ny, np.uint8), n_pml=0, order=2,
+                         courant=float(w["courant"][r]), boundary="impedance", refl=w["refl"][r],
+                         tile_y=a.tile_y, seg_x=a.seg_x)
+        s.set_point_sources([w["src"][r]], w["signals"][r:r + 1])
+        s.set_receivers(w["rec"][r])
+        s.record_receivers(steps + 10)
+        s.step(10)
+        t0 = time.perf_counter()
+        s.step(steps)
+        t_rooms += time.perf_counter() - t0
+        if r < 8:
+            check.append(np.array_equal(s.get_receiver_traces().view(np.uint32),
+                                        tr_b[:, r, :].view(np.uint32)))
+        del s
+    line = {"metric": "cell-updates/s (batched rooms, 1 GPU)", "value": cells * steps / wall_b,
+            "unit": "cell-updates/s", "n_gpus": 1, "steps": steps, "warmup": 10,
+            "ms_per_step": wall_b * 1e3 / steps, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "fp32", "data": "synthetic (seeded rooms)",
+            "config": {"workload": "config3-256-rooms-48to160-order2", "rooms": int(len(dims)),
+                       "cells": cells, "graph_device_ms": ms_dev},
+            "per_room_launches": {"value": cells * steps / t_rooms, "unit": "cell-updates/s",
+                                  "seconds": t_rooms},
+            "speedup_vs_per_room": t_rooms / wall_b,
+            "traces_bitwise_equal_first8": bool(all(check))}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def bench_adjoint(a):
+    """Config 5: 128^3, 2000 steps, checkpoints every 100: forward + adjoint
+    (FP32) timing; float64 central differences on 64 random cells."""
+    import torch
+    from afdtd import workloads as W
+    from afdtd.adjoint import AdjointSolver, simulate_traces
+    w = W.config5(steps=a.steps)
+    shape, steps = w["shape"], w["steps"]
+    N = int(np.prod(shape))
+    res = {}
+    for dtype, tdt in (("float32", torch.float32), ("float64", torch.float64)):
+        s = AdjointSolver(*shape, w["dtdx"], dtype=dtype, checkpoint_every=100)
+        s.set_sources(w["src"], w["signals"])
+        s.set_receivers(w["rec"])
+        dev = lambda x: torch.tensor(np.ascontiguousarray(x).ravel(), dtype=tdt, device="cuda")
+        with torch.no_grad():
+            target = s.forward(dev(w["m"] + w["pert"][0]), dev(w["l"] + w["pert"][1]), dev(w["eta"]), steps).clone()
+        mg, lg, eg = (dev(x).requires_grad_() for x in (w["m"], w["l"], w["eta"]))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        tr = simulate_traces(s, mg, lg, eg, steps)
+        loss = ((tr - target) ** 2).sum()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        loss.backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        res[dtype] = dict(solver=s, grads=[g.grad.double().cpu().numpy() for g in (mg, lg, eg)],
+                          fwd=t1 - t0, bwd=t2 - t1, dev=dev, target=target)
+    # central differences (float64) on 64 random cells
+    r64 = res["float64"]
+    s, dev, target = r64["solver"], r64["dev"], r64["target"]
+    rng = np.random.default_rng(55)
+    cells = rng.choice(N, 64, replace=False)
+    worst, h = 0.0, 1e-5
+    gm = r64["grads"][0]
+    for q, c in enumerate(cells):
+        which = q % 2                                   # alternate log rho / log K
+        g = r64["grads"][which]
+        vals = []
+        for sgn in (1, -1):
+            p = [w["m"].ravel().copy(), w["l"].ravel().copy()]
+            p[which][c] += sgn * h
+            with torch.no_grad():
+                tr = s.forward(dev(p[0]), dev(p[1]), dev(w["eta"]), steps)
+                vals.append(float(((tr - target) ** 2).sum()))
+        fd = (vals[0] - vals[1]) / (2 * h)
+        worst = max(worst, abs(g[c] - fd) / max(abs(fd), 1e-3 * np.abs(g).max()))
+    g32, g64 = res["float32"]["grads"], res["float64"]["grads"]
+    rel32 = [float(np.linalg.norm(x - y) / np.linalg.norm(y)) for x, y in zip(g32, g64)]
+    bwd = res["float32"]["bwd"]
+    line = {"metric": "adjoint reverse-step cell-updates/s (1 GPU)",
+            "value": N * steps / bwd, "unit": "cell-updates/s", "n_gpus": 1, "steps": steps,
+            "higher_is_better": True, "dtype": "fp32", "data": "synthetic (seeded media)",
+            "config": {"workload": "config5-128cube-2000steps-ckpt100", "cells": N},
+            "forward_s_fp32": res["float32"]["fwd"], "backward_s_fp32": bwd,
+            "fd_worst_rel_err_fp64_64cells": worst, "fp32_vs_fp64_grad_rel_l2": rel32}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -127,6 +240,14 @@ def main():
     if a.copy_calibration:
         copy_calibration()
         return 0
+    if a.config == "batch":
+        if a.steps == 4000:
+            a.steps = 1000
+        return bench_batch(a)
+    if a.config == "adjoint":
+        if a.steps == 4000:
+            a.steps = 2000
+        return bench_adjoint(a)
     if a.gpus != 1:
         print("bench.py measures one MI355X: the multi-GPU (slab-sharded) path is a "
               "design only in this build; run with --gpus 1", file=sys.stderr)

@@ -143,3 +143,40 @@ def config4(seed: int = 4, Nx=1024, Ny=1024, Nz=512, steps=3000, full=(1024, 102
     w.src = [int(k + Nz * (j + Ny * i)) for i, j, k in si]
     w.rec = (fr * (dims - 1)).astype(np.float32)
     return w
+
+
+def config3(seed: int = 3, n_rooms=256, lo=48, hi=160, steps=1000):
+    """Batched rooms: dimensions uniform per axis in [lo, hi], per-room
+    Courant in [0.4, 0.55] and reflection coefficient in [0.6, 0.98] (all six
+    sides), one differentiated-Gaussian source and 8 receivers per room."""
+    rng = np.random.default_rng(seed)
+    dims = rng.integers(lo, hi + 1, size=(n_rooms, 3))
+    courant = rng.uniform(0.4, 0.55, n_rooms).astype(np.float32)
+    refl = np.repeat(rng.uniform(0.6, 0.98, (n_rooms, 1)), 6, axis=1).astype(np.float32)
+    src, rec = [], np.zeros((n_rooms, 8, 3), np.float32)
+    for r, (Nx, Ny, Nz) in enumerate(dims):
+        i, j, k = (int(rng.integers(8, d - 8)) for d in (Nx, Ny, Nz))
+        src.append(k + int(Nz) * (j + int(Ny) * i))
+        rec[r] = rng.uniform(8, np.array([Nx, Ny, Nz]) - 9, size=(8, 3))
+    sig = differentiated_gaussian(steps, 8.0, 48.0, 1.0)
+    return dict(dims=dims, courant=courant, refl=refl, src=src, rec=rec, steps=steps,
+                signals=np.repeat(sig[None, :], n_rooms, axis=0))
+
+
+def config5(seed: int = 5, N=128, steps=2000):
+    """Differentiation: N^3 cells, log rho and log K seeded log-normal with
+    sigma 0.2, per-boundary-cell admittance sum eta, 32 receivers, targets
+    from a seeded perturbed parameter set."""
+    rng = np.random.default_rng(seed)
+    shape = (N, N, N)
+    m = np.log(RHO0) + 0.2 * rng.standard_normal(shape)
+    l = np.log(RHO0 * C0 * C0) + 0.2 * rng.standard_normal(shape)
+    eta = np.full(shape, 2 * (1 - 0.9) / (1 + 0.9))
+    pert = [0.05 * rng.standard_normal(shape), 0.05 * rng.standard_normal(shape)]
+    cmax = float(np.exp(0.5 * (l - m)).max())
+    dtdx = 0.5 / cmax
+    src = [int(N // 3 + N * (N // 2 + N * (N // 2)))]
+    rec = rng.uniform(8, N - 9, size=(32, 3))
+    sig = differentiated_gaussian(steps, 8.0, 48.0, 1.0).astype(np.float64)[None, :]
+    return dict(shape=shape, m=m, l=l, eta=eta, pert=pert, dtdx=dtdx, src=src, rec=rec,
+                steps=steps, signals=sig)
