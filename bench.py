@@ -267,7 +267,7 @@ def main():
     if not np.all(np.isfinite(traces)):
         raise SystemExit("non-finite receiver traces")
     value = w.cells * a.steps / wall
-    bpc = w.bytes_per_cell_update if a.storage == "f32" else w.bytes_per_cell_update // 2
+    bpc = w.bytes_per_cell_update_for(a.storage)
     achieved = w.cells * bpc / (ms_stencil * 1e-3) / 1e9
     line = {
         "metric": "cell-updates/s (3D FDTD, 1 GPU)",

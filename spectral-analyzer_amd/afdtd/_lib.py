@@ -9,7 +9,8 @@ import ctypes as C
 import os
 
 _PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_PATH = os.path.join(_PKG_DIR, "libafdtd.so")
+# AFDTD_LIB selects another build of the same library (e.g. a tuning variant).
+LIB_PATH = os.environ.get("AFDTD_LIB") or os.path.join(_PKG_DIR, "libafdtd.so")
 
 FDTD_AIR, FDTD_WALL, FDTD_PLATE, FDTD_PML = 0, 1, 2, 3
 FDTD_OK, FDTD_ERR_NULL, FDTD_ERR_DIM, FDTD_ERR_UNSTABLE = 0, -1, -2, -3

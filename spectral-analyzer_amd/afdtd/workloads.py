@@ -55,7 +55,13 @@ class Workload:
         """Algorithmic HBM bytes per cell-update of the FP32 step: P, Vx, Vy,
         Vz read and written once (32 B) plus one read of rho and K (8 B)
         when the medium is heterogeneous."""
-        return 40 if self.hetero else 32
+        return self.bytes_per_cell_update_for("f32")
+
+    def bytes_per_cell_update_for(self, storage: str) -> int:
+        """Same count for a field storage type: 8 field accesses of 4 B
+        (f32) or 2 B (bf16, f16); per-cell rho and K stay FP32."""
+        elem = 4 if storage == "f32" else 2
+        return 8 * elem + (8 if self.hetero else 0)
 
     def signals(self, n: int) -> np.ndarray:
         return np.stack([differentiated_gaussian(n, self.sigma, self.n0, 1.0)

@@ -13,7 +13,12 @@ typedef __hip_bfloat16 bf16_t;
 typedef _Float16 f16_t;
 
 template <typename T> struct Store;
+// Raw / ldr / cvt split a load from its conversion, so a value loaded one
+// loop iteration ahead stays in flight until it is used.
 template <> struct Store<float> {
+    typedef float Raw;
+    static __device__ __forceinline__ Raw ldr(const float* p, int i) { return p[i]; }
+    static __device__ __forceinline__ float cvt(Raw r) { return r; }
     static __device__ __forceinline__ float ld(const float* p, int i) { return p[i]; }
     static __device__ __forceinline__ void st(float* p, int i, float v) { p[i] = v; }
     // streaming store: written once per step, not re-read by this sweep
@@ -23,6 +28,13 @@ template <> struct Store<float> {
     static constexpr bool exact = true;
 };
 template <> struct Store<bf16_t> {
+    typedef uint16_t Raw;
+    static __device__ __forceinline__ Raw ldr(const bf16_t* p, int i) {
+        return ((const uint16_t*)p)[i];
+    }
+    static __device__ __forceinline__ float cvt(Raw r) {
+        return __uint_as_float((uint32_t)r << 16);
+    }
     static __device__ __forceinline__ float ld(const bf16_t* p, int i) {
         return __bfloat162float(p[i]);
     }
@@ -33,6 +45,9 @@ template <> struct Store<bf16_t> {
     static constexpr bool exact = false;
 };
 template <> struct Store<f16_t> {
+    typedef f16_t Raw;
+    static __device__ __forceinline__ Raw ldr(const f16_t* p, int i) { return p[i]; }
+    static __device__ __forceinline__ float cvt(Raw r) { return (float)r; }
     static __device__ __forceinline__ float ld(const f16_t* p, int i) { return (float)p[i]; }
     static __device__ __forceinline__ void st(f16_t* p, int i, float v) { p[i] = (f16_t)v; }
     static __device__ __forceinline__ void st_nt(f16_t* p, int i, float v) { st(p, i, v); }

@@ -43,8 +43,13 @@ struct FusedArgs {
     int nty, ntz, nseg;   // tile counts
 };
 
+// Minimum waves per SIMD the register allocation must allow (occupancy).
+#ifndef AFDTD_FUSED_WPE
+#define AFDTD_FUSED_WPE 1
+#endif
+
 template <int ORDER, typename T, bool HET, int TY, int NW>
-__global__ void __launch_bounds__(64 * NW)
+__global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(AFDTD_FUSED_WPE)))
 k_fused(FusedArgs a) {
     if (*a.status) return;
     int t = xcd_remap(blockIdx.x, gridDim.x);

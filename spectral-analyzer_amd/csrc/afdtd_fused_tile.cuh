@@ -89,7 +89,10 @@ __device__ __forceinline__ void fused_tile(const StencilParams& p, const FusedPt
 
     const int Nx = p.Nx, Ny = p.Ny, Nz = p.Nz;
     const int sx = Ny * Nz;
-    const int lane = threadIdx.x, w = threadIdx.y, tid = lane + 64 * w;
+    // blockDim.x is one wavefront, so threadIdx.y is wave-uniform; reading it
+    // through readfirstlane lets every per-wave condition be a scalar branch.
+    const int lane = threadIdx.x, w = __builtin_amdgcn_readfirstlane(threadIdx.y);
+    const int tid = lane + 64 * w;
     const int j0 = ty * TY, k0 = tz * TZ;
     const int i0 = sg * seg, i1 = min(i0 + seg, Nx);
     const int k = k0 + lane;
@@ -178,48 +181,66 @@ __device__ __forceinline__ void fused_tile(const StencilParams& p, const FusedPt
     const int dJB = pml_depth(jB, Ny, npml), dKB = pml_depth(kB, Nz, npml),
               dKB1 = pml_depth(kB + 1, Nz, npml);
     // own cells on a y/z domain side (impedance walls)
-    bool edgeYZ[R];
+    bool edgeYZ[R], rowEdge[R];                   // rowEdge: the wave's row r has
+                                                  // an edge cell (wave-uniform)
+    const bool tileZEdge = k0 == 0 || k0 + TZ > Nz - 1;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int j = j0 + w * R + r;
         edgeYZ[r] = (j == 0) | (j == Ny - 1) | (k == 0) | (k == Nz - 1);
+        rowEdge[r] = (j == 0) | (j == Ny - 1) | tileZEdge;
     }
+    // per-cell wall sums are read only for rooms with per-cell media
+    const bool loadXi = HET && p.impedance;
     const int ownLds = (PH + w * R) * PC + PH + lane;    // + r*PC
 
     // ── staging registers (filled one plane ahead) ──
-    float stP[R + NHRW + NHC], stB[HET ? R + NHRW + NHC : 1];
-    float oVx[R], oVy[R], oVz[R], oA = 0.0f, oB = 0.0f, oK[HET ? R : 1];
-    float nVx[R], nVy[R], nVz[R], nA = 0.0f, nB = 0.0f, nK[HET ? R : 1];
+    // They hold the loaded values unconverted and unmasked: the conversion
+    // happens where the value is used, one plane later, so the loads stay in
+    // flight across a whole plane of work.  Lanes outside the room load
+    // element 0 instead (always inside the field); every face computed from
+    // such a value is invalid and replaced by 0, and the P and rho values of
+    // cells outside the room are never read by a valid face.
+    typedef typename S::Raw RawT;
+    RawT stP[R + NHRW + NHC];
+    float stB[HET ? R + NHRW + NHC : 1];
+    // Old velocities / K / wall sums of one plane.  Two sets alternate
+    // between even and odd planes (the loop is unrolled by two), so no
+    // register copy at the loop latch forces an early wait on the loads.
+    struct Old {
+        RawT x[R], y[R], z[R], a, b;
+        float k[HET ? R : 1], xi[HET ? R : 1];
+    };
+    Old oE, oO;
     float vq[NQ][R];
 #pragma unroll
     for (int s = 0; s < NQ; ++s)
 #pragma unroll
         for (int r = 0; r < R; ++r) vq[s][r] = 0.0f;
 
-    auto ldv = [&](const T* __restrict__ F, int off) {
-        float v = S::ld(F, off);
-        return S::exact ? v : v * isv;
-    };
+    auto ldr = [&](const T* __restrict__ F, bool v, int off) { return S::ldr(F, v ? off : 0); };
+    auto ldf = [&](const float* __restrict__ F, bool v, int off) { return F[v ? off : 0]; };
+    auto vel = [&](RawT x) { const float f = S::cvt(x); return S::exact ? f : f * isv; };
     auto load_plane = [&](int pl) {
         const bool pv = pl >= 0 && pl < Nx;
         const int base = pl * sx;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const bool v = pv && ownV[r];
-            stP[r] = v ? S::ld(Pin, base + ownOff[r]) : 0.0f;
-            if (HET) stB[r] = v ? fp.bu[base + ownOff[r]] : 0.0f;
+            stP[r] = ldr(Pin, v, base + ownOff[r]);
+            if (HET) stB[r] = ldf(fp.bu, v, base + ownOff[r]);
         }
 #pragma unroll
         for (int m = 0; m < NHRW; ++m) {
             const bool v = pv && hrV[m];
-            stP[R + m] = v ? S::ld(Pin, base + hrOff[m]) : 0.0f;
-            if (HET) stB[R + m] = v ? fp.bu[base + hrOff[m]] : 0.0f;
+            stP[R + m] = ldr(Pin, v, base + hrOff[m]);
+            if (HET) stB[R + m] = ldf(fp.bu, v, base + hrOff[m]);
         }
 #pragma unroll
         for (int m = 0; m < NHC; ++m) {
             const bool v = pv && hcV[m];
-            stP[R + NHRW + m] = v ? S::ld(Pin, base + hcOff[m]) : 0.0f;
-            if (HET) stB[R + NHRW + m] = v ? fp.bu[base + hcOff[m]] : 0.0f;
+            stP[R + NHRW + m] = ldr(Pin, v, base + hcOff[m]);
+            if (HET) stB[R + NHRW + m] = ldf(fp.bu, v, base + hcOff[m]);
         }
     };
     auto store_plane = [&](int slot) {
@@ -227,59 +248,57 @@ __device__ __forceinline__ void fused_tile(const StencilParams& p, const FusedPt
         float* dB = sB + (HET ? slot * PLANE : 0);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            dP[ownLds + r * PC] = stP[r];
+            dP[ownLds + r * PC] = S::cvt(stP[r]);
             if (HET) dB[ownLds + r * PC] = stB[r];
         }
 #pragma unroll
         for (int m = 0; m < NHRW; ++m)
             if (hrLds[m] >= 0) {
-                dP[hrLds[m]] = stP[R + m];
+                dP[hrLds[m]] = S::cvt(stP[R + m]);
                 if (HET) dB[hrLds[m]] = stB[R + m];
             }
 #pragma unroll
         for (int m = 0; m < NHC; ++m)
             if (hcLds[m] >= 0) {
-                dP[hcLds[m]] = stP[R + NHRW + m];
+                dP[hcLds[m]] = S::cvt(stP[R + NHRW + m]);
                 if (HET) dB[hcLds[m]] = stB[R + NHRW + m];
             }
     };
-    // Old values of plane q (Vy, Vz, kap) and of Vx face f.
-    auto load_old = [&](int q, int f, float* ox, float* oy, float* oz, float& oa, float& ob,
-                        float* ok) {
+    // Old values of plane q (Vy, Vz, K, wall sums) and of Vx face f.
+    auto load_old = [&](int q, int f, Old& o) {
         const bool fv = f >= 0 && f <= Nx - 2;
         const bool qv = q >= i0 && q < i1;
         const int bf = f * sx, bq = q * sx;
+        const bool xe = (q == 0) | (q == Nx - 1);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            ox[r] = (fv && ownV[r]) ? ldv(Vxi, bf + ownOff[r]) : 0.0f;
-            oy[r] = (qv && vyV[r]) ? ldv(Vyi, bq + ownOff[r]) : 0.0f;
-            oz[r] = (qv && vzV[r]) ? ldv(Vzi, bq + ownOff[r]) : 0.0f;
-            if (HET) ok[r] = (qv && ownV[r]) ? fp.kap[bq + ownOff[r]] : 0.0f;
+            o.x[r] = ldr(Vxi, fv && ownV[r], bf + ownOff[r]);
+            o.y[r] = ldr(Vyi, qv && vyV[r], bq + ownOff[r]);
+            o.z[r] = ldr(Vzi, qv && vzV[r], bq + ownOff[r]);
+            if (HET) o.k[r] = ldf(fp.kap, qv && ownV[r], bq + ownOff[r]);
+            if (HET && loadXi && qv && (xe || rowEdge[r]))
+                o.xi[r] = ldf(fp.xi, ownV[r] && (xe || edgeYZ[r]), bq + ownOff[r]);
         }
-        oa = (qv && vA) ? ldv(Vyi, bq + offA) : 0.0f;
-        ob = (qv && vB) ? ldv(Vzi, bq + offB) : 0.0f;
+        o.a = ldr(Vyi, qv && vA, bq + offA);
+        o.b = ldr(Vzi, qv && vB, bq + offB);
     };
     auto slot_of = [](int pl) { return ((pl % NS) + NS) % NS; };
+    auto slot_add = [](int s0, int d) { const int t = s0 + d; return t >= NS ? t - NS : t; };
     auto damp = [&](int dL, int dR) { return face_damp(sAlpha, dL, dR, dt); };
 
-    const int q0 = i0 - FLO - FHI;
-    for (int pl = q0; pl < q0 + LA; ++pl) {
-        load_plane(pl);
-        store_plane(slot_of(pl));
-    }
-    load_plane(q0 + LA);
-    load_old(q0, q0 + FHI, oVx, oVy, oVz, oA, oB, oK);
-    __syncthreads();
-
-    for (int q = q0; q < i1; ++q) {
+    // One plane q (ring slot sq0): consumes the old values in `o`, prefetches
+    // plane q+1's into `n`.
+    auto plane = [&](int q, int sq0, Old& o, Old& n) __attribute__((always_inline)) {
         // The slot of plane q+LA was last read at plane q-2 (ring depth LA+2)
-        // and the face tiles at plane q-1 before the previous iteration's
-        // last barrier, so no barrier is needed before refilling them.
-        store_plane(slot_of(q + LA));
-        if (q + 1 < i1) {
-            load_plane(q + 1 + LA);
-            load_old(q + 1, q + 1 + FHI, nVx, nVy, nVz, nA, nB, nK);
-        }
+        // and the face tiles at plane q-1 before the previous plane's last
+        // barrier, so no barrier is needed before refilling them.
+        store_plane(slot_add(sq0, LA));
+        // Issued on every plane, also the last one (whose prefetch is never
+        // used: plane q+1 >= i1 is invalid, so its loads read element 0 or a
+        // face inside the room).  A conditional prefetch would make the
+        // compiler wait for it at once, on the path that skips it.
+        load_plane(q + 1 + LA);
+        load_old(q + 1, q + 1 + FHI, n);
         __syncthreads();                                   // ring plane visible
 
         // ── Vx face f = q + FHI of own cells ──
@@ -288,28 +307,32 @@ __device__ __forceinline__ void fused_tile(const StencilParams& p, const FusedPt
             const bool fv = f >= 0 && f <= Nx - 2;
             const bool inner = f > 0 && f < Nx - 2;
             const bool own = f >= i0 && f < i1;
-            const float* s0 = sP + slot_of(f) * PLANE;
-            const float* s1 = sP + slot_of(f + 1) * PLANE;
-            const float* sm = sP + slot_of(f - 1) * PLANE;
-            const float* s2 = sP + slot_of(f + 2) * PLANE;
+            // planes f-1 .. f+2 are q+FHI-1 .. q+FHI+2, all >= q-1
+            const int sf = slot_add(sq0, FHI);
+            const int sf1 = slot_add(sf, 1);
+            const int sfm = ORDER == 4 ? sq0 : sf;       // plane f-1 = q when FHI = 1
+            const int sf2 = ORDER == 4 ? slot_add(sf, 2) : sf;
+            const float* s0 = sP + sf * PLANE;
+            const float* s1 = sP + sf1 * PLANE;
+            const float* sm = sP + sfm * PLANE;
+            const float* s2 = sP + sf2 * PLANE;
             int di0 = 0, di1 = 0;
             if (npml > 0) { di0 = pml_depth(f, Nx, npml); di1 = pml_depth(f + 1, Nx, npml); }
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                float nv = 0.0f;
-                if (fv && ownV[r]) {
-                    const int a = ownLds + r * PC;
-                    float coef = vcoef;
-                    if (HET) {
-                        const float* b0 = sB + slot_of(f) * PLANE;
-                        const float* b1 = sB + slot_of(f + 1) * PLANE;
-                        coef = (0.5f * (b0[a] + b1[a])) * dtdx;
-                    }
-                    nv = face_new<ORDER>(oVx[r], ORDER == 4 ? sm[a] : 0.0f, s0[a], s1[a],
-                                         ORDER == 4 ? s2[a] : 0.0f, inner, coef, C1, C2);
-                    if (npml > 0) nv = nv * damp(imax3(di0, dJ[r], dK), imax3(di1, dJ[r], dK));
-                    if (own) S::st_nt(Vxo, f * sx + ownOff[r], S::exact ? nv : nv * sv);
+                const bool val = fv && ownV[r];
+                const int a = ownLds + r * PC;
+                float coef = vcoef;
+                if (HET) {
+                    const float* b0 = sB + sf * PLANE;
+                    const float* b1 = sB + sf1 * PLANE;
+                    coef = (0.5f * (b0[a] + b1[a])) * dtdx;
                 }
+                float nv = face_new<ORDER>(vel(o.x[r]), ORDER == 4 ? sm[a] : 0.0f, s0[a], s1[a],
+                                           ORDER == 4 ? s2[a] : 0.0f, inner, coef, C1, C2);
+                if (npml > 0) nv = nv * damp(imax3(di0, dJ[r], dK), imax3(di1, dJ[r], dK));
+                nv = val ? nv : 0.0f;
+                if (own && val) S::st_nt(Vxo, f * sx + ownOff[r], S::exact ? nv : nv * sv);
 #pragma unroll
                 for (int s = 0; s < NQ - 1; ++s) vq[s][r] = vq[s + 1][r];
                 vq[NQ - 1][r] = nv;
@@ -317,69 +340,60 @@ __device__ __forceinline__ void fused_tile(const StencilParams& p, const FusedPt
         }
 
         if (q >= i0) {
-            const float* sq = sP + slot_of(q) * PLANE;
-            const float* bq = sB + (HET ? slot_of(q) * PLANE : 0);
+            const float* sq = sP + sq0 * PLANE;
+            const float* bq = sB + (HET ? sq0 * PLANE : 0);
             const int base = q * sx;
             const int dI = pml_depth(q, Nx, npml);
             // ── Vy faces of own rows (face j between rows j and j+1) ──
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                float nv = 0.0f;
-                if (vyV[r]) {
-                    const int a = ownLds + r * PC;
-                    const float coef = HET ? (0.5f * (bq[a] + bq[a + PC])) * dtdx : vcoef;
-                    nv = face_new<ORDER>(oVy[r], ORDER == 4 ? sq[a - PC] : 0.0f, sq[a],
-                                         sq[a + PC], ORDER == 4 ? sq[a + 2 * PC] : 0.0f,
-                                         vyIn[r], coef, C1, C2);
-                    if (npml > 0) nv = nv * damp(imax3(dI, dJ[r], dK), imax3(dI, dJ1[r], dK));
-                    S::st_nt(Vyo, base + ownOff[r], S::exact ? nv : nv * sv);
-                }
+                const int a = ownLds + r * PC;
+                const float coef = HET ? (0.5f * (bq[a] + bq[a + PC])) * dtdx : vcoef;
+                float nv = face_new<ORDER>(vel(o.y[r]), ORDER == 4 ? sq[a - PC] : 0.0f, sq[a],
+                                           sq[a + PC], ORDER == 4 ? sq[a + 2 * PC] : 0.0f,
+                                           vyIn[r], coef, C1, C2);
+                if (npml > 0) nv = nv * damp(imax3(dI, dJ[r], dK), imax3(dI, dJ1[r], dK));
+                nv = vyV[r] ? nv : 0.0f;
+                if (vyV[r]) S::st_nt(Vyo, base + ownOff[r], S::exact ? nv : nv * sv);
                 sVy[(FLO + w * R + r) * TZ + lane] = nv;
             }
             // ── halo Vy face row (task A) ──
             if (tA) {
-                float nv = 0.0f;
-                if (vA) {
-                    const int a = (frA - FLO + PH) * PC + PH + lane;
-                    const float coef = HET ? (0.5f * (bq[a] + bq[a + PC])) * dtdx : vcoef;
-                    nv = face_new<ORDER>(oA, ORDER == 4 ? sq[a - PC] : 0.0f, sq[a], sq[a + PC],
-                                         ORDER == 4 ? sq[a + 2 * PC] : 0.0f, inA, coef, C1, C2);
-                    if (npml > 0) nv = nv * damp(imax3(dI, dJA, dK), imax3(dI, dJA1, dK));
-                }
-                sVy[frA * TZ + lane] = nv;
+                const int a = (frA - FLO + PH) * PC + PH + lane;
+                const float coef = HET ? (0.5f * (bq[a] + bq[a + PC])) * dtdx : vcoef;
+                float nv = face_new<ORDER>(vel(o.a), ORDER == 4 ? sq[a - PC] : 0.0f, sq[a],
+                                           sq[a + PC], ORDER == 4 ? sq[a + 2 * PC] : 0.0f, inA,
+                                           coef, C1, C2);
+                if (npml > 0) nv = nv * damp(imax3(dI, dJA, dK), imax3(dI, dJA1, dK));
+                sVy[frA * TZ + lane] = vA ? nv : 0.0f;
             }
             // ── Vz faces of own rows (face k between cols k and k+1) ──
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                float nv = 0.0f;
-                if (vzV[r]) {
-                    const int a = ownLds + r * PC;
-                    const float coef = HET ? (0.5f * (bq[a] + bq[a + 1])) * dtdx : vcoef;
-                    nv = face_new<ORDER>(oVz[r], ORDER == 4 ? sq[a - 1] : 0.0f, sq[a], sq[a + 1],
-                                         ORDER == 4 ? sq[a + 2] : 0.0f, vzIn, coef, C1, C2);
-                    if (npml > 0) nv = nv * damp(imax3(dI, dJ[r], dK), imax3(dI, dJ[r], dK1));
-                    S::st_nt(Vzo, base + ownOff[r], S::exact ? nv : nv * sv);
-                }
+                const int a = ownLds + r * PC;
+                const float coef = HET ? (0.5f * (bq[a] + bq[a + 1])) * dtdx : vcoef;
+                float nv = face_new<ORDER>(vel(o.z[r]), ORDER == 4 ? sq[a - 1] : 0.0f, sq[a],
+                                           sq[a + 1], ORDER == 4 ? sq[a + 2] : 0.0f, vzIn, coef,
+                                           C1, C2);
+                if (npml > 0) nv = nv * damp(imax3(dI, dJ[r], dK), imax3(dI, dJ[r], dK1));
+                nv = vzV[r] ? nv : 0.0f;
+                if (vzV[r]) S::st_nt(Vzo, base + ownOff[r], S::exact ? nv : nv * sv);
                 sVz[(w * R + r) * FC + FLO + lane] = nv;
             }
             // ── halo Vz face (task B) ──
             if (tB) {
-                float nv = 0.0f;
-                if (vB) {
-                    const int a = (PH + lyB) * PC + fcB - FLO + PH;
-                    const float coef = HET ? (0.5f * (bq[a] + bq[a + 1])) * dtdx : vcoef;
-                    nv = face_new<ORDER>(oB, ORDER == 4 ? sq[a - 1] : 0.0f, sq[a], sq[a + 1],
-                                         ORDER == 4 ? sq[a + 2] : 0.0f, inB, coef, C1, C2);
-                    if (npml > 0) nv = nv * damp(imax3(dI, dJB, dKB), imax3(dI, dJB, dKB1));
-                }
-                sVz[lyB * FC + fcB] = nv;
+                const int a = (PH + lyB) * PC + fcB - FLO + PH;
+                const float coef = HET ? (0.5f * (bq[a] + bq[a + 1])) * dtdx : vcoef;
+                float nv = face_new<ORDER>(vel(o.b), ORDER == 4 ? sq[a - 1] : 0.0f, sq[a], sq[a + 1],
+                                           ORDER == 4 ? sq[a + 2] : 0.0f, inB, coef, C1, C2);
+                if (npml > 0) nv = nv * damp(imax3(dI, dJB, dKB), imax3(dI, dJB, dKB1));
+                sVz[lyB * FC + fcB] = vB ? nv : 0.0f;
             }
             __syncthreads();                               // faces visible
             // ── pressure of own cells at plane q ──
             const bool xedge = (q == 0) | (q == Nx - 1);
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                if (!ownV[r]) continue;
                 const int fy = (FLO + w * R + r) * TZ + lane;
                 const int fz = (w * R + r) * FC + FLO + lane;
                 const float xp = vq[FLO][r], xm = vq[FLO - 1][r];
@@ -396,12 +410,12 @@ __device__ __forceinline__ void fused_tile(const StencilParams& p, const FusedPt
                        + C2 * ((xpp - xmm) + (ypp - ymm) + (zpp - zmm));
                 }
                 const float pc = sq[ownLds + r * PC];
-                const float kap = HET ? oK[r] : p.bulk_coeff;
+                const float kap = HET ? o.k[r] : p.bulk_coeff;
                 float out;
-                if (p.impedance && (xedge || edgeYZ[r])) {
+                if (p.impedance && ownV[r] && (xedge || edgeYZ[r])) {
                     float xi;
                     if (HET) {
-                        xi = fp.xi[base + ownOff[r]];
+                        xi = o.xi[r];
                     } else {
                         const int j = j0 + w * R + r;
                         xi = wall_xi(p.xterm, q, j, k, Nx, Ny, Nz);
@@ -411,17 +425,27 @@ __device__ __forceinline__ void fused_tile(const StencilParams& p, const FusedPt
                     out = pc - kap * dv;
                 }
                 if (npml > 0) out = out * sPd[imax3(dI, dJ[r], dK)];
-                S::st_nt(Pout, base + ownOff[r], out);
+                if (ownV[r]) S::st_nt(Pout, base + ownOff[r], out);
             }
         }
-        // rotate staged old values
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            oVx[r] = nVx[r]; oVy[r] = nVy[r]; oVz[r] = nVz[r];
-            if (HET) oK[r] = nK[r];
-        }
-        oA = nA;
-        oB = nB;
+    };
+
+    const int q0 = i0 - FLO - FHI;
+    for (int pl = q0; pl < q0 + LA; ++pl) {
+        load_plane(pl);
+        store_plane(slot_of(pl));
+    }
+    load_plane(q0 + LA);
+    load_old(q0, q0 + FHI, oE);
+    __syncthreads();
+
+    int sq0 = slot_of(q0);                                 // slot of plane q
+    for (int q = q0; q < i1; q += 2) {
+        plane(q, sq0, oE, oO);
+        sq0 = slot_add(sq0, 1);
+        if (q + 1 >= i1) break;
+        plane(q + 1, sq0, oO, oE);
+        sq0 = slot_add(sq0, 1);
     }
 }
 
