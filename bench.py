@@ -67,6 +67,25 @@ def copy_calibration(nbytes=2 << 30, reps=20):
     return out
 
 
+def stream_ceiling(w, storage, reps=10):
+    """Measured streaming ceiling of the sweep's byte mix: the library's
+    no-stencil kernel reading and writing FP32 arrays with the same bytes per
+    cell as the algorithmic count (f32: 4 or 6 arrays read, 4 written; 16-bit
+    fields: half as many field arrays, rho and K unchanged)."""
+    import ctypes as C
+    from afdtd import lib
+    if storage == "f32":
+        n_read, n_write = (6, 4) if w.hetero else (4, 4)
+    else:
+        n_read, n_write = (4, 2) if w.hetero else (2, 2)
+    n = min(w.cells, 64 << 20)
+    ms = C.c_float()
+    rc = lib().fdtd_stream_calibration(n, n_read, n_write, reps, C.byref(ms))
+    if rc:
+        raise SystemExit(f"fdtd_stream_calibration failed: {rc}")
+    return (n_read + n_write) * 4 * n / (ms.value * 1e-3) / 1e9
+
+
 def cpu_baseline(w, budget_s=15.0):
     """Time the oracle (float32, OpenMP over x slabs) on the same grid for a
     bounded number of steps."""
@@ -294,6 +313,9 @@ def main():
                      "stencil_ms_per_launch": ms_stencil, "event_ms_total": ms_events},
     }
     del s
+    ceiling = stream_ceiling(w, a.storage)
+    line["roofline"]["stream_ceiling_GBps"] = ceiling
+    line["roofline"]["frac_of_stream_ceiling"] = achieved / ceiling
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(w)
     print(json.dumps(line), flush=True)

@@ -188,6 +188,7 @@ SK_API int fdtd_get_surface_emission(const AcousticFDTDState* st, int n_surf,
 #define FDTD_KERNEL_AUTO    0
 #define FDTD_KERNEL_GENERIC 1  /* separate velocity / pressure sweeps, all features */
 #define FDTD_KERNEL_FUSED   2  /* fused 2.5-D plane-marching sweep                   */
+#define FDTD_KERNEL_FUSED2  3  /* two leapfrog steps per sweep (order 2)            */
 
 typedef struct FdtdOptions {
     int          struct_size;    /* sizeof(FdtdOptions)                          */
@@ -255,6 +256,14 @@ SK_API int fdtd_step_timed(AcousticFDTDState* st, int n_steps,
  * device-to-device copy of nbytes over `reps` launches; *ms_per_copy is the
  * average launch time.  Reported as the measured copy ceiling. */
 SK_API int fdtd_copy_calibration(size_t nbytes, int reps, float* ms_per_copy);
+
+/* Streaming ceiling of a stencil's byte mix (no reference counterpart): a
+ * kernel that reads n_read (1..8) and writes n_write (1..8) FP32 arrays of
+ * n_cells elements once per pass, with no stencil.  *ms_per_pass is the
+ * HIP-event time per pass over `reps` passes.  Device buffers are allocated
+ * and freed inside the call. */
+SK_API int fdtd_stream_calibration(size_t n_cells, int n_read, int n_write, int reps,
+                                   float* ms_per_pass);
 
 /* ── Batched rooms (no reference counterpart) ─────────────────────────────
  * n_rooms independent rooms of uneven size advanced by ONE fused launch per

@@ -17,7 +17,7 @@ FDTD_OK, FDTD_ERR_NULL, FDTD_ERR_DIM, FDTD_ERR_UNSTABLE = 0, -1, -2, -3
 FDTD_ERR_DEVICE, FDTD_ERR_ARG = -4, -5
 FDTD_BOUNDARY_RIGID_PML, FDTD_BOUNDARY_IMPEDANCE = 0, 1
 FDTD_STORE_F32, FDTD_STORE_BF16, FDTD_STORE_F16 = 0, 1, 2
-FDTD_KERNEL_AUTO, FDTD_KERNEL_GENERIC, FDTD_KERNEL_FUSED = 0, 1, 2
+FDTD_KERNEL_AUTO, FDTD_KERNEL_GENERIC, FDTD_KERNEL_FUSED, FDTD_KERNEL_FUSED2 = 0, 1, 2, 3
 
 # The 26 functions of the reference ABI (csrc/include/acoustic_fdtd.h:137-506).
 REFERENCE_SYMBOLS = (
@@ -95,6 +95,7 @@ _SIGS = {
     "fdtd_batch_get_pressure": (_i, [_vp, _i, _vp, _i]),
     "fdtd_batch_get_step_count": (_i, [_vp]),
     "fdtd_copy_calibration": (_i, [C.c_size_t, _i, _vp]),
+    "fdtd_stream_calibration": (_i, [C.c_size_t, _i, _i, _i, _vp]),
     "fdtd_adjoint_create": (_vp, [_i, _i, _i, C.c_double, _i, _i]),
     "fdtd_adjoint_destroy": (None, [_vp]),
     "fdtd_adjoint_set_sources": (_i, [_vp, _i, _vp, _vp, _i]),

@@ -10,7 +10,8 @@ Extension keywords of the constructor (no reference counterpart):
 ``order`` (4 reference, 2 Yee), ``courant`` (None = reference dt rule),
 ``boundary`` ('rigid' = rigid edges + PML, 'impedance'), ``refl`` (six
 reflection coefficients), ``rho`` / ``bulk`` (per-cell media), ``storage``
-('f32', 'bf16', 'f16') and ``kernel`` ('auto', 'generic', 'fused').
+('f32', 'bf16', 'f16') and ``kernel`` ('auto', 'generic', 'fused', 'fused2': two
+steps per sweep, order-2 stencil).
 """
 from __future__ import annotations
 
@@ -24,7 +25,7 @@ from ._lib import lib
 
 _STORAGE = {"f32": _lib.FDTD_STORE_F32, "bf16": _lib.FDTD_STORE_BF16, "f16": _lib.FDTD_STORE_F16}
 _KERNEL = {"auto": _lib.FDTD_KERNEL_AUTO, "generic": _lib.FDTD_KERNEL_GENERIC,
-           "fused": _lib.FDTD_KERNEL_FUSED}
+           "fused": _lib.FDTD_KERNEL_FUSED, "fused2": _lib.FDTD_KERNEL_FUSED2}
 _BOUNDARY = {"rigid": _lib.FDTD_BOUNDARY_RIGID_PML, "pml": _lib.FDTD_BOUNDARY_RIGID_PML,
              "impedance": _lib.FDTD_BOUNDARY_IMPEDANCE}
 
@@ -259,7 +260,8 @@ class AcousticFDTD:
     # ── extensions ────────────────────────────────────────────────────────
     @property
     def kernel_family(self) -> str:
-        return {1: "generic", 2: "fused"}.get(self._L.fdtd_get_kernel_family(self._h), "?")
+        return {1: "generic", 2: "fused", 3: "fused2"}.get(
+            self._L.fdtd_get_kernel_family(self._h), "?")
 
     def set_point_sources(self, cell_idx, signals):
         idx = np.ascontiguousarray(np.asarray(cell_idx, np.int32).ravel())

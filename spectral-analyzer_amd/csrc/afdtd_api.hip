@@ -246,6 +246,8 @@ void destroy_state(AcousticFDTDState* st) {
     dfree(st->d_pidx); dfree(st->d_psig);
     dfree(st->d_ridx); dfree(st->d_rwgt); dfree(st->d_trace);
     dfree(st->d_gidx); dfree(st->d_gwgt); dfree(st->d_gout);
+    dfree(st->d_f2_src_start); dfree(st->d_f2_src); dfree(st->d_f2_tap_start);
+    dfree(st->d_f2_tap); dfree(st->d_f2_tapbuf);
     if (st->ev0) (void)hipEventDestroy(st->ev0);
     if (st->ev1) (void)hipEventDestroy(st->ev1);
     if (st->stream) (void)hipStreamDestroy(st->stream);
@@ -522,7 +524,16 @@ AcousticFDTDState* create_impl(int Nx, int Ny, int Nz, float dx, float c, float 
     st->fused_tile_y = o->tile_y;
     st->fused_seg_x = o->seg_x;
     st->family = FDTD_KERNEL_GENERIC;
-    if (o->kernel == FDTD_KERNEL_FUSED || o->kernel == FDTD_KERNEL_AUTO) {
+    st->f2_dirty = 1;
+    if (o->kernel == FDTD_KERNEL_FUSED2) {
+        if (!fused2_supported(st)) {
+            afdtd_set_error("two-step sweep needs the order-2 stencil and no walls, plate "
+                            "cells, face fractions or active plate");
+            destroy_state(st);
+            return nullptr;
+        }
+        st->family = FDTD_KERNEL_FUSED2;
+    } else if (o->kernel == FDTD_KERNEL_FUSED || o->kernel == FDTD_KERNEL_AUTO) {
         if (fused_supported(st)) st->family = FDTD_KERNEL_FUSED;
         else if (o->kernel == FDTD_KERNEL_FUSED) {
             afdtd_set_error("fused kernel does not support this configuration");
@@ -530,7 +541,7 @@ AcousticFDTDState* create_impl(int Nx, int Ny, int Nz, float dx, float c, float 
             return nullptr;
         }
     }
-    if (st->family == FDTD_KERNEL_FUSED) {
+    if (st->family == FDTD_KERNEL_FUSED || st->family == FDTD_KERNEL_FUSED2) {
         for (int a = 0; a < 3; ++a) {
             TRY(hipMalloc(&st->V2[a], (size_t)N * st->esz));
             TRY(hipMemsetAsync(st->V2[a], 0, (size_t)N * st->esz, st->stream));
@@ -542,9 +553,11 @@ AcousticFDTDState* create_impl(int Nx, int Ny, int Nz, float dx, float c, float 
 
 #undef TRY
 
+// par[q] = parity of the number of field-buffer swaps (kernel launches) after
+// the first q+1 steps of this call.
 int apply_failure(AcousticFDTDState* st, int start_step, int n_steps, int start_cur,
                   float* const w0[3], void* const v0[3], void* const v20[3], int start_trace,
-                  int start_phase) {
+                  int start_phase, const std::vector<uint8_t>& par) {
     int status[2] = {0, 0};
     AFDTD_CHECK(hipMemcpy(status, st->d_status, sizeof(status), hipMemcpyDeviceToHost));
     if (status[0] == 0) {
@@ -554,18 +567,19 @@ int apply_failure(AcousticFDTDState* st, int start_step, int n_steps, int start_
     const int sf = status[1];
     const int completed = status[0] == 3 ? sf - start_step + 1 : sf - start_step;
     st->step_count = start_step + completed;
-    st->cur = start_cur ^ (completed & 1);
+    const int swaps = completed > 0 ? par[completed - 1] : 0;
+    st->cur = start_cur ^ swaps;
     // Plate buffers rotate once per completed step (curr, prev, tmp) <- (tmp, curr, prev).
     float* w[3] = {w0[0], w0[1], w0[2]};
     for (int r = 0; r < completed % 3; ++r) {
         float* t = w[2]; w[2] = w[1]; w[1] = w[0]; w[0] = t;
     }
     for (int a = 0; a < 3; ++a) st->d_w[a] = w[a];
-    // The fused sweep swaps the velocity buffers once per completed step.
-    if (st->family == FDTD_KERNEL_FUSED) {
+    // The fused sweeps swap the velocity buffers once per launch.
+    if (st->family == FDTD_KERNEL_FUSED || st->family == FDTD_KERNEL_FUSED2) {
         for (int a = 0; a < 3; ++a) {
-            st->V[a] = (completed & 1) ? v20[a] : v0[a];
-            st->V2[a] = (completed & 1) ? v0[a] : v20[a];
+            st->V[a] = swaps ? v20[a] : v0[a];
+            st->V2[a] = swaps ? v0[a] : v20[a];
         }
     }
     if (st->trace_cap > 0) st->trace_n = std::min(st->trace_cap, start_trace + completed);
@@ -593,7 +607,7 @@ int enqueue_step(AcousticFDTDState* st, int s, hipEvent_t ea, hipEvent_t eb) {
         float* t = st->d_w[2]; st->d_w[2] = st->d_w[1]; st->d_w[1] = st->d_w[0]; st->d_w[0] = t;
     }
     if (ea) AFDTD_CHECK(hipEventRecord(ea, st->stream));
-    if (st->family == FDTD_KERNEL_FUSED) {
+    if (st->family == FDTD_KERNEL_FUSED || st->family == FDTD_KERNEL_FUSED2) {
         AFDTD_CHECK(launch_fused_step(st));
     } else {
         AFDTD_CHECK(launch_velocity_generic(st));
@@ -612,6 +626,25 @@ int enqueue_step(AcousticFDTDState* st, int s, hipEvent_t ea, hipEvent_t eb) {
     return FDTD_OK;
 }
 
+// Enqueue steps s and s+1 as one two-step sweep (step s+1 is not a
+// stability-check step: the caller schedules pairs so).
+int enqueue_pair(AcousticFDTDState* st, int s, hipEvent_t ea, hipEvent_t eb) {
+    if (ea) AFDTD_CHECK(hipEventRecord(ea, st->stream));
+    AFDTD_CHECK(launch_fused2_pair(st, s));
+    if (eb) AFDTD_CHECK(hipEventRecord(eb, st->stream));
+    st->cur ^= 1;
+    int row0 = -1, row1 = -1;
+    if (st->d_trace && st->trace_n < st->trace_cap) row0 = st->trace_n++;
+    if (st->d_trace && st->trace_n < st->trace_cap) row1 = st->trace_n++;
+    AFDTD_CHECK(launch_fused2_taps(st, row0));
+    AFDTD_CHECK(launch_post_step(st, s + 1, row1));
+    if (((s + 2) & 0xFF) == 0) {
+        AFDTD_CHECK(launch_stability_check(st, st->phase, s + 1));
+        st->phase = (st->phase + 1) % 1013;
+    }
+    return FDTD_OK;
+}
+
 int step_impl(AcousticFDTDState* st, int n_steps, float* ms_total, float* ms_stencil) {
     const int start_step = st->step_count, start_cur = st->cur;
     const int start_trace = st->trace_n, start_phase = st->phase;
@@ -624,17 +657,32 @@ int step_impl(AcousticFDTDState* st, int n_steps, float* ms_total, float* ms_ste
         for (auto& e : evs) AFDTD_CHECK(hipEventCreate(&e));
     }
     if (ms_total) AFDTD_CHECK(hipEventRecord(st->ev0, st->stream));
-    for (int q = 0; q < n_steps; ++q) {
-        int rc = enqueue_step(st, start_step + q, ms_stencil ? evs[2 * q] : nullptr,
-                              ms_stencil ? evs[2 * q + 1] : nullptr);
+    std::vector<uint8_t> par((size_t)n_steps, 0);
+    int launches = 0;
+    if (st->family == FDTD_KERNEL_FUSED2) {
+        int rc = fused2_prepare(st);
         if (rc) return rc;
+    }
+    for (int q = 0; q < n_steps;) {
+        const int s = start_step + q;
+        hipEvent_t ea = ms_stencil ? evs[2 * launches] : nullptr;
+        hipEvent_t eb = ms_stencil ? evs[2 * launches + 1] : nullptr;
+        const bool pair = st->family == FDTD_KERNEL_FUSED2 && st->f2_ok && q + 1 < n_steps &&
+                          ((s + 1) & 0xFF) != 0 && st->nact == 0;
+        int rc = pair ? enqueue_pair(st, s, ea, eb) : enqueue_step(st, s, ea, eb);
+        if (rc) return rc;
+        ++launches;
+        const int adv = pair ? 2 : 1;
+        for (int u = 0; u < adv; ++u) par[q + u] = (uint8_t)(launches & 1);
+        q += adv;
     }
     if (ms_total) AFDTD_CHECK(hipEventRecord(st->ev1, st->stream));
     AFDTD_CHECK(hipStreamSynchronize(st->stream));
     if (ms_total) AFDTD_CHECK(hipEventElapsedTime(ms_total, st->ev0, st->ev1));
     if (ms_stencil) {
+        // average stencil time per step (a two-step launch counts for two)
         double sum = 0.0;
-        for (int q = 0; q < n_steps; ++q) {
+        for (int q = 0; q < launches; ++q) {
             float t = 0.0f;
             AFDTD_CHECK(hipEventElapsedTime(&t, evs[2 * q], evs[2 * q + 1]));
             sum += t;
@@ -643,7 +691,7 @@ int step_impl(AcousticFDTDState* st, int n_steps, float* ms_total, float* ms_ste
         for (auto& e : evs) (void)hipEventDestroy(e);
     }
     return apply_failure(st, start_step, n_steps, start_cur, w0, v0, v20, start_trace,
-                         start_phase);
+                         start_phase, par);
 }
 
 }  // namespace
@@ -758,7 +806,8 @@ SK_API int fdtd_set_face_fractions(AcousticFDTDState* st, const float* fx, const
     if (fx || fy || fz) {
         st->has_frac = 1;
         st->sp.has_frac = 1;
-        if (st->family == FDTD_KERNEL_FUSED && !fused_supported(st))
+        if ((st->family == FDTD_KERNEL_FUSED || st->family == FDTD_KERNEL_FUSED2) &&
+            !fused_supported(st))
             st->family = FDTD_KERNEL_GENERIC;
     }
     return FDTD_OK;
@@ -985,6 +1034,8 @@ SK_API int fdtd_set_point_sources(AcousticFDTDState* st, int n, const int* cell_
     if (!st) return FDTD_ERR_NULL;
     dfree(st->d_pidx); dfree(st->d_psig);
     st->d_pidx = nullptr; st->d_psig = nullptr; st->npsrc = 0; st->plen = 0;
+    st->h_pidx.clear();
+    st->f2_dirty = 1;
     if (n <= 0) return FDTD_OK;
     if (!cell_idx || !signals || n_samples <= 0) return FDTD_ERR_NULL;
     for (int s = 0; s < n; ++s) {
@@ -1000,6 +1051,7 @@ SK_API int fdtd_set_point_sources(AcousticFDTDState* st, int n, const int* cell_
                           hipMemcpyHostToDevice));
     st->npsrc = n;
     st->plen = n_samples;
+    st->h_pidx.assign(cell_idx, cell_idx + n);
     return FDTD_OK;
 }
 
@@ -1008,6 +1060,8 @@ SK_API int fdtd_set_receivers(AcousticFDTDState* st, int n, const float* xyz) {
     dfree(st->d_ridx); dfree(st->d_rwgt); dfree(st->d_trace);
     st->d_ridx = nullptr; st->d_rwgt = nullptr; st->d_trace = nullptr;
     st->nrec = 0; st->trace_cap = 0; st->trace_n = 0;
+    st->h_ridx.clear();
+    st->f2_dirty = 1;
     if (n <= 0) return FDTD_OK;
     if (!xyz) return FDTD_ERR_NULL;
     std::vector<int> idx((size_t)8 * n);
@@ -1018,6 +1072,7 @@ SK_API int fdtd_set_receivers(AcousticFDTDState* st, int n, const float* xyz) {
     AFDTD_CHECK(hipMemcpy(st->d_ridx, idx.data(), sizeof(int) * 8 * n, hipMemcpyHostToDevice));
     AFDTD_CHECK(hipMemcpy(st->d_rwgt, wgt.data(), sizeof(float) * 8 * n, hipMemcpyHostToDevice));
     st->nrec = n;
+    st->h_ridx = idx;
     return FDTD_OK;
 }
 

@@ -3,6 +3,8 @@
 // stability check, API gathers, storage conversions and the FP64 energy audit.
 #include "afdtd_common.cuh"
 
+#include <vector>
+
 namespace {
 
 // Kirchhoff plate leapfrog over the active nodes (acoustic_fdtd.cpp:984-1053).
@@ -347,7 +349,79 @@ __global__ void __launch_bounds__(256) k_copy16(const float4* __restrict__ a, fl
          i += (size_t)gridDim.x * blockDim.x)
         b[i] = a[i];
 }
+struct StreamArrays {
+    const float* in[8];
+    float* out[8];
+};
+
+// n_read FP32 arrays in, n_write out, one element per lane per pass: the
+// byte mix of a stencil step without the stencil.
+__global__ void __launch_bounds__(256) k_stream_mix(StreamArrays a, int n_read, int n_write,
+                                                    size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        float v[8];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) v[c] = c < n_read ? a.in[c][i] : 0.0f;
+        float s = 0.0f;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) s += v[c];
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+            if (c < n_write) __builtin_nontemporal_store(s + (float)c, a.out[c] + i);
+    }
+}
 }  // namespace
+
+// Streaming ceiling of a stencil's byte mix (no reference counterpart): n_read
+// arrays read and n_write arrays written once per pass over n_cells FP32
+// elements, timed with HIP events over `reps` passes.
+extern "C" SK_API int fdtd_stream_calibration(size_t n_cells, int n_read, int n_write, int reps,
+                                              float* ms_per_pass) {
+    if (!ms_per_pass) return FDTD_ERR_NULL;
+    if (reps <= 0 || n_cells == 0 || n_read < 1 || n_read > 8 || n_write < 1 || n_write > 8)
+        return FDTD_ERR_ARG;
+    StreamArrays a{};
+    std::vector<void*> bufs;
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = FDTD_OK;
+    auto fail = [&](hipError_t e) {
+        afdtd_set_error(std::string("fdtd_stream_calibration: ") + hipGetErrorString(e));
+        rc = FDTD_ERR_DEVICE;
+    };
+    for (int c = 0; c < n_read + n_write && rc == FDTD_OK; ++c) {
+        void* p = nullptr;
+        hipError_t e = hipMalloc(&p, n_cells * sizeof(float));
+        if (e != hipSuccess) { fail(e); break; }
+        bufs.push_back(p);
+        if (c < n_read) a.in[c] = (const float*)p; else a.out[c - n_read] = (float*)p;
+    }
+    hipError_t e = hipSuccess;
+    if (rc == FDTD_OK && (e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking)) != hipSuccess) fail(e);
+    if (rc == FDTD_OK && (e = hipEventCreate(&e0)) != hipSuccess) fail(e);
+    if (rc == FDTD_OK && (e = hipEventCreate(&e1)) != hipSuccess) fail(e);
+    for (int c = 0; c < n_read && rc == FDTD_OK; ++c)
+        if ((e = hipMemsetAsync((void*)a.in[c], 0, n_cells * sizeof(float), s)) != hipSuccess) fail(e);
+    if (rc == FDTD_OK) {
+        const unsigned blocks = 1024;
+        hipLaunchKernelGGL(k_stream_mix, dim3(blocks), dim3(256), 0, s, a, n_read, n_write, n_cells);
+        (void)hipEventRecord(e0, s);
+        for (int r = 0; r < reps; ++r)
+            hipLaunchKernelGGL(k_stream_mix, dim3(blocks), dim3(256), 0, s, a, n_read, n_write, n_cells);
+        (void)hipEventRecord(e1, s);
+        if ((e = hipGetLastError()) != hipSuccess) fail(e);
+        else if ((e = hipStreamSynchronize(s)) != hipSuccess) fail(e);
+        float ms = 0.0f;
+        if (rc == FDTD_OK && (e = hipEventElapsedTime(&ms, e0, e1)) != hipSuccess) fail(e);
+        if (rc == FDTD_OK) *ms_per_pass = ms / reps;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (s) (void)hipStreamDestroy(s);
+    for (void* p : bufs) (void)hipFree(p);
+    return rc;
+}
 
 // Device-bandwidth probe: a 16-byte-per-lane copy of nbytes, timed with HIP
 // events over `reps` launches (no reference counterpart; used to report the

@@ -19,6 +19,7 @@ template <> struct Store<float> {
     typedef float Raw;
     static __device__ __forceinline__ Raw ldr(const float* p, int i) { return p[i]; }
     static __device__ __forceinline__ float cvt(Raw r) { return r; }
+    static __device__ __forceinline__ Raw raw_of(float v) { return v; }
     static __device__ __forceinline__ float ld(const float* p, int i) { return p[i]; }
     static __device__ __forceinline__ void st(float* p, int i, float v) { p[i] = v; }
     // streaming store: written once per step, not re-read by this sweep
@@ -35,6 +36,10 @@ template <> struct Store<bf16_t> {
     static __device__ __forceinline__ float cvt(Raw r) {
         return __uint_as_float((uint32_t)r << 16);
     }
+    // the bits st() would store (round to nearest even)
+    static __device__ __forceinline__ Raw raw_of(float v) {
+        return __bfloat16_as_ushort(__float2bfloat16(v));
+    }
     static __device__ __forceinline__ float ld(const bf16_t* p, int i) {
         return __bfloat162float(p[i]);
     }
@@ -48,6 +53,7 @@ template <> struct Store<f16_t> {
     typedef f16_t Raw;
     static __device__ __forceinline__ Raw ldr(const f16_t* p, int i) { return p[i]; }
     static __device__ __forceinline__ float cvt(Raw r) { return (float)r; }
+    static __device__ __forceinline__ Raw raw_of(float v) { return (f16_t)v; }
     static __device__ __forceinline__ float ld(const f16_t* p, int i) { return (float)p[i]; }
     static __device__ __forceinline__ void st(f16_t* p, int i, float v) { p[i] = (f16_t)v; }
     static __device__ __forceinline__ void st_nt(f16_t* p, int i, float v) { st(p, i, v); }

@@ -1,0 +1,733 @@
+// afdtd_fused2.hip — two leapfrog steps per sweep (order-2 stencil).
+//
+// The single-step fused sweep (afdtd_fused.hip) reads and writes every field
+// once per step, so its speed is the streaming rate of that byte mix.  This
+// sweep advances two steps, n -> n+2, per pass over HBM: a workgroup owns a
+// 16-row x 64-column tile and marches an x-segment of it, and per x-plane q
+//   stage A (step n -> n+1) updates V^{n+1/2} and P^{n+1} on the tile plus a
+//           one-cell ring around it (the "extended" tile: rows 0..17,
+//           columns -1..64, own cells are rows 1..16, columns 0..63), from
+//           P^n planes staged in an LDS ring (rows -1..18, columns -2..65);
+//   stage B (step n+1 -> n+2) trails stage A by one plane: at plane q-1 it
+//           updates V^{n+3/2} of the faces of the own cells and P^{n+2} of
+//           the own cells from P^{n+1} (LDS and registers) and V^{n+1/2}
+//           (registers of the thread that computed it in stage A), and
+//           writes them to HBM.
+// P^{n+1} and V^{n+1/2} never go to HBM, so one pass costs one read and one
+// write of each field for two cell-updates.  The ring cells of stage A are
+// recomputed by the neighbouring tiles with identical arithmetic.
+//
+// Work mapping (640 threads = 10 wavefronts of 64 lanes):
+//   waves 0..8 ("main"): lane = column 0..63, wave w = extended rows 2w, 2w+1;
+//   wave 9 ("halo"):  lanes 0..15 the cells (e, -1), e = 1..16, lanes 16..31
+//                      the cells (e, 64), lanes 32/33 the z-face between
+//                      columns -1 and 0 on rows 0 and 17, and every lane one
+//                      y-face of the face row between rows -1 and 0.
+//
+// Arithmetic is the single-step sweep's, in the same order
+// (acoustic_fdtd.cpp:1107-1113 gradient/velocity, :894-925 divergence,
+// :1198 pressure); with 16-bit storage the stage-A results are rounded to the
+// storage type exactly where the single-step path stores and reloads them,
+// and the point sources of step n+1 are added after that rounding as the
+// post-step kernel does.  So results equal two single-step sweeps bit for
+// bit.  Receiver taps of step n+1 are written to a small buffer by the
+// thread that owns the tap cell; the stability check never falls on step n+1
+// (the host schedules pairs so that it does not).
+#include "afdtd_fused_tile.cuh"
+
+#include <algorithm>
+
+using fused::FusedPtrs;
+using fused::xcd_remap;
+
+namespace {
+
+namespace f2 {
+constexpr int TZ = 64;           // own columns (one per lane)
+constexpr int TYO = 16;          // own rows
+constexpr int NWM = 9;           // main waves: extended rows 0..17
+constexpr int NW = NWM + 1;      // + the halo wave
+constexpr int NT = 64 * NW;
+constexpr int EY = TYO + 2;      // extended rows
+constexpr int GR = EY + 2;       // P^n region rows -1..18
+constexpr int GC = TZ + 4;       // P^n region columns -2..65
+constexpr int GPL = GR * GC;
+constexpr int NSG = 4;           // P^n ring: planes q-1..q+2
+constexpr int EC = TZ + 2;       // P^{n+1} columns -1..64
+constexpr int EPL = EY * EC;
+constexpr int NHL = 80;          // region halo-column elements per plane (20 rows x 4)
+constexpr int MAXS = 32;         // point-source entries cached per tile
+constexpr int MAXT = 256;        // receiver-tap entries cached per tile
+// entry key: plane << 12 | extended row << 7 | (extended column + 1)
+__host__ __device__ __forceinline__ int ekey(int plane, int e, int c) {
+    return (plane << 12) | (e << 7) | (c + 1);
+}
+// LDS index helpers (row / column in extended-tile coordinates)
+__device__ __forceinline__ int gix(int r, int c) { return (r + 1) * GC + (c + 2); }
+__device__ __forceinline__ int p1ix(int e, int c) { return e * EC + (c + 1); }
+__device__ __forceinline__ int ayix(int fr, int l) { return (fr + 1) * TZ + l; }        // fr -1..17
+__device__ __forceinline__ int azix(int e, int fc) { return e * (TZ + 1) + (fc + 1); }  // fc -1..63
+__device__ __forceinline__ int byix(int fr, int l) { return fr * TZ + l; }              // fr 0..16
+__device__ __forceinline__ int bzix(int e, int fc) { return (e - 1) * (TZ + 1) + (fc + 1); }
+}  // namespace f2
+
+struct Pair2Args {
+    StencilParams p;
+    FusedPtrs f;
+    const float* alpha;
+    const float* pdamp;
+    const int* status;
+    int seg, nty, ntz, nseg;
+    const int* src_start;   // per tile t = ty + nty*tz: entries [src_start[t], src_start[t+1])
+    const int2* src;        // (key, source index), sorted by key then source index
+    const int* tap_start;
+    const int2* tap;        // (key, tap slot), sorted by key
+    const float* psig;
+    int plen;
+    int step;               // global index of the first of the two steps
+    float* tapbuf;          // P^{n+1} at the receiver taps
+};
+
+template <typename T, bool HET>
+__global__ void __launch_bounds__(f2::NT) k_fused2(Pair2Args a) {
+    using namespace f2;
+    typedef Store<T> S;
+    typedef typename S::Raw RawT;
+    if (*a.status) return;
+
+    __shared__ float sP[NSG * GPL];
+    __shared__ float sB[HET ? NSG * GPL : 1];
+    __shared__ float sP1[2 * EPL];
+    __shared__ float sAy[(EY + 1) * TZ];
+    __shared__ float sAz[EY * (TZ + 1)];
+    __shared__ float sBy[(TYO + 1) * TZ];
+    __shared__ float sBz[TYO * (TZ + 1)];
+    __shared__ float sAlpha[65], sPd[65];
+    __shared__ int sSrcKey[MAXS], sTapKey[MAXT], sTapSlot[MAXT];
+    __shared__ float sAmp[MAXS];
+
+    int t = xcd_remap(blockIdx.x, gridDim.x);
+    const int ty = t % a.nty; t /= a.nty;
+    const int tz = t % a.ntz;
+    const int sg = t / a.ntz;
+    const int tile = ty + a.nty * tz;
+
+    const StencilParams& p = a.p;
+    const int Nx = p.Nx, Ny = p.Ny, Nz = p.Nz, sx = Ny * Nz;
+    const int lane = threadIdx.x, w = __builtin_amdgcn_readfirstlane(threadIdx.y);
+    const int tid = lane + 64 * w;
+    const int j0 = ty * TYO, k0 = tz * TZ;
+    const int i0 = sg * a.seg, i1 = min(i0 + a.seg, Nx);
+    const int npml = p.n_pml;
+    if (npml > 0)
+        for (int d = tid; d <= npml && d < 65; d += NT) { sAlpha[d] = a.alpha[d]; sPd[d] = a.pdamp[d]; }
+    const float vcoef = p.vel_coeff, dtdx = p.dtdx, dt = p.dt, kap0 = p.bulk_coeff;
+    const float isv = p.inv_sV, sv = p.sV;
+    const bool loadXi = HET && p.impedance;
+
+    const T* __restrict__ Pin = (const T*)a.f.Pin;
+    T* __restrict__ Pout = (T*)a.f.Pout;
+    const T* __restrict__ Vxi = (const T*)a.f.Vin[0];
+    const T* __restrict__ Vyi = (const T*)a.f.Vin[1];
+    const T* __restrict__ Vzi = (const T*)a.f.Vin[2];
+    T* __restrict__ Vxo = (T*)a.f.Vout[0];
+    T* __restrict__ Vyo = (T*)a.f.Vout[1];
+    T* __restrict__ Vzo = (T*)a.f.Vout[2];
+    const float* __restrict__ bu = a.f.bu;
+    const float* __restrict__ kapA = a.f.kap;
+    const float* __restrict__ xiA = a.f.xi;
+
+    // ── helpers ──
+    auto ldr = [&](const T* __restrict__ F, bool v, int off) { return S::ldr(F, v ? off : 0); };
+    auto ldf = [&](const float* __restrict__ F, bool v, int off) { return F[v ? off : 0]; };
+    // stored-and-reloaded value of a velocity (rndv) / pressure (rndp)
+    // under 16-bit storage; identity for FP32
+    auto vel = [&](RawT x) { const float f = S::cvt(x); return S::exact ? f : f * isv; };
+    auto rndv = [&](float x) { return S::exact ? x : S::cvt(S::raw_of(x * sv)) * isv; };
+    auto rndp = [&](float x) { return S::exact ? x : S::cvt(S::raw_of(x)); };
+    auto pd = [&](int x, int n) { return min(pml_depth(x, n, npml), 64); };
+    auto damp = [&](int dL, int dR) { return face_damp(sAlpha, dL, dR, dt); };
+    auto depth3 = [&](int i, int j, int k) { return imax3(pd(i, Nx), pd(j, Ny), pd(k, Nz)); };
+    auto cellv = [&](int j, int k) { return j >= 0 && j < Ny && k >= 0 && k < Nz; };
+    auto isEdge = [&](int i, int j, int k) {
+        return i == 0 || i == Nx - 1 || j == 0 || j == Ny - 1 || k == 0 || k == Nz - 1;
+    };
+    // pressure update of one cell (impedance wall when on the domain side)
+    auto pres = [&](float pc, float dv, float kap, float xi, int i, int j, int k, bool edge) {
+        float out;
+        if (p.impedance && edge) {
+            const float x = HET ? xi : wall_xi(p.xterm, i, j, k, Nx, Ny, Nz);
+            out = impedance_update(pc, dv, kap, x);
+        } else {
+            out = pc - kap * dv;
+        }
+        if (npml > 0) out = out * sPd[depth3(i, j, k)];
+        return out;
+    };
+
+    // ── region loads (P^n and 1/rho of one plane into the ring) ──
+    // main part: wave w loads region rows 2w-1, 2w (columns 0..63);
+    // halo columns -2, -1, 64, 65 of the 20 rows: threads tid < 80.
+    int ldOff[3], ldLds[3];
+    bool ldV[3];
+#pragma unroll
+    for (int m = 0; m < 2; ++m) {
+        const int rr = 2 * w - 1 + m;
+        const int j = j0 - 1 + rr, k = k0 + lane;
+        ldOff[m] = j * Nz + k;
+        ldV[m] = cellv(j, k);
+        ldLds[m] = gix(rr, lane);
+    }
+    {
+        const int rr = -1 + (tid >> 2), cs = tid & 3;
+        const int cc = cs == 0 ? -2 : cs == 1 ? -1 : cs == 2 ? 64 : 65;
+        const int j = j0 - 1 + rr, k = k0 + cc;
+        ldOff[2] = j * Nz + k;
+        ldV[2] = tid < NHL && cellv(j, k);
+        ldLds[2] = tid < NHL ? gix(rr, cc) : -1;
+    }
+    RawT stP[3];
+    float stB[3];
+    auto load_plane = [&](int pl) {
+        const bool pv = pl >= 0 && pl < Nx;
+        const int base = pl * sx;
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+            stP[m] = ldr(Pin, pv && ldV[m], base + ldOff[m]);
+            if (HET) stB[m] = ldf(bu, pv && ldV[m], base + ldOff[m]);
+        }
+    };
+    auto store_plane = [&](int pl) {
+        const int o = (pl & (NSG - 1)) * GPL;
+#pragma unroll
+        for (int m = 0; m < 2; ++m) {
+            sP[o + ldLds[m]] = S::cvt(stP[m]);
+            if (HET) sB[o + ldLds[m]] = stB[m];
+        }
+        if (ldLds[2] >= 0) {
+            sP[o + ldLds[2]] = S::cvt(stP[2]);
+            if (HET) sB[o + ldLds[2]] = stB[2];
+        }
+    };
+    auto Pn = [&](int pl, int r, int c) { return sP[(pl & (NSG - 1)) * GPL + gix(r, c)]; };
+    auto Bn = [&](int pl, int r, int c) { return sB[(pl & (NSG - 1)) * GPL + gix(r, c)]; };
+
+    // ── thread roles ──
+    const bool mainW = w < NWM;
+    const int k = k0 + lane;
+    const bool kval = k < Nz;
+    // main: extended rows e = 2w + r
+    int me[2], mj[2], moff[2];
+    bool mcell[2], mvy[2], mvz[2];
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+        me[r] = 2 * w + r;
+        mj[r] = j0 - 1 + me[r];
+        moff[r] = mj[r] * Nz + k;
+        mcell[r] = mainW && cellv(mj[r], k);
+        mvy[r] = mainW && kval && mj[r] >= 0 && mj[r] <= Ny - 2;
+        mvz[r] = mainW && mj[r] >= 0 && mj[r] < Ny && k <= Nz - 2;
+    }
+    // halo wave roles
+    const bool hH = !mainW && lane < 32;                 // cell (he, hc)
+    // lanes 32..63 have no cell; they get (1, -1) so that the unconditional
+    // stage-A x-face reads stay inside the LDS tiles (result discarded)
+    const int he = lane < 16 ? lane + 1 : (lane < 32 ? lane - 15 : 1);
+    const int hc = (lane < 16 || lane >= 32) ? -1 : 64;
+    const int hj = j0 - 1 + he, hk = k0 + hc;
+    const int hoff = hj * Nz + hk;
+    const bool hcell = hH && cellv(hj, hk);
+    // Vy faces at rows hj-1, hj (column hk); Vz faces at columns hk-1, hk (row hj)
+    const bool hvy0 = hH && hk >= 0 && hk < Nz && hj - 1 >= 0 && hj - 1 <= Ny - 2;
+    const bool hvy1 = hH && hk >= 0 && hk < Nz && hj >= 0 && hj <= Ny - 2;
+    const bool hvz0 = hH && hj >= 0 && hj < Ny && hk - 1 >= 0 && hk - 1 <= Nz - 2;
+    const bool hvz1 = hH && hj >= 0 && hj < Ny && hk >= 0 && hk <= Nz - 2;
+    // T1: z-face between columns -1 and 0 on rows 0 / 17
+    const bool hT1 = !mainW && (lane == 32 || lane == 33);
+    const int t1e = lane == 32 ? 0 : EY - 1;
+    const int t1j = j0 - 1 + t1e;
+    const int t1off = t1j * Nz + k0 - 1;
+    const bool t1v = hT1 && t1j >= 0 && t1j < Ny && k0 - 1 >= 0 && k0 - 1 <= Nz - 2;
+    // T4: y-face between rows -1 and 0 (global face j0-2) at column lane
+    const int t4j = j0 - 2;
+    const int t4off = t4j * Nz + k;
+    const bool t4v = !mainW && kval && t4j >= 0 && t4j <= Ny - 2;
+
+    // Old values (V^{n-1/2}, kappa, wall sums) of one plane; the halo wave
+    // uses x[0] (Vx), y[0..1] (Vy), z[0..1] (Vz / T1), a (T4), k[0], xi[0].
+    struct Old {
+        RawT x[2], y[2], z[2], a;
+        float kk[2], xi[2];
+    };
+    auto load_old = [&](int q) __attribute__((always_inline)) {
+        Old o;
+        const bool qv = q >= 0 && q < Nx;
+        const bool fv = q >= 0 && q <= Nx - 2;
+        const int b = q * sx;
+        const bool xe = q == 0 || q == Nx - 1;
+        if (mainW) {
+#pragma unroll
+            for (int r = 0; r < 2; ++r) {
+                o.x[r] = ldr(Vxi, fv && mcell[r], b + moff[r]);
+                o.y[r] = ldr(Vyi, qv && mvy[r], b + moff[r]);
+                o.z[r] = ldr(Vzi, qv && mvz[r], b + moff[r]);
+                if (HET) o.kk[r] = ldf(kapA, qv && mcell[r], b + moff[r]);
+                if (loadXi) o.xi[r] = ldf(xiA, qv && mcell[r] && (xe || isEdge(q, mj[r], k)), b + moff[r]);
+            }
+            o.a = RawT();
+        } else {
+            o.x[0] = ldr(Vxi, fv && hcell, b + hoff);
+            o.y[0] = ldr(Vyi, qv && hvy0, b + hoff - Nz);
+            o.y[1] = ldr(Vyi, qv && hvy1, b + hoff);
+            o.z[0] = hT1 ? ldr(Vzi, qv && t1v, b + t1off) : ldr(Vzi, qv && hvz0, b + hoff - 1);
+            o.z[1] = ldr(Vzi, qv && hvz1, b + hoff);
+            o.a = ldr(Vyi, qv && t4v, b + t4off);
+            if (HET) o.kk[0] = ldf(kapA, qv && hcell, b + hoff);
+            if (loadXi) o.xi[0] = ldf(xiA, qv && hcell && isEdge(q, hj, hk), b + hoff);
+            o.x[1] = RawT();
+            o.kk[1] = 0.0f;
+            o.xi[1] = 0.0f;
+        }
+        if (!HET) { o.kk[0] = o.kk[1] = 0.0f; }
+        if (!loadXi) { o.xi[0] = o.xi[1] = 0.0f; }
+        return o;
+    };
+
+    // ── stage A: the x-face between planes q and q+1 at the thread's cells ──
+    // main: cells (me[r], lane); halo: cell (he, hc).
+    auto vxA = [&](int q, const Old& o, float* out) {
+        const bool fv = q >= 0 && q <= Nx - 2;
+        if (mainW) {
+#pragma unroll
+            for (int r = 0; r < 2; ++r) {
+                const float coef = HET ? (0.5f * (Bn(q, me[r], lane) + Bn(q + 1, me[r], lane))) * dtdx : vcoef;
+                float nv = vel(o.x[r]) - coef * (Pn(q + 1, me[r], lane) - Pn(q, me[r], lane));
+                if (npml > 0) nv = nv * damp(depth3(q, mj[r], k), depth3(q + 1, mj[r], k));
+                out[r] = (fv && mcell[r]) ? nv : 0.0f;
+            }
+        } else {
+            const float coef = HET ? (0.5f * (Bn(q, he, hc) + Bn(q + 1, he, hc))) * dtdx : vcoef;
+            float nv = vel(o.x[0]) - coef * (Pn(q + 1, he, hc) - Pn(q, he, hc));
+            if (npml > 0) nv = nv * damp(depth3(q, hj, hk), depth3(q + 1, hj, hk));
+            out[0] = (fv && hcell) ? nv : 0.0f;
+            out[1] = 0.0f;
+        }
+    };
+
+    // ── per-thread pipeline registers ──
+    // stage A results: Vx faces q-1 (xa1) and q (xa0); Vy/Vz of plane q (ya, za)
+    // and of plane q-1 (yd, zd); P^{n+1} of plane q (p1) and q-1 (p1d)
+    float xa0[2], xa1[2], ya[2], za[2], yd[2], zd[2], p1[2], p1d[2];
+    // stage B: Vx^{n+3/2} faces q-1 (xb0) and q-2 (xb1)
+    float xb0[2], xb1[2];
+    // kappa / wall sums of plane q-1 (for stage B)
+    float kd[2], xid[2];
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+        xa0[r] = xa1[r] = ya[r] = za[r] = yd[r] = zd[r] = p1[r] = p1d[r] = 0.0f;
+        xb0[r] = xb1[r] = kd[r] = xid[r] = 0.0f;
+    }
+    // halo-wave extras: Vy faces hj-1, hj; Vz faces hk-1, hk (stage A, plane q)
+    float hy[2] = {0.0f, 0.0f}, hz[2] = {0.0f, 0.0f}, hzd = 0.0f;
+
+    // The tile's source and tap entries (sorted by plane) and the source
+    // amplitudes of step n+1, cached in LDS; walked with a uniform cursor.
+    const int s0 = a.src_start[tile], tp0 = a.tap_start[tile];
+    const int ns = a.step < a.plen ? min(a.src_start[tile + 1] - s0, MAXS) : 0;
+    const int nt = min(a.tap_start[tile + 1] - tp0, MAXT);
+    for (int m = tid; m < ns; m += NT) {
+        const int2 en = a.src[s0 + m];
+        sSrcKey[m] = en.x;
+        sAmp[m] = a.psig[(size_t)en.y * a.plen + a.step];
+    }
+    for (int m = tid; m < nt; m += NT) {
+        const int2 en = a.tap[tp0 + m];
+        sTapKey[m] = en.x;
+        sTapSlot[m] = en.y;
+    }
+    int scur = 0, tcur = 0;            // uniform cursors into the cached lists
+
+    // One plane: stage A at plane q, stage B at plane q-1.  `o` holds the old
+    // values of plane q; returns those of plane q+1 (loaded here, used by the
+    // next plane).
+    auto plane = [&](int q, const Old& o) __attribute__((always_inline)) {
+        const bool qv = q >= 0 && q < Nx;
+        // ── S1: stage-A faces ──
+        vxA(q, o, xa0);
+        if (mainW) {
+#pragma unroll
+            for (int r = 0; r < 2; ++r) {
+                const int e = me[r];
+                {   // y-face between rows e and e+1, column lane
+                    const float coef = HET ? (0.5f * (Bn(q, e, lane) + Bn(q, e + 1, lane))) * dtdx : vcoef;
+                    float nv = vel(o.y[r]) - coef * (Pn(q, e + 1, lane) - Pn(q, e, lane));
+                    if (npml > 0) nv = nv * damp(depth3(q, mj[r], k), depth3(q, mj[r] + 1, k));
+                    ya[r] = (qv && mvy[r]) ? nv : 0.0f;
+                    sAy[ayix(e, lane)] = ya[r];
+                }
+                {   // z-face between columns lane and lane+1, row e
+                    const float coef = HET ? (0.5f * (Bn(q, e, lane) + Bn(q, e, lane + 1))) * dtdx : vcoef;
+                    float nv = vel(o.z[r]) - coef * (Pn(q, e, lane + 1) - Pn(q, e, lane));
+                    if (npml > 0) nv = nv * damp(depth3(q, mj[r], k), depth3(q, mj[r], k + 1));
+                    za[r] = (qv && mvz[r]) ? nv : 0.0f;
+                    sAz[azix(e, lane)] = za[r];
+                }
+            }
+        } else {
+            if (hH) {
+#pragma unroll
+                for (int u = 0; u < 2; ++u) {   // y-faces between rows he-1+u and he+u
+                    const int e = he - 1 + u;
+                    const float coef = HET ? (0.5f * (Bn(q, e, hc) + Bn(q, e + 1, hc))) * dtdx : vcoef;
+                    float nv = vel(o.y[u]) - coef * (Pn(q, e + 1, hc) - Pn(q, e, hc));
+                    if (npml > 0) nv = nv * damp(depth3(q, hj - 1 + u, hk), depth3(q, hj + u, hk));
+                    hy[u] = (qv && (u ? hvy1 : hvy0)) ? nv : 0.0f;
+                }
+#pragma unroll
+                for (int u = 0; u < 2; ++u) {   // z-faces between columns hc-1+u and hc+u
+                    const int c = hc - 1 + u;
+                    const float coef = HET ? (0.5f * (Bn(q, he, c) + Bn(q, he, c + 1))) * dtdx : vcoef;
+                    float nv = vel(o.z[u]) - coef * (Pn(q, he, c + 1) - Pn(q, he, c));
+                    if (npml > 0) nv = nv * damp(depth3(q, hj, hk - 1 + u), depth3(q, hj, hk + u));
+                    hz[u] = (qv && (u ? hvz1 : hvz0)) ? nv : 0.0f;
+                }
+                if (hc < 0) sAz[azix(he, -1)] = hz[1];
+            }
+            if (hT1) {
+                const float coef = HET ? (0.5f * (Bn(q, t1e, -1) + Bn(q, t1e, 0))) * dtdx : vcoef;
+                float nv = vel(o.z[0]) - coef * (Pn(q, t1e, 0) - Pn(q, t1e, -1));
+                if (npml > 0) nv = nv * damp(depth3(q, t1j, k0 - 1), depth3(q, t1j, k0));
+                sAz[azix(t1e, -1)] = (qv && t1v) ? nv : 0.0f;
+            }
+            {   // T4: y-face between rows -1 and 0
+                const float coef = HET ? (0.5f * (Bn(q, -1, lane) + Bn(q, 0, lane))) * dtdx : vcoef;
+                float nv = vel(o.a) - coef * (Pn(q, 0, lane) - Pn(q, -1, lane));
+                if (npml > 0) nv = nv * damp(depth3(q, t4j, k), depth3(q, t4j + 1, k));
+                sAy[ayix(-1, lane)] = (qv && t4v) ? nv : 0.0f;
+            }
+        }
+        __syncthreads();                                                   // (a)
+
+        // ── S2: stage-A pressure at plane q (P^{n+1}) ──
+        const bool xe = q == 0 || q == Nx - 1;
+        float* p1s = sP1 + (q & 1) * EPL;
+        if (mainW) {
+#pragma unroll
+            for (int r = 0; r < 2; ++r) {
+                const int e = me[r];
+                const float dv = (xa0[r] - xa1[r]) + (sAy[ayix(e, lane)] - sAy[ayix(e - 1, lane)])
+                               + (sAz[azix(e, lane)] - sAz[azix(e, lane - 1)]);
+                const float kap = HET ? o.kk[r] : kap0;
+                const float xi = loadXi ? o.xi[r] : 0.0f;
+                const float out = pres(Pn(q, e, lane), dv, kap, xi, q, mj[r], k,
+                                       mcell[r] && (xe || isEdge(q, mj[r], k)));
+                p1[r] = rndp(out);
+            }
+        } else if (hH) {
+            const float dv = (xa0[0] - xa1[0]) + (hy[1] - hy[0]) + (hz[1] - hz[0]);
+            const float kap = HET ? o.kk[0] : kap0;
+            const float xi = loadXi ? o.xi[0] : 0.0f;
+            const float out = pres(Pn(q, he, hc), dv, kap, xi, q, hj, hk,
+                                   hcell && (xe || isEdge(q, hj, hk)));
+            p1[0] = rndp(out);
+        }
+        // point sources of step n+1 (after the rounding, in source order)
+        while (scur < ns && (__builtin_amdgcn_readfirstlane(sSrcKey[scur]) >> 12) < q) ++scur;
+        for (; scur < ns; ++scur) {
+            const int key = __builtin_amdgcn_readfirstlane(sSrcKey[scur]);
+            if ((key >> 12) != q) break;
+            const int e = (key >> 7) & 31, c = (key & 127) - 1;
+            const float amp = sAmp[scur];
+            if (mainW) {
+                if ((e >> 1) == w && c == lane) {
+                    if (e & 1) p1[1] = rndp(p1[1] + amp); else p1[0] = rndp(p1[0] + amp);
+                }
+            } else if (hH && e == he && c == hc) {
+                p1[0] = rndp(p1[0] + amp);
+            }
+        }
+        if (mainW) {
+            p1s[p1ix(me[0], lane)] = p1[0];
+            p1s[p1ix(me[1], lane)] = p1[1];
+        } else if (hH) {
+            p1s[p1ix(he, hc)] = p1[0];
+        }
+        // receiver taps of step n+1 (own planes and cells only)
+        while (tcur < nt && (__builtin_amdgcn_readfirstlane(sTapKey[tcur]) >> 12) < q) ++tcur;
+        for (; tcur < nt; ++tcur) {
+            const int key = __builtin_amdgcn_readfirstlane(sTapKey[tcur]);
+            if ((key >> 12) != q) break;
+            const int e = (key >> 7) & 31, c = (key & 127) - 1;
+            if (mainW && q >= i0 && q < i1 && (e >> 1) == w && c == lane)
+                a.tapbuf[sTapSlot[tcur]] = (e & 1) ? p1[1] : p1[0];
+        }
+        // ring refill: plane q+2 from the staged registers, then prefetch q+3
+        // and the old values of plane q+1 (the slot of q+2 last held plane
+        // q-2, whose last reader ran before this plane's first barrier)
+        store_plane(q + 2);
+        load_plane(q + 3);
+        const Old n = load_old(q + 1);
+        __syncthreads();                                                   // (b)
+
+        // ── S3: stage-B faces at plane qb = q-1 ──
+        const int qb = q - 1;
+        if (qb >= i0 - 1) {
+            const bool fvb = qb >= 0 && qb <= Nx - 2;
+            const bool qbv = qb >= 0 && qb < Nx;
+            const bool ownq = qb >= i0 && qb < i1;
+            const int bb = qb * sx;
+            const float* p1b = sP1 + (qb & 1) * EPL;
+            if (mainW) {
+#pragma unroll
+                for (int r = 0; r < 2; ++r) {
+                    const int e = me[r];
+                    const bool ownr = e >= 1 && e <= TYO;
+                    if (ownr) {   // x-face between planes qb and q
+                        const float coef = HET ? (0.5f * (Bn(qb, e, lane) + Bn(q, e, lane))) * dtdx : vcoef;
+                        float nv = rndv(xa1[r]) - coef * (p1[r] - p1d[r]);
+                        if (npml > 0) nv = nv * damp(depth3(qb, mj[r], k), depth3(q, mj[r], k));
+                        nv = (fvb && mcell[r]) ? nv : 0.0f;
+                        xb0[r] = nv;
+                        if (ownq && fvb && mcell[r]) S::st_nt(Vxo, bb + moff[r], S::exact ? nv : nv * sv);
+                    }
+                    if (qb >= i0) {
+                        if (e <= TYO) {   // y-face between rows e and e+1
+                            const float coef = HET ? (0.5f * (Bn(qb, e, lane) + Bn(qb, e + 1, lane))) * dtdx : vcoef;
+                            float nv = rndv(yd[r]) - coef * (p1b[p1ix(e + 1, lane)] - p1b[p1ix(e, lane)]);
+                            if (npml > 0) nv = nv * damp(depth3(qb, mj[r], k), depth3(qb, mj[r] + 1, k));
+                            nv = (qbv && mvy[r]) ? nv : 0.0f;
+                            sBy[byix(e, lane)] = nv;
+                            if (ownr && qbv && mvy[r]) S::st_nt(Vyo, bb + moff[r], S::exact ? nv : nv * sv);
+                        }
+                        if (ownr) {   // z-face between columns lane and lane+1
+                            const float coef = HET ? (0.5f * (Bn(qb, e, lane) + Bn(qb, e, lane + 1))) * dtdx : vcoef;
+                            float nv = rndv(zd[r]) - coef * (p1b[p1ix(e, lane + 1)] - p1b[p1ix(e, lane)]);
+                            if (npml > 0) nv = nv * damp(depth3(qb, mj[r], k), depth3(qb, mj[r], k + 1));
+                            nv = (qbv && mvz[r]) ? nv : 0.0f;
+                            sBz[bzix(e, lane)] = nv;
+                            if (qbv && mvz[r]) S::st_nt(Vzo, bb + moff[r], S::exact ? nv : nv * sv);
+                        }
+                    }
+                }
+            } else if (hH && hc < 0 && qb >= i0) {   // z-face between columns -1 and 0, row he
+                const float coef = HET ? (0.5f * (Bn(qb, he, -1) + Bn(qb, he, 0))) * dtdx : vcoef;
+                float nv = rndv(hzd) - coef * (p1b[p1ix(he, 0)] - p1d[0]);
+                if (npml > 0) nv = nv * damp(depth3(qb, hj, k0 - 1), depth3(qb, hj, k0));
+                sBz[bzix(he, -1)] = (qbv && hvz1) ? nv : 0.0f;
+            }
+        }
+        if (qb >= i0) {
+            __syncthreads();                                               // (c)
+            // ── S4: stage-B pressure at plane qb (P^{n+2}) ──
+            if (mainW && qb < i1) {
+                const int bb = qb * sx;
+                const bool xeb = qb == 0 || qb == Nx - 1;
+#pragma unroll
+                for (int r = 0; r < 2; ++r) {
+                    const int e = me[r];
+                    if (e < 1 || e > TYO) continue;
+                    const float dv = (xb0[r] - xb1[r]) + (sBy[byix(e, lane)] - sBy[byix(e - 1, lane)])
+                                   + (sBz[bzix(e, lane)] - sBz[bzix(e, lane - 1)]);
+                    const float out = pres(p1d[r], dv, HET ? kd[r] : kap0, xid[r], qb, mj[r], k,
+                                           mcell[r] && (xeb || isEdge(qb, mj[r], k)));
+                    if (mcell[r]) S::st_nt(Pout, bb + moff[r], out);
+                }
+            }
+        }
+        // ── rotate the pipeline ──
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+            xa1[r] = xa0[r];
+            yd[r] = ya[r];
+            zd[r] = za[r];
+            p1d[r] = p1[r];
+            xb1[r] = xb0[r];
+            if (HET) kd[r] = o.kk[r];
+            if (loadXi) xid[r] = o.xi[r];
+        }
+        hzd = hz[1];
+        return n;
+    };
+
+    // ── prologue: ring planes i0-2 .. i0, staged plane i0+1, Vx face i0-2 ──
+    const int qf = i0 - 1;
+    for (int pl = qf - 1; pl <= qf + 1; ++pl) {
+        load_plane(pl);
+        store_plane(pl);
+    }
+    load_plane(qf + 2);
+    {
+        const Old o0 = load_old(qf - 1);
+        __syncthreads();
+        vxA(qf - 1, o0, xa1);
+    }
+    Old oa = load_old(qf);
+    // unrolled by two so that the old-value sets alternate without copies
+    for (int q = qf; q <= i1; q += 2) {
+        const Old ob = plane(q, oa);
+        if (q + 1 > i1) break;
+        oa = plane(q + 1, ob);
+    }
+}
+
+// Receiver traces of the intermediate step from the tap buffer, in the
+// post-step kernel's order (w0*p0 + w1*p1 + ... + w7*p7).
+__global__ void __launch_bounds__(256)
+k_tap_trace(const float* __restrict__ tapbuf, const float* __restrict__ rwgt, int nrec,
+            float* __restrict__ trace, int trace_row, const int* status) {
+    if (status[0]) return;
+    for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < nrec; r += gridDim.x * blockDim.x) {
+        const float* t = tapbuf + 8 * r;
+        const float* w = rwgt + 8 * r;
+        const float v = w[0] * t[0] + w[1] * t[1] + w[2] * t[2] + w[3] * t[3] + w[4] * t[4]
+                      + w[5] * t[5] + w[6] * t[6] + w[7] * t[7];
+        trace[(size_t)trace_row * nrec + r] = v;
+    }
+}
+
+template <typename T, bool HET>
+hipError_t launch2(AcousticFDTDState* st, const Pair2Args& a, int nb) {
+    hipLaunchKernelGGL((k_fused2<T, HET>), dim3(nb), dim3(64, f2::NW), 0, st->stream, a);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+bool fused2_supported(const AcousticFDTDState* st) {
+    return st->order == 2 && fused_supported(st);
+}
+
+// Per-tile source and tap tables (host-built, uploaded when sources or
+// receivers change).  Tile t = ty + nty*tz covers extended rows j0-1..j0+16
+// and columns k0-1..k0+64 of every plane.
+static int build_tables(AcousticFDTDState* st) {
+    using namespace f2;
+    const int nty = (st->Ny + TYO - 1) / TYO, ntz = (st->Nz + TZ - 1) / TZ;
+    const int nt = nty * ntz;
+    std::vector<std::vector<int2>> src(nt), tap(nt);
+    // sources: every tile whose stage A computes the cell (main rows 0..17 x
+    // columns 0..63, halo cells rows 1..16 x columns -1 and 64)
+    for (int s = 0; s < st->npsrc; ++s) {
+        const int c = st->h_pidx[s];
+        const int i = c / (st->Ny * st->Nz), rem = c % (st->Ny * st->Nz);
+        const int j = rem / st->Nz, kk = rem % st->Nz;
+        for (int ty = 0; ty < nty; ++ty) {
+            const int e = j - (ty * TYO - 1);
+            if (e < 0 || e >= EY) continue;
+            for (int tz = 0; tz < ntz; ++tz) {
+                const int cc = kk - tz * TZ;
+                const bool mainCell = cc >= 0 && cc < TZ;
+                const bool haloCell = (cc == -1 || cc == TZ) && e >= 1 && e <= TYO;
+                if (mainCell || haloCell) src[ty + nty * tz].push_back(make_int2(ekey(i, e, cc), s));
+            }
+        }
+    }
+    for (int r = 0; r < st->nrec; ++r)
+        for (int tt = 0; tt < 8; ++tt) {
+            const int c = st->h_ridx[8 * r + tt];
+            const int i = c / (st->Ny * st->Nz), rem = c % (st->Ny * st->Nz);
+            const int j = rem / st->Nz, kk = rem % st->Nz;
+            const int ty = j / TYO, tz = kk / TZ;
+            tap[ty + nty * tz].push_back(make_int2(ekey(i, j - (ty * TYO - 1), kk - tz * TZ), 8 * r + tt));
+        }
+    auto by_key = [](const int2& x, const int2& y) { return x.x < y.x || (x.x == y.x && x.y < y.y); };
+    std::vector<int> ss(nt + 1, 0), ts(nt + 1, 0);
+    std::vector<int2> sflat, tflat;
+    st->f2_ok = 1;
+    for (int t = 0; t < nt; ++t) {
+        std::stable_sort(src[t].begin(), src[t].end(), by_key);
+        std::stable_sort(tap[t].begin(), tap[t].end(), by_key);
+        if ((int)src[t].size() > MAXS || (int)tap[t].size() > MAXT) st->f2_ok = 0;
+        ss[t] = (int)sflat.size();
+        sflat.insert(sflat.end(), src[t].begin(), src[t].end());
+        ts[t] = (int)tflat.size();
+        tflat.insert(tflat.end(), tap[t].begin(), tap[t].end());
+    }
+    ss[nt] = (int)sflat.size();
+    ts[nt] = (int)tflat.size();
+    if (sflat.empty()) sflat.push_back(make_int2(-1, 0));
+    if (tflat.empty()) tflat.push_back(make_int2(-1, 0));
+    auto up = [&](void** d, const void* h, size_t bytes) -> hipError_t {
+        if (*d) { hipError_t e = hipFree(*d); *d = nullptr; if (e != hipSuccess) return e; }
+        hipError_t e = hipMalloc(d, bytes);
+        if (e != hipSuccess) return e;
+        return hipMemcpyAsync(*d, h, bytes, hipMemcpyHostToDevice, st->stream);
+    };
+    AFDTD_CHECK(up((void**)&st->d_f2_src_start, ss.data(), ss.size() * sizeof(int)));
+    AFDTD_CHECK(up((void**)&st->d_f2_src, sflat.data(), sflat.size() * sizeof(int2)));
+    AFDTD_CHECK(up((void**)&st->d_f2_tap_start, ts.data(), ts.size() * sizeof(int)));
+    AFDTD_CHECK(up((void**)&st->d_f2_tap, tflat.data(), tflat.size() * sizeof(int2)));
+    const size_t ntap = st->nrec > 0 ? 8 * (size_t)st->nrec : 8;
+    if (!st->d_f2_tapbuf || st->f2_tapcap < ntap) {
+        if (st->d_f2_tapbuf) AFDTD_CHECK(hipFree(st->d_f2_tapbuf));
+        st->d_f2_tapbuf = nullptr;
+        AFDTD_CHECK(hipMalloc(&st->d_f2_tapbuf, ntap * sizeof(float)));
+        AFDTD_CHECK(hipMemsetAsync(st->d_f2_tapbuf, 0, ntap * sizeof(float), st->stream));
+        st->f2_tapcap = ntap;
+    }
+    AFDTD_CHECK(hipStreamSynchronize(st->stream));
+    st->f2_dirty = 0;
+    return FDTD_OK;
+}
+
+int fused2_prepare(AcousticFDTDState* st) {
+    return st->f2_dirty ? build_tables(st) : FDTD_OK;
+}
+
+hipError_t launch_fused2_pair(AcousticFDTDState* st, int step) {
+    using namespace f2;
+    Pair2Args a;
+    a.p = st->sp;
+    a.f.Pin = st->P[st->cur];
+    a.f.Pout = st->P[st->cur ^ 1];
+    for (int c = 0; c < 3; ++c) {
+        a.f.Vin[c] = st->V[c];
+        a.f.Vout[c] = st->V2[c];
+    }
+    a.f.bu = st->d_bu;
+    a.f.kap = st->d_kap;
+    a.f.xi = st->d_xi;
+    a.alpha = st->d_alpha;
+    a.pdamp = st->d_pdamp;
+    a.status = st->d_status;
+    a.seg = st->fused_seg_x > 0 ? st->fused_seg_x : 32;
+    if (a.seg > st->Nx) a.seg = st->Nx;
+    a.nty = (st->Ny + TYO - 1) / TYO;
+    a.ntz = (st->Nz + TZ - 1) / TZ;
+    a.nseg = (st->Nx + a.seg - 1) / a.seg;
+    a.src_start = st->d_f2_src_start;
+    a.src = (const int2*)st->d_f2_src;
+    a.tap_start = st->d_f2_tap_start;
+    a.tap = (const int2*)st->d_f2_tap;
+    a.psig = st->d_psig;
+    a.plen = st->npsrc > 0 ? st->plen : 0;
+    a.step = step;
+    a.tapbuf = st->d_f2_tapbuf;
+    const int nb = a.nty * a.ntz * a.nseg;
+    hipError_t e;
+    switch (st->storage) {
+        case FDTD_STORE_BF16:
+            e = st->hetero ? launch2<bf16_t, true>(st, a, nb) : launch2<bf16_t, false>(st, a, nb);
+            break;
+        case FDTD_STORE_F16:
+            e = st->hetero ? launch2<f16_t, true>(st, a, nb) : launch2<f16_t, false>(st, a, nb);
+            break;
+        default:
+            e = st->hetero ? launch2<float, true>(st, a, nb) : launch2<float, false>(st, a, nb);
+    }
+    if (e != hipSuccess) return e;
+    for (int c = 0; c < 3; ++c) {
+        void* t = st->V[c];
+        st->V[c] = st->V2[c];
+        st->V2[c] = t;
+    }
+    return hipSuccess;
+}
+
+hipError_t launch_fused2_taps(AcousticFDTDState* st, int trace_row) {
+    if (trace_row < 0 || st->nrec <= 0 || !st->d_trace) return hipSuccess;
+    hipLaunchKernelGGL(k_tap_trace, dim3((st->nrec + 255) / 256), dim3(256), 0, st->stream,
+                       st->d_f2_tapbuf, st->d_rwgt, st->nrec, st->d_trace, trace_row,
+                       st->d_status);
+    return hipGetLastError();
+}

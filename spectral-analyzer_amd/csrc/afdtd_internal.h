@@ -127,6 +127,18 @@ struct AcousticFDTDState {
 
     int step_count, phase;
     int fused_tile_y, fused_seg_x;
+
+    // Two-step sweep: host copies of source cells and receiver taps, and the
+    // per-tile tables built from them (rebuilt when f2_dirty).
+    std::vector<int> h_pidx, h_ridx;
+    int* d_f2_src_start;
+    void* d_f2_src;
+    int* d_f2_tap_start;
+    void* d_f2_tap;
+    float* d_f2_tapbuf;
+    size_t f2_tapcap;
+    int f2_dirty;
+    int f2_ok;              // tables fit the per-tile caches (else single steps)
 };
 
 // Launchers (all enqueue on st->stream; return hipError_t of the launch).
@@ -134,6 +146,15 @@ hipError_t launch_velocity_generic(AcousticFDTDState* st);
 hipError_t launch_pressure_generic(AcousticFDTDState* st);
 hipError_t launch_fused_step(AcousticFDTDState* st);
 bool fused_supported(const AcousticFDTDState* st);
+// Two steps (global indices step, step+1) in one sweep; the receiver taps of
+// the first step go to a tap buffer that launch_fused2_taps turns into a
+// trace row.
+hipError_t launch_fused2_pair(AcousticFDTDState* st, int step);
+hipError_t launch_fused2_taps(AcousticFDTDState* st, int trace_row);
+// (Re)build the per-tile source / tap tables when sources or receivers
+// changed; returns FDTD_OK and sets st->f2_ok.
+int fused2_prepare(AcousticFDTDState* st);
+bool fused2_supported(const AcousticFDTDState* st);
 hipError_t launch_plate_step(AcousticFDTDState* st, int step_index);
 hipError_t launch_plate_inject(AcousticFDTDState* st);
 hipError_t launch_post_step(AcousticFDTDState* st, int step_index, int trace_row);
