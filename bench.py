@@ -119,10 +119,11 @@ def measured_traffic(workload, kernel, storage):
     calibration (the `traffic_bytes_per_launch:` line of the matching summary
     in profiles/), or None when no measurement of this configuration exists."""
     import glob
-    key = {"config2-room-512x384x256-hetero-order2": "fused_room_512x384x256_hetero_order2",
-           "room-512x384x256-order4-pml10": "fused_room_512x384x256_order4_pml10"}.get(workload)
-    if key is None or kernel != "fused" or storage != "f32":
+    suffix = {"config2-room-512x384x256-hetero-order2": "room_512x384x256_hetero_order2",
+              "room-512x384x256-order4-pml10": "room_512x384x256_order4_pml10"}.get(workload)
+    if suffix is None or kernel not in ("fused", "fused2"):
         return None
+    key = f"{kernel}_{suffix}" + ("" if storage == "f32" else f"_{storage}")
     for path in glob.glob(os.path.join(ROOT, "profiles", key + ".txt")):
         for line in open(path):
             if line.startswith("traffic_bytes_per_launch:"):
@@ -287,7 +288,13 @@ def main():
         raise SystemExit("non-finite receiver traces")
     value = w.cells * a.steps / wall
     bpc = w.bytes_per_cell_update_for(a.storage)
-    achieved = w.cells * bpc / (ms_stencil * 1e-3) / 1e9
+    # The two-step sweep ("fused2") advances two steps per launch and reads and
+    # writes every field once per launch: its algorithmic bytes per launch are
+    # those of one step.  ms_stencil is the average stencil time per step.
+    steps_per_launch = 2 if s.kernel_family == "fused2" else 1
+    ms_launch = ms_stencil * steps_per_launch
+    achieved = w.cells * bpc / (ms_launch * 1e-3) / 1e9
+    effective = w.cells * bpc / (ms_stencil * 1e-3) / 1e9
     line = {
         "metric": "cell-updates/s (3D FDTD, 1 GPU)",
         "value": value,
@@ -305,12 +312,19 @@ def main():
                    "courant": w.courant, "boundary": w.boundary, "heterogeneous": w.hetero,
                    "sources": len(w.src), "receivers": int(len(w.rec)),
                    "kernel": s.kernel_family, "storage": a.storage,
-                   "tile_y": a.tile_y or 16, "seg_x": a.seg_x or 32},
+                   "tile_y": a.tile_y or 16,
+                   "seg_x": a.seg_x or (64 if s.kernel_family == "fused2" else 32),
+                   "steps_per_launch": steps_per_launch},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(w.name, s.kernel_family, a.storage),
                      "bytes_per_cell_update": bpc,
-                     "stencil_ms_per_launch": ms_stencil, "event_ms_total": ms_events},
+                     "bytes_per_launch": w.cells * bpc,
+                     "stencil_ms_per_launch": ms_launch, "event_ms_total": ms_events,
+                     # cell-update rate times the one-step byte count: the HBM
+                     # rate a one-step-per-pass sweep would need for this speed
+                     "effective_GBps": effective,
+                     "effective_frac": effective / HBM_PEAK_GBS},
     }
     del s
     ceiling = stream_ceiling(w, a.storage)

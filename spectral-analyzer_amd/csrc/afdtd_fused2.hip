@@ -44,18 +44,22 @@ namespace {
 
 namespace f2 {
 constexpr int TZ = 64;           // own columns (one per lane)
-constexpr int TYO = 16;          // own rows
-constexpr int NWM = 9;           // main waves: extended rows 0..17
-constexpr int NW = NWM + 1;      // + the halo wave
-constexpr int NT = 64 * NW;
-constexpr int EY = TYO + 2;      // extended rows
-constexpr int GR = EY + 2;       // P^n region rows -1..18
 constexpr int GC = TZ + 4;       // P^n region columns -2..65
-constexpr int GPL = GR * GC;
-constexpr int NSG = 4;           // P^n ring: planes q-1..q+2
+constexpr int NSG = 4;           // P^n ring: planes q-2..q+1 (+ q+2 refilled)
 constexpr int EC = TZ + 2;       // P^{n+1} columns -1..64
-constexpr int EPL = EY * EC;
-constexpr int NHL = 80;          // region halo-column elements per plane (20 rows x 4)
+// Tile geometry for R extended rows per main thread.
+template <int R>
+struct Geo2 {
+    static constexpr int NWM = R == 2 ? 9 : 10;   // main waves
+    static constexpr int TYO = R * NWM - 2;       // own rows: 16 (R = 2) or 8 (R = 1)
+    static constexpr int NW = NWM + 1;            // + the halo wave
+    static constexpr int NT = 64 * NW;
+    static constexpr int EY = TYO + 2;            // extended rows
+    static constexpr int GR = EY + 2;             // P^n region rows -1..EY
+    static constexpr int GPL = GR * GC;
+    static constexpr int EPL = EY * EC;
+    static constexpr int NHL = 4 * GR;            // region halo-column elements per plane
+};
 constexpr int MAXS = 32;         // point-source entries cached per tile
 constexpr int MAXT = 256;        // receiver-tap entries cached per tile
 // entry key: plane << 12 | extended row << 7 | (extended column + 1)
@@ -70,6 +74,11 @@ __device__ __forceinline__ int azix(int e, int fc) { return e * (TZ + 1) + (fc +
 __device__ __forceinline__ int byix(int fr, int l) { return fr * TZ + l; }              // fr 0..16
 __device__ __forceinline__ int bzix(int e, int fc) { return (e - 1) * (TZ + 1) + (fc + 1); }
 }  // namespace f2
+
+// Minimum waves per SIMD the register allocation must allow.
+#ifndef AFDTD_F2_WPE
+#define AFDTD_F2_WPE 1
+#endif
 
 struct Pair2Args {
     StencilParams p;
@@ -88,16 +97,20 @@ struct Pair2Args {
     float* tapbuf;          // P^{n+1} at the receiver taps
 };
 
-template <typename T, bool HET>
-__global__ void __launch_bounds__(f2::NT) k_fused2(Pair2Args a) {
+template <typename T, bool HET, int R>
+__global__ void __launch_bounds__(f2::Geo2<R>::NT) __attribute__((amdgpu_waves_per_eu(AFDTD_F2_WPE)))
+k_fused2(Pair2Args a) {
     using namespace f2;
+    typedef Geo2<R> G;
+    constexpr int NWM = G::NWM, TYO = G::TYO, NT = G::NT, EY = G::EY, GPL = G::GPL,
+                  EPL = G::EPL, NHL = G::NHL;
     typedef Store<T> S;
     typedef typename S::Raw RawT;
     if (*a.status) return;
 
     __shared__ float sP[NSG * GPL];
     __shared__ float sB[HET ? NSG * GPL : 1];
-    __shared__ float sP1[2 * EPL];
+    __shared__ float sP1[4 * EPL];
     __shared__ float sAy[(EY + 1) * TZ];
     __shared__ float sAz[EY * (TZ + 1)];
     __shared__ float sBy[(TYO + 1) * TZ];
@@ -138,22 +151,30 @@ __global__ void __launch_bounds__(f2::NT) k_fused2(Pair2Args a) {
     const float* __restrict__ xiA = a.f.xi;
 
     // ── helpers ──
-    auto ldr = [&](const T* __restrict__ F, bool v, int off) { return S::ldr(F, v ? off : 0); };
-    auto ldf = [&](const float* __restrict__ F, bool v, int off) { return F[v ? off : 0]; };
+    // Loads take a per-plane base (uniform; the plane index is clamped into the
+    // grid) and a per-lane byte offset fixed for the whole sweep (0 for a lane
+    // whose cell lies outside the grid).  A value loaded for an invalid plane
+    // or lane only reaches faces and cells that are masked to zero or never
+    // stored, so no per-load select or 64-bit address arithmetic is needed.
+    const int esz = (int)sizeof(T);
+    auto cq = [&](int q) __attribute__((always_inline)) { return min(max(q, 0), Nx - 1); };
+    auto ldT = [&](const T* __restrict__ Fq, uint32_t boff) __attribute__((always_inline)) {
+        return *(const RawT*)((const char*)Fq + boff);
+    };
+    auto ldF = [&](const float* __restrict__ Fq, uint32_t boff) __attribute__((always_inline)) {
+        return *(const float*)((const char*)Fq + boff);
+    };
     // stored-and-reloaded value of a velocity (rndv) / pressure (rndp)
     // under 16-bit storage; identity for FP32
-    auto vel = [&](RawT x) { const float f = S::cvt(x); return S::exact ? f : f * isv; };
-    auto rndv = [&](float x) { return S::exact ? x : S::cvt(S::raw_of(x * sv)) * isv; };
-    auto rndp = [&](float x) { return S::exact ? x : S::cvt(S::raw_of(x)); };
-    auto pd = [&](int x, int n) { return min(pml_depth(x, n, npml), 64); };
-    auto damp = [&](int dL, int dR) { return face_damp(sAlpha, dL, dR, dt); };
-    auto depth3 = [&](int i, int j, int k) { return imax3(pd(i, Nx), pd(j, Ny), pd(k, Nz)); };
-    auto cellv = [&](int j, int k) { return j >= 0 && j < Ny && k >= 0 && k < Nz; };
-    auto isEdge = [&](int i, int j, int k) {
-        return i == 0 || i == Nx - 1 || j == 0 || j == Ny - 1 || k == 0 || k == Nz - 1;
-    };
+    auto vel = [&](RawT x) __attribute__((always_inline)) { const float f = S::cvt(x); return S::exact ? f : f * isv; };
+    auto rndv = [&](float x) __attribute__((always_inline)) { return S::exact ? x : S::cvt(S::raw_of(x * sv)) * isv; };
+    auto rndp = [&](float x) __attribute__((always_inline)) { return S::exact ? x : S::cvt(S::raw_of(x)); };
+    auto pd = [&](int x, int n) __attribute__((always_inline)) { return min(pml_depth(x, n, npml), 64); };
+    auto damp = [&](int dL, int dR) __attribute__((always_inline)) { return face_damp(sAlpha, dL, dR, dt); };
+    auto depth3 = [&](int i, int j, int k) __attribute__((always_inline)) { return imax3(pd(i, Nx), pd(j, Ny), pd(k, Nz)); };
+    auto cellv = [&](int j, int k) __attribute__((always_inline)) { return j >= 0 && j < Ny && k >= 0 && k < Nz; };
     // pressure update of one cell (impedance wall when on the domain side)
-    auto pres = [&](float pc, float dv, float kap, float xi, int i, int j, int k, bool edge) {
+    auto pres = [&](float pc, float dv, float kap, float xi, int i, int j, int k, bool edge) __attribute__((always_inline)) {
         float out;
         if (p.impedance && edge) {
             const float x = HET ? xi : wall_xi(p.xterm, i, j, k, Nx, Ny, Nz);
@@ -168,60 +189,64 @@ __global__ void __launch_bounds__(f2::NT) k_fused2(Pair2Args a) {
     // ── region loads (P^n and 1/rho of one plane into the ring) ──
     // main part: wave w loads region rows 2w-1, 2w (columns 0..63);
     // halo columns -2, -1, 64, 65 of the 20 rows: threads tid < 80.
-    int ldOff[3], ldLds[3];
-    bool ldV[3];
+    uint32_t rgT[3], rgF[3];
+    int ldLds[3];
 #pragma unroll
     for (int m = 0; m < 2; ++m) {
-        const int rr = 2 * w - 1 + m;
+        const int rr = 2 * w - 1 + m;                 // waves beyond row EY idle
         const int j = j0 - 1 + rr, k = k0 + lane;
-        ldOff[m] = j * Nz + k;
-        ldV[m] = cellv(j, k);
-        ldLds[m] = gix(rr, lane);
+        const int off = (rr <= EY && cellv(j, k)) ? j * Nz + k : 0;
+        rgT[m] = (uint32_t)(off * esz);
+        rgF[m] = (uint32_t)(off * 4);
+        ldLds[m] = rr <= EY ? gix(rr, lane) : -1;
     }
     {
         const int rr = -1 + (tid >> 2), cs = tid & 3;
         const int cc = cs == 0 ? -2 : cs == 1 ? -1 : cs == 2 ? 64 : 65;
         const int j = j0 - 1 + rr, k = k0 + cc;
-        ldOff[2] = j * Nz + k;
-        ldV[2] = tid < NHL && cellv(j, k);
+        const int off = (tid < NHL && cellv(j, k)) ? j * Nz + k : 0;
+        rgT[2] = (uint32_t)(off * esz);
+        rgF[2] = (uint32_t)(off * 4);
         ldLds[2] = tid < NHL ? gix(rr, cc) : -1;
     }
-    RawT stP[3];
-    float stB[3];
-    auto load_plane = [&](int pl) {
-        const bool pv = pl >= 0 && pl < Nx;
-        const int base = pl * sx;
+    // P^n and 1/rho of one region plane, staged in registers (unconverted)
+    struct Stage {
+        RawT p[3];
+        float b[3];
+    };
+    auto load_plane = [&](int pl) __attribute__((always_inline)) {
+        Stage st;
+        const size_t b = (size_t)cq(pl) * sx;
 #pragma unroll
         for (int m = 0; m < 3; ++m) {
-            stP[m] = ldr(Pin, pv && ldV[m], base + ldOff[m]);
-            if (HET) stB[m] = ldf(bu, pv && ldV[m], base + ldOff[m]);
+            st.p[m] = ldT(Pin + b, rgT[m]);
+            st.b[m] = HET ? ldF(bu + b, rgF[m]) : 0.0f;
         }
+        return st;
     };
-    auto store_plane = [&](int pl) {
+    auto store_plane = [&](int pl, const Stage& st) __attribute__((always_inline)) {
         const int o = (pl & (NSG - 1)) * GPL;
 #pragma unroll
-        for (int m = 0; m < 2; ++m) {
-            sP[o + ldLds[m]] = S::cvt(stP[m]);
-            if (HET) sB[o + ldLds[m]] = stB[m];
-        }
-        if (ldLds[2] >= 0) {
-            sP[o + ldLds[2]] = S::cvt(stP[2]);
-            if (HET) sB[o + ldLds[2]] = stB[2];
+        for (int m = 0; m < 3; ++m) {
+            if (ldLds[m] >= 0) {
+                sP[o + ldLds[m]] = S::cvt(st.p[m]);
+                if (HET) sB[o + ldLds[m]] = st.b[m];
+            }
         }
     };
-    auto Pn = [&](int pl, int r, int c) { return sP[(pl & (NSG - 1)) * GPL + gix(r, c)]; };
-    auto Bn = [&](int pl, int r, int c) { return sB[(pl & (NSG - 1)) * GPL + gix(r, c)]; };
+    auto Pn = [&](int pl, int r, int c) __attribute__((always_inline)) { return sP[(pl & (NSG - 1)) * GPL + gix(r, c)]; };
+    auto Bn = [&](int pl, int r, int c) __attribute__((always_inline)) { return sB[(pl & (NSG - 1)) * GPL + gix(r, c)]; };
 
     // ── thread roles ──
     const bool mainW = w < NWM;
     const int k = k0 + lane;
     const bool kval = k < Nz;
-    // main: extended rows e = 2w + r
-    int me[2], mj[2], moff[2];
-    bool mcell[2], mvy[2], mvz[2];
+    // main: extended rows e = R*w + r
+    int me[R], mj[R], moff[R];
+    bool mcell[R], mvy[R], mvz[R];
 #pragma unroll
-    for (int r = 0; r < 2; ++r) {
-        me[r] = 2 * w + r;
+    for (int r = 0; r < R; ++r) {
+        me[r] = R * w + r;
         mj[r] = j0 - 1 + me[r];
         moff[r] = mj[r] * Nz + k;
         mcell[r] = mainW && cellv(mj[r], k);
@@ -229,11 +254,11 @@ __global__ void __launch_bounds__(f2::NT) k_fused2(Pair2Args a) {
         mvz[r] = mainW && mj[r] >= 0 && mj[r] < Ny && k <= Nz - 2;
     }
     // halo wave roles
-    const bool hH = !mainW && lane < 32;                 // cell (he, hc)
-    // lanes 32..63 have no cell; they get (1, -1) so that the unconditional
+    const bool hH = !mainW && lane < 2 * TYO;            // cell (he, hc)
+    // lanes >= 2*TYO have no cell; they get (1, -1) so that the unconditional
     // stage-A x-face reads stay inside the LDS tiles (result discarded)
-    const int he = lane < 16 ? lane + 1 : (lane < 32 ? lane - 15 : 1);
-    const int hc = (lane < 16 || lane >= 32) ? -1 : 64;
+    const int he = lane < TYO ? lane + 1 : (lane < 2 * TYO ? lane - TYO + 1 : 1);
+    const int hc = (lane < TYO || lane >= 2 * TYO) ? -1 : 64;
     const int hj = j0 - 1 + he, hk = k0 + hc;
     const int hoff = hj * Nz + hk;
     const bool hcell = hH && cellv(hj, hk);
@@ -243,8 +268,8 @@ __global__ void __launch_bounds__(f2::NT) k_fused2(Pair2Args a) {
     const bool hvz0 = hH && hj >= 0 && hj < Ny && hk - 1 >= 0 && hk - 1 <= Nz - 2;
     const bool hvz1 = hH && hj >= 0 && hj < Ny && hk >= 0 && hk <= Nz - 2;
     // T1: z-face between columns -1 and 0 on rows 0 / 17
-    const bool hT1 = !mainW && (lane == 32 || lane == 33);
-    const int t1e = lane == 32 ? 0 : EY - 1;
+    const bool hT1 = !mainW && (lane == 2 * TYO || lane == 2 * TYO + 1);
+    const int t1e = lane == 2 * TYO ? 0 : EY - 1;
     const int t1j = j0 - 1 + t1e;
     const int t1off = t1j * Nz + k0 - 1;
     const bool t1v = hT1 && t1j >= 0 && t1j < Ny && k0 - 1 >= 0 && k0 - 1 <= Nz - 2;
@@ -252,6 +277,45 @@ __global__ void __launch_bounds__(f2::NT) k_fused2(Pair2Args a) {
     const int t4j = j0 - 2;
     const int t4off = t4j * Nz + k;
     const bool t4v = !mainW && kval && t4j >= 0 && t4j <= Ny - 2;
+    // domain-side cells (impedance walls) on a y or z side; x sides per plane
+    bool medge[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+        medge[r] = mcell[r] && (mj[r] == 0 || mj[r] == Ny - 1 || k == 0 || k == Nz - 1);
+    const bool hedge = hcell && (hj == 0 || hj == Ny - 1 || hk == 0 || hk == Nz - 1);
+
+    // Byte offsets of the old-value loads (one set, meaning by role):
+    //   main: oT[r] / oF[r] the cell (mj[r], k), xF[r] its wall sum (edges only)
+    //   halo: oT[0] the cell (hj, hk), oT[1] (hj-1, hk), oT[2] (hj, hk-1) or the
+    //         T1 face, oT[3] the T4 face; oF[0] / xF[0] the cell
+    // xD: added to xF on x-side planes, where every cell is a wall cell
+    uint32_t oT[4], oF[2], xF[2], xD[2];
+    oT[1] = oF[1] = xF[1] = xD[1] = 0u;
+    if (mainW) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int c = mcell[r] ? moff[r] : 0;
+            oT[r] = (uint32_t)(c * esz);
+            oF[r] = (uint32_t)(c * 4);
+            xF[r] = medge[r] ? oF[r] : 0u;
+            xD[r] = oF[r] - xF[r];
+        }
+        oT[2] = oT[3] = 0u;
+    } else {
+        const int c0 = hcell ? hoff : 0;
+        const int c1 = (hH && cellv(hj - 1, hk)) ? hoff - Nz : 0;
+        const int c2 = hT1 ? (cellv(t1j, k0 - 1) ? t1off : 0)
+                           : ((hH && cellv(hj, hk - 1)) ? hoff - 1 : 0);
+        const int c3 = cellv(t4j, k) ? t4off : 0;
+        oT[0] = (uint32_t)(c0 * esz);
+        oT[1] = (uint32_t)(c1 * esz);
+        oT[2] = (uint32_t)(c2 * esz);
+        oT[3] = (uint32_t)(c3 * esz);
+        oF[0] = (uint32_t)(c0 * 4);
+        xF[0] = hedge ? oF[0] : 0u;
+        xD[0] = oF[0] - xF[0];
+        oF[1] = xF[1] = xD[1] = 0u;
+    }
 
     // Old values (V^{n-1/2}, kappa, wall sums) of one plane; the halo wave
     // uses x[0] (Vx), y[0..1] (Vy), z[0..1] (Vz / T1), a (T4), k[0], xi[0].
@@ -261,45 +325,47 @@ __global__ void __launch_bounds__(f2::NT) k_fused2(Pair2Args a) {
     };
     auto load_old = [&](int q) __attribute__((always_inline)) {
         Old o;
-        const bool qv = q >= 0 && q < Nx;
-        const bool fv = q >= 0 && q <= Nx - 2;
-        const int b = q * sx;
+        const size_t b = (size_t)cq(q) * sx;
         const bool xe = q == 0 || q == Nx - 1;
+        const T* __restrict__ X = Vxi + b;
+        const T* __restrict__ Y = Vyi + b;
+        const T* __restrict__ Z = Vzi + b;
+        o.x[1] = o.y[1] = o.z[1] = RawT();
+        o.kk[1] = o.xi[1] = 0.0f;
         if (mainW) {
 #pragma unroll
-            for (int r = 0; r < 2; ++r) {
-                o.x[r] = ldr(Vxi, fv && mcell[r], b + moff[r]);
-                o.y[r] = ldr(Vyi, qv && mvy[r], b + moff[r]);
-                o.z[r] = ldr(Vzi, qv && mvz[r], b + moff[r]);
-                if (HET) o.kk[r] = ldf(kapA, qv && mcell[r], b + moff[r]);
-                if (loadXi) o.xi[r] = ldf(xiA, qv && mcell[r] && (xe || isEdge(q, mj[r], k)), b + moff[r]);
+            for (int r = 0; r < R; ++r) {
+                o.x[r] = ldT(X, oT[r]);
+                o.y[r] = ldT(Y, oT[r]);
+                o.z[r] = ldT(Z, oT[r]);
+                o.kk[r] = HET ? ldF(kapA + b, oF[r]) : 0.0f;
+                // every cell of an x-side plane is a wall cell
+                o.xi[r] = loadXi ? ldF(xiA + b, xF[r] + (xe ? xD[r] : 0u)) : 0.0f;
             }
             o.a = RawT();
         } else {
-            o.x[0] = ldr(Vxi, fv && hcell, b + hoff);
-            o.y[0] = ldr(Vyi, qv && hvy0, b + hoff - Nz);
-            o.y[1] = ldr(Vyi, qv && hvy1, b + hoff);
-            o.z[0] = hT1 ? ldr(Vzi, qv && t1v, b + t1off) : ldr(Vzi, qv && hvz0, b + hoff - 1);
-            o.z[1] = ldr(Vzi, qv && hvz1, b + hoff);
-            o.a = ldr(Vyi, qv && t4v, b + t4off);
-            if (HET) o.kk[0] = ldf(kapA, qv && hcell, b + hoff);
-            if (loadXi) o.xi[0] = ldf(xiA, qv && hcell && isEdge(q, hj, hk), b + hoff);
+            o.x[0] = ldT(X, oT[0]);
+            o.y[0] = ldT(Y, oT[1]);
+            o.y[1] = ldT(Y, oT[0]);
+            o.z[0] = ldT(Z, oT[2]);
+            o.z[1] = ldT(Z, oT[0]);
+            o.a = ldT(Y, oT[3]);
+            o.kk[0] = HET ? ldF(kapA + b, oF[0]) : 0.0f;
+            o.xi[0] = loadXi ? ldF(xiA + b, xF[0] + (xe ? xD[0] : 0u)) : 0.0f;
             o.x[1] = RawT();
             o.kk[1] = 0.0f;
             o.xi[1] = 0.0f;
         }
-        if (!HET) { o.kk[0] = o.kk[1] = 0.0f; }
-        if (!loadXi) { o.xi[0] = o.xi[1] = 0.0f; }
         return o;
     };
 
     // ── stage A: the x-face between planes q and q+1 at the thread's cells ──
     // main: cells (me[r], lane); halo: cell (he, hc).
-    auto vxA = [&](int q, const Old& o, float* out) {
+    auto vxA = [&](int q, const Old& o, float* out) __attribute__((always_inline)) {
         const bool fv = q >= 0 && q <= Nx - 2;
         if (mainW) {
 #pragma unroll
-            for (int r = 0; r < 2; ++r) {
+            for (int r = 0; r < R; ++r) {
                 const float coef = HET ? (0.5f * (Bn(q, me[r], lane) + Bn(q + 1, me[r], lane))) * dtdx : vcoef;
                 float nv = vel(o.x[r]) - coef * (Pn(q + 1, me[r], lane) - Pn(q, me[r], lane));
                 if (npml > 0) nv = nv * damp(depth3(q, mj[r], k), depth3(q + 1, mj[r], k));
@@ -310,25 +376,27 @@ __global__ void __launch_bounds__(f2::NT) k_fused2(Pair2Args a) {
             float nv = vel(o.x[0]) - coef * (Pn(q + 1, he, hc) - Pn(q, he, hc));
             if (npml > 0) nv = nv * damp(depth3(q, hj, hk), depth3(q + 1, hj, hk));
             out[0] = (fv && hcell) ? nv : 0.0f;
-            out[1] = 0.0f;
+#pragma unroll
+            for (int r = 1; r < R; ++r) out[r] = 0.0f;
         }
     };
 
     // ── per-thread pipeline registers ──
-    // stage A results: Vx faces q-1 (xa1) and q (xa0); Vy/Vz of plane q (ya, za)
-    // and of plane q-1 (yd, zd); P^{n+1} of plane q (p1) and q-1 (p1d)
-    float xa0[2], xa1[2], ya[2], za[2], yd[2], zd[2], p1[2], p1d[2];
-    // stage B: Vx^{n+3/2} faces q-1 (xb0) and q-2 (xb1)
-    float xb0[2], xb1[2];
-    // kappa / wall sums of plane q-1 (for stage B)
-    float kd[2], xid[2];
+    // Stage B runs two planes behind stage A.  Stage A: Vx faces q (xa0),
+    // q-1 (xa1), q-2 (xa2); Vy / Vz of planes q, q-1, q-2 (y0..y2, z0..z2);
+    // P^{n+1} of planes q, q-1, q-2 (pa0..pa2).  Stage B: Vx^{n+3/2} faces q-2
+    // (xb0) and q-3 (xb1).  kappa / wall sums of planes q-1 and q-2.
+    float xa0[R], xa1[R], xa2[R], y0[R], y1[R], y2[R], z0[R], z1[R], z2[R];
+    float pa0[R], pa1[R], pa2[R], xb0[R], xb1[R], kk1[R], kk2[R], xx1[R], xx2[R];
 #pragma unroll
-    for (int r = 0; r < 2; ++r) {
-        xa0[r] = xa1[r] = ya[r] = za[r] = yd[r] = zd[r] = p1[r] = p1d[r] = 0.0f;
-        xb0[r] = xb1[r] = kd[r] = xid[r] = 0.0f;
+    for (int r = 0; r < R; ++r) {
+        xa0[r] = xa1[r] = xa2[r] = y0[r] = y1[r] = y2[r] = z0[r] = z1[r] = z2[r] = 0.0f;
+        pa0[r] = pa1[r] = pa2[r] = xb0[r] = xb1[r] = 0.0f;
+        kk1[r] = kk2[r] = xx1[r] = xx2[r] = 0.0f;
     }
-    // halo-wave extras: Vy faces hj-1, hj; Vz faces hk-1, hk (stage A, plane q)
-    float hy[2] = {0.0f, 0.0f}, hz[2] = {0.0f, 0.0f}, hzd = 0.0f;
+    // halo-wave extras: Vy faces hj-1, hj; Vz faces hk-1, hk (stage A, plane
+    // q); the z-face between columns -1 and 0 of planes q-1 and q-2
+    float hy[2] = {0.0f, 0.0f}, hz[2] = {0.0f, 0.0f}, hz1 = 0.0f, hz2 = 0.0f;
 
     // The tile's source and tap entries (sorted by plane) and the source
     // amplitudes of step n+1, cached in LDS; walked with a uniform cursor.
@@ -347,145 +415,95 @@ __global__ void __launch_bounds__(f2::NT) k_fused2(Pair2Args a) {
     }
     int scur = 0, tcur = 0;            // uniform cursors into the cached lists
 
-    // One plane: stage A at plane q, stage B at plane q-1.  `o` holds the old
-    // values of plane q; returns those of plane q+1 (loaded here, used by the
-    // next plane).
-    auto plane = [&](int q, const Old& o) __attribute__((always_inline)) {
+    // One iteration: stage A at plane q (faces, then pressure) and stage B at
+    // plane qb = q-2 (faces, then pressure), in two phases separated by
+    // barriers.  `o` holds the old values of plane q and `st` the staged
+    // region plane q+2; the loads of the next plane's old values and of
+    // region plane q+3 are issued first, so they have a whole iteration to
+    // arrive.
+    struct Next {
+        Old o;
+        Stage st;
+    };
+    auto plane = [&](int q, const Old& o, const Stage& st) __attribute__((always_inline)) {
+        Next n;
+        n.o = load_old(q + 1);
+        n.st = load_plane(q + 3);
+        const bool doA = q <= i1;
         const bool qv = q >= 0 && q < Nx;
-        // ── S1: stage-A faces ──
-        vxA(q, o, xa0);
-        if (mainW) {
-#pragma unroll
-            for (int r = 0; r < 2; ++r) {
-                const int e = me[r];
-                {   // y-face between rows e and e+1, column lane
-                    const float coef = HET ? (0.5f * (Bn(q, e, lane) + Bn(q, e + 1, lane))) * dtdx : vcoef;
-                    float nv = vel(o.y[r]) - coef * (Pn(q, e + 1, lane) - Pn(q, e, lane));
-                    if (npml > 0) nv = nv * damp(depth3(q, mj[r], k), depth3(q, mj[r] + 1, k));
-                    ya[r] = (qv && mvy[r]) ? nv : 0.0f;
-                    sAy[ayix(e, lane)] = ya[r];
-                }
-                {   // z-face between columns lane and lane+1, row e
-                    const float coef = HET ? (0.5f * (Bn(q, e, lane) + Bn(q, e, lane + 1))) * dtdx : vcoef;
-                    float nv = vel(o.z[r]) - coef * (Pn(q, e, lane + 1) - Pn(q, e, lane));
-                    if (npml > 0) nv = nv * damp(depth3(q, mj[r], k), depth3(q, mj[r], k + 1));
-                    za[r] = (qv && mvz[r]) ? nv : 0.0f;
-                    sAz[azix(e, lane)] = za[r];
-                }
-            }
-        } else {
-            if (hH) {
-#pragma unroll
-                for (int u = 0; u < 2; ++u) {   // y-faces between rows he-1+u and he+u
-                    const int e = he - 1 + u;
-                    const float coef = HET ? (0.5f * (Bn(q, e, hc) + Bn(q, e + 1, hc))) * dtdx : vcoef;
-                    float nv = vel(o.y[u]) - coef * (Pn(q, e + 1, hc) - Pn(q, e, hc));
-                    if (npml > 0) nv = nv * damp(depth3(q, hj - 1 + u, hk), depth3(q, hj + u, hk));
-                    hy[u] = (qv && (u ? hvy1 : hvy0)) ? nv : 0.0f;
-                }
-#pragma unroll
-                for (int u = 0; u < 2; ++u) {   // z-faces between columns hc-1+u and hc+u
-                    const int c = hc - 1 + u;
-                    const float coef = HET ? (0.5f * (Bn(q, he, c) + Bn(q, he, c + 1))) * dtdx : vcoef;
-                    float nv = vel(o.z[u]) - coef * (Pn(q, he, c + 1) - Pn(q, he, c));
-                    if (npml > 0) nv = nv * damp(depth3(q, hj, hk - 1 + u), depth3(q, hj, hk + u));
-                    hz[u] = (qv && (u ? hvz1 : hvz0)) ? nv : 0.0f;
-                }
-                if (hc < 0) sAz[azix(he, -1)] = hz[1];
-            }
-            if (hT1) {
-                const float coef = HET ? (0.5f * (Bn(q, t1e, -1) + Bn(q, t1e, 0))) * dtdx : vcoef;
-                float nv = vel(o.z[0]) - coef * (Pn(q, t1e, 0) - Pn(q, t1e, -1));
-                if (npml > 0) nv = nv * damp(depth3(q, t1j, k0 - 1), depth3(q, t1j, k0));
-                sAz[azix(t1e, -1)] = (qv && t1v) ? nv : 0.0f;
-            }
-            {   // T4: y-face between rows -1 and 0
-                const float coef = HET ? (0.5f * (Bn(q, -1, lane) + Bn(q, 0, lane))) * dtdx : vcoef;
-                float nv = vel(o.a) - coef * (Pn(q, 0, lane) - Pn(q, -1, lane));
-                if (npml > 0) nv = nv * damp(depth3(q, t4j, k), depth3(q, t4j + 1, k));
-                sAy[ayix(-1, lane)] = (qv && t4v) ? nv : 0.0f;
-            }
-        }
-        __syncthreads();                                                   // (a)
+        const int qb = q - 2;
+        const bool qbv = qb >= 0 && qb < Nx;
 
-        // ── S2: stage-A pressure at plane q (P^{n+1}) ──
-        const bool xe = q == 0 || q == Nx - 1;
-        float* p1s = sP1 + (q & 1) * EPL;
-        if (mainW) {
-#pragma unroll
-            for (int r = 0; r < 2; ++r) {
-                const int e = me[r];
-                const float dv = (xa0[r] - xa1[r]) + (sAy[ayix(e, lane)] - sAy[ayix(e - 1, lane)])
-                               + (sAz[azix(e, lane)] - sAz[azix(e, lane - 1)]);
-                const float kap = HET ? o.kk[r] : kap0;
-                const float xi = loadXi ? o.xi[r] : 0.0f;
-                const float out = pres(Pn(q, e, lane), dv, kap, xi, q, mj[r], k,
-                                       mcell[r] && (xe || isEdge(q, mj[r], k)));
-                p1[r] = rndp(out);
-            }
-        } else if (hH) {
-            const float dv = (xa0[0] - xa1[0]) + (hy[1] - hy[0]) + (hz[1] - hz[0]);
-            const float kap = HET ? o.kk[0] : kap0;
-            const float xi = loadXi ? o.xi[0] : 0.0f;
-            const float out = pres(Pn(q, he, hc), dv, kap, xi, q, hj, hk,
-                                   hcell && (xe || isEdge(q, hj, hk)));
-            p1[0] = rndp(out);
-        }
-        // point sources of step n+1 (after the rounding, in source order)
-        while (scur < ns && (__builtin_amdgcn_readfirstlane(sSrcKey[scur]) >> 12) < q) ++scur;
-        for (; scur < ns; ++scur) {
-            const int key = __builtin_amdgcn_readfirstlane(sSrcKey[scur]);
-            if ((key >> 12) != q) break;
-            const int e = (key >> 7) & 31, c = (key & 127) - 1;
-            const float amp = sAmp[scur];
+        // ════ phase 1: stage-A faces (plane q), stage-B faces (plane qb) ════
+        if (doA) {
+            vxA(q, o, xa0);
             if (mainW) {
-                if ((e >> 1) == w && c == lane) {
-                    if (e & 1) p1[1] = rndp(p1[1] + amp); else p1[0] = rndp(p1[0] + amp);
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const int e = me[r];
+                    {   // y-face between rows e and e+1, column lane
+                        const float coef = HET ? (0.5f * (Bn(q, e, lane) + Bn(q, e + 1, lane))) * dtdx : vcoef;
+                        float nv = vel(o.y[r]) - coef * (Pn(q, e + 1, lane) - Pn(q, e, lane));
+                        if (npml > 0) nv = nv * damp(depth3(q, mj[r], k), depth3(q, mj[r] + 1, k));
+                        y0[r] = (qv && mvy[r]) ? nv : 0.0f;
+                        sAy[ayix(e, lane)] = y0[r];
+                    }
+                    {   // z-face between columns lane and lane+1, row e
+                        const float coef = HET ? (0.5f * (Bn(q, e, lane) + Bn(q, e, lane + 1))) * dtdx : vcoef;
+                        float nv = vel(o.z[r]) - coef * (Pn(q, e, lane + 1) - Pn(q, e, lane));
+                        if (npml > 0) nv = nv * damp(depth3(q, mj[r], k), depth3(q, mj[r], k + 1));
+                        z0[r] = (qv && mvz[r]) ? nv : 0.0f;
+                        sAz[azix(e, lane)] = z0[r];
+                    }
                 }
-            } else if (hH && e == he && c == hc) {
-                p1[0] = rndp(p1[0] + amp);
+            } else {
+                if (hH) {
+#pragma unroll
+                    for (int u = 0; u < 2; ++u) {   // y-faces between rows he-1+u and he+u
+                        const int e = he - 1 + u;
+                        const float coef = HET ? (0.5f * (Bn(q, e, hc) + Bn(q, e + 1, hc))) * dtdx : vcoef;
+                        float nv = vel(o.y[u]) - coef * (Pn(q, e + 1, hc) - Pn(q, e, hc));
+                        if (npml > 0) nv = nv * damp(depth3(q, hj - 1 + u, hk), depth3(q, hj + u, hk));
+                        hy[u] = (qv && (u ? hvy1 : hvy0)) ? nv : 0.0f;
+                    }
+#pragma unroll
+                    for (int u = 0; u < 2; ++u) {   // z-faces between columns hc-1+u and hc+u
+                        const int c = hc - 1 + u;
+                        const float coef = HET ? (0.5f * (Bn(q, he, c) + Bn(q, he, c + 1))) * dtdx : vcoef;
+                        float nv = vel(o.z[u]) - coef * (Pn(q, he, c + 1) - Pn(q, he, c));
+                        if (npml > 0) nv = nv * damp(depth3(q, hj, hk - 1 + u), depth3(q, hj, hk + u));
+                        hz[u] = (qv && (u ? hvz1 : hvz0)) ? nv : 0.0f;
+                    }
+                    if (hc < 0) sAz[azix(he, -1)] = hz[1];
+                }
+                if (hT1) {
+                    const float coef = HET ? (0.5f * (Bn(q, t1e, -1) + Bn(q, t1e, 0))) * dtdx : vcoef;
+                    float nv = vel(o.z[0]) - coef * (Pn(q, t1e, 0) - Pn(q, t1e, -1));
+                    if (npml > 0) nv = nv * damp(depth3(q, t1j, k0 - 1), depth3(q, t1j, k0));
+                    sAz[azix(t1e, -1)] = (qv && t1v) ? nv : 0.0f;
+                }
+                {   // T4: y-face between rows -1 and 0
+                    const float coef = HET ? (0.5f * (Bn(q, -1, lane) + Bn(q, 0, lane))) * dtdx : vcoef;
+                    float nv = vel(o.a) - coef * (Pn(q, 0, lane) - Pn(q, -1, lane));
+                    if (npml > 0) nv = nv * damp(depth3(q, t4j, k), depth3(q, t4j + 1, k));
+                    sAy[ayix(-1, lane)] = (qv && t4v) ? nv : 0.0f;
+                }
             }
         }
-        if (mainW) {
-            p1s[p1ix(me[0], lane)] = p1[0];
-            p1s[p1ix(me[1], lane)] = p1[1];
-        } else if (hH) {
-            p1s[p1ix(he, hc)] = p1[0];
-        }
-        // receiver taps of step n+1 (own planes and cells only)
-        while (tcur < nt && (__builtin_amdgcn_readfirstlane(sTapKey[tcur]) >> 12) < q) ++tcur;
-        for (; tcur < nt; ++tcur) {
-            const int key = __builtin_amdgcn_readfirstlane(sTapKey[tcur]);
-            if ((key >> 12) != q) break;
-            const int e = (key >> 7) & 31, c = (key & 127) - 1;
-            if (mainW && q >= i0 && q < i1 && (e >> 1) == w && c == lane)
-                a.tapbuf[sTapSlot[tcur]] = (e & 1) ? p1[1] : p1[0];
-        }
-        // ring refill: plane q+2 from the staged registers, then prefetch q+3
-        // and the old values of plane q+1 (the slot of q+2 last held plane
-        // q-2, whose last reader ran before this plane's first barrier)
-        store_plane(q + 2);
-        load_plane(q + 3);
-        const Old n = load_old(q + 1);
-        __syncthreads();                                                   // (b)
-
-        // ── S3: stage-B faces at plane qb = q-1 ──
-        const int qb = q - 1;
         if (qb >= i0 - 1) {
             const bool fvb = qb >= 0 && qb <= Nx - 2;
-            const bool qbv = qb >= 0 && qb < Nx;
             const bool ownq = qb >= i0 && qb < i1;
             const int bb = qb * sx;
-            const float* p1b = sP1 + (qb & 1) * EPL;
+            const float* p1b = sP1 + (qb & 3) * EPL;
             if (mainW) {
 #pragma unroll
-                for (int r = 0; r < 2; ++r) {
+                for (int r = 0; r < R; ++r) {
                     const int e = me[r];
                     const bool ownr = e >= 1 && e <= TYO;
-                    if (ownr) {   // x-face between planes qb and q
-                        const float coef = HET ? (0.5f * (Bn(qb, e, lane) + Bn(q, e, lane))) * dtdx : vcoef;
-                        float nv = rndv(xa1[r]) - coef * (p1[r] - p1d[r]);
-                        if (npml > 0) nv = nv * damp(depth3(qb, mj[r], k), depth3(q, mj[r], k));
+                    if (ownr) {   // x-face between planes qb and qb+1
+                        const float coef = HET ? (0.5f * (Bn(qb, e, lane) + Bn(qb + 1, e, lane))) * dtdx : vcoef;
+                        float nv = rndv(xa2[r]) - coef * (pa1[r] - pa2[r]);
+                        if (npml > 0) nv = nv * damp(depth3(qb, mj[r], k), depth3(qb + 1, mj[r], k));
                         nv = (fvb && mcell[r]) ? nv : 0.0f;
                         xb0[r] = nv;
                         if (ownq && fvb && mcell[r]) S::st_nt(Vxo, bb + moff[r], S::exact ? nv : nv * sv);
@@ -493,7 +511,7 @@ __global__ void __launch_bounds__(f2::NT) k_fused2(Pair2Args a) {
                     if (qb >= i0) {
                         if (e <= TYO) {   // y-face between rows e and e+1
                             const float coef = HET ? (0.5f * (Bn(qb, e, lane) + Bn(qb, e + 1, lane))) * dtdx : vcoef;
-                            float nv = rndv(yd[r]) - coef * (p1b[p1ix(e + 1, lane)] - p1b[p1ix(e, lane)]);
+                            float nv = rndv(y2[r]) - coef * (p1b[p1ix(e + 1, lane)] - p1b[p1ix(e, lane)]);
                             if (npml > 0) nv = nv * damp(depth3(qb, mj[r], k), depth3(qb, mj[r] + 1, k));
                             nv = (qbv && mvy[r]) ? nv : 0.0f;
                             sBy[byix(e, lane)] = nv;
@@ -501,7 +519,7 @@ __global__ void __launch_bounds__(f2::NT) k_fused2(Pair2Args a) {
                         }
                         if (ownr) {   // z-face between columns lane and lane+1
                             const float coef = HET ? (0.5f * (Bn(qb, e, lane) + Bn(qb, e, lane + 1))) * dtdx : vcoef;
-                            float nv = rndv(zd[r]) - coef * (p1b[p1ix(e, lane + 1)] - p1b[p1ix(e, lane)]);
+                            float nv = rndv(z2[r]) - coef * (p1b[p1ix(e, lane + 1)] - p1b[p1ix(e, lane)]);
                             if (npml > 0) nv = nv * damp(depth3(qb, mj[r], k), depth3(qb, mj[r], k + 1));
                             nv = (qbv && mvz[r]) ? nv : 0.0f;
                             sBz[bzix(e, lane)] = nv;
@@ -511,62 +529,134 @@ __global__ void __launch_bounds__(f2::NT) k_fused2(Pair2Args a) {
                 }
             } else if (hH && hc < 0 && qb >= i0) {   // z-face between columns -1 and 0, row he
                 const float coef = HET ? (0.5f * (Bn(qb, he, -1) + Bn(qb, he, 0))) * dtdx : vcoef;
-                float nv = rndv(hzd) - coef * (p1b[p1ix(he, 0)] - p1d[0]);
+                float nv = rndv(hz2) - coef * (p1b[p1ix(he, 0)] - pa2[0]);
                 if (npml > 0) nv = nv * damp(depth3(qb, hj, k0 - 1), depth3(qb, hj, k0));
                 sBz[bzix(he, -1)] = (qbv && hvz1) ? nv : 0.0f;
             }
         }
-        if (qb >= i0) {
-            __syncthreads();                                               // (c)
-            // ── S4: stage-B pressure at plane qb (P^{n+2}) ──
-            if (mainW && qb < i1) {
-                const int bb = qb * sx;
-                const bool xeb = qb == 0 || qb == Nx - 1;
+        __syncthreads();                                                   // (1)
+
+        // ════ phase 2: stage-A pressure (plane q), stage-B pressure (plane qb) ════
+        if (doA) {
+            const bool xe = q == 0 || q == Nx - 1;
+            float* p1s = sP1 + (q & 3) * EPL;
+            if (mainW) {
 #pragma unroll
-                for (int r = 0; r < 2; ++r) {
+                for (int r = 0; r < R; ++r) {
                     const int e = me[r];
-                    if (e < 1 || e > TYO) continue;
-                    const float dv = (xb0[r] - xb1[r]) + (sBy[byix(e, lane)] - sBy[byix(e - 1, lane)])
-                                   + (sBz[bzix(e, lane)] - sBz[bzix(e, lane - 1)]);
-                    const float out = pres(p1d[r], dv, HET ? kd[r] : kap0, xid[r], qb, mj[r], k,
-                                           mcell[r] && (xeb || isEdge(qb, mj[r], k)));
-                    if (mcell[r]) S::st_nt(Pout, bb + moff[r], out);
+                    const float dv = (xa0[r] - xa1[r]) + (sAy[ayix(e, lane)] - sAy[ayix(e - 1, lane)])
+                                   + (sAz[azix(e, lane)] - sAz[azix(e, lane - 1)]);
+                    const float kap = HET ? o.kk[r] : kap0;
+                    const float xi = loadXi ? o.xi[r] : 0.0f;
+                    const float out = pres(Pn(q, e, lane), dv, kap, xi, q, mj[r], k,
+                                           medge[r] || (xe && mcell[r]));
+                    pa0[r] = rndp(out);
+                }
+            } else if (hH) {
+                const float dv = (xa0[0] - xa1[0]) + (hy[1] - hy[0]) + (hz[1] - hz[0]);
+                const float kap = HET ? o.kk[0] : kap0;
+                const float xi = loadXi ? o.xi[0] : 0.0f;
+                const float out = pres(Pn(q, he, hc), dv, kap, xi, q, hj, hk,
+                                       hedge || (xe && hcell));
+                pa0[0] = rndp(out);
+            }
+            // point sources of step n+1 (after the rounding, in source order)
+            while (scur < ns && (__builtin_amdgcn_readfirstlane(sSrcKey[scur]) >> 12) < q) ++scur;
+            for (; scur < ns; ++scur) {
+                const int key = __builtin_amdgcn_readfirstlane(sSrcKey[scur]);
+                if ((key >> 12) != q) break;
+                const int e = (key >> 7) & 31, c = (key & 127) - 1;
+                const float amp = sAmp[scur];
+                if (mainW) {
+                    if (e / R == w && c == lane) {
+#pragma unroll
+                        for (int r = 0; r < R; ++r)
+                            if (e % R == r) pa0[r] = rndp(pa0[r] + amp);
+                    }
+                } else if (hH && e == he && c == hc) {
+                    pa0[0] = rndp(pa0[0] + amp);
                 }
             }
+            if (mainW) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) p1s[p1ix(me[r], lane)] = pa0[r];
+            } else if (hH) {
+                p1s[p1ix(he, hc)] = pa0[0];
+            }
+            // receiver taps of step n+1 (own planes and cells only)
+            while (tcur < nt && (__builtin_amdgcn_readfirstlane(sTapKey[tcur]) >> 12) < q) ++tcur;
+            for (; tcur < nt; ++tcur) {
+                const int key = __builtin_amdgcn_readfirstlane(sTapKey[tcur]);
+                if ((key >> 12) != q) break;
+                const int e = (key >> 7) & 31, c = (key & 127) - 1;
+                if (mainW && q >= i0 && q < i1 && e / R == w && c == lane) {
+                    float v = pa0[0];
+#pragma unroll
+                    for (int r = 1; r < R; ++r)
+                        if (e % R == r) v = pa0[r];
+                    a.tapbuf[sTapSlot[tcur]] = v;
+                }
+            }
+            // ring refill: plane q+2 (its slot last held plane q-2, last read by
+            // this iteration's stage-B faces before barrier (1))
+            store_plane(q + 2, st);
         }
+        if (qb >= i0 && mainW) {
+            const int bb = qb * sx;
+            const bool xeb = qb == 0 || qb == Nx - 1;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int e = me[r];
+                if (e < 1 || e > TYO) continue;
+                const float dv = (xb0[r] - xb1[r]) + (sBy[byix(e, lane)] - sBy[byix(e - 1, lane)])
+                               + (sBz[bzix(e, lane)] - sBz[bzix(e, lane - 1)]);
+                const float out = pres(pa2[r], dv, HET ? kk2[r] : kap0, xx2[r], qb, mj[r], k,
+                                       medge[r] || (xeb && mcell[r]));
+                if (mcell[r]) S::st_nt(Pout, bb + moff[r], out);
+            }
+        }
+        __syncthreads();                                                   // (2)
+
         // ── rotate the pipeline ──
 #pragma unroll
-        for (int r = 0; r < 2; ++r) {
+        for (int r = 0; r < R; ++r) {
+            xa2[r] = xa1[r];
             xa1[r] = xa0[r];
-            yd[r] = ya[r];
-            zd[r] = za[r];
-            p1d[r] = p1[r];
+            y2[r] = y1[r];
+            y1[r] = y0[r];
+            z2[r] = z1[r];
+            z1[r] = z0[r];
+            pa2[r] = pa1[r];
+            pa1[r] = pa0[r];
             xb1[r] = xb0[r];
-            if (HET) kd[r] = o.kk[r];
-            if (loadXi) xid[r] = o.xi[r];
+            kk2[r] = kk1[r];
+            kk1[r] = o.kk[r];
+            xx2[r] = xx1[r];
+            xx1[r] = o.xi[r];
         }
-        hzd = hz[1];
+        hz2 = hz1;
+        hz1 = hz[1];
         return n;
     };
 
     // ── prologue: ring planes i0-2 .. i0, staged plane i0+1, Vx face i0-2 ──
     const int qf = i0 - 1;
-    for (int pl = qf - 1; pl <= qf + 1; ++pl) {
-        load_plane(pl);
-        store_plane(pl);
-    }
-    load_plane(qf + 2);
+    for (int pl = qf - 1; pl <= qf + 1; ++pl) store_plane(pl, load_plane(pl));
+    Stage sa = load_plane(qf + 2);
     {
         const Old o0 = load_old(qf - 1);
         __syncthreads();
         vxA(qf - 1, o0, xa1);
     }
     Old oa = load_old(qf);
-    // unrolled by two so that the old-value sets alternate without copies
-    for (int q = qf; q <= i1; q += 2) {
-        const Old ob = plane(q, oa);
-        if (q + 1 > i1) break;
-        oa = plane(q + 1, ob);
+    // iterations q = i0-1 .. i1+1 (stage B covers planes i0-1 .. i1-1),
+    // unrolled by two so that the register sets alternate without copies
+    for (int q = qf; q <= i1 + 1; q += 2) {
+        const Next nb = plane(q, oa, sa);
+        if (q + 1 > i1 + 1) break;
+        const Next na = plane(q + 1, nb.o, nb.st);
+        oa = na.o;
+        sa = na.st;
     }
 }
 
@@ -587,7 +677,12 @@ k_tap_trace(const float* __restrict__ tapbuf, const float* __restrict__ rwgt, in
 
 template <typename T, bool HET>
 hipError_t launch2(AcousticFDTDState* st, const Pair2Args& a, int nb) {
-    hipLaunchKernelGGL((k_fused2<T, HET>), dim3(nb), dim3(64, f2::NW), 0, st->stream, a);
+    if (st->f2_rows == 1)
+        hipLaunchKernelGGL((k_fused2<T, HET, 1>), dim3(nb), dim3(64, f2::Geo2<1>::NW), 0,
+                           st->stream, a);
+    else
+        hipLaunchKernelGGL((k_fused2<T, HET, 2>), dim3(nb), dim3(64, f2::Geo2<2>::NW), 0,
+                           st->stream, a);
     return hipGetLastError();
 }
 
@@ -602,6 +697,8 @@ bool fused2_supported(const AcousticFDTDState* st) {
 // and columns k0-1..k0+64 of every plane.
 static int build_tables(AcousticFDTDState* st) {
     using namespace f2;
+    const int TYO = st->f2_rows == 1 ? Geo2<1>::TYO : Geo2<2>::TYO;
+    const int EY = TYO + 2;
     const int nty = (st->Ny + TYO - 1) / TYO, ntz = (st->Nz + TZ - 1) / TZ;
     const int nt = nty * ntz;
     std::vector<std::vector<int2>> src(nt), tap(nt);
@@ -690,8 +787,9 @@ hipError_t launch_fused2_pair(AcousticFDTDState* st, int step) {
     a.alpha = st->d_alpha;
     a.pdamp = st->d_pdamp;
     a.status = st->d_status;
-    a.seg = st->fused_seg_x > 0 ? st->fused_seg_x : 32;
+    a.seg = st->fused_seg_x > 0 ? st->fused_seg_x : 64;
     if (a.seg > st->Nx) a.seg = st->Nx;
+    const int TYO = st->f2_rows == 1 ? Geo2<1>::TYO : Geo2<2>::TYO;
     a.nty = (st->Ny + TYO - 1) / TYO;
     a.ntz = (st->Nz + TZ - 1) / TZ;
     a.nseg = (st->Nx + a.seg - 1) / a.seg;

@@ -139,6 +139,7 @@ struct AcousticFDTDState {
     size_t f2_tapcap;
     int f2_dirty;
     int f2_ok;              // tables fit the per-tile caches (else single steps)
+    int f2_rows;            // extended rows per thread: 2 (16-row tiles) or 1 (8-row)
 };
 
 // Launchers (all enqueue on st->stream; return hipError_t of the launch).

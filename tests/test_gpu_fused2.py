@@ -27,7 +27,7 @@ def _media(Nx, Ny, Nz, seed):
 
 
 def _make(oracle_mod, Nx, Ny, Nz, hetero, imped, n_pml, seg_x, kernel="fused2", storage="f32",
-          seed=12):
+          seed=12, tile_y=0):
     kw = dict(order=2, courant=0.5, n_pml=n_pml)
     if imped:
         kw.update(boundary="impedance", refl=[0.6, 0.7, 0.8, 0.85, 0.9, 0.95])
@@ -39,18 +39,20 @@ def _make(oracle_mod, Nx, Ny, Nz, hetero, imped, n_pml, seg_x, kernel="fused2", 
     ct = np.zeros(Nx * Ny * Nz, np.uint8)
     pa = np.zeros(Nx * Ny, np.uint8)
     g = AcousticFDTD(Nx, Ny, Nz, 0.01, c=343.0, rho_air=1.21, cell_type=ct, plate_iz=Nz // 2,
-                     plate_active=pa, kernel=kernel, storage=storage, seg_x=seg_x, **kw)
+                     plate_active=pa, kernel=kernel, storage=storage, seg_x=seg_x,
+                     tile_y=tile_y, **kw)
     assert g.kernel_family == kernel
     return g, o
 
 
 @pytest.mark.parametrize("hetero,imped,n_pml", [(False, False, 0), (True, True, 0),
                                                 (False, True, 0), (True, False, 3)])
-@pytest.mark.parametrize("seg_x", [64, 5, 1])
-def test_fused2_bitwise(oracle_mod, hetero, imped, n_pml, seg_x):
+@pytest.mark.parametrize("seg_x,tile_y", [(64, 16), (5, 16), (1, 16), (64, 8), (3, 8)])
+def test_fused2_bitwise(oracle_mod, hetero, imped, n_pml, seg_x, tile_y):
+    """tile_y 16: two extended rows per thread; 8: one row per thread."""
     Nx, Ny, Nz = 23, 37, 70
-    print(f"fused2 bitwise: het={hetero} imp={imped} pml={n_pml} seg={seg_x}", flush=True)
-    g, o = _make(oracle_mod, Nx, Ny, Nz, hetero, imped, n_pml, seg_x)
+    print(f"fused2 bitwise: het={hetero} imp={imped} pml={n_pml} seg={seg_x} ty={tile_y}", flush=True)
+    g, o = _make(oracle_mod, Nx, Ny, Nz, hetero, imped, n_pml, seg_x, tile_y=tile_y)
     _random_state(o, g, seed=13)
     for chunk in (1, 2, 9, 30):
         g.step(chunk); o.step(chunk)
@@ -68,14 +70,15 @@ def test_fused2_multi_tile_grid(oracle_mod):
     _assert_fields_equal(g, o, "fused2 multi-tile")
 
 
-def test_fused2_sources_receivers_traces(oracle_mod):
+@pytest.mark.parametrize("tile_y", [16, 8])
+def test_fused2_sources_receivers_traces(oracle_mod, tile_y):
     """Sources on tile edges / corners and in the interior, receivers at
     fractional and cell-centre positions; 300 steps cross the stability check
     at 256, so one single step is scheduled before it."""
     from afdtd import differentiated_gaussian
     Nx, Ny, Nz = 30, 34, 130
     print("fused2 sources/receivers 30x34x130", flush=True)
-    g, o = _make(oracle_mod, Nx, Ny, Nz, True, True, 0, 11)
+    g, o = _make(oracle_mod, Nx, Ny, Nz, True, True, 0, 11, tile_y=tile_y)
     cells = [(5, 15, 63), (6, 16, 64), (7, 15, 64), (10, 16, 63), (12, 0, 0), (20, 33, 129),
              (15, 17, 65), (9, 31, 127), (5, 15, 63)]          # last: same cell twice
     src = [int(k + Nz * (j + Ny * i)) for i, j, k in cells]
