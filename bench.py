@@ -140,6 +140,9 @@ def bench_batch(a):
     cells = int(np.prod(dims, axis=1).sum())
     b = RoomBatch(dims, 0.01, w["courant"], w["refl"], c=343.0, rho_air=1.2, order=2,
                   tile_y=a.tile_y, seg_x=a.seg_x)
+    tuned = None
+    if not (a.tile_y or a.seg_x):
+        tuned = b.autotune(trial_steps=10)            # before the first step
     b.set_sources(w["src"], w["signals"])
     b.set_receivers(w["rec"])
     b.record(steps + 10)
@@ -174,7 +177,7 @@ def bench_batch(a):
             "ms_per_step": wall_b * 1e3 / steps, "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": "fp32", "data": "synthetic (seeded rooms)",
             "config": {"workload": "config3-256-rooms-48to160-order2", "rooms": int(len(dims)),
-                       "cells": cells, "graph_device_ms": ms_dev},
+                       "cells": cells, "graph_device_ms": ms_dev, "autotuned": tuned},
             "per_room_launches": {"value": cells * steps / t_rooms, "unit": "cell-updates/s",
                                   "seconds": t_rooms},
             "speedup_vs_per_room": t_rooms / wall_b,
@@ -306,7 +309,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"f32": "fp32", "bf16": "bf16-storage/fp32", "f16": "fp16-storage/fp32"}[a.storage],
+        "dtype": {"f32": "fp32", "bf16": "bf16-storage/fp32", "f16": "fp16-storage/fp32",
+                  "bfp16": "int16-block-float-storage/fp32"}[a.storage],
         "data": "synthetic (seeded media, sources and receivers)",
         "config": {"workload": w.name, "cells": w.cells, "stencil_order": w.order,
                    "courant": w.courant, "boundary": w.boundary, "heterogeneous": w.hetero,

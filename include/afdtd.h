@@ -183,6 +183,11 @@ SK_API int fdtd_get_surface_emission(const AcousticFDTDState* st, int n_surf,
 #define FDTD_STORE_F32  0
 #define FDTD_STORE_BF16 1
 #define FDTD_STORE_F16  2
+/* Block floating point: int16 mantissas plus one power-of-two exponent per
+ * 64-cell z-row segment (absolute error <= 2^-16 of the segment's largest
+ * magnitude).  Fused sweep only: no walls, plate cells, face fractions or
+ * active plate. */
+#define FDTD_STORE_BFP16 3
 
 /* Kernel family selection. */
 #define FDTD_KERNEL_AUTO    0
@@ -277,6 +282,13 @@ SK_API FdtdBatch* fdtd_batch_create(int n_rooms, const int* dims, float dx, floa
                                     int stencil_order, int storage, int tile_y, int seg_x);
 SK_API void fdtd_batch_destroy(FdtdBatch* b);
 SK_API int fdtd_batch_cells(const FdtdBatch* b, long long* total, int* n_tiles);
+/* Autotuner (before the first step): times every tile height (8, 16, 32 rows),
+ * x-segment (16..128 planes) and dispatch order (room order / largest rooms
+ * first) with trial_steps graph-captured steps of this batch, keeps the
+ * fastest, and leaves the batch at step 0 with zero fields.  Results of the
+ * sweep do not depend on the choice (same arithmetic per cell). */
+SK_API int fdtd_batch_autotune(FdtdBatch* b, int trial_steps, int* best_tile_y, int* best_seg,
+                               int* best_schedule, float* best_ms_per_step);
 SK_API int fdtd_batch_set_sources(FdtdBatch* b, const int* cell_idx, const float* signals,
                                   int n_samples);
 SK_API int fdtd_batch_set_receivers(FdtdBatch* b, int per_room, const float* xyz);

@@ -16,7 +16,7 @@ FDTD_AIR, FDTD_WALL, FDTD_PLATE, FDTD_PML = 0, 1, 2, 3
 FDTD_OK, FDTD_ERR_NULL, FDTD_ERR_DIM, FDTD_ERR_UNSTABLE = 0, -1, -2, -3
 FDTD_ERR_DEVICE, FDTD_ERR_ARG = -4, -5
 FDTD_BOUNDARY_RIGID_PML, FDTD_BOUNDARY_IMPEDANCE = 0, 1
-FDTD_STORE_F32, FDTD_STORE_BF16, FDTD_STORE_F16 = 0, 1, 2
+FDTD_STORE_F32, FDTD_STORE_BF16, FDTD_STORE_F16, FDTD_STORE_BFP16 = 0, 1, 2, 3
 FDTD_KERNEL_AUTO, FDTD_KERNEL_GENERIC, FDTD_KERNEL_FUSED, FDTD_KERNEL_FUSED2 = 0, 1, 2, 3
 
 # The 26 functions of the reference ABI (csrc/include/acoustic_fdtd.h:137-506).
@@ -87,6 +87,7 @@ _SIGS = {
     "fdtd_batch_create": (_vp, [_i, _vp, _f, _f, _f, _vp, _vp, _i, _i, _i, _i]),
     "fdtd_batch_destroy": (None, [_vp]),
     "fdtd_batch_cells": (_i, [_vp, _vp, _vp]),
+    "fdtd_batch_autotune": (_i, [_vp, _i, _vp, _vp, _vp, _vp]),
     "fdtd_batch_set_sources": (_i, [_vp, _vp, _vp, _i]),
     "fdtd_batch_set_receivers": (_i, [_vp, _i, _vp]),
     "fdtd_batch_record": (_i, [_vp, _i]),

@@ -57,6 +57,18 @@ class RoomBatch:
     def step_count(self) -> int:
         return int(self._L.fdtd_batch_get_step_count(self._h))
 
+    def autotune(self, trial_steps: int = 10) -> dict:
+        """Pick the tile height, x-segment and dispatch order by timing every
+        candidate on this batch (before the first step; fields stay zero)."""
+        ty, sg, so, ms = C.c_int(), C.c_int(), C.c_int(), C.c_float()
+        rc = self._L.fdtd_batch_autotune(self._h, int(trial_steps), C.byref(ty), C.byref(sg),
+                                         C.byref(so), C.byref(ms))
+        if rc:
+            raise RuntimeError(f"fdtd_batch_autotune error {rc}: {_lib.last_error()}")
+        return {"tile_y": ty.value, "seg_x": sg.value,
+                "schedule": ("room order", "largest first")[so.value],
+                "ms_per_step": ms.value}
+
     def set_sources(self, cell_idx, signals):
         idx = np.ascontiguousarray(np.asarray(cell_idx, np.int32).ravel())
         sig = np.ascontiguousarray(np.asarray(signals, np.float32).reshape(self.n, -1))

@@ -48,6 +48,14 @@ void dfree(void* p) {
 }
 
 int storage_bytes(int storage) { return storage == FDTD_STORE_F32 ? 4 : 2; }
+// bytes of one field buffer: N elements, plus one exponent byte per 64-cell
+// z-row segment for block floating point
+size_t field_bytes(int storage, int Nx, int Ny, int Nz) {
+    const size_t N = (size_t)Nx * Ny * Nz;
+    size_t b = N * storage_bytes(storage);
+    if (storage == FDTD_STORE_BFP16) b += (size_t)Nx * Ny * ((Nz + 63) / 64);
+    return b;
+}
 
 // Copy a packed staggered array (reference layout) into the padded layout.
 // Packed rows of `w` elements at pitch `w`, padded rows at pitch `pitch`.
@@ -69,7 +77,7 @@ const float* field_f32(const AcousticFDTDState* st, int which /*0=P,1..3=V*/) {
     const void* src = which == 0 ? st->P[st->cur] : st->V[which - 1];
     if (st->storage == FDTD_STORE_F32) return (const float*)src;
     float scale = which == 0 ? 1.0f : st->sp.inv_sV;
-    (void)launch_to_f32(src, st->storage, scale, st->stage, (size_t)st->N, st->stream);
+    (void)launch_to_f32(src, st->storage, scale, st->stage, (size_t)st->N, st->stream, st->Nz);
     return st->stage;
 }
 
@@ -100,7 +108,7 @@ int gather_host(AcousticFDTDState* st, int which, const int* idx, const float* w
     if (which == 4) { F = st->d_w[0]; scale = 1.0f; }
     int storage = which == 4 ? FDTD_STORE_F32 : st->storage;
     AFDTD_CHECK(launch_gather(F, storage, scale, st->d_gidx, st->d_gwgt, n, taps, zero_start,
-                              st->d_gout, st->stream));
+                              st->d_gout, st->stream, st->N, st->Nz));
     AFDTD_CHECK(hipMemcpyAsync(out, st->d_gout, sizeof(float) * n, hipMemcpyDeviceToHost,
                                st->stream));
     return sync(st);
@@ -286,7 +294,7 @@ AcousticFDTDState* create_impl(int Nx, int Ny, int Nz, float dx, float c, float 
         afdtd_set_error("impedance boundary requires stencil_order 2");
         return nullptr;
     }
-    if (o->storage < 0 || o->storage > 2) {
+    if (o->storage < 0 || o->storage > 3) {
         afdtd_set_error("unknown storage type");
         return nullptr;
     }
@@ -305,6 +313,7 @@ AcousticFDTDState* create_impl(int Nx, int Ny, int Nz, float dx, float c, float 
     st->n_pml = n_pml;
     st->storage = o->storage;
     st->esz = storage_bytes(o->storage);
+    st->fbytes = field_bytes(o->storage, Nx, Ny, Nz);
     st->hetero = (o->rho != nullptr) || (o->bulk != nullptr);
     st->dx = dx; st->c = c; st->rho_air = rho_air;
 
@@ -410,12 +419,12 @@ AcousticFDTDState* create_impl(int Nx, int Ny, int Nz, float dx, float c, float 
     TRY(hipEventCreate(&st->ev0));
     TRY(hipEventCreate(&st->ev1));
     for (int b = 0; b < 2; ++b) {
-        TRY(hipMalloc(&st->P[b], (size_t)N * st->esz));
-        TRY(hipMemsetAsync(st->P[b], 0, (size_t)N * st->esz, st->stream));
+        TRY(hipMalloc(&st->P[b], st->fbytes));
+        TRY(hipMemsetAsync(st->P[b], 0, st->fbytes, st->stream));
     }
     for (int a = 0; a < 3; ++a) {
-        TRY(hipMalloc(&st->V[a], (size_t)N * st->esz));
-        TRY(hipMemsetAsync(st->V[a], 0, (size_t)N * st->esz, st->stream));
+        TRY(hipMalloc(&st->V[a], st->fbytes));
+        TRY(hipMemsetAsync(st->V[a], 0, st->fbytes, st->stream));
     }
     if (st->storage != FDTD_STORE_F32) TRY(dalloc(&st->stage, N));
     TRY(dalloc(&st->d_status, 2));
@@ -549,10 +558,16 @@ AcousticFDTDState* create_impl(int Nx, int Ny, int Nz, float dx, float c, float 
             return nullptr;
         }
     }
+    if (st->storage == FDTD_STORE_BFP16 && st->family != FDTD_KERNEL_FUSED) {
+        afdtd_set_error("block floating-point storage runs on the single-step fused sweep "
+                        "only (no walls, plate cells, face fractions or active plate)");
+        destroy_state(st);
+        return nullptr;
+    }
     if (st->family == FDTD_KERNEL_FUSED || st->family == FDTD_KERNEL_FUSED2) {
         for (int a = 0; a < 3; ++a) {
-            TRY(hipMalloc(&st->V2[a], (size_t)N * st->esz));
-            TRY(hipMemsetAsync(st->V2[a], 0, (size_t)N * st->esz, st->stream));
+            TRY(hipMalloc(&st->V2[a], st->fbytes));
+            TRY(hipMemsetAsync(st->V2[a], 0, st->fbytes, st->stream));
         }
         TRY(hipStreamSynchronize(st->stream));
     }
@@ -807,6 +822,11 @@ SK_API int fdtd_inject_plate_force(AcousticFDTDState* st, int n, const int* pidx
 SK_API int fdtd_set_face_fractions(AcousticFDTDState* st, const float* fx, const float* fy,
                                    const float* fz) {
     if (!st) return FDTD_ERR_NULL;
+    if (st->storage == FDTD_STORE_BFP16 && (fx || fy || fz)) {
+        afdtd_set_error("face fractions need the generic sweep, which has no block "
+                        "floating-point storage");
+        return FDTD_ERR_ARG;
+    }
     int rc = alloc_fracs(st);
     if (rc) return rc;
     rc = upload_fracs(st, fx, fy, fz);
@@ -907,9 +927,9 @@ SK_API int fdtd_get_kernel_family(const AcousticFDTDState* st) { return st ? st-
 SK_API int fdtd_reset(AcousticFDTDState* st) {
     if (!st) return FDTD_ERR_NULL;                                      // :1297-1310
     for (int b = 0; b < 2; ++b)
-        AFDTD_CHECK(hipMemsetAsync(st->P[b], 0, (size_t)st->N * st->esz, st->stream));
+        AFDTD_CHECK(hipMemsetAsync(st->P[b], 0, st->fbytes, st->stream));
     for (int a = 0; a < 3; ++a)
-        AFDTD_CHECK(hipMemsetAsync(st->V[a], 0, (size_t)st->N * st->esz, st->stream));
+        AFDTD_CHECK(hipMemsetAsync(st->V[a], 0, st->fbytes, st->stream));
     AFDTD_CHECK(hipMemsetAsync(st->d_w[0], 0, sizeof(float) * st->Nplate, st->stream));
     AFDTD_CHECK(hipMemsetAsync(st->d_w[1], 0, sizeof(float) * st->Nplate, st->stream));
     AFDTD_CHECK(hipMemsetAsync(st->d_ext, 0, sizeof(float) * st->Nplate, st->stream));
@@ -1128,7 +1148,8 @@ SK_API int fdtd_set_state(AcousticFDTDState* st, const float* P, const float* vx
     float* tmp = nullptr;
     AFDTD_CHECK(dalloc(&tmp, st->N));
     auto put = [&](void* dst, float scale) -> int {
-        AFDTD_CHECK(launch_from_f32(tmp, dst, st->storage, scale, (size_t)st->N, st->stream));
+        AFDTD_CHECK(launch_from_f32(tmp, dst, st->storage, scale, (size_t)st->N, st->stream,
+                                    st->Nz));
         return FDTD_OK;
     };
     int rc = FDTD_OK;

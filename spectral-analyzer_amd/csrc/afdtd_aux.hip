@@ -65,13 +65,37 @@ __global__ void __launch_bounds__(256)
 k_post_step(T* __restrict__ P, int npsrc, const int* __restrict__ pidx,
             const float* __restrict__ psig, int plen, int nrec,
             const int* __restrict__ ridx, const float* __restrict__ rwgt,
-            float* __restrict__ trace, int trace_row, int step_index, const int* status) {
+            float* __restrict__ trace, int trace_row, int step_index, const int* status,
+            FieldGeo g) {
     typedef Store<T> S;
     if (status[0]) return;
-    if (threadIdx.x == 0 && step_index < plen) {
-        for (int s = 0; s < npsrc; ++s) {
-            int c = pidx[s];
-            S::st(P, c, S::ld(P, c) + psig[(size_t)s * plen + step_index]);
+    if constexpr (IsBlock<T>::value) {
+        // Block storage: the first wavefront re-encodes the 64-cell row of
+        // each source cell after the addition, one source after another.
+        if (threadIdx.x < 64 && step_index < plen) {
+            const int lane = threadIdx.x;
+            int16_t* q = (int16_t*)P;
+            int8_t* E = bfp_exps((void*)P, g.N);
+            for (int s = 0; s < npsrc; ++s) {
+                const int c = pidx[s];
+                const int line = c / g.Nz, kk = c % g.Nz, seg = kk / 64;
+                const int k = seg * 64 + lane;
+                const bool valid = k < g.Nz;
+                const int cell = line * g.Nz + k, row = line * g.nrz + seg;
+                float v = valid ? bfp_dec(q[cell], E[row]) : 0.0f;
+                if (k == kk) v = v + psig[(size_t)s * plen + step_index];
+                const int e = bfp_exp(wave_absmax(v));
+                if (valid) q[cell] = bfp_enc(v, e);
+                if (lane == 0) E[row] = (int8_t)e;
+                __threadfence();      // the next source may re-read this row
+            }
+        }
+    } else {
+        if (threadIdx.x == 0 && step_index < plen) {
+            for (int s = 0; s < npsrc; ++s) {
+                int c = pidx[s];
+                S::st(P, c, S::ld(P, c) + psig[(size_t)s * plen + step_index]);
+            }
         }
     }
     __threadfence_block();
@@ -80,9 +104,9 @@ k_post_step(T* __restrict__ P, int npsrc, const int* __restrict__ pidx,
     for (int r = threadIdx.x; r < nrec; r += blockDim.x) {
         const int* id = ridx + 8 * r;
         const float* w = rwgt + 8 * r;
-        float v = w[0] * S::ld(P, id[0]) + w[1] * S::ld(P, id[1]) + w[2] * S::ld(P, id[2])
-                + w[3] * S::ld(P, id[3]) + w[4] * S::ld(P, id[4]) + w[5] * S::ld(P, id[5])
-                + w[6] * S::ld(P, id[6]) + w[7] * S::ld(P, id[7]);
+        float v = w[0] * fld(P, id[0], g) + w[1] * fld(P, id[1], g) + w[2] * fld(P, id[2], g)
+                + w[3] * fld(P, id[3], g) + w[4] * fld(P, id[4], g) + w[5] * fld(P, id[5], g)
+                + w[6] * fld(P, id[6], g) + w[7] * fld(P, id[7], g);
         trace[(size_t)trace_row * nrec + r] = v;
     }
 }
@@ -91,13 +115,13 @@ k_post_step(T* __restrict__ P, int npsrc, const int* __restrict__ pidx,
 // start+1013, ...; non-finite or |p| > 2e3 Pa fails the step.
 template <typename T>
 __global__ void __launch_bounds__(256)
-k_stability(const T* __restrict__ P, int N, int start, int* status, int step_index) {
-    typedef Store<T> S;
+k_stability(const T* __restrict__ P, int N, int start, int* status, int step_index,
+            FieldGeo g) {
     if (status[0]) return;
     long m = (long)blockIdx.x * blockDim.x + threadIdx.x;
     long q = start + m * 1013;
     if (q >= N) return;
-    float p = S::ld(P, (int)q);
+    float p = fld(P, (int)q, g);
     if (!isfinite(p) || p > 2e3f || p < -2e3f) {
         if (atomicCAS(&status[0], 0, 3) == 0) status[1] = step_index;
     }
@@ -127,28 +151,45 @@ __global__ void k_accumulate(float* ext, const int* idx, const float* val, int n
 template <typename T>
 __global__ void k_gather(const T* __restrict__ F, const int* __restrict__ idx,
                          const float* __restrict__ wgt, int n, int taps, int zero_start,
-                         float scale, float* __restrict__ out) {
+                         float scale, float* __restrict__ out, FieldGeo g) {
     typedef Store<T> S;
     int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n) return;
     const int* id = idx + taps * r;
     const float* w = wgt + taps * r;
+    auto ld = [&](int c) { return S::exact ? fld(F, c, g) : fld(F, c, g) * scale; };
     float v;
     if (zero_start) {
         v = 0.0f;
-        for (int t = 0; t < taps; ++t) v += w[t] * (S::exact ? S::ld(F, id[t]) : S::ld(F, id[t]) * scale);
+        for (int t = 0; t < taps; ++t) v += w[t] * ld(id[t]);
     } else {
-        v = w[0] * (S::exact ? S::ld(F, id[0]) : S::ld(F, id[0]) * scale);
-        for (int t = 1; t < taps; ++t) v = v + w[t] * (S::exact ? S::ld(F, id[t]) : S::ld(F, id[t]) * scale);
+        v = w[0] * ld(id[0]);
+        for (int t = 1; t < taps; ++t) v = v + w[t] * ld(id[t]);
     }
     out[r] = v;
 }
 
 template <typename T>
-__global__ void k_to_f32(const T* __restrict__ src, float* __restrict__ dst, size_t n, float scale) {
-    typedef Store<T> S;
+__global__ void k_to_f32(const T* __restrict__ src, float* __restrict__ dst, size_t n, float scale,
+                         FieldGeo g) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) dst[i] = S::ld(src, (int)i) * scale;
+    if (i < n) dst[i] = fld(src, (int)i, g) * scale;
+}
+
+// FP32 field -> block storage, one wavefront per 64-cell row segment.
+__global__ void __launch_bounds__(256)
+k_from_f32_bfp(const float* __restrict__ src, bfp16_t* __restrict__ dst, int nrows, FieldGeo g) {
+    const int lane = threadIdx.x & 63;
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (row >= nrows) return;                   // whole wavefront leaves together
+    const int line = row / g.nrz, seg = row - line * g.nrz;
+    const int k = seg * 64 + lane;
+    const bool valid = k < g.Nz;
+    const int cell = line * g.Nz + k;
+    const float v = valid ? src[cell] : 0.0f;
+    const int e = bfp_exp(wave_absmax(v));
+    if (valid) ((int16_t*)dst)[cell] = bfp_enc(v, e);
+    if (lane == 0) bfp_exps((void*)dst, g.N)[row] = (int8_t)e;
 }
 
 template <typename T>
@@ -163,7 +204,7 @@ __global__ void k_from_f32(const float* __restrict__ src, T* __restrict__ dst, s
 template <typename T>
 __global__ void __launch_bounds__(256)
 k_energy(StencilParams p, StencilAux a, const T* __restrict__ P, const T* __restrict__ Vx,
-         const T* __restrict__ Vy, const T* __restrict__ Vz, double dV, double* out) {
+         const T* __restrict__ Vy, const T* __restrict__ Vz, double dV, double* out, FieldGeo g) {
     typedef Store<T> S;
     __shared__ double red[2][256];
     const int tid = threadIdx.x;
@@ -173,7 +214,7 @@ k_energy(StencilParams p, StencilAux a, const T* __restrict__ P, const T* __rest
         int k = c % p.Nz, t = c / p.Nz, j = t % p.Ny, i = t / p.Ny;
         if (!(p.has_solid && is_solid(a.ct[c]))) {
             double K = p.hetero ? (double)a.kap[c] / ((double)p.dt / (double)p.dx) : (double)p.K0;
-            double pc = (double)S::ld(P, c);
+            double pc = (double)fld(P, c, g);
             ep += pc * pc / (2.0 * K);
         }
         double vs = S::exact ? 1.0 : (double)p.inv_sV;
@@ -182,15 +223,15 @@ k_energy(StencilParams p, StencilAux a, const T* __restrict__ P, const T* __rest
             return p.hetero ? 2.0 / ((double)a.bu[L] + (double)a.bu[R]) : (double)p.rho_air;
         };
         if (i < p.Nx - 1) {
-            double v = (double)S::ld(Vx, c) * vs;
+            double v = (double)fld(Vx, c, g) * vs;
             ek += 0.5 * rf(c, c + p.Ny * p.Nz) * v * v;
         }
         if (j < p.Ny - 1) {
-            double v = (double)S::ld(Vy, c) * vs;
+            double v = (double)fld(Vy, c, g) * vs;
             ek += 0.5 * rf(c, c + p.Nz) * v * v;
         }
         if (k < p.Nz - 1) {
-            double v = (double)S::ld(Vz, c) * vs;
+            double v = (double)fld(Vz, c, g) * vs;
             ek += 0.5 * rf(c, c + 1) * v * v;
         }
     }
@@ -232,13 +273,15 @@ hipError_t launch_post_step(AcousticFDTDState* st, int step_index, int trace_row
     if (st->npsrc == 0 && (st->nrec == 0 || trace_row < 0)) return hipSuccess;
     float* tr = (st->nrec > 0 && trace_row >= 0) ? st->d_trace : nullptr;
     void* P = st->P[st->cur];
+    const FieldGeo g = field_geo(st->N, st->Nz);
 #define POST(T)                                                                         \
     hipLaunchKernelGGL(k_post_step<T>, dim3(1), dim3(256), 0, st->stream, (T*)P,        \
                        st->npsrc, st->d_pidx, st->d_psig, st->plen, st->nrec, st->d_ridx, \
-                       st->d_rwgt, tr, trace_row, step_index, st->d_status)
+                       st->d_rwgt, tr, trace_row, step_index, st->d_status, g)
     switch (st->storage) {
         case FDTD_STORE_BF16: POST(bf16_t); break;
         case FDTD_STORE_F16: POST(f16_t); break;
+        case FDTD_STORE_BFP16: POST(bfp16_t); break;
         default: POST(float);
     }
 #undef POST
@@ -249,12 +292,14 @@ hipError_t launch_stability_check(AcousticFDTDState* st, int start, int step_ind
     long cnt = (st->N > start) ? ((long)(st->N - start) + 1012) / 1013 : 0;
     if (cnt <= 0) return hipSuccess;
     const void* P = st->P[st->cur];
+    const FieldGeo g = field_geo(st->N, st->Nz);
 #define CHK(T)                                                                          \
     hipLaunchKernelGGL(k_stability<T>, dim3(blocks_for(cnt, 256)), dim3(256), 0, st->stream, \
-                       (const T*)P, st->N, start, st->d_status, step_index)
+                       (const T*)P, st->N, start, st->d_status, step_index, g)
     switch (st->storage) {
         case FDTD_STORE_BF16: CHK(bf16_t); break;
         case FDTD_STORE_F16: CHK(f16_t); break;
+        case FDTD_STORE_BFP16: CHK(bfp16_t); break;
         default: CHK(float);
     }
 #undef CHK
@@ -277,14 +322,16 @@ hipError_t launch_accumulate(float* ext, const int* idx, const float* val, int n
 
 hipError_t launch_gather(const void* F, int storage, float scale, const int* idx,
                           const float* wgt, int n, int taps, int zero_start, float* out,
-                          hipStream_t s) {
+                          hipStream_t s, int Ncells, int Nz) {
     if (n <= 0) return hipSuccess;
+    const FieldGeo g = field_geo(Ncells, Nz);
 #define G(T)                                                                              \
     hipLaunchKernelGGL(k_gather<T>, dim3(blocks_for(n, 256)), dim3(256), 0, s, (const T*)F, \
-                       idx, wgt, n, taps, zero_start, scale, out)
+                       idx, wgt, n, taps, zero_start, scale, out, g)
     switch (storage) {
         case FDTD_STORE_BF16: G(bf16_t); break;
         case FDTD_STORE_F16: G(f16_t); break;
+        case FDTD_STORE_BFP16: G(bfp16_t); break;
         default: G(float);
     }
 #undef G
@@ -292,14 +339,16 @@ hipError_t launch_gather(const void* F, int storage, float scale, const int* idx
 }
 
 hipError_t launch_to_f32(const void* src, int storage, float scale, float* dst, size_t n,
-                           hipStream_t s) {
+                           hipStream_t s, int Nz) {
     if (n == 0) return hipSuccess;
+    const FieldGeo g = field_geo((int)n, Nz);
 #define C(T)                                                                               \
     hipLaunchKernelGGL(k_to_f32<T>, dim3(blocks_for(n, 256)), dim3(256), 0, s, (const T*)src, \
-                       dst, n, scale)
+                       dst, n, scale, g)
     switch (storage) {
         case FDTD_STORE_BF16: C(bf16_t); break;
         case FDTD_STORE_F16: C(f16_t); break;
+        case FDTD_STORE_BFP16: C(bfp16_t); break;
         default: C(float);
     }
 #undef C
@@ -307,8 +356,15 @@ hipError_t launch_to_f32(const void* src, int storage, float scale, float* dst, 
 }
 
 hipError_t launch_from_f32(const float* src, void* dst, int storage, float scale, size_t n,
-                             hipStream_t s) {
+                             hipStream_t s, int Nz) {
     if (n == 0) return hipSuccess;
+    if (storage == FDTD_STORE_BFP16) {
+        const FieldGeo g = field_geo((int)n, Nz);
+        const int nrows = (int)(n / Nz) * g.nrz;
+        hipLaunchKernelGGL(k_from_f32_bfp, dim3((nrows + 3) / 4), dim3(256), 0, s, src,
+                           (bfp16_t*)dst, nrows, g);
+        return hipGetLastError();
+    }
 #define C(T)                                                                                \
     hipLaunchKernelGGL(k_from_f32<T>, dim3(blocks_for(n, 256)), dim3(256), 0, s, src, (T*)dst, \
                        n, scale)
@@ -329,13 +385,15 @@ hipError_t launch_energy(AcousticFDTDState* st, double* d_out) {
     double dV = (double)st->dx * (double)st->dx * (double)st->dx;
     unsigned nb = blocks_for(st->N, 256);
     if (nb > 2048) nb = 2048;
+    const FieldGeo g = field_geo(st->N, st->Nz);
 #define E(T)                                                                                 \
     hipLaunchKernelGGL(k_energy<T>, dim3(nb), dim3(256), 0, st->stream, st->sp, a,           \
                        (const T*)st->P[st->cur], (const T*)st->V[0], (const T*)st->V[1],     \
-                       (const T*)st->V[2], dV, d_out)
+                       (const T*)st->V[2], dV, d_out, g)
     switch (st->storage) {
         case FDTD_STORE_BF16: E(bf16_t); break;
         case FDTD_STORE_F16: E(f16_t); break;
+        case FDTD_STORE_BFP16: E(bfp16_t); break;
         default: E(float);
     }
 #undef E

@@ -40,6 +40,7 @@ struct FdtdBatch {
     RoomDesc* d_rooms;
     int* d_tile_room;
     int ntiles, tile_y, seg;
+    int schedule;           // tile-table order: 0 room order, 1 largest rooms first
     float* d_alpha;
     float* d_pdamp;
     int* d_status;
@@ -203,6 +204,46 @@ int enqueue_batch_step(FdtdBatch* b) {
     return FDTD_OK;
 }
 
+// (Re)build the tile table for tile height tile_y, x-segment seg and the
+// dispatch order `schedule` (0: rooms in order; 1: largest rooms first, so
+// the long rooms start first and the tail of the launch holds small ones).
+// Each room's tiles stay contiguous (the kernel finds its tile by
+// t - tile_base).
+int build_tiles(FdtdBatch* b, int tile_y, int seg, int schedule) {
+    b->tile_y = tile_y;
+    b->seg = seg;
+    b->schedule = schedule;
+    std::vector<int> order(b->n);
+    for (int r = 0; r < b->n; ++r) order[r] = r;
+    if (schedule == 1)
+        std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+            const RoomDesc &a = b->rooms[x], &c = b->rooms[y];
+            return (long long)a.Nx * a.Ny * a.Nz > (long long)c.Nx * c.Ny * c.Nz;
+        });
+    std::vector<int> tile_room;
+    for (int r : order) {
+        RoomDesc& d = b->rooms[r];
+        d.nty = (d.Ny + tile_y - 1) / tile_y;
+        d.ntz = (d.Nz + 63) / 64;
+        d.nseg = (d.Nx + seg - 1) / seg;
+        d.tile_base = (int)tile_room.size();
+        for (int q = 0; q < d.nty * d.ntz * d.nseg; ++q) tile_room.push_back(r);
+    }
+    if (b->d_tile_room && (int)tile_room.size() != b->ntiles) {
+        dfree(b->d_tile_room);
+        b->d_tile_room = nullptr;
+    }
+    b->ntiles = (int)tile_room.size();
+    if (!b->d_tile_room) AFDTD_CHECK(hipMalloc(&b->d_tile_room, sizeof(int) * b->ntiles));
+    AFDTD_CHECK(hipMemcpyAsync(b->d_tile_room, tile_room.data(), sizeof(int) * b->ntiles,
+                               hipMemcpyHostToDevice, b->stream));
+    if (!b->d_rooms) AFDTD_CHECK(hipMalloc(&b->d_rooms, sizeof(RoomDesc) * b->n));
+    AFDTD_CHECK(hipMemcpyAsync(b->d_rooms, b->rooms.data(), sizeof(RoomDesc) * b->n,
+                               hipMemcpyHostToDevice, b->stream));
+    AFDTD_CHECK(hipStreamSynchronize(b->stream));
+    return FDTD_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -238,8 +279,12 @@ SK_API FdtdBatch* fdtd_batch_create(int n_rooms, const int* dims, float dx, floa
     b->esz = storage == FDTD_STORE_F32 ? 4 : 2;
     b->tile_y = tile_y > 0 ? tile_y : 16;
     b->seg = seg_x > 0 ? seg_x : 32;
+    if (b->tile_y != 8 && b->tile_y != 16 && b->tile_y != 32) {
+        afdtd_set_error("fdtd_batch_create: tile_y must be 8, 16 or 32");
+        delete b;
+        return nullptr;
+    }
     b->boundary = refl ? FDTD_BOUNDARY_IMPEDANCE : FDTD_BOUNDARY_RIGID_PML;
-    const int TYv = b->tile_y;
     StencilParams& sp = b->common;
     memset(&sp, 0, sizeof(sp));
     sp.order = stencil_order;
@@ -256,16 +301,11 @@ SK_API FdtdBatch* fdtd_batch_create(int n_rooms, const int* dims, float dx, floa
     sp.inv_sV = 1.0f / sp.sV;
     const float inv_dx = 1.0f / dx;
     long long off = 0;
-    std::vector<int> tile_room;
     for (int r = 0; r < n_rooms; ++r) {
         RoomDesc d;
         memset(&d, 0, sizeof(d));
         d.Nx = dims[3 * r]; d.Ny = dims[3 * r + 1]; d.Nz = dims[3 * r + 2];
         if (d.Nx <= 0 || d.Ny <= 0 || d.Nz <= 0) { fdtd_batch_destroy(b); return nullptr; }
-        d.nty = (d.Ny + TYv - 1) / TYv;
-        d.ntz = (d.Nz + 63) / 64;
-        d.nseg = (d.Nx + b->seg - 1) / b->seg;
-        d.tile_base = (int)tile_room.size();
         d.off = off;
         // Per-room dt from its Courant number; coefficients as the single-room
         // set-up (acoustic_fdtd.cpp:1089, :1176).
@@ -279,12 +319,10 @@ SK_API FdtdBatch* fdtd_batch_create(int n_rooms, const int* dims, float dx, floa
             d.gside[s] = (1.0f - R) / (1.0f + R);
             d.xterm[s] = hk * (d.gside[s] / z0);
         }
-        for (int q = 0; q < d.nty * d.ntz * d.nseg; ++q) tile_room.push_back(r);
         off += (long long)d.Nx * d.Ny * d.Nz;
         b->rooms.push_back(d);
     }
     b->total = off;
-    b->ntiles = (int)tile_room.size();
     if (off >= (1LL << 31)) {
         afdtd_set_error("batch too large for 32-bit element indices");
         fdtd_batch_destroy(b);
@@ -311,12 +349,10 @@ SK_API FdtdBatch* fdtd_batch_create(int n_rooms, const int* dims, float dx, floa
         BT(hipMemsetAsync(b->V[q], 0, bytes, b->stream));
         BT(hipMemsetAsync(b->V2[q], 0, bytes, b->stream));
     }
-    BT(hipMalloc(&b->d_rooms, sizeof(RoomDesc) * n_rooms));
-    BT(hipMemcpyAsync(b->d_rooms, b->rooms.data(), sizeof(RoomDesc) * n_rooms,
-                      hipMemcpyHostToDevice, b->stream));
-    BT(hipMalloc(&b->d_tile_room, sizeof(int) * b->ntiles));
-    BT(hipMemcpyAsync(b->d_tile_room, tile_room.data(), sizeof(int) * b->ntiles,
-                      hipMemcpyHostToDevice, b->stream));
+    if (build_tiles(b, b->tile_y, b->seg, 0) != FDTD_OK) {
+        fdtd_batch_destroy(b);
+        return nullptr;
+    }
     float one = 1.0f, zero = 0.0f;
     BT(hipMalloc(&b->d_alpha, sizeof(float)));
     BT(hipMalloc(&b->d_pdamp, sizeof(float)));
@@ -333,6 +369,63 @@ SK_API int fdtd_batch_cells(const FdtdBatch* b, long long* total, int* n_tiles) 
     if (!b) return FDTD_ERR_NULL;
     if (total) *total = b->total;
     if (n_tiles) *n_tiles = b->ntiles;
+    return FDTD_OK;
+}
+
+// Autotuner: time every (tile height, x-segment, dispatch order) candidate
+// with `trial_steps` graph-captured steps of the batch itself (zero fields,
+// sources and recording off, so the timing does not depend on the data),
+// keep the fastest and leave the batch at step 0 with zero fields.  Must be
+// called before the first step.
+SK_API int fdtd_batch_autotune(FdtdBatch* b, int trial_steps, int* best_tile_y, int* best_seg,
+                               int* best_schedule, float* best_ms_per_step) {
+    if (!b) return FDTD_ERR_NULL;
+    if (b->step_count != 0 || trial_steps <= 0) {
+        afdtd_set_error("fdtd_batch_autotune: call before the first step, trial_steps > 0");
+        return FDTD_ERR_ARG;
+    }
+    int* src = b->d_src;
+    float* trace = b->d_trace;
+    b->d_src = nullptr;
+    b->d_trace = nullptr;
+    const int tys[3] = {8, 16, 32};
+    const int segs[4] = {16, 32, 64, 128};
+    float best = 1e30f;
+    int bt = b->tile_y, bs = b->seg, bo = b->schedule;
+    int rc = FDTD_OK;
+    for (int it = 0; it < 3 && rc == FDTD_OK; ++it)
+        for (int is = 0; is < 4 && rc == FDTD_OK; ++is)
+            for (int o = 0; o < 2 && rc == FDTD_OK; ++o) {
+                rc = build_tiles(b, tys[it], segs[is], o);
+                if (rc) break;
+                rc = fdtd_batch_step(b, 2, 0, nullptr);               // warm-up
+                float ms = 0.0f;
+                if (!rc) rc = fdtd_batch_step(b, trial_steps, 1, &ms);
+                if (!rc && ms / trial_steps < best) {
+                    best = ms / trial_steps;
+                    bt = tys[it]; bs = segs[is]; bo = o;
+                }
+            }
+    b->d_src = src;
+    b->d_trace = trace;
+    if (rc) return rc;
+    rc = build_tiles(b, bt, bs, bo);
+    if (rc) return rc;
+    const size_t bytes = (size_t)b->total * b->esz;
+    for (int i = 0; i < 2; ++i) AFDTD_CHECK(hipMemsetAsync(b->P[i], 0, bytes, b->stream));
+    for (int q = 0; q < 3; ++q) {
+        AFDTD_CHECK(hipMemsetAsync(b->V[q], 0, bytes, b->stream));
+        AFDTD_CHECK(hipMemsetAsync(b->V2[q], 0, bytes, b->stream));
+    }
+    AFDTD_CHECK(hipMemsetAsync(b->d_status, 0, 2 * sizeof(int), b->stream));
+    AFDTD_CHECK(hipStreamSynchronize(b->stream));
+    b->cur = 0;
+    b->step_count = 0;
+    b->trace_n = 0;
+    if (best_tile_y) *best_tile_y = bt;
+    if (best_seg) *best_seg = bs;
+    if (best_schedule) *best_schedule = bo;
+    if (best_ms_per_step) *best_ms_per_step = best;
     return FDTD_OK;
 }
 
@@ -477,7 +570,7 @@ SK_API int fdtd_batch_get_pressure(const FdtdBatch* b, int room, float* out, int
     float* tmp = nullptr;
     AFDTD_CHECK(hipMalloc(&tmp, sizeof(float) * N));
     const char* base = (const char*)b->P[b->cur] + (size_t)d.off * b->esz;
-    AFDTD_CHECK(launch_to_f32(base, b->storage, 1.0f, tmp, N, b->stream));
+    AFDTD_CHECK(launch_to_f32(base, b->storage, 1.0f, tmp, N, b->stream, 1));
     AFDTD_CHECK(hipStreamSynchronize(b->stream));
     AFDTD_CHECK(hipMemcpy(out, tmp, sizeof(float) * N, hipMemcpyDeviceToHost));
     dfree(tmp);

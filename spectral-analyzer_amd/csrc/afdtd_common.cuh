@@ -60,6 +60,69 @@ template <> struct Store<f16_t> {
     static constexpr bool exact = false;
 };
 
+// Block floating point (bfp16): a field of N cells is N int16 mantissas
+// followed by one int8 exponent per 64-cell z-row segment (row index
+// (i*Ny + j)*nrz + k/64, nrz = ceil(Nz/64)); a value is q * 2^e.  A segment is
+// written as a whole: e is the smallest exponent with max|v| * 2^-e < 2^15 and
+// q = rint(v * 2^-e), so every value carries an absolute error of at most
+// 2^-16 of its segment's maximum.
+struct bfp16_t {
+    int16_t q;
+};
+template <> struct Store<bfp16_t> {
+    typedef int16_t Raw;
+    static __device__ __forceinline__ Raw ldr(const bfp16_t* p, int i) {
+        return ((const int16_t*)p)[i];
+    }
+    static constexpr bool exact = false;
+};
+template <typename T> struct IsBlock { static constexpr bool value = false; };
+template <> struct IsBlock<bfp16_t> { static constexpr bool value = true; };
+
+__device__ __forceinline__ float bfp_dec(int16_t q, int e) { return ldexpf((float)q, e); }
+// exponent of a segment whose largest magnitude is m (m >= 0, or NaN / Inf,
+// which give the largest exponent so that the stability check sees them)
+__device__ __forceinline__ int bfp_exp(float m) {
+    const int biased = (int)((__float_as_uint(m) >> 23) & 0xFF);
+    return max(biased - 141, -128);
+}
+__device__ __forceinline__ int16_t bfp_enc(float v, int e) {
+    const float x = rintf(ldexpf(v, -e));
+    return (int16_t)fminf(fmaxf(x, -32767.0f), 32767.0f);
+}
+// Largest |v| over the 64 lanes of the wavefront (all lanes must be active).
+__device__ __forceinline__ float wave_absmax(float v) {
+    unsigned m = __float_as_uint(v) & 0x7FFFFFFFu;
+#pragma unroll
+    for (int s = 32; s >= 1; s >>= 1) {
+        const unsigned o = (unsigned)__shfl_xor((int)m, s, 64);
+        m = o > m ? o : m;
+    }
+    return __uint_as_float(m);
+}
+__device__ __forceinline__ const int8_t* bfp_exps(const void* F, int N) {
+    return (const int8_t*)((const int16_t*)F + N);
+}
+__device__ __forceinline__ int8_t* bfp_exps(void* F, int N) { return (int8_t*)((int16_t*)F + N); }
+// element c of a bfp16 field (aux kernels; rows are found from c)
+__device__ __forceinline__ float bfp_ld(const void* F, int c, int N, int Nz, int nrz) {
+    const int row = (c / Nz) * nrz + (c % Nz) / 64;
+    return bfp_dec(((const int16_t*)F)[c], bfp_exps(F, N)[row]);
+}
+
+// Cell geometry for element access to block-floating-point fields.
+struct FieldGeo {
+    int N, Nz, nrz;
+};
+inline FieldGeo field_geo(int N, int Nz) { return FieldGeo{N, Nz, (Nz + 63) / 64}; }
+// element c of a field of any storage type (FP32 value)
+template <typename T>
+__device__ __forceinline__ float fld(const T* F, int c, FieldGeo) { return Store<T>::ld(F, c); }
+template <>
+__device__ __forceinline__ float fld<bfp16_t>(const bfp16_t* F, int c, FieldGeo g) {
+    return bfp_ld(F, c, g.N, g.Nz, g.nrz);
+}
+
 __device__ __forceinline__ int is_solid(uint8_t c) { return c == 1 || c == 2; }
 
 // PML depth of one axis position (acoustic_fdtd.cpp:381-383).

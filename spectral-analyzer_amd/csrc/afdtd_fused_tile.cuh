@@ -110,9 +110,14 @@ __device__ __forceinline__ void fused_tile(const StencilParams& p, const FusedPt
         for (int d = tid; d <= npml && d < 65; d += NT) { sAlpha[d] = alpha_tab[d]; sPd[d] = pdamp_tab[d]; }
     const float C1 = p.C1, C2 = p.C2, vcoef = p.vel_coeff, dtdx = p.dtdx, dt = p.dt;
     const float isv = p.inv_sV, sv = p.sV;
+    // block floating point: per-row exponents after the N mantissas of each
+    // field; exponent index (plane*Ny + j)*nrz + k/64
+    constexpr bool BLK = IsBlock<T>::value;
+    const int NC = Nx * Ny * Nz, nrz = (Nz + 63) >> 6, esx = Ny * nrz;
 
     // ── per-thread geometry, computed once ──
     int ownOff[R];          // in-plane offset of own cell (row w*R+r, col lane)
+    int ownE[R];            // in-plane exponent index of the own row (block storage)
     bool ownV[R];           // own cell inside the room
     bool vyV[R], vzV[R];    // own Vy / Vz face exists (j <= Ny-2 / k <= Nz-2)
     bool vyIn[R];           // order-4 taps for the own Vy face
@@ -121,13 +126,14 @@ __device__ __forceinline__ void fused_tile(const StencilParams& p, const FusedPt
         const int j = j0 + w * R + r;
         ownOff[r] = j * Nz + k;
         ownV[r] = kval && j < Ny;
+        ownE[r] = j < Ny ? j * nrz + tz : 0;
         vyV[r] = kval && j <= Ny - 2;
         vzV[r] = j < Ny && k <= Nz - 2;
         vyIn[r] = j > 0 && j < Ny - 2;
     }
     const bool vzIn = k > 0 && k < Nz - 2;
     // halo rows of the P plane (cols PH .. PH+63)
-    int hrOff[NHRW], hrLds[NHRW];
+    int hrOff[NHRW], hrLds[NHRW], hrE[NHRW];
     bool hrV[NHRW];
 #pragma unroll
     for (int m = 0; m < NHRW; ++m) {
@@ -137,10 +143,11 @@ __device__ __forceinline__ void fused_tile(const StencilParams& p, const FusedPt
         hrLds[m] = row * PC + PH + lane;
         hrOff[m] = gj * Nz + k;
         hrV[m] = h < HR && gj >= 0 && gj < Ny && kval;
+        hrE[m] = hrV[m] ? gj * nrz + tz : 0;
         if (h >= HR) hrLds[m] = -1;
     }
     // halo columns of the P plane (all PR rows)
-    int hcOff[NHC], hcLds[NHC];
+    int hcOff[NHC], hcLds[NHC], hcE[NHC];
     bool hcV[NHC];
 #pragma unroll
     for (int m = 0; m < NHC; ++m) {
@@ -151,6 +158,7 @@ __device__ __forceinline__ void fused_tile(const StencilParams& p, const FusedPt
         hcLds[m] = e < HC ? row * PC + col : -1;
         hcOff[m] = gj * Nz + gk;
         hcV[m] = e < HC && gj >= 0 && gj < Ny && gk >= 0 && gk < Nz;
+        hcE[m] = hcV[m] ? gj * nrz + (gk >> 6) : 0;
     }
     // halo Vy face row handled by this wave (task A)
     const bool tA = w < NHY;
@@ -159,6 +167,7 @@ __device__ __forceinline__ void fused_tile(const StencilParams& p, const FusedPt
     const bool vA = tA && kval && jA >= 0 && jA <= Ny - 2;
     const bool inA = jA > 0 && jA < Ny - 2;
     const int offA = jA * Nz + k;
+    const int eA = vA ? jA * nrz + tz : 0;
     // halo Vz face handled by this thread (task B)
     const int uB = tid - (NT - NHZ);
     const bool tB = uB >= 0;
@@ -169,6 +178,7 @@ __device__ __forceinline__ void fused_tile(const StencilParams& p, const FusedPt
     const bool vB = tB && jB < Ny && kB >= 0 && kB <= Nz - 2;
     const bool inB = kB > 0 && kB < Nz - 2;
     const int offB = jB * Nz + kB;
+    const int eB = vB ? jB * nrz + (kB >> 6) : 0;
     // PML depths (used only when npml > 0)
     int dJ[R], dJ1[R];
     const int dK = pml_depth(k, Nz, npml), dK1 = pml_depth(k + 1, Nz, npml);
@@ -204,12 +214,14 @@ __device__ __forceinline__ void fused_tile(const StencilParams& p, const FusedPt
     typedef typename S::Raw RawT;
     RawT stP[R + NHRW + NHC];
     float stB[HET ? R + NHRW + NHC : 1];
+    int stE[R + NHRW + NHC];                // row exponents (block storage only)
     // Old velocities / K / wall sums of one plane.  Two sets alternate
     // between even and odd planes (the loop is unrolled by two), so no
     // register copy at the loop latch forces an early wait on the loads.
     struct Old {
         RawT x[R], y[R], z[R], a, b;
         float k[HET ? R : 1], xi[HET ? R : 1];
+        int ex[R], ey[R], ez[R], ea, eb;        // row exponents (block storage only)
     };
     Old oE, oO;
     float vq[NQ][R];
@@ -220,7 +232,33 @@ __device__ __forceinline__ void fused_tile(const StencilParams& p, const FusedPt
 
     auto ldr = [&](const T* __restrict__ F, bool v, int off) { return S::ldr(F, v ? off : 0); };
     auto ldf = [&](const float* __restrict__ F, bool v, int off) { return F[v ? off : 0]; };
-    auto vel = [&](RawT x) { const float f = S::cvt(x); return S::exact ? f : f * isv; };
+    // row exponent of field F at plane pl (block storage; 0 otherwise)
+    auto lde = [&](const T* __restrict__ F, int pl, bool v, int eoff) -> int {
+        if constexpr (BLK) return bfp_exps(F, NC)[v ? pl * esx + eoff : 0];
+        else return 0;
+    };
+    auto dec = [&](RawT x, int e) -> float {
+        if constexpr (BLK) return bfp_dec(x, e);
+        else return S::cvt(x);
+    };
+    auto vel = [&](RawT x, int e) {
+        const float f = dec(x, e);
+        return (S::exact || BLK) ? f : f * isv;
+    };
+    // store the own row r of a field at plane pl (velocities carry the
+    // storage scale sv, pressures none); with block storage every lane of the
+    // wavefront takes part in the row's exponent (the caller's condition is
+    // wave-uniform), lane 0 writes it
+    auto put_row = [&](T* __restrict__ F, int pl, int r, float v, bool valid, bool isV) {
+        if constexpr (BLK) {
+            const int e = bfp_exp(wave_absmax(valid ? v : 0.0f));
+            if (valid) __builtin_nontemporal_store(bfp_enc(v, e), (int16_t*)F + pl * sx + ownOff[r]);
+            if (lane == 0 && j0 + w * R + r < Ny)
+                bfp_exps((void*)F, NC)[pl * esx + ownE[r]] = (int8_t)e;
+        } else {
+            if (valid) S::st_nt(F, pl * sx + ownOff[r], (S::exact || !isV) ? v : v * sv);
+        }
+    };
     auto load_plane = [&](int pl) {
         const bool pv = pl >= 0 && pl < Nx;
         const int base = pl * sx;
@@ -229,18 +267,21 @@ __device__ __forceinline__ void fused_tile(const StencilParams& p, const FusedPt
             const bool v = pv && ownV[r];
             stP[r] = ldr(Pin, v, base + ownOff[r]);
             if (HET) stB[r] = ldf(fp.bu, v, base + ownOff[r]);
+            if (BLK) stE[r] = lde(Pin, pl, v, ownE[r]);
         }
 #pragma unroll
         for (int m = 0; m < NHRW; ++m) {
             const bool v = pv && hrV[m];
             stP[R + m] = ldr(Pin, v, base + hrOff[m]);
             if (HET) stB[R + m] = ldf(fp.bu, v, base + hrOff[m]);
+            if (BLK) stE[R + m] = lde(Pin, pl, v, hrE[m]);
         }
 #pragma unroll
         for (int m = 0; m < NHC; ++m) {
             const bool v = pv && hcV[m];
             stP[R + NHRW + m] = ldr(Pin, v, base + hcOff[m]);
             if (HET) stB[R + NHRW + m] = ldf(fp.bu, v, base + hcOff[m]);
+            if (BLK) stE[R + NHRW + m] = lde(Pin, pl, v, hcE[m]);
         }
     };
     auto store_plane = [&](int slot) {
@@ -248,19 +289,19 @@ __device__ __forceinline__ void fused_tile(const StencilParams& p, const FusedPt
         float* dB = sB + (HET ? slot * PLANE : 0);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            dP[ownLds + r * PC] = S::cvt(stP[r]);
+            dP[ownLds + r * PC] = dec(stP[r], BLK ? stE[r] : 0);
             if (HET) dB[ownLds + r * PC] = stB[r];
         }
 #pragma unroll
         for (int m = 0; m < NHRW; ++m)
             if (hrLds[m] >= 0) {
-                dP[hrLds[m]] = S::cvt(stP[R + m]);
+                dP[hrLds[m]] = dec(stP[R + m], BLK ? stE[R + m] : 0);
                 if (HET) dB[hrLds[m]] = stB[R + m];
             }
 #pragma unroll
         for (int m = 0; m < NHC; ++m)
             if (hcLds[m] >= 0) {
-                dP[hcLds[m]] = S::cvt(stP[R + NHRW + m]);
+                dP[hcLds[m]] = dec(stP[R + NHRW + m], BLK ? stE[R + NHRW + m] : 0);
                 if (HET) dB[hcLds[m]] = stB[R + NHRW + m];
             }
     };
@@ -275,12 +316,21 @@ __device__ __forceinline__ void fused_tile(const StencilParams& p, const FusedPt
             o.x[r] = ldr(Vxi, fv && ownV[r], bf + ownOff[r]);
             o.y[r] = ldr(Vyi, qv && vyV[r], bq + ownOff[r]);
             o.z[r] = ldr(Vzi, qv && vzV[r], bq + ownOff[r]);
+            if (BLK) {
+                o.ex[r] = lde(Vxi, f, fv && ownV[r], ownE[r]);
+                o.ey[r] = lde(Vyi, q, qv && vyV[r], ownE[r]);
+                o.ez[r] = lde(Vzi, q, qv && vzV[r], ownE[r]);
+            }
             if (HET) o.k[r] = ldf(fp.kap, qv && ownV[r], bq + ownOff[r]);
             if (HET && loadXi && qv && (xe || rowEdge[r]))
                 o.xi[r] = ldf(fp.xi, ownV[r] && (xe || edgeYZ[r]), bq + ownOff[r]);
         }
         o.a = ldr(Vyi, qv && vA, bq + offA);
         o.b = ldr(Vzi, qv && vB, bq + offB);
+        if (BLK) {
+            o.ea = lde(Vyi, q, qv && vA, eA);
+            o.eb = lde(Vzi, q, qv && vB, eB);
+        }
     };
     auto slot_of = [](int pl) { return ((pl % NS) + NS) % NS; };
     auto slot_add = [](int s0, int d) { const int t = s0 + d; return t >= NS ? t - NS : t; };
@@ -328,11 +378,11 @@ __device__ __forceinline__ void fused_tile(const StencilParams& p, const FusedPt
                     const float* b1 = sB + sf1 * PLANE;
                     coef = (0.5f * (b0[a] + b1[a])) * dtdx;
                 }
-                float nv = face_new<ORDER>(vel(o.x[r]), ORDER == 4 ? sm[a] : 0.0f, s0[a], s1[a],
+                float nv = face_new<ORDER>(vel(o.x[r], BLK ? o.ex[r] : 0), ORDER == 4 ? sm[a] : 0.0f, s0[a], s1[a],
                                            ORDER == 4 ? s2[a] : 0.0f, inner, coef, C1, C2);
                 if (npml > 0) nv = nv * damp(imax3(di0, dJ[r], dK), imax3(di1, dJ[r], dK));
                 nv = val ? nv : 0.0f;
-                if (own && val) S::st_nt(Vxo, f * sx + ownOff[r], S::exact ? nv : nv * sv);
+                if (own) put_row(Vxo, f, r, nv, val, true);
 #pragma unroll
                 for (int s = 0; s < NQ - 1; ++s) vq[s][r] = vq[s + 1][r];
                 vq[NQ - 1][r] = nv;
@@ -342,26 +392,25 @@ __device__ __forceinline__ void fused_tile(const StencilParams& p, const FusedPt
         if (q >= i0) {
             const float* sq = sP + sq0 * PLANE;
             const float* bq = sB + (HET ? sq0 * PLANE : 0);
-            const int base = q * sx;
             const int dI = pml_depth(q, Nx, npml);
             // ── Vy faces of own rows (face j between rows j and j+1) ──
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 const int a = ownLds + r * PC;
                 const float coef = HET ? (0.5f * (bq[a] + bq[a + PC])) * dtdx : vcoef;
-                float nv = face_new<ORDER>(vel(o.y[r]), ORDER == 4 ? sq[a - PC] : 0.0f, sq[a],
+                float nv = face_new<ORDER>(vel(o.y[r], BLK ? o.ey[r] : 0), ORDER == 4 ? sq[a - PC] : 0.0f, sq[a],
                                            sq[a + PC], ORDER == 4 ? sq[a + 2 * PC] : 0.0f,
                                            vyIn[r], coef, C1, C2);
                 if (npml > 0) nv = nv * damp(imax3(dI, dJ[r], dK), imax3(dI, dJ1[r], dK));
                 nv = vyV[r] ? nv : 0.0f;
-                if (vyV[r]) S::st_nt(Vyo, base + ownOff[r], S::exact ? nv : nv * sv);
+                put_row(Vyo, q, r, nv, vyV[r], true);
                 sVy[(FLO + w * R + r) * TZ + lane] = nv;
             }
             // ── halo Vy face row (task A) ──
             if (tA) {
                 const int a = (frA - FLO + PH) * PC + PH + lane;
                 const float coef = HET ? (0.5f * (bq[a] + bq[a + PC])) * dtdx : vcoef;
-                float nv = face_new<ORDER>(vel(o.a), ORDER == 4 ? sq[a - PC] : 0.0f, sq[a],
+                float nv = face_new<ORDER>(vel(o.a, BLK ? o.ea : 0), ORDER == 4 ? sq[a - PC] : 0.0f, sq[a],
                                            sq[a + PC], ORDER == 4 ? sq[a + 2 * PC] : 0.0f, inA,
                                            coef, C1, C2);
                 if (npml > 0) nv = nv * damp(imax3(dI, dJA, dK), imax3(dI, dJA1, dK));
@@ -372,19 +421,19 @@ __device__ __forceinline__ void fused_tile(const StencilParams& p, const FusedPt
             for (int r = 0; r < R; ++r) {
                 const int a = ownLds + r * PC;
                 const float coef = HET ? (0.5f * (bq[a] + bq[a + 1])) * dtdx : vcoef;
-                float nv = face_new<ORDER>(vel(o.z[r]), ORDER == 4 ? sq[a - 1] : 0.0f, sq[a],
+                float nv = face_new<ORDER>(vel(o.z[r], BLK ? o.ez[r] : 0), ORDER == 4 ? sq[a - 1] : 0.0f, sq[a],
                                            sq[a + 1], ORDER == 4 ? sq[a + 2] : 0.0f, vzIn, coef,
                                            C1, C2);
                 if (npml > 0) nv = nv * damp(imax3(dI, dJ[r], dK), imax3(dI, dJ[r], dK1));
                 nv = vzV[r] ? nv : 0.0f;
-                if (vzV[r]) S::st_nt(Vzo, base + ownOff[r], S::exact ? nv : nv * sv);
+                put_row(Vzo, q, r, nv, vzV[r], true);
                 sVz[(w * R + r) * FC + FLO + lane] = nv;
             }
             // ── halo Vz face (task B) ──
             if (tB) {
                 const int a = (PH + lyB) * PC + fcB - FLO + PH;
                 const float coef = HET ? (0.5f * (bq[a] + bq[a + 1])) * dtdx : vcoef;
-                float nv = face_new<ORDER>(vel(o.b), ORDER == 4 ? sq[a - 1] : 0.0f, sq[a], sq[a + 1],
+                float nv = face_new<ORDER>(vel(o.b, BLK ? o.eb : 0), ORDER == 4 ? sq[a - 1] : 0.0f, sq[a], sq[a + 1],
                                            ORDER == 4 ? sq[a + 2] : 0.0f, inB, coef, C1, C2);
                 if (npml > 0) nv = nv * damp(imax3(dI, dJB, dKB), imax3(dI, dJB, dKB1));
                 sVz[lyB * FC + fcB] = vB ? nv : 0.0f;
@@ -425,7 +474,7 @@ __device__ __forceinline__ void fused_tile(const StencilParams& p, const FusedPt
                     out = pc - kap * dv;
                 }
                 if (npml > 0) out = out * sPd[imax3(dI, dJ[r], dK)];
-                if (ownV[r]) S::st_nt(Pout, base + ownOff[r], out);
+                put_row(Pout, q, r, out, ownV[r], false);
             }
         }
     };

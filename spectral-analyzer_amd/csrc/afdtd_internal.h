@@ -73,8 +73,11 @@ struct AcousticFDTDState {
     hipStream_t stream;
     hipEvent_t ev0, ev1;
 
-    // Fields (element type per `storage`; bytes per element in esz).
+    // Fields (element type per `storage`; bytes per element in esz; bytes per
+    // field buffer in fbytes, which for block floating point includes the
+    // row exponents after the mantissas).
     int esz;
+    size_t fbytes;
     void* P[2];             // P[cur] = P^n
     int cur;
     void* V[3];             // V^{n-1/2} (current)
@@ -164,11 +167,13 @@ hipError_t launch_fill(float* p, size_t n, float v, hipStream_t s);
 hipError_t launch_accumulate(float* ext, const int* idx, const float* val, int n,
                              float scale, float inv_dx2, int div, const uint8_t* mask,
                              int nmax, const int* status, hipStream_t s);
+// Ncells / Nz / n locate the per-row exponents of block-floating-point fields
+// (ignored for the other storage types).
 hipError_t launch_gather(const void* F, int storage, float scale, const int* idx,
                          const float* wgt, int n, int taps, int zero_start, float* out,
-                         hipStream_t s);
+                         hipStream_t s, int Ncells, int Nz);
 hipError_t launch_to_f32(const void* src, int storage, float scale, float* dst, size_t n,
-                         hipStream_t s);
+                         hipStream_t s, int Nz);
 hipError_t launch_from_f32(const float* src, void* dst, int storage, float scale, size_t n,
-                           hipStream_t s);
+                           hipStream_t s, int Nz);
 hipError_t launch_energy(AcousticFDTDState* st, double* d_out);

@@ -48,3 +48,38 @@ def test_batch_rooms_match_oracle(oracle_mod, graph):
         Po = o.get_pressure_field()
         assert np.array_equal(Pg.view(np.uint32), Po.view(np.uint32)), f"room {r} field"
         assert np.abs(to).max() > 0
+
+
+def test_batch_autotune_keeps_results():
+    """The autotuner changes the tile table only: after tuning (which must
+    leave the batch at step 0 with zero fields) the traces equal those of an
+    untuned batch bit for bit; tuning after a step is refused."""
+    from afdtd import RoomBatch, differentiated_gaussian
+    n, steps = 6, 40
+    rng, dims, courant, refl = _rooms(n, 7, lo=18, hi=70)
+    src = []
+    rec = np.zeros((n, 4, 3), np.float32)
+    for r, (Nx, Ny, Nz) in enumerate(dims):
+        i, j, k = (rng.integers(3, d - 3) for d in (Nx, Ny, Nz))
+        src.append(int(k + Nz * (j + Ny * i)))
+        rec[r] = rng.uniform(2, np.array([Nx, Ny, Nz]) - 3, size=(4, 3))
+    sig = np.stack([differentiated_gaussian(steps, 5.0, 15.0, 1.0)] * n)
+    traces = []
+    for tune in (False, True):
+        b = RoomBatch(dims, 0.01, courant, refl, c=343.0, rho_air=1.2, order=2)
+        b.set_sources(src, sig)
+        if tune:
+            choice = b.autotune(trial_steps=3)
+            print("autotune choice", choice, flush=True)
+            assert choice["tile_y"] in (8, 16, 32) and choice["seg_x"] in (16, 32, 64, 128)
+            assert b.step_count == 0
+        b.set_receivers(rec)
+        b.record(steps)
+        b.step(steps)
+        traces.append(b.get_traces())
+        if tune:
+            with pytest.raises(RuntimeError):
+                b.autotune(trial_steps=2)
+        del b
+    assert np.abs(traces[0]).max() > 0
+    assert np.array_equal(traces[0].view(np.uint32), traces[1].view(np.uint32))
