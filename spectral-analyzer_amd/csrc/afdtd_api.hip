@@ -48,12 +48,13 @@ void dfree(void* p) {
 }
 
 int storage_bytes(int storage) { return storage == FDTD_STORE_F32 ? 4 : 2; }
-// bytes of one field buffer: N elements, plus one exponent byte per 64-cell
-// z-row segment for block floating point
+// bytes of one field buffer: N elements, plus (at the next 4-byte boundary)
+// one int32 exponent per 64-cell z-row segment for block floating point
 size_t field_bytes(int storage, int Nx, int Ny, int Nz) {
     const size_t N = (size_t)Nx * Ny * Nz;
     size_t b = N * storage_bytes(storage);
-    if (storage == FDTD_STORE_BFP16) b += (size_t)Nx * Ny * ((Nz + 63) / 64);
+    if (storage == FDTD_STORE_BFP16)
+        b = ((N * 2 + 3) & ~(size_t)3) + sizeof(int) * (size_t)Nx * Ny * ((Nz + 63) / 64);
     return b;
 }
 

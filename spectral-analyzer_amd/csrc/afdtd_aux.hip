@@ -75,7 +75,7 @@ k_post_step(T* __restrict__ P, int npsrc, const int* __restrict__ pidx,
         if (threadIdx.x < 64 && step_index < plen) {
             const int lane = threadIdx.x;
             int16_t* q = (int16_t*)P;
-            int8_t* E = bfp_exps((void*)P, g.N);
+            int* E = bfp_exps((void*)P, g.N);
             for (int s = 0; s < npsrc; ++s) {
                 const int c = pidx[s];
                 const int line = c / g.Nz, kk = c % g.Nz, seg = kk / 64;
@@ -86,7 +86,7 @@ k_post_step(T* __restrict__ P, int npsrc, const int* __restrict__ pidx,
                 if (k == kk) v = v + psig[(size_t)s * plen + step_index];
                 const int e = bfp_exp(wave_absmax(v));
                 if (valid) q[cell] = bfp_enc(v, e);
-                if (lane == 0) E[row] = (int8_t)e;
+                if (lane == 0) E[row] = e;
                 __threadfence();      // the next source may re-read this row
             }
         }
@@ -189,7 +189,7 @@ k_from_f32_bfp(const float* __restrict__ src, bfp16_t* __restrict__ dst, int nro
     const float v = valid ? src[cell] : 0.0f;
     const int e = bfp_exp(wave_absmax(v));
     if (valid) ((int16_t*)dst)[cell] = bfp_enc(v, e);
-    if (lane == 0) bfp_exps((void*)dst, g.N)[row] = (int8_t)e;
+    if (lane == 0) bfp_exps((void*)dst, g.N)[row] = e;
 }
 
 template <typename T>

@@ -61,11 +61,11 @@ template <> struct Store<f16_t> {
 };
 
 // Block floating point (bfp16): a field of N cells is N int16 mantissas
-// followed by one int8 exponent per 64-cell z-row segment (row index
-// (i*Ny + j)*nrz + k/64, nrz = ceil(Nz/64)); a value is q * 2^e.  A segment is
-// written as a whole: e is the smallest exponent with max|v| * 2^-e < 2^15 and
-// q = rint(v * 2^-e), so every value carries an absolute error of at most
-// 2^-16 of its segment's maximum.
+// followed (at the next 4-byte boundary) by one int32 exponent per 64-cell
+// z-row segment (row index (i*Ny + j)*nrz + k/64, nrz = ceil(Nz/64)); a value
+// is q * 2^e.  A segment is written as a whole: e is the smallest exponent
+// with max|v| * 2^-e < 2^15 and q = rint(v * 2^-e), so every value carries an
+// absolute error of at most 2^-16 of its segment's maximum.
 struct bfp16_t {
     int16_t q;
 };
@@ -79,6 +79,7 @@ template <> struct Store<bfp16_t> {
 template <typename T> struct IsBlock { static constexpr bool value = false; };
 template <> struct IsBlock<bfp16_t> { static constexpr bool value = true; };
 
+__host__ __device__ __forceinline__ size_t bfp_exp_offset(size_t N) { return (N * 2 + 3) & ~(size_t)3; }
 __device__ __forceinline__ float bfp_dec(int16_t q, int e) { return ldexpf((float)q, e); }
 // exponent of a segment whose largest magnitude is m (m >= 0, or NaN / Inf,
 // which give the largest exponent so that the stability check sees them)
@@ -90,20 +91,38 @@ __device__ __forceinline__ int16_t bfp_enc(float v, int e) {
     const float x = rintf(ldexpf(v, -e));
     return (int16_t)fminf(fmaxf(x, -32767.0f), 32767.0f);
 }
-// Largest |v| over the 64 lanes of the wavefront (all lanes must be active).
+// Largest |v| over the 64 lanes of the wavefront (all lanes must be active;
+// the result is wave-uniform).  DPP row shifts 1, 2, 4, 8 leave the maximum
+// of each 16-lane row in its lane 15 (lanes without a source keep 0, the
+// identity); the four rows are then combined through readlane.
 __device__ __forceinline__ float wave_absmax(float v) {
-    unsigned m = __float_as_uint(v) & 0x7FFFFFFFu;
-#pragma unroll
-    for (int s = 32; s >= 1; s >>= 1) {
-        const unsigned o = (unsigned)__shfl_xor((int)m, s, 64);
-        m = o > m ? o : m;
-    }
-    return __uint_as_float(m);
+    int m = (int)(__float_as_uint(v) & 0x7FFFFFFFu);      // |v| bits: order of magnitudes
+    m = max(m, __builtin_amdgcn_update_dpp(0, m, 0x111, 0xF, 0xF, false));   // row_shr:1
+    m = max(m, __builtin_amdgcn_update_dpp(0, m, 0x112, 0xF, 0xF, false));   // row_shr:2
+    m = max(m, __builtin_amdgcn_update_dpp(0, m, 0x114, 0xF, 0xF, false));   // row_shr:4
+    m = max(m, __builtin_amdgcn_update_dpp(0, m, 0x118, 0xF, 0xF, false));   // row_shr:8
+    const int r = max(max(__builtin_amdgcn_readlane(m, 15), __builtin_amdgcn_readlane(m, 31)),
+                      max(__builtin_amdgcn_readlane(m, 47), __builtin_amdgcn_readlane(m, 63)));
+    return __uint_as_float((unsigned)r);
 }
-__device__ __forceinline__ const int8_t* bfp_exps(const void* F, int N) {
-    return (const int8_t*)((const int16_t*)F + N);
+// Exponent a single value would need on its own (the segment exponent is
+// the largest of these over the segment's lanes).
+__device__ __forceinline__ int bfp_lane_exp(float v) { return bfp_exp(fabsf(v)); }
+// Segment exponent of the wavefront's 64 values, given the exponent the same
+// segment had before (e_old, wave-uniform): when no valid lane needs more
+// and one needs exactly e_old, the result is e_old (two ballots); otherwise
+// the full reduction.  Equal to bfp_exp(wave_absmax(...)) in every case.
+__device__ __forceinline__ int bfp_row_exp(float v, bool valid, int e_old) {
+    const int le = valid ? bfp_lane_exp(v) : -1000;
+    if (!__any(le > e_old) && __any(le == e_old)) return e_old;
+    return bfp_exp(wave_absmax(valid ? v : 0.0f));
 }
-__device__ __forceinline__ int8_t* bfp_exps(void* F, int N) { return (int8_t*)((int16_t*)F + N); }
+__device__ __forceinline__ const int* bfp_exps(const void* F, int N) {
+    return (const int*)((const char*)F + bfp_exp_offset((size_t)N));
+}
+__device__ __forceinline__ int* bfp_exps(void* F, int N) {
+    return (int*)((char*)F + bfp_exp_offset((size_t)N));
+}
 // element c of a bfp16 field (aux kernels; rows are found from c)
 __device__ __forceinline__ float bfp_ld(const void* F, int c, int N, int Nz, int nrz) {
     const int row = (c / Nz) * nrz + (c % Nz) / 64;

@@ -232,9 +232,16 @@ __device__ __forceinline__ void fused_tile(const StencilParams& p, const FusedPt
 
     auto ldr = [&](const T* __restrict__ F, bool v, int off) { return S::ldr(F, v ? off : 0); };
     auto ldf = [&](const float* __restrict__ F, bool v, int off) { return F[v ? off : 0]; };
-    // row exponent of field F at plane pl (block storage; 0 otherwise)
+    // row exponent of field F at plane pl (block storage; 0 otherwise); the
+    // _u form is for rows shared by the whole wavefront (lane 0 always lies
+    // inside the room), whose index is made uniform so it is a scalar load
     auto lde = [&](const T* __restrict__ F, int pl, bool v, int eoff) -> int {
         if constexpr (BLK) return bfp_exps(F, NC)[v ? pl * esx + eoff : 0];
+        else return 0;
+    };
+    auto lde_u = [&](const T* __restrict__ F, int pl, bool v, int eoff) -> int {
+        if constexpr (BLK)
+            return bfp_exps(F, NC)[__builtin_amdgcn_readfirstlane(v ? pl * esx + eoff : 0)];
         else return 0;
     };
     auto dec = [&](RawT x, int e) -> float {
@@ -249,12 +256,15 @@ __device__ __forceinline__ void fused_tile(const StencilParams& p, const FusedPt
     // storage scale sv, pressures none); with block storage every lane of the
     // wavefront takes part in the row's exponent (the caller's condition is
     // wave-uniform), lane 0 writes it
-    auto put_row = [&](T* __restrict__ F, int pl, int r, float v, bool valid, bool isV) {
+    // e_old: the segment's exponent before this step (wave-uniform), or a
+    // value no lane can match (-1000) to force the full reduction
+    auto put_row = [&](T* __restrict__ F, int pl, int r, float v, bool valid, bool isV,
+                       int e_old) {
         if constexpr (BLK) {
-            const int e = bfp_exp(wave_absmax(valid ? v : 0.0f));
+            const int e = bfp_row_exp(v, valid, e_old);
             if (valid) __builtin_nontemporal_store(bfp_enc(v, e), (int16_t*)F + pl * sx + ownOff[r]);
             if (lane == 0 && j0 + w * R + r < Ny)
-                bfp_exps((void*)F, NC)[pl * esx + ownE[r]] = (int8_t)e;
+                bfp_exps((void*)F, NC)[pl * esx + ownE[r]] = e;
         } else {
             if (valid) S::st_nt(F, pl * sx + ownOff[r], (S::exact || !isV) ? v : v * sv);
         }
@@ -267,14 +277,14 @@ __device__ __forceinline__ void fused_tile(const StencilParams& p, const FusedPt
             const bool v = pv && ownV[r];
             stP[r] = ldr(Pin, v, base + ownOff[r]);
             if (HET) stB[r] = ldf(fp.bu, v, base + ownOff[r]);
-            if (BLK) stE[r] = lde(Pin, pl, v, ownE[r]);
+            if (BLK) stE[r] = lde_u(Pin, pl, v, ownE[r]);
         }
 #pragma unroll
         for (int m = 0; m < NHRW; ++m) {
             const bool v = pv && hrV[m];
             stP[R + m] = ldr(Pin, v, base + hrOff[m]);
             if (HET) stB[R + m] = ldf(fp.bu, v, base + hrOff[m]);
-            if (BLK) stE[R + m] = lde(Pin, pl, v, hrE[m]);
+            if (BLK) stE[R + m] = lde_u(Pin, pl, v, hrE[m]);
         }
 #pragma unroll
         for (int m = 0; m < NHC; ++m) {
@@ -317,9 +327,9 @@ __device__ __forceinline__ void fused_tile(const StencilParams& p, const FusedPt
             o.y[r] = ldr(Vyi, qv && vyV[r], bq + ownOff[r]);
             o.z[r] = ldr(Vzi, qv && vzV[r], bq + ownOff[r]);
             if (BLK) {
-                o.ex[r] = lde(Vxi, f, fv && ownV[r], ownE[r]);
-                o.ey[r] = lde(Vyi, q, qv && vyV[r], ownE[r]);
-                o.ez[r] = lde(Vzi, q, qv && vzV[r], ownE[r]);
+                o.ex[r] = lde_u(Vxi, f, fv && ownV[r], ownE[r]);
+                o.ey[r] = lde_u(Vyi, q, qv && vyV[r], ownE[r]);
+                o.ez[r] = lde_u(Vzi, q, qv && vzV[r], ownE[r]);
             }
             if (HET) o.k[r] = ldf(fp.kap, qv && ownV[r], bq + ownOff[r]);
             if (HET && loadXi && qv && (xe || rowEdge[r]))
@@ -328,7 +338,7 @@ __device__ __forceinline__ void fused_tile(const StencilParams& p, const FusedPt
         o.a = ldr(Vyi, qv && vA, bq + offA);
         o.b = ldr(Vzi, qv && vB, bq + offB);
         if (BLK) {
-            o.ea = lde(Vyi, q, qv && vA, eA);
+            o.ea = lde_u(Vyi, q, qv && vA, eA);
             o.eb = lde(Vzi, q, qv && vB, eB);
         }
     };
@@ -382,7 +392,7 @@ __device__ __forceinline__ void fused_tile(const StencilParams& p, const FusedPt
                                            ORDER == 4 ? s2[a] : 0.0f, inner, coef, C1, C2);
                 if (npml > 0) nv = nv * damp(imax3(di0, dJ[r], dK), imax3(di1, dJ[r], dK));
                 nv = val ? nv : 0.0f;
-                if (own) put_row(Vxo, f, r, nv, val, true);
+                if (own) put_row(Vxo, f, r, nv, val, true, BLK ? o.ex[r] : 0);
 #pragma unroll
                 for (int s = 0; s < NQ - 1; ++s) vq[s][r] = vq[s + 1][r];
                 vq[NQ - 1][r] = nv;
@@ -403,7 +413,7 @@ __device__ __forceinline__ void fused_tile(const StencilParams& p, const FusedPt
                                            vyIn[r], coef, C1, C2);
                 if (npml > 0) nv = nv * damp(imax3(dI, dJ[r], dK), imax3(dI, dJ1[r], dK));
                 nv = vyV[r] ? nv : 0.0f;
-                put_row(Vyo, q, r, nv, vyV[r], true);
+                put_row(Vyo, q, r, nv, vyV[r], true, BLK ? o.ey[r] : 0);
                 sVy[(FLO + w * R + r) * TZ + lane] = nv;
             }
             // ── halo Vy face row (task A) ──
@@ -426,7 +436,7 @@ __device__ __forceinline__ void fused_tile(const StencilParams& p, const FusedPt
                                            C1, C2);
                 if (npml > 0) nv = nv * damp(imax3(dI, dJ[r], dK), imax3(dI, dJ[r], dK1));
                 nv = vzV[r] ? nv : 0.0f;
-                put_row(Vzo, q, r, nv, vzV[r], true);
+                put_row(Vzo, q, r, nv, vzV[r], true, BLK ? o.ez[r] : 0);
                 sVz[(w * R + r) * FC + FLO + lane] = nv;
             }
             // ── halo Vz face (task B) ──
@@ -474,7 +484,7 @@ __device__ __forceinline__ void fused_tile(const StencilParams& p, const FusedPt
                     out = pc - kap * dv;
                 }
                 if (npml > 0) out = out * sPd[imax3(dI, dJ[r], dK)];
-                put_row(Pout, q, r, out, ownV[r], false);
+                put_row(Pout, q, r, out, ownV[r], false, -1000);
             }
         }
     };
