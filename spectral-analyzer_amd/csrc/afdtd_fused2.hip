@@ -45,7 +45,7 @@ namespace {
 namespace f2 {
 constexpr int TZ = 64;           // own columns (one per lane)
 constexpr int GC = TZ + 4;       // P^n region columns -2..65
-constexpr int NSG = 4;           // P^n ring: planes q-2..q+1 (+ q+2 refilled)
+constexpr int NSG = 8;           // P^n ring: planes q-2..q+1 read, q+2 refilled (power of two)
 constexpr int EC = TZ + 2;       // P^{n+1} columns -1..64
 // Tile geometry for R extended rows per main thread.
 template <int R>
@@ -65,6 +65,34 @@ constexpr int MAXT = 256;        // receiver-tap entries cached per tile
 // entry key: plane << 12 | extended row << 7 | (extended column + 1)
 __host__ __device__ __forceinline__ int ekey(int plane, int e, int c) {
     return (plane << 12) | (e << 7) | (c + 1);
+}
+// Buffer access: a wave-uniform base in a scalar descriptor plus a 32-bit
+// per-lane byte offset (no 64-bit address arithmetic per lane).  `bytes`
+// bounds the offsets: reads beyond it return 0, writes beyond it are
+// dropped.  Stores of FP32 fields are non-temporal (written once per sweep).
+typedef __amdgpu_buffer_rsrc_t Rsrc;
+__device__ __forceinline__ Rsrc rsrc(const void* base, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes,
+                                             0x00020000);
+}
+template <typename Raw> __device__ __forceinline__ Raw bld(Rsrc r, uint32_t off);
+template <> __device__ __forceinline__ float bld<float>(Rsrc r, uint32_t off) {
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+template <> __device__ __forceinline__ uint16_t bld<uint16_t>(Rsrc r, uint32_t off) {
+    return __builtin_amdgcn_raw_buffer_load_b16(r, off, 0, 0);
+}
+template <> __device__ __forceinline__ f16_t bld<f16_t>(Rsrc r, uint32_t off) {
+    return __builtin_bit_cast(f16_t, (uint16_t)__builtin_amdgcn_raw_buffer_load_b16(r, off, 0, 0));
+}
+__device__ __forceinline__ void bst(Rsrc r, uint32_t off, float v) {
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, off, 0, 2);
+}
+__device__ __forceinline__ void bst(Rsrc r, uint32_t off, uint16_t v) {
+    __builtin_amdgcn_raw_buffer_store_b16(v, r, off, 0, 0);
+}
+__device__ __forceinline__ void bst(Rsrc r, uint32_t off, f16_t v) {
+    __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(uint16_t, v), r, off, 0, 0);
 }
 // LDS index helpers (row / column in extended-tile coordinates)
 __device__ __forceinline__ int gix(int r, int c) { return (r + 1) * GC + (c + 2); }
@@ -97,7 +125,9 @@ struct Pair2Args {
     float* tapbuf;          // P^{n+1} at the receiver taps
 };
 
-template <typename T, bool HET, int R>
+// PML: the grid has an absorbing layer (PML = false compiles the layer's
+// damping out; the host picks the variant from n_pml).
+template <typename T, bool HET, int R, bool PML>
 __global__ void __launch_bounds__(f2::Geo2<R>::NT) __attribute__((amdgpu_waves_per_eu(AFDTD_F2_WPE)))
 k_fused2(Pair2Args a) {
     using namespace f2;
@@ -110,11 +140,14 @@ k_fused2(Pair2Args a) {
 
     __shared__ float sP[NSG * GPL];
     __shared__ float sB[HET ? NSG * GPL : 1];
-    __shared__ float sP1[4 * EPL];
-    __shared__ float sAy[(EY + 1) * TZ];
-    __shared__ float sAz[EY * (TZ + 1)];
-    __shared__ float sBy[(TYO + 1) * TZ];
-    __shared__ float sBz[TYO * (TZ + 1)];
+    // double-buffered by plane parity: one iteration writes the faces (or
+    // P^{n+1}) of one plane while it reads those of the plane before
+    constexpr int AYN = (EY + 1) * TZ, AZN = EY * (TZ + 1), BYN = (TYO + 1) * TZ, BZN = TYO * (TZ + 1);
+    __shared__ float sP1[2 * EPL];
+    __shared__ float sAy[2 * AYN];
+    __shared__ float sAz[2 * AZN];
+    __shared__ float sBy[2 * BYN];
+    __shared__ float sBz[2 * BZN];
     __shared__ float sAlpha[65], sPd[65];
     __shared__ int sSrcKey[MAXS], sTapKey[MAXT], sTapSlot[MAXT];
     __shared__ float sAmp[MAXS];
@@ -131,7 +164,7 @@ k_fused2(Pair2Args a) {
     const int tid = lane + 64 * w;
     const int j0 = ty * TYO, k0 = tz * TZ;
     const int i0 = sg * a.seg, i1 = min(i0 + a.seg, Nx);
-    const int npml = p.n_pml;
+    const int npml = PML ? p.n_pml : 0;
     if (npml > 0)
         for (int d = tid; d <= npml && d < 65; d += NT) { sAlpha[d] = a.alpha[d]; sPd[d] = a.pdamp[d]; }
     const float vcoef = p.vel_coeff, dtdx = p.dtdx, dt = p.dt, kap0 = p.bulk_coeff;
@@ -155,14 +188,18 @@ k_fused2(Pair2Args a) {
     // grid) and a per-lane byte offset fixed for the whole sweep (0 for a lane
     // whose cell lies outside the grid).  A value loaded for an invalid plane
     // or lane only reaches faces and cells that are masked to zero or never
-    // stored, so no per-load select or 64-bit address arithmetic is needed.
+    // stored, so no per-load select is needed.  Stores use the same offsets
+    // (a stored cell is always a valid one).
     const int esz = (int)sizeof(T);
+    const uint32_t pbT = (uint32_t)sx * esz, pbF = (uint32_t)sx * 4;   // plane bytes
     auto cq = [&](int q) __attribute__((always_inline)) { return min(max(q, 0), Nx - 1); };
-    auto ldT = [&](const T* __restrict__ Fq, uint32_t boff) __attribute__((always_inline)) {
-        return *(const RawT*)((const char*)Fq + boff);
-    };
-    auto ldF = [&](const float* __restrict__ Fq, uint32_t boff) __attribute__((always_inline)) {
-        return *(const float*)((const char*)Fq + boff);
+    auto rT = [&](const T* F, size_t b) __attribute__((always_inline)) { return rsrc(F + b, pbT); };
+    auto rF = [&](const float* F, size_t b) __attribute__((always_inline)) { return rsrc(F + b, pbF); };
+    auto ldT = [&](Rsrc r, uint32_t boff) __attribute__((always_inline)) { return bld<RawT>(r, boff); };
+    auto ldF = [&](Rsrc r, uint32_t boff) __attribute__((always_inline)) { return bld<float>(r, boff); };
+    // store of a velocity (isV: scaled for 16-bit storage) or a pressure
+    auto stT = [&](T* F, int plane, uint32_t boff, float v, bool isV) __attribute__((always_inline)) {
+        bst(rT(F, (size_t)plane * sx), boff, S::raw_of((S::exact || !isV) ? v : v * sv));
     };
     // stored-and-reloaded value of a velocity (rndv) / pressure (rndp)
     // under 16-bit storage; identity for FP32
@@ -217,10 +254,12 @@ k_fused2(Pair2Args a) {
     auto load_plane = [&](int pl) __attribute__((always_inline)) {
         Stage st;
         const size_t b = (size_t)cq(pl) * sx;
+        const Rsrc rp = rT(Pin, b);
+        const Rsrc rb = rF(HET ? bu : (const float*)Pin, b);
 #pragma unroll
         for (int m = 0; m < 3; ++m) {
-            st.p[m] = ldT(Pin + b, rgT[m]);
-            st.b[m] = HET ? ldF(bu + b, rgF[m]) : 0.0f;
+            st.p[m] = ldT(rp, rgT[m]);
+            st.b[m] = HET ? ldF(rb, rgF[m]) : 0.0f;
         }
         return st;
     };
@@ -316,45 +355,57 @@ k_fused2(Pair2Args a) {
         xD[0] = oF[0] - xF[0];
         oF[1] = xF[1] = xD[1] = 0u;
     }
+    // Per-slot offsets of the old-value loads, so that both roles issue the
+    // same ten loads (slot 1 reads Vx for main waves and Vy for the halo
+    // wave's T4 face; the descriptor is chosen per wave):
+    //   slots x0 x1 y0 y1 z0 z1 (T fields), k0 k1 (kappa), xi via xF / xD
+    uint32_t lo[6], lf[2];
+    if (mainW) {
+        lo[0] = lo[2] = lo[4] = oT[0];
+        lo[1] = lo[3] = lo[5] = oT[1];
+        lf[0] = oF[0];
+        lf[1] = oF[1];
+    } else {
+        lo[0] = oT[0];
+        lo[1] = oT[3];
+        lo[2] = oT[1];
+        lo[3] = oT[0];
+        lo[4] = oT[2];
+        lo[5] = oT[0];
+        lf[0] = lf[1] = oF[0];
+    }
 
-    // Old values (V^{n-1/2}, kappa, wall sums) of one plane; the halo wave
-    // uses x[0] (Vx), y[0..1] (Vy), z[0..1] (Vz / T1), a (T4), k[0], xi[0].
+    // Old values of plane q: V^{n-1/2} (stage-A faces of q) and the kappa /
+    // wall sums of plane q-1 (stage-A pressure of q-1, one iteration later).
+    // Main waves: x[r], y[r], z[r], kk[r], xi[r] of row r.  The halo wave
+    // uses x[0] (Vx), x[1] (the T4 face), y[0..1] (Vy), z[0..1] (Vz / T1),
+    // kk[0], xi[0] (kk[1], xi[1] are loaded for the same cell and unused).
     struct Old {
-        RawT x[2], y[2], z[2], a;
+        RawT x[2], y[2], z[2];
         float kk[2], xi[2];
     };
     auto load_old = [&](int q) __attribute__((always_inline)) {
         Old o;
         const size_t b = (size_t)cq(q) * sx;
-        const bool xe = q == 0 || q == Nx - 1;
-        const T* __restrict__ X = Vxi + b;
-        const T* __restrict__ Y = Vyi + b;
-        const T* __restrict__ Z = Vzi + b;
-        o.x[1] = o.y[1] = o.z[1] = RawT();
-        o.kk[1] = o.xi[1] = 0.0f;
-        if (mainW) {
+        const size_t bk = (size_t)cq(q - 1) * sx;
+        const bool xe = q - 1 == 0 || q - 1 == Nx - 1;
+        const Rsrc X = rT(Vxi, b), Y = rT(Vyi, b), Z = rT(Vzi, b);
+        const Rsrc X1 = mainW ? X : Y;
+        const Rsrc K = rF(HET ? kapA : (const float*)Vxi, bk);
+        // without impedance walls the wall-sum descriptor is empty: its loads
+        // return 0 and touch no memory
+        const Rsrc XI = rsrc((loadXi ? xiA : (const float*)Vxi) + bk, loadXi ? pbF : 0u);
+        o.x[0] = ldT(X, lo[0]);
+        o.x[1] = ldT(X1, lo[1]);
+        o.y[0] = ldT(Y, lo[2]);
+        o.y[1] = ldT(Y, lo[3]);
+        o.z[0] = ldT(Z, lo[4]);
+        o.z[1] = ldT(Z, lo[5]);
 #pragma unroll
-            for (int r = 0; r < R; ++r) {
-                o.x[r] = ldT(X, oT[r]);
-                o.y[r] = ldT(Y, oT[r]);
-                o.z[r] = ldT(Z, oT[r]);
-                o.kk[r] = HET ? ldF(kapA + b, oF[r]) : 0.0f;
-                // every cell of an x-side plane is a wall cell
-                o.xi[r] = loadXi ? ldF(xiA + b, xF[r] + (xe ? xD[r] : 0u)) : 0.0f;
-            }
-            o.a = RawT();
-        } else {
-            o.x[0] = ldT(X, oT[0]);
-            o.y[0] = ldT(Y, oT[1]);
-            o.y[1] = ldT(Y, oT[0]);
-            o.z[0] = ldT(Z, oT[2]);
-            o.z[1] = ldT(Z, oT[0]);
-            o.a = ldT(Y, oT[3]);
-            o.kk[0] = HET ? ldF(kapA + b, oF[0]) : 0.0f;
-            o.xi[0] = loadXi ? ldF(xiA + b, xF[0] + (xe ? xD[0] : 0u)) : 0.0f;
-            o.x[1] = RawT();
-            o.kk[1] = 0.0f;
-            o.xi[1] = 0.0f;
+        for (int r = 0; r < 2; ++r) {
+            // every cell of an x-side plane is a wall cell
+            o.kk[r] = (HET && r < R) ? ldF(K, lf[r]) : 0.0f;
+            o.xi[r] = (HET && r < R) ? ldF(XI, xF[r] + (xe ? xD[r] : 0u)) : 0.0f;
         }
         return o;
     };
@@ -381,22 +432,44 @@ k_fused2(Pair2Args a) {
         }
     };
 
-    // ── per-thread pipeline registers ──
-    // Stage B runs two planes behind stage A.  Stage A: Vx faces q (xa0),
-    // q-1 (xa1), q-2 (xa2); Vy / Vz of planes q, q-1, q-2 (y0..y2, z0..z2);
-    // P^{n+1} of planes q, q-1, q-2 (pa0..pa2).  Stage B: Vx^{n+3/2} faces q-2
-    // (xb0) and q-3 (xb1).  kappa / wall sums of planes q-1 and q-2.
+    // ── per-thread pipeline registers (iteration q, after its updates) ──
+    // Stage A: Vx faces q (xa0), q-1 (xa1), q-2 (xa2); Vy / Vz of planes q,
+    // q-1, q-2 (y0..y2, z0..z2); P^{n+1} of planes q-1, q-2, q-3 (pa0..pa2).
+    // Stage B: Vx^{n+3/2} faces q-2 (xb0), q-3 (xb1), q-4 (xb2).  kappa / wall
+    // sums of planes q-2, q-3 for stage-B pressure (kb1, kb2, xb1w, xb2w;
+    // those of plane q-1 arrive with `o`).
     float xa0[R], xa1[R], xa2[R], y0[R], y1[R], y2[R], z0[R], z1[R], z2[R];
-    float pa0[R], pa1[R], pa2[R], xb0[R], xb1[R], kk1[R], kk2[R], xx1[R], xx2[R];
+    float pa0[R], pa1[R], pa2[R], xb0[R], xb1[R], xb2[R];
+    float kb1[R], kb2[R], xw1[R], xw2[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         xa0[r] = xa1[r] = xa2[r] = y0[r] = y1[r] = y2[r] = z0[r] = z1[r] = z2[r] = 0.0f;
-        pa0[r] = pa1[r] = pa2[r] = xb0[r] = xb1[r] = 0.0f;
-        kk1[r] = kk2[r] = xx1[r] = xx2[r] = 0.0f;
+        pa0[r] = pa1[r] = pa2[r] = xb0[r] = xb1[r] = xb2[r] = 0.0f;
+        kb1[r] = kb2[r] = xw1[r] = xw2[r] = 0.0f;
     }
-    // halo-wave extras: Vy faces hj-1, hj; Vz faces hk-1, hk (stage A, plane
-    // q); the z-face between columns -1 and 0 of planes q-1 and q-2
-    float hy[2] = {0.0f, 0.0f}, hz[2] = {0.0f, 0.0f}, hz1 = 0.0f, hz2 = 0.0f;
+    // Halo-wave pipeline: Vy faces hj-1, hj and Vz faces hk-1, hk of plane q
+    // (stage-A faces) and of plane q-1 (stage-A pressure), and the z-face
+    // between columns -1 and 0 of plane q-2 (stage-B face).  With two rows
+    // per main thread they ride in the second-row slots of the main
+    // pipeline arrays, which the halo wave does not use, and rotate with
+    // them: hy0 -> y0[1], hy1 -> z0[1], hz0 -> xb0[1], hz1 -> pa0[1] (so
+    // pa2[1] is hz1 of plane q-2).
+    float hq[4] = {0.0f, 0.0f, 0.0f, 0.0f}, hp[4] = {0.0f, 0.0f, 0.0f, 0.0f}, hz2s = 0.0f;
+    auto hset = [&](float a0, float a1, float b0, float b1) __attribute__((always_inline)) {
+        if constexpr (R == 2) {
+            y0[1] = a0; z0[1] = a1; xb0[1] = b0; pa0[1] = b1;
+        } else {
+            hq[0] = a0; hq[1] = a1; hq[2] = b0; hq[3] = b1;
+        }
+    };
+    auto hprev = [&](int i) __attribute__((always_inline)) {   // plane q-1 values
+        if constexpr (R == 2) return i == 0 ? y1[1] : i == 1 ? z1[1] : i == 2 ? xb1[1] : pa1[1];
+        else return hp[i];
+    };
+    auto hz2v = [&]() __attribute__((always_inline)) {
+        if constexpr (R == 2) return pa2[1];
+        else return hz2s;
+    };
 
     // The tile's source and tap entries (sorted by plane) and the source
     // amplitudes of step n+1, cached in LDS; walked with a uniform cursor.
@@ -414,28 +487,30 @@ k_fused2(Pair2Args a) {
         sTapSlot[m] = en.y;
     }
     int scur = 0, tcur = 0;            // uniform cursors into the cached lists
+    const int qf = i0 - 1;             // first plane of stage A (one-plane ring in x)
 
-    // One iteration: stage A at plane q (faces, then pressure) and stage B at
-    // plane qb = q-2 (faces, then pressure), in two phases separated by
-    // barriers.  `o` holds the old values of plane q and `st` the staged
-    // region plane q+2; the loads of the next plane's old values and of
-    // region plane q+3 are issued first, so they have a whole iteration to
-    // arrive.
-    struct Next {
-        Old o;
-        Stage st;
-    };
-    auto plane = [&](int q, const Old& o, const Stage& st) __attribute__((always_inline)) {
-        Next n;
-        n.o = load_old(q + 1);
-        n.st = load_plane(q + 3);
-        const bool doA = q <= i1;
+    // One iteration, one barrier: stage-A faces of plane q, stage-A pressure
+    // of plane q-1, stage-B faces of plane q-2 and stage-B pressure of plane
+    // q-3.  Each reads only LDS data written before the previous barrier (the
+    // face and P^{n+1} buffers alternate by plane parity, the P^n ring holds
+    // planes q-2..q+2) or registers of the same thread written earlier in the
+    // iteration.  On entry `o` holds the old values of plane q and `st` the
+    // staged region plane q+2.  The iteration first uses them up (stage-A
+    // faces, ring refill, kappa / wall sums into the pipeline) and then
+    // loads the next plane's into the same registers, so no loaded value is
+    // copied while its load may still be in flight; the loads have the rest
+    // of the iteration to arrive.
+    auto plane = [&](int q, Old& o, Stage& st) __attribute__((always_inline)) {
+        const int qa = q - 1, qb = q - 2, qc = q - 3;
         const bool qv = q >= 0 && q < Nx;
-        const int qb = q - 2;
         const bool qbv = qb >= 0 && qb < Nx;
+        float* sAyW = sAy + (q & 1) * AYN;
+        float* sAzW = sAz + (q & 1) * AZN;
+        const float* sAyR = sAy + (qa & 1) * AYN;
+        const float* sAzR = sAz + (qa & 1) * AZN;
 
-        // ════ phase 1: stage-A faces (plane q), stage-B faces (plane qb) ════
-        if (doA) {
+        // ════ stage-A faces of plane q ════
+        if (q <= i1) {
             vxA(q, o, xa0);
             if (mainW) {
 #pragma unroll
@@ -446,18 +521,19 @@ k_fused2(Pair2Args a) {
                         float nv = vel(o.y[r]) - coef * (Pn(q, e + 1, lane) - Pn(q, e, lane));
                         if (npml > 0) nv = nv * damp(depth3(q, mj[r], k), depth3(q, mj[r] + 1, k));
                         y0[r] = (qv && mvy[r]) ? nv : 0.0f;
-                        sAy[ayix(e, lane)] = y0[r];
+                        sAyW[ayix(e, lane)] = y0[r];
                     }
                     {   // z-face between columns lane and lane+1, row e
                         const float coef = HET ? (0.5f * (Bn(q, e, lane) + Bn(q, e, lane + 1))) * dtdx : vcoef;
                         float nv = vel(o.z[r]) - coef * (Pn(q, e, lane + 1) - Pn(q, e, lane));
                         if (npml > 0) nv = nv * damp(depth3(q, mj[r], k), depth3(q, mj[r], k + 1));
                         z0[r] = (qv && mvz[r]) ? nv : 0.0f;
-                        sAz[azix(e, lane)] = z0[r];
+                        sAzW[azix(e, lane)] = z0[r];
                     }
                 }
             } else {
                 if (hH) {
+                    float hy[2], hz[2];
 #pragma unroll
                     for (int u = 0; u < 2; ++u) {   // y-faces between rows he-1+u and he+u
                         const int e = he - 1 + u;
@@ -474,97 +550,63 @@ k_fused2(Pair2Args a) {
                         if (npml > 0) nv = nv * damp(depth3(q, hj, hk - 1 + u), depth3(q, hj, hk + u));
                         hz[u] = (qv && (u ? hvz1 : hvz0)) ? nv : 0.0f;
                     }
-                    if (hc < 0) sAz[azix(he, -1)] = hz[1];
+                    if (hc < 0) sAzW[azix(he, -1)] = hz[1];
+                    hset(hy[0], hy[1], hz[0], hz[1]);
                 }
                 if (hT1) {
                     const float coef = HET ? (0.5f * (Bn(q, t1e, -1) + Bn(q, t1e, 0))) * dtdx : vcoef;
                     float nv = vel(o.z[0]) - coef * (Pn(q, t1e, 0) - Pn(q, t1e, -1));
                     if (npml > 0) nv = nv * damp(depth3(q, t1j, k0 - 1), depth3(q, t1j, k0));
-                    sAz[azix(t1e, -1)] = (qv && t1v) ? nv : 0.0f;
+                    sAzW[azix(t1e, -1)] = (qv && t1v) ? nv : 0.0f;
                 }
                 {   // T4: y-face between rows -1 and 0
                     const float coef = HET ? (0.5f * (Bn(q, -1, lane) + Bn(q, 0, lane))) * dtdx : vcoef;
-                    float nv = vel(o.a) - coef * (Pn(q, 0, lane) - Pn(q, -1, lane));
+                    float nv = vel(o.x[1]) - coef * (Pn(q, 0, lane) - Pn(q, -1, lane));
                     if (npml > 0) nv = nv * damp(depth3(q, t4j, k), depth3(q, t4j + 1, k));
-                    sAy[ayix(-1, lane)] = (qv && t4v) ? nv : 0.0f;
+                    sAyW[ayix(-1, lane)] = (qv && t4v) ? nv : 0.0f;
                 }
             }
+            // ring refill: plane q+2 (its slot last held plane q-6)
+            store_plane(q + 2, st);
         }
-        if (qb >= i0 - 1) {
-            const bool fvb = qb >= 0 && qb <= Nx - 2;
-            const bool ownq = qb >= i0 && qb < i1;
-            const int bb = qb * sx;
-            const float* p1b = sP1 + (qb & 3) * EPL;
-            if (mainW) {
+        float kA[2], xA[2];   // kappa / wall sums of plane q-1
 #pragma unroll
-                for (int r = 0; r < R; ++r) {
-                    const int e = me[r];
-                    const bool ownr = e >= 1 && e <= TYO;
-                    if (ownr) {   // x-face between planes qb and qb+1
-                        const float coef = HET ? (0.5f * (Bn(qb, e, lane) + Bn(qb + 1, e, lane))) * dtdx : vcoef;
-                        float nv = rndv(xa2[r]) - coef * (pa1[r] - pa2[r]);
-                        if (npml > 0) nv = nv * damp(depth3(qb, mj[r], k), depth3(qb + 1, mj[r], k));
-                        nv = (fvb && mcell[r]) ? nv : 0.0f;
-                        xb0[r] = nv;
-                        if (ownq && fvb && mcell[r]) S::st_nt(Vxo, bb + moff[r], S::exact ? nv : nv * sv);
-                    }
-                    if (qb >= i0) {
-                        if (e <= TYO) {   // y-face between rows e and e+1
-                            const float coef = HET ? (0.5f * (Bn(qb, e, lane) + Bn(qb, e + 1, lane))) * dtdx : vcoef;
-                            float nv = rndv(y2[r]) - coef * (p1b[p1ix(e + 1, lane)] - p1b[p1ix(e, lane)]);
-                            if (npml > 0) nv = nv * damp(depth3(qb, mj[r], k), depth3(qb, mj[r] + 1, k));
-                            nv = (qbv && mvy[r]) ? nv : 0.0f;
-                            sBy[byix(e, lane)] = nv;
-                            if (ownr && qbv && mvy[r]) S::st_nt(Vyo, bb + moff[r], S::exact ? nv : nv * sv);
-                        }
-                        if (ownr) {   // z-face between columns lane and lane+1
-                            const float coef = HET ? (0.5f * (Bn(qb, e, lane) + Bn(qb, e, lane + 1))) * dtdx : vcoef;
-                            float nv = rndv(z2[r]) - coef * (p1b[p1ix(e, lane + 1)] - p1b[p1ix(e, lane)]);
-                            if (npml > 0) nv = nv * damp(depth3(qb, mj[r], k), depth3(qb, mj[r], k + 1));
-                            nv = (qbv && mvz[r]) ? nv : 0.0f;
-                            sBz[bzix(e, lane)] = nv;
-                            if (qbv && mvz[r]) S::st_nt(Vzo, bb + moff[r], S::exact ? nv : nv * sv);
-                        }
-                    }
-                }
-            } else if (hH && hc < 0 && qb >= i0) {   // z-face between columns -1 and 0, row he
-                const float coef = HET ? (0.5f * (Bn(qb, he, -1) + Bn(qb, he, 0))) * dtdx : vcoef;
-                float nv = rndv(hz2) - coef * (p1b[p1ix(he, 0)] - pa2[0]);
-                if (npml > 0) nv = nv * damp(depth3(qb, hj, k0 - 1), depth3(qb, hj, k0));
-                sBz[bzix(he, -1)] = (qbv && hvz1) ? nv : 0.0f;
-            }
+        for (int r = 0; r < 2; ++r) {
+            kA[r] = o.kk[r];
+            xA[r] = o.xi[r];
         }
-        __syncthreads();                                                   // (1)
+        o = load_old(q + 1);
+        st = load_plane(q + 3);
 
-        // ════ phase 2: stage-A pressure (plane q), stage-B pressure (plane qb) ════
-        if (doA) {
-            const bool xe = q == 0 || q == Nx - 1;
-            float* p1s = sP1 + (q & 3) * EPL;
+        // ════ stage-A pressure of plane qa = q-1 ════
+        if (qa >= qf && qa <= i1) {
+            const bool xe = qa == 0 || qa == Nx - 1;
+            float* p1s = sP1 + (qa & 1) * EPL;
             if (mainW) {
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
                     const int e = me[r];
-                    const float dv = (xa0[r] - xa1[r]) + (sAy[ayix(e, lane)] - sAy[ayix(e - 1, lane)])
-                                   + (sAz[azix(e, lane)] - sAz[azix(e, lane - 1)]);
-                    const float kap = HET ? o.kk[r] : kap0;
-                    const float xi = loadXi ? o.xi[r] : 0.0f;
-                    const float out = pres(Pn(q, e, lane), dv, kap, xi, q, mj[r], k,
+                    const float dv = (xa1[r] - xa2[r]) + (sAyR[ayix(e, lane)] - sAyR[ayix(e - 1, lane)])
+                                   + (sAzR[azix(e, lane)] - sAzR[azix(e, lane - 1)]);
+                    const float kap = HET ? kA[r] : kap0;
+                    const float xi = loadXi ? xA[r] : 0.0f;
+                    const float out = pres(Pn(qa, e, lane), dv, kap, xi, qa, mj[r], k,
                                            medge[r] || (xe && mcell[r]));
                     pa0[r] = rndp(out);
                 }
             } else if (hH) {
-                const float dv = (xa0[0] - xa1[0]) + (hy[1] - hy[0]) + (hz[1] - hz[0]);
-                const float kap = HET ? o.kk[0] : kap0;
-                const float xi = loadXi ? o.xi[0] : 0.0f;
-                const float out = pres(Pn(q, he, hc), dv, kap, xi, q, hj, hk,
+                const float dv = (xa1[0] - xa2[0]) + (hprev(1) - hprev(0)) + (hprev(3) - hprev(2));
+                const float kap = HET ? kA[0] : kap0;
+                const float xi = loadXi ? xA[0] : 0.0f;
+                const float out = pres(Pn(qa, he, hc), dv, kap, xi, qa, hj, hk,
                                        hedge || (xe && hcell));
                 pa0[0] = rndp(out);
             }
             // point sources of step n+1 (after the rounding, in source order)
-            while (scur < ns && (__builtin_amdgcn_readfirstlane(sSrcKey[scur]) >> 12) < q) ++scur;
+            while (scur < ns && (__builtin_amdgcn_readfirstlane(sSrcKey[scur]) >> 12) < qa) ++scur;
             for (; scur < ns; ++scur) {
                 const int key = __builtin_amdgcn_readfirstlane(sSrcKey[scur]);
-                if ((key >> 12) != q) break;
+                if ((key >> 12) != qa) break;
                 const int e = (key >> 7) & 31, c = (key & 127) - 1;
                 const float amp = sAmp[scur];
                 if (mainW) {
@@ -584,12 +626,12 @@ k_fused2(Pair2Args a) {
                 p1s[p1ix(he, hc)] = pa0[0];
             }
             // receiver taps of step n+1 (own planes and cells only)
-            while (tcur < nt && (__builtin_amdgcn_readfirstlane(sTapKey[tcur]) >> 12) < q) ++tcur;
+            while (tcur < nt && (__builtin_amdgcn_readfirstlane(sTapKey[tcur]) >> 12) < qa) ++tcur;
             for (; tcur < nt; ++tcur) {
                 const int key = __builtin_amdgcn_readfirstlane(sTapKey[tcur]);
-                if ((key >> 12) != q) break;
+                if ((key >> 12) != qa) break;
                 const int e = (key >> 7) & 31, c = (key & 127) - 1;
-                if (mainW && q >= i0 && q < i1 && e / R == w && c == lane) {
+                if (mainW && qa >= i0 && qa < i1 && e / R == w && c == lane) {
                     float v = pa0[0];
 #pragma unroll
                     for (int r = 1; r < R; ++r)
@@ -597,25 +639,72 @@ k_fused2(Pair2Args a) {
                     a.tapbuf[sTapSlot[tcur]] = v;
                 }
             }
-            // ring refill: plane q+2 (its slot last held plane q-2, last read by
-            // this iteration's stage-B faces before barrier (1))
-            store_plane(q + 2, st);
         }
-        if (qb >= i0 && mainW) {
-            const int bb = qb * sx;
-            const bool xeb = qb == 0 || qb == Nx - 1;
+
+        // ════ stage-B faces of plane qb = q-2 ════
+        if (qb >= i0 - 1 && qb <= i1 - 1) {
+            const bool fvb = qb >= 0 && qb <= Nx - 2;
+            const bool ownq = qb >= i0;
+            const float* p1b = sP1 + (qb & 1) * EPL;
+            float* sByW = sBy + (qb & 1) * BYN;
+            float* sBzW = sBz + (qb & 1) * BZN;
+            if (mainW) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const int e = me[r];
+                    const bool ownr = e >= 1 && e <= TYO;
+                    if (ownr) {   // x-face between planes qb and qb+1
+                        const float coef = HET ? (0.5f * (Bn(qb, e, lane) + Bn(qb + 1, e, lane))) * dtdx : vcoef;
+                        float nv = rndv(xa2[r]) - coef * (pa0[r] - pa1[r]);
+                        if (npml > 0) nv = nv * damp(depth3(qb, mj[r], k), depth3(qb + 1, mj[r], k));
+                        nv = (fvb && mcell[r]) ? nv : 0.0f;
+                        xb0[r] = nv;
+                        if (ownq && fvb && mcell[r]) stT(Vxo, qb, oT[r], nv, true);
+                    }
+                    if (ownq) {
+                        if (e <= TYO) {   // y-face between rows e and e+1
+                            const float coef = HET ? (0.5f * (Bn(qb, e, lane) + Bn(qb, e + 1, lane))) * dtdx : vcoef;
+                            float nv = rndv(y2[r]) - coef * (p1b[p1ix(e + 1, lane)] - p1b[p1ix(e, lane)]);
+                            if (npml > 0) nv = nv * damp(depth3(qb, mj[r], k), depth3(qb, mj[r] + 1, k));
+                            nv = (qbv && mvy[r]) ? nv : 0.0f;
+                            sByW[byix(e, lane)] = nv;
+                            if (ownr && qbv && mvy[r]) stT(Vyo, qb, oT[r], nv, true);
+                        }
+                        if (ownr) {   // z-face between columns lane and lane+1
+                            const float coef = HET ? (0.5f * (Bn(qb, e, lane) + Bn(qb, e, lane + 1))) * dtdx : vcoef;
+                            float nv = rndv(z2[r]) - coef * (p1b[p1ix(e, lane + 1)] - p1b[p1ix(e, lane)]);
+                            if (npml > 0) nv = nv * damp(depth3(qb, mj[r], k), depth3(qb, mj[r], k + 1));
+                            nv = (qbv && mvz[r]) ? nv : 0.0f;
+                            sBzW[bzix(e, lane)] = nv;
+                            if (qbv && mvz[r]) stT(Vzo, qb, oT[r], nv, true);
+                        }
+                    }
+                }
+            } else if (hH && hc < 0 && ownq) {   // z-face between columns -1 and 0, row he
+                const float coef = HET ? (0.5f * (Bn(qb, he, -1) + Bn(qb, he, 0))) * dtdx : vcoef;
+                float nv = rndv(hz2v()) - coef * (p1b[p1ix(he, 0)] - pa1[0]);
+                if (npml > 0) nv = nv * damp(depth3(qb, hj, k0 - 1), depth3(qb, hj, k0));
+                sBzW[bzix(he, -1)] = (qbv && hvz1) ? nv : 0.0f;
+            }
+        }
+
+        // ════ stage-B pressure of plane qc = q-3 ════
+        if (qc >= i0 && qc <= i1 - 1 && mainW) {
+            const bool xec = qc == 0 || qc == Nx - 1;
+            const float* sByR = sBy + (qc & 1) * BYN;
+            const float* sBzR = sBz + (qc & 1) * BZN;
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 const int e = me[r];
                 if (e < 1 || e > TYO) continue;
-                const float dv = (xb0[r] - xb1[r]) + (sBy[byix(e, lane)] - sBy[byix(e - 1, lane)])
-                               + (sBz[bzix(e, lane)] - sBz[bzix(e, lane - 1)]);
-                const float out = pres(pa2[r], dv, HET ? kk2[r] : kap0, xx2[r], qb, mj[r], k,
-                                       medge[r] || (xeb && mcell[r]));
-                if (mcell[r]) S::st_nt(Pout, bb + moff[r], out);
+                const float dv = (xb1[r] - xb2[r]) + (sByR[byix(e, lane)] - sByR[byix(e - 1, lane)])
+                               + (sBzR[bzix(e, lane)] - sBzR[bzix(e, lane - 1)]);
+                const float out = pres(pa2[r], dv, HET ? kb2[r] : kap0, xw2[r], qc, mj[r], k,
+                                       medge[r] || (xec && mcell[r]));
+                if (mcell[r]) stT(Pout, qc, oT[r], out, false);
             }
         }
-        __syncthreads();                                                   // (2)
+        __syncthreads();
 
         // ── rotate the pipeline ──
 #pragma unroll
@@ -628,19 +717,21 @@ k_fused2(Pair2Args a) {
             z1[r] = z0[r];
             pa2[r] = pa1[r];
             pa1[r] = pa0[r];
+            xb2[r] = xb1[r];
             xb1[r] = xb0[r];
-            kk2[r] = kk1[r];
-            kk1[r] = o.kk[r];
-            xx2[r] = xx1[r];
-            xx1[r] = o.xi[r];
+            kb2[r] = kb1[r];
+            kb1[r] = kA[r];
+            xw2[r] = xw1[r];
+            xw1[r] = xA[r];
         }
-        hz2 = hz1;
-        hz1 = hz[1];
-        return n;
+        if constexpr (R == 1) {
+            hz2s = hp[3];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) hp[i] = hq[i];
+        }
     };
 
     // ── prologue: ring planes i0-2 .. i0, staged plane i0+1, Vx face i0-2 ──
-    const int qf = i0 - 1;
     for (int pl = qf - 1; pl <= qf + 1; ++pl) store_plane(pl, load_plane(pl));
     Stage sa = load_plane(qf + 2);
     {
@@ -649,15 +740,9 @@ k_fused2(Pair2Args a) {
         vxA(qf - 1, o0, xa1);
     }
     Old oa = load_old(qf);
-    // iterations q = i0-1 .. i1+1 (stage B covers planes i0-1 .. i1-1),
-    // unrolled by two so that the register sets alternate without copies
-    for (int q = qf; q <= i1 + 1; q += 2) {
-        const Next nb = plane(q, oa, sa);
-        if (q + 1 > i1 + 1) break;
-        const Next na = plane(q + 1, nb.o, nb.st);
-        oa = na.o;
-        sa = na.st;
-    }
+    // iterations q = i0-1 .. i1+2 (stage-B pressure covers planes i0 .. i1-1)
+#pragma unroll 1
+    for (int q = qf; q <= i1 + 2; ++q) plane(q, oa, sa);
 }
 
 // Receiver traces of the intermediate step from the tap buffer, in the
@@ -675,15 +760,19 @@ k_tap_trace(const float* __restrict__ tapbuf, const float* __restrict__ rwgt, in
     }
 }
 
-template <typename T, bool HET>
-hipError_t launch2(AcousticFDTDState* st, const Pair2Args& a, int nb) {
+template <typename T, bool HET, bool PML>
+hipError_t launch2p(AcousticFDTDState* st, const Pair2Args& a, int nb) {
     if (st->f2_rows == 1)
-        hipLaunchKernelGGL((k_fused2<T, HET, 1>), dim3(nb), dim3(64, f2::Geo2<1>::NW), 0,
+        hipLaunchKernelGGL((k_fused2<T, HET, 1, PML>), dim3(nb), dim3(64, f2::Geo2<1>::NW), 0,
                            st->stream, a);
     else
-        hipLaunchKernelGGL((k_fused2<T, HET, 2>), dim3(nb), dim3(64, f2::Geo2<2>::NW), 0,
+        hipLaunchKernelGGL((k_fused2<T, HET, 2, PML>), dim3(nb), dim3(64, f2::Geo2<2>::NW), 0,
                            st->stream, a);
     return hipGetLastError();
+}
+template <typename T, bool HET>
+hipError_t launch2(AcousticFDTDState* st, const Pair2Args& a, int nb) {
+    return a.p.n_pml > 0 ? launch2p<T, HET, true>(st, a, nb) : launch2p<T, HET, false>(st, a, nb);
 }
 
 }  // namespace
