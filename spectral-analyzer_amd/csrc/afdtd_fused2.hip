@@ -45,7 +45,8 @@ namespace {
 namespace f2 {
 constexpr int TZ = 64;           // own columns (one per lane)
 constexpr int GC = TZ + 4;       // P^n region columns -2..65
-constexpr int NSG = 8;           // P^n ring: planes q-2..q+1 read, q+2 refilled (power of two)
+constexpr int NSP = 4;           // P^n ring: planes q-1..q+1 read, q+2 refilled (power of two)
+constexpr int NSB = 8;           // 1/rho ring: planes q-2..q+1 read, q+2 refilled
 constexpr int EC = TZ + 2;       // P^{n+1} columns -1..64
 // Tile geometry for R extended rows per main thread.
 template <int R>
@@ -97,15 +98,26 @@ __device__ __forceinline__ void bst(Rsrc r, uint32_t off, f16_t v) {
 // LDS index helpers (row / column in extended-tile coordinates)
 __device__ __forceinline__ int gix(int r, int c) { return (r + 1) * GC + (c + 2); }
 __device__ __forceinline__ int p1ix(int e, int c) { return e * EC + (c + 1); }
-__device__ __forceinline__ int ayix(int fr, int l) { return (fr + 1) * TZ + l; }        // fr -1..17
-__device__ __forceinline__ int azix(int e, int fc) { return e * (TZ + 1) + (fc + 1); }  // fc -1..63
-__device__ __forceinline__ int byix(int fr, int l) { return fr * TZ + l; }              // fr 0..16
-__device__ __forceinline__ int bzix(int e, int fc) { return (e - 1) * (TZ + 1) + (fc + 1); }
+// Value of lane l-1 of the wavefront (lane 0 gets `at0`): the z-face to the
+// left of a cell, which its left neighbour computed (DPP wave_shr:1; every
+// lane of the wavefront must be active).
+__device__ __forceinline__ float lane_left(float v, float at0) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, at0),
+                                                                 __builtin_bit_cast(int, v),
+                                                                 0x138, 0xF, 0xF, false));
+}
 }  // namespace f2
 
-// Minimum waves per SIMD the register allocation must allow.
+// Minimum waves per SIMD the register allocation must allow.  Without
+// per-cell media a workgroup needs under 52 KB of LDS, so two fit a CU when
+// their waves fit the register file too: two 10-wave workgroups can put six
+// waves on one SIMD, so the budget is 512 / 6 -> 80 VGPRs.  With per-cell
+// media (larger 1/rho ring) one workgroup per CU is the limit anyway.
 #ifndef AFDTD_F2_WPE
 #define AFDTD_F2_WPE 1
+#endif
+#ifndef AFDTD_F2_WPE_HOMOG
+#define AFDTD_F2_WPE_HOMOG 6
 #endif
 
 struct Pair2Args {
@@ -128,7 +140,8 @@ struct Pair2Args {
 // PML: the grid has an absorbing layer (PML = false compiles the layer's
 // damping out; the host picks the variant from n_pml).
 template <typename T, bool HET, int R, bool PML>
-__global__ void __launch_bounds__(f2::Geo2<R>::NT) __attribute__((amdgpu_waves_per_eu(AFDTD_F2_WPE)))
+__global__ void __launch_bounds__(f2::Geo2<R>::NT)
+__attribute__((amdgpu_waves_per_eu(HET ? AFDTD_F2_WPE : AFDTD_F2_WPE_HOMOG)))
 k_fused2(Pair2Args a) {
     using namespace f2;
     typedef Geo2<R> G;
@@ -138,24 +151,39 @@ k_fused2(Pair2Args a) {
     typedef typename S::Raw RawT;
     if (*a.status) return;
 
-    __shared__ float sP[NSG * GPL];
-    __shared__ float sB[HET ? NSG * GPL : 1];
-    // double-buffered by plane parity: one iteration writes the faces (or
-    // P^{n+1}) of one plane while it reads those of the plane before
-    constexpr int AYN = (EY + 1) * TZ, AZN = EY * (TZ + 1), BYN = (TYO + 1) * TZ, BZN = TYO * (TZ + 1);
+    __shared__ float sP[NSP * GPL];
+    __shared__ float sB[HET ? NSB * GPL : 1];
+    // Double-buffered by plane parity: one iteration writes the faces (or
+    // P^{n+1}) of one plane while it reads those of the plane before.  A
+    // cell's own faces and those of the rows of the same thread stay in
+    // registers, the z-face to its left comes from the neighbouring lane, so
+    // only the faces crossing a wavefront go through LDS: the y-face above a
+    // wavefront's first row (sAyX / sByX, slot w for wave w; slot 0 of sAyX
+    // is the face row between rows -1 and 0) and the z-face between columns
+    // -1 and 0 (sAzC / sBzC, by row).
+    constexpr int AXN = (NWM + 1) * TZ, ACN = EY, BXN = (NWM + 1) * TZ, BCN = TYO + 2;
     __shared__ float sP1[2 * EPL];
-    __shared__ float sAy[2 * AYN];
-    __shared__ float sAz[2 * AZN];
-    __shared__ float sBy[2 * BYN];
-    __shared__ float sBz[2 * BZN];
+    __shared__ float sAyX[2 * AXN];
+    __shared__ float sAzC[2 * ACN];
+    __shared__ float sByX[2 * BXN];
+    __shared__ float sBzC[2 * BCN];
     __shared__ float sAlpha[65], sPd[65];
     __shared__ int sSrcKey[MAXS], sTapKey[MAXT], sTapSlot[MAXT];
     __shared__ float sAmp[MAXS];
 
+    // Each XCD takes a contiguous range of work items (xcd_remap); within a
+    // segment the items run in bands of up to four z-tiles, z fastest, that
+    // sweep y, so the workgroups resident on one XCD at a time form a compact
+    // (y, z) block whose shared halo lines are read once into its L2.
     int t = xcd_remap(blockIdx.x, gridDim.x);
-    const int ty = t % a.nty; t /= a.nty;
-    const int tz = t % a.ntz;
-    const int sg = t / a.ntz;
+    const int per_sg = a.nty * a.ntz;
+    const int sg = t / per_sg;
+    t -= sg * per_sg;
+    const int band = t / (a.nty * 4);                   // bands of four z-tiles
+    const int gz = min(4, a.ntz - 4 * band);            // z-tiles in this band
+    t -= band * a.nty * 4;
+    const int ty = t / gz;
+    const int tz = 4 * band + t % gz;
     const int tile = ty + a.nty * tz;
 
     const StencilParams& p = a.p;
@@ -264,17 +292,16 @@ k_fused2(Pair2Args a) {
         return st;
     };
     auto store_plane = [&](int pl, const Stage& st) __attribute__((always_inline)) {
-        const int o = (pl & (NSG - 1)) * GPL;
 #pragma unroll
         for (int m = 0; m < 3; ++m) {
             if (ldLds[m] >= 0) {
-                sP[o + ldLds[m]] = S::cvt(st.p[m]);
-                if (HET) sB[o + ldLds[m]] = st.b[m];
+                sP[(pl & (NSP - 1)) * GPL + ldLds[m]] = S::cvt(st.p[m]);
+                if (HET) sB[(pl & (NSB - 1)) * GPL + ldLds[m]] = st.b[m];
             }
         }
     };
-    auto Pn = [&](int pl, int r, int c) __attribute__((always_inline)) { return sP[(pl & (NSG - 1)) * GPL + gix(r, c)]; };
-    auto Bn = [&](int pl, int r, int c) __attribute__((always_inline)) { return sB[(pl & (NSG - 1)) * GPL + gix(r, c)]; };
+    auto Pn = [&](int pl, int r, int c) __attribute__((always_inline)) { return sP[(pl & (NSP - 1)) * GPL + gix(r, c)]; };
+    auto Bn = [&](int pl, int r, int c) __attribute__((always_inline)) { return sB[(pl & (NSB - 1)) * GPL + gix(r, c)]; };
 
     // ── thread roles ──
     const bool mainW = w < NWM;
@@ -440,11 +467,13 @@ k_fused2(Pair2Args a) {
     // those of plane q-1 arrive with `o`).
     float xa0[R], xa1[R], xa2[R], y0[R], y1[R], y2[R], z0[R], z1[R], z2[R];
     float pa0[R], pa1[R], pa2[R], xb0[R], xb1[R], xb2[R];
+    float yb0[R], yb1[R], zb0[R], zb1[R];   // stage-B y / z faces of planes q-2, q-3
     float kb1[R], kb2[R], xw1[R], xw2[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         xa0[r] = xa1[r] = xa2[r] = y0[r] = y1[r] = y2[r] = z0[r] = z1[r] = z2[r] = 0.0f;
         pa0[r] = pa1[r] = pa2[r] = xb0[r] = xb1[r] = xb2[r] = 0.0f;
+        yb0[r] = yb1[r] = zb0[r] = zb1[r] = 0.0f;
         kb1[r] = kb2[r] = xw1[r] = xw2[r] = 0.0f;
     }
     // Halo-wave pipeline: Vy faces hj-1, hj and Vz faces hk-1, hk of plane q
@@ -504,10 +533,10 @@ k_fused2(Pair2Args a) {
         const int qa = q - 1, qb = q - 2, qc = q - 3;
         const bool qv = q >= 0 && q < Nx;
         const bool qbv = qb >= 0 && qb < Nx;
-        float* sAyW = sAy + (q & 1) * AYN;
-        float* sAzW = sAz + (q & 1) * AZN;
-        const float* sAyR = sAy + (qa & 1) * AYN;
-        const float* sAzR = sAz + (qa & 1) * AZN;
+        float* sAyW = sAyX + (q & 1) * AXN;
+        float* sAzW = sAzC + (q & 1) * ACN;
+        const float* sAyR = sAyX + (qa & 1) * AXN;
+        const float* sAzR = sAzC + (qa & 1) * ACN;
 
         // ════ stage-A faces of plane q ════
         if (q <= i1) {
@@ -521,14 +550,13 @@ k_fused2(Pair2Args a) {
                         float nv = vel(o.y[r]) - coef * (Pn(q, e + 1, lane) - Pn(q, e, lane));
                         if (npml > 0) nv = nv * damp(depth3(q, mj[r], k), depth3(q, mj[r] + 1, k));
                         y0[r] = (qv && mvy[r]) ? nv : 0.0f;
-                        sAyW[ayix(e, lane)] = y0[r];
+                        if (r == R - 1) sAyW[(w + 1) * TZ + lane] = y0[r];
                     }
                     {   // z-face between columns lane and lane+1, row e
                         const float coef = HET ? (0.5f * (Bn(q, e, lane) + Bn(q, e, lane + 1))) * dtdx : vcoef;
                         float nv = vel(o.z[r]) - coef * (Pn(q, e, lane + 1) - Pn(q, e, lane));
                         if (npml > 0) nv = nv * damp(depth3(q, mj[r], k), depth3(q, mj[r], k + 1));
                         z0[r] = (qv && mvz[r]) ? nv : 0.0f;
-                        sAzW[azix(e, lane)] = z0[r];
                     }
                 }
             } else {
@@ -550,23 +578,24 @@ k_fused2(Pair2Args a) {
                         if (npml > 0) nv = nv * damp(depth3(q, hj, hk - 1 + u), depth3(q, hj, hk + u));
                         hz[u] = (qv && (u ? hvz1 : hvz0)) ? nv : 0.0f;
                     }
-                    if (hc < 0) sAzW[azix(he, -1)] = hz[1];
+                    if (hc < 0) sAzW[he] = hz[1];
                     hset(hy[0], hy[1], hz[0], hz[1]);
                 }
                 if (hT1) {
                     const float coef = HET ? (0.5f * (Bn(q, t1e, -1) + Bn(q, t1e, 0))) * dtdx : vcoef;
                     float nv = vel(o.z[0]) - coef * (Pn(q, t1e, 0) - Pn(q, t1e, -1));
                     if (npml > 0) nv = nv * damp(depth3(q, t1j, k0 - 1), depth3(q, t1j, k0));
-                    sAzW[azix(t1e, -1)] = (qv && t1v) ? nv : 0.0f;
+                    sAzW[t1e] = (qv && t1v) ? nv : 0.0f;
                 }
                 {   // T4: y-face between rows -1 and 0
                     const float coef = HET ? (0.5f * (Bn(q, -1, lane) + Bn(q, 0, lane))) * dtdx : vcoef;
                     float nv = vel(o.x[1]) - coef * (Pn(q, 0, lane) - Pn(q, -1, lane));
                     if (npml > 0) nv = nv * damp(depth3(q, t4j, k), depth3(q, t4j + 1, k));
-                    sAyW[ayix(-1, lane)] = (qv && t4v) ? nv : 0.0f;
+                    sAyW[lane] = (qv && t4v) ? nv : 0.0f;
                 }
             }
-            // ring refill: plane q+2 (its slot last held plane q-6)
+            // ring refill: plane q+2 (its P slot last held plane q-2, read
+            // up to the previous iteration; its 1/rho slot plane q-6)
             store_plane(q + 2, st);
         }
         float kA[2], xA[2];   // kappa / wall sums of plane q-1
@@ -586,8 +615,9 @@ k_fused2(Pair2Args a) {
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
                     const int e = me[r];
-                    const float dv = (xa1[r] - xa2[r]) + (sAyR[ayix(e, lane)] - sAyR[ayix(e - 1, lane)])
-                                   + (sAzR[azix(e, lane)] - sAzR[azix(e, lane - 1)]);
+                    const float yd = r > 0 ? y1[r > 0 ? r - 1 : 0] : sAyR[w * TZ + lane];
+                    const float dv = (xa1[r] - xa2[r]) + (y1[r] - yd)
+                                   + (z1[r] - lane_left(z1[r], sAzR[e]));
                     const float kap = HET ? kA[r] : kap0;
                     const float xi = loadXi ? xA[r] : 0.0f;
                     const float out = pres(Pn(qa, e, lane), dv, kap, xi, qa, mj[r], k,
@@ -646,8 +676,8 @@ k_fused2(Pair2Args a) {
             const bool fvb = qb >= 0 && qb <= Nx - 2;
             const bool ownq = qb >= i0;
             const float* p1b = sP1 + (qb & 1) * EPL;
-            float* sByW = sBy + (qb & 1) * BYN;
-            float* sBzW = sBz + (qb & 1) * BZN;
+            float* sByW = sByX + (qb & 1) * BXN;
+            float* sBzW = sBzC + (qb & 1) * BCN;
             if (mainW) {
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
@@ -667,7 +697,8 @@ k_fused2(Pair2Args a) {
                             float nv = rndv(y2[r]) - coef * (p1b[p1ix(e + 1, lane)] - p1b[p1ix(e, lane)]);
                             if (npml > 0) nv = nv * damp(depth3(qb, mj[r], k), depth3(qb, mj[r] + 1, k));
                             nv = (qbv && mvy[r]) ? nv : 0.0f;
-                            sByW[byix(e, lane)] = nv;
+                            yb0[r] = nv;
+                            if (r == R - 1) sByW[(w + 1) * TZ + lane] = nv;
                             if (ownr && qbv && mvy[r]) stT(Vyo, qb, oT[r], nv, true);
                         }
                         if (ownr) {   // z-face between columns lane and lane+1
@@ -675,7 +706,7 @@ k_fused2(Pair2Args a) {
                             float nv = rndv(z2[r]) - coef * (p1b[p1ix(e, lane + 1)] - p1b[p1ix(e, lane)]);
                             if (npml > 0) nv = nv * damp(depth3(qb, mj[r], k), depth3(qb, mj[r], k + 1));
                             nv = (qbv && mvz[r]) ? nv : 0.0f;
-                            sBzW[bzix(e, lane)] = nv;
+                            zb0[r] = nv;
                             if (qbv && mvz[r]) stT(Vzo, qb, oT[r], nv, true);
                         }
                     }
@@ -684,21 +715,22 @@ k_fused2(Pair2Args a) {
                 const float coef = HET ? (0.5f * (Bn(qb, he, -1) + Bn(qb, he, 0))) * dtdx : vcoef;
                 float nv = rndv(hz2v()) - coef * (p1b[p1ix(he, 0)] - pa1[0]);
                 if (npml > 0) nv = nv * damp(depth3(qb, hj, k0 - 1), depth3(qb, hj, k0));
-                sBzW[bzix(he, -1)] = (qbv && hvz1) ? nv : 0.0f;
+                sBzW[he] = (qbv && hvz1) ? nv : 0.0f;
             }
         }
 
         // ════ stage-B pressure of plane qc = q-3 ════
         if (qc >= i0 && qc <= i1 - 1 && mainW) {
             const bool xec = qc == 0 || qc == Nx - 1;
-            const float* sByR = sBy + (qc & 1) * BYN;
-            const float* sBzR = sBz + (qc & 1) * BZN;
+            const float* sByR = sByX + (qc & 1) * BXN;
+            const float* sBzR = sBzC + (qc & 1) * BCN;
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 const int e = me[r];
                 if (e < 1 || e > TYO) continue;
-                const float dv = (xb1[r] - xb2[r]) + (sByR[byix(e, lane)] - sByR[byix(e - 1, lane)])
-                               + (sBzR[bzix(e, lane)] - sBzR[bzix(e, lane - 1)]);
+                const float yd = r > 0 ? yb1[r > 0 ? r - 1 : 0] : sByR[w * TZ + lane];
+                const float dv = (xb1[r] - xb2[r]) + (yb1[r] - yd)
+                               + (zb1[r] - lane_left(zb1[r], sBzR[e]));
                 const float out = pres(pa2[r], dv, HET ? kb2[r] : kap0, xw2[r], qc, mj[r], k,
                                        medge[r] || (xec && mcell[r]));
                 if (mcell[r]) stT(Pout, qc, oT[r], out, false);
@@ -719,6 +751,8 @@ k_fused2(Pair2Args a) {
             pa1[r] = pa0[r];
             xb2[r] = xb1[r];
             xb1[r] = xb0[r];
+            yb1[r] = yb0[r];
+            zb1[r] = zb0[r];
             kb2[r] = kb1[r];
             kb1[r] = kA[r];
             xw2[r] = xw1[r];
