@@ -282,13 +282,21 @@ SK_API FdtdBatch* fdtd_batch_create(int n_rooms, const int* dims, float dx, floa
                                     int stencil_order, int storage, int tile_y, int seg_x);
 SK_API void fdtd_batch_destroy(FdtdBatch* b);
 SK_API int fdtd_batch_cells(const FdtdBatch* b, long long* total, int* n_tiles);
-/* Autotuner (before the first step): times every tile height (8, 16, 32 rows),
- * x-segment (16..128 planes) and dispatch order (room order / largest rooms
- * first) with trial_steps graph-captured steps of this batch, keeps the
- * fastest, and leaves the batch at step 0 with zero fields.  Results of the
- * sweep do not depend on the choice (same arithmetic per cell). */
+/* Autotuner (before the first step): times every sweep (single-step; for
+ * order 2 also the two-step sweep), tile height (8, 16, 32 rows; 8, 16 for
+ * the two-step sweep), x-segment (16..128 planes) and dispatch order (room
+ * order / largest rooms first) with trial_steps graph-captured steps of this
+ * batch, keeps the fastest, and leaves the batch at step 0 with zero fields.
+ * Results do not depend on the choice (same arithmetic per cell). */
 SK_API int fdtd_batch_autotune(FdtdBatch* b, int trial_steps, int* best_tile_y, int* best_seg,
                                int* best_schedule, float* best_ms_per_step);
+/* Sweep of the batch: FDTD_KERNEL_AUTO (default: two steps per launch with the
+ * two-step sweep where it applies, i.e. order 2 and tile height 8 or 16),
+ * FDTD_KERNEL_FUSED (single steps) or FDTD_KERNEL_FUSED2 (two-step sweep
+ * required).  No reference counterpart (extension). */
+SK_API int fdtd_batch_set_kernel(FdtdBatch* b, int family);
+/* The sweep the next steps use: FDTD_KERNEL_FUSED2 or FDTD_KERNEL_FUSED. */
+SK_API int fdtd_batch_get_kernel_family(FdtdBatch* b);
 SK_API int fdtd_batch_set_sources(FdtdBatch* b, const int* cell_idx, const float* signals,
                                   int n_samples);
 SK_API int fdtd_batch_set_receivers(FdtdBatch* b, int per_room, const float* xyz);

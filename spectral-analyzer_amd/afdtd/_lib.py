@@ -95,6 +95,8 @@ _SIGS = {
     "fdtd_batch_get_traces": (_i, [_vp, _vp, _i, _vp]),
     "fdtd_batch_get_pressure": (_i, [_vp, _i, _vp, _i]),
     "fdtd_batch_get_step_count": (_i, [_vp]),
+    "fdtd_batch_set_kernel": (_i, [_vp, _i]),
+    "fdtd_batch_get_kernel_family": (_i, [_vp]),
     "fdtd_copy_calibration": (_i, [C.c_size_t, _i, _vp]),
     "fdtd_stream_calibration": (_i, [C.c_size_t, _i, _i, _i, _vp]),
     "fdtd_adjoint_create": (_vp, [_i, _i, _i, C.c_double, _i, _i]),

@@ -1,8 +1,9 @@
 """``RoomBatch`` — many independent rooms of uneven size in one sweep.
 
-One fused launch per step covers every tile of every room; the n-step
-sequence of a call is captured in a hipGraph and launched once
-(fdtd_batch_* in include/afdtd.h).  No reference counterpart: the
+One fused launch per step covers every tile of every room (for order 2,
+one launch of the two-step sweep per two steps); the n-step sequence of a
+call is captured in a hipGraph and launched once (fdtd_batch_* in
+include/afdtd.h).  No reference counterpart: the
 reference steps one ``AcousticFDTD`` handle at a time.
 """
 from __future__ import annotations
@@ -16,6 +17,9 @@ from . import _lib
 from ._lib import lib
 
 _STORAGE = {"f32": 0, "bf16": 1, "f16": 2}
+_KERNEL = {"auto": _lib.FDTD_KERNEL_AUTO, "fused": _lib.FDTD_KERNEL_FUSED,
+           "fused2": _lib.FDTD_KERNEL_FUSED2}
+_FAMILY = {_lib.FDTD_KERNEL_FUSED: "fused", _lib.FDTD_KERNEL_FUSED2: "fused2"}
 
 
 def _p(a):
@@ -25,7 +29,8 @@ def _p(a):
 class RoomBatch:
     def __init__(self, dims: Sequence[Sequence[int]], dx: float, courant: Sequence[float],
                  refl: Optional[np.ndarray] = None, c: float = 343.0, rho_air: float = 1.2,
-                 order: int = 2, storage: str = "f32", tile_y: int = 0, seg_x: int = 0):
+                 order: int = 2, storage: str = "f32", tile_y: int = 0, seg_x: int = 0,
+                 kernel: str = "auto"):
         self._L = lib()
         self._h = None
         self.dims = np.ascontiguousarray(np.asarray(dims, np.int32).reshape(-1, 3))
@@ -40,6 +45,10 @@ class RoomBatch:
         self._h = h
         self.per_room = 0
         self._cap = 0
+        if kernel != "auto":
+            rc = self._L.fdtd_batch_set_kernel(h, _KERNEL[kernel])
+            if rc:
+                raise RuntimeError(f"fdtd_batch_set_kernel error {rc}: {_lib.last_error()}")
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -54,18 +63,25 @@ class RoomBatch:
         return int(t.value)
 
     @property
+    def kernel_family(self) -> str:
+        """The sweep the next steps use: "fused2" (two steps per launch) or
+        "fused"."""
+        return _FAMILY[int(self._L.fdtd_batch_get_kernel_family(self._h))]
+
+    @property
     def step_count(self) -> int:
         return int(self._L.fdtd_batch_get_step_count(self._h))
 
     def autotune(self, trial_steps: int = 10) -> dict:
-        """Pick the tile height, x-segment and dispatch order by timing every
-        candidate on this batch (before the first step; fields stay zero)."""
+        """Pick the sweep, tile height, x-segment and dispatch order by timing
+        every candidate on this batch (before the first step; fields stay
+        zero)."""
         ty, sg, so, ms = C.c_int(), C.c_int(), C.c_int(), C.c_float()
         rc = self._L.fdtd_batch_autotune(self._h, int(trial_steps), C.byref(ty), C.byref(sg),
                                          C.byref(so), C.byref(ms))
         if rc:
             raise RuntimeError(f"fdtd_batch_autotune error {rc}: {_lib.last_error()}")
-        return {"tile_y": ty.value, "seg_x": sg.value,
+        return {"kernel": self.kernel_family, "tile_y": ty.value, "seg_x": sg.value,
                 "schedule": ("room order", "largest first")[so.value],
                 "ms_per_step": ms.value}
 

@@ -4,10 +4,14 @@
 // off[r], each in its own padded z-fastest layout).  A tile table lists every
 // (room, y-tile, z-tile, x-segment) work item of every room; ONE launch of the
 // batched fused sweep per step covers all of them, followed by ONE launch
-// that injects each room's source and gathers its receivers.  fdtd_batch_step
-// captures the whole n-step sequence into a hipGraph and launches it once, so
-// the host issues one graph launch per call instead of 2 x rooms x steps
-// kernel launches.  Rooms are independent: no room reads another's cells.
+// that injects each room's source and gathers its receivers.  For order 2
+// the batch advances two steps per launch with the two-step sweep
+// (afdtd_fused2.hip), whose in-kernel source / receiver-tap lists for the
+// first step are built per room here; the second step's source and
+// receivers go through the post-step launch.  fdtd_batch_step captures the
+// whole n-step sequence into a hipGraph and launches it once, so the host
+// issues one graph launch per call instead of 2 x rooms x steps kernel
+// launches.  Rooms are independent: no room reads another's cells.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <string.h>
@@ -18,14 +22,6 @@
 #include "afdtd_fused_tile.cuh"
 
 using namespace fused;
-
-struct RoomDesc {
-    int Nx, Ny, Nz, nty, ntz, nseg, tile_base, pad;
-    long long off;          // element offset of the room in every field array
-    float dt, vel_coeff, bulk_coeff;
-    float gside[6];
-    float xterm[6];         // wall terms (kap/2) * g / sqrt(rho K)
-};
 
 struct FdtdBatch {
     int n, order, storage, esz, boundary;
@@ -54,6 +50,17 @@ struct FdtdBatch {
     float* d_trace;
     int trace_cap, trace_n;
     int step_count;
+    // two-step sweep
+    int family;             // FDTD_KERNEL_AUTO, _FUSED (single steps) or _FUSED2
+    std::vector<int> h_src; // room-local source cells (-1: none)
+    std::vector<int> h_ridx;// tap cells of the receivers (global element indices)
+    int* d_ls_start;        // per-tile source / tap lists
+    int2* d_ls;
+    int* d_lt_start;
+    int2* d_lt;
+    float* d_tapbuf;
+    size_t tapcap;
+    int lists_dirty, lists_ok;
 };
 
 namespace {
@@ -204,6 +211,142 @@ int enqueue_batch_step(FdtdBatch* b) {
     return FDTD_OK;
 }
 
+// Whether the batch advances two steps per launch: order 2, a two-step tile
+// height (8 or 16 rows), not forced to single steps, lists within the caps.
+bool pairs_active(const FdtdBatch* b) {
+    return b->order == 2 && (b->tile_y == 8 || b->tile_y == 16) &&
+           b->family != FDTD_KERNEL_FUSED && b->lists_ok;
+}
+
+// Per-tile source / tap lists of every room for the two-step sweep (list
+// index list_base + ty + nty*tz), rebuilt when the tiles, the sources or the
+// receivers change.
+int build_pair_lists(FdtdBatch* b) {
+    if (!b->lists_dirty) return FDTD_OK;
+    b->lists_ok = 1;
+    if (b->order != 2 || (b->tile_y != 8 && b->tile_y != 16)) {
+        b->lists_dirty = 0;
+        return FDTD_OK;
+    }
+    const int TYO = f2_tile_rows(b->tile_y);
+    auto by_key = [](const int2& x, const int2& y) { return x.x < y.x || (x.x == y.x && x.y < y.y); };
+    std::vector<int> ss, ts;
+    std::vector<int2> sflat, tflat;
+    const int per = b->per_room;
+    std::vector<int> cells, slots;
+    for (int r = 0; r < b->n; ++r) {
+        const RoomDesc& d = b->rooms[r];
+        const int sc = (int)b->h_src.size() == b->n ? b->h_src[r] : -1;
+        cells.clear();
+        slots.clear();
+        for (int q = 0; q < 8 * per && !b->h_ridx.empty(); ++q) {
+            const size_t slot = (size_t)8 * r * per + q;
+            cells.push_back((int)(b->h_ridx[slot] - d.off));
+            slots.push_back((int)slot);
+        }
+        std::vector<std::vector<int2>> src, tap;
+        f2_tile_lists(d.Nx, d.Ny, d.Nz, TYO, 1, &sc, &r, (int)cells.size(), cells.data(),
+                      slots.data(), src, tap);
+        for (size_t t = 0; t < src.size(); ++t) {
+            std::stable_sort(src[t].begin(), src[t].end(), by_key);
+            std::stable_sort(tap[t].begin(), tap[t].end(), by_key);
+            if ((int)src[t].size() > F2_MAX_SRC || (int)tap[t].size() > F2_MAX_TAP) b->lists_ok = 0;
+            ss.push_back((int)sflat.size());
+            sflat.insert(sflat.end(), src[t].begin(), src[t].end());
+            ts.push_back((int)tflat.size());
+            tflat.insert(tflat.end(), tap[t].begin(), tap[t].end());
+        }
+    }
+    ss.push_back((int)sflat.size());
+    ts.push_back((int)tflat.size());
+    if (sflat.empty()) sflat.push_back(make_int2(-1, 0));
+    if (tflat.empty()) tflat.push_back(make_int2(-1, 0));
+    auto up = [&](void** dptr, const void* h, size_t bytes) -> hipError_t {
+        if (*dptr) { hipError_t e = hipFree(*dptr); *dptr = nullptr; if (e != hipSuccess) return e; }
+        hipError_t e = hipMalloc(dptr, bytes);
+        if (e != hipSuccess) return e;
+        return hipMemcpyAsync(*dptr, h, bytes, hipMemcpyHostToDevice, b->stream);
+    };
+    AFDTD_CHECK(up((void**)&b->d_ls_start, ss.data(), ss.size() * sizeof(int)));
+    AFDTD_CHECK(up((void**)&b->d_ls, sflat.data(), sflat.size() * sizeof(int2)));
+    AFDTD_CHECK(up((void**)&b->d_lt_start, ts.data(), ts.size() * sizeof(int)));
+    AFDTD_CHECK(up((void**)&b->d_lt, tflat.data(), tflat.size() * sizeof(int2)));
+    const size_t ntap = per > 0 ? (size_t)8 * b->n * per : 8;
+    if (!b->d_tapbuf || b->tapcap < ntap) {
+        dfree(b->d_tapbuf);
+        b->d_tapbuf = nullptr;
+        AFDTD_CHECK(hipMalloc(&b->d_tapbuf, ntap * sizeof(float)));
+        AFDTD_CHECK(hipMemsetAsync(b->d_tapbuf, 0, ntap * sizeof(float), b->stream));
+        b->tapcap = ntap;
+    }
+    AFDTD_CHECK(hipStreamSynchronize(b->stream));
+    b->lists_dirty = 0;
+    return FDTD_OK;
+}
+
+// Steps s and s+1 of every room in one two-step launch: the source and taps
+// of step s inside the sweep, those of step s+1 in the post-step launch.
+int enqueue_batch_pair(FdtdBatch* b) {
+    F2BatchLaunch L;
+    L.common = b->common;
+    L.Pin = b->P[b->cur];
+    L.Pout = b->P[b->cur ^ 1];
+    for (int c = 0; c < 3; ++c) {
+        L.Vin[c] = b->V[c];
+        L.Vout[c] = b->V2[c];
+    }
+    L.rooms = b->d_rooms;
+    L.tile_room = b->d_tile_room;
+    L.ntiles = b->ntiles;
+    L.seg = b->seg;
+    L.tile_y = b->tile_y;
+    L.storage = b->storage;
+    L.alpha = b->d_alpha;
+    L.pdamp = b->d_pdamp;
+    L.status = b->d_status;
+    L.src_start = b->d_ls_start;
+    L.src = b->d_ls;
+    L.tap_start = b->d_lt_start;
+    L.tap = b->d_lt;
+    L.psig = b->d_sig;
+    L.plen = b->d_src ? b->sig_len : 0;
+    L.step = b->step_count;
+    L.tapbuf = b->d_tapbuf;
+    L.stream = b->stream;
+    hipError_t e = launch_fused2_batch(L);
+    if (e != hipSuccess) { afdtd_set_error(hipGetErrorString(e)); return FDTD_ERR_DEVICE; }
+    b->cur ^= 1;
+    for (int c = 0; c < 3; ++c) std::swap(b->V[c], b->V2[c]);
+    int row = -1;
+    if (b->d_trace && b->trace_n < b->trace_cap) row = b->trace_n++;
+    e = launch_tap_trace(b->d_tapbuf, b->d_rwgt, b->n * b->per_room, b->d_trace, row,
+                         b->d_status, b->stream);
+    if (e != hipSuccess) { afdtd_set_error(hipGetErrorString(e)); return FDTD_ERR_DEVICE; }
+    row = -1;
+    if (b->d_trace && b->trace_n < b->trace_cap) row = b->trace_n++;
+    e = launch_batch_post(b, b->step_count + 1, row);
+    if (e != hipSuccess) { afdtd_set_error(hipGetErrorString(e)); return FDTD_ERR_DEVICE; }
+    b->step_count += 2;
+    return FDTD_OK;
+}
+
+// n steps: pairs while two or more remain and the two-step sweep applies.
+int enqueue_batch_steps(FdtdBatch* b, int n_steps) {
+    const bool pairs = pairs_active(b);
+    for (int s = 0; s < n_steps;) {
+        int rc;
+        if (pairs && s + 1 < n_steps) {
+            rc = enqueue_batch_pair(b);
+            s += 2;
+        } else {
+            rc = enqueue_batch_step(b);
+            s += 1;
+        }
+        if (rc) return rc;
+    }
+    return FDTD_OK;
+}
+
 // (Re)build the tile table for tile height tile_y, x-segment seg and the
 // dispatch order `schedule` (0: rooms in order; 1: largest rooms first, so
 // the long rooms start first and the tail of the launch holds small ones).
@@ -229,6 +372,12 @@ int build_tiles(FdtdBatch* b, int tile_y, int seg, int schedule) {
         d.tile_base = (int)tile_room.size();
         for (int q = 0; q < d.nty * d.ntz * d.nseg; ++q) tile_room.push_back(r);
     }
+    int lists = 0;
+    for (RoomDesc& d : b->rooms) {
+        d.list_base = lists;
+        lists += d.nty * d.ntz;
+    }
+    b->lists_dirty = 1;
     if (b->d_tile_room && (int)tile_room.size() != b->ntiles) {
         dfree(b->d_tile_room);
         b->d_tile_room = nullptr;
@@ -256,6 +405,8 @@ SK_API void fdtd_batch_destroy(FdtdBatch* b) {
     dfree(b->d_rooms); dfree(b->d_tile_room); dfree(b->d_alpha); dfree(b->d_pdamp);
     dfree(b->d_status); dfree(b->d_src); dfree(b->d_sig); dfree(b->d_ridx); dfree(b->d_rwgt);
     dfree(b->d_trace);
+    dfree(b->d_ls_start); dfree(b->d_ls); dfree(b->d_lt_start); dfree(b->d_lt);
+    dfree(b->d_tapbuf);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
 }
@@ -274,6 +425,9 @@ SK_API FdtdBatch* fdtd_batch_create(int n_rooms, const int* dims, float dx, floa
     }
     FdtdBatch* b = new FdtdBatch();
     b->n = n_rooms;
+    b->family = FDTD_KERNEL_AUTO;
+    b->lists_ok = 1;
+    b->lists_dirty = 1;
     b->order = stencil_order;
     b->storage = storage;
     b->esz = storage == FDTD_STORE_F32 ? 4 : 2;
@@ -372,11 +526,14 @@ SK_API int fdtd_batch_cells(const FdtdBatch* b, long long* total, int* n_tiles) 
     return FDTD_OK;
 }
 
-// Autotuner: time every (tile height, x-segment, dispatch order) candidate
-// with `trial_steps` graph-captured steps of the batch itself (zero fields,
-// sources and recording off, so the timing does not depend on the data),
-// keep the fastest and leave the batch at step 0 with zero fields.  Must be
-// called before the first step.
+// Autotuner: time every (sweep, tile height, x-segment, dispatch order)
+// candidate with `trial_steps` graph-captured steps of the batch itself
+// (zero fields, sources and recording off, so the timing does not depend on
+// the data; an even number for the two-step sweep), keep the fastest and
+// leave the batch at step 0 with zero fields.  The sweeps tried are the
+// single-step one (tile heights 8, 16, 32) and, for order 2, the two-step
+// one (8, 16), unless fdtd_batch_set_kernel fixed one.  Must be called
+// before the first step.
 SK_API int fdtd_batch_autotune(FdtdBatch* b, int trial_steps, int* best_tile_y, int* best_seg,
                                int* best_schedule, float* best_ms_per_step) {
     if (!b) return FDTD_ERR_NULL;
@@ -388,27 +545,38 @@ SK_API int fdtd_batch_autotune(FdtdBatch* b, int trial_steps, int* best_tile_y, 
     float* trace = b->d_trace;
     b->d_src = nullptr;
     b->d_trace = nullptr;
-    const int tys[3] = {8, 16, 32};
     const int segs[4] = {16, 32, 64, 128};
+    struct Cand { int family, tile_y; };
+    std::vector<Cand> cands;
+    const int fixed = b->family;
+    if (fixed != FDTD_KERNEL_FUSED2)
+        for (int ty : {8, 16, 32}) cands.push_back({FDTD_KERNEL_FUSED, ty});
+    if (b->order == 2 && fixed != FDTD_KERNEL_FUSED)
+        for (int ty : {8, 16}) cands.push_back({FDTD_KERNEL_FUSED2, ty});
     float best = 1e30f;
-    int bt = b->tile_y, bs = b->seg, bo = b->schedule;
+    int bt = b->tile_y, bs = b->seg, bo = b->schedule, bf = fixed;
     int rc = FDTD_OK;
-    for (int it = 0; it < 3 && rc == FDTD_OK; ++it)
+    for (size_t ic = 0; ic < cands.size() && rc == FDTD_OK; ++ic)
         for (int is = 0; is < 4 && rc == FDTD_OK; ++is)
             for (int o = 0; o < 2 && rc == FDTD_OK; ++o) {
-                rc = build_tiles(b, tys[it], segs[is], o);
+                b->family = cands[ic].family;
+                rc = build_tiles(b, cands[ic].tile_y, segs[is], o);
                 if (rc) break;
+                const int n = cands[ic].family == FDTD_KERNEL_FUSED2 ? trial_steps + (trial_steps & 1)
+                                                                      : trial_steps;
                 rc = fdtd_batch_step(b, 2, 0, nullptr);               // warm-up
                 float ms = 0.0f;
-                if (!rc) rc = fdtd_batch_step(b, trial_steps, 1, &ms);
-                if (!rc && ms / trial_steps < best) {
-                    best = ms / trial_steps;
-                    bt = tys[it]; bs = segs[is]; bo = o;
+                if (!rc) rc = fdtd_batch_step(b, n, 1, &ms);
+                if (!rc && ms / n < best) {
+                    best = ms / n;
+                    bt = cands[ic].tile_y; bs = segs[is]; bo = o; bf = cands[ic].family;
                 }
             }
     b->d_src = src;
     b->d_trace = trace;
-    if (rc) return rc;
+    if (rc) { b->family = fixed; return rc; }
+    b->family = fixed == FDTD_KERNEL_AUTO ? (bf == FDTD_KERNEL_FUSED2 ? FDTD_KERNEL_AUTO : FDTD_KERNEL_FUSED)
+                                          : fixed;
     rc = build_tiles(b, bt, bs, bo);
     if (rc) return rc;
     const size_t bytes = (size_t)b->total * b->esz;
@@ -446,6 +614,8 @@ SK_API int fdtd_batch_set_sources(FdtdBatch* b, const int* cell_idx, const float
     AFDTD_CHECK(hipMemcpy(b->d_sig, signals, sizeof(float) * (size_t)b->n * n_samples,
                           hipMemcpyHostToDevice));
     b->sig_len = n_samples;
+    b->h_src.assign(cell_idx, cell_idx + b->n);
+    b->lists_dirty = 1;
     return FDTD_OK;
 }
 
@@ -489,6 +659,8 @@ SK_API int fdtd_batch_set_receivers(FdtdBatch* b, int per_room, const float* xyz
     AFDTD_CHECK(hipMemcpy(b->d_ridx, idx.data(), sizeof(int) * idx.size(), hipMemcpyHostToDevice));
     AFDTD_CHECK(hipMemcpy(b->d_rwgt, wgt.data(), sizeof(float) * wgt.size(), hipMemcpyHostToDevice));
     b->per_room = per_room;
+    b->h_ridx = idx;
+    b->lists_dirty = 1;
     return FDTD_OK;
 }
 
@@ -509,6 +681,8 @@ SK_API int fdtd_batch_record(FdtdBatch* b, int max_steps) {
 SK_API int fdtd_batch_step(FdtdBatch* b, int n_steps, int use_graph, float* ms_total) {
     if (!b) return FDTD_ERR_NULL;
     if (n_steps <= 0) return FDTD_OK;
+    int rl = build_pair_lists(b);
+    if (rl) return rl;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (ms_total) {
         AFDTD_CHECK(hipEventCreate(&e0));
@@ -518,8 +692,7 @@ SK_API int fdtd_batch_step(FdtdBatch* b, int n_steps, int use_graph, float* ms_t
         hipGraph_t g = nullptr;
         hipGraphExec_t ge = nullptr;
         AFDTD_CHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-        int rc = FDTD_OK;
-        for (int s = 0; s < n_steps && rc == FDTD_OK; ++s) rc = enqueue_batch_step(b);
+        int rc = enqueue_batch_steps(b, n_steps);
         hipError_t ec = hipStreamEndCapture(b->stream, &g);
         if (rc != FDTD_OK) return rc;
         AFDTD_CHECK(ec);
@@ -532,10 +705,8 @@ SK_API int fdtd_batch_step(FdtdBatch* b, int n_steps, int use_graph, float* ms_t
         (void)hipGraphDestroy(g);
     } else {
         if (ms_total) AFDTD_CHECK(hipEventRecord(e0, b->stream));
-        for (int s = 0; s < n_steps; ++s) {
-            int rc = enqueue_batch_step(b);
-            if (rc) return rc;
-        }
+        int rc = enqueue_batch_steps(b, n_steps);
+        if (rc) return rc;
         if (ms_total) AFDTD_CHECK(hipEventRecord(e1, b->stream));
         AFDTD_CHECK(hipStreamSynchronize(b->stream));
     }
@@ -578,5 +749,30 @@ SK_API int fdtd_batch_get_pressure(const FdtdBatch* b, int room, float* out, int
 }
 
 SK_API int fdtd_batch_get_step_count(const FdtdBatch* b) { return b ? b->step_count : 0; }
+
+// FDTD_KERNEL_AUTO (two steps per launch where the two-step sweep applies),
+// FDTD_KERNEL_FUSED (single steps) or FDTD_KERNEL_FUSED2 (two-step sweep;
+// needs order 2 and tile height 8 or 16).
+SK_API int fdtd_batch_set_kernel(FdtdBatch* b, int family) {
+    if (!b) return FDTD_ERR_NULL;
+    if (family != FDTD_KERNEL_AUTO && family != FDTD_KERNEL_FUSED && family != FDTD_KERNEL_FUSED2) {
+        afdtd_set_error("fdtd_batch_set_kernel: unknown kernel family");
+        return FDTD_ERR_ARG;
+    }
+    if (family == FDTD_KERNEL_FUSED2 && (b->order != 2 || (b->tile_y != 8 && b->tile_y != 16))) {
+        afdtd_set_error("fdtd_batch_set_kernel: the two-step sweep needs order 2 and tile_y 8 or 16");
+        return FDTD_ERR_ARG;
+    }
+    b->family = family;
+    b->lists_dirty = 1;
+    return FDTD_OK;
+}
+
+// The sweep the next steps use: FDTD_KERNEL_FUSED2 or FDTD_KERNEL_FUSED.
+SK_API int fdtd_batch_get_kernel_family(FdtdBatch* b) {
+    if (!b) return FDTD_ERR_NULL;
+    if (build_pair_lists(b)) return FDTD_ERR_DEVICE;
+    return pairs_active(b) ? FDTD_KERNEL_FUSED2 : FDTD_KERNEL_FUSED;
+}
 
 }  // extern "C"

@@ -35,6 +35,8 @@
 // (the host schedules pairs so that it does not).
 #include "afdtd_fused_tile.cuh"
 
+#include <string.h>
+
 #include <algorithm>
 
 using fused::FusedPtrs;
@@ -61,8 +63,8 @@ struct Geo2 {
     static constexpr int EPL = EY * EC;
     static constexpr int NHL = 4 * GR;            // region halo-column elements per plane
 };
-constexpr int MAXS = 32;         // point-source entries cached per tile
-constexpr int MAXT = 256;        // receiver-tap entries cached per tile
+constexpr int MAXS = F2_MAX_SRC; // point-source entries cached per tile
+constexpr int MAXT = F2_MAX_TAP; // receiver-tap entries cached per tile
 // entry key: plane << 12 | extended row << 7 | (extended column + 1)
 __host__ __device__ __forceinline__ int ekey(int plane, int e, int c) {
     return (plane << 12) | (e << 7) | (c + 1);
@@ -137,20 +139,20 @@ struct Pair2Args {
     float* tapbuf;          // P^{n+1} at the receiver taps
 };
 
-// PML: the grid has an absorbing layer (PML = false compiles the layer's
-// damping out; the host picks the variant from n_pml).
+// One work item: the tile (ty, tz) of x-segment sg of the grid described by
+// p (dimensions, coefficients) and fp (field arrays); `tile` selects its
+// source and tap lists.  PML: the grid has an absorbing layer (PML = false
+// compiles the layer's damping out; the host picks the variant from n_pml).
 template <typename T, bool HET, int R, bool PML>
-__global__ void __launch_bounds__(f2::Geo2<R>::NT)
-__attribute__((amdgpu_waves_per_eu(HET ? AFDTD_F2_WPE : AFDTD_F2_WPE_HOMOG)))
-k_fused2(Pair2Args a) {
+__device__ __forceinline__ void fused2_tile(const Pair2Args& a, const StencilParams& p,
+                                            const FusedPtrs& fp, int ty, int tz, int sg,
+                                            int tile) {
     using namespace f2;
     typedef Geo2<R> G;
     constexpr int NWM = G::NWM, TYO = G::TYO, NT = G::NT, EY = G::EY, GPL = G::GPL,
                   EPL = G::EPL, NHL = G::NHL;
     typedef Store<T> S;
     typedef typename S::Raw RawT;
-    if (*a.status) return;
-
     __shared__ float sP[NSP * GPL];
     __shared__ float sB[HET ? NSB * GPL : 1];
     // Double-buffered by plane parity: one iteration writes the faces (or
@@ -171,22 +173,7 @@ k_fused2(Pair2Args a) {
     __shared__ int sSrcKey[MAXS], sTapKey[MAXT], sTapSlot[MAXT];
     __shared__ float sAmp[MAXS];
 
-    // Each XCD takes a contiguous range of work items (xcd_remap); within a
-    // segment the items run in bands of up to four z-tiles, z fastest, that
-    // sweep y, so the workgroups resident on one XCD at a time form a compact
-    // (y, z) block whose shared halo lines are read once into its L2.
-    int t = xcd_remap(blockIdx.x, gridDim.x);
-    const int per_sg = a.nty * a.ntz;
-    const int sg = t / per_sg;
-    t -= sg * per_sg;
-    const int band = t / (a.nty * 4);                   // bands of four z-tiles
-    const int gz = min(4, a.ntz - 4 * band);            // z-tiles in this band
-    t -= band * a.nty * 4;
-    const int ty = t / gz;
-    const int tz = 4 * band + t % gz;
-    const int tile = ty + a.nty * tz;
 
-    const StencilParams& p = a.p;
     const int Nx = p.Nx, Ny = p.Ny, Nz = p.Nz, sx = Ny * Nz;
     const int lane = threadIdx.x, w = __builtin_amdgcn_readfirstlane(threadIdx.y);
     const int tid = lane + 64 * w;
@@ -199,17 +186,17 @@ k_fused2(Pair2Args a) {
     const float isv = p.inv_sV, sv = p.sV;
     const bool loadXi = HET && p.impedance;
 
-    const T* __restrict__ Pin = (const T*)a.f.Pin;
-    T* __restrict__ Pout = (T*)a.f.Pout;
-    const T* __restrict__ Vxi = (const T*)a.f.Vin[0];
-    const T* __restrict__ Vyi = (const T*)a.f.Vin[1];
-    const T* __restrict__ Vzi = (const T*)a.f.Vin[2];
-    T* __restrict__ Vxo = (T*)a.f.Vout[0];
-    T* __restrict__ Vyo = (T*)a.f.Vout[1];
-    T* __restrict__ Vzo = (T*)a.f.Vout[2];
-    const float* __restrict__ bu = a.f.bu;
-    const float* __restrict__ kapA = a.f.kap;
-    const float* __restrict__ xiA = a.f.xi;
+    const T* __restrict__ Pin = (const T*)fp.Pin;
+    T* __restrict__ Pout = (T*)fp.Pout;
+    const T* __restrict__ Vxi = (const T*)fp.Vin[0];
+    const T* __restrict__ Vyi = (const T*)fp.Vin[1];
+    const T* __restrict__ Vzi = (const T*)fp.Vin[2];
+    T* __restrict__ Vxo = (T*)fp.Vout[0];
+    T* __restrict__ Vyo = (T*)fp.Vout[1];
+    T* __restrict__ Vzo = (T*)fp.Vout[2];
+    const float* __restrict__ bu = fp.bu;
+    const float* __restrict__ kapA = fp.kap;
+    const float* __restrict__ xiA = fp.xi;
 
     // ── helpers ──
     // Loads take a per-plane base (uniform; the plane index is clamped into the
@@ -779,6 +766,68 @@ k_fused2(Pair2Args a) {
     for (int q = qf; q <= i1 + 2; ++q) plane(q, oa, sa);
 }
 
+template <typename T, bool HET, int R, bool PML>
+__global__ void __launch_bounds__(f2::Geo2<R>::NT)
+__attribute__((amdgpu_waves_per_eu(HET ? AFDTD_F2_WPE : AFDTD_F2_WPE_HOMOG)))
+k_fused2(Pair2Args a) {
+    if (*a.status) return;
+    // Each XCD takes a contiguous range of work items (xcd_remap); within a
+    // segment the items run in bands of up to four z-tiles, z fastest, that
+    // sweep y, so the workgroups resident on one XCD at a time form a compact
+    // (y, z) block whose shared halo lines are read once into its L2.
+    int t = xcd_remap(blockIdx.x, gridDim.x);
+    const int per_sg = a.nty * a.ntz;
+    const int sg = t / per_sg;
+    t -= sg * per_sg;
+    const int band = t / (a.nty * 4);                   // bands of four z-tiles
+    const int gz = min(4, a.ntz - 4 * band);            // z-tiles in this band
+    t -= band * a.nty * 4;
+    const int ty = t / gz;
+    const int tz = 4 * band + t % gz;
+    const int tile = ty + a.nty * tz;
+    fused2_tile<T, HET, R, PML>(a, a.p, a.f, ty, tz, sg, tile);
+}
+
+// Batched rooms: work item t belongs to room tile_room[t]; the room's
+// dimensions, coefficients and field offset replace the shared ones.  Rooms
+// are homogeneous (no per-cell media); a room's work items run in y-fastest
+// order.
+template <typename T, int R, bool PML>
+__global__ void __launch_bounds__(f2::Geo2<R>::NT)
+__attribute__((amdgpu_waves_per_eu(AFDTD_F2_WPE_HOMOG)))
+k_fused2_batch(Pair2Args a, const RoomDesc* __restrict__ rooms, const int* __restrict__ tile_room) {
+    if (*a.status) return;
+    const int t = xcd_remap(blockIdx.x, gridDim.x);
+    const RoomDesc& d = rooms[tile_room[t]];
+    int local = t - d.tile_base;
+    const int ty = local % d.nty;
+    local /= d.nty;
+    const int tz = local % d.ntz;
+    const int sg = local / d.ntz;
+    StencilParams p = a.p;
+    p.Nx = d.Nx;
+    p.Ny = d.Ny;
+    p.Nz = d.Nz;
+    p.dt = d.dt;
+    p.vel_coeff = d.vel_coeff;
+    p.bulk_coeff = d.bulk_coeff;
+    for (int s = 0; s < 6; ++s) {
+        p.gside[s] = d.gside[s];
+        p.xterm[s] = d.xterm[s];
+    }
+    FusedPtrs f;
+    f.Pin = (const T*)a.f.Pin + d.off;
+    f.Pout = (T*)a.f.Pout + d.off;
+    for (int c = 0; c < 3; ++c) {
+        f.Vin[c] = (const T*)a.f.Vin[c] + d.off;
+        f.Vout[c] = (T*)a.f.Vout[c] + d.off;
+    }
+    f.bu = nullptr;
+    f.kap = nullptr;
+    f.xi = nullptr;
+    fused2_tile<T, false, R, PML>(a, p, f, ty, tz, sg, d.list_base + ty + d.nty * tz);
+}
+
 // Receiver traces of the intermediate step from the tap buffer, in the
 // post-step kernel's order (w0*p0 + w1*p1 + ... + w7*p7).
 __global__ void __launch_bounds__(256)
@@ -815,22 +864,25 @@ bool fused2_supported(const AcousticFDTDState* st) {
     return st->order == 2 && fused_supported(st);
 }
 
-// Per-tile source and tap tables (host-built, uploaded when sources or
-// receivers change).  Tile t = ty + nty*tz covers extended rows j0-1..j0+16
-// and columns k0-1..k0+64 of every plane.
-static int build_tables(AcousticFDTDState* st) {
+// Per-tile source and tap list entries of one grid (host side).  Tile
+// t = ty + nty*tz covers extended rows j0-1..j0+TYO and columns k0-1..k0+64
+// of every plane: a source goes to every tile whose stage A computes its cell
+// (main rows 0..TYO+1 x columns 0..63, halo cells rows 1..TYO x columns -1
+// and 64), a tap to the tile that owns its cell.
+void f2_tile_lists(int Nx, int Ny, int Nz, int TYO, int nsrc, const int* src_cell,
+                   const int* src_id, int ntap, const int* tap_cell, const int* tap_slot,
+                   std::vector<std::vector<int2>>& src, std::vector<std::vector<int2>>& tap) {
     using namespace f2;
-    const int TYO = st->f2_rows == 1 ? Geo2<1>::TYO : Geo2<2>::TYO;
+    (void)Nx;
     const int EY = TYO + 2;
-    const int nty = (st->Ny + TYO - 1) / TYO, ntz = (st->Nz + TZ - 1) / TZ;
-    const int nt = nty * ntz;
-    std::vector<std::vector<int2>> src(nt), tap(nt);
-    // sources: every tile whose stage A computes the cell (main rows 0..17 x
-    // columns 0..63, halo cells rows 1..16 x columns -1 and 64)
-    for (int s = 0; s < st->npsrc; ++s) {
-        const int c = st->h_pidx[s];
-        const int i = c / (st->Ny * st->Nz), rem = c % (st->Ny * st->Nz);
-        const int j = rem / st->Nz, kk = rem % st->Nz;
+    const int nty = (Ny + TYO - 1) / TYO, ntz = (Nz + TZ - 1) / TZ;
+    src.assign(nty * ntz, {});
+    tap.assign(nty * ntz, {});
+    for (int s = 0; s < nsrc; ++s) {
+        const int c = src_cell[s];
+        if (c < 0) continue;
+        const int i = c / (Ny * Nz), rem = c % (Ny * Nz);
+        const int j = rem / Nz, kk = rem % Nz;
         for (int ty = 0; ty < nty; ++ty) {
             const int e = j - (ty * TYO - 1);
             if (e < 0 || e >= EY) continue;
@@ -838,18 +890,32 @@ static int build_tables(AcousticFDTDState* st) {
                 const int cc = kk - tz * TZ;
                 const bool mainCell = cc >= 0 && cc < TZ;
                 const bool haloCell = (cc == -1 || cc == TZ) && e >= 1 && e <= TYO;
-                if (mainCell || haloCell) src[ty + nty * tz].push_back(make_int2(ekey(i, e, cc), s));
+                if (mainCell || haloCell) src[ty + nty * tz].push_back(make_int2(ekey(i, e, cc), src_id[s]));
             }
         }
     }
-    for (int r = 0; r < st->nrec; ++r)
-        for (int tt = 0; tt < 8; ++tt) {
-            const int c = st->h_ridx[8 * r + tt];
-            const int i = c / (st->Ny * st->Nz), rem = c % (st->Ny * st->Nz);
-            const int j = rem / st->Nz, kk = rem % st->Nz;
-            const int ty = j / TYO, tz = kk / TZ;
-            tap[ty + nty * tz].push_back(make_int2(ekey(i, j - (ty * TYO - 1), kk - tz * TZ), 8 * r + tt));
-        }
+    for (int q = 0; q < ntap; ++q) {
+        const int c = tap_cell[q];
+        const int i = c / (Ny * Nz), rem = c % (Ny * Nz);
+        const int j = rem / Nz, kk = rem % Nz;
+        const int ty = j / TYO, tz = kk / TZ;
+        tap[ty + nty * tz].push_back(make_int2(ekey(i, j - (ty * TYO - 1), kk - tz * TZ), tap_slot[q]));
+    }
+}
+
+int f2_tile_rows(int tile_y) { return tile_y == 8 ? f2::Geo2<1>::TYO : f2::Geo2<2>::TYO; }
+
+// Per-tile source and tap tables (uploaded when sources or receivers change).
+static int build_tables(AcousticFDTDState* st) {
+    using namespace f2;
+    const int TYO = st->f2_rows == 1 ? Geo2<1>::TYO : Geo2<2>::TYO;
+    const int nt = ((st->Ny + TYO - 1) / TYO) * ((st->Nz + TZ - 1) / TZ);
+    std::vector<std::vector<int2>> src, tap;
+    std::vector<int> sid(st->npsrc), tslot(8 * (size_t)st->nrec);
+    for (int s = 0; s < st->npsrc; ++s) sid[s] = s;
+    for (size_t q = 0; q < tslot.size(); ++q) tslot[q] = (int)q;
+    f2_tile_lists(st->Nx, st->Ny, st->Nz, TYO, st->npsrc, st->h_pidx.data(), sid.data(),
+                  8 * st->nrec, st->h_ridx.data(), tslot.data(), src, tap);
     auto by_key = [](const int2& x, const int2& y) { return x.x < y.x || (x.x == y.x && x.y < y.y); };
     std::vector<int> ss(nt + 1, 0), ts(nt + 1, 0);
     std::vector<int2> sflat, tflat;
@@ -951,4 +1017,59 @@ hipError_t launch_fused2_taps(AcousticFDTDState* st, int trace_row) {
                        st->d_f2_tapbuf, st->d_rwgt, st->nrec, st->d_trace, trace_row,
                        st->d_status);
     return hipGetLastError();
+}
+
+hipError_t launch_tap_trace(const float* tapbuf, const float* rwgt, int nrec, float* trace,
+                            int trace_row, const int* status, hipStream_t s) {
+    if (trace_row < 0 || nrec <= 0 || !trace) return hipSuccess;
+    hipLaunchKernelGGL(k_tap_trace, dim3((nrec + 255) / 256), dim3(256), 0, s, tapbuf, rwgt, nrec,
+                       trace, trace_row, status);
+    return hipGetLastError();
+}
+
+namespace {
+template <typename T, int R, bool PML>
+hipError_t launch_batch2(const F2BatchLaunch& L, const Pair2Args& a) {
+    hipLaunchKernelGGL((k_fused2_batch<T, R, PML>), dim3(L.ntiles), dim3(64, f2::Geo2<R>::NW), 0,
+                       L.stream, a, L.rooms, L.tile_room);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_batch2_t(const F2BatchLaunch& L, const Pair2Args& a) {
+    const bool pml = L.common.n_pml > 0;
+    if (L.tile_y == 8) return pml ? launch_batch2<T, 1, true>(L, a) : launch_batch2<T, 1, false>(L, a);
+    return pml ? launch_batch2<T, 2, true>(L, a) : launch_batch2<T, 2, false>(L, a);
+}
+}  // namespace
+
+hipError_t launch_fused2_batch(const F2BatchLaunch& L) {
+    Pair2Args a;
+    memset(&a, 0, sizeof(a));
+    a.p = L.common;
+    a.f.Pin = L.Pin;
+    a.f.Pout = L.Pout;
+    for (int c = 0; c < 3; ++c) {
+        a.f.Vin[c] = L.Vin[c];
+        a.f.Vout[c] = L.Vout[c];
+    }
+    a.f.bu = nullptr;
+    a.f.kap = nullptr;
+    a.f.xi = nullptr;
+    a.alpha = L.alpha;
+    a.pdamp = L.pdamp;
+    a.status = L.status;
+    a.seg = L.seg;
+    a.src_start = L.src_start;
+    a.src = L.src;
+    a.tap_start = L.tap_start;
+    a.tap = L.tap;
+    a.psig = L.psig;
+    a.plen = L.plen;
+    a.step = L.step;
+    a.tapbuf = L.tapbuf;
+    switch (L.storage) {
+        case FDTD_STORE_BF16: return launch_batch2_t<bf16_t>(L, a);
+        case FDTD_STORE_F16: return launch_batch2_t<f16_t>(L, a);
+        default: return launch_batch2_t<float>(L, a);
+    }
 }

@@ -159,6 +159,52 @@ hipError_t launch_fused2_taps(AcousticFDTDState* st, int trace_row);
 // changed; returns FDTD_OK and sets st->f2_ok.
 int fused2_prepare(AcousticFDTDState* st);
 bool fused2_supported(const AcousticFDTDState* st);
+// Trace row of `nrec` 8-tap receivers from a tap buffer (w0*p0 + ... + w7*p7).
+hipError_t launch_tap_trace(const float* tapbuf, const float* rwgt, int nrec, float* trace,
+                            int trace_row, const int* status, hipStream_t s);
+// Entries (key, id) of the per-tile source and tap lists of one grid for the
+// two-step sweep with TYO-row tiles: list t = ty + nty*tz, unsorted.
+void f2_tile_lists(int Nx, int Ny, int Nz, int TYO, int nsrc, const int* src_cell,
+                   const int* src_id, int ntap, const int* tap_cell, const int* tap_slot,
+                   std::vector<std::vector<int2>>& src, std::vector<std::vector<int2>>& tap);
+// Own rows of a two-step tile for tile_y 8 or 16.
+int f2_tile_rows(int tile_y);
+// Longest per-tile source / tap list the two-step sweep caches in LDS.
+constexpr int F2_MAX_SRC = 32, F2_MAX_TAP = 256;
+
+// One room of a batch (afdtd_batch.hip): its grid, its work items
+// [tile_base, tile_base + nty*ntz*nseg) and, for the two-step sweep, its
+// per-tile source / tap lists [list_base, list_base + nty*ntz).
+struct RoomDesc {
+    int Nx, Ny, Nz, nty, ntz, nseg, tile_base, list_base;
+    long long off;          // element offset of the room in every field array
+    float dt, vel_coeff, bulk_coeff;
+    float gside[6];
+    float xterm[6];         // wall terms (kap/2) * g / sqrt(rho K)
+};
+// Two steps of every room of a batch in one launch of the two-step sweep.
+struct F2BatchLaunch {
+    StencilParams common;   // shared parameters (per-room fields from RoomDesc)
+    const void* Pin;        // room-0 bases of the field arrays
+    void* Pout;
+    const void* Vin[3];
+    void* Vout[3];
+    const RoomDesc* rooms;
+    const int* tile_room;   // room of every work item
+    int ntiles, seg, tile_y, storage;
+    const float* alpha;
+    const float* pdamp;
+    const int* status;
+    const int* src_start;   // per list: entries [src_start[l], src_start[l+1])
+    const int2* src;        // (key, room), sorted by key
+    const int* tap_start;
+    const int2* tap;        // (key, tap slot), sorted by key
+    const float* psig;      // rooms x plen source samples
+    int plen, step;
+    float* tapbuf;
+    hipStream_t stream;
+};
+hipError_t launch_fused2_batch(const F2BatchLaunch& L);
 hipError_t launch_plate_step(AcousticFDTDState* st, int step_index);
 hipError_t launch_plate_inject(AcousticFDTDState* st);
 hipError_t launch_post_step(AcousticFDTDState* st, int step_index, int trace_row);
