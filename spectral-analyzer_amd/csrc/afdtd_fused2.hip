@@ -204,7 +204,7 @@ __device__ __forceinline__ void fused2_tile(const Pair2Args& a, const StencilPar
     // whose cell lies outside the grid).  A value loaded for an invalid plane
     // or lane only reaches faces and cells that are masked to zero or never
     // stored, so no per-load select is needed.  Stores use the same offsets
-    // (a stored cell is always a valid one).
+    // (see OOB below for cells outside the grid).
     const int esz = (int)sizeof(T);
     const uint32_t pbT = (uint32_t)sx * esz, pbF = (uint32_t)sx * 4;   // plane bytes
     auto cq = [&](int q) __attribute__((always_inline)) { return min(max(q, 0), Nx - 1); };
@@ -341,14 +341,19 @@ __device__ __forceinline__ void fused2_tile(const Pair2Args& a, const StencilPar
     //   main: oT[r] / oF[r] the cell (mj[r], k), xF[r] its wall sum (edges only)
     //   halo: oT[0] the cell (hj, hk), oT[1] (hj-1, hk), oT[2] (hj, hk-1) or the
     //         T1 face, oT[3] the T4 face; oF[0] / xF[0] the cell
-    // xD: added to xF on x-side planes, where every cell is a wall cell
+    // xD: added to xF on x-side planes, where every cell is a wall cell.
+    // oT of a cell outside the grid is OOB, past every plane's bytes: loads
+    // there return 0 and stores are dropped, so stores need no per-lane
+    // condition (faces of a valid cell that lie outside the grid are stored
+    // as 0 into their unused padding slots).
+    constexpr uint32_t OOB = 0x80000000u;
     uint32_t oT[4], oF[2], xF[2], xD[2];
     oT[1] = oF[1] = xF[1] = xD[1] = 0u;
     if (mainW) {
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const int c = mcell[r] ? moff[r] : 0;
-            oT[r] = (uint32_t)(c * esz);
+            oT[r] = mcell[r] ? (uint32_t)(c * esz) : OOB;
             oF[r] = (uint32_t)(c * 4);
             xF[r] = medge[r] ? oF[r] : 0u;
             xD[r] = oF[r] - xF[r];
@@ -360,7 +365,7 @@ __device__ __forceinline__ void fused2_tile(const Pair2Args& a, const StencilPar
         const int c2 = hT1 ? (cellv(t1j, k0 - 1) ? t1off : 0)
                            : ((hH && cellv(hj, hk - 1)) ? hoff - 1 : 0);
         const int c3 = cellv(t4j, k) ? t4off : 0;
-        oT[0] = (uint32_t)(c0 * esz);
+        oT[0] = hcell ? (uint32_t)(c0 * esz) : OOB;
         oT[1] = (uint32_t)(c1 * esz);
         oT[2] = (uint32_t)(c2 * esz);
         oT[3] = (uint32_t)(c3 * esz);
@@ -676,7 +681,7 @@ __device__ __forceinline__ void fused2_tile(const Pair2Args& a, const StencilPar
                         if (npml > 0) nv = nv * damp(depth3(qb, mj[r], k), depth3(qb + 1, mj[r], k));
                         nv = (fvb && mcell[r]) ? nv : 0.0f;
                         xb0[r] = nv;
-                        if (ownq && fvb && mcell[r]) stT(Vxo, qb, oT[r], nv, true);
+                        if (ownq && fvb) stT(Vxo, qb, oT[r], nv, true);
                     }
                     if (ownq) {
                         if (e <= TYO) {   // y-face between rows e and e+1
@@ -686,7 +691,7 @@ __device__ __forceinline__ void fused2_tile(const Pair2Args& a, const StencilPar
                             nv = (qbv && mvy[r]) ? nv : 0.0f;
                             yb0[r] = nv;
                             if (r == R - 1) sByW[(w + 1) * TZ + lane] = nv;
-                            if (ownr && qbv && mvy[r]) stT(Vyo, qb, oT[r], nv, true);
+                            if (ownr && qbv) stT(Vyo, qb, oT[r], nv, true);
                         }
                         if (ownr) {   // z-face between columns lane and lane+1
                             const float coef = HET ? (0.5f * (Bn(qb, e, lane) + Bn(qb, e, lane + 1))) * dtdx : vcoef;
@@ -694,7 +699,7 @@ __device__ __forceinline__ void fused2_tile(const Pair2Args& a, const StencilPar
                             if (npml > 0) nv = nv * damp(depth3(qb, mj[r], k), depth3(qb, mj[r], k + 1));
                             nv = (qbv && mvz[r]) ? nv : 0.0f;
                             zb0[r] = nv;
-                            if (qbv && mvz[r]) stT(Vzo, qb, oT[r], nv, true);
+                            if (qbv) stT(Vzo, qb, oT[r], nv, true);
                         }
                     }
                 }
@@ -720,7 +725,7 @@ __device__ __forceinline__ void fused2_tile(const Pair2Args& a, const StencilPar
                                + (zb1[r] - lane_left(zb1[r], sBzR[e]));
                 const float out = pres(pa2[r], dv, HET ? kb2[r] : kap0, xw2[r], qc, mj[r], k,
                                        medge[r] || (xec && mcell[r]));
-                if (mcell[r]) stT(Pout, qc, oT[r], out, false);
+                stT(Pout, qc, oT[r], out, false);
             }
         }
         __syncthreads();
