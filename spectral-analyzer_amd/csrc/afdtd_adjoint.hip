@@ -27,6 +27,7 @@
 #include <math.h>
 #include <string.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "afdtd_internal.h"
@@ -117,12 +118,20 @@ __global__ void k_fpost(R* P, int nsrc, const int* sidx, const R* sig, int slen,
     }
 }
 
-// Pb += R^T gy_n (one thread, fixed order: deterministic).
+// Pb += R^T gy_n: one thread per distinct tap cell, adding that cell's
+// contributions in receiver-then-tap order (the order of a sequential
+// loop over receivers and taps, so the sums are deterministic).
 template <typename R>
-__global__ void k_inject(R* Pb, int nrec, const int* ridx, const R* rwgt, const R* gy) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    for (int r = 0; r < nrec; ++r)
-        for (int t = 0; t < 8; ++t) Pb[ridx[8 * r + t]] += rwgt[8 * r + t] * gy[r];
+__global__ void __launch_bounds__(256)
+k_inject(R* __restrict__ Pb, int ncell, const int* __restrict__ cell,
+         const int* __restrict__ start, const int* __restrict__ rec, const R* __restrict__ w,
+         const R* __restrict__ gy) {
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= ncell) return;
+    const int c = cell[u];
+    R v = Pb[c];
+    for (int q = start[u]; q < start[u + 1]; ++q) v += w[q] * gy[rec[q]];
+    Pb[c] = v;
 }
 
 // Velocity-like adjoint sweep; also the per-cell (abar, ebar) and per-face
@@ -229,6 +238,10 @@ struct FdtdAdjoint {
     int nsrc, slen, nrec;
     int *d_sidx, *d_ridx;
     void *d_sig, *d_rwgt;
+    // receiver taps grouped by cell for the adjoint injection
+    int n_icell;
+    int *d_icell, *d_istart, *d_irec;
+    void* d_iw;
     int n_steps;
     const void *m, *l, *eta;       // parameters of the last forward (device, caller-owned)
 };
@@ -322,8 +335,9 @@ int run_backward(FdtdAdjoint* a, const R* gy, R* gm, R* gl, R* geta) {
         // reverse sweeps over the segment
         for (int j = len - 1; j >= 0; --j) {
             const int n = s0 + j;
-            hipLaunchKernelGGL(k_inject<R>, dim3(1), dim3(64), 0, a->stream, (R*)a->Pb, a->nrec,
-                               a->d_ridx, (const R*)a->d_rwgt, gy + (size_t)n * a->nrec);
+            hipLaunchKernelGGL(k_inject<R>, dim3((a->n_icell + 255) / 256), dim3(256), 0, a->stream,
+                               (R*)a->Pb, a->n_icell, a->d_icell, a->d_istart, a->d_irec,
+                               (const R*)a->d_iw, gy + (size_t)n * a->nrec);
             hipLaunchKernelGGL(k_avel<R>, grd, blk, 0, a->stream, g, (const R*)a->Pb, (R*)a->Vb[0],
                                (R*)a->Vb[1], (R*)a->Vb[2], (const R*)a->segP[j],
                                (const R*)a->segV[3 * j], (const R*)a->segV[3 * j + 1],
@@ -357,6 +371,7 @@ SK_API void fdtd_adjoint_destroy(FdtdAdjoint* a) {
     for (void* p : a->segP) afree(p);
     for (void* p : a->segV) afree(p);
     afree(a->d_sidx); afree(a->d_ridx); afree(a->d_sig); afree(a->d_rwgt);
+    afree(a->d_icell); afree(a->d_istart); afree(a->d_irec); afree(a->d_iw);
     if (a->stream) (void)hipStreamDestroy(a->stream);
     delete a;
 }
@@ -414,6 +429,35 @@ SK_API int fdtd_adjoint_set_receivers(FdtdAdjoint* a, int n_rec, const int* idx8
     AFDTD_CHECK(hipMemcpy(a->d_ridx, idx8, sizeof(int) * 8 * n_rec, hipMemcpyHostToDevice));
     AFDTD_CHECK(hipMemcpy(a->d_rwgt, wgt8, (size_t)a->rbytes * 8 * n_rec, hipMemcpyHostToDevice));
     a->nrec = n_rec;
+    // taps grouped by cell, each group in receiver-then-tap order
+    const int m = 8 * n_rec;
+    std::vector<int> ord(m);
+    for (int q = 0; q < m; ++q) ord[q] = q;
+    std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return idx8[x] < idx8[y]; });
+    std::vector<int> cell, start, rec(m);
+    std::vector<char> w((size_t)m * a->rbytes);
+    for (int k = 0; k < m; ++k) {
+        const int q = ord[k];
+        if (k == 0 || idx8[q] != idx8[ord[k - 1]]) {
+            cell.push_back(idx8[q]);
+            start.push_back(k);
+        }
+        rec[k] = q / 8;
+        memcpy(&w[(size_t)k * a->rbytes], (const char*)wgt8 + (size_t)q * a->rbytes, a->rbytes);
+    }
+    start.push_back(m);
+    afree(a->d_icell); afree(a->d_istart); afree(a->d_irec); afree(a->d_iw);
+    a->d_icell = a->d_istart = a->d_irec = nullptr;
+    a->d_iw = nullptr;
+    a->n_icell = (int)cell.size();
+    AFDTD_CHECK(hipMalloc(&a->d_icell, sizeof(int) * cell.size()));
+    AFDTD_CHECK(hipMalloc(&a->d_istart, sizeof(int) * start.size()));
+    AFDTD_CHECK(hipMalloc(&a->d_irec, sizeof(int) * m));
+    AFDTD_CHECK(hipMalloc(&a->d_iw, w.size()));
+    AFDTD_CHECK(hipMemcpy(a->d_icell, cell.data(), sizeof(int) * cell.size(), hipMemcpyHostToDevice));
+    AFDTD_CHECK(hipMemcpy(a->d_istart, start.data(), sizeof(int) * start.size(), hipMemcpyHostToDevice));
+    AFDTD_CHECK(hipMemcpy(a->d_irec, rec.data(), sizeof(int) * m, hipMemcpyHostToDevice));
+    AFDTD_CHECK(hipMemcpy(a->d_iw, w.data(), w.size(), hipMemcpyHostToDevice));
     return FDTD_OK;
 }
 
