@@ -111,10 +111,11 @@ __device__ __forceinline__ float lane_left(float v, float at0) {
 }  // namespace f2
 
 // Minimum waves per SIMD the register allocation must allow.  Without
-// per-cell media a workgroup needs under 52 KB of LDS, so two fit a CU when
-// their waves fit the register file too: two 10-wave workgroups can put six
-// waves on one SIMD, so the budget is 512 / 6 -> 80 VGPRs.  With per-cell
-// media (larger 1/rho ring) one workgroup per CU is the limit anyway.
+// per-cell media a workgroup needs 48 KB of LDS, and two share a CU (the
+// limit measured on MI355X is about 54 KB each) when their waves fit the
+// register file too: two 10-wave workgroups can put six waves on one SIMD,
+// so the budget is 512 / 6 -> 80 VGPRs.  With per-cell media (larger 1/rho
+// ring, 93 KB) one workgroup per CU is the limit anyway.
 #ifndef AFDTD_F2_WPE
 #define AFDTD_F2_WPE 1
 #endif
