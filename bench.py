@@ -291,13 +291,19 @@ def main():
         raise SystemExit("non-finite receiver traces")
     value = w.cells * a.steps / wall
     bpc = w.bytes_per_cell_update_for(a.storage)
-    # The two-step sweep ("fused2") advances two steps per launch and reads and
-    # writes every field once per launch: its algorithmic bytes per launch are
-    # those of one step.  ms_stencil is the average stencil time per step.
+    # Algorithmic bytes: B per cell-update (every field read and written once
+    # per step) times the cell-updates one launch performs.  The two-step
+    # sweep ("fused2") performs two steps per launch but moves each field
+    # through HBM only once, so its minimum HBM bytes per launch are half its
+    # algorithmic bytes; both are reported.  ms_stencil is the average
+    # stencil time per step (HIP events on the handle's stream).
     steps_per_launch = 2 if s.kernel_family == "fused2" else 1
     ms_launch = ms_stencil * steps_per_launch
-    achieved = w.cells * bpc / (ms_launch * 1e-3) / 1e9
-    effective = w.cells * bpc / (ms_stencil * 1e-3) / 1e9
+    alg_bytes_launch = steps_per_launch * w.cells * bpc
+    achieved = alg_bytes_launch / (ms_launch * 1e-3) / 1e9
+    once_bytes_launch = w.cells * bpc
+    once_rate = once_bytes_launch / (ms_launch * 1e-3) / 1e9
+    traffic = measured_traffic(w.name, s.kernel_family, a.storage)
     line = {
         "metric": "cell-updates/s (3D FDTD, 1 GPU)",
         "value": value,
@@ -321,16 +327,22 @@ def main():
                    "steps_per_launch": steps_per_launch},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(w.name, s.kernel_family, a.storage),
+                     "traffic": traffic,
                      "bytes_per_cell_update": bpc,
-                     "bytes_per_launch": w.cells * bpc,
+                     "cell_updates_per_launch": steps_per_launch * w.cells,
+                     "bytes_per_launch": alg_bytes_launch,
                      "stencil_ms_per_launch": ms_launch, "event_ms_total": ms_events,
-                     # cell-update rate times the one-step byte count: the HBM
-                     # rate a one-step-per-pass sweep would need for this speed
-                     "effective_GBps": effective,
-                     "effective_frac": effective / HBM_PEAK_GBS},
+                     # each field read and written once per launch: the least
+                     # HBM traffic a launch can have, and its rate
+                     "once_bytes_per_launch": once_bytes_launch,
+                     "once_GBps": once_rate,
+                     "once_frac": once_rate / HBM_PEAK_GBS,
+                     # measured HBM bytes (counters) over the launch time
+                     "traffic_GBps": (traffic / (ms_launch * 1e-3) / 1e9) if traffic else None},
     }
     del s
+    # rate of a plain streaming kernel with one step's read/write mix: above 1
+    # the path beats any sweep that passes over HBM once per step
     ceiling = stream_ceiling(w, a.storage)
     line["roofline"]["stream_ceiling_GBps"] = ceiling
     line["roofline"]["frac_of_stream_ceiling"] = achieved / ceiling
