@@ -10,6 +10,10 @@
   host memory at this size, so the two tilings check each other on the
   GPU), and scaling the state by 2 scales the result exactly (every
   operation of the scheme commutes with an exact power-of-two scale).
+* 1024 x 1024 x 512 room, 16-bit storage: receiver traces of BF16, FP16 and
+  bfp16 storage against FP32 storage on the full grid (600 steps: the pulse
+  reaches 49 of the 128 receivers), with bounds set by each format's
+  mantissa; the FP64 comparison runs on the crop (test_gpu_precision.py).
 * 256 batched rooms (281.9 M cells): the two-step batch equals the
   single-step batch bit for bit (traces of every room, fields of the first
   and last room), and the first room equals the oracle.
@@ -97,6 +101,39 @@ def test_room_1024x1024x512_two_step_equals_single_step_and_scales():
         nbad = _same_bits(a, b * np.float32(2.0))
         assert nbad == 0, f"scaled state {name}: {nbad} values differ from 2x"
     assert np.abs(out[("fused2", 1.0)][0]).max() > 0
+
+
+def test_room_1024x1024x512_16bit_storage_vs_f32():
+    from afdtd import AcousticFDTD, workloads as W
+    from conftest import rel_l2
+    steps = 600
+    w = W.config4(steps=steps)
+    Nx, Ny, Nz = w.Nx, w.Ny, w.Nz
+    ct = np.zeros(w.cells, np.uint8)
+    pa = np.zeros(Nx * Ny, np.uint8)
+    sig = w.signals(steps)
+    out = {}
+    for st in ("f32", "bf16", "f16", "bfp16"):
+        print(f"full-size room {Nx}x{Ny}x{Nz}, {steps} steps, storage {st}", flush=True)
+        g = AcousticFDTD(Nx, Ny, Nz, w.dx, c=343.0, rho_air=1.2, cell_type=ct, plate_iz=Nz // 2,
+                         plate_active=pa, n_pml=0, storage=st, **w.solver_kwargs())
+        g.set_point_sources(w.src, sig)
+        g.set_receivers(w.rec)
+        g.record_receivers(steps)
+        g.step(steps)
+        out[st] = (g.get_receiver_traces().astype(np.float64), float(sum(g.energy())))
+        g.close()
+        del g
+    ref, e_ref = out["f32"]
+    assert np.abs(ref).max() > 0 and e_ref > 0
+    bound = {"bfp16": 1e-3, "f16": 5e-3, "bf16": 5e-2}
+    for st, tol in bound.items():
+        err = rel_l2(out[st][0], ref)
+        de = abs(out[st][1] - e_ref) / e_ref
+        print(f"{st:6s} vs f32, full size: receiver rel-L2 {err:.3e}, energy rel. diff {de:.3e}",
+              flush=True)
+        assert err <= tol
+        assert de <= tol
 
 
 def test_batch_256_rooms_two_step_equals_single_step(oracle_mod):
