@@ -23,6 +23,17 @@
 // The forward stores checkpoints (P, V) every K steps; the backward recomputes
 // each K-step segment into HBM (P^n and V^{n+1/2} of every step) and runs the
 // reverse sweeps over it.
+//
+// Each step is one sweep in either direction: a thread owns cell c and the
+// three faces on its + side, and also evaluates the three faces on its -
+// side (owned by the neighbours) with the owners' expressions, so it has all
+// six faces of its divergence without a second launch.  The recomputed faces
+// are bit-identical to the owners' (same operands in the same order), and
+// the old/new fields are separate buffers (no thread reads a value another
+// thread of the same sweep writes).  The per-face accumulators betabar are
+// folded into one per-cell sum over the cell's six faces, which is what the
+// chain rule for log rho needs; abar is only accumulated on domain-edge cells,
+// the only cells whose gradient it reaches (xi = 0 elsewhere).
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <string.h>
@@ -35,6 +46,7 @@
 namespace {
 
 constexpr int BZ = 64, BY = 4;
+constexpr int FUSED_SRC_MAX = 16;   // sources added inside the forward sweep
 
 struct AdjGeom {
     int Nx, Ny, Nz;
@@ -62,21 +74,6 @@ k_coef(AdjGeom g, const R* __restrict__ m, const R* __restrict__ l, const R* __r
     E[c] = kp / ((R)1 + x);
 }
 
-// V_out = V_in - beta (G P) on the faces owned by each cell.
-template <typename R>
-__global__ void __launch_bounds__(BZ * BY)
-k_fvel(AdjGeom g, const R* __restrict__ P, const R* __restrict__ b, const R* Vxi, const R* Vyi,
-       const R* Vzi, R* Vxo, R* Vyo, R* Vzo) {
-    const int k = blockIdx.x * BZ + threadIdx.x, j = blockIdx.y * BY + threadIdx.y, i = blockIdx.z;
-    if (k >= g.Nz || j >= g.Ny) return;
-    const int sy = g.Nz, sx = g.Ny * g.Nz;
-    const int c = k + g.Nz * (j + g.Ny * i);
-    const R h = (R)0.5 * (R)g.dtdx;
-    Vxo[c] = i < g.Nx - 1 ? Vxi[c] - h * (b[c] + b[c + sx]) * (P[c + sx] - P[c]) : (R)0;
-    Vyo[c] = j < g.Ny - 1 ? Vyi[c] - h * (b[c] + b[c + sy]) * (P[c + sy] - P[c]) : (R)0;
-    Vzo[c] = k < g.Nz - 1 ? Vzi[c] - h * (b[c] + b[c + 1]) * (P[c + 1] - P[c]) : (R)0;
-}
-
 template <typename R>
 __device__ __forceinline__ R div_at(const R* Vx, const R* Vy, const R* Vz, int c, int i, int j,
                                     int k, const AdjGeom& g) {
@@ -91,19 +88,51 @@ __device__ __forceinline__ R div_at(const R* Vx, const R* Vy, const R* Vz, int c
     return d;
 }
 
-// P_out = a P_in - e (D V)
+// One forward step: V_out = V_in - beta (G P_in) on the cell's + faces
+// (written) and - faces (the owners' values, recomputed), then
+// P_out = a P_in - e (D V_out).  Faces that do not exist are written as 0.
 template <typename R>
 __global__ void __launch_bounds__(BZ * BY)
-k_fpres(AdjGeom g, const R* __restrict__ Pin, R* __restrict__ Pout, const R* __restrict__ Vx,
-        const R* __restrict__ Vy, const R* __restrict__ Vz, const R* __restrict__ A,
-        const R* __restrict__ E) {
+k_fstep(AdjGeom g, const R* __restrict__ P, const R* __restrict__ b, const R* __restrict__ Vxi,
+        const R* __restrict__ Vyi, const R* __restrict__ Vzi, R* __restrict__ Vxo,
+        R* __restrict__ Vyo, R* __restrict__ Vzo, R* __restrict__ Pout, const R* __restrict__ A,
+        const R* __restrict__ E, int nsrc, const int* __restrict__ sidx,
+        const R* __restrict__ sig, int slen, int n) {
     const int k = blockIdx.x * BZ + threadIdx.x, j = blockIdx.y * BY + threadIdx.y, i = blockIdx.z;
     if (k >= g.Nz || j >= g.Ny) return;
+    const int sy = g.Nz, sx = g.Ny * g.Nz;
     const int c = k + g.Nz * (j + g.Ny * i);
-    Pout[c] = A[c] * Pin[c] - E[c] * div_at(Vx, Vy, Vz, c, i, j, k, g);
+    const R h = (R)0.5 * (R)g.dtdx;
+    const R pc = P[c], bc = b[c];
+    // face between cells L and L + s: V_in - h (b_L + b_{L+s}) (P_{L+s} - P_L)
+    auto face = [&](const R* Vi, int L, int s) __attribute__((always_inline)) {
+        return Vi[L] - h * (b[L] + b[L + s]) * (P[L + s] - P[L]);
+    };
+    const bool xp = i < g.Nx - 1, yp = j < g.Ny - 1, zp = k < g.Nz - 1;
+    const R vxp = xp ? Vxi[c] - h * (bc + b[c + sx]) * (P[c + sx] - pc) : (R)0;
+    const R vyp = yp ? Vyi[c] - h * (bc + b[c + sy]) * (P[c + sy] - pc) : (R)0;
+    const R vzp = zp ? Vzi[c] - h * (bc + b[c + 1]) * (P[c + 1] - pc) : (R)0;
+    Vxo[c] = vxp;
+    Vyo[c] = vyp;
+    Vzo[c] = vzp;
+    R d = (R)0;
+    if (xp) d += vxp;
+    if (i > 0) d -= face(Vxi, c - sx, sx);
+    if (yp) d += vyp;
+    if (j > 0) d -= face(Vyi, c - sy, sy);
+    if (zp) d += vzp;
+    if (k > 0) d -= face(Vzi, c - 1, 1);
+    R out = A[c] * pc - E[c] * d;
+    // point sources of this step, added in source order (nsrc = 0: added
+    // by k_fpost instead)
+    if (n < slen)
+        for (int q = 0; q < nsrc; ++q)
+            if (sidx[q] == c) out += sig[(size_t)q * slen + n];
+    Pout[c] = out;
 }
 
-// Sources after the pressure update; receivers record y_n (forward only).
+// Sources after the pressure update (when k_fstep did not add them);
+// receivers record y_n (forward only).
 template <typename R>
 __global__ void k_fpost(R* P, int nsrc, const int* sidx, const R* sig, int slen, int n, int nrec,
                         const int* ridx, const R* rwgt, R* traces) {
@@ -134,68 +163,87 @@ k_inject(R* __restrict__ Pb, int ncell, const int* __restrict__ cell,
     Pb[c] = v;
 }
 
-// Velocity-like adjoint sweep; also the per-cell (abar, ebar) and per-face
-// (betabar) accumulations of this step.
+// One reverse step (after the receiver injection into Pb_in):
+//   Vb_out = Vb_in + G (e Pb_in) on the cell's + faces (written) and - faces
+//   (recomputed), Pb_out = a Pb_in + D (beta Vb_out), and the accumulations
+//   abar += Pb P^n (edge cells), ebar -= Pb (D V^{n+1/2}),
+//   fbar -= sum over the cell's faces of Vb_out (G P^n),
+// then adds the receiver injection R^T gy of the next reverse step into
+// Pb_out (gy null: none; cmap[c] = the cell's entry in the tap-cell lists or
+// -1).
 template <typename R>
 __global__ void __launch_bounds__(BZ * BY)
-k_avel(AdjGeom g, const R* __restrict__ Pb, R* __restrict__ Vbx, R* __restrict__ Vby,
-       R* __restrict__ Vbz, const R* __restrict__ Pn, const R* __restrict__ Vx,
-       const R* __restrict__ Vy, const R* __restrict__ Vz, const R* __restrict__ E,
-       R* __restrict__ acc_a, R* __restrict__ acc_e, R* __restrict__ acc_bx,
-       R* __restrict__ acc_by, R* __restrict__ acc_bz) {
-    const int k = blockIdx.x * BZ + threadIdx.x, j = blockIdx.y * BY + threadIdx.y, i = blockIdx.z;
-    if (k >= g.Nz || j >= g.Ny) return;
-    const int sy = g.Nz, sx = g.Ny * g.Nz;
-    const int c = k + g.Nz * (j + g.Ny * i);
-    const R pb = Pb[c];
-    acc_a[c] += pb * Pn[c];
-    acc_e[c] -= pb * div_at(Vx, Vy, Vz, c, i, j, k, g);
-    const R ep = E[c] * pb;
-    if (i < g.Nx - 1) {
-        const R v = Vbx[c] + (E[c + sx] * Pb[c + sx] - ep);
-        Vbx[c] = v;
-        acc_bx[c] -= v * (Pn[c + sx] - Pn[c]);
-    }
-    if (j < g.Ny - 1) {
-        const R v = Vby[c] + (E[c + sy] * Pb[c + sy] - ep);
-        Vby[c] = v;
-        acc_by[c] -= v * (Pn[c + sy] - Pn[c]);
-    }
-    if (k < g.Nz - 1) {
-        const R v = Vbz[c] + (E[c + 1] * Pb[c + 1] - ep);
-        Vbz[c] = v;
-        acc_bz[c] -= v * (Pn[c + 1] - Pn[c]);
-    }
-}
-
-// Pb = a Pb + D (beta Vb)
-template <typename R>
-__global__ void __launch_bounds__(BZ * BY)
-k_apres(AdjGeom g, R* __restrict__ Pb, const R* __restrict__ Vbx, const R* __restrict__ Vby,
-        const R* __restrict__ Vbz, const R* __restrict__ A, const R* __restrict__ b) {
+k_astep(AdjGeom g, const R* __restrict__ Pb, R* __restrict__ Pbo, const R* __restrict__ Vbxi,
+        const R* __restrict__ Vbyi, const R* __restrict__ Vbzi, R* __restrict__ Vbxo,
+        R* __restrict__ Vbyo, R* __restrict__ Vbzo, const R* __restrict__ Pn,
+        const R* __restrict__ Vx, const R* __restrict__ Vy, const R* __restrict__ Vz,
+        const R* __restrict__ A, const R* __restrict__ E, const R* __restrict__ b,
+        R* __restrict__ acc_a, R* __restrict__ acc_e, R* __restrict__ acc_f,
+        const int* __restrict__ cmap, const int* __restrict__ start, const int* __restrict__ rec,
+        const R* __restrict__ w, const R* __restrict__ gy) {
     const int k = blockIdx.x * BZ + threadIdx.x, j = blockIdx.y * BY + threadIdx.y, i = blockIdx.z;
     if (k >= g.Nz || j >= g.Ny) return;
     const int sy = g.Nz, sx = g.Ny * g.Nz;
     const int c = k + g.Nz * (j + g.Ny * i);
     const R h = (R)0.5 * (R)g.dtdx;
-    R d = (R)0;
-    if (i < g.Nx - 1) d += h * (b[c] + b[c + sx]) * Vbx[c];
-    if (i > 0) d -= h * (b[c - sx] + b[c]) * Vbx[c - sx];
-    if (j < g.Ny - 1) d += h * (b[c] + b[c + sy]) * Vby[c];
-    if (j > 0) d -= h * (b[c - sy] + b[c]) * Vby[c - sy];
-    if (k < g.Nz - 1) d += h * (b[c] + b[c + 1]) * Vbz[c];
-    if (k > 0) d -= h * (b[c - 1] + b[c]) * Vbz[c - 1];
-    Pb[c] = A[c] * Pb[c] + d;
+    const R pb = Pb[c], pnc = Pn[c], bc = b[c];
+    const bool edge = i == 0 || i == g.Nx - 1 || j == 0 || j == g.Ny - 1 || k == 0 || k == g.Nz - 1;
+    if (edge) acc_a[c] += pb * pnc;
+    acc_e[c] -= pb * div_at(Vx, Vy, Vz, c, i, j, k, g);
+    const R ep = E[c] * pb;
+    R f = acc_f[c], d = (R)0;
+    if (i < g.Nx - 1) {
+        const R v = Vbxi[c] + (E[c + sx] * Pb[c + sx] - ep);
+        Vbxo[c] = v;
+        f -= v * (Pn[c + sx] - pnc);
+        d += h * (bc + b[c + sx]) * v;
+    }
+    if (i > 0) {
+        const R v = Vbxi[c - sx] + (ep - E[c - sx] * Pb[c - sx]);
+        f -= v * (pnc - Pn[c - sx]);
+        d -= h * (b[c - sx] + bc) * v;
+    }
+    if (j < g.Ny - 1) {
+        const R v = Vbyi[c] + (E[c + sy] * Pb[c + sy] - ep);
+        Vbyo[c] = v;
+        f -= v * (Pn[c + sy] - pnc);
+        d += h * (bc + b[c + sy]) * v;
+    }
+    if (j > 0) {
+        const R v = Vbyi[c - sy] + (ep - E[c - sy] * Pb[c - sy]);
+        f -= v * (pnc - Pn[c - sy]);
+        d -= h * (b[c - sy] + bc) * v;
+    }
+    if (k < g.Nz - 1) {
+        const R v = Vbzi[c] + (E[c + 1] * Pb[c + 1] - ep);
+        Vbzo[c] = v;
+        f -= v * (Pn[c + 1] - pnc);
+        d += h * (bc + b[c + 1]) * v;
+    }
+    if (k > 0) {
+        const R v = Vbzi[c - 1] + (ep - E[c - 1] * Pb[c - 1]);
+        f -= v * (pnc - Pn[c - 1]);
+        d -= h * (b[c - 1] + bc) * v;
+    }
+    acc_f[c] = f;
+    R out = A[c] * pb + d;
+    // the next reverse step's receiver injection (gy of the step before; as
+    // k_inject: this cell's taps in receiver-then-tap order)
+    if (gy) {
+        const int u = cmap[c];
+        if (u >= 0)
+            for (int q = start[u]; q < start[u + 1]; ++q) out += w[q] * gy[rec[q]];
+    }
+    Pbo[c] = out;
 }
 
 // Chain rule from (abar, ebar, betabar) to (m, l, eta).
 template <typename R>
 __global__ void __launch_bounds__(BZ * BY)
-k_grad(AdjGeom g, const R* acc_a, const R* acc_e, const R* acc_bx, const R* acc_by,
-       const R* acc_bz, const R* b, const R* kap, const R* xi, const R* l, R* gm, R* gl, R* geta) {
+k_grad(AdjGeom g, const R* acc_a, const R* acc_e, const R* acc_f, const R* b, const R* kap,
+       const R* xi, const R* l, R* gm, R* gl, R* geta) {
     const int k = blockIdx.x * BZ + threadIdx.x, j = blockIdx.y * BY + threadIdx.y, i = blockIdx.z;
     if (k >= g.Nz || j >= g.Ny) return;
-    const int sy = g.Nz, sx = g.Ny * g.Nz;
     const int c = k + g.Nz * (j + g.Ny * i);
     const R x = xi[c], kp = kap[c];
     const R d1 = (R)1 + x;
@@ -203,14 +251,9 @@ k_grad(AdjGeom g, const R* acc_a, const R* acc_e, const R* acc_bx, const R* acc_
     const R kbar = acc_e[c] / d1;
     // d(kappa)/dl = kappa, d(xi)/dl = xi/2, d(xi)/dm = -xi/2
     gl[c] = kbar * kp + xbar * x * (R)0.5;
-    // beta_f = (b_L + b_R)/2 dt/dx, db/dm = -b
-    R fb = (R)0;
-    if (i < g.Nx - 1) fb += acc_bx[c];
-    if (i > 0) fb += acc_bx[c - sx];
-    if (j < g.Ny - 1) fb += acc_by[c];
-    if (j > 0) fb += acc_by[c - sy];
-    if (k < g.Nz - 1) fb += acc_bz[c];
-    if (k > 0) fb += acc_bz[c - 1];
+    // beta_f = (b_L + b_R)/2 dt/dx, db/dm = -b; fb: betabar summed over the
+    // cell's faces
+    const R fb = acc_f[c];
     gm[c] = -xbar * x * (R)0.5 - fb * (R)0.5 * (R)g.dtdx * b[c];
     const bool edge = i == 0 || i == g.Nx - 1 || j == 0 || j == g.Ny - 1 || k == 0 || k == g.Nz - 1;
     // d(xi)/d(eta) = (dt/(2 dx)) sqrt(K b)
@@ -222,25 +265,29 @@ k_grad(AdjGeom g, const R* acc_a, const R* acc_e, const R* acc_bx, const R* acc_
 struct FdtdAdjoint {
     int Nx, Ny, Nz, N, rbytes, ckpt_every;
     double dtdx;
-    hipStream_t stream;
+    hipStream_t stream;     // forward, reverse sweeps
+    hipStream_t stream2;    // segment recompute, overlapped with the reverse sweeps
+    hipEvent_t ev_seg[2];   // segment store s recomputed
+    hipEvent_t ev_rev[2];   // reverse sweeps done reading segment store s
     // coefficients
     void *b, *kap, *xi, *A, *E;
-    // working forward state (ping-pong P)
-    void *P[2], *V[3];
+    // working forward state (ping-pong P and V)
+    void *P[2], *V[2][3];
     // checkpoints (P, Vx, Vy, Vz per checkpoint)
     std::vector<void*> ck;
     int n_ck;
-    // segment store: P^n (K+1) and V^{n+1/2} (K) per component
-    std::vector<void*> segP, segV;
-    // adjoint fields and accumulators
-    void *Pb, *Vb[3], *acc[5];
+    // two segment stores (alternating by segment): P^n (K+1) and V^{n+1/2}
+    // (K) per component
+    std::vector<void*> segP[2], segV[2];
+    // adjoint fields (ping-pong) and accumulators abar, ebar, fbar
+    void *Pb[2], *Vb[2][3], *acc[3];
     // sources / receivers
     int nsrc, slen, nrec;
     int *d_sidx, *d_ridx;
     void *d_sig, *d_rwgt;
     // receiver taps grouped by cell for the adjoint injection
     int n_icell;
-    int *d_icell, *d_istart, *d_irec;
+    int *d_icell, *d_istart, *d_irec, *d_icmap;
     void* d_iw;
     int n_steps;
     const void *m, *l, *eta;       // parameters of the last forward (device, caller-owned)
@@ -258,7 +305,7 @@ int run_forward(FdtdAdjoint* a, const R* m, const R* l, const R* eta, int n_step
     hipLaunchKernelGGL(k_coef<R>, grd, blk, 0, a->stream, g, m, l, eta, (R*)a->b, (R*)a->kap,
                        (R*)a->xi, (R*)a->A, (R*)a->E);
     for (int q = 0; q < 2; ++q) AFDTD_CHECK(hipMemsetAsync(a->P[q], 0, fb, a->stream));
-    for (int q = 0; q < 3; ++q) AFDTD_CHECK(hipMemsetAsync(a->V[q], 0, fb, a->stream));
+    for (int q = 0; q < 3; ++q) AFDTD_CHECK(hipMemsetAsync(a->V[0][q], 0, fb, a->stream));
     const int K = a->ckpt_every;
     const int nck = (n_steps + K - 1) / K;
     // (re)allocate checkpoints
@@ -268,24 +315,25 @@ int run_forward(FdtdAdjoint* a, const R* m, const R* l, const R* eta, int n_step
         for (auto& p : a->ck) AFDTD_CHECK(hipMalloc(&p, fb));
         a->n_ck = nck;
     }
+    // few sources are added inside the sweep, many by k_fpost
+    const int fsrc = a->nsrc <= FUSED_SRC_MAX ? a->nsrc : 0, psrc = a->nsrc - fsrc;
     int cur = 0;
     for (int n = 0; n < n_steps; ++n) {
+        void* const* V = a->V[cur];
+        void* const* Vn = a->V[cur ^ 1];
         if (n % K == 0) {
             const int q = n / K;
             AFDTD_CHECK(hipMemcpyAsync(a->ck[4 * q], a->P[cur], fb, hipMemcpyDeviceToDevice, a->stream));
             for (int c = 0; c < 3; ++c)
-                AFDTD_CHECK(hipMemcpyAsync(a->ck[4 * q + 1 + c], a->V[c], fb, hipMemcpyDeviceToDevice,
+                AFDTD_CHECK(hipMemcpyAsync(a->ck[4 * q + 1 + c], V[c], fb, hipMemcpyDeviceToDevice,
                                            a->stream));
         }
-        R* P = (R*)a->P[cur];
         R* Pn = (R*)a->P[cur ^ 1];
-        R* Vx = (R*)a->V[0];
-        R* Vy = (R*)a->V[1];
-        R* Vz = (R*)a->V[2];
-        hipLaunchKernelGGL(k_fvel<R>, grd, blk, 0, a->stream, g, P, (const R*)a->b, Vx, Vy, Vz, Vx, Vy, Vz);
-        hipLaunchKernelGGL(k_fpres<R>, grd, blk, 0, a->stream, g, P, Pn, Vx, Vy, Vz, (const R*)a->A,
-                           (const R*)a->E);
-        hipLaunchKernelGGL(k_fpost<R>, dim3(1), dim3(64), 0, a->stream, Pn, a->nsrc, a->d_sidx,
+        hipLaunchKernelGGL(k_fstep<R>, grd, blk, 0, a->stream, g, (const R*)a->P[cur], (const R*)a->b,
+                           (const R*)V[0], (const R*)V[1], (const R*)V[2], (R*)Vn[0], (R*)Vn[1],
+                           (R*)Vn[2], Pn, (const R*)a->A, (const R*)a->E, fsrc, a->d_sidx,
+                           (const R*)a->d_sig, a->slen, n);
+        hipLaunchKernelGGL(k_fpost<R>, dim3(1), dim3(64), 0, a->stream, Pn, psrc, a->d_sidx,
                            (const R*)a->d_sig, a->slen, n, a->nrec, a->d_ridx, (const R*)a->d_rwgt,
                            traces);
         cur ^= 1;
@@ -303,54 +351,81 @@ int run_backward(FdtdAdjoint* a, const R* gy, R* gm, R* gl, R* geta) {
     dim3 blk(BZ, BY), grd((a->Nz + BZ - 1) / BZ, (a->Ny + BY - 1) / BY, a->Nx);
     const size_t fb = (size_t)a->N * sizeof(R);
     const int K = a->ckpt_every, n_steps = a->n_steps;
-    if (a->segP.empty()) {
-        a->segP.assign(K + 1, nullptr);
-        a->segV.assign((size_t)3 * K, nullptr);
-        for (auto& p : a->segP) AFDTD_CHECK(hipMalloc(&p, fb));
-        for (auto& p : a->segV) AFDTD_CHECK(hipMalloc(&p, fb));
-    }
-    AFDTD_CHECK(hipMemsetAsync(a->Pb, 0, fb, a->stream));
-    for (int c = 0; c < 3; ++c) AFDTD_CHECK(hipMemsetAsync(a->Vb[c], 0, fb, a->stream));
-    for (int c = 0; c < 5; ++c) AFDTD_CHECK(hipMemsetAsync(a->acc[c], 0, fb, a->stream));
-    const int nck = (n_steps + K - 1) / K;
-    for (int q = nck - 1; q >= 0; --q) {
-        const int s0 = q * K, len = std::min(K, n_steps - s0);
-        // recompute the segment: segP[j] = P^{s0+j}, segV[3j+c] = V^{s0+j+1/2}
-        AFDTD_CHECK(hipMemcpyAsync(a->segP[0], a->ck[4 * q], fb, hipMemcpyDeviceToDevice, a->stream));
-        for (int j = 0; j < len; ++j) {
-            const R* Vix = j ? (const R*)a->segV[3 * (j - 1)] : (const R*)a->ck[4 * q + 1];
-            const R* Viy = j ? (const R*)a->segV[3 * (j - 1) + 1] : (const R*)a->ck[4 * q + 2];
-            const R* Viz = j ? (const R*)a->segV[3 * (j - 1) + 2] : (const R*)a->ck[4 * q + 3];
-            R* Vox = (R*)a->segV[3 * j];
-            R* Voy = (R*)a->segV[3 * j + 1];
-            R* Voz = (R*)a->segV[3 * j + 2];
-            hipLaunchKernelGGL(k_fvel<R>, grd, blk, 0, a->stream, g, (const R*)a->segP[j],
-                               (const R*)a->b, Vix, Viy, Viz, Vox, Voy, Voz);
-            hipLaunchKernelGGL(k_fpres<R>, grd, blk, 0, a->stream, g, (const R*)a->segP[j],
-                               (R*)a->segP[j + 1], Vox, Voy, Voz, (const R*)a->A, (const R*)a->E);
-            hipLaunchKernelGGL(k_fpost<R>, dim3(1), dim3(64), 0, a->stream, (R*)a->segP[j + 1],
-                               a->nsrc, a->d_sidx, (const R*)a->d_sig, a->slen, s0 + j, a->nrec,
-                               a->d_ridx, (const R*)a->d_rwgt, (R*)nullptr);
+    if (a->segP[0].empty()) {
+        for (int u = 0; u < 2; ++u) {
+            a->segP[u].assign(K + 1, nullptr);
+            a->segV[u].assign((size_t)3 * K, nullptr);
+            for (auto& p : a->segP[u]) AFDTD_CHECK(hipMalloc(&p, fb));
+            for (auto& p : a->segV[u]) AFDTD_CHECK(hipMalloc(&p, fb));
         }
-        // reverse sweeps over the segment
+    }
+    // both Vb buffers start at 0: faces that do not exist are never written
+    AFDTD_CHECK(hipMemsetAsync(a->Pb[0], 0, fb, a->stream));
+    for (int u = 0; u < 2; ++u)
+        for (int c = 0; c < 3; ++c) AFDTD_CHECK(hipMemsetAsync(a->Vb[u][c], 0, fb, a->stream));
+    for (int c = 0; c < 3; ++c) AFDTD_CHECK(hipMemsetAsync(a->acc[c], 0, fb, a->stream));
+    const int nck = (n_steps + K - 1) / K;
+    const int fsrc = a->nsrc <= FUSED_SRC_MAX ? a->nsrc : 0;
+    // Segment q is recomputed into store q % 2 on stream2 while the reverse
+    // sweeps of segment q + 1 run on stream over the other store.
+    auto recompute = [&](int q) -> int {
+        const int u = q & 1, s0 = q * K, len = std::min(K, n_steps - s0);
+        std::vector<void*>& sP = a->segP[u];
+        std::vector<void*>& sV = a->segV[u];
+        if (q + 2 < nck) AFDTD_CHECK(hipStreamWaitEvent(a->stream2, a->ev_rev[u], 0));
+        // segP[j] = P^{s0+j}, segV[3j+c] = V^{s0+j+1/2}
+        AFDTD_CHECK(hipMemcpyAsync(sP[0], a->ck[4 * q], fb, hipMemcpyDeviceToDevice, a->stream2));
+        for (int j = 0; j < len; ++j) {
+            const R* Vix = j ? (const R*)sV[3 * (j - 1)] : (const R*)a->ck[4 * q + 1];
+            const R* Viy = j ? (const R*)sV[3 * (j - 1) + 1] : (const R*)a->ck[4 * q + 2];
+            const R* Viz = j ? (const R*)sV[3 * (j - 1) + 2] : (const R*)a->ck[4 * q + 3];
+            hipLaunchKernelGGL(k_fstep<R>, grd, blk, 0, a->stream2, g, (const R*)sP[j],
+                               (const R*)a->b, Vix, Viy, Viz, (R*)sV[3 * j], (R*)sV[3 * j + 1],
+                               (R*)sV[3 * j + 2], (R*)sP[j + 1], (const R*)a->A, (const R*)a->E,
+                               fsrc, a->d_sidx, (const R*)a->d_sig, a->slen, s0 + j);
+            if (a->nsrc > fsrc)
+                hipLaunchKernelGGL(k_fpost<R>, dim3(1), dim3(64), 0, a->stream2, (R*)sP[j + 1],
+                                   a->nsrc, a->d_sidx, (const R*)a->d_sig, a->slen, s0 + j, a->nrec,
+                                   a->d_ridx, (const R*)a->d_rwgt, (R*)nullptr);
+        }
+        AFDTD_CHECK(hipEventRecord(a->ev_seg[u], a->stream2));
+        return FDTD_OK;
+    };
+    int cur = 0;
+    // receiver injection of the last step; each reverse sweep adds the next one
+    hipLaunchKernelGGL(k_inject<R>, dim3((a->n_icell + 255) / 256), dim3(256), 0, a->stream,
+                       (R*)a->Pb[0], a->n_icell, a->d_icell, a->d_istart, a->d_irec,
+                       (const R*)a->d_iw, gy + (size_t)(n_steps - 1) * a->nrec);
+    int rc = recompute(nck - 1);
+    if (rc != FDTD_OK) return rc;
+    for (int q = nck - 1; q >= 0; --q) {
+        const int u = q & 1, s0 = q * K, len = std::min(K, n_steps - s0);
+        if (q > 0 && (rc = recompute(q - 1)) != FDTD_OK) return rc;
+        AFDTD_CHECK(hipStreamWaitEvent(a->stream, a->ev_seg[u], 0));
+        std::vector<void*>& sP = a->segP[u];
+        std::vector<void*>& sV = a->segV[u];
         for (int j = len - 1; j >= 0; --j) {
             const int n = s0 + j;
-            hipLaunchKernelGGL(k_inject<R>, dim3((a->n_icell + 255) / 256), dim3(256), 0, a->stream,
-                               (R*)a->Pb, a->n_icell, a->d_icell, a->d_istart, a->d_irec,
-                               (const R*)a->d_iw, gy + (size_t)n * a->nrec);
-            hipLaunchKernelGGL(k_avel<R>, grd, blk, 0, a->stream, g, (const R*)a->Pb, (R*)a->Vb[0],
-                               (R*)a->Vb[1], (R*)a->Vb[2], (const R*)a->segP[j],
-                               (const R*)a->segV[3 * j], (const R*)a->segV[3 * j + 1],
-                               (const R*)a->segV[3 * j + 2], (const R*)a->E, (R*)a->acc[0],
-                               (R*)a->acc[1], (R*)a->acc[2], (R*)a->acc[3], (R*)a->acc[4]);
-            hipLaunchKernelGGL(k_apres<R>, grd, blk, 0, a->stream, g, (R*)a->Pb, (const R*)a->Vb[0],
-                               (const R*)a->Vb[1], (const R*)a->Vb[2], (const R*)a->A, (const R*)a->b);
+            const R* Pb = (const R*)a->Pb[cur];
+            void* const* Vb = a->Vb[cur];
+            void* const* Vbn = a->Vb[cur ^ 1];
+            const R* gyn = n > 0 ? gy + (size_t)(n - 1) * a->nrec : nullptr;
+            hipLaunchKernelGGL(k_astep<R>, grd, blk, 0, a->stream, g, Pb, (R*)a->Pb[cur ^ 1],
+                               (const R*)Vb[0], (const R*)Vb[1], (const R*)Vb[2], (R*)Vbn[0],
+                               (R*)Vbn[1], (R*)Vbn[2], (const R*)sP[j], (const R*)sV[3 * j],
+                               (const R*)sV[3 * j + 1], (const R*)sV[3 * j + 2], (const R*)a->A,
+                               (const R*)a->E, (const R*)a->b, (R*)a->acc[0], (R*)a->acc[1],
+                               (R*)a->acc[2], a->d_icmap, a->d_istart, a->d_irec,
+                               (const R*)a->d_iw, gyn);
+            cur ^= 1;
         }
+        AFDTD_CHECK(hipEventRecord(a->ev_rev[u], a->stream));
     }
     hipLaunchKernelGGL(k_grad<R>, grd, blk, 0, a->stream, g, (const R*)a->acc[0], (const R*)a->acc[1],
-                       (const R*)a->acc[2], (const R*)a->acc[3], (const R*)a->acc[4], (const R*)a->b,
-                       (const R*)a->kap, (const R*)a->xi, (const R*)a->l, gm, gl, geta);
+                       (const R*)a->acc[2], (const R*)a->b, (const R*)a->kap, (const R*)a->xi,
+                       (const R*)a->l, gm, gl, geta);
     AFDTD_CHECK(hipGetLastError());
+    AFDTD_CHECK(hipStreamSynchronize(a->stream2));
     AFDTD_CHECK(hipStreamSynchronize(a->stream));
     return FDTD_OK;
 }
@@ -363,15 +438,25 @@ SK_API void fdtd_adjoint_destroy(FdtdAdjoint* a) {
     if (!a) return;
     if (a->stream) (void)hipStreamSynchronize(a->stream);
     afree(a->b); afree(a->kap); afree(a->xi); afree(a->A); afree(a->E);
-    afree(a->P[0]); afree(a->P[1]);
-    for (int c = 0; c < 3; ++c) { afree(a->V[c]); afree(a->Vb[c]); }
-    afree(a->Pb);
-    for (int c = 0; c < 5; ++c) afree(a->acc[c]);
+    for (int u = 0; u < 2; ++u) {
+        afree(a->P[u]);
+        afree(a->Pb[u]);
+        for (int c = 0; c < 3; ++c) { afree(a->V[u][c]); afree(a->Vb[u][c]); }
+    }
+    for (int c = 0; c < 3; ++c) afree(a->acc[c]);
     for (void* p : a->ck) afree(p);
-    for (void* p : a->segP) afree(p);
-    for (void* p : a->segV) afree(p);
+    for (int u = 0; u < 2; ++u) {
+        for (void* p : a->segP[u]) afree(p);
+        for (void* p : a->segV[u]) afree(p);
+    }
     afree(a->d_sidx); afree(a->d_ridx); afree(a->d_sig); afree(a->d_rwgt);
-    afree(a->d_icell); afree(a->d_istart); afree(a->d_irec); afree(a->d_iw);
+    afree(a->d_icell); afree(a->d_istart); afree(a->d_irec); afree(a->d_iw); afree(a->d_icmap);
+    if (a->stream2) (void)hipStreamSynchronize(a->stream2);
+    for (int u = 0; u < 2; ++u) {
+        if (a->ev_seg[u]) (void)hipEventDestroy(a->ev_seg[u]);
+        if (a->ev_rev[u]) (void)hipEventDestroy(a->ev_rev[u]);
+    }
+    if (a->stream2) (void)hipStreamDestroy(a->stream2);
     if (a->stream) (void)hipStreamDestroy(a->stream);
     delete a;
 }
@@ -388,9 +473,15 @@ SK_API FdtdAdjoint* fdtd_adjoint_create(int Nx, int Ny, int Nz, double dtdx, int
     a->dtdx = dtdx;
     const size_t fb = (size_t)a->N * precision_bytes;
     bool ok = hipStreamCreateWithFlags(&a->stream, hipStreamNonBlocking) == hipSuccess;
-    void** all[] = {&a->b, &a->kap, &a->xi, &a->A, &a->E, &a->P[0], &a->P[1], &a->V[0], &a->V[1],
-                    &a->V[2], &a->Pb, &a->Vb[0], &a->Vb[1], &a->Vb[2], &a->acc[0], &a->acc[1],
-                    &a->acc[2], &a->acc[3], &a->acc[4]};
+    ok = ok && hipStreamCreateWithFlags(&a->stream2, hipStreamNonBlocking) == hipSuccess;
+    for (int u = 0; u < 2; ++u) {
+        ok = ok && hipEventCreateWithFlags(&a->ev_seg[u], hipEventDisableTiming) == hipSuccess;
+        ok = ok && hipEventCreateWithFlags(&a->ev_rev[u], hipEventDisableTiming) == hipSuccess;
+    }
+    void** all[] = {&a->b, &a->kap, &a->xi, &a->A, &a->E, &a->P[0], &a->P[1], &a->V[0][0],
+                    &a->V[0][1], &a->V[0][2], &a->V[1][0], &a->V[1][1], &a->V[1][2], &a->Pb[0],
+                    &a->Pb[1], &a->Vb[0][0], &a->Vb[0][1], &a->Vb[0][2], &a->Vb[1][0],
+                    &a->Vb[1][1], &a->Vb[1][2], &a->acc[0], &a->acc[1], &a->acc[2]};
     for (void** p : all) ok = ok && hipMalloc(p, fb) == hipSuccess;
     if (!ok) {
         afdtd_set_error("fdtd_adjoint_create: device allocation failed");
@@ -446,10 +537,14 @@ SK_API int fdtd_adjoint_set_receivers(FdtdAdjoint* a, int n_rec, const int* idx8
         memcpy(&w[(size_t)k * a->rbytes], (const char*)wgt8 + (size_t)q * a->rbytes, a->rbytes);
     }
     start.push_back(m);
-    afree(a->d_icell); afree(a->d_istart); afree(a->d_irec); afree(a->d_iw);
-    a->d_icell = a->d_istart = a->d_irec = nullptr;
+    afree(a->d_icell); afree(a->d_istart); afree(a->d_irec); afree(a->d_iw); afree(a->d_icmap);
+    a->d_icell = a->d_istart = a->d_irec = a->d_icmap = nullptr;
     a->d_iw = nullptr;
     a->n_icell = (int)cell.size();
+    std::vector<int> cmap((size_t)a->N, -1);
+    for (int u = 0; u < a->n_icell; ++u) cmap[cell[u]] = u;
+    AFDTD_CHECK(hipMalloc(&a->d_icmap, sizeof(int) * cmap.size()));
+    AFDTD_CHECK(hipMemcpy(a->d_icmap, cmap.data(), sizeof(int) * cmap.size(), hipMemcpyHostToDevice));
     AFDTD_CHECK(hipMalloc(&a->d_icell, sizeof(int) * cell.size()));
     AFDTD_CHECK(hipMalloc(&a->d_istart, sizeof(int) * start.size()));
     AFDTD_CHECK(hipMalloc(&a->d_irec, sizeof(int) * m));
