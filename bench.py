@@ -196,7 +196,9 @@ def bench_adjoint(a):
     shape, steps = w["shape"], w["steps"]
     N = int(np.prod(shape))
     res = {}
-    for dtype, tdt in (("float32", torch.float32), ("float64", torch.float64)):
+    dtypes = (("float32", torch.float32),) if a.no_fd else (("float32", torch.float32),
+                                                             ("float64", torch.float64))
+    for dtype, tdt in dtypes:
         s = AdjointSolver(*shape, w["dtdx"], dtype=dtype, checkpoint_every=100)
         s.set_sources(w["src"], w["signals"])
         s.set_receivers(w["rec"])
@@ -218,6 +220,15 @@ def bench_adjoint(a):
         t2 = time.perf_counter()
         res[dtype] = dict(solver=s, grads=[g.grad.double().cpu().numpy() for g in (mg, lg, eg)],
                           fwd=t1 - t0, bwd=t2 - t1, dev=dev, target=target)
+    bwd = res["float32"]["bwd"]
+    line = {"metric": "adjoint reverse-step cell-updates/s (1 GPU)",
+            "value": N * steps / bwd, "unit": "cell-updates/s", "n_gpus": 1, "steps": steps,
+            "higher_is_better": True, "dtype": "fp32", "data": "synthetic (seeded media)",
+            "config": {"workload": "config5-128cube-2000steps-ckpt100", "cells": N},
+            "forward_s_fp32": res["float32"]["fwd"], "backward_s_fp32": bwd}
+    if a.no_fd:
+        print(json.dumps(line), flush=True)
+        return 0
     # central differences (float64) on 64 random cells
     r64 = res["float64"]
     s, dev, target = r64["solver"], r64["dev"], r64["target"]
@@ -239,13 +250,8 @@ def bench_adjoint(a):
         worst = max(worst, abs(g[c] - fd) / max(abs(fd), 1e-3 * np.abs(g).max()))
     g32, g64 = res["float32"]["grads"], res["float64"]["grads"]
     rel32 = [float(np.linalg.norm(x - y) / np.linalg.norm(y)) for x, y in zip(g32, g64)]
-    bwd = res["float32"]["bwd"]
-    line = {"metric": "adjoint reverse-step cell-updates/s (1 GPU)",
-            "value": N * steps / bwd, "unit": "cell-updates/s", "n_gpus": 1, "steps": steps,
-            "higher_is_better": True, "dtype": "fp32", "data": "synthetic (seeded media)",
-            "config": {"workload": "config5-128cube-2000steps-ckpt100", "cells": N},
-            "forward_s_fp32": res["float32"]["fwd"], "backward_s_fp32": bwd,
-            "fd_worst_rel_err_fp64_64cells": worst, "fp32_vs_fp64_grad_rel_l2": rel32}
+    line["fd_worst_rel_err_fp64_64cells"] = worst
+    line["fp32_vs_fp64_grad_rel_l2"] = rel32
     print(json.dumps(line), flush=True)
     return 0
 
@@ -262,7 +268,13 @@ def main():
     ap.add_argument("--tile-y", type=int, default=0)
     ap.add_argument("--seg-x", type=int, default=0)
     ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--no-fd", action="store_true",
+                    help="adjoint: float32 timing only (no float64 pass, no finite differences)")
     a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py measures one MI355X: the multi-GPU (slab-sharded) path is a "
+              "design only in this build; run with --gpus 1", file=sys.stderr)
+        return 2
     if a.copy_calibration:
         copy_calibration()
         return 0
@@ -274,10 +286,6 @@ def main():
         if a.steps == 4000:
             a.steps = 2000
         return bench_adjoint(a)
-    if a.gpus != 1:
-        print("bench.py measures one MI355X: the multi-GPU (slab-sharded) path is a "
-              "design only in this build; run with --gpus 1", file=sys.stderr)
-        return 2
     w = make_workload(a.config, a.warmup + a.steps)
     s = make_solver(w, a.storage, a.kernel, a.tile_y, a.seg_x)
     total = a.warmup + a.steps

@@ -46,6 +46,23 @@
 namespace {
 
 constexpr int BZ = 64, BY = 4;
+
+// Cell of this thread in the one-dimensional launches of the two step
+// sweeps.  Consecutive workgroups go to the eight XCDs in turn; the remap
+// gives each XCD a contiguous range of (z-block, y-block, x-plane) tiles in
+// plane order, so the +-x neighbours a tile reads were fetched into the same
+// XCD's L2 by the tiles of the planes next to it.
+struct Cell3 {
+    int i, j, k;
+};
+__device__ __forceinline__ Cell3 sweep_cell(int Ny, int Nz) {
+    const int nb = (int)gridDim.x, b = (int)blockIdx.x;
+    const int x = b & 7, per = nb >> 3, rem = nb & 7;
+    const int L = (x < rem ? x * (per + 1) : rem * (per + 1) + (x - rem) * per) + (b >> 3);
+    const int gz = (Nz + BZ - 1) / BZ, gy = (Ny + BY - 1) / BY;
+    const int bz = L % gz, t = L / gz;
+    return Cell3{t / gy, (t % gy) * BY + (int)threadIdx.y, bz * BZ + (int)threadIdx.x};
+}
 constexpr int FUSED_SRC_MAX = 16;   // sources added inside the forward sweep
 
 struct AdjGeom {
@@ -98,7 +115,8 @@ k_fstep(AdjGeom g, const R* __restrict__ P, const R* __restrict__ b, const R* __
         R* __restrict__ Vyo, R* __restrict__ Vzo, R* __restrict__ Pout, const R* __restrict__ A,
         const R* __restrict__ E, int nsrc, const int* __restrict__ sidx,
         const R* __restrict__ sig, int slen, int n) {
-    const int k = blockIdx.x * BZ + threadIdx.x, j = blockIdx.y * BY + threadIdx.y, i = blockIdx.z;
+    const Cell3 cl = sweep_cell(g.Ny, g.Nz);
+    const int i = cl.i, j = cl.j, k = cl.k;
     if (k >= g.Nz || j >= g.Ny) return;
     const int sy = g.Nz, sx = g.Ny * g.Nz;
     const int c = k + g.Nz * (j + g.Ny * i);
@@ -181,7 +199,8 @@ k_astep(AdjGeom g, const R* __restrict__ Pb, R* __restrict__ Pbo, const R* __res
         R* __restrict__ acc_a, R* __restrict__ acc_e, R* __restrict__ acc_f,
         const int* __restrict__ cmap, const int* __restrict__ start, const int* __restrict__ rec,
         const R* __restrict__ w, const R* __restrict__ gy) {
-    const int k = blockIdx.x * BZ + threadIdx.x, j = blockIdx.y * BY + threadIdx.y, i = blockIdx.z;
+    const Cell3 cl = sweep_cell(g.Ny, g.Nz);
+    const int i = cl.i, j = cl.j, k = cl.k;
     if (k >= g.Nz || j >= g.Ny) return;
     const int sy = g.Nz, sx = g.Ny * g.Nz;
     const int c = k + g.Nz * (j + g.Ny * i);
@@ -301,6 +320,7 @@ template <typename R>
 int run_forward(FdtdAdjoint* a, const R* m, const R* l, const R* eta, int n_steps, R* traces) {
     AdjGeom g{a->Nx, a->Ny, a->Nz, a->dtdx};
     dim3 blk(BZ, BY), grd((a->Nz + BZ - 1) / BZ, (a->Ny + BY - 1) / BY, a->Nx);
+    const dim3 grd1(grd.x * grd.y * grd.z);
     const size_t fb = (size_t)a->N * sizeof(R);
     hipLaunchKernelGGL(k_coef<R>, grd, blk, 0, a->stream, g, m, l, eta, (R*)a->b, (R*)a->kap,
                        (R*)a->xi, (R*)a->A, (R*)a->E);
@@ -329,7 +349,7 @@ int run_forward(FdtdAdjoint* a, const R* m, const R* l, const R* eta, int n_step
                                            a->stream));
         }
         R* Pn = (R*)a->P[cur ^ 1];
-        hipLaunchKernelGGL(k_fstep<R>, grd, blk, 0, a->stream, g, (const R*)a->P[cur], (const R*)a->b,
+        hipLaunchKernelGGL(k_fstep<R>, grd1, blk, 0, a->stream, g, (const R*)a->P[cur], (const R*)a->b,
                            (const R*)V[0], (const R*)V[1], (const R*)V[2], (R*)Vn[0], (R*)Vn[1],
                            (R*)Vn[2], Pn, (const R*)a->A, (const R*)a->E, fsrc, a->d_sidx,
                            (const R*)a->d_sig, a->slen, n);
@@ -349,6 +369,7 @@ template <typename R>
 int run_backward(FdtdAdjoint* a, const R* gy, R* gm, R* gl, R* geta) {
     AdjGeom g{a->Nx, a->Ny, a->Nz, a->dtdx};
     dim3 blk(BZ, BY), grd((a->Nz + BZ - 1) / BZ, (a->Ny + BY - 1) / BY, a->Nx);
+    const dim3 grd1(grd.x * grd.y * grd.z);
     const size_t fb = (size_t)a->N * sizeof(R);
     const int K = a->ckpt_every, n_steps = a->n_steps;
     if (a->segP[0].empty()) {
@@ -379,7 +400,7 @@ int run_backward(FdtdAdjoint* a, const R* gy, R* gm, R* gl, R* geta) {
             const R* Vix = j ? (const R*)sV[3 * (j - 1)] : (const R*)a->ck[4 * q + 1];
             const R* Viy = j ? (const R*)sV[3 * (j - 1) + 1] : (const R*)a->ck[4 * q + 2];
             const R* Viz = j ? (const R*)sV[3 * (j - 1) + 2] : (const R*)a->ck[4 * q + 3];
-            hipLaunchKernelGGL(k_fstep<R>, grd, blk, 0, a->stream2, g, (const R*)sP[j],
+            hipLaunchKernelGGL(k_fstep<R>, grd1, blk, 0, a->stream2, g, (const R*)sP[j],
                                (const R*)a->b, Vix, Viy, Viz, (R*)sV[3 * j], (R*)sV[3 * j + 1],
                                (R*)sV[3 * j + 2], (R*)sP[j + 1], (const R*)a->A, (const R*)a->E,
                                fsrc, a->d_sidx, (const R*)a->d_sig, a->slen, s0 + j);
@@ -410,7 +431,7 @@ int run_backward(FdtdAdjoint* a, const R* gy, R* gm, R* gl, R* geta) {
             void* const* Vb = a->Vb[cur];
             void* const* Vbn = a->Vb[cur ^ 1];
             const R* gyn = n > 0 ? gy + (size_t)(n - 1) * a->nrec : nullptr;
-            hipLaunchKernelGGL(k_astep<R>, grd, blk, 0, a->stream, g, Pb, (R*)a->Pb[cur ^ 1],
+            hipLaunchKernelGGL(k_astep<R>, grd1, blk, 0, a->stream, g, Pb, (R*)a->Pb[cur ^ 1],
                                (const R*)Vb[0], (const R*)Vb[1], (const R*)Vb[2], (R*)Vbn[0],
                                (R*)Vbn[1], (R*)Vbn[2], (const R*)sP[j], (const R*)sV[3 * j],
                                (const R*)sV[3 * j + 1], (const R*)sV[3 * j + 2], (const R*)a->A,
