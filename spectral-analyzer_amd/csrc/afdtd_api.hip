@@ -543,14 +543,14 @@ AcousticFDTDState* create_impl(int Nx, int Ny, int Nz, float dx, float c, float 
             return nullptr;
         }
         st->family = FDTD_KERNEL_FUSED2;
-        st->f2_rows = (o->tile_y == 8) ? 1 : 2;
+        st->f2_rows = f2_rows_for(o->tile_y);
     } else if (o->kernel == FDTD_KERNEL_AUTO && st->storage == FDTD_STORE_F32 &&
                fused2_supported(st)) {
         // FP32 order-2 rooms: two steps per sweep (the single-step sweep is
         // bound by the streaming rate of its byte mix; 16-bit storage halves
         // that mix and keeps the single-step sweep ahead)
         st->family = FDTD_KERNEL_FUSED2;
-        st->f2_rows = (o->tile_y == 8) ? 1 : 2;
+        st->f2_rows = f2_rows_for(o->tile_y);
     } else if (o->kernel == FDTD_KERNEL_FUSED || o->kernel == FDTD_KERNEL_AUTO) {
         if (fused_supported(st)) st->family = FDTD_KERNEL_FUSED;
         else if (o->kernel == FDTD_KERNEL_FUSED) {

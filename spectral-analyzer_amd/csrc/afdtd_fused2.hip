@@ -62,6 +62,8 @@ struct Geo2 {
     static constexpr int GPL = GR * GC;
     static constexpr int EPL = EY * EC;
     static constexpr int NHL = 4 * GR;            // region halo-column elements per plane
+    static constexpr int LR = (GR + NW - 1) / NW; // region rows loaded per wave
+    static constexpr int NS = R > 2 ? R : 2;      // old-value slots per field
 };
 constexpr int MAXS = F2_MAX_SRC; // point-source entries cached per tile
 constexpr int MAXT = F2_MAX_TAP; // receiver-tap entries cached per tile
@@ -151,7 +153,7 @@ __device__ __forceinline__ void fused2_tile(const Pair2Args& a, const StencilPar
     using namespace f2;
     typedef Geo2<R> G;
     constexpr int NWM = G::NWM, TYO = G::TYO, NT = G::NT, EY = G::EY, GPL = G::GPL,
-                  EPL = G::EPL, NHL = G::NHL;
+                  EPL = G::EPL, NHL = G::NHL, LR = G::LR, NS = G::NS;
     typedef Store<T> S;
     typedef typename S::Raw RawT;
     __shared__ float sP[NSP * GPL];
@@ -240,13 +242,13 @@ __device__ __forceinline__ void fused2_tile(const Pair2Args& a, const StencilPar
     };
 
     // ── region loads (P^n and 1/rho of one plane into the ring) ──
-    // main part: wave w loads region rows 2w-1, 2w (columns 0..63);
-    // halo columns -2, -1, 64, 65 of the 20 rows: threads tid < 80.
-    uint32_t rgT[3], rgF[3];
-    int ldLds[3];
+    // main part: wave w loads region rows LR*w-1 .. LR*w+LR-2 (columns
+    // 0..63); halo columns -2, -1, 64, 65 of the GR rows: threads tid < NHL.
+    uint32_t rgT[LR + 1], rgF[LR + 1];
+    int ldLds[LR + 1];
 #pragma unroll
-    for (int m = 0; m < 2; ++m) {
-        const int rr = 2 * w - 1 + m;                 // waves beyond row EY idle
+    for (int m = 0; m < LR; ++m) {
+        const int rr = LR * w - 1 + m;                // waves beyond row EY idle
         const int j = j0 - 1 + rr, k = k0 + lane;
         const int off = (rr <= EY && cellv(j, k)) ? j * Nz + k : 0;
         rgT[m] = (uint32_t)(off * esz);
@@ -258,14 +260,14 @@ __device__ __forceinline__ void fused2_tile(const Pair2Args& a, const StencilPar
         const int cc = cs == 0 ? -2 : cs == 1 ? -1 : cs == 2 ? 64 : 65;
         const int j = j0 - 1 + rr, k = k0 + cc;
         const int off = (tid < NHL && cellv(j, k)) ? j * Nz + k : 0;
-        rgT[2] = (uint32_t)(off * esz);
-        rgF[2] = (uint32_t)(off * 4);
-        ldLds[2] = tid < NHL ? gix(rr, cc) : -1;
+        rgT[LR] = (uint32_t)(off * esz);
+        rgF[LR] = (uint32_t)(off * 4);
+        ldLds[LR] = tid < NHL ? gix(rr, cc) : -1;
     }
     // P^n and 1/rho of one region plane, staged in registers (unconverted)
     struct Stage {
-        RawT p[3];
-        float b[3];
+        RawT p[LR + 1];
+        float b[LR + 1];
     };
     auto load_plane = [&](int pl) __attribute__((always_inline)) {
         Stage st;
@@ -273,7 +275,7 @@ __device__ __forceinline__ void fused2_tile(const Pair2Args& a, const StencilPar
         const Rsrc rp = rT(Pin, b);
         const Rsrc rb = rF(HET ? bu : (const float*)Pin, b);
 #pragma unroll
-        for (int m = 0; m < 3; ++m) {
+        for (int m = 0; m <= LR; ++m) {
             st.p[m] = ldT(rp, rgT[m]);
             st.b[m] = HET ? ldF(rb, rgF[m]) : 0.0f;
         }
@@ -281,7 +283,7 @@ __device__ __forceinline__ void fused2_tile(const Pair2Args& a, const StencilPar
     };
     auto store_plane = [&](int pl, const Stage& st) __attribute__((always_inline)) {
 #pragma unroll
-        for (int m = 0; m < 3; ++m) {
+        for (int m = 0; m <= LR; ++m) {
             if (ldLds[m] >= 0) {
                 sP[(pl & (NSP - 1)) * GPL + ldLds[m]] = S::cvt(st.p[m]);
                 if (HET) sB[(pl & (NSB - 1)) * GPL + ldLds[m]] = st.b[m];
@@ -348,8 +350,10 @@ __device__ __forceinline__ void fused2_tile(const Pair2Args& a, const StencilPar
     // condition (faces of a valid cell that lie outside the grid are stored
     // as 0 into their unused padding slots).
     constexpr uint32_t OOB = 0x80000000u;
-    uint32_t oT[4], oF[2], xF[2], xD[2];
-    oT[1] = oF[1] = xF[1] = xD[1] = 0u;
+    uint32_t oT[R > 4 ? R : 4], oF[NS], xF[NS], xD[NS];
+#pragma unroll
+    for (int r = 0; r < NS; ++r) oF[r] = xF[r] = xD[r] = 0u;
+    oT[1] = 0u;
     if (mainW) {
 #pragma unroll
         for (int r = 0; r < R; ++r) {
@@ -359,7 +363,8 @@ __device__ __forceinline__ void fused2_tile(const Pair2Args& a, const StencilPar
             xF[r] = medge[r] ? oF[r] : 0u;
             xD[r] = oF[r] - xF[r];
         }
-        oT[2] = oT[3] = 0u;
+#pragma unroll
+        for (int r = R; r < 4; ++r) oT[r] = 0u;
     } else {
         const int c0 = hcell ? hoff : 0;
         const int c1 = (hH && cellv(hj - 1, hk)) ? hoff - Nz : 0;
@@ -373,26 +378,26 @@ __device__ __forceinline__ void fused2_tile(const Pair2Args& a, const StencilPar
         oF[0] = (uint32_t)(c0 * 4);
         xF[0] = hedge ? oF[0] : 0u;
         xD[0] = oF[0] - xF[0];
-        oF[1] = xF[1] = xD[1] = 0u;
     }
     // Per-slot offsets of the old-value loads, so that both roles issue the
-    // same ten loads (slot 1 reads Vx for main waves and Vy for the halo
-    // wave's T4 face; the descriptor is chosen per wave):
-    //   slots x0 x1 y0 y1 z0 z1 (T fields), k0 k1 (kappa), xi via xF / xD
-    uint32_t lo[6], lf[2];
+    // same loads (slot x1 reads Vx for main waves and Vy for the halo wave's
+    // T4 face; the descriptor is chosen per wave):
+    //   slots x[0..NS) y[0..NS) z[0..NS) (T fields), kappa / xi via lf, xF, xD
+    uint32_t lo[3 * NS], lf[NS];
     if (mainW) {
-        lo[0] = lo[2] = lo[4] = oT[0];
-        lo[1] = lo[3] = lo[5] = oT[1];
-        lf[0] = oF[0];
-        lf[1] = oF[1];
+#pragma unroll
+        for (int r = 0; r < NS; ++r) {
+            lo[r] = lo[NS + r] = lo[2 * NS + r] = oT[r < 4 ? r : 0];
+            lf[r] = oF[r];
+        }
     } else {
-        lo[0] = oT[0];
+#pragma unroll
+        for (int r = 0; r < 3 * NS; ++r) lo[r] = oT[0];
         lo[1] = oT[3];
-        lo[2] = oT[1];
-        lo[3] = oT[0];
-        lo[4] = oT[2];
-        lo[5] = oT[0];
-        lf[0] = lf[1] = oF[0];
+        lo[NS] = oT[1];
+        lo[2 * NS] = oT[2];
+#pragma unroll
+        for (int r = 0; r < NS; ++r) lf[r] = oF[0];
     }
 
     // Old values of plane q: V^{n-1/2} (stage-A faces of q) and the kappa /
@@ -401,8 +406,8 @@ __device__ __forceinline__ void fused2_tile(const Pair2Args& a, const StencilPar
     // uses x[0] (Vx), x[1] (the T4 face), y[0..1] (Vy), z[0..1] (Vz / T1),
     // kk[0], xi[0] (kk[1], xi[1] are loaded for the same cell and unused).
     struct Old {
-        RawT x[2], y[2], z[2];
-        float kk[2], xi[2];
+        RawT x[NS], y[NS], z[NS];
+        float kk[NS], xi[NS];
     };
     auto load_old = [&](int q) __attribute__((always_inline)) {
         Old o;
@@ -415,14 +420,14 @@ __device__ __forceinline__ void fused2_tile(const Pair2Args& a, const StencilPar
         // without impedance walls the wall-sum descriptor is empty: its loads
         // return 0 and touch no memory
         const Rsrc XI = rsrc((loadXi ? xiA : (const float*)Vxi) + bk, loadXi ? pbF : 0u);
-        o.x[0] = ldT(X, lo[0]);
-        o.x[1] = ldT(X1, lo[1]);
-        o.y[0] = ldT(Y, lo[2]);
-        o.y[1] = ldT(Y, lo[3]);
-        o.z[0] = ldT(Z, lo[4]);
-        o.z[1] = ldT(Z, lo[5]);
 #pragma unroll
-        for (int r = 0; r < 2; ++r) {
+        for (int r = 0; r < NS; ++r) {
+            o.x[r] = ldT(r == 1 ? X1 : X, lo[r]);
+            o.y[r] = ldT(Y, lo[NS + r]);
+            o.z[r] = ldT(Z, lo[2 * NS + r]);
+        }
+#pragma unroll
+        for (int r = 0; r < NS; ++r) {
             // every cell of an x-side plane is a wall cell
             o.kk[r] = (HET && r < R) ? ldF(K, lf[r]) : 0.0f;
             o.xi[r] = (HET && r < R) ? ldF(XI, xF[r] + (xe ? xD[r] : 0u)) : 0.0f;
@@ -478,18 +483,18 @@ __device__ __forceinline__ void fused2_tile(const Pair2Args& a, const StencilPar
     // pa2[1] is hz1 of plane q-2).
     float hq[4] = {0.0f, 0.0f, 0.0f, 0.0f}, hp[4] = {0.0f, 0.0f, 0.0f, 0.0f}, hz2s = 0.0f;
     auto hset = [&](float a0, float a1, float b0, float b1) __attribute__((always_inline)) {
-        if constexpr (R == 2) {
+        if constexpr (R >= 2) {
             y0[1] = a0; z0[1] = a1; xb0[1] = b0; pa0[1] = b1;
         } else {
             hq[0] = a0; hq[1] = a1; hq[2] = b0; hq[3] = b1;
         }
     };
     auto hprev = [&](int i) __attribute__((always_inline)) {   // plane q-1 values
-        if constexpr (R == 2) return i == 0 ? y1[1] : i == 1 ? z1[1] : i == 2 ? xb1[1] : pa1[1];
+        if constexpr (R >= 2) return i == 0 ? y1[1] : i == 1 ? z1[1] : i == 2 ? xb1[1] : pa1[1];
         else return hp[i];
     };
     auto hz2v = [&]() __attribute__((always_inline)) {
-        if constexpr (R == 2) return pa2[1];
+        if constexpr (R >= 2) return pa2[1];
         else return hz2s;
     };
 
@@ -591,9 +596,9 @@ __device__ __forceinline__ void fused2_tile(const Pair2Args& a, const StencilPar
             // up to the previous iteration; its 1/rho slot plane q-6)
             store_plane(q + 2, st);
         }
-        float kA[2], xA[2];   // kappa / wall sums of plane q-1
+        float kA[NS], xA[NS];   // kappa / wall sums of plane q-1
 #pragma unroll
-        for (int r = 0; r < 2; ++r) {
+        for (int r = 0; r < NS; ++r) {
             kA[r] = o.kk[r];
             xA[r] = o.xi[r];
         }
@@ -909,12 +914,14 @@ void f2_tile_lists(int Nx, int Ny, int Nz, int TYO, int nsrc, const int* src_cel
     }
 }
 
-int f2_tile_rows(int tile_y) { return tile_y == 8 ? f2::Geo2<1>::TYO : f2::Geo2<2>::TYO; }
+int f2_rows_for(int tile_y) { return tile_y == 8 ? 1 : 2; }
+static int f2_tyo(int rows) { return rows == 1 ? f2::Geo2<1>::TYO : f2::Geo2<2>::TYO; }
+int f2_tile_rows(int tile_y) { return f2_tyo(f2_rows_for(tile_y)); }
 
 // Per-tile source and tap tables (uploaded when sources or receivers change).
 static int build_tables(AcousticFDTDState* st) {
     using namespace f2;
-    const int TYO = st->f2_rows == 1 ? Geo2<1>::TYO : Geo2<2>::TYO;
+    const int TYO = f2_tyo(st->f2_rows);
     const int nt = ((st->Ny + TYO - 1) / TYO) * ((st->Nz + TZ - 1) / TZ);
     std::vector<std::vector<int2>> src, tap;
     std::vector<int> sid(st->npsrc), tslot(8 * (size_t)st->nrec);
@@ -984,7 +991,7 @@ hipError_t launch_fused2_pair(AcousticFDTDState* st, int step) {
     a.status = st->d_status;
     a.seg = st->fused_seg_x > 0 ? st->fused_seg_x : 64;
     if (a.seg > st->Nx) a.seg = st->Nx;
-    const int TYO = st->f2_rows == 1 ? Geo2<1>::TYO : Geo2<2>::TYO;
+    const int TYO = f2_tyo(st->f2_rows);
     a.nty = (st->Ny + TYO - 1) / TYO;
     a.ntz = (st->Nz + TZ - 1) / TZ;
     a.nseg = (st->Nx + a.seg - 1) / a.seg;

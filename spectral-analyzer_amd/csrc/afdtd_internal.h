@@ -168,6 +168,7 @@ void f2_tile_lists(int Nx, int Ny, int Nz, int TYO, int nsrc, const int* src_cel
                    const int* src_id, int ntap, const int* tap_cell, const int* tap_slot,
                    std::vector<std::vector<int2>>& src, std::vector<std::vector<int2>>& tap);
 // Own rows of a two-step tile for tile_y 8 or 16.
+int f2_rows_for(int tile_y);   // two-step sweep: rows per thread for a tile height
 int f2_tile_rows(int tile_y);
 // Longest per-tile source / tap list the two-step sweep caches in LDS.
 constexpr int F2_MAX_SRC = 32, F2_MAX_TAP = 256;
