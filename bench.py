@@ -186,6 +186,84 @@ def bench_batch(a):
     return 0
 
 
+def bench_geometry(a):
+    """Geometry input formats: voxelise a two-bout body (256-vertex outline,
+    rib height 0.1 m) at dx = 1 mm with 64 cells of padding and compute its
+    cut-cell face fractions, from and into device memory; the oracle's numpy
+    restatement of the reference's host code is the CPU baseline."""
+    import ctypes as C
+    import torch
+    from afdtd import lib
+    from afdtd.geometry import body_grid, two_bout_outline
+    poly = two_bout_outline(256)
+    body_h, dx, pad = 0.1, 0.001, 64
+    g = body_grid(poly, body_h, dx, pad)
+    Nx, Ny, Nz = g["Nx"], g["Ny"], g["Nz"]
+    N = Nx * Ny * Nz
+    L = lib()
+    dpoly = torch.tensor(poly, device="cuda")
+    dct = torch.empty(N, dtype=torch.uint8, device="cuda")
+    dpa = torch.empty(Nx * Ny, dtype=torch.uint8, device="cuda")
+    fr = [torch.empty(n, dtype=torch.float32, device="cuda")
+          for n in ((Nx - 1) * Ny * Nz, Nx * (Ny - 1) * Nz, Nx * Ny * (Nz - 1))]
+    torch.cuda.synchronize()
+    p = lambda t: C.c_void_p(t.data_ptr())
+
+    def once():
+        m1, m2 = C.c_float(0.0), C.c_float(0.0)
+        rc = L.fdtd_voxelise_body_device(p(dpoly), len(poly), Nx, Ny, Nz, g["gx_min"],
+                                         g["gy_min"], g["gz_min"], dx, 0.0, body_h, g["iz_back"],
+                                         g["plate_iz"], p(dct), p(dpa), None, C.byref(m1))
+        rc = rc or L.fdtd_cut_cell_fractions_device(
+            p(dct), p(dpa), p(dpoly), len(poly), Nx, Ny, Nz, g["gx_min"], g["gy_min"],
+            g["gz_min"], dx, g["plate_iz"], g["iz_back"], p(fr[0]), p(fr[1]), p(fr[2]), None,
+            C.byref(m2))
+        if rc:
+            raise RuntimeError(f"geometry entry points failed (rc={rc})")
+        return m1.value, m2.value
+
+    steps = a.steps if a.steps != 4000 else 50
+    for _ in range(min(a.warmup, 5)):
+        once()
+    t0 = time.perf_counter()
+    ms = [once() for _ in range(steps)]
+    wall = time.perf_counter() - t0
+    vox_ms = sum(m[0] for m in ms) / steps
+    frac_ms = sum(m[1] for m in ms) / steps
+    # algorithmic bytes per cell: cell type written (1), three face fractions
+    # written (12), the cell type read once per face array (3)
+    bpc = 16
+    kern_s = (vox_ms + frac_ms) * 1e-3
+    achieved = N * bpc / kern_s / 1e9
+    line = {"metric": "voxelised cells/s (cell types + cut-cell face fractions, 1 GPU)",
+            "value": N / kern_s, "unit": "cells/s", "n_gpus": 1, "steps": steps,
+            "warmup": min(a.warmup, 5), "ms_per_step": vox_ms + frac_ms,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u8/f32",
+            "data": "synthetic (two-bout body outline)",
+            "config": {"workload": f"geometry-two-bout-{Nx}x{Ny}x{Nz}-dx1mm", "cells": N,
+                       "outline_vertices": len(poly), "plate_nodes": int(dpa.sum().item())},
+            "voxelise_ms": vox_ms, "fractions_ms": frac_ms,
+            "wall_ms_per_call_pair": wall * 1e3 / steps,
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                         "bytes_per_cell": bpc}}
+    if not a.no_cpu:
+        from oracle import geometry as OG
+        t0 = time.perf_counter()
+        ct, pa, og = OG.voxelise(poly, body_h, dx, pad)
+        OG.face_fractions(ct, pa, poly, og)
+        dt = time.perf_counter() - t0
+        line["cpu_baseline"] = {"value": N / dt, "unit": "cells/s", "cores": 1, "kind": "port",
+                                "sample": f"the same grid once ({N} cells, {dt:.1f} s, numpy "
+                                          f"restatement of the reference's host code)"}
+        line["equal_to_oracle"] = bool(
+            np.array_equal(dct.cpu().numpy(), ct.ravel())
+            and all(np.array_equal(f.cpu().numpy(), o)
+                    for f, o in zip(fr, OG.face_fractions(ct, pa, poly, og))))
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def bench_adjoint(a):
     """Config 5: 128^3, 2000 steps, checkpoints every 100: forward + adjoint
     (FP32) timing; float64 central differences on 64 random cells."""
@@ -286,6 +364,8 @@ def main():
         if a.steps == 4000:
             a.steps = 2000
         return bench_adjoint(a)
+    if a.config == "geometry":
+        return bench_geometry(a)
     w = make_workload(a.config, a.warmup + a.steps)
     s = make_solver(w, a.storage, a.kernel, a.tile_y, a.seg_x)
     total = a.warmup + a.steps

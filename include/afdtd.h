@@ -327,6 +327,46 @@ SK_API int fdtd_adjoint_forward(FdtdAdjoint* a, const void* m, const void* l,
 SK_API int fdtd_adjoint_backward(FdtdAdjoint* a, const void* grad_traces, void* grad_m,
                                  void* grad_l, void* grad_eta);
 
+/* ── Geometry input formats (extension) ───────────────────────────────────
+ * The body voxeliser and the cut-cell face fractions that produce the
+ * cell_type / plate_active arrays of fdtd_create and the face fractions of
+ * fdtd_set_face_fractions, computed on the GPU.  The reference computes them
+ * in Python above its C ABI: voxelise_guitar_body
+ * (acoustic_fdtd_bridge.py:255-359, non-AMR branch) and
+ * compute_cut_cell_fractions (acoustic_fdtd_bridge.py:456-552); the results
+ * are equal element for element.  outline_xy: n_pts (x, y) float pairs, the
+ * closed outline polygon, 3 <= n_pts <= 4096 (else FDTD_ERR_DIM).  The grid (dimensions, offsets g*_min, spacing dx,
+ * body extent z_min..z_max and the back / plate layers iz_back / iz_top) is
+ * the one voxelise_guitar_body derives from the outline's bounding box.
+ * Layouts: cell_type Nx*Ny*Nz (k fastest), plate_active Nx*Ny, fractions in
+ * the packed staggered layouts of fdtd_get_velocity_x/y/z.  The plain forms
+ * take host memory; the _device forms take device memory and run on `stream`
+ * (a hipStream_t of this library's HIP runtime, NULL = its default stream),
+ * and when kernel_ms is not NULL store the kernels' time in milliseconds
+ * (HIP events on that stream).  Every form returns when its results are
+ * written. */
+SK_API int fdtd_voxelise_body(const float* outline_xy, int n_pts, int Nx, int Ny, int Nz,
+                              double gx_min, double gy_min, double gz_min, double dx,
+                              double z_min, double z_max, int iz_back, int iz_top,
+                              unsigned char* cell_type, unsigned char* plate_active);
+SK_API int fdtd_voxelise_body_device(const float* outline_xy, int n_pts, int Nx, int Ny, int Nz,
+                                     double gx_min, double gy_min, double gz_min, double dx,
+                                     double z_min, double z_max, int iz_back, int iz_top,
+                                     unsigned char* cell_type, unsigned char* plate_active,
+                                     void* stream, float* kernel_ms);
+SK_API int fdtd_cut_cell_fractions(const unsigned char* cell_type,
+                                   const unsigned char* plate_active, const float* outline_xy,
+                                   int n_pts, int Nx, int Ny, int Nz, double gx_min,
+                                   double gy_min, double gz_min, double dx, int plate_iz,
+                                   int iz_back, float* vx_frac, float* vy_frac, float* vz_frac);
+SK_API int fdtd_cut_cell_fractions_device(const unsigned char* cell_type,
+                                          const unsigned char* plate_active,
+                                          const float* outline_xy, int n_pts, int Nx, int Ny,
+                                          int Nz, double gx_min, double gy_min, double gz_min,
+                                          double dx, int plate_iz, int iz_back, float* vx_frac,
+                                          float* vy_frac, float* vz_frac, void* stream,
+                                          float* kernel_ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -105,7 +105,28 @@ _SIGS = {
     "fdtd_adjoint_set_receivers": (_i, [_vp, _i, _vp, _vp]),
     "fdtd_adjoint_forward": (_i, [_vp, _vp, _vp, _vp, _i, _vp]),
     "fdtd_adjoint_backward": (_i, [_vp, _vp, _vp, _vp, _vp]),
+    "fdtd_voxelise_body": (_i, [_vp, _i, _i, _i, _i] + [C.c_double] * 6 + [_i, _i, _vp, _vp]),
+    "fdtd_voxelise_body_device": (_i, [_vp, _i, _i, _i, _i] + [C.c_double] * 6
+                                  + [_i, _i, _vp, _vp, _vp, _vp]),
+    "fdtd_cut_cell_fractions": (_i, [_vp, _vp, _vp, _i, _i, _i, _i] + [C.c_double] * 4
+                                + [_i, _i, _vp, _vp, _vp]),
+    "fdtd_cut_cell_fractions_device": (_i, [_vp, _vp, _vp, _i, _i, _i, _i] + [C.c_double] * 4
+                                       + [_i, _i, _vp, _vp, _vp, _vp, _vp]),
 }
+
+
+def _init_torch_first():
+    """A process that also uses PyTorch holds two HIP runtimes: PyTorch's own
+    (with its own HSA runtime) and the system ROCm one this library links.
+    PyTorch finds no GPU if the system runtime opened the device first, while
+    the other order works, so PyTorch's device context is created before the
+    library is loaded whenever PyTorch is installed and sees a GPU."""
+    try:
+        import torch
+    except ImportError:
+        return
+    if torch.cuda.is_available():
+        torch.cuda.init()
 
 
 def lib():
@@ -116,6 +137,7 @@ def lib():
             raise ImportError(
                 f"libafdtd.so not found at {LIB_PATH}: build it with "
                 "`python -c 'import __graft_entry__ as g; g.build()'` (hipcc, gfx950)")
+        _init_torch_first()
         L = C.CDLL(LIB_PATH)
         for name, (res, args) in _SIGS.items():
             fn = getattr(L, name)
