@@ -59,7 +59,7 @@ struct Geo2 {
     static constexpr int NT = 64 * NW;
     static constexpr int EY = TYO + 2;            // extended rows
     static constexpr int GR = EY + 2;             // P^n region rows -1..EY
-    static constexpr int GPL = GR * GC;
+    static constexpr int GPL = GR * GC + 1;       // + one spare word (stores of idle lanes)
     static constexpr int EPL = EY * EC;
     static constexpr int NHL = 4 * GR;            // region halo-column elements per plane
     static constexpr int LR = (GR + NW - 1) / NW; // region rows loaded per wave
@@ -98,6 +98,19 @@ __device__ __forceinline__ void bst(Rsrc r, uint32_t off, uint16_t v) {
 }
 __device__ __forceinline__ void bst(Rsrc r, uint32_t off, f16_t v) {
     __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(uint16_t, v), r, off, 0, 0);
+}
+// Per-lane condition held as a VGPR word (all ones or zero) and applied with
+// a bitwise AND.  Kept as 64-bit SGPR lane masks, the many per-lane
+// conditions of the sweep exceed the scalar register budget and are spilled
+// into VGPR lanes and reloaded on every plane; the empty asm makes the word
+// opaque, so the compiler does not turn it back into a lane mask.
+__device__ __forceinline__ uint32_t vmask(bool c) {
+    uint32_t m = c ? 0xFFFFFFFFu : 0u;
+    asm volatile("" : "+v"(m));
+    return m;
+}
+__device__ __forceinline__ float keep(float v, uint32_t m) {
+    return __uint_as_float(__float_as_uint(v) & m);
 }
 // LDS index helpers (row / column in extended-tile coordinates)
 __device__ __forceinline__ int gix(int r, int c) { return (r + 1) * GC + (c + 2); }
@@ -253,7 +266,7 @@ __device__ __forceinline__ void fused2_tile(const Pair2Args& a, const StencilPar
         const int off = (rr <= EY && cellv(j, k)) ? j * Nz + k : 0;
         rgT[m] = (uint32_t)(off * esz);
         rgF[m] = (uint32_t)(off * 4);
-        ldLds[m] = rr <= EY ? gix(rr, lane) : -1;
+        ldLds[m] = rr <= EY ? gix(rr, lane) : GPL - 1;   // idle lanes: the spare word
     }
     {
         const int rr = -1 + (tid >> 2), cs = tid & 3;
@@ -262,7 +275,7 @@ __device__ __forceinline__ void fused2_tile(const Pair2Args& a, const StencilPar
         const int off = (tid < NHL && cellv(j, k)) ? j * Nz + k : 0;
         rgT[LR] = (uint32_t)(off * esz);
         rgF[LR] = (uint32_t)(off * 4);
-        ldLds[LR] = tid < NHL ? gix(rr, cc) : -1;
+        ldLds[LR] = tid < NHL ? gix(rr, cc) : GPL - 1;
     }
     // P^n and 1/rho of one region plane, staged in registers (unconverted)
     struct Stage {
@@ -284,10 +297,8 @@ __device__ __forceinline__ void fused2_tile(const Pair2Args& a, const StencilPar
     auto store_plane = [&](int pl, const Stage& st) __attribute__((always_inline)) {
 #pragma unroll
         for (int m = 0; m <= LR; ++m) {
-            if (ldLds[m] >= 0) {
-                sP[(pl & (NSP - 1)) * GPL + ldLds[m]] = S::cvt(st.p[m]);
-                if (HET) sB[(pl & (NSB - 1)) * GPL + ldLds[m]] = st.b[m];
-            }
+            sP[(pl & (NSP - 1)) * GPL + ldLds[m]] = S::cvt(st.p[m]);
+            if (HET) sB[(pl & (NSB - 1)) * GPL + ldLds[m]] = st.b[m];
         }
     };
     auto Pn = [&](int pl, int r, int c) __attribute__((always_inline)) { return sP[(pl & (NSP - 1)) * GPL + gix(r, c)]; };
@@ -339,6 +350,42 @@ __device__ __forceinline__ void fused2_tile(const Pair2Args& a, const StencilPar
     for (int r = 0; r < R; ++r)
         medge[r] = mcell[r] && (mj[r] == 0 || mj[r] == Ny - 1 || k == 0 || k == Nz - 1);
     const bool hedge = hcell && (hj == 0 || hj == Ny - 1 || hk == 0 || hk == Nz - 1);
+    // With per-cell media (one workgroup per CU, VGPRs to spare) the same
+    // conditions are used inside the sweep as VGPR words; the homogeneous
+    // build, held to 80 VGPRs, keeps the lane masks.
+    uint32_t mcellM[R], mvyM[R], mvzM[R], medgeM[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        if constexpr (HET) {
+            mcellM[r] = vmask(mcell[r]);
+            mvyM[r] = vmask(mvy[r]);
+            mvzM[r] = vmask(mvz[r]);
+            medgeM[r] = vmask(medge[r]);
+        } else {
+            mcellM[r] = mvyM[r] = mvzM[r] = medgeM[r] = 0u;
+        }
+    }
+    // halo-wave conditions, likewise
+    uint32_t hcellM = 0u, hvyM[2] = {0u, 0u}, hvzM[2] = {0u, 0u}, t1vM = 0u, t4vM = 0u,
+             hedgeM = 0u;
+    if constexpr (HET) {
+        hcellM = vmask(hcell);
+        hvyM[0] = vmask(hvy0);
+        hvyM[1] = vmask(hvy1);
+        hvzM[0] = vmask(hvz0);
+        hvzM[1] = vmask(hvz1);
+        t1vM = vmask(t1v);
+        t4vM = vmask(t4v);
+        hedgeM = vmask(hedge);
+    }
+    auto pick = [&](float v, bool c, uint32_t m) __attribute__((always_inline)) {
+        if constexpr (HET) return keep(v, m);
+        else return c ? v : 0.0f;
+    };
+    auto edge_or_x = [&](int r, bool xside) __attribute__((always_inline)) {
+        if constexpr (HET) return (medgeM[r] | (mcellM[r] & (xside ? ~0u : 0u))) != 0u;
+        else return medge[r] || (xside && mcell[r]);
+    };
 
     // Byte offsets of the old-value loads (one set, meaning by role):
     //   main: oT[r] / oF[r] the cell (mj[r], k), xF[r] its wall sum (edges only)
@@ -445,13 +492,13 @@ __device__ __forceinline__ void fused2_tile(const Pair2Args& a, const StencilPar
                 const float coef = HET ? (0.5f * (Bn(q, me[r], lane) + Bn(q + 1, me[r], lane))) * dtdx : vcoef;
                 float nv = vel(o.x[r]) - coef * (Pn(q + 1, me[r], lane) - Pn(q, me[r], lane));
                 if (npml > 0) nv = nv * damp(depth3(q, mj[r], k), depth3(q + 1, mj[r], k));
-                out[r] = (fv && mcell[r]) ? nv : 0.0f;
+                out[r] = fv ? pick(nv, mcell[r], mcellM[r]) : 0.0f;
             }
         } else {
             const float coef = HET ? (0.5f * (Bn(q, he, hc) + Bn(q + 1, he, hc))) * dtdx : vcoef;
             float nv = vel(o.x[0]) - coef * (Pn(q + 1, he, hc) - Pn(q, he, hc));
             if (npml > 0) nv = nv * damp(depth3(q, hj, hk), depth3(q + 1, hj, hk));
-            out[0] = (fv && hcell) ? nv : 0.0f;
+            out[0] = fv ? pick(nv, hcell, hcellM) : 0.0f;
 #pragma unroll
             for (int r = 1; r < R; ++r) out[r] = 0.0f;
         }
@@ -547,14 +594,14 @@ __device__ __forceinline__ void fused2_tile(const Pair2Args& a, const StencilPar
                         const float coef = HET ? (0.5f * (Bn(q, e, lane) + Bn(q, e + 1, lane))) * dtdx : vcoef;
                         float nv = vel(o.y[r]) - coef * (Pn(q, e + 1, lane) - Pn(q, e, lane));
                         if (npml > 0) nv = nv * damp(depth3(q, mj[r], k), depth3(q, mj[r] + 1, k));
-                        y0[r] = (qv && mvy[r]) ? nv : 0.0f;
+                        y0[r] = qv ? pick(nv, mvy[r], mvyM[r]) : 0.0f;
                         if (r == R - 1) sAyW[(w + 1) * TZ + lane] = y0[r];
                     }
                     {   // z-face between columns lane and lane+1, row e
                         const float coef = HET ? (0.5f * (Bn(q, e, lane) + Bn(q, e, lane + 1))) * dtdx : vcoef;
                         float nv = vel(o.z[r]) - coef * (Pn(q, e, lane + 1) - Pn(q, e, lane));
                         if (npml > 0) nv = nv * damp(depth3(q, mj[r], k), depth3(q, mj[r], k + 1));
-                        z0[r] = (qv && mvz[r]) ? nv : 0.0f;
+                        z0[r] = qv ? pick(nv, mvz[r], mvzM[r]) : 0.0f;
                     }
                 }
             } else {
@@ -566,7 +613,7 @@ __device__ __forceinline__ void fused2_tile(const Pair2Args& a, const StencilPar
                         const float coef = HET ? (0.5f * (Bn(q, e, hc) + Bn(q, e + 1, hc))) * dtdx : vcoef;
                         float nv = vel(o.y[u]) - coef * (Pn(q, e + 1, hc) - Pn(q, e, hc));
                         if (npml > 0) nv = nv * damp(depth3(q, hj - 1 + u, hk), depth3(q, hj + u, hk));
-                        hy[u] = (qv && (u ? hvy1 : hvy0)) ? nv : 0.0f;
+                        hy[u] = qv ? pick(nv, u ? hvy1 : hvy0, hvyM[u]) : 0.0f;
                     }
 #pragma unroll
                     for (int u = 0; u < 2; ++u) {   // z-faces between columns hc-1+u and hc+u
@@ -574,7 +621,7 @@ __device__ __forceinline__ void fused2_tile(const Pair2Args& a, const StencilPar
                         const float coef = HET ? (0.5f * (Bn(q, he, c) + Bn(q, he, c + 1))) * dtdx : vcoef;
                         float nv = vel(o.z[u]) - coef * (Pn(q, he, c + 1) - Pn(q, he, c));
                         if (npml > 0) nv = nv * damp(depth3(q, hj, hk - 1 + u), depth3(q, hj, hk + u));
-                        hz[u] = (qv && (u ? hvz1 : hvz0)) ? nv : 0.0f;
+                        hz[u] = qv ? pick(nv, u ? hvz1 : hvz0, hvzM[u]) : 0.0f;
                     }
                     if (hc < 0) sAzW[he] = hz[1];
                     hset(hy[0], hy[1], hz[0], hz[1]);
@@ -583,13 +630,13 @@ __device__ __forceinline__ void fused2_tile(const Pair2Args& a, const StencilPar
                     const float coef = HET ? (0.5f * (Bn(q, t1e, -1) + Bn(q, t1e, 0))) * dtdx : vcoef;
                     float nv = vel(o.z[0]) - coef * (Pn(q, t1e, 0) - Pn(q, t1e, -1));
                     if (npml > 0) nv = nv * damp(depth3(q, t1j, k0 - 1), depth3(q, t1j, k0));
-                    sAzW[t1e] = (qv && t1v) ? nv : 0.0f;
+                    sAzW[t1e] = qv ? pick(nv, t1v, t1vM) : 0.0f;
                 }
                 {   // T4: y-face between rows -1 and 0
                     const float coef = HET ? (0.5f * (Bn(q, -1, lane) + Bn(q, 0, lane))) * dtdx : vcoef;
                     float nv = vel(o.x[1]) - coef * (Pn(q, 0, lane) - Pn(q, -1, lane));
                     if (npml > 0) nv = nv * damp(depth3(q, t4j, k), depth3(q, t4j + 1, k));
-                    sAyW[lane] = (qv && t4v) ? nv : 0.0f;
+                    sAyW[lane] = qv ? pick(nv, t4v, t4vM) : 0.0f;
                 }
             }
             // ring refill: plane q+2 (its P slot last held plane q-2, read
@@ -619,7 +666,7 @@ __device__ __forceinline__ void fused2_tile(const Pair2Args& a, const StencilPar
                     const float kap = HET ? kA[r] : kap0;
                     const float xi = loadXi ? xA[r] : 0.0f;
                     const float out = pres(Pn(qa, e, lane), dv, kap, xi, qa, mj[r], k,
-                                           medge[r] || (xe && mcell[r]));
+                                           edge_or_x(r, xe));
                     pa0[r] = rndp(out);
                 }
             } else if (hH) {
@@ -627,7 +674,8 @@ __device__ __forceinline__ void fused2_tile(const Pair2Args& a, const StencilPar
                 const float kap = HET ? kA[0] : kap0;
                 const float xi = loadXi ? xA[0] : 0.0f;
                 const float out = pres(Pn(qa, he, hc), dv, kap, xi, qa, hj, hk,
-                                       hedge || (xe && hcell));
+                                       HET ? (hedgeM | (hcellM & (xe ? ~0u : 0u))) != 0u
+                                           : hedge || (xe && hcell));
                 pa0[0] = rndp(out);
             }
             // point sources of step n+1 (after the rounding, in source order)
@@ -685,7 +733,7 @@ __device__ __forceinline__ void fused2_tile(const Pair2Args& a, const StencilPar
                         const float coef = HET ? (0.5f * (Bn(qb, e, lane) + Bn(qb + 1, e, lane))) * dtdx : vcoef;
                         float nv = rndv(xa2[r]) - coef * (pa0[r] - pa1[r]);
                         if (npml > 0) nv = nv * damp(depth3(qb, mj[r], k), depth3(qb + 1, mj[r], k));
-                        nv = (fvb && mcell[r]) ? nv : 0.0f;
+                        nv = fvb ? pick(nv, mcell[r], mcellM[r]) : 0.0f;
                         xb0[r] = nv;
                         if (ownq && fvb) stT(Vxo, qb, oT[r], nv, true);
                     }
@@ -694,7 +742,7 @@ __device__ __forceinline__ void fused2_tile(const Pair2Args& a, const StencilPar
                             const float coef = HET ? (0.5f * (Bn(qb, e, lane) + Bn(qb, e + 1, lane))) * dtdx : vcoef;
                             float nv = rndv(y2[r]) - coef * (p1b[p1ix(e + 1, lane)] - p1b[p1ix(e, lane)]);
                             if (npml > 0) nv = nv * damp(depth3(qb, mj[r], k), depth3(qb, mj[r] + 1, k));
-                            nv = (qbv && mvy[r]) ? nv : 0.0f;
+                            nv = qbv ? pick(nv, mvy[r], mvyM[r]) : 0.0f;
                             yb0[r] = nv;
                             if (r == R - 1) sByW[(w + 1) * TZ + lane] = nv;
                             if (ownr && qbv) stT(Vyo, qb, oT[r], nv, true);
@@ -703,7 +751,7 @@ __device__ __forceinline__ void fused2_tile(const Pair2Args& a, const StencilPar
                             const float coef = HET ? (0.5f * (Bn(qb, e, lane) + Bn(qb, e, lane + 1))) * dtdx : vcoef;
                             float nv = rndv(z2[r]) - coef * (p1b[p1ix(e, lane + 1)] - p1b[p1ix(e, lane)]);
                             if (npml > 0) nv = nv * damp(depth3(qb, mj[r], k), depth3(qb, mj[r], k + 1));
-                            nv = (qbv && mvz[r]) ? nv : 0.0f;
+                            nv = qbv ? pick(nv, mvz[r], mvzM[r]) : 0.0f;
                             zb0[r] = nv;
                             if (qbv) stT(Vzo, qb, oT[r], nv, true);
                         }
@@ -713,7 +761,7 @@ __device__ __forceinline__ void fused2_tile(const Pair2Args& a, const StencilPar
                 const float coef = HET ? (0.5f * (Bn(qb, he, -1) + Bn(qb, he, 0))) * dtdx : vcoef;
                 float nv = rndv(hz2v()) - coef * (p1b[p1ix(he, 0)] - pa1[0]);
                 if (npml > 0) nv = nv * damp(depth3(qb, hj, k0 - 1), depth3(qb, hj, k0));
-                sBzW[he] = (qbv && hvz1) ? nv : 0.0f;
+                sBzW[he] = qbv ? pick(nv, hvz1, hvzM[1]) : 0.0f;
             }
         }
 
@@ -730,7 +778,7 @@ __device__ __forceinline__ void fused2_tile(const Pair2Args& a, const StencilPar
                 const float dv = (xb1[r] - xb2[r]) + (yb1[r] - yd)
                                + (zb1[r] - lane_left(zb1[r], sBzR[e]));
                 const float out = pres(pa2[r], dv, HET ? kb2[r] : kap0, xw2[r], qc, mj[r], k,
-                                       medge[r] || (xec && mcell[r]));
+                                       edge_or_x(r, xec));
                 stT(Pout, qc, oT[r], out, false);
             }
         }
