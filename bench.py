@@ -334,6 +334,88 @@ def bench_adjoint(a):
     return 0
 
 
+def bench_amr(a):
+    """AMR (topology-driven CSR) stepper, gradient order 2: a 256^3 uniform
+    box grid (16.8 M cells, 50.1 M faces) from AcousticAMRGrid.build_uniform_box
+    with an 8-cell PML-typed shell under the anechoic border condition, one
+    pressure pulse; the order-2 AMR oracle (C, OpenMP) is the CPU baseline."""
+    from afdtd.amr import AcousticAMR, AcousticAMRGrid
+    n, dx, n_pml = 256, 0.01, 8
+    grid = AcousticAMRGrid.build_uniform_box(x_min=0, x_max=n * dx, y_min=0, y_max=n * dx,
+                                             z_min=0, z_max=n * dx, dx=dx).with_pml_shell(n_pml)
+    args = (grid.cell_centers, grid.cell_volumes, grid.open_volume_fraction, grid.cell_types,
+            grid.face_cell_neg, grid.face_cell_pos, grid.face_area, grid.face_open_fraction,
+            grid.face_distance, 343.0, 1.21, dx, 2)
+    g = AcousticAMR(*args)
+    g.set_border_condition(0, 3.0, 0.0, 0.0, n_pml, grid.bounds_min, grid.bounds_max)
+    g.inject_pressure_nearest([1.0, 1.2, 1.4], 1.0)
+    steps = a.steps if a.steps != 4000 else 1000
+    warm = min(a.warmup, 20)
+    if warm:
+        g.step(warm)
+    chunk = 250
+    t0 = time.perf_counter()
+    ms_f = ms_c = ms_t = 0.0
+    done = 0
+    while done < steps:
+        k = min(chunk, steps - done)
+        rc, t, f, c = g.step_timed(k)
+        if rc != 0:
+            raise SystemExit(f"amr_step_timed failed rc={rc}")
+        ms_t, ms_f, ms_c = ms_t + t, ms_f + f, ms_c + c
+        done += k
+    wall = time.perf_counter() - t0
+    if not np.all(np.isfinite(g.get_pressure())):
+        raise SystemExit("non-finite AMR pressure")
+    C_, F, E = g.n_cells, g.n_faces, g.n_csr_entries
+    # algorithmic HBM bytes per step (each array touched once):
+    #   faces kernel: record 16 + v read 4 + v write 4 per face, p read 4 per cell
+    #   cells kernel: record 16 + row start 4 + p read/write 8 per cell,
+    #                 CSR entry 8 per entry, v read 4 per face
+    b_faces = 24 * F + 4 * C_
+    b_cells = 28 * C_ + 8 * E + 4 * F
+    fa = b_faces / (ms_f / steps * 1e-3) / 1e9
+    ca = b_cells / (ms_c / steps * 1e-3) / 1e9
+    dom = "cells" if ms_c >= ms_f else "faces"
+    ach = ca if dom == "cells" else fa
+    line = {"metric": "cell-updates/s (AMR order-2 CSR stepper, 1 GPU)",
+            "value": C_ * steps / wall, "unit": "cell-updates/s", "n_gpus": 1, "steps": steps,
+            "warmup": warm, "ms_per_step": wall * 1e3 / steps, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+            "data": "synthetic (uniform box grid, one pulse)",
+            "config": {"workload": f"amr-uniform-box-{n}x{n}x{n}-order2-anechoic{n_pml}",
+                       "cells": C_, "faces": F, "csr_entries": E},
+            "kernel_ms_per_step": {"faces": ms_f / steps, "cells": ms_c / steps},
+            "event_ms_total": ms_t,
+            "roofline": {"bound": "hbm", "kernel": f"k_amr_{dom}", "achieved": ach,
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+                         "traffic": None, "bytes_per_launch": b_cells if dom == "cells"
+                         else b_faces,
+                         "bytes_per_cell_update": (b_faces + b_cells) / C_,
+                         "faces_GBps": fa, "cells_GBps": ca,
+                         "step_GBps": (b_faces + b_cells) / ((ms_f + ms_c) / steps * 1e-3) / 1e9}}
+    del g
+    if not a.no_cpu:
+        from oracle.amr_oracle import OracleAMR
+        o = OracleAMR(*args)
+        o.set_border_condition(0, 3.0, 0.0, 0.0, n_pml, grid.bounds_min, grid.bounds_max)
+        o.inject_pressure_nearest([1.0, 1.2, 1.4], 1.0)
+        o.step(1)
+        t0 = time.perf_counter()
+        k = 0
+        while time.perf_counter() - t0 < 12.0:
+            o.step(1)
+            k += 1
+        dt = time.perf_counter() - t0
+        cores = int(os.environ.get("OMP_NUM_THREADS") or len(os.sched_getaffinity(0)))
+        line["cpu_baseline"] = {"value": C_ * k / dt, "unit": "cell-updates/s", "cores": cores,
+                                "kind": "port",
+                                "sample": f"{k} steps of the same grid ({dt:.1f} s, float32 AMR "
+                                          f"oracle, OpenMP over faces and cells)"}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -366,6 +448,8 @@ def main():
         return bench_adjoint(a)
     if a.config == "geometry":
         return bench_geometry(a)
+    if a.config == "amr":
+        return bench_amr(a)
     w = make_workload(a.config, a.warmup + a.steps)
     s = make_solver(w, a.storage, a.kernel, a.tile_y, a.seg_x)
     total = a.warmup + a.steps

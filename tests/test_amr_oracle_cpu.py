@@ -1,0 +1,313 @@
+"""Pins of the AMR (CSR, gradient order 2) CPU oracle, and the AMR C-ABI
+checks that need no GPU.
+
+The reference's AMR stepper (csrc/kernels/acoustic_amr.cpp) needs Eigen and
+cannot be built here, and its tests hold no golden vectors for it.  The
+oracle is pinned instead by:
+  * the hand-derived one-step answer on the 8-cell line of
+    test_amr_throughput.py:26-64 (pulse in cell 0);
+  * exact flux conservation (sum of p*V*open) on a random irregular topology;
+  * reduction to the independently pinned uniform Yee oracle on a box grid;
+  * the border condition's special cases (R = 1 is the identity, graded
+    absorption reflects less than a partial panel, which reflects less than
+    a rigid end);
+  * brute-force nearest-cell injection (ties, skipped wall cells) and
+    uniform-grid scatter.
+"""
+import ctypes as C
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+from oracle.amr_oracle import OracleAMR
+from oracle.oracle import OracleFDTD
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def line_grid(nc=8, dx=0.05):
+    """The 8-cell row of test_amr_throughput.py:26-64."""
+    from afdtd.amr import AcousticAMRGrid
+    hx = dx / 2
+    nf = nc - 1
+    cc = np.zeros((nc, 3))
+    cc[:, 0] = np.arange(nc) * dx
+    return AcousticAMRGrid(
+        base_dx=dx, min_dx=dx, max_refinement_level=0,
+        bounds_min=np.array([0.0, -hx, -hx]), bounds_max=np.array([nc * dx, hx, hx]),
+        cell_centers=cc, cell_half_sizes=np.full(nc, hx), cell_levels=np.zeros(nc, np.int16),
+        cell_types=np.zeros(nc, np.uint8), importance=np.zeros(nc, np.uint8),
+        cell_volumes=np.full(nc, dx ** 3), open_volume_fraction=np.ones(nc),
+        face_cell_neg=np.arange(nf, dtype=np.int32), face_cell_pos=np.arange(1, nc, dtype=np.int32),
+        face_axis=np.zeros(nf, np.uint8), face_area=np.full(nf, dx ** 2),
+        face_open_fraction=np.ones(nf), face_distance=np.full(nf, dx))
+
+
+def random_topology(seed, nc=400, nf=1400, walls=True):
+    """Irregular cells and faces with random volumes, open fractions, areas,
+    distances; a few wall/plate cells; no self-faces."""
+    rng = np.random.default_rng(seed)
+    cc = rng.uniform(0.0, 1.0, (nc, 3))
+    vol = rng.uniform(2e-6, 4e-6, nc)     # stable at the order-2 dt for this degree
+    ov = rng.uniform(0.2, 1.0, nc)
+    ct = np.zeros(nc, np.uint8)
+    if walls:
+        ct[rng.choice(nc, nc // 20, replace=False)] = rng.integers(1, 3, nc // 20)
+        ov[rng.choice(nc, 5, replace=False)] = 0.0
+    a = rng.integers(0, nc, nf)
+    b = (a + rng.integers(1, nc, nf)) % nc
+    area = rng.uniform(0.5e-4, 1.5e-4, nf)
+    fo = rng.uniform(0.0, 1.0, nf)
+    dist = rng.uniform(0.008, 0.012, nf)
+    return dict(cell_centers=cc, cell_volumes=vol, open_volume_fraction=ov, cell_types=ct,
+                face_cell_neg=a.astype(np.int32), face_cell_pos=b.astype(np.int32),
+                face_area=area, face_open_fraction=fo, face_distance=dist,
+                c=343.0, rho_air=1.21, min_dx=0.01, gradient_order=2)
+
+
+def test_line_one_step_known_answer():
+    g = line_grid()
+    o = OracleAMR.from_grid(g)
+    dt = 0.77 * 0.05 / (343.0 * math.sqrt(3.0))
+    assert o.dt == pytest.approx(dt, rel=1e-15)
+    o.inject_pressure_nearest(g.cell_centers[0], 1.0)
+    o.step(1)
+    p, v = o.get_pressure().astype(np.float64), o.get_velocity().astype(np.float64)
+    # v0 = -(dt/rho) * (0 - 1)/dx ; p0 = 1 - C^2 ; p1 = C^2 with C = c dt/dx = 0.77/sqrt(3)
+    C2 = 0.77 ** 2 / 3.0
+    assert v[0] == pytest.approx(dt / (1.21 * 0.05), rel=1e-6)
+    assert np.all(v[1:] == 0.0)
+    assert p[0] == pytest.approx(1.0 - C2, rel=1e-6)
+    assert p[1] == pytest.approx(C2, rel=1e-5)
+    assert np.all(p[2:] == 0.0)
+    assert o.step_count == 1
+
+
+def test_flux_conservation_irregular():
+    t = random_topology(3, walls=False)
+    o = OracleAMR(**t)
+    rng = np.random.default_rng(4)
+    o.set_state(rng.standard_normal(400).astype(np.float32) * 0.1)
+    w = t["cell_volumes"] * np.maximum(t["open_volume_fraction"], 1e-12)
+    m0 = float(np.sum(o.get_pressure() * w))
+    scale = float(np.sum(np.abs(o.get_pressure()) * w))
+    o.step(200)
+    m1 = float(np.sum(o.get_pressure() * w))
+    assert abs(m1 - m0) < 1e-5 * scale
+    assert np.abs(o.get_pressure()).max() > 1e-4     # the field did move
+
+
+def test_uniform_box_reduces_to_yee_oracle():
+    from afdtd.amr import AcousticAMRGrid
+    Nx, Ny, Nz, dx = 12, 10, 14, 0.01
+    g = AcousticAMRGrid.build_uniform_box(x_min=0, x_max=Nx * dx, y_min=0, y_max=Ny * dx,
+                                          z_min=0, z_max=Nz * dx, dx=dx)
+    a = OracleAMR.from_grid(g)
+    y = OracleFDTD(Nx, Ny, Nz, dx, n_pml=0, order=2, courant=0.77 / math.sqrt(3.0),
+                   boundary=0, precision="f64")
+    x, yy, z = np.meshgrid(np.arange(Nx), np.arange(Ny), np.arange(Nz), indexing="ij")
+    p0 = np.exp(-((x - 5.3) ** 2 + (yy - 4.1) ** 2 + (z - 6.7) ** 2) / 4.0).ravel()
+    p0[17] += 0.5          # an off-centre spike breaks every symmetry
+    a.set_state(p0.astype(np.float32))
+    y.set_fields(P=p0.astype(np.float32).astype(np.float64))
+    a.step(150)
+    y.step(150)
+    pa = a.get_pressure().astype(np.float64)
+    py = y.get_pressure_field().ravel()
+    assert np.linalg.norm(pa - py) <= 1e-5 * np.linalg.norm(py)
+    # faces: x faces then y faces then z faces, each in neg-cell order
+    va = a.get_velocity().astype(np.float64)
+    vy_ = np.concatenate([y.get_velocity_x(), y.get_velocity_y(), y.get_velocity_z()])
+    assert np.linalg.norm(va - vy_) <= 1e-5 * np.linalg.norm(vy_)
+
+
+def _column(n=400, n_pml=20, mode=None, R=0.0):
+    """1-D column of cells along x, PML-typed at both x ends, with the border
+    box far away in y and z so only the x distance grades the absorber.
+    Returns the peak of the pulse that comes back from the +x end."""
+    from afdtd.amr import AcousticAMRGrid
+    dx = 0.01
+    g = AcousticAMRGrid.build_uniform_box(x_min=0, x_max=n * dx, y_min=0, y_max=dx, z_min=0,
+                                          z_max=dx, dx=dx)
+    xs = np.arange(n)
+    ct = np.zeros(n, np.uint8)
+    if mode is not None:
+        ct[(xs < n_pml) | (xs >= n - n_pml)] = 3
+    o = OracleAMR(g.cell_centers, g.cell_volumes, g.open_volume_fraction, ct, g.face_cell_neg,
+                  g.face_cell_pos, g.face_area, g.face_open_fraction, g.face_distance,
+                  343.0, 1.21, dx)
+    if mode is not None:
+        o.set_border_condition(mode, 3.0 if mode == 0 else 2.0, R, 0.0, n_pml,
+                               [0.0, -1.0, -1.0], [n * dx, 1.0, 1.0])
+    # right-going pulse: v = p/(rho c), v taken half a step earlier
+    C_ = 343.0 * o.dt / dx
+    p0 = np.exp(-((xs - 150) / 8.0) ** 2)
+    v0 = np.exp(-((xs[:-1] + 0.5 - 150 + 0.5 * C_) / 8.0) ** 2) / (1.21 * 343.0)
+    o.set_state(p0.astype(np.float32), v0.astype(np.float32))
+    o.step(int(round((n - 150 + n - 200) / C_)))      # out to x = n and back to x = 200
+    return float(np.abs(o.get_pressure()[150:250]).max())
+
+
+def test_border_condition_orders_reflection():
+    """Absorption behaves as graded damping should: a rigid end returns the
+    pulse, an absorber returns a small fraction that shrinks with depth, and a
+    room panel returns more the larger its R.  (At equal open volumes the
+    reference's face sigma, 2*(vp*a_n + vn*a_p)/(vn + vp), is the SUM of the two
+    cell values, twice the cells' own damping; with the steep anechoic profile
+    that leaves about 12 % at n_pml = 20.)"""
+    rigid = _column(mode=None)
+    anech20, anech40 = _column(mode=0), _column(mode=0, n_pml=40)
+    panel5, panel9 = _column(mode=1, R=0.5), _column(mode=1, R=0.9)
+    assert rigid > 0.95
+    assert anech20 < 0.15 * rigid
+    assert anech40 < 0.7 * anech20
+    assert panel5 < panel9 < 0.6 * rigid
+
+
+def test_room_panel_full_reflection_is_identity():
+    t = random_topology(5, walls=False)
+    t["cell_types"][::3] = 3
+    a, b = OracleAMR(**t), OracleAMR(**t)
+    b.set_border_condition(1, 2.0, 1.0, 0.0, 4, [0, 0, 0], [1, 1, 1])
+    pd, ps, vd = b.get_coeffs()
+    assert np.all(pd == 1.0) and np.all(ps == 1.0) and np.all(vd == 1.0)
+    s = np.random.default_rng(6).standard_normal(400).astype(np.float32)
+    for o in (a, b):
+        o.set_state(s)
+        o.step(40)
+    assert np.array_equal(a.get_pressure().view(np.uint32), b.get_pressure().view(np.uint32))
+
+
+def test_border_grades_only_pml_cells():
+    t = random_topology(7, walls=False)
+    t["cell_types"][:200] = 3
+    o = OracleAMR(**t)
+    o.set_border_condition(0, 3.0, 0.0, 0.0, 30, [0, 0, 0], [1, 1, 1])
+    pd, ps, vd = o.get_coeffs()
+    assert np.all(pd[200:] == 1.0) and np.all(ps[200:] == 1.0)
+    assert np.all(pd[:200] <= 1.0) and np.all(pd[:200] > 0.0) and np.any(pd[:200] < 1.0)
+    # P_src = (1 - P_damp)/s with s = -log(P_damp): the exact integrating
+    # factor; float cancellation in 1 - exp(-s) limits it for small s
+    assert np.all((ps > 0) & (ps <= 1.0 + 1e-3))
+    s = -np.log(pd[:200].astype(np.float64))
+    big = s > 1e-2
+    assert big.sum() > 20
+    assert np.allclose(ps[:200][big], (1 - np.exp(-s[big])) / s[big], rtol=1e-4)
+    neg, pos = t["face_cell_neg"], t["face_cell_pos"]
+    interior = (neg >= 200) & (pos >= 200)
+    assert np.all(vd[interior] == 1.0) and np.all(vd[~interior] <= 1.0)
+
+
+def _lround(x):
+    return np.where(x >= 0, np.floor(x + 0.5), -np.floor(-x + 0.5)).astype(np.int64)
+
+
+def test_inject_nearest_brute_force_and_ties():
+    t = random_topology(8)
+    o = OracleAMR(**t)
+    cc, ct, ov = t["cell_centers"], t["cell_types"], t["open_volume_fraction"]
+    cand = np.where(((ct == 0) | (ct == 3)) & (ov > 0))[0]
+    rng = np.random.default_rng(9)
+    for q in rng.uniform(0, 1, (20, 3)):
+        d2 = ((cc[cand] - q) ** 2).sum(1)
+        want = cand[np.argmin(d2)]
+        o.reset()
+        o.inject_pressure_nearest(q, 2.5)
+        p = o.get_pressure()
+        assert p[want] == np.float32(2.5) and np.count_nonzero(p) == 1
+    # exact tie between two cells: the lower index wins
+    g = line_grid()
+    o2 = OracleAMR.from_grid(g)
+    o2.inject_pressure_nearest([0.025, 0.0, 0.0], 1.0)
+    p = o2.get_pressure()
+    assert p[0] == 1.0 and p[1] == 0.0
+    with pytest.raises(RuntimeError):
+        o2.inject_pressure_nearest([0.0, 0.0, 0.0], float("nan"))
+
+
+def test_scatter_uniform_brute_force():
+    t = random_topology(10)
+    o = OracleAMR(**t)
+    o.set_state(np.arange(400, dtype=np.float32) + 1.0)
+    Nx, Ny, Nz = 7, 5, 6
+    bmin, bmax = np.array([0.0, 0.1, 0.0]), np.array([1.0, 0.9, 1.1])
+    got = o.scatter_pressure_uniform(Nx, Ny, Nz, bmin, bmax)
+    want = np.zeros((Nx, Ny, Nz), np.float32)
+    r = (np.array([Nx, Ny, Nz]) - 1) / (bmax - bmin)
+    for ci, q in enumerate(t["cell_centers"]):
+        ix, iy, iz = _lround((q - bmin) * r)
+        if 0 <= ix < Nx and 0 <= iy < Ny and 0 <= iz < Nz:
+            want[ix, iy, iz] = ci + 1.0
+    assert np.array_equal(got, want)
+
+
+def test_stability_guard_and_validation():
+    g = line_grid()
+    o = OracleAMR.from_grid(g)
+    o.inject_pressure_nearest(g.cell_centers[3], 5e3)
+    assert o.step_rc(5) == -3 and o.step_count == 0      # |p| > 2e3 at step 0
+    o.reset()
+    o.step(3)
+    s = np.zeros(8, np.float32)
+    s[2] = np.inf
+    o.set_state(s)
+    assert o.step_rc(5) == -3 and o.step_count == 3      # non-finite: every step
+    t = random_topology(11)
+    t["face_cell_pos"] = t["face_cell_neg"].copy()       # self-face
+    with pytest.raises(RuntimeError):
+        OracleAMR(**t)
+    with pytest.raises(RuntimeError):
+        OracleAMR.from_grid(g, gradient_order=8)         # order 8 not restated
+
+
+# ── C ABI without a GPU ─────────────────────────────────────────────────
+
+def _amr_declared():
+    txt = open(os.path.join(ROOT, "include", "afdtd_amr.h")).read()
+    return sorted(set(re.findall(r"SK_API\s+[\w\s\*]*?\b(amr_\w+)\s*\(", txt)))
+
+
+REFERENCE_AMR_ORDER2 = (
+    "amr_create", "amr_get_last_create_error_stage", "amr_get_last_create_error_message",
+    "amr_get_create_progress_active", "amr_get_create_progress_done_faces",
+    "amr_get_create_progress_total_faces", "amr_destroy", "amr_step", "amr_reset",
+    "amr_inject_pressure_nearest", "amr_get_pressure", "amr_get_cell_centers",
+    "amr_get_velocity", "amr_get_dt", "amr_get_step_count", "amr_scatter_pressure_uniform",
+    "amr_set_border_condition")
+
+
+def test_amr_header_and_exports():
+    from afdtd import LIB_PATH
+    names = _amr_declared()
+    assert all(n in names for n in REFERENCE_AMR_ORDER2)
+    L = C.CDLL(LIB_PATH)
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_amr_abi_argument_checks_without_device():
+    from afdtd.amr import lib, last_create_error
+    L = lib()
+    assert L.amr_step(None, 1) == -1
+    assert L.amr_reset(None) == -1
+    assert L.amr_get_pressure(None, None, 0) == -1
+    assert L.amr_get_dt(None) == 0.0 and L.amr_get_step_count(None) == 0
+    L.amr_destroy(None)
+    t = random_topology(12)
+    t["face_distance"][5] = 0.0
+    from afdtd.amr import _p
+    k = [np.ascontiguousarray(t[n], d) for n, d in (
+        ("cell_centers", np.float64), ("cell_volumes", np.float64),
+        ("open_volume_fraction", np.float64), ("cell_types", np.uint8),
+        ("face_cell_neg", np.int32), ("face_cell_pos", np.int32), ("face_area", np.float64),
+        ("face_open_fraction", np.float64), ("face_distance", np.float64))]
+    h = L.amr_create(400, *[_p(a) for a in k[:4]], k[4].size, *[_p(a) for a in k[4:]],
+                     343.0, 1.21, 0.01, 2)
+    assert not h and last_create_error()[0] == "validate_inputs"
+    k[8][5] = 0.01
+    h = L.amr_create(400, *[_p(a) for a in k[:4]], k[4].size, *[_p(a) for a in k[4:]],
+                     343.0, 1.21, 0.01, 8)
+    assert not h and "gradient_order 8" in last_create_error()[1]
