@@ -371,12 +371,19 @@ def bench_amr(a):
     # algorithmic HBM bytes per step (each array touched once):
     #   faces kernel: record 16 + v read 4 + v write 4 per face, p read 4 per cell
     #   cells kernel: record 16 + row start 4 + p read/write 8 per cell,
-    #                 CSR entry 8 per entry, v read 4 per face
+    #                 CSR index 4 per entry, v and face weight 4 + 4 per face
     b_faces = 24 * F + 4 * C_
-    b_cells = 28 * C_ + 8 * E + 4 * F
+    b_cells = 28 * C_ + 4 * E + 8 * F
     fa = b_faces / (ms_f / steps * 1e-3) / 1e9
     ca = b_cells / (ms_c / steps * 1e-3) / 1e9
     dom = "cells" if ms_c >= ms_f else "faces"
+    # HBM bytes per k_amr_cells launch counted with rocprofv3 on this workload
+    amr_traffic = None
+    prof = os.path.join(ROOT, "profiles", "amr_uniform_box_256cube_order2_anechoic8.txt")
+    if os.path.exists(prof):
+        for ln in open(prof):
+            if ln.startswith("traffic_bytes_per_launch:"):
+                amr_traffic = int(ln.split(":")[1])
     ach = ca if dom == "cells" else fa
     line = {"metric": "cell-updates/s (AMR order-2 CSR stepper, 1 GPU)",
             "value": C_ * steps / wall, "unit": "cell-updates/s", "n_gpus": 1, "steps": steps,
@@ -389,8 +396,8 @@ def bench_amr(a):
             "event_ms_total": ms_t,
             "roofline": {"bound": "hbm", "kernel": f"k_amr_{dom}", "achieved": ach,
                          "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-                         "traffic": None, "bytes_per_launch": b_cells if dom == "cells"
-                         else b_faces,
+                         "traffic": amr_traffic if dom == "cells" else None,
+                         "bytes_per_launch": b_cells if dom == "cells" else b_faces,
                          "bytes_per_cell_update": (b_faces + b_cells) / C_,
                          "faces_GBps": fa, "cells_GBps": ca,
                          "step_GBps": (b_faces + b_cells) / ((ms_f + ms_c) / steps * 1e-3) / 1e9}}

@@ -2,8 +2,8 @@
 // the reference's amr_* C ABI (csrc/include/acoustic_amr.h) for gradient
 // order 2, on the GPU.
 //
-// Device layout (HBM), chosen so that every per-element record is one
-// 128-bit (faces, cells) or 64-bit (CSR entries) load:
+// Device layout (HBM): per-face and per-cell records are one 128-bit load,
+// a CSR entry is one 32-bit index whose sign selects the weight's sign:
 //   face[f]  int4   {neg cell, pos cell, bits(1/d), bits(V_damp)}
 //   v[f]     float  face velocity (neg -> pos)
 //   p[c]     float  cell pressure
@@ -11,7 +11,9 @@
 //                   kind bit 0: wall/plate cell (zeroed after the update)
 //                   kind bit 1: acoustic cell (injection candidate)
 //   start[c] int    CSR row start (n_cells + 1)
-//   csr[k]   int2   {face, bits(+-area*open)} in face order per cell
+//   csr[k]   int    face index f (the cell is the face's neg cell) or ~f (its
+//                   pos cell), in face order per cell
+//   w[f]     float  area*open; the CSR weight is +w[f] or -w[f]
 // A step is two kernels on the handle's stream: k_amr_faces (one thread per
 // face, fully coalesced record + gathered pressure pair) and k_amr_cells
 // (one thread per cell: CSR walk, fused divergence + pressure update, wall
@@ -42,7 +44,8 @@ struct AcousticAMRState {
     float* d_p = nullptr;
     float4* d_cell = nullptr;
     int* d_start = nullptr;
-    int2* d_csr = nullptr;
+    int* d_csr = nullptr;
+    float* d_w = nullptr;
     double* d_cent = nullptr;
     int* d_status = nullptr;                 // [0] failing step, [1] nearest cell
     unsigned long long* d_best = nullptr;    // nearest squared distance (bits)
@@ -58,35 +61,85 @@ namespace {
 
 void set_stage(const char* s, const char* m) { g_amr_stage = s; g_amr_msg = m; }
 
+// Blocks are dispatched round-robin over the 8 XCDs (block b -> XCD b % 8);
+// give each XCD a contiguous range of tiles instead, so the neighbour
+// pressures and velocities one tile gathers were most likely brought into
+// that XCD's L2 by the tiles just before it.
+__device__ inline int xcd_tile(int nblk) {
+    const int b = blockIdx.x, x = b & 7, i = b >> 3;
+    const int q = nblk >> 3, r = nblk & 7;           // XCD x owns q + (x < r) tiles
+    return x * q + min(x, r) + i;
+}
+
+constexpr int kFaceTile = 512;                       // faces per 256-thread block
+
 __global__ __launch_bounds__(256) void k_amr_faces(const int4* __restrict__ face,
                                                    float* __restrict__ v,
                                                    const float* __restrict__ p, int nf,
                                                    float dt_over_rho,
                                                    const int* __restrict__ status, int s) {
     if (status[0] < s) return;                       // an earlier step of this call failed
-    const int f = blockIdx.x * 256 + threadIdx.x;
-    if (f >= nf) return;
-    const int4 r = face[f];
-    const float g = (p[r.y] - p[r.x]) * __int_as_float(r.z);      // :1149-1150
-    v[f] = (v[f] - dt_over_rho * g) * __int_as_float(r.w);        // :1151
+    const int f0 = xcd_tile(gridDim.x) * kFaceTile + threadIdx.x;
+    const int f1 = f0 + 256;
+    // both faces' loads are issued before either update
+    int4 r0 = make_int4(0, 0, 0, 0), r1 = r0;
+    float v0 = 0.0f, v1 = 0.0f;
+    if (f0 < nf) { r0 = face[f0]; v0 = v[f0]; }
+    if (f1 < nf) { r1 = face[f1]; v1 = v[f1]; }
+    const float a0 = p[r0.y], b0 = p[r0.x], a1 = p[r1.y], b1 = p[r1.x];
+    if (f0 < nf) {
+        const float g = (a0 - b0) * __int_as_float(r0.z);            // :1149-1150
+        v[f0] = (v0 - dt_over_rho * g) * __int_as_float(r0.w);        // :1151
+    }
+    if (f1 < nf) {
+        const float g = (a1 - b1) * __int_as_float(r1.z);
+        v[f1] = (v1 - dt_over_rho * g) * __int_as_float(r1.w);
+    }
 }
+
+// One 256-thread block updates 256 consecutive cells.  The block's CSR rows
+// are one contiguous entry range; when it fits kCsrStage entries it is first
+// copied into LDS with coalesced loads, so each lane's walk over its own row
+// (about six entries on a box grid, 24 bytes apart between lanes) reads LDS
+// instead of scattering loads over HBM lines.  Negating the face weight is
+// exact, so +-w[f] equals the reference's pre-signed CSR weight.
+constexpr int kCsrStage = 2048;
 
 __global__ __launch_bounds__(256) void k_amr_cells(const float4* __restrict__ cell,
                                                    const int* __restrict__ start,
-                                                   const int2* __restrict__ csr,
+                                                   const int* __restrict__ csr,
+                                                   const float* __restrict__ w,
                                                    const float* __restrict__ v,
                                                    float* __restrict__ p, int nc, float bulk,
                                                    int check_max, int* status, int s) {
+    __shared__ int s_csr[kCsrStage];
     if (status[0] < s) return;
-    const int c = blockIdx.x * 256 + threadIdx.x;
+    const int c0 = xcd_tile(gridDim.x) * 256;
+    const int c = c0 + threadIdx.x;
+    const int k_lo = start[c0];
+    const int k_hi = start[min(c0 + 256, nc)];
+    const bool staged = (k_hi - k_lo) <= kCsrStage;  // uniform over the block
+    if (staged) {
+        for (int k = k_lo + threadIdx.x; k < k_hi; k += 256) s_csr[k - k_lo] = csr[k];
+        __syncthreads();
+    }
     bool bad = false;
     if (c < nc) {
         const float4 m = cell[c];
-        const int k1 = start[c + 1];
+        const int k0 = start[c], k1 = start[c + 1];
         float d = 0.0f;
-        for (int k = start[c]; k < k1; ++k) {                        // :1166-1168
-            const int2 e = csr[k];
-            d += __int_as_float(e.y) * v[e.x];
+        if (staged) {
+            for (int k = k0; k < k1; ++k) {                          // :1166-1168
+                const int e = s_csr[k - k_lo];
+                const int f = e < 0 ? ~e : e;
+                d += (e < 0 ? -w[f] : w[f]) * v[f];
+            }
+        } else {
+            for (int k = k0; k < k1; ++k) {
+                const int e = csr[k];
+                const int f = e < 0 ? ~e : e;
+                d += (e < 0 ? -w[f] : w[f]) * v[f];
+            }
         }
         float pn = p[c] * m.x - bulk * d * m.y * m.z;                 // :1169-1170
         if (__float_as_int(m.w) & 1) pn = 0.0f;                       // :1174-1176
@@ -195,7 +248,7 @@ void destroy_state(AcousticAMRState* st) {
     if (!st) return;
     if (st->stream) (void)hipStreamSynchronize(st->stream);
     void* bufs[] = {st->d_face, st->d_v, st->d_p, st->d_cell, st->d_start,
-                    st->d_csr,  st->d_cent, st->d_status, st->d_best};
+                    st->d_csr,  st->d_w,    st->d_cent, st->d_status, st->d_best};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (st->stream) (void)hipStreamDestroy(st->stream);
@@ -266,12 +319,12 @@ int step_impl(AcousticAMRState* st, int n, float* ms_total, float* ms_faces, flo
     k_amr_status_init<<<1, 1, 0, st->stream>>>(st->d_status, st->d_best);
     for (int s = 0; s < n; ++s) {
         if (ms_total) (void)hipEventRecord(ev[2 * (size_t)s], st->stream);
-        k_amr_faces<<<blocks(st->nf), 256, 0, st->stream>>>(st->d_face, st->d_v, st->d_p, st->nf,
-                                                          dt_over_rho, st->d_status, s);
+        k_amr_faces<<<(st->nf + kFaceTile - 1) / kFaceTile, 256, 0, st->stream>>>(
+            st->d_face, st->d_v, st->d_p, st->nf, dt_over_rho, st->d_status, s);
         if (ms_total) (void)hipEventRecord(ev[2 * (size_t)s + 1], st->stream);
         const int check_max = ((st->step_count + s) & 0xFF) == 0;
         k_amr_cells<<<blocks(st->nc), 256, 0, st->stream>>>(st->d_cell, st->d_start, st->d_csr,
-                                                          st->d_v, st->d_p, st->nc, bulk,
+                                                          st->d_w, st->d_v, st->d_p, st->nc, bulk,
                                                           check_max, st->d_status, s);
     }
     if (ms_total) (void)hipEventRecord(ev[2 * (size_t)n], st->stream);
@@ -363,16 +416,12 @@ SK_API AcousticAMRState* amr_create(int nc, const double* cc, const double* vol,
     for (int f = 0; f < nf; ++f) { start[fn[f] + 1]++; start[fp[f] + 1]++; }
     for (int i = 1; i <= nc; ++i) start[i] += start[i - 1];
     st->ne = start[nc];
-    std::vector<int2> csr(st->ne);
+    std::vector<int> csr(st->ne);
     {
         std::vector<int> fill(nc, 0);
         for (int f = 0; f < nf; ++f) {                               // :837-848
-            int wp, wn;
-            const float w = ffc[f], mw = -ffc[f];
-            memcpy(&wp, &w, 4);
-            memcpy(&wn, &mw, 4);
-            csr[start[fn[f]] + fill[fn[f]]++] = make_int2(f, wp);
-            csr[start[fp[f]] + fill[fp[f]]++] = make_int2(f, wn);
+            csr[start[fn[f]] + fill[fn[f]]++] = f;           // weight +ffc
+            csr[start[fp[f]] + fill[fp[f]]++] = ~f;          // weight -ffc
         }
     }
     st->pdamp.assign(nc, 1.0f);                                      // :852-859
@@ -388,7 +437,8 @@ SK_API AcousticAMRState* amr_create(int nc, const double* cc, const double* vol,
     al((void**)&st->d_p, sizeof(float) * (size_t)nc);
     al((void**)&st->d_cell, sizeof(float4) * (size_t)nc);
     al((void**)&st->d_start, sizeof(int) * ((size_t)nc + 1));
-    al((void**)&st->d_csr, sizeof(int2) * (size_t)st->ne);
+    al((void**)&st->d_csr, sizeof(int) * (size_t)st->ne);
+    al((void**)&st->d_w, sizeof(float) * (size_t)nf);
     al((void**)&st->d_cent, sizeof(double) * 3 * (size_t)nc);
     al((void**)&st->d_status, sizeof(int) * 2);
     al((void**)&st->d_best, sizeof(unsigned long long));
@@ -398,7 +448,10 @@ SK_API AcousticAMRState* amr_create(int nc, const double* cc, const double* vol,
         e = hipMemcpyAsync(st->d_start, start.data(), sizeof(int) * ((size_t)nc + 1),
                            hipMemcpyHostToDevice, st->stream);
     if (e == hipSuccess)
-        e = hipMemcpyAsync(st->d_csr, csr.data(), sizeof(int2) * (size_t)st->ne,
+        e = hipMemcpyAsync(st->d_csr, csr.data(), sizeof(int) * (size_t)st->ne,
+                           hipMemcpyHostToDevice, st->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(st->d_w, ffc.data(), sizeof(float) * (size_t)nf,
                            hipMemcpyHostToDevice, st->stream);
     if (e == hipSuccess)
         e = hipMemcpyAsync(st->d_cent, cc, sizeof(double) * 3 * (size_t)nc,
