@@ -7,8 +7,7 @@
  * return codes and ownership rules; each cites the declaration it replaces
  * (csrc/include/acoustic_amr.h:line) and the definition whose behaviour it
  * reproduces (csrc/kernels/acoustic_amr.cpp:line).  Not provided: gradient
- * order 8 (amr_create returns NULL with stage "validate_inputs"), the AMR
- * plate (amr_setup_plate and its samplers) and the mic samplers.
+ * order 8 (amr_create returns NULL with stage "validate_inputs").
  *
  * Part 2 declares extensions under new names (state upload, timed steps).
  *
@@ -128,6 +127,57 @@ SK_API int amr_scatter_pressure_uniform(const AcousticAMRState* st, int Nx, int 
 SK_API int amr_set_border_condition(AcousticAMRState* st, int mode, float sigma_order,
                                     float R_reflection, float border_Z_match, int n_pml,
                                     const double* bounds_min, const double* bounds_max);
+
+/* acoustic_amr.h:165-186; acoustic_amr.cpp:1412-1531.  Kirchhoff plate on a
+ * plate_Nx x plate_Ny grid (flat index j + plate_Ny*i) coupled to AMR faces
+ * above (neg = plate side, air on pos) and below (air on neg), or to one
+ * fallback cell per side.  Each step: the coupled faces take the plate
+ * velocity (+-(w - w_prev)/dt) after the face update; after the pressure
+ * update the plate advances (13-point biharmonic, Rayleigh damping, acoustic
+ * load (P_below - P_above)*dx^2, external force, which is then cleared).
+ * Besides the reference's checks, active flat indices outside the plate
+ * grid are rejected (SK_ERR_DIM_MISMATCH); they must be distinct. */
+SK_API int amr_setup_plate(AcousticAMRState* st, int plate_Nx, int plate_Ny, float plate_dx,
+                           const float plate_origin[3], const uint8_t* plate_active,
+                           float plate_rho_h, float plate_D, float plate_alpha_M,
+                           float plate_beta_K, int n_active_plate,
+                           const int32_t* plate_active_flat_idx,
+                           const int32_t* face_above_starts, const int32_t* face_above_idx,
+                           const float* face_above_wgt, const int32_t* face_below_starts,
+                           const int32_t* face_below_idx, const float* face_below_wgt,
+                           const int32_t* cell_above, const int32_t* cell_below);
+
+/* acoustic_amr.h:195-210; acoustic_amr.cpp:1534-1563 */
+SK_API int amr_set_bridge_plate_sources(AcousticAMRState* st, int n, const int32_t* idx,
+                                        const float* wgt);
+SK_API int amr_set_neck_plate_sources(AcousticAMRState* st, int n, const int32_t* idx,
+                                      const float* wgt);
+
+/* acoustic_amr.h:226-244; acoustic_amr.cpp:1567-1597 (external force
+ * accumulated in call order; an out-of-range node stops the loop with
+ * SK_ERR_DIM_MISMATCH after the entries before it were applied) */
+SK_API int amr_inject_bridge_drive(AcousticAMRState* st, const float* cell_drives, int n_cells,
+                                   float force_scale);
+SK_API int amr_inject_plate_force(AcousticAMRState* st, int n, const int32_t* plate_idx,
+                                  const float* wgt, float total_force_N);
+
+/* acoustic_amr.h:256-283; acoustic_amr.cpp:1600-1644 */
+SK_API int amr_sample_plate_displacement_batch(const AcousticAMRState* st, int n_points,
+                                               const int* idx4, const float* wgt4,
+                                               float* out_w);
+SK_API int amr_sample_plate_weighted(const AcousticAMRState* st, int n, const int* plate_idx,
+                                     const float* wgt, float* out_w);
+SK_API int amr_get_plate_displacement(const AcousticAMRState* st, float* out, int out_len);
+
+/* acoustic_amr.h:301-325; acoustic_amr.cpp:1647-1792.  Pressure: inverse-
+ * distance weights over acoustic cells within 2*cbrt(mean acoustic volume)
+ * of the (envelope-clamped) position; velocity (need_velocity != 0):
+ * area-weighted face normals within the same radius.  A second set-up call
+ * appends to each mic's lists, as the reference does. */
+SK_API int amr_setup_mic_samplers(AcousticAMRState* st, int n_mics, const float* pos_xyz,
+                                  const float* axis_xyz, int need_velocity);
+SK_API int amr_sample_mics(const AcousticAMRState* st, int n_mics, float* out_p, float* out_vx,
+                           float* out_vy, float* out_vz);
 
 /* ═════════════════════ Part 2: extensions (no reference counterpart) ═══ */
 

@@ -50,6 +50,26 @@ typedef struct Amo {
     int *acoustic; int n_acoustic;   /* (:783-785) */
     int *wallplate; int n_wallplate; /* (:813-816) */
     int *csr_start, *csr_face; float *csr_w;  /* (:818-849) */
+    double *volumes;                          /* nc (mic search radius) */
+    double *face_area, *face_open;            /* nf (mic velocity weights) */
+    /* Kirchhoff plate (:1412-1531) */
+    int plate_on, pNx, pNy, n_pnodes, n_active;
+    float p_dx, p_rh, p_D, p_aM, p_bK;
+    float *w, *w_prev, *w_new, *ext;
+    int *active;                              /* flat plate index per active node */
+    int *ab_start, *ab_face, *ab_side;        /* faces above: CSR over nodes, air cell */
+    float *ab_wgt;
+    int *bl_start, *bl_face, *bl_side;        /* faces below */
+    float *bl_wgt;
+    int *cell_above, *cell_below;
+    int n_bridge, n_neck;
+    int *bridge_idx, *neck_idx;
+    float *bridge_wgt, *neck_wgt;
+    /* mic samplers (:1647-1760): per mic growing lists, as the reference's
+     * vectors (a second set-up appends) */
+    int n_mics;
+    int *mp_n, **mp_cell; float **mp_w;       /* pressure: cells, weights */
+    int *mv_n, **mv_face; float **mv_wx, **mv_wy, **mv_wz;
 } Amo;
 
 /* Input validation, acoustic_amr.cpp:275-311. */
@@ -84,6 +104,18 @@ SK_EXPORT void amo_destroy(Amo* a)
     free(a->pdamp); free(a->psrc); free(a->vdamp); free(a->alpha);
     free(a->acoustic); free(a->wallplate);
     free(a->csr_start); free(a->csr_face); free(a->csr_w);
+    free(a->volumes); free(a->face_area); free(a->face_open);
+    free(a->w); free(a->w_prev); free(a->w_new); free(a->ext); free(a->active);
+    free(a->ab_start); free(a->ab_face); free(a->ab_side); free(a->ab_wgt);
+    free(a->bl_start); free(a->bl_face); free(a->bl_side); free(a->bl_wgt);
+    free(a->cell_above); free(a->cell_below);
+    free(a->bridge_idx); free(a->neck_idx); free(a->bridge_wgt); free(a->neck_wgt);
+    for (int m = 0; m < a->n_mics; ++m) {
+        free(a->mp_cell[m]); free(a->mp_w[m]);
+        free(a->mv_face[m]); free(a->mv_wx[m]); free(a->mv_wy[m]); free(a->mv_wz[m]);
+    }
+    free(a->mp_n); free(a->mp_cell); free(a->mp_w);
+    free(a->mv_n); free(a->mv_face); free(a->mv_wx); free(a->mv_wy); free(a->mv_wz);
     free(a);
 }
 
@@ -110,6 +142,12 @@ SK_EXPORT Amo* amo_create(int nc, const double* cc, const double* vol, const dou
     a->pos = (int*)malloc(sizeof(int) * (size_t)nf);
     memcpy(a->neg, fn, sizeof(int) * (size_t)nf);
     memcpy(a->pos, fp, sizeof(int) * (size_t)nf);
+    a->volumes = (double*)malloc(sizeof(double) * (size_t)nc);
+    memcpy(a->volumes, vol, sizeof(double) * (size_t)nc);
+    a->face_area = (double*)malloc(sizeof(double) * (size_t)nf);
+    memcpy(a->face_area, fa, sizeof(double) * (size_t)nf);
+    a->face_open = (double*)malloc(sizeof(double) * (size_t)nf);
+    memcpy(a->face_open, fo, sizeof(double) * (size_t)nf);
     a->p = (float*)calloc((size_t)nc, sizeof(float));
     a->v = (float*)calloc((size_t)nf, sizeof(float));
     a->acoustic = (int*)malloc(sizeof(int) * (size_t)nc);
@@ -165,6 +203,11 @@ SK_EXPORT int amo_reset(Amo* a)
     if (!a) return AMO_ERR_NULL;
     memset(a->p, 0, sizeof(float) * (size_t)a->nc);
     memset(a->v, 0, sizeof(float) * (size_t)a->nf);
+    if (a->plate_on) {
+        memset(a->w, 0, sizeof(float) * (size_t)a->n_pnodes);
+        memset(a->w_prev, 0, sizeof(float) * (size_t)a->n_pnodes);
+        memset(a->ext, 0, sizeof(float) * (size_t)a->n_pnodes);
+    }
     a->step_count = 0;
     return AMO_OK;
 }
@@ -178,8 +221,380 @@ SK_EXPORT int amo_set_state(Amo* a, const float* p, const float* v)
     return AMO_OK;
 }
 
-/* amr_step, order-2 branch, acoustic_amr.cpp:1100-1204 (plate hooks are
- * no-ops without a plate). */
+/* plate_biharmonic, acoustic_amr.cpp:939-954: 13-point stencil, zero
+ * outside the plate grid. */
+static float amo_biharm(const float* w, int i, int j, int Nx, int Ny, float idx4)
+{
+#define W_(ii, jj) (((ii) < 0 || (ii) >= Nx || (jj) < 0 || (jj) >= Ny) ? 0.0f : w[(ii) * Ny + (jj)])
+    float L = W_(i - 2, j) + W_(i + 2, j) + W_(i, j - 2) + W_(i, j + 2)
+            + 2.0f * (W_(i - 1, j - 1) + W_(i - 1, j + 1) + W_(i + 1, j - 1) + W_(i + 1, j + 1))
+            - 8.0f * (W_(i - 1, j) + W_(i + 1, j) + W_(i, j - 1) + W_(i, j + 1))
+            + 20.0f * W_(i, j);
+#undef W_
+    return L * idx4;
+}
+
+/* amr_apply_plate_bc, acoustic_amr.cpp:1050-1072 */
+static void amo_plate_bc(Amo* a, float dt)
+{
+    for (int n = 0; n < a->n_active; ++n) {
+        const int flat = a->active[n];
+        const float vp = (a->w[flat] - a->w_prev[flat]) / dt;
+        for (int k = a->ab_start[n]; k < a->ab_start[n + 1]; ++k) a->v[a->ab_face[k]] = vp;
+        for (int k = a->bl_start[n]; k < a->bl_start[n + 1]; ++k) a->v[a->bl_face[k]] = -vp;
+    }
+}
+
+/* amr_plate_step, acoustic_amr.cpp:956-1047 */
+static void amo_plate_step(Amo* a, float dt)
+{
+    const int Nx = a->pNx, Ny = a->pNy;
+    const float dx = a->p_dx;
+    const float dx4 = 1.0f / (dx * dx * dx * dx);
+    const float dt2 = dt * dt, inv_rh = 1.0f / a->p_rh;
+    const float damp_fwd = 1.0f + 0.5f * a->p_aM * dt;
+    const float damp_bwd = 1.0f - 0.5f * a->p_aM * dt;
+    const float cD0 = a->p_D * (1.0f + a->p_bK / dt);
+    const float cDp = a->p_D * (a->p_bK / dt);
+    memset(a->w_new, 0, sizeof(float) * (size_t)a->n_pnodes);
+    for (int n = 0; n < a->n_active; ++n) {
+        const int flat = a->active[n], i = flat / Ny, j = flat % Ny;
+        float Pa = 0.0f, Pb = 0.0f;
+        if (a->ab_start[n + 1] > a->ab_start[n]) {
+            for (int k = a->ab_start[n]; k < a->ab_start[n + 1]; ++k)
+                Pa += a->ab_wgt[k] * a->p[a->ab_side[k]];
+        } else if (a->cell_above[n] >= 0) {
+            Pa = a->p[a->cell_above[n]];
+        }
+        if (a->bl_start[n + 1] > a->bl_start[n]) {
+            for (int k = a->bl_start[n]; k < a->bl_start[n + 1]; ++k)
+                Pb += a->bl_wgt[k] * a->p[a->bl_side[k]];
+        } else if (a->cell_below[n] >= 0) {
+            Pb = a->p[a->cell_below[n]];
+        }
+        const float F_acou = (Pb - Pa) * dx * dx;
+        const float F_ext = a->ext[flat];
+        const float L4w = amo_biharm(a->w, i, j, Nx, Ny, dx4);
+        const float L4wp = amo_biharm(a->w_prev, i, j, Nx, Ny, dx4);
+        const float rhs = F_acou + F_ext - cD0 * L4w + cDp * L4wp;
+        a->w_new[flat] = (damp_bwd * (2.0f * a->w[flat] - a->w_prev[flat]) + dt2 * inv_rh * rhs)
+                         / damp_fwd;
+    }
+    for (int n = 0; n < a->n_active; ++n) {
+        const int flat = a->active[n];
+        a->w_prev[flat] = a->w[flat];
+        a->w[flat] = a->w_new[flat];
+    }
+    memset(a->ext, 0, sizeof(float) * (size_t)a->n_pnodes);
+}
+
+/* amr_setup_plate, acoustic_amr.cpp:1412-1531 */
+SK_EXPORT int amo_setup_plate(Amo* a, int Nx, int Ny, float pdx, const float* origin,
+                              const uint8_t* act, float rh, float D, float aM, float bK,
+                              int n_active, const int32_t* flat_idx,
+                              const int32_t* ab_start, const int32_t* ab_idx, const float* ab_w,
+                              const int32_t* bl_start, const int32_t* bl_idx, const float* bl_w,
+                              const int32_t* c_above, const int32_t* c_below)
+{
+    (void)origin;
+    if (!a) return AMO_ERR_NULL;
+    if (!act || !flat_idx || !ab_start || !bl_start || !c_above || !c_below) return AMO_ERR_NULL;
+    if (Nx <= 0 || Ny <= 0 || pdx <= 0.0f) return AMO_ERR_DIM;
+    if (rh <= 0.0f || D <= 0.0f) return AMO_ERR_DIM;
+    if (n_active < 0) return AMO_ERR_DIM;
+    const int nn = Nx * Ny;
+    int cnt = 0;
+    for (int k = 0; k < nn; ++k) if (act[k]) ++cnt;
+    if (cnt != n_active) return AMO_ERR_DIM;
+    const int na = ab_start[n_active], nb = bl_start[n_active];
+    for (int k = 0; k < na; ++k) if (ab_idx[k] < 0 || ab_idx[k] >= a->nf) return AMO_ERR_DIM;
+    for (int k = 0; k < nb; ++k) if (bl_idx[k] < 0 || bl_idx[k] >= a->nf) return AMO_ERR_DIM;
+    for (int n = 0; n < n_active; ++n) {
+        if (c_above[n] != -1 && (c_above[n] < 0 || c_above[n] >= a->nc)) return AMO_ERR_DIM;
+        if (c_below[n] != -1 && (c_below[n] < 0 || c_below[n] >= a->nc)) return AMO_ERR_DIM;
+    }
+    a->pNx = Nx; a->pNy = Ny; a->p_dx = pdx; a->n_pnodes = nn;
+    a->p_rh = rh; a->p_D = D; a->p_aM = aM; a->p_bK = bK;
+    free(a->w); free(a->w_prev); free(a->w_new); free(a->ext); free(a->active);
+    free(a->ab_start); free(a->ab_face); free(a->ab_side); free(a->ab_wgt);
+    free(a->bl_start); free(a->bl_face); free(a->bl_side); free(a->bl_wgt);
+    free(a->cell_above); free(a->cell_below);
+    a->w = (float*)calloc((size_t)nn, sizeof(float));
+    a->w_prev = (float*)calloc((size_t)nn, sizeof(float));
+    a->w_new = (float*)calloc((size_t)nn, sizeof(float));
+    a->ext = (float*)calloc((size_t)nn, sizeof(float));
+    a->n_active = n_active;
+    a->active = (int*)malloc(sizeof(int) * ((size_t)n_active + 1));
+    memcpy(a->active, flat_idx, sizeof(int) * (size_t)n_active);
+    a->ab_start = (int*)malloc(sizeof(int) * ((size_t)n_active + 1));
+    a->bl_start = (int*)malloc(sizeof(int) * ((size_t)n_active + 1));
+    for (int n = 0; n <= n_active; ++n) {
+        a->ab_start[n] = ab_start[n] - ab_start[0];
+        a->bl_start[n] = bl_start[n] - bl_start[0];
+    }
+    const int la = ab_start[n_active] - ab_start[0], lb = bl_start[n_active] - bl_start[0];
+    a->ab_face = (int*)malloc(sizeof(int) * ((size_t)la + 1));
+    a->ab_side = (int*)malloc(sizeof(int) * ((size_t)la + 1));
+    a->ab_wgt = (float*)malloc(sizeof(float) * ((size_t)la + 1));
+    a->bl_face = (int*)malloc(sizeof(int) * ((size_t)lb + 1));
+    a->bl_side = (int*)malloc(sizeof(int) * ((size_t)lb + 1));
+    a->bl_wgt = (float*)malloc(sizeof(float) * ((size_t)lb + 1));
+    for (int k = 0; k < la; ++k) {
+        a->ab_face[k] = ab_idx[ab_start[0] + k];
+        a->ab_wgt[k] = ab_w[ab_start[0] + k];
+        a->ab_side[k] = a->pos[a->ab_face[k]];      /* air cell above: pos side */
+    }
+    for (int k = 0; k < lb; ++k) {
+        a->bl_face[k] = bl_idx[bl_start[0] + k];
+        a->bl_wgt[k] = bl_w[bl_start[0] + k];
+        a->bl_side[k] = a->neg[a->bl_face[k]];      /* air cell below: neg side */
+    }
+    a->cell_above = (int*)malloc(sizeof(int) * ((size_t)n_active + 1));
+    a->cell_below = (int*)malloc(sizeof(int) * ((size_t)n_active + 1));
+    memcpy(a->cell_above, c_above, sizeof(int) * (size_t)n_active);
+    memcpy(a->cell_below, c_below, sizeof(int) * (size_t)n_active);
+    for (int n = 0; n < n_active; ++n) {                  /* every node coupled */
+        if (a->ab_start[n + 1] == a->ab_start[n] && a->cell_above[n] < 0) return AMO_ERR_DIM;
+        if (a->bl_start[n + 1] == a->bl_start[n] && a->cell_below[n] < 0) return AMO_ERR_DIM;
+    }
+    a->plate_on = 1;
+    return AMO_OK;
+}
+
+/* amr_set_bridge_plate_sources / amr_set_neck_plate_sources, :1534-1563 */
+SK_EXPORT int amo_set_bridge_sources(Amo* a, int n, const int32_t* idx, const float* wgt)
+{
+    if (!a) return AMO_ERR_NULL;
+    if (!a->plate_on) return AMO_ERR_DIM;
+    if (n <= 0 || !idx || !wgt) return AMO_ERR_DIM;
+    for (int i = 0; i < n; ++i) if (idx[i] < 0 || idx[i] >= a->n_pnodes) return AMO_ERR_DIM;
+    free(a->bridge_idx); free(a->bridge_wgt);
+    a->bridge_idx = (int*)malloc(sizeof(int) * (size_t)n);
+    a->bridge_wgt = (float*)malloc(sizeof(float) * (size_t)n);
+    memcpy(a->bridge_idx, idx, sizeof(int) * (size_t)n);
+    memcpy(a->bridge_wgt, wgt, sizeof(float) * (size_t)n);
+    a->n_bridge = n;
+    return AMO_OK;
+}
+
+SK_EXPORT int amo_set_neck_sources(Amo* a, int n, const int32_t* idx, const float* wgt)
+{
+    if (!a) return AMO_ERR_NULL;
+    if (n == 0) { a->n_neck = 0; return AMO_OK; }
+    if (!a->plate_on) return AMO_ERR_DIM;
+    if (!idx || !wgt) return AMO_ERR_DIM;
+    for (int i = 0; i < n; ++i) if (idx[i] < 0 || idx[i] >= a->n_pnodes) return AMO_ERR_DIM;
+    free(a->neck_idx); free(a->neck_wgt);
+    a->neck_idx = (int*)malloc(sizeof(int) * (size_t)n);
+    a->neck_wgt = (float*)malloc(sizeof(float) * (size_t)n);
+    memcpy(a->neck_idx, idx, sizeof(int) * (size_t)n);
+    memcpy(a->neck_wgt, wgt, sizeof(float) * (size_t)n);
+    a->n_neck = n;
+    return AMO_OK;
+}
+
+/* amr_inject_bridge_drive / amr_inject_plate_force, :1567-1597 */
+SK_EXPORT int amo_inject_bridge_drive(Amo* a, const float* drives, int n, float scale)
+{
+    if (!a || !drives) return AMO_ERR_NULL;
+    if (!a->plate_on) return AMO_ERR_DIM;
+    if (n != a->n_bridge) return AMO_ERR_DIM;
+    for (int i = 0; i < n; ++i)
+        a->ext[a->bridge_idx[i]] += drives[i] * a->bridge_wgt[i] * scale;
+    return AMO_OK;
+}
+
+SK_EXPORT int amo_inject_plate_force(Amo* a, int n, const int32_t* idx, const float* wgt,
+                                     float total)
+{
+    if (!a) return AMO_ERR_NULL;
+    if (!a->plate_on) return AMO_ERR_DIM;
+    if (n <= 0 || !idx || !wgt) return AMO_ERR_DIM;
+    for (int i = 0; i < n; ++i) {
+        if (idx[i] < 0 || idx[i] >= a->n_pnodes) return AMO_ERR_DIM;
+        a->ext[idx[i]] += wgt[i] * total;
+    }
+    return AMO_OK;
+}
+
+/* plate samplers and getter, :1600-1644 */
+SK_EXPORT int amo_sample_plate_batch(const Amo* a, int n, const int* idx4, const float* wgt4,
+                                     float* out)
+{
+    if (!a || !idx4 || !wgt4 || !out) return AMO_ERR_NULL;
+    if (!a->plate_on) return AMO_ERR_DIM;
+    for (int q = 0; q < n; ++q) {
+        float v = 0.0f;
+        for (int k = 0; k < 4; ++k) {
+            const int f = idx4[q * 4 + k];
+            if (f < 0 || f >= a->n_pnodes) return AMO_ERR_DIM;
+            v += wgt4[q * 4 + k] * a->w[f];
+        }
+        out[q] = v;
+    }
+    return AMO_OK;
+}
+
+SK_EXPORT int amo_sample_plate_weighted(const Amo* a, int n, const int* idx, const float* wgt,
+                                        float* out)
+{
+    if (!a || !idx || !wgt || !out) return AMO_ERR_NULL;
+    if (!a->plate_on) return AMO_ERR_DIM;
+    float v = 0.0f;
+    for (int i = 0; i < n; ++i) {
+        if (idx[i] < 0 || idx[i] >= a->n_pnodes) return AMO_ERR_DIM;
+        v += wgt[i] * a->w[idx[i]];
+    }
+    *out = v;
+    return AMO_OK;
+}
+
+SK_EXPORT int amo_get_plate(const Amo* a, float* out, int n)
+{
+    if (!a || !out) return AMO_ERR_NULL;
+    if (!a->plate_on) return AMO_ERR_DIM;
+    if (n != a->n_pnodes) return AMO_ERR_DIM;
+    memcpy(out, a->w, sizeof(float) * (size_t)n);
+    return AMO_OK;
+}
+
+static void amo_push(int** ia, float** fa, int n, int iv, float fv)
+{
+    *ia = (int*)realloc(*ia, sizeof(int) * ((size_t)n + 1));
+    *fa = (float*)realloc(*fa, sizeof(float) * ((size_t)n + 1));
+    (*ia)[n] = iv;
+    (*fa)[n] = fv;
+}
+
+/* amr_setup_mic_samplers, acoustic_amr.cpp:1647-1760 */
+SK_EXPORT int amo_setup_mics(Amo* a, int n_mics, const float* pos, const float* axis,
+                             int need_v)
+{
+    if (!a || !pos || !axis) return AMO_ERR_NULL;
+    if (n_mics <= 0) return AMO_ERR_DIM;
+    if (n_mics > a->n_mics) {                              /* resize: keep, add empty */
+        a->mp_n = (int*)realloc(a->mp_n, sizeof(int) * (size_t)n_mics);
+        a->mp_cell = (int**)realloc(a->mp_cell, sizeof(int*) * (size_t)n_mics);
+        a->mp_w = (float**)realloc(a->mp_w, sizeof(float*) * (size_t)n_mics);
+        a->mv_n = (int*)realloc(a->mv_n, sizeof(int) * (size_t)n_mics);
+        a->mv_face = (int**)realloc(a->mv_face, sizeof(int*) * (size_t)n_mics);
+        a->mv_wx = (float**)realloc(a->mv_wx, sizeof(float*) * (size_t)n_mics);
+        a->mv_wy = (float**)realloc(a->mv_wy, sizeof(float*) * (size_t)n_mics);
+        a->mv_wz = (float**)realloc(a->mv_wz, sizeof(float*) * (size_t)n_mics);
+        for (int m = a->n_mics; m < n_mics; ++m) {
+            a->mp_n[m] = 0; a->mp_cell[m] = NULL; a->mp_w[m] = NULL;
+            a->mv_n[m] = 0; a->mv_face[m] = NULL;
+            a->mv_wx[m] = NULL; a->mv_wy[m] = NULL; a->mv_wz[m] = NULL;
+        }
+    } else {
+        for (int m = n_mics; m < a->n_mics; ++m) {        /* resize: drop the tail */
+            free(a->mp_cell[m]); free(a->mp_w[m]);
+            free(a->mv_face[m]); free(a->mv_wx[m]); free(a->mv_wy[m]); free(a->mv_wz[m]);
+        }
+    }
+    a->n_mics = n_mics;
+    double mean_vol = 0.0;
+    float lo[3] = {3.402823466e38f, 3.402823466e38f, 3.402823466e38f};
+    float hi[3] = {-3.402823466e38f, -3.402823466e38f, -3.402823466e38f};
+    for (int t = 0; t < a->n_acoustic; ++t) {
+        const int idx = a->acoustic[t];
+        mean_vol += a->volumes[idx];
+        for (int d = 0; d < 3; ++d) {
+            const float c = (float)a->centers[3 * idx + d];
+            if (c < lo[d]) lo[d] = c;
+            if (c > hi[d]) hi[d] = c;
+        }
+    }
+    if (a->n_acoustic > 0) mean_vol /= (double)a->n_acoustic;
+    const float R = 2.0f * (float)cbrt(mean_vol);
+    for (int m = 0; m < n_mics; ++m) {
+        float q[3];
+        for (int d = 0; d < 3; ++d) {
+            const float x = pos[3 * m + d];
+            q[d] = fmaxf(lo[d], fminf(hi[d], x));
+        }
+        float wsum = 0.0f;
+        for (int t = 0; t < a->n_acoustic; ++t) {
+            const int idx = a->acoustic[t];
+            const float dx = q[0] - (float)a->centers[3 * idx + 0];
+            const float dy = q[1] - (float)a->centers[3 * idx + 1];
+            const float dz = q[2] - (float)a->centers[3 * idx + 2];
+            const float r = sqrtf(dx * dx + dy * dy + dz * dz);
+            if (r < R) {
+                const float w = (r < 1e-9f) ? 1e9f : 1.0f / r;
+                amo_push(&a->mp_cell[m], &a->mp_w[m], a->mp_n[m]++, idx, w);
+                wsum += w;
+            }
+        }
+        if (a->mp_n[m] == 0) return AMO_ERR_DIM;
+        const float inv = 1.0f / wsum;
+        for (int k = 0; k < a->mp_n[m]; ++k) a->mp_w[m][k] *= inv;
+        if (need_v) {
+            float tot = 0.0f;
+            const int k0 = a->mv_n[m];
+            for (int f = 0; f < a->nf; ++f) {
+                const int p0 = a->neg[f], p1 = a->pos[f];
+                const float fx = 0.5f * (float)(a->centers[3 * p0] + a->centers[3 * p1]);
+                const float fy = 0.5f * (float)(a->centers[3 * p0 + 1] + a->centers[3 * p1 + 1]);
+                const float fz = 0.5f * (float)(a->centers[3 * p0 + 2] + a->centers[3 * p1 + 2]);
+                const float ex = fx - q[0], ey = fy - q[1], ez = fz - q[2];
+                const float r2 = sqrtf(ex * ex + ey * ey + ez * ez);
+                if (r2 >= R || a->face_open[f] < 0.5) continue;
+                float nx = (float)(a->centers[3 * p1] - a->centers[3 * p0]);
+                float ny = (float)(a->centers[3 * p1 + 1] - a->centers[3 * p0 + 1]);
+                float nz = (float)(a->centers[3 * p1 + 2] - a->centers[3 * p0 + 2]);
+                const float nl = sqrtf(nx * nx + ny * ny + nz * nz);
+                if (nl < 1e-12f) continue;
+                nx /= nl; ny /= nl; nz /= nl;
+                const float aw = (float)(a->face_area[f] * a->face_open[f]);
+                const int k = a->mv_n[m]++;
+                a->mv_face[m] = (int*)realloc(a->mv_face[m], sizeof(int) * ((size_t)k + 1));
+                a->mv_wx[m] = (float*)realloc(a->mv_wx[m], sizeof(float) * ((size_t)k + 1));
+                a->mv_wy[m] = (float*)realloc(a->mv_wy[m], sizeof(float) * ((size_t)k + 1));
+                a->mv_wz[m] = (float*)realloc(a->mv_wz[m], sizeof(float) * ((size_t)k + 1));
+                a->mv_face[m][k] = f;
+                a->mv_wx[m][k] = nx * aw;
+                a->mv_wy[m][k] = ny * aw;
+                a->mv_wz[m][k] = nz * aw;
+                tot += aw;
+            }
+            (void)k0;
+            if (tot > 1e-20f) {
+                const float inv_a = 1.0f / tot;
+                for (int k = 0; k < a->mv_n[m]; ++k) {
+                    a->mv_wx[m][k] *= inv_a; a->mv_wy[m][k] *= inv_a; a->mv_wz[m][k] *= inv_a;
+                }
+            }
+        }
+    }
+    return AMO_OK;
+}
+
+/* amr_sample_mics, acoustic_amr.cpp:1762-1792 */
+SK_EXPORT int amo_sample_mics(const Amo* a, int n_mics, float* op, float* ovx, float* ovy,
+                              float* ovz)
+{
+    if (!a || !op) return AMO_ERR_NULL;
+    if (n_mics != a->n_mics) return AMO_ERR_DIM;
+    for (int m = 0; m < n_mics; ++m) {
+        float p = 0.0f;
+        for (int k = 0; k < a->mp_n[m]; ++k) p += a->mp_w[m][k] * a->p[a->mp_cell[m][k]];
+        op[m] = p;
+        if (ovx && ovy && ovz) {
+            float vx = 0.0f, vy = 0.0f, vz = 0.0f;
+            for (int k = 0; k < a->mv_n[m]; ++k) {
+                const float vf = a->v[a->mv_face[m][k]];
+                vx += a->mv_wx[m][k] * vf;
+                vy += a->mv_wy[m][k] * vf;
+                vz += a->mv_wz[m][k] * vf;
+            }
+            ovx[m] = vx; ovy[m] = vy; ovz[m] = vz;
+        }
+    }
+    return AMO_OK;
+}
+
+/* amr_step, order-2 branch, acoustic_amr.cpp:1100-1204. */
 SK_EXPORT int amo_step(Amo* a, int n_steps)
 {
     if (!a) return AMO_ERR_NULL;
@@ -196,6 +611,7 @@ SK_EXPORT int amo_step(Amo* a, int n_steps)
             const float g = (a->p[a->pos[f]] - a->p[a->neg[f]]) * a->inv_d[f];
             a->v[f] = (a->v[f] - dt_over_rho * g) * a->vdamp[f];
         }
+        if (a->plate_on) amo_plate_bc(a, dt);                         /* :1157 */
         #pragma omp parallel for schedule(static)
         for (int c = 0; c < nc; ++c) {                                /* :1163-1172 */
             float d = 0.0f;
@@ -204,6 +620,7 @@ SK_EXPORT int amo_step(Amo* a, int n_steps)
             a->p[c] = a->p[c] * a->pdamp[c] - bulk * d * a->inv_den[c] * a->psrc[c];
         }
         for (int t = 0; t < a->n_wallplate; ++t) a->p[a->wallplate[t]] = 0.0f;  /* :1174-1176 */
+        if (a->plate_on) amo_plate_step(a, dt);                        /* :1179 */
         int bad = 0;                                                  /* :1181-1192 */
         for (int c = 0; c < nc; ++c) if (!isfinite(a->p[c])) { bad = 1; break; }
         if (!bad && (a->step_count & 0xFF) == 0) {

@@ -48,6 +48,16 @@ _SIGS = {
     "amr_get_csr_entries": (_i, [_vp]),
     "amr_scatter_pressure_uniform": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp]),
     "amr_set_border_condition": (_i, [_vp, _i, _f, _f, _f, _i, _vp, _vp]),
+    "amr_setup_plate": (_i, [_vp, _i, _i, _f, _vp, _vp, _f, _f, _f, _f, _i] + [_vp] * 9),
+    "amr_set_bridge_plate_sources": (_i, [_vp, _i, _vp, _vp]),
+    "amr_set_neck_plate_sources": (_i, [_vp, _i, _vp, _vp]),
+    "amr_inject_bridge_drive": (_i, [_vp, _vp, _i, _f]),
+    "amr_inject_plate_force": (_i, [_vp, _i, _vp, _vp, _f]),
+    "amr_sample_plate_displacement_batch": (_i, [_vp, _i, _vp, _vp, _vp]),
+    "amr_sample_plate_weighted": (_i, [_vp, _i, _vp, _vp, _vp]),
+    "amr_get_plate_displacement": (_i, [_vp, _vp, _i]),
+    "amr_setup_mic_samplers": (_i, [_vp, _i, _vp, _vp, _i]),
+    "amr_sample_mics": (_i, [_vp, _i, _vp, _vp, _vp, _vp]),
 }
 _bound = None
 
@@ -175,6 +185,94 @@ class AcousticAMR:
         if rc != 0:
             raise RuntimeError(f"amr_scatter_pressure_uniform failed: rc={rc}")
         return out.reshape(int(Nx), int(Ny), int(Nz))
+
+    # -- plate and mic samplers (acoustic_amr.h:165-325) -----------------
+    def _rc(self, name, rc):
+        if rc != 0:
+            raise RuntimeError(f"{name} failed: rc={rc}")
+        return rc
+
+    def setup_plate(self, plate_Nx, plate_Ny, plate_dx, plate_origin, plate_active, plate_rho_h,
+                    plate_D, plate_alpha_M, plate_beta_K, plate_active_flat_idx,
+                    face_above_starts, face_above_idx, face_above_wgt, face_below_starts,
+                    face_below_idx, face_below_wgt, cell_above, cell_below) -> int:
+        a = [np.ascontiguousarray(plate_origin, np.float32),
+             np.ascontiguousarray(plate_active, np.uint8),
+             np.ascontiguousarray(plate_active_flat_idx, np.int32),
+             np.ascontiguousarray(face_above_starts, np.int32),
+             np.ascontiguousarray(face_above_idx, np.int32),
+             np.ascontiguousarray(face_above_wgt, np.float32),
+             np.ascontiguousarray(face_below_starts, np.int32),
+             np.ascontiguousarray(face_below_idx, np.int32),
+             np.ascontiguousarray(face_below_wgt, np.float32),
+             np.ascontiguousarray(cell_above, np.int32),
+             np.ascontiguousarray(cell_below, np.int32)]
+        rc = lib().amr_setup_plate(self._h, int(plate_Nx), int(plate_Ny), float(plate_dx),
+                                   _p(a[0]), _p(a[1]), float(plate_rho_h), float(plate_D),
+                                   float(plate_alpha_M), float(plate_beta_K), int(a[2].size),
+                                   *[_p(x) for x in a[2:]])
+        return self._rc("amr_setup_plate", rc)
+
+    def set_bridge_plate_sources(self, idx, wgt) -> int:
+        i = np.ascontiguousarray(idx, np.int32)
+        w = np.ascontiguousarray(wgt, np.float32)
+        return self._rc("amr_set_bridge_plate_sources",
+                        lib().amr_set_bridge_plate_sources(self._h, i.size, _p(i), _p(w)))
+
+    def set_neck_plate_sources(self, idx, wgt) -> int:
+        i = np.ascontiguousarray(idx, np.int32)
+        w = np.ascontiguousarray(wgt, np.float32)
+        return self._rc("amr_set_neck_plate_sources",
+                        lib().amr_set_neck_plate_sources(self._h, i.size, _p(i), _p(w)))
+
+    def inject_bridge_drive(self, cell_drives, force_scale: float = 1.0) -> int:
+        d = np.ascontiguousarray(cell_drives, np.float32)
+        return self._rc("amr_inject_bridge_drive",
+                        lib().amr_inject_bridge_drive(self._h, _p(d), d.size, float(force_scale)))
+
+    def inject_plate_force(self, plate_idx, wgt, total_force_N: float) -> int:
+        i = np.ascontiguousarray(plate_idx, np.int32)
+        w = np.ascontiguousarray(wgt, np.float32)
+        return self._rc("amr_inject_plate_force",
+                        lib().amr_inject_plate_force(self._h, i.size, _p(i), _p(w),
+                                                     float(total_force_N)))
+
+    def sample_plate_displacement_batch(self, idx4, wgt4) -> np.ndarray:
+        i = np.ascontiguousarray(idx4, np.int32).reshape(-1, 4)
+        w = np.ascontiguousarray(wgt4, np.float32).reshape(-1, 4)
+        out = np.zeros(len(i), np.float32)
+        self._rc("amr_sample_plate_displacement_batch",
+                 lib().amr_sample_plate_displacement_batch(self._h, len(i), _p(i), _p(w),
+                                                           _p(out)))
+        return out
+
+    def sample_plate_weighted(self, plate_idx, wgt) -> float:
+        i = np.ascontiguousarray(plate_idx, np.int32)
+        w = np.ascontiguousarray(wgt, np.float32)
+        out = np.zeros(1, np.float32)
+        self._rc("amr_sample_plate_weighted",
+                 lib().amr_sample_plate_weighted(self._h, i.size, _p(i), _p(w), _p(out)))
+        return float(out[0])
+
+    def get_plate_displacement(self, n_nodes: int) -> np.ndarray:
+        out = np.empty(int(n_nodes), np.float32)
+        self._rc("amr_get_plate_displacement",
+                 lib().amr_get_plate_displacement(self._h, _p(out), out.size))
+        return out
+
+    def setup_mic_samplers(self, pos_xyz, axis_xyz, need_velocity: bool = True) -> int:
+        pp = np.ascontiguousarray(pos_xyz, np.float32).reshape(-1, 3)
+        ax = np.ascontiguousarray(axis_xyz, np.float32).reshape(-1, 3)
+        self._n_mics = len(pp)
+        return self._rc("amr_setup_mic_samplers",
+                        lib().amr_setup_mic_samplers(self._h, len(pp), _p(pp), _p(ax),
+                                                     int(bool(need_velocity))))
+
+    def sample_mics(self):
+        n = getattr(self, "_n_mics", 0)
+        o = [np.zeros(n, np.float32) for _ in range(4)]
+        self._rc("amr_sample_mics", lib().amr_sample_mics(self._h, n, *[_p(x) for x in o]))
+        return tuple(o)
 
     # -- extensions ---------------------------------------------------------
     def set_state(self, p=None, v=None) -> None:

@@ -55,6 +55,35 @@ struct AcousticAMRState {
     std::vector<int> neg, pos;
     std::vector<float> inv_d, inv_den, pdamp, psrc, vdamp;
     std::vector<int> kind;
+    std::vector<double> volumes, face_area, face_open;
+    // Kirchhoff plate: three rotating displacement buffers (w_prev, w, w_new
+    // at rotation prot), external force, per-node face lists (CSR over the
+    // active nodes) and the faces the plate drives (one writer per face)
+    bool plate_on = false;
+    int pNx = 0, pNy = 0, n_pnodes = 0, n_active = 0, prot = 0, n_bc = 0;
+    float p_dx = 0, p_rh = 0, p_D = 0, p_aM = 0, p_bK = 0;
+    float* d_pw[3] = {nullptr, nullptr, nullptr};
+    float* d_ext = nullptr;
+    int* d_pint = nullptr;     // active[n_active], ab_start/bl_start[n_active+1],
+                               // ab_side[la], bl_side[lb], cabove/cbelow[n_active],
+                               // bc_face/bc_node/bc_sign[n_bc]
+    float* d_pflt = nullptr;   // ab_wgt[la], bl_wgt[lb]
+    int o_active = 0, o_ab = 0, o_bl = 0, o_abs = 0, o_bls = 0, o_ca = 0, o_cb = 0;
+    int o_bcf = 0, o_bcn = 0, o_bcs = 0, o_abw = 0, o_blw = 0;
+    std::vector<int> bridge_idx, neck_idx;
+    std::vector<float> bridge_wgt, neck_wgt;
+    // mic samplers: growing per-mic lists as the reference keeps them, and
+    // their CSR copy on the device
+    std::vector<std::vector<int>> mp_cell, mv_face;
+    std::vector<std::vector<float>> mp_w, mv_wx, mv_wy, mv_wz;
+    int n_mics = 0;
+    int* d_mint = nullptr;     // mp_start[n+1], mv_start[n+1], cells, faces
+    float* d_mflt = nullptr;   // p weights, then wx, wy, wz
+    float* d_mout = nullptr;   // 4*n_mics outputs
+    int o_mvs = 0, o_mpc = 0, o_mvf = 0, o_mwx = 0, o_mwy = 0, o_mwz = 0;
+    // small upload buffers (drives, forces, sampler stencils)
+    void* d_tmp = nullptr;
+    size_t tmp_bytes = 0;
 };
 
 namespace {
@@ -223,6 +252,132 @@ __global__ __launch_bounds__(256) void k_amr_scatter_val(const int* __restrict__
     out[i] = c >= 0 ? p[c] : 0.0f;
 }
 
+// Plate velocity boundary condition (acoustic_amr.cpp:1050-1072): each
+// driven face takes +-(w - w_prev)/dt of the node that writes it last in the
+// reference's node order (resolved on the host, one writer per face).
+__global__ __launch_bounds__(256) void k_amr_plate_bc(const int* __restrict__ bc_face,
+                                                      const int* __restrict__ bc_node,
+                                                      const int* __restrict__ bc_sign, int n_bc,
+                                                      const float* __restrict__ w,
+                                                      const float* __restrict__ w_prev, float dt,
+                                                      float* __restrict__ v,
+                                                      const int* __restrict__ status, int s) {
+    if (status[0] < s) return;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n_bc) return;
+    const int flat = bc_node[i];
+    const float vp = (w[flat] - w_prev[flat]) / dt;
+    v[bc_face[i]] = bc_sign[i] > 0 ? vp : -vp;
+}
+
+__device__ inline float plate_tap(const float* w, int i, int j, int Nx, int Ny) {
+    return (i < 0 || i >= Nx || j < 0 || j >= Ny) ? 0.0f : w[i * Ny + j];
+}
+
+// 13-point biharmonic with zero outside the plate grid (acoustic_amr.cpp:939-954)
+__device__ inline float plate_l4(const float* w, int i, int j, int Nx, int Ny, float idx4) {
+    const float L = plate_tap(w, i - 2, j, Nx, Ny) + plate_tap(w, i + 2, j, Nx, Ny) +
+                    plate_tap(w, i, j - 2, Nx, Ny) + plate_tap(w, i, j + 2, Nx, Ny) +
+                    2.0f * (plate_tap(w, i - 1, j - 1, Nx, Ny) + plate_tap(w, i - 1, j + 1, Nx, Ny) +
+                            plate_tap(w, i + 1, j - 1, Nx, Ny) + plate_tap(w, i + 1, j + 1, Nx, Ny)) -
+                    8.0f * (plate_tap(w, i - 1, j, Nx, Ny) + plate_tap(w, i + 1, j, Nx, Ny) +
+                            plate_tap(w, i, j - 1, Nx, Ny) + plate_tap(w, i, j + 1, Nx, Ny)) +
+                    20.0f * plate_tap(w, i, j, Nx, Ny);
+    return L * idx4;
+}
+
+struct PlateConsts {
+    int Nx, Ny, n_active;
+    float dx, dx4, dt2, inv_rh, damp_fwd, damp_bwd, cD0, cDp;
+};
+
+// One plate step (acoustic_amr.cpp:956-1047), one thread per active node;
+// w_new is a separate buffer, so the rotation w_prev <- w <- w_new is a
+// pointer swap on the host.  Each node clears its own external force.
+__global__ __launch_bounds__(256) void k_amr_plate_step(
+    PlateConsts k, const int* __restrict__ active, const int* __restrict__ ab_start,
+    const int* __restrict__ ab_side, const float* __restrict__ ab_wgt,
+    const int* __restrict__ bl_start, const int* __restrict__ bl_side,
+    const float* __restrict__ bl_wgt, const int* __restrict__ c_above,
+    const int* __restrict__ c_below, const float* __restrict__ p, const float* __restrict__ w,
+    const float* __restrict__ w_prev, float* __restrict__ w_new, float* __restrict__ ext,
+    const int* __restrict__ status, int s) {
+    if (status[0] < s) return;
+    const int n = blockIdx.x * 256 + threadIdx.x;
+    if (n >= k.n_active) return;
+    const int flat = active[n], i = flat / k.Ny, j = flat % k.Ny;
+    float Pa = 0.0f, Pb = 0.0f;
+    if (ab_start[n + 1] > ab_start[n]) {
+        for (int q = ab_start[n]; q < ab_start[n + 1]; ++q) Pa += ab_wgt[q] * p[ab_side[q]];
+    } else if (c_above[n] >= 0) {
+        Pa = p[c_above[n]];
+    }
+    if (bl_start[n + 1] > bl_start[n]) {
+        for (int q = bl_start[n]; q < bl_start[n + 1]; ++q) Pb += bl_wgt[q] * p[bl_side[q]];
+    } else if (c_below[n] >= 0) {
+        Pb = p[c_below[n]];
+    }
+    const float F_acou = (Pb - Pa) * k.dx * k.dx;
+    const float F_ext = ext[flat];
+    const float L4w = plate_l4(w, i, j, k.Nx, k.Ny, k.dx4);
+    const float L4wp = plate_l4(w_prev, i, j, k.Nx, k.Ny, k.dx4);
+    const float rhs = F_acou + F_ext - k.cD0 * L4w + k.cDp * L4wp;
+    w_new[flat] = (k.damp_bwd * (2.0f * w[flat] - w_prev[flat]) + k.dt2 * k.inv_rh * rhs) /
+                  k.damp_fwd;
+    ext[flat] = 0.0f;
+}
+
+// External force from the bridge drive or a distributed force, accumulated
+// in call order by one thread (entries may repeat a node).
+__global__ void k_amr_plate_force(float* ext, const int* idx, const float* a, const float* b,
+                                  float scale, int n, int bridge) {
+    for (int q = 0; q < n; ++q)
+        ext[idx[q]] += bridge ? a[q] * b[q] * scale : a[q] * scale;
+}
+
+__global__ __launch_bounds__(256) void k_amr_plate_sample4(const float* __restrict__ w,
+                                                           const int* __restrict__ idx4,
+                                                           const float* __restrict__ wgt4, int n,
+                                                           float* __restrict__ out) {
+    const int q = blockIdx.x * 256 + threadIdx.x;
+    if (q >= n) return;
+    float val = 0.0f;
+    for (int k = 0; k < 4; ++k) val += wgt4[4 * q + k] * w[idx4[4 * q + k]];
+    out[q] = val;
+}
+
+__global__ void k_amr_plate_weighted(const float* w, const int* idx, const float* wgt, int n,
+                                     float* out) {
+    float val = 0.0f;
+    for (int q = 0; q < n; ++q) val += wgt[q] * w[idx[q]];
+    out[0] = val;
+}
+
+// Mic samples (acoustic_amr.cpp:1762-1792), one thread per mic, sums in
+// stencil order.
+__global__ void k_amr_mics(const int* __restrict__ mp_start, const int* __restrict__ mp_cell,
+                           const float* __restrict__ mp_w, const int* __restrict__ mv_start,
+                           const int* __restrict__ mv_face, const float* __restrict__ wx,
+                           const float* __restrict__ wy, const float* __restrict__ wz,
+                           const float* __restrict__ p, const float* __restrict__ v, int n_mics,
+                           float* __restrict__ out) {
+    const int m = blockIdx.x * 64 + threadIdx.x;
+    if (m >= n_mics) return;
+    float pp = 0.0f;
+    for (int k = mp_start[m]; k < mp_start[m + 1]; ++k) pp += mp_w[k] * p[mp_cell[k]];
+    float vx = 0.0f, vy = 0.0f, vz = 0.0f;
+    for (int k = mv_start[m]; k < mv_start[m + 1]; ++k) {
+        const float vf = v[mv_face[k]];
+        vx += wx[k] * vf;
+        vy += wy[k] * vf;
+        vz += wz[k] * vf;
+    }
+    out[m] = pp;
+    out[n_mics + m] = vx;
+    out[2 * n_mics + m] = vy;
+    out[3 * n_mics + m] = vz;
+}
+
 inline unsigned blocks(size_t n) { return (unsigned)((n + 255) / 256); }
 
 int validate(int nc, const double* cc, const double* vol, const double* ov, const uint8_t* ct,
@@ -250,6 +405,10 @@ void destroy_state(AcousticAMRState* st) {
     void* bufs[] = {st->d_face, st->d_v, st->d_p, st->d_cell, st->d_start,
                     st->d_csr,  st->d_w,    st->d_cent, st->d_status, st->d_best};
     for (void* b : bufs)
+        if (b) (void)hipFree(b);
+    void* more[] = {st->d_pw[0], st->d_pw[1], st->d_pw[2], st->d_ext, st->d_pint,
+                    st->d_pflt, st->d_mint, st->d_mflt, st->d_mout, st->d_tmp};
+    for (void* b : more)
         if (b) (void)hipFree(b);
     if (st->stream) (void)hipStreamDestroy(st->stream);
     delete st;
@@ -321,11 +480,39 @@ int step_impl(AcousticAMRState* st, int n, float* ms_total, float* ms_faces, flo
         if (ms_total) (void)hipEventRecord(ev[2 * (size_t)s], st->stream);
         k_amr_faces<<<(st->nf + kFaceTile - 1) / kFaceTile, 256, 0, st->stream>>>(
             st->d_face, st->d_v, st->d_p, st->nf, dt_over_rho, st->d_status, s);
+        if (st->plate_on && st->n_bc > 0) {                      // :1157
+            const float* w = st->d_pw[(st->prot + s + 1) % 3];
+            const float* wp = st->d_pw[(st->prot + s) % 3];
+            k_amr_plate_bc<<<blocks(st->n_bc), 256, 0, st->stream>>>(
+                st->d_pint + st->o_bcf, st->d_pint + st->o_bcn, st->d_pint + st->o_bcs,
+                st->n_bc, w, wp, dt, st->d_v, st->d_status, s);
+        }
         if (ms_total) (void)hipEventRecord(ev[2 * (size_t)s + 1], st->stream);
         const int check_max = ((st->step_count + s) & 0xFF) == 0;
         k_amr_cells<<<blocks(st->nc), 256, 0, st->stream>>>(st->d_cell, st->d_start, st->d_csr,
                                                           st->d_w, st->d_v, st->d_p, st->nc, bulk,
                                                           check_max, st->d_status, s);
+        if (st->plate_on && st->n_active > 0) {                  // :1179
+            PlateConsts k;
+            k.Nx = st->pNx;
+            k.Ny = st->pNy;
+            k.n_active = st->n_active;
+            k.dx = st->p_dx;
+            k.dx4 = 1.0f / (st->p_dx * st->p_dx * st->p_dx * st->p_dx);
+            k.dt2 = dt * dt;
+            k.inv_rh = 1.0f / st->p_rh;
+            k.damp_fwd = 1.0f + 0.5f * st->p_aM * dt;
+            k.damp_bwd = 1.0f - 0.5f * st->p_aM * dt;
+            k.cD0 = st->p_D * (1.0f + st->p_bK / dt);
+            k.cDp = st->p_D * (st->p_bK / dt);
+            const int* I = st->d_pint;
+            const float* F = st->d_pflt;
+            k_amr_plate_step<<<blocks(st->n_active), 256, 0, st->stream>>>(
+                k, I + st->o_active, I + st->o_ab, I + st->o_abs, F + st->o_abw, I + st->o_bl,
+                I + st->o_bls, F + st->o_blw, I + st->o_ca, I + st->o_cb, st->d_p,
+                st->d_pw[(st->prot + s + 1) % 3], st->d_pw[(st->prot + s) % 3],
+                st->d_pw[(st->prot + s + 2) % 3], st->d_ext, st->d_status, s);
+        }
     }
     if (ms_total) (void)hipEventRecord(ev[2 * (size_t)n], st->stream);
     int fail = INT_MAX;
@@ -348,6 +535,9 @@ int step_impl(AcousticAMRState* st, int n, float* ms_total, float* ms_faces, flo
     }
     for (auto& v : ev) (void)hipEventDestroy(v);
     if (e != hipSuccess) return AMR_ERR_DEVICE;
+    // plate buffers rotate once per step whose plate update ran (the failing
+    // step's included, as the reference advances the plate before its test)
+    if (st->plate_on && st->n_active > 0) st->prot = (st->prot + (fail < n ? fail + 1 : n)) % 3;
     if (fail < n) {                                  // steps before it completed
         st->step_count += fail;
         report_diverged(st);
@@ -392,6 +582,9 @@ SK_API AcousticAMRState* amr_create(int nc, const double* cc, const double* vol,
     st->type.assign(ct, ct + nc);
     st->neg.assign(fn, fn + nf);
     st->pos.assign(fp, fp + nf);
+    st->volumes.assign(vol, vol + nc);
+    st->face_area.assign(fa, fa + nf);
+    st->face_open.assign(fo, fo + nf);
     st->kind.assign(nc, 0);
     int n_acoustic = 0;
     for (int i = 0; i < nc; ++i) {
@@ -487,6 +680,12 @@ SK_API int amr_reset(AcousticAMRState* st) {
     if (!st) return SK_ERR_NULL_STATE;                               // :924-936
     hipError_t e = hipMemsetAsync(st->d_p, 0, sizeof(float) * (size_t)st->nc, st->stream);
     if (e == hipSuccess) e = hipMemsetAsync(st->d_v, 0, sizeof(float) * (size_t)st->nf, st->stream);
+    if (st->plate_on) {
+        for (int b = 0; b < 3 && e == hipSuccess; ++b)
+            e = hipMemsetAsync(st->d_pw[b], 0, sizeof(float) * (size_t)st->n_pnodes, st->stream);
+        if (e == hipSuccess)
+            e = hipMemsetAsync(st->d_ext, 0, sizeof(float) * (size_t)st->n_pnodes, st->stream);
+    }
     if (e == hipSuccess) e = hipStreamSynchronize(st->stream);
     st->step_count = 0;
     return e == hipSuccess ? SK_OK : AMR_ERR_DEVICE;
@@ -616,6 +815,444 @@ SK_API int amr_set_border_condition(AcousticAMRState* st, int mode, float sigma_
         st->vdamp[f] = expf(-sf * dt);
     }
     return upload_coeffs(st) == hipSuccess ? SK_OK : AMR_ERR_DEVICE;
+}
+
+// ── Kirchhoff plate (acoustic_amr.cpp:1412-1644) ────────────────────────
+
+static hipError_t ensure_tmp(AcousticAMRState* st, size_t bytes) {
+    if (bytes <= st->tmp_bytes) return hipSuccess;
+    if (st->d_tmp) (void)hipFree(st->d_tmp);
+    st->d_tmp = nullptr;
+    st->tmp_bytes = 0;
+    hipError_t e = hipMalloc(&st->d_tmp, bytes);
+    if (e == hipSuccess) st->tmp_bytes = bytes;
+    return e;
+}
+
+SK_API int amr_setup_plate(AcousticAMRState* st, int Nx, int Ny, float pdx, const float origin[3],
+                           const uint8_t* act, float rh, float D, float aM, float bK,
+                           int n_active, const int32_t* flat_idx, const int32_t* ab_start,
+                           const int32_t* ab_idx, const float* ab_w, const int32_t* bl_start,
+                           const int32_t* bl_idx, const float* bl_w, const int32_t* c_above,
+                           const int32_t* c_below) {
+    (void)origin;
+    if (!st) return SK_ERR_NULL_STATE;
+    if (!act || !flat_idx || !ab_start || !bl_start || !c_above || !c_below)
+        return SK_ERR_NULL_STATE;
+    if (Nx <= 0 || Ny <= 0 || pdx <= 0.0f) return SK_ERR_DIM_MISMATCH;
+    if (rh <= 0.0f || D <= 0.0f) return SK_ERR_DIM_MISMATCH;
+    if (n_active < 0) return SK_ERR_DIM_MISMATCH;
+    const int nn = Nx * Ny;
+    int cnt = 0;
+    for (int k = 0; k < nn; ++k) cnt += act[k] ? 1 : 0;
+    if (cnt != n_active) return SK_ERR_DIM_MISMATCH;
+    for (int k = 0; k < ab_start[n_active]; ++k)
+        if (ab_idx[k] < 0 || ab_idx[k] >= st->nf) return SK_ERR_DIM_MISMATCH;
+    for (int k = 0; k < bl_start[n_active]; ++k)
+        if (bl_idx[k] < 0 || bl_idx[k] >= st->nf) return SK_ERR_DIM_MISMATCH;
+    for (int n = 0; n < n_active; ++n) {
+        if (c_above[n] != -1 && (c_above[n] < 0 || c_above[n] >= st->nc)) return SK_ERR_DIM_MISMATCH;
+        if (c_below[n] != -1 && (c_below[n] < 0 || c_below[n] >= st->nc)) return SK_ERR_DIM_MISMATCH;
+        // not checked by the reference; needed to keep device accesses in bounds
+        if (flat_idx[n] < 0 || flat_idx[n] >= nn) return SK_ERR_DIM_MISMATCH;
+    }
+    // node maps, face lists rebased to start at 0, air cell of each face
+    const int a0 = ab_start[0], b0 = bl_start[0];
+    const int la = ab_start[n_active] - a0, lb = bl_start[n_active] - b0;
+    std::vector<int> I;
+    auto put = [&](int& off, const int* src, int n) {
+        off = (int)I.size();
+        I.insert(I.end(), src, src + n);
+    };
+    std::vector<int> abs_(n_active + 1), bls_(n_active + 1), aside(la), bside(lb);
+    for (int n = 0; n <= n_active; ++n) { abs_[n] = ab_start[n] - a0; bls_[n] = bl_start[n] - b0; }
+    for (int k = 0; k < la; ++k) aside[k] = st->pos[ab_idx[a0 + k]];
+    for (int k = 0; k < lb; ++k) bside[k] = st->neg[bl_idx[b0 + k]];
+    // driven faces: the last writer in node order (above faces, then below) wins
+    std::vector<int> owner_node(st->nf, -1), owner_sign(st->nf, 0), order;
+    for (int n = 0; n < n_active; ++n) {
+        for (int k = abs_[n]; k < abs_[n + 1]; ++k) {
+            const int f = ab_idx[a0 + k];
+            if (owner_node[f] < 0) order.push_back(f);
+            owner_node[f] = flat_idx[n];
+            owner_sign[f] = 1;
+        }
+        for (int k = bls_[n]; k < bls_[n + 1]; ++k) {
+            const int f = bl_idx[b0 + k];
+            if (owner_node[f] < 0) order.push_back(f);
+            owner_node[f] = flat_idx[n];
+            owner_sign[f] = -1;
+        }
+    }
+    std::vector<int> bcn(order.size()), bcs(order.size());
+    for (size_t q = 0; q < order.size(); ++q) {
+        bcn[q] = owner_node[order[q]];
+        bcs[q] = owner_sign[order[q]];
+    }
+    int o_active, o_ab, o_bl, o_abs, o_bls, o_ca, o_cb, o_bcf, o_bcn, o_bcs;
+    put(o_active, flat_idx, n_active);
+    put(o_ab, abs_.data(), n_active + 1);
+    put(o_bl, bls_.data(), n_active + 1);
+    put(o_abs, aside.data(), la);
+    put(o_bls, bside.data(), lb);
+    put(o_ca, c_above, n_active);
+    put(o_cb, c_below, n_active);
+    put(o_bcf, order.data(), (int)order.size());
+    put(o_bcn, bcn.data(), (int)bcn.size());
+    put(o_bcs, bcs.data(), (int)bcs.size());
+    std::vector<float> F(ab_w + a0, ab_w + a0 + la);
+    F.insert(F.end(), bl_w + b0, bl_w + b0 + lb);
+
+    // replace any previous plate
+    for (int b = 0; b < 3; ++b) {
+        if (st->d_pw[b]) (void)hipFree(st->d_pw[b]);
+        st->d_pw[b] = nullptr;
+    }
+    if (st->d_ext) (void)hipFree(st->d_ext);
+    if (st->d_pint) (void)hipFree(st->d_pint);
+    if (st->d_pflt) (void)hipFree(st->d_pflt);
+    st->d_ext = nullptr;
+    st->d_pint = nullptr;
+    st->d_pflt = nullptr;
+    const bool was_on = st->plate_on;
+    st->plate_on = false;
+    hipError_t e = hipSuccess;
+    for (int b = 0; b < 3 && e == hipSuccess; ++b) e = hipMalloc((void**)&st->d_pw[b], sizeof(float) * nn);
+    if (e == hipSuccess) e = hipMalloc((void**)&st->d_ext, sizeof(float) * nn);
+    if (e == hipSuccess) e = hipMalloc((void**)&st->d_pint, sizeof(int) * (I.size() + 1));
+    if (e == hipSuccess) e = hipMalloc((void**)&st->d_pflt, sizeof(float) * (F.size() + 1));
+    for (int b = 0; b < 3 && e == hipSuccess; ++b)
+        e = hipMemsetAsync(st->d_pw[b], 0, sizeof(float) * nn, st->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(st->d_ext, 0, sizeof(float) * nn, st->stream);
+    if (e == hipSuccess && !I.empty())
+        e = hipMemcpyAsync(st->d_pint, I.data(), sizeof(int) * I.size(), hipMemcpyHostToDevice,
+                           st->stream);
+    if (e == hipSuccess && !F.empty())
+        e = hipMemcpyAsync(st->d_pflt, F.data(), sizeof(float) * F.size(), hipMemcpyHostToDevice,
+                           st->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(st->stream);
+    if (e != hipSuccess) return AMR_ERR_DEVICE;
+    st->pNx = Nx;
+    st->pNy = Ny;
+    st->n_pnodes = nn;
+    st->n_active = n_active;
+    st->p_dx = pdx;
+    st->p_rh = rh;
+    st->p_D = D;
+    st->p_aM = aM;
+    st->p_bK = bK;
+    st->prot = 0;
+    st->n_bc = (int)order.size();
+    st->o_active = o_active; st->o_ab = o_ab; st->o_bl = o_bl; st->o_abs = o_abs;
+    st->o_bls = o_bls; st->o_ca = o_ca; st->o_cb = o_cb; st->o_bcf = o_bcf;
+    st->o_bcn = o_bcn; st->o_bcs = o_bcs;
+    st->o_abw = 0;
+    st->o_blw = la;
+    for (int n = 0; n < n_active; ++n)               // every node coupled on both sides
+        if ((abs_[n + 1] == abs_[n] && c_above[n] < 0) || (bls_[n + 1] == bls_[n] && c_below[n] < 0)) {
+            st->plate_on = was_on;                   // the reference leaves its flag as it was
+            return SK_ERR_DIM_MISMATCH;
+        }
+    st->plate_on = true;
+    return SK_OK;
+}
+
+SK_API int amr_set_bridge_plate_sources(AcousticAMRState* st, int n, const int32_t* idx,
+                                        const float* wgt) {
+    if (!st) return SK_ERR_NULL_STATE;                               // :1534-1546
+    if (!st->plate_on) return SK_ERR_DIM_MISMATCH;
+    if (n <= 0 || !idx || !wgt) return SK_ERR_DIM_MISMATCH;
+    for (int i = 0; i < n; ++i)
+        if (idx[i] < 0 || idx[i] >= st->n_pnodes) return SK_ERR_DIM_MISMATCH;
+    st->bridge_idx.assign(idx, idx + n);
+    st->bridge_wgt.assign(wgt, wgt + n);
+    return SK_OK;
+}
+
+SK_API int amr_set_neck_plate_sources(AcousticAMRState* st, int n, const int32_t* idx,
+                                      const float* wgt) {
+    if (!st) return SK_ERR_NULL_STATE;                               // :1548-1563
+    if (n == 0) {
+        st->neck_idx.clear();
+        st->neck_wgt.clear();
+        return SK_OK;
+    }
+    if (!st->plate_on) return SK_ERR_DIM_MISMATCH;
+    if (!idx || !wgt) return SK_ERR_DIM_MISMATCH;
+    for (int i = 0; i < n; ++i)
+        if (idx[i] < 0 || idx[i] >= st->n_pnodes) return SK_ERR_DIM_MISMATCH;
+    st->neck_idx.assign(idx, idx + n);
+    st->neck_wgt.assign(wgt, wgt + n);
+    return SK_OK;
+}
+
+// Upload n (index, a, b) triples and accumulate them into the external force.
+static int plate_force(AcousticAMRState* st, const int* idx, const float* a, const float* b,
+                       float scale, int n, int bridge) {
+    if (n <= 0) return SK_OK;
+    const size_t bytes = (size_t)n * (sizeof(int) + 2 * sizeof(float));
+    hipError_t e = ensure_tmp(st, bytes);
+    int* di = (int*)st->d_tmp;
+    float* da = (float*)(di + n);
+    float* db = da + n;
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(di, idx, sizeof(int) * n, hipMemcpyHostToDevice, st->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(da, a, sizeof(float) * n, hipMemcpyHostToDevice, st->stream);
+    if (e == hipSuccess && b)
+        e = hipMemcpyAsync(db, b, sizeof(float) * n, hipMemcpyHostToDevice, st->stream);
+    if (e == hipSuccess) {
+        k_amr_plate_force<<<1, 1, 0, st->stream>>>(st->d_ext, di, da, db, scale, n, bridge);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(st->stream);      // caller may reuse buffers
+    return e == hipSuccess ? SK_OK : AMR_ERR_DEVICE;
+}
+
+SK_API int amr_inject_bridge_drive(AcousticAMRState* st, const float* drives, int n, float scale) {
+    if (!st || !drives) return SK_ERR_NULL_STATE;                    // :1567-1579
+    if (!st->plate_on) return SK_ERR_DIM_MISMATCH;
+    if (n != (int)st->bridge_idx.size()) return SK_ERR_DIM_MISMATCH;
+    return plate_force(st, st->bridge_idx.data(), drives, st->bridge_wgt.data(), scale, n, 1);
+}
+
+SK_API int amr_inject_plate_force(AcousticAMRState* st, int n, const int32_t* idx,
+                                  const float* wgt, float total) {
+    if (!st) return SK_ERR_NULL_STATE;                               // :1581-1597
+    if (!st->plate_on) return SK_ERR_DIM_MISMATCH;
+    if (n <= 0 || !idx || !wgt) return SK_ERR_DIM_MISMATCH;
+    int ok = 0;                                      // entries before the first bad node
+    while (ok < n && idx[ok] >= 0 && idx[ok] < st->n_pnodes) ++ok;
+    const int rc = plate_force(st, idx, wgt, nullptr, total, ok, 0);
+    if (rc != SK_OK) return rc;
+    return ok == n ? SK_OK : SK_ERR_DIM_MISMATCH;
+}
+
+static const float* plate_w(const AcousticAMRState* st) { return st->d_pw[(st->prot + 1) % 3]; }
+
+SK_API int amr_sample_plate_displacement_batch(const AcousticAMRState* cst, int n,
+                                               const int* idx4, const float* wgt4, float* out) {
+    if (!cst || !idx4 || !wgt4 || !out) return SK_ERR_NULL_STATE;    // :1600-1616
+    if (!cst->plate_on) return SK_ERR_DIM_MISMATCH;
+    AcousticAMRState* st = const_cast<AcousticAMRState*>(cst);
+    int ok = 0;                                      // points before the first bad tap
+    while (ok < n) {
+        bool good = true;
+        for (int k = 0; k < 4; ++k)
+            good = good && idx4[4 * ok + k] >= 0 && idx4[4 * ok + k] < st->n_pnodes;
+        if (!good) break;
+        ++ok;
+    }
+    if (ok > 0) {
+        hipError_t e = ensure_tmp(st, (size_t)ok * 9 * sizeof(float));
+        int* di = (int*)st->d_tmp;
+        float* dw = (float*)(di + 4 * ok);
+        float* dout = dw + 4 * ok;
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(di, idx4, sizeof(int) * 4 * ok, hipMemcpyHostToDevice, st->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(dw, wgt4, sizeof(float) * 4 * ok, hipMemcpyHostToDevice, st->stream);
+        if (e == hipSuccess) {
+            k_amr_plate_sample4<<<blocks(ok), 256, 0, st->stream>>>(plate_w(st), di, dw, ok, dout);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(out, dout, sizeof(float) * ok, hipMemcpyDeviceToHost, st->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(st->stream);
+        if (e != hipSuccess) return AMR_ERR_DEVICE;
+    }
+    return ok == n || n <= 0 ? SK_OK : SK_ERR_DIM_MISMATCH;
+}
+
+SK_API int amr_sample_plate_weighted(const AcousticAMRState* cst, int n, const int* idx,
+                                     const float* wgt, float* out) {
+    if (!cst || !idx || !wgt || !out) return SK_ERR_NULL_STATE;      // :1618-1632
+    if (!cst->plate_on) return SK_ERR_DIM_MISMATCH;
+    AcousticAMRState* st = const_cast<AcousticAMRState*>(cst);
+    for (int i = 0; i < n; ++i)
+        if (idx[i] < 0 || idx[i] >= st->n_pnodes) return SK_ERR_DIM_MISMATCH;
+    const int m = n > 0 ? n : 0;
+    hipError_t e = ensure_tmp(st, (size_t)(2 * m + 1) * sizeof(float));
+    int* di = (int*)st->d_tmp;
+    float* dw = (float*)(di + m);
+    float* dout = dw + m;
+    if (e == hipSuccess && m)
+        e = hipMemcpyAsync(di, idx, sizeof(int) * m, hipMemcpyHostToDevice, st->stream);
+    if (e == hipSuccess && m)
+        e = hipMemcpyAsync(dw, wgt, sizeof(float) * m, hipMemcpyHostToDevice, st->stream);
+    if (e == hipSuccess) {
+        k_amr_plate_weighted<<<1, 1, 0, st->stream>>>(plate_w(st), di, dw, m, dout);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(out, dout, sizeof(float), hipMemcpyDeviceToHost, st->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(st->stream);
+    return e == hipSuccess ? SK_OK : AMR_ERR_DEVICE;
+}
+
+SK_API int amr_get_plate_displacement(const AcousticAMRState* cst, float* out, int n) {
+    if (!cst || !out) return SK_ERR_NULL_STATE;                      // :1634-1641
+    if (!cst->plate_on) return SK_ERR_DIM_MISMATCH;
+    if (n != cst->n_pnodes) return SK_ERR_DIM_MISMATCH;
+    AcousticAMRState* st = const_cast<AcousticAMRState*>(cst);
+    hipError_t e = hipMemcpyAsync(out, plate_w(st), sizeof(float) * n, hipMemcpyDeviceToHost,
+                                  st->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(st->stream);
+    return e == hipSuccess ? SK_OK : AMR_ERR_DEVICE;
+}
+
+// ── Mic samplers (acoustic_amr.cpp:1647-1792) ───────────────────────────
+
+static hipError_t upload_mics(AcousticAMRState* st) {
+    const int M = st->n_mics;
+    std::vector<int> I(2 * (M + 1), 0);
+    for (int m = 0; m < M; ++m) {
+        I[m + 1] = I[m] + (int)st->mp_cell[m].size();
+        I[M + 1 + m + 1] = I[M + 1 + m] + (int)st->mv_face[m].size();
+    }
+    const int np = I[M], nv = I[2 * M + 1];
+    std::vector<float> F;
+    for (int m = 0; m < M; ++m) {
+        I.insert(I.end(), st->mp_cell[m].begin(), st->mp_cell[m].end());
+        F.insert(F.end(), st->mp_w[m].begin(), st->mp_w[m].end());
+    }
+    for (int m = 0; m < M; ++m) I.insert(I.end(), st->mv_face[m].begin(), st->mv_face[m].end());
+    for (auto* lst : {&st->mv_wx, &st->mv_wy, &st->mv_wz})
+        for (int m = 0; m < M; ++m) F.insert(F.end(), (*lst)[m].begin(), (*lst)[m].end());
+    st->o_mvs = M + 1;
+    st->o_mpc = 2 * (M + 1);
+    st->o_mvf = st->o_mpc + np;
+    st->o_mwx = np;
+    st->o_mwy = np + nv;
+    st->o_mwz = np + 2 * nv;
+    if (st->d_mint) (void)hipFree(st->d_mint);
+    if (st->d_mflt) (void)hipFree(st->d_mflt);
+    if (st->d_mout) (void)hipFree(st->d_mout);
+    st->d_mint = nullptr;
+    st->d_mflt = nullptr;
+    st->d_mout = nullptr;
+    hipError_t e = hipMalloc((void**)&st->d_mint, sizeof(int) * I.size());
+    if (e == hipSuccess) e = hipMalloc((void**)&st->d_mflt, sizeof(float) * (F.size() + 1));
+    if (e == hipSuccess) e = hipMalloc((void**)&st->d_mout, sizeof(float) * 4 * M);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(st->d_mint, I.data(), sizeof(int) * I.size(), hipMemcpyHostToDevice,
+                           st->stream);
+    if (e == hipSuccess && !F.empty())
+        e = hipMemcpyAsync(st->d_mflt, F.data(), sizeof(float) * F.size(), hipMemcpyHostToDevice,
+                           st->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(st->stream);
+    return e;
+}
+
+SK_API int amr_setup_mic_samplers(AcousticAMRState* st, int n_mics, const float* pos,
+                                  const float* axis, int need_v) {
+    if (!st || !pos || !axis) return SK_ERR_NULL_STATE;
+    if (n_mics <= 0) return SK_ERR_DIM_MISMATCH;
+    st->n_mics = n_mics;
+    for (auto* l : {&st->mp_cell, &st->mv_face}) l->resize(n_mics);
+    for (auto* l : {&st->mp_w, &st->mv_wx, &st->mv_wy, &st->mv_wz}) l->resize(n_mics);
+    double mean_vol = 0.0;
+    int n_ac = 0;
+    float lo[3] = {3.402823466e38f, 3.402823466e38f, 3.402823466e38f};
+    float hi[3] = {-3.402823466e38f, -3.402823466e38f, -3.402823466e38f};
+    for (int i = 0; i < st->nc; ++i) {
+        if (!(st->kind[i] & 2)) continue;
+        mean_vol += st->volumes[i];
+        ++n_ac;
+        for (int d = 0; d < 3; ++d) {
+            const float c = (float)st->centers[3 * (size_t)i + d];
+            lo[d] = c < lo[d] ? c : lo[d];
+            hi[d] = c > hi[d] ? c : hi[d];
+        }
+    }
+    if (n_ac > 0) mean_vol /= (double)n_ac;
+    const float R = 2.0f * (float)cbrt(mean_vol);
+    int rc = SK_OK;
+    for (int m = 0; m < n_mics && rc == SK_OK; ++m) {
+        float q[3];
+        for (int d = 0; d < 3; ++d) q[d] = std::max(lo[d], std::min(hi[d], pos[3 * m + d]));
+        auto& pc = st->mp_cell[m];
+        auto& pw = st->mp_w[m];
+        float wsum = 0.0f;
+        const size_t before = pc.size();
+        for (int i = 0; i < st->nc; ++i) {
+            if (!(st->kind[i] & 2)) continue;
+            const float dx = q[0] - (float)st->centers[3 * (size_t)i];
+            const float dy = q[1] - (float)st->centers[3 * (size_t)i + 1];
+            const float dz = q[2] - (float)st->centers[3 * (size_t)i + 2];
+            const float r = sqrtf(dx * dx + dy * dy + dz * dz);
+            if (r < R) {
+                const float w = (r < 1e-9f) ? 1e9f : 1.0f / r;
+                pc.push_back(i);
+                pw.push_back(w);
+                wsum += w;
+            }
+        }
+        if (pc.empty()) { rc = SK_ERR_DIM_MISMATCH; break; }
+        (void)before;
+        const float inv = 1.0f / wsum;
+        for (float& w : pw) w *= inv;
+        if (!need_v) continue;
+        float tot = 0.0f;
+        for (int f = 0; f < st->nf; ++f) {
+            const double* A = &st->centers[3 * (size_t)st->neg[f]];
+            const double* B = &st->centers[3 * (size_t)st->pos[f]];
+            const float fx = 0.5f * (float)(A[0] + B[0]);
+            const float fy = 0.5f * (float)(A[1] + B[1]);
+            const float fz = 0.5f * (float)(A[2] + B[2]);
+            const float ex = fx - q[0], ey = fy - q[1], ez = fz - q[2];
+            const float r2 = sqrtf(ex * ex + ey * ey + ez * ez);
+            if (r2 >= R || st->face_open[f] < 0.5) continue;
+            float nx = (float)(B[0] - A[0]), ny = (float)(B[1] - A[1]), nz = (float)(B[2] - A[2]);
+            const float nl = sqrtf(nx * nx + ny * ny + nz * nz);
+            if (nl < 1e-12f) continue;
+            nx /= nl;
+            ny /= nl;
+            nz /= nl;
+            const float aw = (float)(st->face_area[f] * st->face_open[f]);
+            st->mv_face[m].push_back(f);
+            st->mv_wx[m].push_back(nx * aw);
+            st->mv_wy[m].push_back(ny * aw);
+            st->mv_wz[m].push_back(nz * aw);
+            tot += aw;
+        }
+        if (tot > 1e-20f) {
+            const float inv_a = 1.0f / tot;
+            for (auto* l : {&st->mv_wx, &st->mv_wy, &st->mv_wz})
+                for (float& w : (*l)[m]) w *= inv_a;
+        }
+    }
+    if (upload_mics(st) != hipSuccess) return AMR_ERR_DEVICE;
+    return rc;
+}
+
+SK_API int amr_sample_mics(const AcousticAMRState* cst, int n_mics, float* op, float* ovx,
+                           float* ovy, float* ovz) {
+    if (!cst || !op) return SK_ERR_NULL_STATE;                       // :1762-1792
+    if (n_mics != cst->n_mics) return SK_ERR_DIM_MISMATCH;
+    if (n_mics == 0) return SK_OK;
+    AcousticAMRState* st = const_cast<AcousticAMRState*>(cst);
+    const int* I = st->d_mint;
+    const float* F = st->d_mflt;
+    k_amr_mics<<<(n_mics + 63) / 64, 64, 0, st->stream>>>(
+        I, I + st->o_mpc, F, I + st->o_mvs, I + st->o_mvf, F + st->o_mwx, F + st->o_mwy,
+        F + st->o_mwz, st->d_p, st->d_v, n_mics, st->d_mout);
+    const bool vel = ovx && ovy && ovz;
+    std::vector<float> h(4 * (size_t)n_mics);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(h.data(), st->d_mout, sizeof(float) * h.size(), hipMemcpyDeviceToHost,
+                           st->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(st->stream);
+    if (e != hipSuccess) return AMR_ERR_DEVICE;
+    for (int m = 0; m < n_mics; ++m) {
+        op[m] = h[m];
+        if (vel) {
+            ovx[m] = h[n_mics + m];
+            ovy[m] = h[2 * (size_t)n_mics + m];
+            ovz[m] = h[3 * (size_t)n_mics + m];
+        }
+    }
+    return SK_OK;
 }
 
 }  // extern "C"

@@ -311,3 +311,199 @@ def test_amr_abi_argument_checks_without_device():
     h = L.amr_create(400, *[_p(a) for a in k[:4]], k[4].size, *[_p(a) for a in k[4:]],
                      343.0, 1.21, 0.01, 8)
     assert not h and "gradient_order 8" in last_create_error()[1]
+
+
+# ── Plate coupling and mic samplers (acoustic_amr.cpp:939-1072, :1412-1792) ──
+
+def plated_box(nx, ny, nz, kp, active=None, dx=0.01):
+    """Box grid with a plate layer at z index kp: plate node (i, j) is cell
+    (i, j, kp), typed PLATE when active; its above face joins it to (i, j,
+    kp+1) and its below face joins (i, j, kp-1) to it.  Returns the grid and
+    the keyword arguments of setup_plate."""
+    from afdtd.amr import AcousticAMRGrid
+    g = AcousticAMRGrid.build_uniform_box(x_min=0, x_max=nx * dx, y_min=0, y_max=ny * dx,
+                                          z_min=0, z_max=nz * dx, dx=dx)
+    act = np.ones((nx, ny), np.uint8) if active is None else np.asarray(active, np.uint8)
+    ct = g.cell_types.copy().reshape(nx, ny, nz)
+    ct[:, :, kp][act > 0] = 2
+    flat = np.flatnonzero(act.ravel()).astype(np.int32)
+    i, j = flat // ny, flat % ny
+    F0 = (nx - 1) * ny * nz + nx * (ny - 1) * nz          # first z face
+    zf = lambda ii, jj, kk: F0 + (ii * ny + jj) * (nz - 1) + kk
+    cell = lambda ii, jj, kk: (ii * ny + jj) * nz + kk
+    n = flat.size
+    # plate node spacing 5 dx keeps the explicit plate update stable at the air dt
+    kw = dict(plate_Nx=nx, plate_Ny=ny, plate_dx=5 * dx, plate_origin=[0, 0, kp * dx],
+              plate_active=act.ravel(), plate_rho_h=7.0 * 0.003, plate_D=0.45,
+              plate_alpha_M=2.0, plate_beta_K=1e-5, plate_active_flat_idx=flat,
+              face_above_starts=np.arange(n + 1, dtype=np.int32), face_above_idx=zf(i, j, kp),
+              face_above_wgt=np.ones(n, np.float32),
+              face_below_starts=np.arange(n + 1, dtype=np.int32),
+              face_below_idx=zf(i, j, kp - 1), face_below_wgt=np.ones(n, np.float32),
+              cell_above=cell(i, j, kp + 1), cell_below=cell(i, j, kp - 1))
+    grid = AcousticAMRGrid(**{**g.__dict__, "cell_types": ct.ravel()})
+    return grid, kw
+
+
+def _plate_consts(o, kw):
+    dt = np.float64(np.float32(o.dt))
+    rh, D, aM, bK = (np.float64(np.float32(kw[k])) for k in
+                     ("plate_rho_h", "plate_D", "plate_alpha_M", "plate_beta_K"))
+    dx4 = 1.0 / np.float64(np.float32(kw["plate_dx"])) ** 4
+    return dt, rh, D, aM, bK, dx4
+
+
+def test_plate_acoustic_load_and_velocity_bc_known_answer():
+    """1-cell column, plate at kp: pressure pa above and pb below load the
+    plate by (pb - pa) dx^2; the next step drives the plate faces with
+    +-(w - w_prev)/dt."""
+    nz, kp, pa, pb = 12, 5, 0.3, 1.1
+    grid, kw = plated_box(1, 1, nz, kp)
+    o = OracleAMR.from_grid(grid)
+    o.setup_plate(**kw)
+    p0 = np.zeros(nz, np.float32)
+    p0[kp + 1:] = pa
+    p0[:kp] = pb
+    o.set_state(p0)
+    o.step(1)
+    dt, rh, D, aM, bK, dx4 = _plate_consts(o, kw)
+    dxp = np.float64(np.float32(kw["plate_dx"]))
+    w1 = dt * dt / rh * ((pb - pa) * dxp * dxp) / (1 + 0.5 * aM * dt)
+    assert o.get_plate_displacement(1)[0] == pytest.approx(w1, rel=1e-5)
+    assert np.allclose(o.get_pressure()[kp + 1:], pa) and np.allclose(o.get_pressure()[:kp], pb)
+    o.step(1)
+    v = o.get_velocity()
+    assert v[kw["face_above_idx"][0]] == pytest.approx(w1 / dt, rel=1e-5)
+    assert v[kw["face_below_idx"][0]] == pytest.approx(-w1 / dt, rel=1e-5)
+
+
+def test_plate_isolated_recurrence_and_stencil_weights():
+    """Plate decoupled from the air (empty face lists, the same fallback cell
+    on both sides): a point force gives the node recurrence with the
+    biharmonic centre weight 20, and after two steps the neighbours carry
+    the 13-point weights -8 : 2 : 1."""
+    grid, kw = plated_box(5, 5, 6, 2)
+    n = kw["plate_active_flat_idx"].size
+    kw.update(face_above_starts=np.zeros(n + 1, np.int32), face_above_idx=np.zeros(0, np.int32),
+              face_above_wgt=np.zeros(0, np.float32), face_below_starts=np.zeros(n + 1, np.int32),
+              face_below_idx=np.zeros(0, np.int32), face_below_wgt=np.zeros(0, np.float32),
+              cell_above=np.zeros(n, np.int32), cell_below=np.zeros(n, np.int32))
+    o = OracleAMR.from_grid(grid)
+    o.setup_plate(**kw)
+    o.inject_plate_force([12], [1.0], 2.0)                # centre node (2, 2)
+    o.step(1)
+    w1 = o.get_plate_displacement(25).reshape(5, 5)
+    dt, rh, D, aM, bK, dx4 = _plate_consts(o, kw)
+    df, db = 1 + 0.5 * aM * dt, 1 - 0.5 * aM * dt
+    assert w1[2, 2] == pytest.approx(dt * dt / rh * 2.0 / df, rel=1e-6)
+    assert np.count_nonzero(w1) == 1
+    o.step(1)
+    w2 = o.get_plate_displacement(25).reshape(5, 5).astype(np.float64)
+    c0 = D * (1 + bK / dt)
+    assert w2[2, 3] / w2[3, 3] == pytest.approx(-8 / 2, rel=1e-5)
+    assert w2[2, 4] / w2[3, 3] == pytest.approx(1 / 2, rel=1e-5)
+    assert w2[2, 3] == pytest.approx(dt * dt / rh * (c0 * 8 * dx4 * w1[2, 2]) / df, rel=1e-4)
+    # the centre node alone: free recurrence in float64 over 150 steps
+    g1, kw1 = plated_box(1, 1, 6, 2)
+    kw1.update(face_above_starts=[0, 0], face_above_idx=np.zeros(0, np.int32),
+               face_above_wgt=np.zeros(0, np.float32), face_below_starts=[0, 0],
+               face_below_idx=np.zeros(0, np.int32), face_below_wgt=np.zeros(0, np.float32),
+               cell_above=[0], cell_below=[0])
+    o1 = OracleAMR.from_grid(g1)
+    o1.setup_plate(**kw1)
+    o1.inject_plate_force([0], [1.0], 2.0)
+    ws = []
+    for _ in range(150):
+        o1.step(1)
+        ws.append(o1.get_plate_displacement(1)[0])
+    dt, rh, D, aM, bK, dx4 = _plate_consts(o1, kw1)
+    c0, cp = D * (1 + bK / dt), D * (bK / dt)
+    w, wp, ref = 0.0, 0.0, []
+    f = 2.0
+    for _ in range(150):
+        rhs = f - c0 * 20 * dx4 * w + cp * 20 * dx4 * wp
+        w, wp = (db * (2 * w - wp) + dt * dt / rh * rhs) / df, w
+        ref.append(w)
+        f = 0.0
+    ref = np.array(ref)
+    assert np.linalg.norm(np.array(ws) - ref) <= 1e-4 * np.linalg.norm(ref)
+
+
+def test_plate_injection_order_and_samplers():
+    grid, kw = plated_box(4, 3, 6, 2)
+    o = OracleAMR.from_grid(grid)
+    o.setup_plate(**kw)
+    o.set_bridge_plate_sources([1, 5, 5], [0.5, 0.25, 0.25])
+    o.inject_bridge_drive([2.0, 4.0, -1.0], 3.0)
+    with pytest.raises(RuntimeError, match="rc=-2"):     # stops at the bad node
+        o.inject_plate_force([7, 99, 8], [1.0, 1.0, 1.0], 0.5)
+    o.step(1)
+    w = o.get_plate_displacement(12).astype(np.float64)
+    dt, rh, D, aM, bK, dx4 = _plate_consts(o, kw)
+    s = dt * dt / rh / (1 + 0.5 * aM * dt)
+    # loads: node 1: 3.0, node 5: 3.0 - 0.75 = 2.25, node 7: 0.5, node 8: none
+    assert w[1] / s == pytest.approx(3.0, rel=1e-4)
+    assert w[5] / s == pytest.approx(2.25, rel=1e-4)
+    assert w[7] / s == pytest.approx(0.5, rel=1e-4)
+    assert w[8] == 0.0
+    idx4 = np.array([[1, 5, 7, 0], [2, 3, 4, 6]], np.int32)
+    wgt4 = np.array([[0.1, 0.2, 0.3, 0.4], [0.25, 0.25, 0.25, 0.25]], np.float32)
+    got = o.sample_plate_displacement_batch(idx4, wgt4)
+    wf = o.get_plate_displacement(12)
+    assert np.allclose(got, (wgt4 * wf[idx4]).sum(1), rtol=1e-6, atol=0)
+    assert o.sample_plate_weighted([1, 5], [0.5, 0.5]) == pytest.approx(
+        0.5 * wf[1] + 0.5 * wf[5], rel=1e-6)
+
+
+def mic_grid():
+    """10x9x8 box with a few wall cells and half-open faces."""
+    from afdtd.amr import AcousticAMRGrid
+    g = AcousticAMRGrid.build_uniform_box(x_min=0, x_max=0.1, y_min=0, y_max=0.09, z_min=0,
+                                          z_max=0.08, dx=0.01)
+    rng = np.random.default_rng(15)
+    ct = g.cell_types.copy()
+    ct[rng.choice(g.n_cells, 20, replace=False)] = 1
+    fo = g.face_open_fraction.copy()
+    fo[rng.choice(g.n_faces, 300, replace=False)] = 0.4
+    return dict(cell_centers=g.cell_centers, cell_volumes=g.cell_volumes,
+                open_volume_fraction=g.open_volume_fraction, cell_types=ct,
+                face_cell_neg=g.face_cell_neg, face_cell_pos=g.face_cell_pos,
+                face_area=g.face_area, face_open_fraction=fo, face_distance=g.face_distance,
+                c=343.0, rho_air=1.21, min_dx=0.01, gradient_order=2)
+
+
+# off the lattice, so no cell or face centre sits at the search radius
+# (ties there would decide by float rounding); the third lies outside the
+# grid and is clamped onto its envelope
+MIC_POS = np.array([[0.0513, 0.0447, 0.0391], [0.0132, 0.071, 0.0277], [0.2, -0.05, 0.0333]],
+                   np.float32)
+
+
+def test_mic_samplers_brute_force():
+    t = mic_grid()
+    o = OracleAMR(**t)
+    rng = np.random.default_rng(14)
+    p = rng.standard_normal(o.n_cells).astype(np.float32)
+    v = rng.standard_normal(o.n_faces).astype(np.float32)
+    o.set_state(p, v)
+    pos = MIC_POS
+    o.setup_mic_samplers(pos, np.tile([0, 0, 1], (3, 1)), True)
+    op, vx, vy, vz = o.sample_mics()
+    cc, ct, ov = t["cell_centers"], t["cell_types"], t["open_volume_fraction"]
+    ac = np.flatnonzero(((ct == 0) | (ct == 3)) & (ov > 0))
+    R = 2.0 * np.cbrt(t["cell_volumes"][ac].mean())
+    lo, hi = cc[ac].min(0), cc[ac].max(0)
+    for m, q in enumerate(np.clip(pos.astype(np.float64), lo, hi)):
+        r = np.linalg.norm(cc[ac] - q, axis=1)
+        sel = r < R
+        w = 1.0 / r[sel]
+        assert op[m] == pytest.approx(np.sum(w * p[ac[sel]]) / w.sum(), rel=1e-4, abs=1e-6)
+        a, b = t["face_cell_neg"], t["face_cell_pos"]
+        fc = 0.5 * (cc[a] + cc[b])
+        nrm = cc[b] - cc[a]
+        nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+        aw = t["face_area"] * t["face_open_fraction"]
+        fs = (np.linalg.norm(fc - q, axis=1) < R) & (t["face_open_fraction"] >= 0.5)
+        if fs.any():
+            want = (nrm[fs] * (aw[fs] * v[fs])[:, None]).sum(0) / aw[fs].sum()
+            assert np.allclose([vx[m], vy[m], vz[m]], want, rtol=1e-4, atol=1e-6)

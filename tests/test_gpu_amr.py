@@ -173,3 +173,66 @@ def test_full_size_boxes():
     g.step(100)
     m1 = float(np.sum(g.get_pressure().astype(np.float64))) * w
     assert abs(m1 - m0) <= 1e-6 * scale
+
+
+def test_plate_coupled_run_bitwise():
+    """Plate layer with a ragged active mask, bridge drive and a distributed
+    force injected between steps, single steps and a 50-step call."""
+    from afdtd.amr import AcousticAMR
+    from test_amr_oracle_cpu import plated_box
+    print("amr: plate-coupled box 12x10x16", flush=True)
+    rng = np.random.default_rng(41)
+    act = (rng.uniform(size=(12, 10)) < 0.8).astype(np.uint8)
+    grid, kw = plated_box(12, 10, 16, 6, active=act)
+    g, o = _pair_grid(grid)
+    flat = kw["plate_active_flat_idx"]
+    for m in (g, o):
+        m.setup_plate(**kw)
+        m.set_bridge_plate_sources(flat[[3, 4, 9]], [0.5, 0.3, 0.2])
+        m.inject_pressure_nearest([0.061, 0.052, 0.113], 1.0)
+    drives = rng.standard_normal((80, 3)).astype(np.float32)
+    for s in range(80):
+        for m in (g, o):
+            m.inject_bridge_drive(drives[s], 0.01)
+            if s % 7 == 0:
+                m.inject_plate_force(flat[[1, 2, 1]], [0.5, 0.25, 0.25], 0.02)
+            m.step(1)
+    _same(g, o)
+    n = 12 * 10
+    assert np.array_equal(_bits(g.get_plate_displacement(n)), _bits(o.get_plate_displacement(n)))
+    assert np.abs(g.get_plate_displacement(n)).max() > 0
+    for m in (g, o):
+        m.step(50)
+    _same(g, o)
+    assert np.array_equal(_bits(g.get_plate_displacement(n)), _bits(o.get_plate_displacement(n)))
+    idx4 = flat[rng.integers(0, flat.size, (9, 4))]
+    wgt4 = rng.uniform(size=(9, 4)).astype(np.float32)
+    assert np.array_equal(_bits(g.sample_plate_displacement_batch(idx4, wgt4)),
+                          _bits(o.sample_plate_displacement_batch(idx4, wgt4)))
+    assert g.sample_plate_weighted(flat[:5], [0.2] * 5) == o.sample_plate_weighted(flat[:5],
+                                                                                  [0.2] * 5)
+    for m in (g, o):
+        m.reset()
+        assert not np.any(m.get_plate_displacement(n))
+
+
+def test_mic_samplers_bitwise():
+    from afdtd.amr import AcousticAMR
+    from test_amr_oracle_cpu import MIC_POS, mic_grid
+    print("amr: mic samplers", flush=True)
+    t = mic_grid()
+    g, o = AcousticAMR(**t), OracleAMR(**t)
+    rng = np.random.default_rng(42)
+    p = rng.standard_normal(g.n_cells).astype(np.float32)
+    v = rng.standard_normal(g.n_faces).astype(np.float32)
+    axis = np.tile([0, 0, 1], (3, 1))
+    for m in (g, o):
+        m.set_state(p, v)
+        m.setup_mic_samplers(MIC_POS, axis, True)
+    for a, b in zip(g.sample_mics(), o.sample_mics()):
+        assert np.array_equal(_bits(a), _bits(b))
+    for m in (g, o):
+        m.step(20)
+        m.setup_mic_samplers(MIC_POS[:2], axis[:2], False)   # a second set-up appends
+    for a, b in zip(g.sample_mics(), o.sample_mics()):
+        assert np.array_equal(_bits(a), _bits(b))
