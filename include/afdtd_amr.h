@@ -16,8 +16,9 @@
  * centre distance.  Velocity is positive from neg to pos.
  *
  * Device execution: every handle owns one HIP stream.  amr_step enqueues two
- * kernels per step (faces, then cells) and synchronises once before
- * returning (it must report SK_ERR_DIVERGED).  Getters are synchronous on
+ * kernels per step (faces, then cells; with a plate also the plate's face
+ * condition and the plate step) and synchronises once before returning (it
+ * must report SK_ERR_DIVERGED).  Getters are synchronous on
  * return.  Handles are not thread-safe (as in the reference).
  */
 #ifndef AFDTD_AMR_H
