@@ -343,9 +343,10 @@ def bench_amr(a):
     n, dx, n_pml = 256, 0.01, 8
     grid = AcousticAMRGrid.build_uniform_box(x_min=0, x_max=n * dx, y_min=0, y_max=n * dx,
                                              z_min=0, z_max=n * dx, dx=dx).with_pml_shell(n_pml)
+    order = a.amr_order
     args = (grid.cell_centers, grid.cell_volumes, grid.open_volume_fraction, grid.cell_types,
             grid.face_cell_neg, grid.face_cell_pos, grid.face_area, grid.face_open_fraction,
-            grid.face_distance, 343.0, 1.21, dx, 2)
+            grid.face_distance, 343.0, 1.21, dx, order)
     g = AcousticAMR(*args)
     g.set_border_condition(0, 3.0, 0.0, 0.0, n_pml, grid.bounds_min, grid.bounds_max)
     g.inject_pressure_nearest([1.0, 1.2, 1.4], 1.0)
@@ -369,28 +370,29 @@ def bench_amr(a):
         raise SystemExit("non-finite AMR pressure")
     C_, F, E = g.n_cells, g.n_faces, g.n_csr_entries
     # algorithmic HBM bytes per step (each array touched once):
-    #   faces kernel: record 16 + v read 4 + v write 4 per face, p read 4 per cell
+    #   faces kernel: record 16 + v read 4 + v write 4 per face, p read 4 per cell;
+    #                 order 8 adds the 8-cell stencil (32 B cells + 32 B weights)
     #   cells kernel: record 16 + row start 4 + p read/write 8 per cell,
     #                 CSR index 4 per entry, v and face weight 4 + 4 per face
-    b_faces = 24 * F + 4 * C_
+    b_faces = (24 + (64 if order == 8 else 0)) * F + 4 * C_
     b_cells = 28 * C_ + 4 * E + 8 * F
     fa = b_faces / (ms_f / steps * 1e-3) / 1e9
     ca = b_cells / (ms_c / steps * 1e-3) / 1e9
     dom = "cells" if ms_c >= ms_f else "faces"
     # HBM bytes per k_amr_cells launch counted with rocprofv3 on this workload
     amr_traffic = None
-    prof = os.path.join(ROOT, "profiles", "amr_uniform_box_256cube_order2_anechoic8.txt")
+    prof = os.path.join(ROOT, "profiles", f"amr_uniform_box_256cube_order{order}_anechoic8.txt")
     if os.path.exists(prof):
         for ln in open(prof):
             if ln.startswith("traffic_bytes_per_launch:"):
                 amr_traffic = int(ln.split(":")[1])
     ach = ca if dom == "cells" else fa
-    line = {"metric": "cell-updates/s (AMR order-2 CSR stepper, 1 GPU)",
+    line = {"metric": f"cell-updates/s (AMR order-{order} CSR stepper, 1 GPU)",
             "value": C_ * steps / wall, "unit": "cell-updates/s", "n_gpus": 1, "steps": steps,
             "warmup": warm, "ms_per_step": wall * 1e3 / steps, "higher_is_better": True,
             "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
             "data": "synthetic (uniform box grid, one pulse)",
-            "config": {"workload": f"amr-uniform-box-{n}x{n}x{n}-order2-anechoic{n_pml}",
+            "config": {"workload": f"amr-uniform-box-{n}x{n}x{n}-order{order}-anechoic{n_pml}",
                        "cells": C_, "faces": F, "csr_entries": E},
             "kernel_ms_per_step": {"faces": ms_f / steps, "cells": ms_c / steps},
             "event_ms_total": ms_t,
@@ -435,6 +437,8 @@ def main():
     ap.add_argument("--tile-y", type=int, default=0)
     ap.add_argument("--seg-x", type=int, default=0)
     ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--amr-order", type=int, default=2, choices=(2, 8),
+                    help="amr: gradient order of the AMR stepper")
     ap.add_argument("--no-fd", action="store_true",
                     help="adjoint: float32 timing only (no float64 pass, no finite differences)")
     a = ap.parse_args()

@@ -6,8 +6,9 @@
  * that cover the gradient-order-2 stepper, with the same names, signatures,
  * return codes and ownership rules; each cites the declaration it replaces
  * (csrc/include/acoustic_amr.h:line) and the definition whose behaviour it
- * reproduces (csrc/kernels/acoustic_amr.cpp:line).  Not provided: gradient
- * order 8 (amr_create returns NULL with stage "validate_inputs").
+ * reproduces (csrc/kernels/acoustic_amr.cpp:line).  Both gradient orders
+ * are provided: 2 (face-pair difference) and 8 (per-face 8-cell stencil
+ * built at create with the reference's weight recurrence, as written).
  *
  * Part 2 declares extensions under new names (state upload, timed steps).
  *
@@ -58,7 +59,11 @@ typedef struct AcousticAMRState AcousticAMRState;
 /* acoustic_amr.h:20-35; acoustic_amr.cpp:682-880 (inputs validated as
  * :275-311, copied; dt = 0.77*min_dx/(c*sqrt 3); per-face 1/d and area*open,
  * per-cell 1/(V*max(open,1e-12)) on acoustic cells, signed CSR of faces per
- * cell in face order).  gradient_order must be 2 here; 8 returns NULL. */
+ * cell in face order; gradient order 8 also builds each face's stencil:
+ * up to 4 cells on each side along the face normal, walked through
+ * axis-neighbour tables or the most aligned face, weighted by the
+ * reference's recurrence (acoustic_amr.cpp:313-679), else the two-point
+ * difference; dt uses 0.10 instead of 0.77 for order 8). */
 SK_API AcousticAMRState* amr_create(
     int            n_cells,
     const double*  cell_centers,
@@ -80,8 +85,9 @@ SK_API AcousticAMRState* amr_create(
 SK_API const char* amr_get_last_create_error_stage(void);
 SK_API const char* amr_get_last_create_error_message(void);
 
-/* acoustic_amr.h:53-55; acoustic_amr.cpp:902-915.  Order 2 builds no
- * stencil, so these stay 0. */
+/* acoustic_amr.h:53-55; acoustic_amr.cpp:902-915.  The order-8 stencil is
+ * built synchronously inside amr_create, so "active" reads 0; done and total
+ * faces report the last order-8 build. */
 SK_API int amr_get_create_progress_active(void);
 SK_API int amr_get_create_progress_done_faces(void);
 SK_API int amr_get_create_progress_total_faces(void);

@@ -14,9 +14,8 @@
  * reference's expression order and is compiled without FMA contraction, so
  * the HIP kernels are expected to equal it bit for bit.
  *
- * Not restated: gradient order 8 (the Fornberg stencil builder,
- * :398-631), the AMR plate (:956-1048) and the mic samplers (:1647-1792);
- * amo_create returns NULL for order 8.
+ * Gradient order 8 (the Fornberg stencil builder, :313-679) is restated
+ * too; its progress reporting is not.
  *
  * The reference cannot be compiled here (acoustic_amr.cpp includes
  * <Eigen/Dense>, which the image lacks), so this restatement is pinned by a
@@ -50,6 +49,8 @@ typedef struct Amo {
     int *acoustic; int n_acoustic;   /* (:783-785) */
     int *wallplate; int n_wallplate; /* (:813-816) */
     int *csr_start, *csr_face; float *csr_w;  /* (:818-849) */
+    int *sc; float *sw;                       /* order 8: 8 cells + weights per face */
+    int *dnb[6];                              /* order 8: +x,-x,+y,-y,+z,-z neighbours */
     double *volumes;                          /* nc (mic search radius) */
     double *face_area, *face_open;            /* nf (mic velocity weights) */
     /* Kirchhoff plate (:1412-1531) */
@@ -96,6 +97,192 @@ static int amo_validate(int nc, const double* cc, const double* vol, const doubl
     return AMO_OK;
 }
 
+/* ── Gradient order 8 (acoustic_amr.cpp:313-679) ───────────────────────── */
+
+/* Fornberg's first-derivative weights at xi over N <= 8 nodes, float, the
+ * reference's loop (acoustic_amr.cpp:398-426). */
+SK_EXPORT void amo_fornberg_d1(const float* x, int N, float xi, float* w)
+{
+    float c[16];
+    for (int k = 0; k < 16; ++k) c[k] = 0.0f;
+#define CC(k, m) c[(k) * 2 + (m)]
+    float c1 = 1.0f;
+    CC(0, 0) = 1.0f;
+    for (int n = 1; n < N; ++n) {
+        const int mn = n < 1 ? n : 1;
+        float c2 = 1.0f;
+        for (int nu = 0; nu < n; ++nu) {
+            const float c3 = x[n] - x[nu];
+            c2 *= c3;
+            if (nu == n - 1) {
+                for (int m = mn; m >= 1; --m)
+                    CC(n, m) = c1 / c2 * ((float)m * CC(n - 1, m - 1) - (x[n] - xi) * CC(n - 1, m));
+                CC(n, 0) = c1 / c2 * (-(x[n] - xi)) * CC(n - 1, 0);
+            }
+            for (int m = mn; m >= 1; --m)
+                CC(nu, m) = ((x[n] - xi) * CC(nu, m) - (float)m * CC(nu, m - 1)) / c3;
+            CC(nu, 0) = (x[n] - xi) * CC(nu, 0) / c3;
+        }
+        c1 = c2;
+    }
+    for (int k = 0; k < N; ++k) w[k] = CC(k, 1);
+#undef CC
+}
+
+static float amo_proj(const Amo* a, int ci, const float* d)
+{
+    return (float)(a->centers[3 * ci] * d[0] + a->centers[3 * ci + 1] * d[1]
+                   + a->centers[3 * ci + 2] * d[2]);
+}
+
+/* Dominant axis of a direction if its largest |component| > 0.9, else -1
+ * (acoustic_amr.cpp:351-357, :465-471). */
+static int amo_axis(float ax, float ay, float az)
+{
+    float best = ax > ay ? ax : ay;
+    best = best > az ? best : az;
+    if (!(best > 0.9f)) return -1;
+    if (ax >= ay && ax >= az) return 0;
+    if (ay >= ax && ay >= az) return 1;
+    return 2;
+}
+
+/* amr_build_dir_neighbors, acoustic_amr.cpp:329-391: per axis and side the
+ * unique axis-aligned neighbour, -1 if none or conflicting. */
+static void amo_build_dir_neighbors(Amo* a)
+{
+    uint8_t* st[6];
+    for (int q = 0; q < 6; ++q) {
+        a->dnb[q] = (int*)malloc(sizeof(int) * (size_t)a->nc);
+        st[q] = (uint8_t*)calloc((size_t)a->nc, 1);
+        for (int i = 0; i < a->nc; ++i) a->dnb[q][i] = -1;
+    }
+    for (int f = 0; f < a->nf; ++f) {
+        const int cn = a->neg[f], cp = a->pos[f];
+        const float dx = (float)(a->centers[3 * cp] - a->centers[3 * cn]);
+        const float dy = (float)(a->centers[3 * cp + 1] - a->centers[3 * cn + 1]);
+        const float dz = (float)(a->centers[3 * cp + 2] - a->centers[3 * cn + 2]);
+        const float len = sqrtf(dx * dx + dy * dy + dz * dz);
+        if (len < 1e-15f) continue;
+        const int axis = amo_axis(fabsf(dx / len), fabsf(dy / len), fabsf(dz / len));
+        if (axis < 0) continue;
+        const float sd = axis == 0 ? dx : (axis == 1 ? dy : dz);
+        /* table 2*axis holds the + neighbour, 2*axis+1 the - neighbour */
+        const int lo = sd > 0.0f ? cn : cp, hi = sd > 0.0f ? cp : cn;
+        int* tp = &a->dnb[2 * axis][lo];
+        uint8_t* sp = &st[2 * axis][lo];
+        if (*sp == 0) { *tp = hi; *sp = 1; } else if (*sp == 1 && *tp != hi) { *tp = -1; *sp = 2; }
+        int* tn = &a->dnb[2 * axis + 1][hi];
+        uint8_t* sn = &st[2 * axis + 1][hi];
+        if (*sn == 0) { *tn = lo; *sn = 1; } else if (*sn == 1 && *tn != lo) { *tn = -1; *sn = 2; }
+    }
+    for (int q = 0; q < 6; ++q) free(st[q]);
+}
+
+/* amr_walk_stencil, acoustic_amr.cpp:444-547 */
+static int amo_walk(const Amo* a, int seed, float fproj, const float* cd, const float* wd,
+                    int depth, int* oc, float* ox)
+{
+    oc[0] = seed;
+    ox[0] = amo_proj(a, seed, cd) - fproj;
+    int found = 1, cur = seed;
+    const int axis = amo_axis(fabsf(wd[0]), fabsf(wd[1]), fabsf(wd[2]));
+    if (axis >= 0) {
+        const int* tbl = a->dnb[2 * axis + (wd[axis] > 0.0f ? 0 : 1)];
+        while (found < depth) {
+            const int nb = tbl[cur];
+            if (nb < 0 || nb >= a->nc) break;
+            if (a->type[nb] == 1 || a->type[nb] == 2) break;
+            oc[found] = nb;
+            ox[found] = amo_proj(a, nb, cd) - fproj;
+            ++found;
+            cur = nb;
+        }
+        return found;
+    }
+    while (found < depth) {
+        int best_next = -1;
+        float best_dot = 0.5f;
+        for (int k = a->csr_start[cur]; k < a->csr_start[cur + 1]; ++k) {
+            const int fk = a->csr_face[k];
+            const float sg = (a->csr_w[k] > 0.f) ? 1.f : -1.f;
+            const int fn = a->neg[fk], fp = a->pos[fk];
+            const float dx = (float)(a->centers[3 * fp] - a->centers[3 * fn]);
+            const float dy = (float)(a->centers[3 * fp + 1] - a->centers[3 * fn + 1]);
+            const float dz = (float)(a->centers[3 * fp + 2] - a->centers[3 * fn + 2]);
+            const float len = sqrtf(dx * dx + dy * dy + dz * dz);
+            if (len < 1e-15f) continue;
+            const float dot = sg * (dx * wd[0] + dy * wd[1] + dz * wd[2]) / len;
+            if (dot > best_dot) {
+                const int nb = (sg > 0.f) ? fp : fn;
+                if (nb >= 0 && nb < a->nc && a->type[nb] != 1 && a->type[nb] != 2) {
+                    best_dot = dot;
+                    best_next = nb;
+                }
+            }
+        }
+        if (best_next < 0) break;
+        oc[found] = best_next;
+        ox[found] = amo_proj(a, best_next, cd) - fproj;
+        ++found;
+        cur = best_next;
+    }
+    return found;
+}
+
+/* amr_build_stencil_range, acoustic_amr.cpp:549-629: 4 cells on each side,
+ * Fornberg weights when both sides are complete, else the two-point
+ * difference; unused slots hold cell 0 with weight 0. */
+static void amo_build_stencil(Amo* a)
+{
+    amo_build_dir_neighbors(a);
+    a->sc = (int*)calloc((size_t)a->nf * 8, sizeof(int));
+    a->sw = (float*)calloc((size_t)a->nf * 8, sizeof(float));
+    #pragma omp parallel for schedule(dynamic, 4096)
+    for (int f = 0; f < a->nf; ++f) {
+        const int cn = a->neg[f], cp = a->pos[f];
+        const float dxf = (float)(a->centers[3 * cp] - a->centers[3 * cn]);
+        const float dyf = (float)(a->centers[3 * cp + 1] - a->centers[3 * cn + 1]);
+        const float dzf = (float)(a->centers[3 * cp + 2] - a->centers[3 * cn + 2]);
+        const float len = sqrtf(dxf * dxf + dyf * dyf + dzf * dzf);
+        if (!(len >= 1e-15f)) continue;
+        const float dir[3] = {dxf / len, dyf / len, dzf / len};
+        const float mdir[3] = {-dir[0], -dir[1], -dir[2]};
+        const float fproj = 0.5f * (amo_proj(a, cn, dir) + amo_proj(a, cp, dir));
+        int nc_[4], pc_[4];
+        float nx_[4], px_[4];
+        const int nn = amo_walk(a, cn, fproj, dir, mdir, 4, nc_, nx_);
+        const int np = amo_walk(a, cp, fproj, dir, dir, 4, pc_, px_);
+        if (nn == 4 && np == 4) {
+            float xs[8], w[8];
+            int cs[8];
+            for (int k = 0; k < 4; ++k) { cs[k] = nc_[3 - k]; xs[k] = nx_[3 - k]; }
+            for (int k = 0; k < 4; ++k) { cs[4 + k] = pc_[k]; xs[4 + k] = px_[k]; }
+            amo_fornberg_d1(xs, 8, 0.0f, w);
+            for (int k = 0; k < 4; ++k) {
+                a->sc[8 * (size_t)f + (3 - k)] = cs[k];
+                a->sw[8 * (size_t)f + (3 - k)] = w[k];
+                a->sc[8 * (size_t)f + 4 + k] = cs[4 + k];
+                a->sw[8 * (size_t)f + 4 + k] = w[4 + k];
+            }
+        } else if (nn + np >= 2) {
+            const float inv_d = a->inv_d[f];  /* face_distance > 0 is validated */
+            a->sc[8 * (size_t)f] = cn;
+            a->sw[8 * (size_t)f] = -inv_d;
+            a->sc[8 * (size_t)f + 4] = cp;
+            a->sw[8 * (size_t)f + 4] = inv_d;
+        }
+    }
+}
+
+SK_EXPORT int amo_get_stencil(const Amo* a, int* sc, float* sw)
+{
+    if (!a || !a->sc) return AMO_ERR_NULL;
+    memcpy(sc, a->sc, sizeof(int) * 8 * (size_t)a->nf);
+    memcpy(sw, a->sw, sizeof(float) * 8 * (size_t)a->nf);
+    return AMO_OK;
+}
+
 SK_EXPORT void amo_destroy(Amo* a)
 {
     if (!a) return;
@@ -105,6 +292,8 @@ SK_EXPORT void amo_destroy(Amo* a)
     free(a->acoustic); free(a->wallplate);
     free(a->csr_start); free(a->csr_face); free(a->csr_w);
     free(a->volumes); free(a->face_area); free(a->face_open);
+    free(a->sc); free(a->sw);
+    for (int q = 0; q < 6; ++q) free(a->dnb[q]);
     free(a->w); free(a->w_prev); free(a->w_new); free(a->ext); free(a->active);
     free(a->ab_start); free(a->ab_face); free(a->ab_side); free(a->ab_wgt);
     free(a->bl_start); free(a->bl_face); free(a->bl_side); free(a->bl_wgt);
@@ -127,11 +316,11 @@ SK_EXPORT Amo* amo_create(int nc, const double* cc, const double* vol, const dou
 {
     if (amo_validate(nc, cc, vol, ov, ct, nf, fn, fp, fa, fo, fd, c, rho, min_dx) != AMO_OK)
         return NULL;
-    if (order != 2) return NULL;
+    if (order != 2 && order != 8) return NULL;
     Amo* a = (Amo*)calloc(1, sizeof(Amo));
     a->nc = nc; a->nf = nf; a->order = order;
     a->c = c; a->rho = rho;
-    a->dt = 0.77 * min_dx / (c * sqrt(3.0));                         /* :744-745 */
+    a->dt = ((order == 8) ? 0.10 : 0.77) * min_dx / (c * sqrt(3.0));  /* :744-745 */
     a->centers = (double*)malloc(sizeof(double) * 3 * (size_t)nc);
     memcpy(a->centers, cc, sizeof(double) * 3 * (size_t)nc);
     a->open_vol = (double*)malloc(sizeof(double) * (size_t)nc);
@@ -190,6 +379,7 @@ SK_EXPORT Amo* amo_create(int nc, const double* cc, const double* vol, const dou
     a->psrc = (float*)malloc(sizeof(float) * (size_t)nc);
     for (int f = 0; f < nf; ++f) a->vdamp[f] = 1.0f;
     for (int i = 0; i < nc; ++i) { a->pdamp[i] = 1.0f; a->psrc[i] = 1.0f; }
+    if (order == 8) amo_build_stencil(a);                             /* :862-869 */
     return a;
 }
 
@@ -606,10 +796,19 @@ SK_EXPORT int amo_step(Amo* a, int n_steps)
     const float dt_over_rho = dt / rho;
     const int nf = a->nf, nc = a->nc;
     for (int s = 0; s < n_steps; ++s) {
-        #pragma omp parallel for schedule(static)
-        for (int f = 0; f < nf; ++f) {                                /* :1147-1154 */
-            const float g = (a->p[a->pos[f]] - a->p[a->neg[f]]) * a->inv_d[f];
-            a->v[f] = (a->v[f] - dt_over_rho * g) * a->vdamp[f];
+        if (a->order == 8) {
+            #pragma omp parallel for schedule(static)
+            for (int f = 0; f < nf; ++f) {                            /* :1127-1143 */
+                float g = 0.0f;
+                for (int k = 0; k < 8; ++k) g += a->sw[8 * (size_t)f + k] * a->p[a->sc[8 * (size_t)f + k]];
+                a->v[f] = (a->v[f] - dt_over_rho * g) * a->vdamp[f];
+            }
+        } else {
+            #pragma omp parallel for schedule(static)
+            for (int f = 0; f < nf; ++f) {                            /* :1147-1154 */
+                const float g = (a->p[a->pos[f]] - a->p[a->neg[f]]) * a->inv_d[f];
+                a->v[f] = (a->v[f] - dt_over_rho * g) * a->vdamp[f];
+            }
         }
         if (a->plate_on) amo_plate_bc(a, dt);                         /* :1157 */
         #pragma omp parallel for schedule(static)

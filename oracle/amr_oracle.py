@@ -52,6 +52,8 @@ def lib():
             "amo_get_plate": (_i, [_vp, _vp, _i]),
             "amo_setup_mics": (_i, [_vp, _i, _vp, _vp, _i]),
             "amo_sample_mics": (_i, [_vp, _i, _vp, _vp, _vp, _vp]),
+            "amo_fornberg_d1": (None, [_vp, _i, _f, _vp]),
+            "amo_get_stencil": (_i, [_vp, _vp, _vp]),
         }
         for n, (r, a) in sigs.items():
             getattr(L, n).restype = r
@@ -62,6 +64,14 @@ def lib():
 
 def _p(a):
     return a.ctypes.data_as(C.c_void_p)
+
+
+def fornberg_d1(x, xi=0.0):
+    """First-derivative weights at xi over the nodes x (float32, <= 8 nodes)."""
+    xs = np.ascontiguousarray(x, np.float32)
+    w = np.zeros(xs.size, np.float32)
+    lib().amo_fornberg_d1(_p(xs), xs.size, float(xi), _p(w))
+    return w
 
 
 class OracleAMR:
@@ -256,3 +266,10 @@ class OracleAMR:
         o = [np.zeros(n, np.float32) for _ in range(4)]
         self._rc("amr_sample_mics", lib().amo_sample_mics(self._h, n, *[_p(x) for x in o]))
         return tuple(o)
+
+    def get_stencil(self):
+        """Order 8: (cells, weights), each (n_faces, 8)."""
+        sc = np.zeros((self.n_faces, 8), np.int32)
+        sw = np.zeros((self.n_faces, 8), np.float32)
+        self._rc("stencil", lib().amo_get_stencil(self._h, _p(sc), _p(sw)))
+        return sc, sw

@@ -27,12 +27,16 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
+#include <thread>
 #include <string>
 #include <vector>
 
 #include "../../include/afdtd_amr.h"
 
 static thread_local std::string g_amr_stage = "ok";
+// stencil-build progress of the last order-8 create (global, as the reference's)
+static std::atomic<int> g_done_faces{0}, g_total_faces{0};
 static thread_local std::string g_amr_msg;
 
 struct AcousticAMRState {
@@ -47,6 +51,8 @@ struct AcousticAMRState {
     int* d_csr = nullptr;
     float* d_w = nullptr;
     double* d_cent = nullptr;
+    int4* d_sc = nullptr;                    // order 8: 8 cells per face (two int4)
+    float4* d_sw = nullptr;                  // order 8: 8 weights per face (two float4)
     int* d_status = nullptr;                 // [0] failing step, [1] nearest cell
     unsigned long long* d_best = nullptr;    // nearest squared distance (bits)
     // host copies (set-up inputs, coefficient rebuilds, getters)
@@ -124,6 +130,35 @@ __global__ __launch_bounds__(256) void k_amr_faces(const int4* __restrict__ face
         const float g = (a1 - b1) * __int_as_float(r1.z);
         v[f1] = (v1 - dt_over_rho * g) * __int_as_float(r1.w);
     }
+}
+
+// Gradient order 8 (acoustic_amr.cpp:1127-1143): the face's 8-cell stencil
+// (two 16-byte cell records, two 16-byte weight records per face), summed
+// in slot order from 0 including the zero-weight padding slots (cell 0).
+__global__ __launch_bounds__(256) void k_amr_faces8(const int4* __restrict__ sc,
+                                                    const float4* __restrict__ sw,
+                                                    const int4* __restrict__ face,
+                                                    float* __restrict__ v,
+                                                    const float* __restrict__ p, int nf,
+                                                    float dt_over_rho,
+                                                    const int* __restrict__ status, int s) {
+    if (status[0] < s) return;
+    const int f = xcd_tile(gridDim.x) * 256 + threadIdx.x;
+    if (f >= nf) return;
+    const int4 a = sc[2 * (size_t)f], b = sc[2 * (size_t)f + 1];
+    const float4 wa = sw[2 * (size_t)f], wb = sw[2 * (size_t)f + 1];
+    const float vd = __int_as_float(face[f].w);
+    const float pa[8] = {p[a.x], p[a.y], p[a.z], p[a.w], p[b.x], p[b.y], p[b.z], p[b.w]};
+    float g = 0.0f;
+    g += wa.x * pa[0];
+    g += wa.y * pa[1];
+    g += wa.z * pa[2];
+    g += wa.w * pa[3];
+    g += wb.x * pa[4];
+    g += wb.y * pa[5];
+    g += wb.z * pa[6];
+    g += wb.w * pa[7];
+    v[f] = (v[f] - dt_over_rho * g) * vd;
 }
 
 // One 256-thread block updates 256 consecutive cells.  The block's CSR rows
@@ -380,6 +415,175 @@ __global__ void k_amr_mics(const int* __restrict__ mp_start, const int* __restri
 
 inline unsigned blocks(size_t n) { return (unsigned)((n + 255) / 256); }
 
+// ── Gradient order 8: per-face Fornberg stencil (acoustic_amr.cpp:313-679) ──
+// Built once on the host, as in the reference; each face's stencil depends
+// only on the topology, so faces are split over host threads.
+
+// The reference's first-derivative weight recurrence, as written
+// (acoustic_amr.cpp:398-426: the new node's row uses x[n] - xi).
+void fornberg_ref(const float* x, int N, float xi, float* w) {
+    float c[16] = {};
+    float c1 = 1.0f;
+    c[0] = 1.0f;
+    for (int n = 1; n < N; ++n) {
+        const int mn = std::min(n, 1);
+        float c2 = 1.0f;
+        for (int nu = 0; nu < n; ++nu) {
+            const float c3 = x[n] - x[nu];
+            c2 *= c3;
+            if (nu == n - 1) {
+                for (int m = mn; m >= 1; --m)
+                    c[2 * n + m] = c1 / c2 * ((float)m * c[2 * (n - 1) + m - 1] - (x[n] - xi) * c[2 * (n - 1) + m]);
+                c[2 * n] = c1 / c2 * (-(x[n] - xi)) * c[2 * (n - 1)];
+            }
+            for (int m = mn; m >= 1; --m)
+                c[2 * nu + m] = ((x[n] - xi) * c[2 * nu + m] - (float)m * c[2 * nu + m - 1]) / c3;
+            c[2 * nu] = (x[n] - xi) * c[2 * nu] / c3;
+        }
+        c1 = c2;
+    }
+    for (int k = 0; k < N; ++k) w[k] = c[2 * k + 1];
+}
+
+struct StencilBuilder {
+    const AcousticAMRState* st;
+    const std::vector<int>& start;
+    const std::vector<int>& csr;       // f, or ~f for the face's pos cell
+    const std::vector<float>& ffc;
+    std::vector<int> nb[6];            // +x, -x, +y, -y, +z, -z neighbour (-1: none)
+
+    float proj(int ci, const float* d) const {
+        const double* q = &st->centers[3 * (size_t)ci];
+        return (float)(q[0] * d[0] + q[1] * d[1] + q[2] * d[2]);
+    }
+    float3 delta(int a, int b) const {    // centre b - centre a, per component in float
+        const double* A = &st->centers[3 * (size_t)a];
+        const double* B = &st->centers[3 * (size_t)b];
+        return make_float3((float)(B[0] - A[0]), (float)(B[1] - A[1]), (float)(B[2] - A[2]));
+    }
+    static int axis_of(float ax, float ay, float az) {
+        const float best = std::max({ax, ay, az});
+        if (!(best > 0.9f)) return -1;
+        if (ax >= ay && ax >= az) return 0;
+        if (ay >= ax && ay >= az) return 1;
+        return 2;
+    }
+    bool blocked(int c) const { return st->type[c] == 1 || st->type[c] == 2; }
+
+    // acoustic_amr.cpp:329-391
+    void build_neighbours() {
+        std::vector<uint8_t> state[6];
+        for (int q = 0; q < 6; ++q) {
+            nb[q].assign(st->nc, -1);
+            state[q].assign(st->nc, 0);
+        }
+        auto set = [&](int q, int c, int val) {
+            int& t = nb[q][c];
+            uint8_t& k = state[q][c];
+            if (k == 0) { t = val; k = 1; }
+            else if (k == 1 && t != val) { t = -1; k = 2; }
+        };
+        for (int f = 0; f < st->nf; ++f) {
+            const int cn = st->neg[f], cp = st->pos[f];
+            const float3 d = delta(cn, cp);
+            const float len = sqrtf(d.x * d.x + d.y * d.y + d.z * d.z);
+            if (len < 1e-15f) continue;
+            const int ax = axis_of(fabsf(d.x / len), fabsf(d.y / len), fabsf(d.z / len));
+            if (ax < 0) continue;
+            const float sd = ax == 0 ? d.x : (ax == 1 ? d.y : d.z);
+            const int lo = sd > 0.0f ? cn : cp, hi = sd > 0.0f ? cp : cn;
+            set(2 * ax, lo, hi);
+            set(2 * ax + 1, hi, lo);
+        }
+    }
+
+    // acoustic_amr.cpp:444-547
+    int walk(int seed, float fproj, const float* cd, const float* wd, int* oc, float* ox) const {
+        oc[0] = seed;
+        ox[0] = proj(seed, cd) - fproj;
+        int found = 1, cur = seed;
+        const int ax = axis_of(fabsf(wd[0]), fabsf(wd[1]), fabsf(wd[2]));
+        if (ax >= 0) {
+            const std::vector<int>& t = nb[2 * ax + (wd[ax] > 0.0f ? 0 : 1)];
+            while (found < 4) {
+                const int n = t[cur];
+                if (n < 0 || n >= st->nc || blocked(n)) break;
+                oc[found] = n;
+                ox[found] = proj(n, cd) - fproj;
+                ++found;
+                cur = n;
+            }
+            return found;
+        }
+        while (found < 4) {
+            int best_next = -1;
+            float best_dot = 0.5f;
+            for (int k = start[cur]; k < start[cur + 1]; ++k) {
+                const int e = csr[k];
+                const int fk = e < 0 ? ~e : e;
+                const float wk = e < 0 ? -ffc[fk] : ffc[fk];  // the reference's signed weight
+                const float sg = wk > 0.f ? 1.f : -1.f;
+                const int fn = st->neg[fk], fp = st->pos[fk];
+                const float3 d = delta(fn, fp);
+                const float len = sqrtf(d.x * d.x + d.y * d.y + d.z * d.z);
+                if (len < 1e-15f) continue;
+                const float dot = sg * (d.x * wd[0] + d.y * wd[1] + d.z * wd[2]) / len;
+                if (dot > best_dot) {
+                    const int n = sg > 0.f ? fp : fn;
+                    if (n >= 0 && n < st->nc && !blocked(n)) {
+                        best_dot = dot;
+                        best_next = n;
+                    }
+                }
+            }
+            if (best_next < 0) break;
+            oc[found] = best_next;
+            ox[found] = proj(best_next, cd) - fproj;
+            ++found;
+            cur = best_next;
+        }
+        return found;
+    }
+
+    // acoustic_amr.cpp:549-629, for faces [f0, f1)
+    void faces(int f0, int f1, int* sc, float* sw) const {
+        for (int f = f0; f < f1; ++f) {
+            const int cn = st->neg[f], cp = st->pos[f];
+            const float3 d = delta(cn, cp);
+            const float len = sqrtf(d.x * d.x + d.y * d.y + d.z * d.z);
+            if (!(len >= 1e-15f)) continue;
+            const float dir[3] = {d.x / len, d.y / len, d.z / len};
+            const float mdir[3] = {-dir[0], -dir[1], -dir[2]};
+            const float fproj = 0.5f * (proj(cn, dir) + proj(cp, dir));
+            int nc4[4], pc4[4];
+            float nx4[4], px4[4];
+            const int nn = walk(cn, fproj, dir, mdir, nc4, nx4);
+            const int np = walk(cp, fproj, dir, dir, pc4, px4);
+            int* c8 = sc + 8 * (size_t)f;
+            float* w8 = sw + 8 * (size_t)f;
+            if (nn == 4 && np == 4) {
+                float xs[8], w[8];
+                for (int k = 0; k < 4; ++k) {
+                    xs[k] = nx4[3 - k];
+                    xs[4 + k] = px4[k];
+                }
+                fornberg_ref(xs, 8, 0.0f, w);
+                for (int k = 0; k < 4; ++k) {
+                    c8[k] = nc4[k];
+                    w8[k] = w[3 - k];
+                    c8[4 + k] = pc4[k];
+                    w8[4 + k] = w[4 + k];
+                }
+            } else if (nn + np >= 2) {
+                c8[0] = cn;
+                w8[0] = -st->inv_d[f];
+                c8[4] = cp;
+                w8[4] = st->inv_d[f];
+            }
+        }
+    }
+};
+
 int validate(int nc, const double* cc, const double* vol, const double* ov, const uint8_t* ct,
              int nf, const int32_t* fn, const int32_t* fp, const double* fa, const double* fo,
              const double* fd, double c, double rho, double min_dx) {
@@ -406,7 +610,7 @@ void destroy_state(AcousticAMRState* st) {
                     st->d_csr,  st->d_w,    st->d_cent, st->d_status, st->d_best};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
-    void* more[] = {st->d_pw[0], st->d_pw[1], st->d_pw[2], st->d_ext, st->d_pint,
+    void* more[] = {st->d_sc, st->d_sw, st->d_pw[0], st->d_pw[1], st->d_pw[2], st->d_ext, st->d_pint,
                     st->d_pflt, st->d_mint, st->d_mflt, st->d_mout, st->d_tmp};
     for (void* b : more)
         if (b) (void)hipFree(b);
@@ -478,8 +682,13 @@ int step_impl(AcousticAMRState* st, int n, float* ms_total, float* ms_faces, flo
     k_amr_status_init<<<1, 1, 0, st->stream>>>(st->d_status, st->d_best);
     for (int s = 0; s < n; ++s) {
         if (ms_total) (void)hipEventRecord(ev[2 * (size_t)s], st->stream);
-        k_amr_faces<<<(st->nf + kFaceTile - 1) / kFaceTile, 256, 0, st->stream>>>(
-            st->d_face, st->d_v, st->d_p, st->nf, dt_over_rho, st->d_status, s);
+        if (st->order == 8)
+            k_amr_faces8<<<blocks(st->nf), 256, 0, st->stream>>>(
+                st->d_sc, st->d_sw, st->d_face, st->d_v, st->d_p, st->nf, dt_over_rho,
+                st->d_status, s);
+        else
+            k_amr_faces<<<(st->nf + kFaceTile - 1) / kFaceTile, 256, 0, st->stream>>>(
+                st->d_face, st->d_v, st->d_p, st->nf, dt_over_rho, st->d_status, s);
         if (st->plate_on && st->n_bc > 0) {                      // :1157
             const float* w = st->d_pw[(st->prot + s + 1) % 3];
             const float* wp = st->d_pw[(st->prot + s) % 3];
@@ -565,18 +774,13 @@ SK_API AcousticAMRState* amr_create(int nc, const double* cc, const double* vol,
         set_stage("validate_inputs", "gradient_order must be 2 or 8");
         return nullptr;
     }
-    if (order == 8) {
-        set_stage("validate_inputs",
-                  "gradient_order 8 (Fornberg stencil) is not provided by this library");
-        return nullptr;
-    }
     auto* st = new AcousticAMRState();
     st->nc = nc;
     st->nf = nf;
     st->order = order;
     st->c = c;
     st->rho = rho;
-    st->dt = 0.77 * min_dx / (c * sqrt(3.0));                        // :744-745
+    st->dt = (order == 8 ? 0.10 : 0.77) * min_dx / (c * sqrt(3.0));  // :744-745
     st->centers.assign(cc, cc + 3 * (size_t)nc);
     st->open_vol.assign(ov, ov + nc);
     st->type.assign(ct, ct + nc);
@@ -617,6 +821,26 @@ SK_API AcousticAMRState* amr_create(int nc, const double* cc, const double* vol,
             csr[start[fp[f]] + fill[fp[f]]++] = ~f;          // weight -ffc
         }
     }
+    std::vector<int> sc8, csr_copy;
+    std::vector<float> sw8;
+    if (order == 8) {                                                // :862-869
+        set_stage("stencil.build", "");
+        sc8.assign(8 * (size_t)nf, 0);
+        sw8.assign(8 * (size_t)nf, 0.0f);
+        StencilBuilder sb{st, start, csr, ffc, {}};
+        sb.build_neighbours();
+        g_total_faces = nf;
+        const int nt = std::max(1, std::min(64, (int)std::thread::hardware_concurrency()));
+        std::vector<std::thread> pool;
+        for (int t = 0; t < (nf < 256 ? 1 : nt); ++t) {
+            const int n = nf < 256 ? 1 : nt;
+            const int f0 = (int)((int64_t)nf * t / n), f1 = (int)((int64_t)nf * (t + 1) / n);
+            pool.emplace_back([&sb, f0, f1, &sc8, &sw8] { sb.faces(f0, f1, sc8.data(), sw8.data()); });
+        }
+        for (auto& th : pool) th.join();
+        g_done_faces = nf;
+        set_stage("ok", "");
+    }
     st->pdamp.assign(nc, 1.0f);                                      // :852-859
     st->psrc.assign(nc, 1.0f);
     st->vdamp.assign(nf, 1.0f);
@@ -633,6 +857,10 @@ SK_API AcousticAMRState* amr_create(int nc, const double* cc, const double* vol,
     al((void**)&st->d_csr, sizeof(int) * (size_t)st->ne);
     al((void**)&st->d_w, sizeof(float) * (size_t)nf);
     al((void**)&st->d_cent, sizeof(double) * 3 * (size_t)nc);
+    if (order == 8) {
+        al((void**)&st->d_sc, sizeof(int) * 8 * (size_t)nf);
+        al((void**)&st->d_sw, sizeof(float) * 8 * (size_t)nf);
+    }
     al((void**)&st->d_status, sizeof(int) * 2);
     al((void**)&st->d_best, sizeof(unsigned long long));
     if (e == hipSuccess) e = hipMemsetAsync(st->d_v, 0, sizeof(float) * (size_t)nf, st->stream);
@@ -649,6 +877,12 @@ SK_API AcousticAMRState* amr_create(int nc, const double* cc, const double* vol,
     if (e == hipSuccess)
         e = hipMemcpyAsync(st->d_cent, cc, sizeof(double) * 3 * (size_t)nc,
                            hipMemcpyHostToDevice, st->stream);
+    if (e == hipSuccess && order == 8)
+        e = hipMemcpyAsync(st->d_sc, sc8.data(), sizeof(int) * 8 * (size_t)nf,
+                           hipMemcpyHostToDevice, st->stream);
+    if (e == hipSuccess && order == 8)
+        e = hipMemcpyAsync(st->d_sw, sw8.data(), sizeof(float) * 8 * (size_t)nf,
+                           hipMemcpyHostToDevice, st->stream);
     if (e == hipSuccess) e = upload_coeffs(st);
     if (e != hipSuccess) {
         set_stage("device_alloc", hipGetErrorString(e));
@@ -660,9 +894,9 @@ SK_API AcousticAMRState* amr_create(int nc, const double* cc, const double* vol,
 
 SK_API const char* amr_get_last_create_error_stage(void) { return g_amr_stage.c_str(); }
 SK_API const char* amr_get_last_create_error_message(void) { return g_amr_msg.c_str(); }
-SK_API int amr_get_create_progress_active(void) { return 0; }
-SK_API int amr_get_create_progress_done_faces(void) { return 0; }
-SK_API int amr_get_create_progress_total_faces(void) { return 0; }
+SK_API int amr_get_create_progress_active(void) { return 0; }   // the build is synchronous
+SK_API int amr_get_create_progress_done_faces(void) { return g_done_faces.load(); }
+SK_API int amr_get_create_progress_total_faces(void) { return g_total_faces.load(); }
 
 SK_API void amr_destroy(AcousticAMRState* st) { destroy_state(st); }
 

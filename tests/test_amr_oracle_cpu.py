@@ -260,7 +260,7 @@ def test_stability_guard_and_validation():
     with pytest.raises(RuntimeError):
         OracleAMR(**t)
     with pytest.raises(RuntimeError):
-        OracleAMR.from_grid(g, gradient_order=8)         # order 8 not restated
+        OracleAMR.from_grid(g, gradient_order=4)         # only orders 2 and 8
 
 
 # ── C ABI without a GPU ─────────────────────────────────────────────────
@@ -309,8 +309,8 @@ def test_amr_abi_argument_checks_without_device():
     assert not h and last_create_error()[0] == "validate_inputs"
     k[8][5] = 0.01
     h = L.amr_create(400, *[_p(a) for a in k[:4]], k[4].size, *[_p(a) for a in k[4:]],
-                     343.0, 1.21, 0.01, 8)
-    assert not h and "gradient_order 8" in last_create_error()[1]
+                     343.0, 1.21, 0.01, 4)
+    assert not h and last_create_error()[1] == "gradient_order must be 2 or 8"
 
 
 # ── Plate coupling and mic samplers (acoustic_amr.cpp:939-1072, :1412-1792) ──
@@ -507,3 +507,116 @@ def test_mic_samplers_brute_force():
         if fs.any():
             want = (nrm[fs] * (aw[fs] * v[fs])[:, None]).sum(0) / aw[fs].sum()
             assert np.allclose([vx[m], vy[m], vz[m]], want, rtol=1e-4, atol=1e-6)
+
+
+# ── Gradient order 8: the Fornberg stencil (acoustic_amr.cpp:313-679) ─────
+#
+# The reference's fornberg_d1 (acoustic_amr.cpp:398-426) updates the new
+# node's weights with (x[n] - xi) where Fornberg's recurrence uses
+# (x[n-1] - xi).  For two nodes the result is still the exact difference;
+# for three or more the weights are not first-derivative weights (on the
+# symmetric 8-point stencil they annihilate x as well as 1).  The oracle and
+# the GPU path reproduce the reference's weights as written; their values
+# beyond two nodes are "parity unpinned": only the properties below pin them.
+
+
+def test_fornberg_reference_recurrence_properties():
+    from oracle.amr_oracle import fornberg_d1
+    rng = np.random.default_rng(16)
+    for _ in range(5):                      # two nodes: the exact difference
+        x = np.sort(rng.uniform(-4.0, 4.0, 2)).astype(np.float32)
+        w = fornberg_d1(x)
+        assert w[1] == pytest.approx(1.0 / (float(x[1]) - float(x[0])), rel=1e-6)
+        assert w[0] == -w[1]
+    for n in (3, 5, 8):                     # constants are annihilated
+        x = np.sort(rng.uniform(-4.0, 4.0, n)).astype(np.float32)
+        w = fornberg_d1(x).astype(np.float64)
+        assert abs(w.sum()) < 1e-5 * np.abs(w).sum()
+    xs = np.float32([-3.5, -2.5, -1.5, -0.5, 0.5, 1.5, 2.5, 3.5])
+    w = fornberg_d1(xs).astype(np.float64)
+    assert np.allclose(w, -w[::-1], rtol=1e-5, atol=1e-9)   # odd on symmetric nodes
+    # the defect itself: not the 8th-order weights, and blind to a linear field
+    assert not np.allclose(w[4:], C8, rtol=0.1)
+    assert abs(np.sum(w * xs)) < 1e-6
+
+
+# 8th-order staggered first-derivative coefficients (closed form)
+C8 = np.array([1225 / 1024, -245 / 3072, 49 / 5120, -5 / 7168])
+
+
+def _box(nx, ny, nz, dx=0.01, rot=0.0):
+    from afdtd.amr import AcousticAMRGrid
+    g = AcousticAMRGrid.build_uniform_box(x_min=0, x_max=nx * dx, y_min=0, y_max=ny * dx,
+                                          z_min=0, z_max=nz * dx, dx=dx)
+    if rot:
+        c, s_ = np.cos(rot), np.sin(rot)
+        cc = g.cell_centers.copy()
+        cc[:, 0], cc[:, 1] = c * g.cell_centers[:, 0] - s_ * g.cell_centers[:, 1], \
+            s_ * g.cell_centers[:, 0] + c * g.cell_centers[:, 1]
+        g = type(g)(**{**g.__dict__, "cell_centers": cc})
+    return g
+
+
+@pytest.mark.parametrize("rot", [0.0, np.pi / 4])
+def test_order8_stencil_closed_form_on_box(rot):
+    """Interior faces of a box grid (also rotated by 45 degrees, which takes
+    the CSR-search walk instead of the axis tables) take the 4 + 4 cells
+    along the face normal, nearest first on each side, weighted by the
+    reference recurrence over their offsets; faces with fewer than 4 cells
+    on a side fall back to -1/d, +1/d."""
+    from oracle.amr_oracle import fornberg_d1
+    nx, ny, nz, dx = 11, 10, 9, 0.01
+    g = _box(nx, ny, nz, dx, rot)
+    o = OracleAMR.from_grid(g, gradient_order=8)
+    assert o.dt == pytest.approx(0.10 * dx / (343.0 * np.sqrt(3.0)), rel=1e-14)
+    sc, sw = o.get_stencil()
+    cid = np.arange(nx * ny * nz).reshape(nx, ny, nz)
+    negs = [cid[:-1].ravel(), cid[:, :-1].ravel(), cid[:, :, :-1].ravel()]
+    strides = (ny * nz, nz, 1)
+    dims = (nx, ny, nz)
+    f = 0
+    n_full = n_fb = 0
+    for ax in range(3):
+        for c0 in negs[ax]:
+            ijk = np.unravel_index(c0, dims)
+            lo, hi = ijk[ax], dims[ax] - 2 - ijk[ax]      # cells beyond each side
+            if lo >= 3 and hi >= 3:
+                cells = c0 + strides[ax] * np.array([0, -1, -2, -3, 1, 2, 3, 4])
+                assert np.array_equal(sc[f], cells)
+                # weights of the reference recurrence over the node offsets
+                # along the normal, farthest neg node first
+                want = fornberg_d1(np.float32([-3.5, -2.5, -1.5, -0.5, 0.5, 1.5, 2.5, 3.5])
+                                   * np.float32(dx))
+                want = np.concatenate([want[3::-1], want[4:]])
+                assert np.allclose(sw[f], want, rtol=2e-3, atol=1e-3 * np.abs(want).max())
+                n_full += 1
+            else:
+                assert sc[f, 0] == c0 and sc[f, 4] == c0 + strides[ax]
+                assert sw[f, 0] == -np.float32(1 / np.float32(dx)) or \
+                    sw[f, 0] == -(np.float32(1.0) / np.float32(dx))
+                assert sw[f, 4] == -sw[f, 0] and not np.any(sw[f, [1, 2, 3, 5, 6, 7]])
+                n_fb += 1
+            f += 1
+    assert n_full > 100 and n_fb > 100
+
+
+def test_order8_stops_at_walls_and_conserves_flux():
+    """A wall cell ends a stencil walk (two-point fallback next to it), and
+    the order-8 step keeps sum(p*V) (the divergence is the same telescoping
+    sum)."""
+    g = _box(14, 6, 6)
+    ct = g.cell_types.copy()
+    wall = (7 * 6 + 3) * 6 + 3                             # cell (7, 3, 3)
+    ct[wall] = 1
+    g2 = type(g)(**{**g.__dict__, "cell_types": ct})
+    o = OracleAMR.from_grid(g2, gradient_order=8)
+    sc, sw = o.get_stencil()
+    xf = np.flatnonzero((g.face_cell_neg == (5 * 6 + 3) * 6 + 3) & (g.face_axis == 0))[0]
+    assert not np.any(sw[xf, [1, 2, 3, 5, 6, 7]])         # face (5|6): walk hits the wall at 7
+    rng = np.random.default_rng(17)
+    o2 = OracleAMR.from_grid(g, gradient_order=8)
+    o2.set_state(rng.standard_normal(g.n_cells).astype(np.float32))
+    m0 = float(np.sum(o2.get_pressure().astype(np.float64)))
+    sc0 = float(np.sum(np.abs(o2.get_pressure()).astype(np.float64)))
+    o2.step(300)
+    assert abs(float(np.sum(o2.get_pressure().astype(np.float64))) - m0) < 1e-5 * sc0

@@ -236,3 +236,30 @@ def test_mic_samplers_bitwise():
         m.setup_mic_samplers(MIC_POS[:2], axis[:2], False)   # a second set-up appends
     for a, b in zip(g.sample_mics(), o.sample_mics()):
         assert np.array_equal(_bits(a), _bits(b))
+
+
+@pytest.mark.parametrize("rot", [0.0, np.pi / 4])
+def test_order8_bitwise(rot):
+    """Gradient order 8: the per-face 8-cell stencil built on the host (axis
+    tables on the box, the CSR-search walk on the rotated box) and the
+    order-8 face kernel equal the oracle bit for bit."""
+    from test_amr_oracle_cpu import _box
+    print(f"amr: order 8 rot={rot:.3f}", flush=True)
+    grid = _box(20, 16, 24, rot=rot)
+    if not rot:
+        grid = grid.with_pml_shell(4)
+    args = (grid.cell_centers, grid.cell_volumes, grid.open_volume_fraction, grid.cell_types,
+            grid.face_cell_neg, grid.face_cell_pos, grid.face_area, grid.face_open_fraction,
+            grid.face_distance, 343.0, 1.21, grid.min_dx, 8)
+    from afdtd.amr import AcousticAMR, lib
+    g, o = AcousticAMR(*args), OracleAMR(*args)
+    assert g.dt == o.dt
+    assert lib().amr_get_create_progress_total_faces() == grid.n_faces
+    rng = np.random.default_rng(43)
+    p = rng.standard_normal(grid.n_cells).astype(np.float32)
+    for m in (g, o):
+        if not rot:
+            m.set_border_condition(0, 3.0, 0.0, 0.0, 4, grid.bounds_min, grid.bounds_max)
+        m.set_state(p)
+        m.step(120)
+    _same(g, o)
