@@ -348,7 +348,9 @@ def bench_amr(a):
             grid.face_cell_neg, grid.face_cell_pos, grid.face_area, grid.face_open_fraction,
             grid.face_distance, 343.0, 1.21, dx, order)
     g = AcousticAMR(*args)
+    g.record_layout = a.amr_layout
     g.set_border_condition(0, 3.0, 0.0, 0.0, n_pml, grid.bounds_min, grid.bounds_max)
+    layout = g.record_layout
     g.inject_pressure_nearest([1.0, 1.2, 1.4], 1.0)
     steps = a.steps if a.steps != 4000 else 1000
     warm = min(a.warmup, 20)
@@ -370,18 +372,23 @@ def bench_amr(a):
         raise SystemExit("non-finite AMR pressure")
     C_, F, E = g.n_cells, g.n_faces, g.n_csr_entries
     # algorithmic HBM bytes per step (each array touched once):
-    #   faces kernel: record 16 + v read 4 + v write 4 per face, p read 4 per cell;
-    #                 order 8 adds the 8-cell stencil (32 B cells + 32 B weights)
-    #   cells kernel: record 16 + row start 4 + p read/write 8 per cell,
-    #                 CSR index 4 per entry, v and face weight 4 + 4 per face
-    b_faces = (24 + (64 if order == 8 else 0)) * F + 4 * C_
-    b_cells = 28 * C_ + 4 * E + 8 * F
+    #   faces kernel: v read 4 + v write 4 per face, p read 4 per cell, plus the
+    #                 face record: direct 16 B (order 8: + 64 B of stencil),
+    #                 compact 8 B {neg, class} (class tables stay in cache)
+    #   cells kernel: row start 4 + p read/write 8 per cell, CSR index 4 per
+    #                 entry, v and face weight 4 + 4 per face, plus the cell
+    #                 record: direct 16 B, compact a 4 B class id
+    rec_f = 8 if layout == "compact" else (16 + (64 if order == 8 else 0))
+    rec_c = 4 if layout == "compact" else 16
+    b_faces = (8 + rec_f) * F + 4 * C_
+    b_cells = (12 + rec_c) * C_ + 4 * E + 8 * F
     fa = b_faces / (ms_f / steps * 1e-3) / 1e9
     ca = b_cells / (ms_c / steps * 1e-3) / 1e9
     dom = "cells" if ms_c >= ms_f else "faces"
     # HBM bytes per k_amr_cells launch counted with rocprofv3 on this workload
     amr_traffic = None
-    prof = os.path.join(ROOT, "profiles", f"amr_uniform_box_256cube_order{order}_anechoic8.txt")
+    prof = os.path.join(ROOT, "profiles",
+                        f"amr_uniform_box_256cube_order{order}_anechoic8_{layout}.txt")
     if os.path.exists(prof):
         for ln in open(prof):
             if ln.startswith("traffic_bytes_per_launch:"):
@@ -393,7 +400,7 @@ def bench_amr(a):
             "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
             "data": "synthetic (uniform box grid, one pulse)",
             "config": {"workload": f"amr-uniform-box-{n}x{n}x{n}-order{order}-anechoic{n_pml}",
-                       "cells": C_, "faces": F, "csr_entries": E},
+                       "cells": C_, "faces": F, "csr_entries": E, "record_layout": layout},
             "kernel_ms_per_step": {"faces": ms_f / steps, "cells": ms_c / steps},
             "event_ms_total": ms_t,
             "roofline": {"bound": "hbm", "kernel": f"k_amr_{dom}", "achieved": ach,
@@ -439,6 +446,8 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--amr-order", type=int, default=2, choices=(2, 8),
                     help="amr: gradient order of the AMR stepper")
+    ap.add_argument("--amr-layout", default="auto", choices=("auto", "direct", "compact"),
+                    help="amr: record layout of the step kernels")
     ap.add_argument("--no-fd", action="store_true",
                     help="adjoint: float32 timing only (no float64 pass, no finite differences)")
     a = ap.parse_args()

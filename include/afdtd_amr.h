@@ -201,6 +201,17 @@ SK_API int amr_step_timed(AcousticAMRState* st, int n_steps, float* ms_total,
 /* Number of CSR (cell, face) entries = 2 * n_faces; for byte accounting. */
 SK_API int amr_get_csr_entries(const AcousticAMRState* st);
 
+/* Record layout of the step kernels.  Direct: every face and cell streams its
+ * full record (16 B per face and cell; order 8 adds 64 B of stencil per
+ * face).  Compact: each face streams {neg cell, class} (8 B) and each cell a
+ * class id (4 B) into tables of the distinct records, bit for bit the same
+ * values.  layout 0 = automatic (compact while each table has at most 65536
+ * records), 1 = direct, 2 = compact up to 2^24 records.  Rebuilt on every
+ * coefficient change (border condition).  amr_get_record_layout returns the
+ * layout in use (1 direct, 2 compact). */
+SK_API int amr_set_record_layout(AcousticAMRState* st, int layout);
+SK_API int amr_get_record_layout(const AcousticAMRState* st);
+
 #ifdef __cplusplus
 }
 #endif

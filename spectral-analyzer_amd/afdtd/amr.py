@@ -46,6 +46,8 @@ _SIGS = {
     "amr_get_dt": (_d, [_vp]),
     "amr_get_step_count": (_i, [_vp]),
     "amr_get_csr_entries": (_i, [_vp]),
+    "amr_set_record_layout": (_i, [_vp, _i]),
+    "amr_get_record_layout": (_i, [_vp]),
     "amr_scatter_pressure_uniform": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp]),
     "amr_set_border_condition": (_i, [_vp, _i, _f, _f, _f, _i, _vp, _vp]),
     "amr_setup_plate": (_i, [_vp, _i, _i, _f, _vp, _vp, _f, _f, _f, _f, _i] + [_vp] * 9),
@@ -301,6 +303,16 @@ class AcousticAMR:
     @property
     def n_csr_entries(self) -> int:
         return lib().amr_get_csr_entries(self._h)
+
+    @property
+    def record_layout(self) -> str:
+        return {1: "direct", 2: "compact"}.get(lib().amr_get_record_layout(self._h), "?")
+
+    @record_layout.setter
+    def record_layout(self, layout: str) -> None:
+        code = {"auto": 0, "direct": 1, "compact": 2}[layout]
+        if lib().amr_set_record_layout(self._h, code) != 0:
+            raise RuntimeError("amr_set_record_layout failed")
 
 
 def last_create_error() -> tuple[str, str]:

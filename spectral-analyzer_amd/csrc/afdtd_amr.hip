@@ -27,8 +27,10 @@
 #include <string.h>
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <thread>
+#include <unordered_map>
 #include <string>
 #include <vector>
 
@@ -53,6 +55,18 @@ struct AcousticAMRState {
     double* d_cent = nullptr;
     int4* d_sc = nullptr;                    // order 8: 8 cells per face (two int4)
     float4* d_sw = nullptr;                  // order 8: 8 weights per face (two float4)
+    std::vector<int> sc8;                    // order 8 stencils, kept for record rebuilds
+    std::vector<float> sw8;
+    // Compact records: each face stores {neg cell, class} and each cell a
+    // class id into small tables of the distinct records (a box grid has a
+    // handful), cutting the streamed bytes; used when the tables stay small.
+    int layout_req = 0;                      // 0 automatic, 1 direct, 2 compact
+    bool compact = false;
+    int n_fclass = 0, n_cclass = 0;
+    int2* d_fc = nullptr;                    // per face {neg, class}
+    int4* d_fcls = nullptr;                  // order 2: {pos-neg, 1/d, Vdamp, 0}; order 8: 5 per class
+    int* d_cc = nullptr;                     // per cell class
+    float4* d_ccls = nullptr;                // {P_damp, 1/(V open), P_src, kind}
     int* d_status = nullptr;                 // [0] failing step, [1] nearest cell
     unsigned long long* d_best = nullptr;    // nearest squared distance (bits)
     // host copies (set-up inputs, coefficient rebuilds, getters)
@@ -161,6 +175,61 @@ __global__ __launch_bounds__(256) void k_amr_faces8(const int4* __restrict__ sc,
     v[f] = (v[f] - dt_over_rho * g) * vd;
 }
 
+// Compact-record variants of the face kernels: a face stores {neg, class};
+// the class table (a few hundred bytes on a box grid, cache resident)
+// holds the rest of the record, bit for bit the direct values.
+__global__ __launch_bounds__(256) void k_amr_faces_c(const int2* __restrict__ fc,
+                                                     const int4* __restrict__ cls,
+                                                     float* __restrict__ v,
+                                                     const float* __restrict__ p, int nf,
+                                                     float dt_over_rho,
+                                                     const int* __restrict__ status, int s) {
+    if (status[0] < s) return;
+    const int f0 = xcd_tile(gridDim.x) * kFaceTile + threadIdx.x;
+    const int f1 = f0 + 256;
+    int2 r0 = make_int2(0, 0), r1 = r0;
+    float v0 = 0.0f, v1 = 0.0f;
+    if (f0 < nf) { r0 = fc[f0]; v0 = v[f0]; }
+    if (f1 < nf) { r1 = fc[f1]; v1 = v[f1]; }
+    const int4 c0 = cls[r0.y], c1 = cls[r1.y];
+    // lanes past the last face read cell 0 only (class 0's offset may be negative)
+    const int q0 = f0 < nf ? r0.x + c0.x : 0, q1 = f1 < nf ? r1.x + c1.x : 0;
+    const float a0 = p[q0], b0 = p[r0.x], a1 = p[q1], b1 = p[r1.x];
+    if (f0 < nf) v[f0] = (v0 - dt_over_rho * ((a0 - b0) * __int_as_float(c0.y))) * __int_as_float(c0.z);
+    if (f1 < nf) v[f1] = (v1 - dt_over_rho * ((a1 - b1) * __int_as_float(c1.y))) * __int_as_float(c1.z);
+}
+
+__device__ inline int stencil_cell(int neg, int d) { return d == (int)0x80000000 ? 0 : neg + d; }
+
+__global__ __launch_bounds__(256) void k_amr_faces8_c(const int2* __restrict__ fc,
+                                                      const int4* __restrict__ cls,
+                                                      float* __restrict__ v,
+                                                      const float* __restrict__ p, int nf,
+                                                      float dt_over_rho,
+                                                      const int* __restrict__ status, int s) {
+    if (status[0] < s) return;
+    const int f = xcd_tile(gridDim.x) * 256 + threadIdx.x;
+    if (f >= nf) return;
+    const int2 r = fc[f];
+    const int4* c = cls + 5 * (size_t)r.y;
+    const int4 a = c[0], b = c[1], wa = c[2], wb = c[3];
+    const float vd = __int_as_float(c[4].x);
+    const int n = r.x;
+    const float pa[8] = {p[stencil_cell(n, a.x)], p[stencil_cell(n, a.y)], p[stencil_cell(n, a.z)],
+                         p[stencil_cell(n, a.w)], p[stencil_cell(n, b.x)], p[stencil_cell(n, b.y)],
+                         p[stencil_cell(n, b.z)], p[stencil_cell(n, b.w)]};
+    float g = 0.0f;
+    g += __int_as_float(wa.x) * pa[0];
+    g += __int_as_float(wa.y) * pa[1];
+    g += __int_as_float(wa.z) * pa[2];
+    g += __int_as_float(wa.w) * pa[3];
+    g += __int_as_float(wb.x) * pa[4];
+    g += __int_as_float(wb.y) * pa[5];
+    g += __int_as_float(wb.z) * pa[6];
+    g += __int_as_float(wb.w) * pa[7];
+    v[f] = (v[f] - dt_over_rho * g) * vd;
+}
+
 // One 256-thread block updates 256 consecutive cells.  The block's CSR rows
 // are one contiguous entry range; when it fits kCsrStage entries it is first
 // copied into LDS with coalesced loads, so each lane's walk over its own row
@@ -169,7 +238,9 @@ __global__ __launch_bounds__(256) void k_amr_faces8(const int4* __restrict__ sc,
 // exact, so +-w[f] equals the reference's pre-signed CSR weight.
 constexpr int kCsrStage = 2048;
 
+template <bool COMPACT>
 __global__ __launch_bounds__(256) void k_amr_cells(const float4* __restrict__ cell,
+                                                   const int* __restrict__ cell_class,
                                                    const int* __restrict__ start,
                                                    const int* __restrict__ csr,
                                                    const float* __restrict__ w,
@@ -189,7 +260,7 @@ __global__ __launch_bounds__(256) void k_amr_cells(const float4* __restrict__ ce
     }
     bool bad = false;
     if (c < nc) {
-        const float4 m = cell[c];
+        const float4 m = COMPACT ? cell[cell_class[c]] : cell[c];
         const int k0 = start[c], k1 = start[c + 1];
         float d = 0.0f;
         if (staged) {
@@ -610,12 +681,133 @@ void destroy_state(AcousticAMRState* st) {
                     st->d_csr,  st->d_w,    st->d_cent, st->d_status, st->d_best};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
+    void* comp[] = {st->d_fc, st->d_fcls, st->d_cc, st->d_ccls};
+    for (void* b : comp)
+        if (b) (void)hipFree(b);
     void* more[] = {st->d_sc, st->d_sw, st->d_pw[0], st->d_pw[1], st->d_pw[2], st->d_ext, st->d_pint,
                     st->d_pflt, st->d_mint, st->d_mflt, st->d_mout, st->d_tmp};
     for (void* b : more)
         if (b) (void)hipFree(b);
     if (st->stream) (void)hipStreamDestroy(st->stream);
     delete st;
+}
+
+// Distinct-record tables.  Keys are the records' exact bit patterns, so a
+// class holds exactly the values the direct record would.
+template <int K>
+struct KeyHash {
+    size_t operator()(const std::array<uint32_t, K>& k) const {
+        uint64_t h = 1469598103934665603ull;
+        for (uint32_t x : k) h = (h ^ x) * 1099511628211ull;
+        return (size_t)(h ^ (h >> 29));
+    }
+};
+
+template <int K>
+struct ClassTable {
+    std::unordered_map<std::array<uint32_t, K>, int, KeyHash<K>> ids;
+    std::vector<std::array<uint32_t, K>> rows;
+    int limit;
+    explicit ClassTable(int lim) : limit(lim) {}
+    int id(const std::array<uint32_t, K>& k) {       // -1 once over the limit
+        auto it = ids.find(k);
+        if (it != ids.end()) return it->second;
+        if ((int)rows.size() >= limit) return -1;
+        ids.emplace(k, (int)rows.size());
+        rows.push_back(k);
+        return (int)rows.size() - 1;
+    }
+};
+
+constexpr int kMaxClasses = 1 << 16;
+constexpr uint32_t kAbsZero = 0x80000000u;   // order-8 slot that names cell 0 itself
+
+inline uint32_t bits(float x) {
+    uint32_t u;
+    memcpy(&u, &x, 4);
+    return u;
+}
+
+template <class T>
+hipError_t upload_vec(T** dst, const std::vector<T>& v, hipStream_t s) {
+    if (*dst) (void)hipFree(*dst);
+    *dst = nullptr;
+    hipError_t e = hipMalloc((void**)dst, sizeof(T) * (v.size() + 1));
+    if (e == hipSuccess && !v.empty())
+        e = hipMemcpyAsync(*dst, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, s);
+    return e;
+}
+
+// Build the compact records from the direct ones; falls back to the direct
+// layout when requested or when a table would exceed kMaxClasses.
+hipError_t build_compact(AcousticAMRState* st, const std::vector<int4>& face,
+                         const std::vector<float4>& cell) {
+    st->compact = false;
+    if (st->layout_req == 1) return hipSuccess;
+    const int lim = st->layout_req == 2 ? (1 << 24) : kMaxClasses;
+    std::vector<int2> fc(st->nf);
+    std::vector<int4> fcls;
+    bool ok = true;
+    if (st->order == 8) {
+        ClassTable<17> t(lim);
+        for (int f = 0; f < st->nf && ok; ++f) {
+            std::array<uint32_t, 17> k;
+            for (int q = 0; q < 8; ++q) {
+                const int c = st->sc8[8 * (size_t)f + q];
+                k[q] = c == 0 ? kAbsZero : (uint32_t)(c - st->neg[f]);
+                k[8 + q] = bits(st->sw8[8 * (size_t)f + q]);
+            }
+            k[16] = (uint32_t)face[f].w;
+            const int id = t.id(k);
+            ok = id >= 0;
+            fc[f] = make_int2(st->neg[f], id);
+        }
+        if (ok)
+            for (auto& r : t.rows) {
+                fcls.push_back(make_int4(r[0], r[1], r[2], r[3]));
+                fcls.push_back(make_int4(r[4], r[5], r[6], r[7]));
+                fcls.push_back(make_int4(r[8], r[9], r[10], r[11]));
+                fcls.push_back(make_int4(r[12], r[13], r[14], r[15]));
+                fcls.push_back(make_int4(r[16], 0, 0, 0));
+            }
+        st->n_fclass = (int)t.rows.size();
+    } else {
+        ClassTable<3> t(lim);
+        for (int f = 0; f < st->nf && ok; ++f) {
+            const int4 r = face[f];
+            const int id = t.id({(uint32_t)(r.y - r.x), (uint32_t)r.z, (uint32_t)r.w});
+            ok = id >= 0;
+            fc[f] = make_int2(r.x, id);
+        }
+        if (ok)
+            for (auto& r : t.rows) fcls.push_back(make_int4(r[0], r[1], r[2], 0));
+        st->n_fclass = (int)t.rows.size();
+    }
+    std::vector<int> cc(st->nc);
+    std::vector<float4> ccls;
+    if (ok) {
+        ClassTable<4> t(lim);
+        for (int i = 0; i < st->nc && ok; ++i) {
+            const float4 m = cell[i];
+            cc[i] = t.id({bits(m.x), bits(m.y), bits(m.z), bits(m.w)});
+            ok = cc[i] >= 0;
+        }
+        if (ok)
+            for (auto& r : t.rows) {
+                float4 m;
+                memcpy(&m, r.data(), 16);
+                ccls.push_back(m);
+            }
+        st->n_cclass = (int)t.rows.size();
+    }
+    if (!ok) return hipSuccess;                      // too many distinct records: direct
+    hipError_t e = upload_vec(&st->d_fc, fc, st->stream);
+    if (e == hipSuccess) e = upload_vec(&st->d_fcls, fcls, st->stream);
+    if (e == hipSuccess) e = upload_vec(&st->d_cc, cc, st->stream);
+    if (e == hipSuccess) e = upload_vec(&st->d_ccls, ccls, st->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(st->stream);
+    st->compact = e == hipSuccess;
+    return e;
 }
 
 // Pack the per-cell and per-face coefficient records and upload them.
@@ -640,6 +832,7 @@ hipError_t upload_coeffs(AcousticAMRState* st) {
         e = hipMemcpyAsync(st->d_face, face.data(), sizeof(int4) * st->nf, hipMemcpyHostToDevice,
                            st->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(st->stream);
+    if (e == hipSuccess) e = build_compact(st, face, cell);
     return e;
 }
 
@@ -682,10 +875,16 @@ int step_impl(AcousticAMRState* st, int n, float* ms_total, float* ms_faces, flo
     k_amr_status_init<<<1, 1, 0, st->stream>>>(st->d_status, st->d_best);
     for (int s = 0; s < n; ++s) {
         if (ms_total) (void)hipEventRecord(ev[2 * (size_t)s], st->stream);
-        if (st->order == 8)
+        if (st->order == 8 && st->compact)
+            k_amr_faces8_c<<<blocks(st->nf), 256, 0, st->stream>>>(
+                st->d_fc, st->d_fcls, st->d_v, st->d_p, st->nf, dt_over_rho, st->d_status, s);
+        else if (st->order == 8)
             k_amr_faces8<<<blocks(st->nf), 256, 0, st->stream>>>(
                 st->d_sc, st->d_sw, st->d_face, st->d_v, st->d_p, st->nf, dt_over_rho,
                 st->d_status, s);
+        else if (st->compact)
+            k_amr_faces_c<<<(st->nf + kFaceTile - 1) / kFaceTile, 256, 0, st->stream>>>(
+                st->d_fc, st->d_fcls, st->d_v, st->d_p, st->nf, dt_over_rho, st->d_status, s);
         else
             k_amr_faces<<<(st->nf + kFaceTile - 1) / kFaceTile, 256, 0, st->stream>>>(
                 st->d_face, st->d_v, st->d_p, st->nf, dt_over_rho, st->d_status, s);
@@ -698,9 +897,14 @@ int step_impl(AcousticAMRState* st, int n, float* ms_total, float* ms_faces, flo
         }
         if (ms_total) (void)hipEventRecord(ev[2 * (size_t)s + 1], st->stream);
         const int check_max = ((st->step_count + s) & 0xFF) == 0;
-        k_amr_cells<<<blocks(st->nc), 256, 0, st->stream>>>(st->d_cell, st->d_start, st->d_csr,
-                                                          st->d_w, st->d_v, st->d_p, st->nc, bulk,
-                                                          check_max, st->d_status, s);
+        if (st->compact)
+            k_amr_cells<true><<<blocks(st->nc), 256, 0, st->stream>>>(
+                st->d_ccls, st->d_cc, st->d_start, st->d_csr, st->d_w, st->d_v, st->d_p, st->nc,
+                bulk, check_max, st->d_status, s);
+        else
+            k_amr_cells<false><<<blocks(st->nc), 256, 0, st->stream>>>(
+                st->d_cell, nullptr, st->d_start, st->d_csr, st->d_w, st->d_v, st->d_p, st->nc,
+                bulk, check_max, st->d_status, s);
         if (st->plate_on && st->n_active > 0) {                  // :1179
             PlateConsts k;
             k.Nx = st->pNx;
@@ -821,8 +1025,8 @@ SK_API AcousticAMRState* amr_create(int nc, const double* cc, const double* vol,
             csr[start[fp[f]] + fill[fp[f]]++] = ~f;          // weight -ffc
         }
     }
-    std::vector<int> sc8, csr_copy;
-    std::vector<float> sw8;
+    std::vector<int>& sc8 = st->sc8;
+    std::vector<float>& sw8 = st->sw8;
     if (order == 8) {                                                // :862-869
         set_stage("stencil.build", "");
         sc8.assign(8 * (size_t)nf, 0);
@@ -985,6 +1189,17 @@ SK_API int amr_get_cell_centers(const AcousticAMRState* st, float* out, int n3) 
 SK_API double amr_get_dt(const AcousticAMRState* st) { return st ? st->dt : 0.0; }
 SK_API int amr_get_step_count(const AcousticAMRState* st) { return st ? st->step_count : 0; }
 SK_API int amr_get_csr_entries(const AcousticAMRState* st) { return st ? st->ne : 0; }
+
+SK_API int amr_set_record_layout(AcousticAMRState* st, int layout) {
+    if (!st) return SK_ERR_NULL_STATE;
+    if (layout < 0 || layout > 2) return SK_ERR_DIM_MISMATCH;
+    st->layout_req = layout;
+    return upload_coeffs(st) == hipSuccess ? SK_OK : AMR_ERR_DEVICE;
+}
+
+SK_API int amr_get_record_layout(const AcousticAMRState* st) {
+    return st ? (st->compact ? 2 : 1) : 0;
+}
 
 SK_API int amr_scatter_pressure_uniform(const AcousticAMRState* st, int Nx, int Ny, int Nz,
                                         const double bmin[3], const double bmax[3], float* out) {

@@ -263,3 +263,45 @@ def test_order8_bitwise(rot):
         m.set_state(p)
         m.step(120)
     _same(g, o)
+
+
+@pytest.mark.parametrize("layout", ["direct", "compact"])
+def test_record_layouts_bitwise(layout):
+    """Both record layouts of the step kernels (full per-face/per-cell
+    records, or class ids into tables of distinct records) give the oracle's
+    results bit for bit: a bordered box (order 2), an irregular topology
+    with walls, and an order-8 box."""
+    from afdtd.amr import AcousticAMR, AcousticAMRGrid
+    from test_amr_oracle_cpu import _box
+    print(f"amr: layout {layout}", flush=True)
+    dx = 0.01
+    box = AcousticAMRGrid.build_uniform_box(x_min=0, x_max=18 * dx, y_min=0, y_max=14 * dx,
+                                            z_min=0, z_max=22 * dx, dx=dx).with_pml_shell(3)
+    g, o = _pair_grid(box)
+    g.record_layout = layout
+    assert g.record_layout == layout
+    for m in (g, o):
+        m.set_border_condition(1, 2.0, 0.7, 0.0, 3, box.bounds_min, box.bounds_max)
+        m.inject_pressure_nearest([0.071, 0.063, 0.104], 1.0)
+        m.step(150)
+    assert g.record_layout == layout              # kept across the coefficient rebuild
+    _same(g, o)
+    t = random_topology(44)
+    g, o = AcousticAMR(**t), OracleAMR(**t)
+    g.record_layout = layout
+    p = np.random.default_rng(45).standard_normal(g.n_cells).astype(np.float32)
+    for m in (g, o):
+        m.set_state(p)
+        m.step(40)
+    _same(g, o)
+    b8 = _box(16, 12, 18)
+    args = (b8.cell_centers, b8.cell_volumes, b8.open_volume_fraction, b8.cell_types,
+            b8.face_cell_neg, b8.face_cell_pos, b8.face_area, b8.face_open_fraction,
+            b8.face_distance, 343.0, 1.21, b8.min_dx, 8)
+    g, o = AcousticAMR(*args), OracleAMR(*args)
+    g.record_layout = layout
+    p = np.random.default_rng(46).standard_normal(g.n_cells).astype(np.float32)
+    for m in (g, o):
+        m.set_state(p)
+        m.step(60)
+    _same(g, o)
