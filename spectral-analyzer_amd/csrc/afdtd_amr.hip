@@ -230,14 +230,26 @@ __global__ __launch_bounds__(256) void k_amr_faces8_c(const int2* __restrict__ f
     v[f] = (v[f] - dt_over_rho * g) * vd;
 }
 
-// One 256-thread block updates 256 consecutive cells.  The block's CSR rows
+// One 256-thread block updates 512 consecutive cells.  The block's CSR rows
 // are one contiguous entry range; when it fits kCsrStage entries it is first
 // copied into LDS with coalesced loads, so each lane's walk over its own row
 // (about six entries on a box grid, 24 bytes apart between lanes) reads LDS
 // instead of scattering loads over HBM lines.  Negating the face weight is
 // exact, so +-w[f] equals the reference's pre-signed CSR weight.
-constexpr int kCsrStage = 2048;
+constexpr int kCellTile = 512;                       // cells per 256-thread block
+constexpr int kCsrStage = 4096;
 
+// Accumulate one CSR row entry (the reference's signed weight times v).
+__device__ inline float csr_term(int e, const float* __restrict__ w,
+                                 const float* __restrict__ v) {
+    const int f = e < 0 ? ~e : e;
+    return (e < 0 ? -w[f] : w[f]) * v[f];
+}
+
+// Each lane updates two cells, c and c + 256, walking both rows in step so
+// twice as many gathers are in flight per lane (the kernel is bound by the
+// latency of its gathers at full occupancy); each row is still summed in
+// CSR order.
 template <bool COMPACT>
 __global__ __launch_bounds__(256) void k_amr_cells(const float4* __restrict__ cell,
                                                    const int* __restrict__ cell_class,
@@ -249,37 +261,45 @@ __global__ __launch_bounds__(256) void k_amr_cells(const float4* __restrict__ ce
                                                    int check_max, int* status, int s) {
     __shared__ int s_csr[kCsrStage];
     if (status[0] < s) return;
-    const int c0 = xcd_tile(gridDim.x) * 256;
-    const int c = c0 + threadIdx.x;
+    const int c0 = xcd_tile(gridDim.x) * kCellTile;
     const int k_lo = start[c0];
-    const int k_hi = start[min(c0 + 256, nc)];
+    const int k_hi = start[min(c0 + kCellTile, nc)];
     const bool staged = (k_hi - k_lo) <= kCsrStage;  // uniform over the block
     if (staged) {
         for (int k = k_lo + threadIdx.x; k < k_hi; k += 256) s_csr[k - k_lo] = csr[k];
         __syncthreads();
     }
-    bool bad = false;
-    if (c < nc) {
-        const float4 m = COMPACT ? cell[cell_class[c]] : cell[c];
-        const int k0 = start[c], k1 = start[c + 1];
-        float d = 0.0f;
-        if (staged) {
-            for (int k = k0; k < k1; ++k) {                          // :1166-1168
-                const int e = s_csr[k - k_lo];
-                const int f = e < 0 ? ~e : e;
-                d += (e < 0 ? -w[f] : w[f]) * v[f];
-            }
-        } else {
-            for (int k = k0; k < k1; ++k) {
-                const int e = csr[k];
-                const int f = e < 0 ? ~e : e;
-                d += (e < 0 ? -w[f] : w[f]) * v[f];
-            }
+    const int ca = c0 + threadIdx.x, cb = ca + 256;
+    const bool va = ca < nc, vb = cb < nc;
+    const int a0 = va ? start[ca] : 0, a1 = va ? start[ca + 1] : 0;
+    const int b0 = vb ? start[cb] : 0, b1 = vb ? start[cb + 1] : 0;
+    const int na = a1 - a0, nb = b1 - b0;
+    float da = 0.0f, db = 0.0f;
+    if (staged) {
+        for (int j = 0; j < max(na, nb); ++j) {                      // :1166-1168
+            if (j < na) da += csr_term(s_csr[a0 - k_lo + j], w, v);
+            if (j < nb) db += csr_term(s_csr[b0 - k_lo + j], w, v);
         }
-        float pn = p[c] * m.x - bulk * d * m.y * m.z;                 // :1169-1170
-        if (__float_as_int(m.w) & 1) pn = 0.0f;                       // :1174-1176
-        p[c] = pn;
-        bad = !isfinite(pn) || (check_max && fabsf(pn) > 2e3f);       // :1181-1192
+    } else {
+        for (int j = 0; j < max(na, nb); ++j) {
+            if (j < na) da += csr_term(csr[a0 + j], w, v);
+            if (j < nb) db += csr_term(csr[b0 + j], w, v);
+        }
+    }
+    bool bad = false;
+    if (va) {
+        const float4 m = COMPACT ? cell[cell_class[ca]] : cell[ca];
+        float pn = p[ca] * m.x - bulk * da * m.y * m.z;              // :1169-1170
+        if (__float_as_int(m.w) & 1) pn = 0.0f;                      // :1174-1176
+        p[ca] = pn;
+        bad = !isfinite(pn) || (check_max && fabsf(pn) > 2e3f);      // :1181-1192
+    }
+    if (vb) {
+        const float4 m = COMPACT ? cell[cell_class[cb]] : cell[cb];
+        float pn = p[cb] * m.x - bulk * db * m.y * m.z;
+        if (__float_as_int(m.w) & 1) pn = 0.0f;
+        p[cb] = pn;
+        bad = bad || !isfinite(pn) || (check_max && fabsf(pn) > 2e3f);
     }
     if (__ballot(bad) != 0ull && (threadIdx.x & 63) == 0) atomicMin(&status[0], s);
 }
@@ -897,12 +917,13 @@ int step_impl(AcousticAMRState* st, int n, float* ms_total, float* ms_faces, flo
         }
         if (ms_total) (void)hipEventRecord(ev[2 * (size_t)s + 1], st->stream);
         const int check_max = ((st->step_count + s) & 0xFF) == 0;
+        const unsigned cell_blocks = (unsigned)((st->nc + kCellTile - 1) / kCellTile);
         if (st->compact)
-            k_amr_cells<true><<<blocks(st->nc), 256, 0, st->stream>>>(
+            k_amr_cells<true><<<cell_blocks, 256, 0, st->stream>>>(
                 st->d_ccls, st->d_cc, st->d_start, st->d_csr, st->d_w, st->d_v, st->d_p, st->nc,
                 bulk, check_max, st->d_status, s);
         else
-            k_amr_cells<false><<<blocks(st->nc), 256, 0, st->stream>>>(
+            k_amr_cells<false><<<cell_blocks, 256, 0, st->stream>>>(
                 st->d_cell, nullptr, st->d_start, st->d_csr, st->d_w, st->d_v, st->d_p, st->nc,
                 bulk, check_max, st->d_status, s);
         if (st->plate_on && st->n_active > 0) {                  // :1179
