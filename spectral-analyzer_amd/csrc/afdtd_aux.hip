@@ -412,21 +412,46 @@ struct StreamArrays {
     float* out[8];
 };
 
-// n_read FP32 arrays in, n_write out, one element per lane per pass: the
-// byte mix of a stencil step without the stencil.
-__global__ void __launch_bounds__(256) k_stream_mix(StreamArrays a, int n_read, int n_write,
-                                                    size_t n) {
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (size_t)gridDim.x * blockDim.x) {
-        float v[8];
+// R FP32 arrays in, W out, one element per lane and one lane per element:
+// the byte mix of a stencil step without the stencil.  R and W are compile-
+// time constants and the grid covers the arrays once (no grid-stride loop),
+// so the pass is limited by the memory system, not by issue.
+template <int R, int W>
+__global__ void __launch_bounds__(256) k_stream_mix(StreamArrays a, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float s = 0.0f;
 #pragma unroll
-        for (int c = 0; c < 8; ++c) v[c] = c < n_read ? a.in[c][i] : 0.0f;
-        float s = 0.0f;
+    for (int c = 0; c < R; ++c) s += a.in[c][i];
 #pragma unroll
-        for (int c = 0; c < 8; ++c) s += v[c];
-#pragma unroll
-        for (int c = 0; c < 8; ++c)
-            if (c < n_write) __builtin_nontemporal_store(s + (float)c, a.out[c] + i);
+    for (int c = 0; c < W; ++c) __builtin_nontemporal_store(s + (float)c, a.out[c] + i);
+}
+
+template <int R>
+void launch_mix_w(StreamArrays a, int n_write, size_t n, unsigned blocks, hipStream_t s) {
+    switch (n_write) {
+        case 1: hipLaunchKernelGGL((k_stream_mix<R, 1>), dim3(blocks), dim3(256), 0, s, a, n); break;
+        case 2: hipLaunchKernelGGL((k_stream_mix<R, 2>), dim3(blocks), dim3(256), 0, s, a, n); break;
+        case 3: hipLaunchKernelGGL((k_stream_mix<R, 3>), dim3(blocks), dim3(256), 0, s, a, n); break;
+        case 4: hipLaunchKernelGGL((k_stream_mix<R, 4>), dim3(blocks), dim3(256), 0, s, a, n); break;
+        case 5: hipLaunchKernelGGL((k_stream_mix<R, 5>), dim3(blocks), dim3(256), 0, s, a, n); break;
+        case 6: hipLaunchKernelGGL((k_stream_mix<R, 6>), dim3(blocks), dim3(256), 0, s, a, n); break;
+        case 7: hipLaunchKernelGGL((k_stream_mix<R, 7>), dim3(blocks), dim3(256), 0, s, a, n); break;
+        default: hipLaunchKernelGGL((k_stream_mix<R, 8>), dim3(blocks), dim3(256), 0, s, a, n); break;
+    }
+}
+
+void launch_mix(StreamArrays a, int n_read, int n_write, size_t n, hipStream_t s) {
+    const unsigned blocks = (unsigned)((n + 255) / 256);
+    switch (n_read) {
+        case 1: launch_mix_w<1>(a, n_write, n, blocks, s); break;
+        case 2: launch_mix_w<2>(a, n_write, n, blocks, s); break;
+        case 3: launch_mix_w<3>(a, n_write, n, blocks, s); break;
+        case 4: launch_mix_w<4>(a, n_write, n, blocks, s); break;
+        case 5: launch_mix_w<5>(a, n_write, n, blocks, s); break;
+        case 6: launch_mix_w<6>(a, n_write, n, blocks, s); break;
+        case 7: launch_mix_w<7>(a, n_write, n, blocks, s); break;
+        default: launch_mix_w<8>(a, n_write, n, blocks, s); break;
     }
 }
 }  // namespace
@@ -462,11 +487,9 @@ extern "C" SK_API int fdtd_stream_calibration(size_t n_cells, int n_read, int n_
     for (int c = 0; c < n_read && rc == FDTD_OK; ++c)
         if ((e = hipMemsetAsync((void*)a.in[c], 0, n_cells * sizeof(float), s)) != hipSuccess) fail(e);
     if (rc == FDTD_OK) {
-        const unsigned blocks = 1024;
-        hipLaunchKernelGGL(k_stream_mix, dim3(blocks), dim3(256), 0, s, a, n_read, n_write, n_cells);
+        launch_mix(a, n_read, n_write, n_cells, s);
         (void)hipEventRecord(e0, s);
-        for (int r = 0; r < reps; ++r)
-            hipLaunchKernelGGL(k_stream_mix, dim3(blocks), dim3(256), 0, s, a, n_read, n_write, n_cells);
+        for (int r = 0; r < reps; ++r) launch_mix(a, n_read, n_write, n_cells, s);
         (void)hipEventRecord(e1, s);
         if ((e = hipGetLastError()) != hipSuccess) fail(e);
         else if ((e = hipStreamSynchronize(s)) != hipSuccess) fail(e);
