@@ -432,6 +432,74 @@ def bench_amr(a):
     return 0
 
 
+def bench_amr_audio(a):
+    """The acoustic half of the co-evolver's per-sample loop on the AMR
+    stepper: 96^3 box grid (884,736 cells) with an 8-cell anechoic shell, 4
+    mics, `substeps` steps per audio sample (the co-evolver's sub-stepping at
+    48 kHz for dx = 5 mm), each sample's polar-mixed mic output recorded on
+    the device (amr_record_mic_samples); the oracle running the same loop
+    (step, sample_mics, mix) is the CPU baseline."""
+    from afdtd.amr import AcousticAMR, AcousticAMRGrid
+    n, dx, n_pml = 96, 0.005, 8
+    grid = AcousticAMRGrid.build_uniform_box(x_min=0, x_max=n * dx, y_min=0, y_max=n * dx,
+                                             z_min=0, z_max=n * dx, dx=dx).with_pml_shell(n_pml)
+    args = (grid.cell_centers, grid.cell_volumes, grid.open_volume_fraction, grid.cell_types,
+            grid.face_cell_neg, grid.face_cell_pos, grid.face_area, grid.face_open_fraction,
+            grid.face_distance, 343.0, 1.21, dx, 2)
+    sr = 48000.0
+    g = AcousticAMR(*args)
+    substeps = int(np.ceil((1.0 / sr) / g.dt))
+    mics = np.array([[0.11, 0.23, 0.31], [0.35, 0.2, 0.12], [0.24, 0.24, 0.4],
+                     [0.3, 0.37, 0.25]], np.float32)
+    axis = np.tile(np.float32([0, 0, 1]), (4, 1))
+    pa, pb, gain = np.float32([1, 0.5, 0, 1]), np.float32([0, 0.5, 1, 0]), np.ones(4, np.float32)
+    rho_c = np.float32(np.float32(1.21) * np.float32(343.0))
+
+    def setup(m):
+        m.set_border_condition(0, 3.0, 0.0, 0.0, n_pml, grid.bounds_min, grid.bounds_max)
+        m.inject_pressure_nearest([0.2, 0.25, 0.22], 1.0)
+        m.setup_mic_samplers(mics, axis, True)
+
+    setup(g)
+    samples = a.steps if a.steps != 4000 else 2000
+    g.record_mic_samples(min(a.warmup, 20), substeps, pa, pb, axis, gain, rho_c)
+    t0 = time.perf_counter()
+    out, rc, done = g.record_mic_samples(samples, substeps, pa, pb, axis, gain, rho_c)
+    wall = time.perf_counter() - t0
+    if rc != 0 or not np.all(np.isfinite(out)):
+        raise SystemExit(f"amr_record_mic_samples failed rc={rc}")
+    line = {"metric": "audio samples/s (AMR order-2 stepper, sub-stepped, mic polar mix, 1 GPU)",
+            "value": samples / wall, "unit": "samples/s", "n_gpus": 1, "steps": samples,
+            "warmup": min(a.warmup, 20), "ms_per_step": wall * 1e3 / samples,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+            "data": "synthetic (uniform box grid, one pulse)",
+            "config": {"workload": f"amr-audio-box-{n}x{n}x{n}-dx5mm-48kHz", "cells": g.n_cells,
+                       "faces": g.n_faces, "substeps": substeps, "mics": 4,
+                       "record_layout": g.record_layout},
+            "realtime_factor_48k": samples / wall / sr,
+            "cell_updates_per_s": samples * substeps * g.n_cells / wall}
+    del g
+    if not a.no_cpu:
+        from oracle.amr_oracle import OracleAMR
+        o = OracleAMR(*args)
+        setup(o)
+        t0 = time.perf_counter()
+        k = 0
+        while time.perf_counter() - t0 < 10.0:
+            o.step(substeps)
+            sp, vx, vy, vz = o.sample_mics()
+            _ = (sp * pa + (pb * rho_c) * (vx * axis[:, 0] + vy * axis[:, 1] + vz * axis[:, 2])) * gain
+            k += 1
+        dt = time.perf_counter() - t0
+        cores = int(os.environ.get("OMP_NUM_THREADS") or len(os.sched_getaffinity(0)))
+        line["cpu_baseline"] = {"value": k / dt, "unit": "samples/s", "cores": cores,
+                                "kind": "port",
+                                "sample": f"{k} samples of the same loop ({dt:.1f} s, float32 AMR "
+                                          f"oracle, OpenMP over faces and cells)"}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -470,6 +538,8 @@ def main():
         return bench_geometry(a)
     if a.config == "amr":
         return bench_amr(a)
+    if a.config == "amr-audio":
+        return bench_amr_audio(a)
     w = make_workload(a.config, a.warmup + a.steps)
     s = make_solver(w, a.storage, a.kernel, a.tile_y, a.seg_x)
     total = a.warmup + a.steps

@@ -212,6 +212,19 @@ SK_API int amr_get_csr_entries(const AcousticAMRState* st);
 SK_API int amr_set_record_layout(AcousticAMRState* st, int layout);
 SK_API int amr_get_record_layout(const AcousticAMRState* st);
 
+/* The acoustic half of the co-evolver's per-sample loop on the device:
+ * n_samples audio samples of `substeps` steps each; after each sample every
+ * mic (amr_setup_mic_samplers) records (p*a + b*rho_c*(v . axis)) * gain,
+ * the co-evolver's polar mix (reference acoustic_coevolver.cpp:2208-2219),
+ * into out[sample * n_mics + mic].  One host synchronisation per call.
+ * polar_a, polar_b, gain: n_mics floats; axis_xyz: 3 per mic.  On a
+ * divergence returns SK_ERR_DIVERGED with *n_done = samples completed
+ * before the failing step (later rows are 0). */
+SK_API int amr_record_mic_samples(AcousticAMRState* st, int n_samples, int substeps,
+                                  const float* polar_a, const float* polar_b,
+                                  const float* axis_xyz, const float* gain, float rho_c,
+                                  float* out, int* n_done);
+
 #ifdef __cplusplus
 }
 #endif

@@ -47,6 +47,7 @@ _SIGS = {
     "amr_get_step_count": (_i, [_vp]),
     "amr_get_csr_entries": (_i, [_vp]),
     "amr_set_record_layout": (_i, [_vp, _i]),
+    "amr_record_mic_samples": (_i, [_vp, _i, _i, _vp, _vp, _vp, _vp, _f, _vp, _vp]),
     "amr_get_record_layout": (_i, [_vp]),
     "amr_scatter_pressure_uniform": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp]),
     "amr_set_border_condition": (_i, [_vp, _i, _f, _f, _f, _i, _vp, _vp]),
@@ -303,6 +304,24 @@ class AcousticAMR:
     @property
     def n_csr_entries(self) -> int:
         return lib().amr_get_csr_entries(self._h)
+
+    def record_mic_samples(self, n_samples: int, substeps: int, polar_a, polar_b, axis_xyz,
+                           gain, rho_c: float):
+        """n_samples audio samples of `substeps` steps with every mic's polar
+        mix recorded on the device; returns (out (n_samples, n_mics), rc,
+        samples done)."""
+        m = getattr(self, "_n_mics", 0)
+        a = np.ascontiguousarray(polar_a, np.float32).reshape(m)
+        b = np.ascontiguousarray(polar_b, np.float32).reshape(m)
+        ax = np.ascontiguousarray(axis_xyz, np.float32).reshape(m, 3)
+        gn = np.ascontiguousarray(gain, np.float32).reshape(m)
+        out = np.zeros((int(n_samples), m), np.float32)
+        done = C.c_int(0)
+        rc = lib().amr_record_mic_samples(self._h, int(n_samples), int(substeps), _p(a), _p(b),
+                                          _p(ax), _p(gn), float(rho_c), _p(out), C.byref(done))
+        if rc not in (0, -3):
+            raise RuntimeError(f"amr_record_mic_samples failed: rc={rc}")
+        return out, rc, int(done.value)
 
     @property
     def record_layout(self) -> str:

@@ -504,6 +504,47 @@ __global__ void k_amr_mics(const int* __restrict__ mp_start, const int* __restri
     out[3 * n_mics + m] = vz;
 }
 
+// One audio sample of the co-evolver's mic output (acoustic_coevolver.cpp:
+// 2208-2219): the mic samplers' p and v, mixed as
+// (p*a + b*(rho c)*(v . axis)) * gain, one thread per mic, into row `row`.
+// Skips when a step up to and including s_end failed (the co-evolver
+// returns before sampling).
+__global__ void k_amr_mic_mix(const int* __restrict__ mp_start, const int* __restrict__ mp_cell,
+                              const float* __restrict__ mp_w, const int* __restrict__ mv_start,
+                              const int* __restrict__ mv_face, const float* __restrict__ wx,
+                              const float* __restrict__ wy, const float* __restrict__ wz,
+                              const float* __restrict__ p, const float* __restrict__ v,
+                              int n_mics, const float* __restrict__ mix, float rho_c,
+                              float* __restrict__ out, int row, const int* __restrict__ status,
+                              int s_end) {
+    if (status[0] <= s_end) return;
+    const int m = blockIdx.x * 64 + threadIdx.x;
+    if (m >= n_mics) return;
+    float pp = 0.0f;
+    for (int k = mp_start[m]; k < mp_start[m + 1]; ++k) pp += mp_w[k] * p[mp_cell[k]];
+    float vx = 0.0f, vy = 0.0f, vz = 0.0f;
+    for (int k = mv_start[m]; k < mv_start[m + 1]; ++k) {
+        const float vf = v[mv_face[k]];
+        vx += wx[k] * vf;
+        vy += wy[k] * vf;
+        vz += wz[k] * vf;
+    }
+    const float* q = mix + 6 * m;                    // a, b, axis x, y, z, gain
+    float sig = pp * q[0];
+    if (q[1] != 0.0f) {
+        const float vn = vx * q[2] + vy * q[3] + vz * q[4];
+        sig += q[1] * rho_c * vn;
+    }
+    out[(size_t)row * n_mics + m] = sig * q[5];
+}
+
+struct MicMix {                                      // per-sample mic recording
+    int substeps;
+    const float* mix;                                // device, 6 floats per mic
+    float rho_c;
+    float* out;                                      // device, n_samples x n_mics
+};
+
 inline unsigned blocks(size_t n) { return (unsigned)((n + 255) / 256); }
 
 // ── Gradient order 8: per-face Fornberg stencil (acoustic_amr.cpp:313-679) ──
@@ -875,7 +916,8 @@ void report_diverged(AcousticAMRState* st) {
             n_bad);
 }
 
-int step_impl(AcousticAMRState* st, int n, float* ms_total, float* ms_faces, float* ms_cells) {
+int step_impl(AcousticAMRState* st, int n, float* ms_total, float* ms_faces, float* ms_cells,
+              const MicMix* mix = nullptr) {
     if (!st) return SK_ERR_NULL_STATE;
     if (n < 0) return SK_ERR_DIM_MISMATCH;
     if (n == 0) {
@@ -946,6 +988,14 @@ int step_impl(AcousticAMRState* st, int n, float* ms_total, float* ms_faces, flo
                 I + st->o_bls, F + st->o_blw, I + st->o_ca, I + st->o_cb, st->d_p,
                 st->d_pw[(st->prot + s + 1) % 3], st->d_pw[(st->prot + s) % 3],
                 st->d_pw[(st->prot + s + 2) % 3], st->d_ext, st->d_status, s);
+        }
+        if (mix && (s + 1) % mix->substeps == 0) {
+            const int* I = st->d_mint;
+            const float* F = st->d_mflt;
+            k_amr_mic_mix<<<(st->n_mics + 63) / 64, 64, 0, st->stream>>>(
+                I, I + st->o_mpc, F, I + st->o_mvs, I + st->o_mvf, F + st->o_mwx, F + st->o_mwy,
+                F + st->o_mwz, st->d_p, st->d_v, st->n_mics, mix->mix, mix->rho_c, mix->out,
+                (s + 1) / mix->substeps - 1, st->d_status, s);
         }
     }
     if (ms_total) (void)hipEventRecord(ev[2 * (size_t)n], st->stream);
@@ -1723,6 +1773,47 @@ SK_API int amr_sample_mics(const AcousticAMRState* cst, int n_mics, float* op, f
         }
     }
     return SK_OK;
+}
+
+// Extension (no reference counterpart): n_samples audio samples of
+// `substeps` steps each with the mic output of every sample recorded on the
+// device, one host synchronisation for the whole block.
+SK_API int amr_record_mic_samples(AcousticAMRState* st, int n_samples, int substeps,
+                                  const float* polar_a, const float* polar_b,
+                                  const float* axis_xyz, const float* gain, float rho_c,
+                                  float* out, int* n_done) {
+    if (!st || !polar_a || !polar_b || !axis_xyz || !gain || !out || !n_done)
+        return SK_ERR_NULL_STATE;
+    *n_done = 0;
+    if (n_samples < 0 || substeps <= 0 || st->n_mics <= 0) return SK_ERR_DIM_MISMATCH;
+    if (n_samples == 0) return SK_OK;
+    const int M = st->n_mics;
+    std::vector<float> mix(6 * (size_t)M);
+    for (int m = 0; m < M; ++m) {
+        mix[6 * m] = polar_a[m];
+        mix[6 * m + 1] = polar_b[m];
+        for (int d = 0; d < 3; ++d) mix[6 * m + 2 + d] = axis_xyz[3 * m + d];
+        mix[6 * m + 5] = gain[m];
+    }
+    const size_t out_n = (size_t)n_samples * M;
+    hipError_t e = ensure_tmp(st, sizeof(float) * (mix.size() + out_n));
+    float* dmix = (float*)st->d_tmp;
+    float* dout = dmix + mix.size();
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(dmix, mix.data(), sizeof(float) * mix.size(), hipMemcpyHostToDevice,
+                           st->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(dout, 0, sizeof(float) * out_n, st->stream);
+    if (e != hipSuccess) return AMR_ERR_DEVICE;
+    const int c0 = st->step_count;
+    MicMix mm{substeps, dmix, rho_c, dout};
+    const int rc = step_impl(st, n_samples * substeps, nullptr, nullptr, nullptr, &mm);
+    if (rc == AMR_ERR_DEVICE) return rc;
+    // samples whose steps all completed before any failure
+    *n_done = rc == SK_OK ? n_samples : (st->step_count - c0) / substeps;
+    e = hipMemcpyAsync(out, dout, sizeof(float) * out_n, hipMemcpyDeviceToHost, st->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(st->stream);
+    if (e != hipSuccess) return AMR_ERR_DEVICE;
+    return rc;
 }
 
 }  // extern "C"

@@ -305,3 +305,44 @@ def test_record_layouts_bitwise(layout):
         m.set_state(p)
         m.step(60)
     _same(g, o)
+
+
+def test_record_mic_samples_bitwise():
+    """The acoustic half of the co-evolver's per-sample loop on the device:
+    sub-steps, then each mic's polar mix (p*a + b*rho_c*(v . axis)) * gain
+    per sample, equals the oracle's step / sample_mics / mix loop bit for
+    bit; a divergence stops the recording at the failing sample."""
+    from afdtd.amr import AcousticAMR
+    from test_amr_oracle_cpu import MIC_POS, line_grid, mic_grid
+    print("amr: recorded mic samples", flush=True)
+    t = mic_grid()
+    g, o = AcousticAMR(**t), OracleAMR(**t)
+    rng = np.random.default_rng(47)
+    p = rng.standard_normal(g.n_cells).astype(np.float32)
+    axis = np.array([[0, 0, 1], [0.6, 0.8, 0], [1, 0, 0]], np.float32)
+    pa = np.float32([1.0, 0.5, 0.0])
+    pb = np.float32([0.0, 0.5, 1.0])
+    gain = np.float32([1.0, 2.0, 0.25])
+    rho_c = np.float32(np.float32(1.21) * np.float32(343.0))
+    for m in (g, o):
+        m.set_state(p)
+        m.setup_mic_samplers(MIC_POS, axis, True)
+    out, rc, done = g.record_mic_samples(25, 4, pa, pb, axis, gain, rho_c)
+    assert rc == 0 and done == 25 and g.step_count == 100
+    want = np.zeros((25, 3), np.float32)
+    for j in range(25):
+        o.step(4)
+        sp, vx, vy, vz = o.sample_mics()
+        sig = sp * pa
+        vn = vx * axis[:, 0] + vy * axis[:, 1] + vz * axis[:, 2]
+        sig = np.where(pb != 0, sig + (pb * rho_c) * vn, sig).astype(np.float32)
+        want[j] = sig * gain
+    assert np.array_equal(out.view(np.uint32), want.view(np.uint32))
+    _same(g, o)
+    gl, _ = _pair_grid(line_grid())
+    gl.setup_mic_samplers([[0.1, 0, 0]], [[1, 0, 0]], True)
+    gl.step(250)                                   # next stability test at step 256
+    gl.inject_pressure_nearest([0.15, 0, 0], 5e4)
+    out, rc, done = gl.record_mic_samples(5, 4, [1.0], [0.0], [[1, 0, 0]], [1.0], rho_c)
+    assert rc == -3 and done == 1 and gl.step_count == 256   # failed at step 256
+    assert out[0, 0] != 0 and not np.any(out[1:])
