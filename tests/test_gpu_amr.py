@@ -346,3 +346,46 @@ def test_record_mic_samples_bitwise():
     out, rc, done = gl.record_mic_samples(5, 4, [1.0], [0.0], [[1, 0, 0]], [1.0], rho_c)
     assert rc == -3 and done == 1 and gl.step_count == 256   # failed at step 256
     assert out[0, 0] != 0 and not np.any(out[1:])
+
+
+def test_record_mic_samples_graph_replays():
+    """Blocks of the same shape replay the captured launch sequence: three
+    blocks in a row with eager work between them (an injection, a border
+    change that forces a new capture) each equal the oracle bit for bit."""
+    from afdtd.amr import AcousticAMR, AcousticAMRGrid
+    print("amr: recorded mic samples, graph replays", flush=True)
+    dx = 0.01
+    box = AcousticAMRGrid.build_uniform_box(x_min=0, x_max=14 * dx, y_min=0, y_max=12 * dx,
+                                            z_min=0, z_max=16 * dx, dx=dx).with_pml_shell(2)
+    g, o = _pair_grid(box)
+    mics = np.array([[0.031, 0.052, 0.077], [0.093, 0.061, 0.112]], np.float32)
+    axis = np.array([[0, 0, 1], [1, 0, 0]], np.float32)
+    pa, pb, gain = np.float32([1.0, 0.3]), np.float32([0.0, 0.7]), np.float32([1.0, 3.0])
+    rho_c = np.float32(np.float32(1.21) * np.float32(343.0))
+    for m in (g, o):
+        m.setup_mic_samplers(mics, axis, True)
+        m.inject_pressure_nearest([0.07, 0.06, 0.08], 1.0)
+
+    def oracle_block(n, k):
+        rows = []
+        for _ in range(n):
+            o.step(k)
+            sp, vx, vy, vz = o.sample_mics()
+            sig = sp * pa
+            vn = vx * axis[:, 0] + vy * axis[:, 1] + vz * axis[:, 2]
+            sig = np.where(pb != 0, sig + (pb * rho_c) * vn, sig).astype(np.float32)
+            rows.append(sig * gain)
+        return np.stack(rows)
+
+    for blk in range(4):
+        if blk == 1:
+            for m in (g, o):
+                m.inject_pressure_nearest([0.05, 0.04, 0.1], -0.5)
+        if blk == 2:
+            for m in (g, o):
+                m.set_border_condition(0, 3.0, 0.0, 0.0, 2, box.bounds_min, box.bounds_max)
+        out, rc, done = g.record_mic_samples(7, 3, pa, pb, axis, gain, rho_c)
+        assert rc == 0 and done == 7
+        want = oracle_block(7, 3)
+        assert np.array_equal(out.view(np.uint32), want.view(np.uint32)), blk
+        _same(g, o)
