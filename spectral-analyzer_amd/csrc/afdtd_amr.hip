@@ -1,9 +1,8 @@
 // afdtd_amr.hip — the topology-driven (AMR) acoustic stepper of libafdtd.so:
-// the reference's amr_* C ABI (csrc/include/acoustic_amr.h) for gradient
-// order 2, on the GPU.
+// the reference's amr_* C ABI (csrc/include/acoustic_amr.h) on the GPU,
+// gradient orders 2 and 8, border conditions, Kirchhoff plate, mic samplers.
 //
-// Device layout (HBM): per-face and per-cell records are one 128-bit load,
-// a CSR entry is one 32-bit index whose sign selects the weight's sign:
+// Device layout (HBM), direct records (every record streamed in full):
 //   face[f]  int4   {neg cell, pos cell, bits(1/d), bits(V_damp)}
 //   v[f]     float  face velocity (neg -> pos)
 //   p[c]     float  cell pressure
@@ -14,12 +13,16 @@
 //   csr[k]   int    face index f (the cell is the face's neg cell) or ~f (its
 //                   pos cell), in face order per cell
 //   w[f]     float  area*open; the CSR weight is +w[f] or -w[f]
-// A step is two kernels on the handle's stream: k_amr_faces (one thread per
-// face, fully coalesced record + gathered pressure pair) and k_amr_cells
-// (one thread per cell: CSR walk, fused divergence + pressure update, wall
-// zeroing and the stability test).  FP32 arithmetic follows the reference's
-// expression order (acoustic_amr.cpp:1147-1172) and the library is compiled
-// without FMA contraction, so results equal the reference's float path.
+//   sc/sw           order 8: 8 stencil cells and weights per face
+// Compact records (used while the tables stay small): per face {neg, class},
+// per cell a class id, into tables of the distinct records.
+// A step is two kernels on the handle's stream, the face update
+// (k_amr_faces / _faces_c / _faces8 / _faces8_c) and the cell update
+// (k_amr_cells: CSR walk, fused divergence + pressure update, wall zeroing,
+// stability test), plus the plate's two kernels when a plate is attached.
+// FP32 arithmetic follows the reference's expression order
+// (acoustic_amr.cpp:1127-1172) and the library is compiled without FMA
+// contraction, so results equal the reference's float path.
 #include <hip/hip_runtime.h>
 #include <limits.h>
 #include <math.h>
