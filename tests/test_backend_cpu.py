@@ -13,7 +13,7 @@ HARNESS = os.path.join(HERE, "harness", "backend_harness")
 
 
 def write_case(d, grid, kw, n_bridge=3, steps=10, seed=0, n_pml=0, border_mode=0,
-               border_R=0.0, mics=None, uni=(6, 5, 7)):
+               border_R=0.0, mics=None, uni=(6, 5, 7), drive_scale=0.01):
     """Write one backend case (descriptor arrays, drives, mics) into d."""
     rng = np.random.default_rng(seed)
     flat = np.asarray(kw["plate_active_flat_idx"], np.int32)
@@ -51,7 +51,7 @@ def write_case(d, grid, kw, n_bridge=3, steps=10, seed=0, n_pml=0, border_mode=0
             "plate_alpha_M": kw["plate_alpha_M"], "plate_beta_K": kw["plate_beta_K"],
             "n_active_plate": flat.size, "n_bridge": n_bridge, "border_mode": border_mode,
             "border_sigma_order": 3.0, "border_R": border_R, "n_pml": n_pml, "gradient_order": 2,
-            "n_mics": len(mics), "steps": steps, "scale": 0.01, "uni_Nx": uni[0],
+            "n_mics": len(mics), "steps": steps, "scale": drive_scale, "uni_Nx": uni[0],
             "uni_Ny": uni[1], "uni_Nz": uni[2]}
     os.makedirs(d, exist_ok=True)
     with open(os.path.join(d, "meta.txt"), "w") as f:

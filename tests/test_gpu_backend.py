@@ -52,3 +52,29 @@ def test_amr_backend_matches_oracle(tmp_path, n_pml, mode, R):
     assert np.array_equal(load("uni"), uni.ravel())
     assert np.abs(tr).max() > 0
     assert r.stdout.startswith("OK ") and f"dt={o.dt!r}" in r.stdout
+
+
+def test_amr_backend_idle_silence_and_bridge_drive(tmp_path):
+    """The reference's co-evolver checks on the pressure side
+    (test_amr_stability.py:291-296, test_amr_coevolver.py:278-338): without
+    excitation the backend stays exactly silent; a bridge drive reaches the
+    air on both sides of the plate and the mics."""
+    from test_amr_oracle_cpu import plated_box
+    print("backend: idle silence and bridge drive", flush=True)
+    grid, kw = plated_box(10, 8, 14, 5)
+    mics = np.array([[0.0437, 0.0361, 0.0913], [0.0512, 0.0388, 0.0215]], np.float32)
+    load = lambda d, k: np.fromfile(os.path.join(d, k + ".bin"), np.float32)
+    d0 = str(tmp_path / "idle")
+    write_case(d0, grid, kw, n_bridge=4, steps=200, mics=mics, uni=(10, 8, 14), drive_scale=0.0)
+    r = run_harness(d0)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert not np.any(load(d0, "p")) and not np.any(load(d0, "w")) and not np.any(load(d0, "trace"))
+    d1 = str(tmp_path / "drive")
+    write_case(d1, grid, kw, n_bridge=4, steps=200, mics=mics, uni=(10, 8, 14), drive_scale=1.0)
+    r = run_harness(d1)
+    assert r.returncode == 0, r.stdout + r.stderr
+    p = load(d1, "p").reshape(10, 8, 14)
+    assert np.abs(p[:, :, 6:]).max() > 0 and np.abs(p[:, :, :5]).max() > 0   # above and below
+    assert np.all(p[:, :, 5] == 0)                                           # the plate layer
+    tr = load(d1, "trace").reshape(200, 4, 2)
+    assert np.abs(tr[:, 0]).max() > 0 and np.all(np.isfinite(tr))
