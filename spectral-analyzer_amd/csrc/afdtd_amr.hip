@@ -495,63 +495,93 @@ __global__ void k_amr_plate_weighted(const float* w, const int* idx, const float
     out[0] = val;
 }
 
-// Mic samples (acoustic_amr.cpp:1762-1792), one thread per mic, sums in
-// stencil order.
-__global__ void k_amr_mics(const int* __restrict__ mp_start, const int* __restrict__ mp_cell,
-                           const float* __restrict__ mp_w, const int* __restrict__ mv_start,
-                           const int* __restrict__ mv_face, const float* __restrict__ wx,
-                           const float* __restrict__ wy, const float* __restrict__ wz,
-                           const float* __restrict__ p, const float* __restrict__ v, int n_mics,
-                           float* __restrict__ out) {
-    const int m = blockIdx.x * 64 + threadIdx.x;
-    if (m >= n_mics) return;
-    float pp = 0.0f;
-    for (int k = mp_start[m]; k < mp_start[m + 1]; ++k) pp += mp_w[k] * p[mp_cell[k]];
-    float vx = 0.0f, vy = 0.0f, vz = 0.0f;
-    for (int k = mv_start[m]; k < mv_start[m + 1]; ++k) {
-        const float vf = v[mv_face[k]];
-        vx += wx[k] * vf;
-        vy += wy[k] * vf;
-        vz += wz[k] * vf;
+// Mic samples (acoustic_amr.cpp:1762-1792): one 256-thread block per mic.
+// The stencil's products are formed by the block's lanes in parallel, 256
+// at a time, into LDS; lane 0 then adds them in stencil order, so each sum
+// is the reference's sequential sum of the same rounded products (a single
+// lane walking the stencil spent ~0.6 us per entry on the dependent loads).
+struct MicSums {
+    float p, vx, vy, vz;
+};
+
+__device__ MicSums mic_sums(int m, const int* __restrict__ mp_start,
+                            const int* __restrict__ mp_cell, const float* __restrict__ mp_w,
+                            const int* __restrict__ mv_start, const int* __restrict__ mv_face,
+                            const float* __restrict__ wx, const float* __restrict__ wy,
+                            const float* __restrict__ wz, const float* __restrict__ p,
+                            const float* __restrict__ v) {
+    __shared__ float sh[3][256];
+    const int t = threadIdx.x;
+    MicSums r{0.0f, 0.0f, 0.0f, 0.0f};
+    const int p0 = mp_start[m], p1 = mp_start[m + 1];
+    for (int base = p0; base < p1; base += 256) {
+        const int k = base + t;
+        if (k < p1) sh[0][t] = mp_w[k] * p[mp_cell[k]];
+        __syncthreads();
+        if (t == 0)
+            for (int j = 0; j < min(256, p1 - base); ++j) r.p += sh[0][j];
+        __syncthreads();
     }
-    out[m] = pp;
-    out[n_mics + m] = vx;
-    out[2 * n_mics + m] = vy;
-    out[3 * n_mics + m] = vz;
+    const int v0 = mv_start[m], v1 = mv_start[m + 1];
+    for (int base = v0; base < v1; base += 256) {
+        const int k = base + t;
+        if (k < v1) {
+            const float vf = v[mv_face[k]];
+            sh[0][t] = wx[k] * vf;
+            sh[1][t] = wy[k] * vf;
+            sh[2][t] = wz[k] * vf;
+        }
+        __syncthreads();
+        if (t == 0)
+            for (int j = 0; j < min(256, v1 - base); ++j) {
+                r.vx += sh[0][j];
+                r.vy += sh[1][j];
+                r.vz += sh[2][j];
+            }
+        __syncthreads();
+    }
+    return r;
+}
+
+__global__ __launch_bounds__(256) void k_amr_mics(
+    const int* __restrict__ mp_start, const int* __restrict__ mp_cell,
+    const float* __restrict__ mp_w, const int* __restrict__ mv_start,
+    const int* __restrict__ mv_face, const float* __restrict__ wx, const float* __restrict__ wy,
+    const float* __restrict__ wz, const float* __restrict__ p, const float* __restrict__ v,
+    int n_mics, float* __restrict__ out) {
+    const int m = blockIdx.x;
+    const MicSums r = mic_sums(m, mp_start, mp_cell, mp_w, mv_start, mv_face, wx, wy, wz, p, v);
+    if (threadIdx.x == 0) {
+        out[m] = r.p;
+        out[n_mics + m] = r.vx;
+        out[2 * n_mics + m] = r.vy;
+        out[3 * n_mics + m] = r.vz;
+    }
 }
 
 // One audio sample of the co-evolver's mic output (acoustic_coevolver.cpp:
 // 2208-2219): the mic samplers' p and v, mixed as
-// (p*a + b*(rho c)*(v . axis)) * gain, one thread per mic, into row `row`.
-// Skips when a step up to and including s_end failed (the co-evolver
-// returns before sampling).
-__global__ void k_amr_mic_mix(const int* __restrict__ mp_start, const int* __restrict__ mp_cell,
-                              const float* __restrict__ mp_w, const int* __restrict__ mv_start,
-                              const int* __restrict__ mv_face, const float* __restrict__ wx,
-                              const float* __restrict__ wy, const float* __restrict__ wz,
-                              const float* __restrict__ p, const float* __restrict__ v,
-                              int n_mics, const float* __restrict__ mix, float rho_c,
-                              float* __restrict__ out, int row, const int* __restrict__ status,
-                              int s_end) {
-    if (status[0] <= s_end) return;
-    const int m = blockIdx.x * 64 + threadIdx.x;
-    if (m >= n_mics) return;
-    float pp = 0.0f;
-    for (int k = mp_start[m]; k < mp_start[m + 1]; ++k) pp += mp_w[k] * p[mp_cell[k]];
-    float vx = 0.0f, vy = 0.0f, vz = 0.0f;
-    for (int k = mv_start[m]; k < mv_start[m + 1]; ++k) {
-        const float vf = v[mv_face[k]];
-        vx += wx[k] * vf;
-        vy += wy[k] * vf;
-        vz += wz[k] * vf;
+// (p*a + b*(rho c)*(v . axis)) * gain, into row `row`.  Skips when a step up
+// to and including s_end failed (the co-evolver returns before sampling).
+__global__ __launch_bounds__(256) void k_amr_mic_mix(
+    const int* __restrict__ mp_start, const int* __restrict__ mp_cell,
+    const float* __restrict__ mp_w, const int* __restrict__ mv_start,
+    const int* __restrict__ mv_face, const float* __restrict__ wx, const float* __restrict__ wy,
+    const float* __restrict__ wz, const float* __restrict__ p, const float* __restrict__ v,
+    int n_mics, const float* __restrict__ mix, float rho_c, float* __restrict__ out, int row,
+    const int* __restrict__ status, int s_end) {
+    if (status[0] <= s_end) return;                  // uniform over the block
+    const int m = blockIdx.x;
+    const MicSums r = mic_sums(m, mp_start, mp_cell, mp_w, mv_start, mv_face, wx, wy, wz, p, v);
+    if (threadIdx.x == 0) {
+        const float* q = mix + 6 * m;                // a, b, axis x, y, z, gain
+        float sig = r.p * q[0];
+        if (q[1] != 0.0f) {
+            const float vn = r.vx * q[2] + r.vy * q[3] + r.vz * q[4];
+            sig += q[1] * rho_c * vn;
+        }
+        out[(size_t)row * n_mics + m] = sig * q[5];
     }
-    const float* q = mix + 6 * m;                    // a, b, axis x, y, z, gain
-    float sig = pp * q[0];
-    if (q[1] != 0.0f) {
-        const float vn = vx * q[2] + vy * q[3] + vz * q[4];
-        sig += q[1] * rho_c * vn;
-    }
-    out[(size_t)row * n_mics + m] = sig * q[5];
 }
 
 constexpr int kGraphSteps = 1024;
@@ -1031,7 +1061,7 @@ int step_impl(AcousticAMRState* st, int n, float* ms_total, float* ms_faces, flo
         if (mix && (s + 1) % mix->substeps == 0) {
             const int* I = st->d_mint;
             const float* F = st->d_mflt;
-            k_amr_mic_mix<<<(st->n_mics + 63) / 64, 64, 0, st->stream>>>(
+            k_amr_mic_mix<<<st->n_mics, 256, 0, st->stream>>>(
                 I, I + st->o_mpc, F, I + st->o_mvs, I + st->o_mvf, F + st->o_mwx, F + st->o_mwy,
                 F + st->o_mwz, st->d_p, st->d_v, st->n_mics, mix->mix, mix->rho_c, mix->out,
                 (s + 1) / mix->substeps - 1, st->d_status, s);
@@ -1811,7 +1841,7 @@ SK_API int amr_sample_mics(const AcousticAMRState* cst, int n_mics, float* op, f
     AcousticAMRState* st = const_cast<AcousticAMRState*>(cst);
     const int* I = st->d_mint;
     const float* F = st->d_mflt;
-    k_amr_mics<<<(n_mics + 63) / 64, 64, 0, st->stream>>>(
+    k_amr_mics<<<n_mics, 256, 0, st->stream>>>(
         I, I + st->o_mpc, F, I + st->o_mvs, I + st->o_mvf, F + st->o_mwx, F + st->o_mwy,
         F + st->o_mwz, st->d_p, st->d_v, n_mics, st->d_mout);
     const bool vel = ovx && ovy && ovz;
