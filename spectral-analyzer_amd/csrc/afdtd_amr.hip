@@ -246,6 +246,7 @@ __global__ __launch_bounds__(256) void k_amr_faces8_c(const int2* __restrict__ f
 // instead of scattering loads over HBM lines.  Negating the face weight is
 // exact, so +-w[f] equals the reference's pre-signed CSR weight.
 constexpr int kCellTile = 512;                       // cells per 256-thread block
+constexpr int kRowMax = 8;                           // CSR rows walked unrolled up to this
 constexpr int kCsrStage = 4096;
 
 // Accumulate one CSR row entry (the reference's signed weight times v).
@@ -286,7 +287,23 @@ __global__ __launch_bounds__(256) void k_amr_cells(const float4* __restrict__ ce
     const int b0 = vb ? start[cb] : 0, b1 = vb ? start[cb + 1] : 0;
     const int na = a1 - a0, nb = b1 - b0;
     float da = 0.0f, db = 0.0f;
-    if (staged) {
+    if (staged && na <= kRowMax && nb <= kRowMax) {
+        // rows of up to kRowMax faces (six on a box grid): every gather of
+        // both rows is issued before the first add, then each row is summed
+        // in CSR order (:1166-1168); on small grids, where a launch is a
+        // chain of dependent loads, this shortens the chain
+        float ta[kRowMax], tb[kRowMax];
+#pragma unroll
+        for (int j = 0; j < kRowMax; ++j) {
+            ta[j] = j < na ? csr_term(s_csr[a0 - k_lo + j], w, v) : 0.0f;
+            tb[j] = j < nb ? csr_term(s_csr[b0 - k_lo + j], w, v) : 0.0f;
+        }
+#pragma unroll
+        for (int j = 0; j < kRowMax; ++j) {
+            if (j < na) da += ta[j];
+            if (j < nb) db += tb[j];
+        }
+    } else if (staged) {
         for (int j = 0; j < max(na, nb); ++j) {                      // :1166-1168
             if (j < na) da += csr_term(s_csr[a0 - k_lo + j], w, v);
             if (j < nb) db += csr_term(s_csr[b0 - k_lo + j], w, v);
