@@ -210,20 +210,12 @@ __global__ __launch_bounds__(256) void k_amr_faces_c(const int2* __restrict__ fc
 
 __device__ inline int stencil_cell(int neg, int d) { return d == (int)0x80000000 ? 0 : neg + d; }
 
-__global__ __launch_bounds__(256) void k_amr_faces8_c(const int2* __restrict__ fc,
-                                                      const int4* __restrict__ cls,
-                                                      float* __restrict__ v,
-                                                      const float* __restrict__ p, int nf,
-                                                      float dt_over_rho,
-                                                      const int* __restrict__ status, int s) {
-    if (status[0] < s) return;
-    const int f = xcd_tile(gridDim.x) * 256 + threadIdx.x;
-    if (f >= nf) return;
-    const int2 r = fc[f];
-    const int4* c = cls + 5 * (size_t)r.y;
+// Two faces per lane over 512-face tiles: both faces' 16 gathers are in
+// flight together.
+__device__ inline float faces8_grad(const int4* __restrict__ c, int n,
+                                    const float* __restrict__ p, float* vd) {
     const int4 a = c[0], b = c[1], wa = c[2], wb = c[3];
-    const float vd = __int_as_float(c[4].x);
-    const int n = r.x;
+    *vd = __int_as_float(c[4].x);
     const float pa[8] = {p[stencil_cell(n, a.x)], p[stencil_cell(n, a.y)], p[stencil_cell(n, a.z)],
                          p[stencil_cell(n, a.w)], p[stencil_cell(n, b.x)], p[stencil_cell(n, b.y)],
                          p[stencil_cell(n, b.z)], p[stencil_cell(n, b.w)]};
@@ -236,7 +228,27 @@ __global__ __launch_bounds__(256) void k_amr_faces8_c(const int2* __restrict__ f
     g += __int_as_float(wb.y) * pa[5];
     g += __int_as_float(wb.z) * pa[6];
     g += __int_as_float(wb.w) * pa[7];
-    v[f] = (v[f] - dt_over_rho * g) * vd;
+    return g;
+}
+
+__global__ __launch_bounds__(256) void k_amr_faces8_c(const int2* __restrict__ fc,
+                                                      const int4* __restrict__ cls,
+                                                      float* __restrict__ v,
+                                                      const float* __restrict__ p, int nf,
+                                                      float dt_over_rho,
+                                                      const int* __restrict__ status, int s) {
+    if (status[0] < s) return;
+    const int f0 = xcd_tile(gridDim.x) * kFaceTile + threadIdx.x;
+    const int f1 = f0 + 256;
+    // lanes past the last face evaluate face 0's stencil (in bounds) and store nothing
+    const int2 r0 = fc[f0 < nf ? f0 : 0];
+    const int2 r1 = fc[f1 < nf ? f1 : 0];
+    const float v0 = f0 < nf ? v[f0] : 0.0f, v1 = f1 < nf ? v[f1] : 0.0f;
+    float vd0, vd1;
+    const float g0 = faces8_grad(cls + 5 * (size_t)r0.y, r0.x, p, &vd0);
+    const float g1 = faces8_grad(cls + 5 * (size_t)r1.y, r1.x, p, &vd1);
+    if (f0 < nf) v[f0] = (v0 - dt_over_rho * g0) * vd0;
+    if (f1 < nf) v[f1] = (v1 - dt_over_rho * g1) * vd1;
 }
 
 // One 256-thread block updates 512 consecutive cells.  The block's CSR rows
@@ -1025,7 +1037,7 @@ int step_impl(AcousticAMRState* st, int n, float* ms_total, float* ms_faces, flo
     for (int s = 0; s < n; ++s) {
         if (ms_total) (void)hipEventRecord(ev[2 * (size_t)s], st->stream);
         if (st->order == 8 && st->compact)
-            k_amr_faces8_c<<<blocks(st->nf), 256, 0, st->stream>>>(
+            k_amr_faces8_c<<<(st->nf + kFaceTile - 1) / kFaceTile, 256, 0, st->stream>>>(
                 st->d_fc, st->d_fcls, st->d_v, st->d_p, st->nf, dt_over_rho, st->d_status, s);
         else if (st->order == 8)
             k_amr_faces8<<<blocks(st->nf), 256, 0, st->stream>>>(
