@@ -119,6 +119,14 @@ namespace {
 
 void set_stage(const char* s, const char* m) { g_amr_stage = s; g_amr_msg = m; }
 
+// Drops the captured block graph.  Called before any buffer a captured
+// launch references is freed or replaced, so that no failure path between
+// the free and the bump of `gen` can leave a graph pointing at freed memory.
+void drop_graph(AcousticAMRState* st) {
+    if (st->gexec) (void)hipGraphExecDestroy(st->gexec);
+    st->gexec = nullptr;
+}
+
 // Blocks are dispatched round-robin over the 8 XCDs (block b -> XCD b % 8);
 // give each XCD a contiguous range of tiles instead, so the neighbour
 // pressures and velocities one tile gathers were most likely brought into
@@ -951,6 +959,7 @@ hipError_t build_compact(AcousticAMRState* st, const std::vector<int4>& face,
 
 // Pack the per-cell and per-face coefficient records and upload them.
 hipError_t upload_coeffs(AcousticAMRState* st) {
+    drop_graph(st);
     std::vector<float4> cell(st->nc);
     for (int i = 0; i < st->nc; ++i) {
         float kf;
@@ -1539,6 +1548,7 @@ SK_API int amr_setup_plate(AcousticAMRState* st, int Nx, int Ny, float pdx, cons
     F.insert(F.end(), bl_w + b0, bl_w + b0 + lb);
 
     // replace any previous plate
+    drop_graph(st);
     for (int b = 0; b < 3; ++b) {
         if (st->d_pw[b]) (void)hipFree(st->d_pw[b]);
         st->d_pw[b] = nullptr;
@@ -1760,6 +1770,7 @@ static hipError_t upload_mics(AcousticAMRState* st) {
     st->o_mwx = np;
     st->o_mwy = np + nv;
     st->o_mwz = np + 2 * nv;
+    drop_graph(st);
     if (st->d_mint) (void)hipFree(st->d_mint);
     if (st->d_mflt) (void)hipFree(st->d_mflt);
     if (st->d_mout) (void)hipFree(st->d_mout);

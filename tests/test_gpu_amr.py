@@ -351,7 +351,8 @@ def test_record_mic_samples_bitwise():
 def test_record_mic_samples_graph_replays():
     """Blocks of the same shape replay the captured launch sequence: three
     blocks in a row with eager work between them (an injection, a border
-    change that forces a new capture) each equal the oracle bit for bit."""
+    change and a second mic set-up that reallocates the mic buffers, both
+    forcing a new capture) each equal the oracle bit for bit."""
     from afdtd.amr import AcousticAMR, AcousticAMRGrid
     print("amr: recorded mic samples, graph replays", flush=True)
     dx = 0.01
@@ -377,13 +378,20 @@ def test_record_mic_samples_graph_replays():
             rows.append(sig * gain)
         return np.stack(rows)
 
-    for blk in range(4):
+    for blk in range(6):
         if blk == 1:
             for m in (g, o):
                 m.inject_pressure_nearest([0.05, 0.04, 0.1], -0.5)
         if blk == 2:
             for m in (g, o):
                 m.set_border_condition(0, 3.0, 0.0, 0.0, 2, box.bounds_min, box.bounds_max)
+        if blk == 4:
+            # one mic now; its stencil keeps the old entries and appends the new ones
+            extra = np.array([[0.062, 0.043, 0.094]], np.float32)
+            for m in (g, o):
+                m.setup_mic_samplers(extra, [[0, 1, 0]], True)
+            axis = np.float32([[0, 1, 0]])
+            pa, pb, gain = pa[:1], np.float32([0.5]), gain[:1]
         out, rc, done = g.record_mic_samples(7, 3, pa, pb, axis, gain, rho_c)
         assert rc == 0 and done == 7
         want = oracle_block(7, 3)
