@@ -137,7 +137,8 @@ __device__ inline int xcd_tile(int nblk) {
     return x * q + min(x, r) + i;
 }
 
-constexpr int kFaceTile = 512;                       // faces per 256-thread block
+constexpr int kFacesPerLane = 2;                     // direct order-2 face kernel
+constexpr int kFaceTileD = 256 * kFacesPerLane;
 
 __global__ __launch_bounds__(256) void k_amr_faces(const int4* __restrict__ face,
                                                    float* __restrict__ v,
@@ -145,21 +146,28 @@ __global__ __launch_bounds__(256) void k_amr_faces(const int4* __restrict__ face
                                                    float dt_over_rho,
                                                    const int* __restrict__ status, int s) {
     if (status[0] < s) return;                       // an earlier step of this call failed
-    const int f0 = xcd_tile(gridDim.x) * kFaceTile + threadIdx.x;
-    const int f1 = f0 + 256;
-    // both faces' loads are issued before either update
-    int4 r0 = make_int4(0, 0, 0, 0), r1 = r0;
-    float v0 = 0.0f, v1 = 0.0f;
-    if (f0 < nf) { r0 = face[f0]; v0 = v[f0]; }
-    if (f1 < nf) { r1 = face[f1]; v1 = v[f1]; }
-    const float a0 = p[r0.y], b0 = p[r0.x], a1 = p[r1.y], b1 = p[r1.x];
-    if (f0 < nf) {
-        const float g = (a0 - b0) * __int_as_float(r0.z);            // :1149-1150
-        v[f0] = (v0 - dt_over_rho * g) * __int_as_float(r0.w);        // :1151
+    constexpr int U = kFacesPerLane;
+    const int fb = xcd_tile(gridDim.x) * kFaceTileD + threadIdx.x;
+    // every face's loads are issued before any update
+    int4 r[U];
+    float vo[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int f = fb + 256 * u;
+        r[u] = make_int4(0, 0, 0, 0);
+        vo[u] = 0.0f;
+        if (f < nf) { r[u] = face[f]; vo[u] = v[f]; }
     }
-    if (f1 < nf) {
-        const float g = (a1 - b1) * __int_as_float(r1.z);
-        v[f1] = (v1 - dt_over_rho * g) * __int_as_float(r1.w);
+    float a[U], b[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) { a[u] = p[r[u].y]; b[u] = p[r[u].x]; }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int f = fb + 256 * u;
+        if (f < nf) {
+            const float g = (a[u] - b[u]) * __int_as_float(r[u].z);    // :1149-1150
+            v[f] = (vo[u] - dt_over_rho * g) * __int_as_float(r[u].w); // :1151
+        }
     }
 }
 
@@ -195,6 +203,9 @@ __global__ __launch_bounds__(256) void k_amr_faces8(const int4* __restrict__ sc,
 // Compact-record variants of the face kernels: a face stores {neg, class};
 // the class table (a few hundred bytes on a box grid, cache resident)
 // holds the rest of the record, bit for bit the direct values.
+constexpr int kFacesPerLaneC = 4;                    // compact order-2 face kernel
+constexpr int kFaceTileC = 256 * kFacesPerLaneC;
+
 __global__ __launch_bounds__(256) void k_amr_faces_c(const int2* __restrict__ fc,
                                                      const int4* __restrict__ cls,
                                                      float* __restrict__ v,
@@ -202,24 +213,41 @@ __global__ __launch_bounds__(256) void k_amr_faces_c(const int2* __restrict__ fc
                                                      float dt_over_rho,
                                                      const int* __restrict__ status, int s) {
     if (status[0] < s) return;
-    const int f0 = xcd_tile(gridDim.x) * kFaceTile + threadIdx.x;
-    const int f1 = f0 + 256;
-    int2 r0 = make_int2(0, 0), r1 = r0;
-    float v0 = 0.0f, v1 = 0.0f;
-    if (f0 < nf) { r0 = fc[f0]; v0 = v[f0]; }
-    if (f1 < nf) { r1 = fc[f1]; v1 = v[f1]; }
-    const int4 c0 = cls[r0.y], c1 = cls[r1.y];
-    // lanes past the last face read cell 0 only (class 0's offset may be negative)
-    const int q0 = f0 < nf ? r0.x + c0.x : 0, q1 = f1 < nf ? r1.x + c1.x : 0;
-    const float a0 = p[q0], b0 = p[r0.x], a1 = p[q1], b1 = p[r1.x];
-    if (f0 < nf) v[f0] = (v0 - dt_over_rho * ((a0 - b0) * __int_as_float(c0.y))) * __int_as_float(c0.z);
-    if (f1 < nf) v[f1] = (v1 - dt_over_rho * ((a1 - b1) * __int_as_float(c1.y))) * __int_as_float(c1.z);
+    constexpr int U = kFacesPerLaneC;
+    const int fb = xcd_tile(gridDim.x) * kFaceTileC + threadIdx.x;
+    // every face's loads are issued before any update
+    int2 r[U];
+    float vo[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int f = fb + 256 * u;
+        r[u] = make_int2(0, 0);
+        vo[u] = 0.0f;
+        if (f < nf) { r[u] = fc[f]; vo[u] = v[f]; }
+    }
+    int4 c[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) c[u] = cls[r[u].y];
+    float a[U], b[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        // lanes past the last face read cell 0 only (class 0's offset may be negative)
+        const int q = fb + 256 * u < nf ? r[u].x + c[u].x : 0;
+        a[u] = p[q];
+        b[u] = p[r[u].x];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int f = fb + 256 * u;
+        if (f < nf)
+            v[f] = (vo[u] - dt_over_rho * ((a[u] - b[u]) * __int_as_float(c[u].y))) *
+                   __int_as_float(c[u].z);
+    }
 }
 
 __device__ inline int stencil_cell(int neg, int d) { return d == (int)0x80000000 ? 0 : neg + d; }
 
-// Two faces per lane over 512-face tiles: both faces' 16 gathers are in
-// flight together.
+// Several faces per lane: every face's 8 gathers are issued before any sum.
 __device__ inline float faces8_grad(const int4* __restrict__ c, int n,
                                     const float* __restrict__ p, float* vd) {
     const int4 a = c[0], b = c[1], wa = c[2], wb = c[3];
@@ -239,6 +267,9 @@ __device__ inline float faces8_grad(const int4* __restrict__ c, int n,
     return g;
 }
 
+constexpr int kFacesPerLane8C = 2;
+constexpr int kFaceTile8C = 256 * kFacesPerLane8C;
+
 __global__ __launch_bounds__(256) void k_amr_faces8_c(const int2* __restrict__ fc,
                                                       const int4* __restrict__ cls,
                                                       float* __restrict__ v,
@@ -246,17 +277,25 @@ __global__ __launch_bounds__(256) void k_amr_faces8_c(const int2* __restrict__ f
                                                       float dt_over_rho,
                                                       const int* __restrict__ status, int s) {
     if (status[0] < s) return;
-    const int f0 = xcd_tile(gridDim.x) * kFaceTile + threadIdx.x;
-    const int f1 = f0 + 256;
+    constexpr int U = kFacesPerLane8C;
+    const int fb = xcd_tile(gridDim.x) * kFaceTile8C + threadIdx.x;
     // lanes past the last face evaluate face 0's stencil (in bounds) and store nothing
-    const int2 r0 = fc[f0 < nf ? f0 : 0];
-    const int2 r1 = fc[f1 < nf ? f1 : 0];
-    const float v0 = f0 < nf ? v[f0] : 0.0f, v1 = f1 < nf ? v[f1] : 0.0f;
-    float vd0, vd1;
-    const float g0 = faces8_grad(cls + 5 * (size_t)r0.y, r0.x, p, &vd0);
-    const float g1 = faces8_grad(cls + 5 * (size_t)r1.y, r1.x, p, &vd1);
-    if (f0 < nf) v[f0] = (v0 - dt_over_rho * g0) * vd0;
-    if (f1 < nf) v[f1] = (v1 - dt_over_rho * g1) * vd1;
+    int2 r[U];
+    float vo[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int f = fb + 256 * u;
+        r[u] = fc[f < nf ? f : 0];
+        vo[u] = f < nf ? v[f] : 0.0f;
+    }
+    float g[U], vd[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) g[u] = faces8_grad(cls + 5 * (size_t)r[u].y, r[u].x, p, &vd[u]);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int f = fb + 256 * u;
+        if (f < nf) v[f] = (vo[u] - dt_over_rho * g[u]) * vd[u];
+    }
 }
 
 // One 256-thread block updates 512 consecutive cells.  The block's CSR rows
@@ -1046,17 +1085,17 @@ int step_impl(AcousticAMRState* st, int n, float* ms_total, float* ms_faces, flo
     for (int s = 0; s < n; ++s) {
         if (ms_total) (void)hipEventRecord(ev[2 * (size_t)s], st->stream);
         if (st->order == 8 && st->compact)
-            k_amr_faces8_c<<<(st->nf + kFaceTile - 1) / kFaceTile, 256, 0, st->stream>>>(
+            k_amr_faces8_c<<<(st->nf + kFaceTile8C - 1) / kFaceTile8C, 256, 0, st->stream>>>(
                 st->d_fc, st->d_fcls, st->d_v, st->d_p, st->nf, dt_over_rho, st->d_status, s);
         else if (st->order == 8)
             k_amr_faces8<<<blocks(st->nf), 256, 0, st->stream>>>(
                 st->d_sc, st->d_sw, st->d_face, st->d_v, st->d_p, st->nf, dt_over_rho,
                 st->d_status, s);
         else if (st->compact)
-            k_amr_faces_c<<<(st->nf + kFaceTile - 1) / kFaceTile, 256, 0, st->stream>>>(
+            k_amr_faces_c<<<(st->nf + kFaceTileC - 1) / kFaceTileC, 256, 0, st->stream>>>(
                 st->d_fc, st->d_fcls, st->d_v, st->d_p, st->nf, dt_over_rho, st->d_status, s);
         else
-            k_amr_faces<<<(st->nf + kFaceTile - 1) / kFaceTile, 256, 0, st->stream>>>(
+            k_amr_faces<<<(st->nf + kFaceTileD - 1) / kFaceTileD, 256, 0, st->stream>>>(
                 st->d_face, st->d_v, st->d_p, st->nf, dt_over_rho, st->d_status, s);
         if (st->plate_on && st->n_bc > 0) {                      // :1157
             const float* w = st->d_pw[(st->prot + s + 1) % 3];
