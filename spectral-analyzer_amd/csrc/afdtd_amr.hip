@@ -345,6 +345,11 @@ __global__ __launch_bounds__(256) void k_amr_cells(const float4* __restrict__ ce
     const int a0 = va ? start[ca] : 0, a1 = va ? start[ca + 1] : 0;
     const int b0 = vb ? start[cb] : 0, b1 = vb ? start[cb + 1] : 0;
     const int na = a1 - a0, nb = b1 - b0;
+    // the cells' own records and pressures load alongside the row gathers
+    float4 ma = make_float4(0.0f, 0.0f, 0.0f, 0.0f), mb = ma;
+    float pa = 0.0f, pb = 0.0f;
+    if (va) { ma = COMPACT ? cell[cell_class[ca]] : cell[ca]; pa = p[ca]; }
+    if (vb) { mb = COMPACT ? cell[cell_class[cb]] : cell[cb]; pb = p[cb]; }
     float da = 0.0f, db = 0.0f;
     if (staged && na <= kRowMax && nb <= kRowMax) {
         // rows of up to kRowMax faces (six on a box grid): every gather of
@@ -375,16 +380,14 @@ __global__ __launch_bounds__(256) void k_amr_cells(const float4* __restrict__ ce
     }
     bool bad = false;
     if (va) {
-        const float4 m = COMPACT ? cell[cell_class[ca]] : cell[ca];
-        float pn = p[ca] * m.x - bulk * da * m.y * m.z;              // :1169-1170
-        if (__float_as_int(m.w) & 1) pn = 0.0f;                      // :1174-1176
+        float pn = pa * ma.x - bulk * da * ma.y * ma.z;              // :1169-1170
+        if (__float_as_int(ma.w) & 1) pn = 0.0f;                     // :1174-1176
         p[ca] = pn;
         bad = !isfinite(pn) || (check_max && fabsf(pn) > 2e3f);      // :1181-1192
     }
     if (vb) {
-        const float4 m = COMPACT ? cell[cell_class[cb]] : cell[cb];
-        float pn = p[cb] * m.x - bulk * db * m.y * m.z;
-        if (__float_as_int(m.w) & 1) pn = 0.0f;
+        float pn = pb * mb.x - bulk * db * mb.y * mb.z;
+        if (__float_as_int(mb.w) & 1) pn = 0.0f;
         p[cb] = pn;
         bad = bad || !isfinite(pn) || (check_max && fabsf(pn) > 2e3f);
     }
