@@ -30,6 +30,8 @@ typedef struct SUF(OrcState) {
     REAL *vxd, *vyd, *vzd;  /* per-face damping multipliers                 */
     REAL *fx, *fy, *fz;     /* per-face open fractions                      */
     int has_frac;
+    int *cfx, *cfy, *cfz;   /* closed-face lists (frac == 0), :297-323        */
+    int ncfx, ncfy, ncfz;
     REAL *rho, *K, *kap;    /* per-cell density, bulk, K*dt/dx (hetero)      */
     REAL *bu;               /* per-cell buoyancy 1/rho (hetero)              */
     REAL dtdx;              /* dt/dx                                         */
@@ -74,7 +76,31 @@ SK_EXPORT void SUF(orc_destroy)(OST* s)
     free(s->act); free(s->above); free(s->below); free(s->b13); free(s->vzb); free(s->vza);
     free(s->bidx); free(s->bwgt); free(s->pidx); free(s->psig);
     free(s->ridx); free(s->rwgt); free(s->trace);
+    free(s->cfx); free(s->cfy); free(s->cfz);
     free(s);
+}
+
+/* Closed-face worklists (faces with frac == 0), as build_closed_face_lists
+ * (acoustic_fdtd.cpp:297-323); rebuilt whenever fractions change. */
+static int SUF(build_closed)(OST* s)
+{
+    int n[3] = {(s->Nx - 1) * s->Ny * s->Nz, s->Nx * (s->Ny - 1) * s->Nz,
+                s->Nx * s->Ny * (s->Nz - 1)};
+    REAL* F[3] = {s->fx, s->fy, s->fz};
+    int** L[3] = {&s->cfx, &s->cfy, &s->cfz};
+    int* C[3] = {&s->ncfx, &s->ncfy, &s->ncfz};
+    for (int a = 0; a < 3; ++a) {
+        free(*L[a]); *L[a] = NULL; *C[a] = 0;
+        int m = 0;
+        for (int q = 0; q < n[a]; ++q) if (F[a][q] == (REAL)0) ++m;
+        if (m == 0) continue;
+        *L[a] = (int*)malloc((size_t)m * sizeof(int));
+        if (!*L[a]) return -1;
+        m = 0;
+        for (int q = 0; q < n[a]; ++q) if (F[a][q] == (REAL)0) (*L[a])[m++] = q;
+        *C[a] = m;
+    }
+    return 0;
 }
 
 /* Construction — acoustic_fdtd.cpp:327-768 plus the extension options. */
@@ -283,6 +309,7 @@ SK_EXPORT OST* SUF(orc_create)(const OrcConfig* cf, const uint8_t* cell_type,
                 ++n;
             }
     }
+    if (SUF(build_closed)(s)) goto fail;
     return s;
 fail:
     SUF(orc_destroy)(s);
@@ -302,7 +329,7 @@ SK_EXPORT int SUF(orc_set_face_fractions)(OST* s, const float* fx, const float* 
     if (fy) for (int q = 0; q < nvy; ++q) s->fy[q] = (REAL)fy[q];
     if (fz) for (int q = 0; q < nvz; ++q) s->fz[q] = (REAL)fz[q];
     if (fx || fy || fz) s->has_frac = 1;
-    return 0;
+    return SUF(build_closed)(s);
 }
 
 /* acoustic_fdtd.cpp:798-830 */
@@ -538,13 +565,10 @@ static void SUF(pressure_step)(OST* s)
                 REAL g = SUF(gradient)(s, inner ? L - 1 : L, L, R, inner ? R + 1 : R, inner);
                 s->vz[v] = (s->vz[v] - SUF(face_coeff)(s, L, R) * g) * s->vzd[v];
             }
-    /* Closed faces (frac == 0) are zeroed (:1152-1154) */
-    {
-        int nvx = (Nx - 1) * Ny * Nz, nvy = Nx * (Ny - 1) * Nz, nvz = Nx * Ny * (Nz - 1);
-        for (int q = 0; q < nvx; ++q) if (s->fx[q] == (REAL)0) s->vx[q] = (REAL)0;
-        for (int q = 0; q < nvy; ++q) if (s->fy[q] == (REAL)0) s->vy[q] = (REAL)0;
-        for (int q = 0; q < nvz; ++q) if (s->fz[q] == (REAL)0) s->vz[q] = (REAL)0;
-    }
+    /* Closed faces (frac == 0) are zeroed from the worklists (:1152-1154) */
+    for (int q = 0; q < s->ncfx; ++q) s->vx[s->cfx[q]] = (REAL)0;
+    for (int q = 0; q < s->ncfy; ++q) s->vy[s->cfy[q]] = (REAL)0;
+    for (int q = 0; q < s->ncfz; ++q) s->vz[s->cfz[q]] = (REAL)0;
     /* Plate velocity injection (:1158-1163) */
     for (int n = 0; n < s->nact; ++n) {
         int p = s->act[n];
@@ -555,10 +579,23 @@ static void SUF(pressure_step)(OST* s)
     /* Pressure update (:1176-1234): every non-solid cell; solid cells keep 0. */
 #pragma omp parallel for schedule(static)
     for (int i = 0; i < Nx; ++i)
-        for (int j = 0; j < Ny; ++j)
+        for (int j = 0; j < Ny; ++j) {
+            const int yee_run = s->order == 2 && !s->has_frac && i > 0 && i < Nx - 1
+                              && j > 0 && j < Ny - 1;
             for (int k = 0; k < Nz; ++k) {
                 int c = SUF(idx3)(s, i, j, k);
                 if (SUF(solid)(s->ct[c])) continue;
+                /* Interior cells of the Yee stencil without fractions: every
+                 * face exists, so the divergence is the same three differences
+                 * in the same order as divergence(), with direct offsets. */
+                if (yee_run && k > 0 && k < Nz - 1) {
+                    const REAL* X = s->vx; const REAL* Y = s->vy; const REAL* Z = s->vz;
+                    int ox = c, oy = k + Nz * (j + (Ny - 1) * i), oz = k + (Nz - 1) * (j + Ny * i);
+                    REAL dv = (X[ox] - X[ox - sx]) + (Y[oy] - Y[oy - Nz]) + (Z[oz] - Z[oz - 1]);
+                    REAL kap = s->hetero ? s->kap[c] : s->bulk_coeff;
+                    s->Ps[c] = (s->P[c] - kap * dv) * s->pdamp[c];
+                    continue;
+                }
                 REAL dv = SUF(divergence)(s, i, j, k, s->has_frac);
                 REAL kap = s->hetero ? s->kap[c] : s->bulk_coeff;
                 int edge = (i == 0) | (i == Nx - 1) | (j == 0) | (j == Ny - 1)
@@ -583,6 +620,7 @@ static void SUF(pressure_step)(OST* s)
                     s->Ps[c] = (s->P[c] - kap * dv) * s->pdamp[c];
                 }
             }
+        }
     REAL* t = s->P; s->P = s->Ps; s->Ps = t;                            /* :1236-1239 */
 }
 
@@ -668,6 +706,34 @@ SK_EXPORT int SUF(orc_get_plate)(const OST* s, REAL* w)
 SK_EXPORT int SUF(orc_get_cell_type)(const OST* s, uint8_t* out)
 {
     memcpy(out, s->ct, s->N);
+    return 0;
+}
+
+/* acoustic_fdtd.cpp:1744-1760: weighted sum of displacements from 0, term by
+ * term, skipping out-of-range indices. */
+SK_EXPORT int SUF(orc_sample_plate_weighted)(const OST* s, int n, const int* pidx,
+                                             const float* wgt, REAL* out)
+{
+    REAL acc = (REAL)0;
+    for (int i = 0; i < n; ++i) {
+        int p = pidx[i];
+        if (p < 0 || p >= s->Nplate) continue;
+        acc += s->w[p] * (REAL)wgt[i];
+    }
+    *out = acc;
+    return 0;
+}
+
+/* acoustic_fdtd.cpp:1332-1353: four-tap gather per point, left to right. */
+SK_EXPORT int SUF(orc_sample_plate_batch)(const OST* s, int n, const int* idx4,
+                                          const float* wgt4, REAL* out)
+{
+    for (int q = 0; q < n; ++q) {
+        const int* id = idx4 + 4 * q;
+        const float* g = wgt4 + 4 * q;
+        out[q] = (REAL)g[0] * s->w[id[0]] + (REAL)g[1] * s->w[id[1]]
+               + (REAL)g[2] * s->w[id[2]] + (REAL)g[3] * s->w[id[3]];
+    }
     return 0;
 }
 

@@ -77,6 +77,10 @@ def lib():
             g("orc_precompute_velocity_samplers").argtypes = [C.c_void_p, C.c_int, C.c_void_p] \
                 + [C.c_void_p] * 6
             g("orc_energy").argtypes = [C.c_void_p, C.c_void_p]
+            g("orc_sample_plate_weighted").argtypes = [C.c_void_p, C.c_int, C.c_void_p,
+                                                       C.c_void_p, C.c_void_p]
+            g("orc_sample_plate_batch").argtypes = [C.c_void_p, C.c_int, C.c_void_p,
+                                                    C.c_void_p, C.c_void_p]
         _lib = L
     return _lib
 
@@ -253,6 +257,20 @@ class OracleFDTD:
             arrs += [np.zeros(8 * n, np.int32), np.zeros(8 * n, np.float32)]
         self._f("orc_precompute_velocity_samplers")(self._h, n, _ptr(xyz), *[_ptr(a) for a in arrs])
         return tuple(arrs)
+
+    def sample_plate_weighted(self, plate_indices, weights):
+        i = np.ascontiguousarray(plate_indices, np.int32).ravel()
+        w = np.ascontiguousarray(weights, np.float32).ravel()
+        out = np.zeros(1, self.dtype)
+        self._f("orc_sample_plate_weighted")(self._h, i.size, _ptr(i), _ptr(w), _ptr(out))
+        return out[0]
+
+    def sample_plate_displacement_batch(self, idx4, wgt4) -> np.ndarray:
+        i = np.ascontiguousarray(idx4, np.int32).ravel()
+        w = np.ascontiguousarray(wgt4, np.float32).ravel()
+        out = np.zeros(i.size // 4, self.dtype)
+        self._f("orc_sample_plate_batch")(self._h, out.size, _ptr(i), _ptr(w), _ptr(out))
+        return out
 
     def energy(self) -> np.ndarray:
         out = np.zeros(3, np.float64)

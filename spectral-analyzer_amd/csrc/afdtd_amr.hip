@@ -107,12 +107,15 @@ struct AcousticAMRState {
     // small upload buffers (drives, forces, sampler stencils)
     void* d_tmp = nullptr;
     size_t tmp_bytes = 0;
-    // Captured launch sequence of a block of recorded audio samples, replayed
-    // while its key (block shape, plate rotation, device pointers and
-    // coefficients via `gen`) is unchanged.
+    // Captured launch sequences of a block of recorded audio samples, replayed
+    // while their key (block shape, device pointers and coefficients via
+    // `gen`) is unchanged.  The plate's three displacement buffers rotate by
+    // the block's step count, so a plated grid keeps one graph per rotation
+    // (slot prot % 3): blocks whose step count is not a multiple of 3 cycle
+    // through the slots instead of capturing anew every call.
     uint64_t gen = 0;                        // bumped whenever captured arguments change
-    hipGraphExec_t gexec = nullptr;
-    std::array<uint64_t, 6> gkey{};
+    hipGraphExec_t gexec[3] = {nullptr, nullptr, nullptr};
+    std::array<uint64_t, 6> gkey[3]{};
 };
 
 namespace {
@@ -123,8 +126,10 @@ void set_stage(const char* s, const char* m) { g_amr_stage = s; g_amr_msg = m; }
 // launch references is freed or replaced, so that no failure path between
 // the free and the bump of `gen` can leave a graph pointing at freed memory.
 void drop_graph(AcousticAMRState* st) {
-    if (st->gexec) (void)hipGraphExecDestroy(st->gexec);
-    st->gexec = nullptr;
+    for (auto& g : st->gexec) {
+        if (g) (void)hipGraphExecDestroy(g);
+        g = nullptr;
+    }
 }
 
 // Blocks are dispatched round-robin over the 8 XCDs (block b -> XCD b % 8);
@@ -876,7 +881,7 @@ void destroy_state(AcousticAMRState* st) {
                     st->d_pflt, st->d_mint, st->d_mflt, st->d_mout, st->d_tmp};
     for (void* b : more)
         if (b) (void)hipFree(b);
-    if (st->gexec) (void)hipGraphExecDestroy(st->gexec);
+    drop_graph(st);
     if (st->stream) (void)hipStreamDestroy(st->stream);
     delete st;
 }
@@ -1071,13 +1076,14 @@ int step_impl(AcousticAMRState* st, int n, float* ms_total, float* ms_faces, flo
     // capturing a long block costs more than it saves, hence the limit.
     bool graph = false, replay = false;
     std::array<uint64_t, 6> gkey{};
+    const int slot = (st->plate_on && st->n_active > 0) ? st->prot % 3 : 0;
     if (mix && !ms_total && n <= kGraphSteps) {
-        gkey = {(uint64_t)n, (uint64_t)mix->substeps, (uint64_t)(st->prot % 3), st->gen,
+        gkey = {(uint64_t)n, (uint64_t)mix->substeps, (uint64_t)slot, st->gen,
                 (uint64_t)(uintptr_t)mix->out, (uint64_t)bits(mix->rho_c)};
         replay = true;
-        if (!st->gexec || st->gkey != gkey) {
-            if (st->gexec) (void)hipGraphExecDestroy(st->gexec);
-            st->gexec = nullptr;
+        if (!st->gexec[slot] || st->gkey[slot] != gkey) {
+            if (st->gexec[slot]) (void)hipGraphExecDestroy(st->gexec[slot]);
+            st->gexec[slot] = nullptr;
             if (hipStreamBeginCapture(st->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
                 return AMR_ERR_DEVICE;
             graph = true;
@@ -1152,15 +1158,15 @@ int step_impl(AcousticAMRState* st, int n, float* ms_total, float* ms_faces, flo
     if (graph) {
         hipGraph_t gr = nullptr;
         hipError_t ge = hipStreamEndCapture(st->stream, &gr);
-        if (ge == hipSuccess) ge = hipGraphInstantiate(&st->gexec, gr, nullptr, nullptr, 0);
+        if (ge == hipSuccess) ge = hipGraphInstantiate(&st->gexec[slot], gr, nullptr, nullptr, 0);
         if (gr) (void)hipGraphDestroy(gr);
         if (ge != hipSuccess) {
-            st->gexec = nullptr;
+            st->gexec[slot] = nullptr;
             return AMR_ERR_DEVICE;
         }
-        st->gkey = gkey;
+        st->gkey[slot] = gkey;
     }
-    if (replay && hipGraphLaunch(st->gexec, st->stream) != hipSuccess) return AMR_ERR_DEVICE;
+    if (replay && hipGraphLaunch(st->gexec[slot], st->stream) != hipSuccess) return AMR_ERR_DEVICE;
     if (ms_total) (void)hipEventRecord(ev[2 * (size_t)n], st->stream);
     int fail = INT_MAX;
     hipError_t e = hipGetLastError();
@@ -1506,6 +1512,7 @@ SK_API int amr_set_border_condition(AcousticAMRState* st, int mode, float sigma_
 
 static hipError_t ensure_tmp(AcousticAMRState* st, size_t bytes) {
     if (bytes <= st->tmp_bytes) return hipSuccess;
+    drop_graph(st);                 // captured mic blocks reference d_tmp
     if (st->d_tmp) (void)hipFree(st->d_tmp);
     st->d_tmp = nullptr;
     st->tmp_bytes = 0;
@@ -1956,6 +1963,7 @@ SK_API int amr_record_mic_samples(AcousticAMRState* st, int n_samples, int subst
         return SK_ERR_NULL_STATE;
     *n_done = 0;
     if (n_samples < 0 || substeps <= 0 || st->n_mics <= 0) return SK_ERR_DIM_MISMATCH;
+    if (n_samples > INT_MAX / substeps) return SK_ERR_DIM_MISMATCH;   // step count overflows
     if (n_samples == 0) return SK_OK;
     const int M = st->n_mics;
     std::vector<float> mix(6 * (size_t)M);

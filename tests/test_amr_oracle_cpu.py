@@ -620,3 +620,34 @@ def test_order8_stops_at_walls_and_conserves_flux():
     sc0 = float(np.sum(np.abs(o2.get_pressure()).astype(np.float64)))
     o2.step(300)
     assert abs(float(np.sum(o2.get_pressure().astype(np.float64))) - m0) < 1e-5 * sc0
+
+
+@pytest.mark.parametrize("mode, order, R", [(0, 3.0, 0.0), (1, 2.0, 0.6)])
+def test_border_sigma_closed_form_three_cells(mode, order, R):
+    """amr_set_border_condition (acoustic_amr.cpp:1318-1409) on three cells
+    in a row along x, two PML-typed at distances 0.02 and 0.06 from the
+    x- bound (all other bounds far), one air cell, n_pml = 10 at min_dx =
+    0.01 (depth 0.1).  By hand: sigma_i = s_max (1 - d_i/0.1)^order with
+    s_max = 3.45/(dt n_pml) (times 1 - R for a room panel); P_damp =
+    exp(-sigma dt), P_src = (1 - exp(-sigma dt))/(sigma dt); each face's
+    sigma is 2 (o_p sigma_n + o_n sigma_p)/(o_n + o_p) over the cells'
+    open fractions o (so at equal fractions the SUM of the two cell
+    values), V_damp = exp(-sigma_f dt)."""
+    cc = np.array([[0.02, 0.5, 0.5], [0.06, 0.5, 0.5], [0.10, 0.5, 0.5]])
+    ovf = np.array([0.3, 0.9, 0.6])
+    ct = np.array([3, 3, 0], np.uint8)
+    o = OracleAMR(cc, np.full(3, 1e-6), ovf, ct, np.array([0, 1], np.int32),
+                  np.array([1, 2], np.int32), np.full(2, 1e-4), np.ones(2), np.full(2, 0.04),
+                  343.0, 1.21, 0.01)
+    assert o.set_border_condition(mode, order, R, 0.0, 10, [0.0, 0.0, 0.0], [1.0, 1.0, 1.0]) == 0
+    pd, ps, vd = (x.astype(np.float64) for x in o.get_coeffs())
+    dt = float(np.float32(o.dt))
+    smax = 3.45 / (dt * 10) * (1.0 - R)
+    sig = np.array([smax * 0.8 ** order, smax * 0.4 ** order, 0.0])
+    np.testing.assert_allclose(pd, np.exp(-sig * dt), rtol=2e-6)
+    s = sig[:2] * dt
+    np.testing.assert_allclose(ps[:2], (1 - np.exp(-s)) / s, rtol=1e-4)
+    assert ps[2] == 1.0 and pd[2] == 1.0
+    sf = np.array([2 * (ovf[1] * sig[0] + ovf[0] * sig[1]) / (ovf[0] + ovf[1]),
+                   2 * (ovf[2] * sig[1] + ovf[1] * sig[2]) / (ovf[1] + ovf[2])])
+    np.testing.assert_allclose(vd, np.exp(-sf * dt), rtol=2e-6)

@@ -397,3 +397,71 @@ def test_record_mic_samples_graph_replays():
         want = oracle_block(7, 3)
         assert np.array_equal(out.view(np.uint32), want.view(np.uint32)), blk
         _same(g, o)
+
+
+def test_record_mic_samples_with_plate_graph_slots():
+    """The co-evolver's acoustic half with a plate attached: blocks of 5
+    samples x 4 sub-steps (20 steps, not a multiple of the plate's three
+    displacement buffers) with a bridge drive and a plate force injected
+    between blocks.  Consecutive blocks start at plate rotations 0, 2, 1, 0,
+    ... and replay the graph captured for their rotation; every block's mic
+    output, the fields and the plate displacement equal the oracle's step /
+    sample_mics / mix loop bit for bit."""
+    from test_amr_oracle_cpu import plated_box
+    print("amr: recorded mic samples with a plate, graph slots", flush=True)
+    rng = np.random.default_rng(48)
+    act = (rng.uniform(size=(12, 10)) < 0.85).astype(np.uint8)
+    grid, kw = plated_box(12, 10, 16, 6, active=act)
+    g, o = _pair_grid(grid)
+    flat = kw["plate_active_flat_idx"]
+    mics = np.array([[0.031, 0.052, 0.097], [0.083, 0.041, 0.032]], np.float32)
+    axis = np.array([[0, 0, 1], [0.6, 0, 0.8]], np.float32)
+    pa, pb, gain = np.float32([1.0, 0.4]), np.float32([0.0, 0.6]), np.float32([1.0, 2.5])
+    rho_c = np.float32(np.float32(1.21) * np.float32(343.0))
+    for m in (g, o):
+        m.setup_plate(**kw)
+        m.set_bridge_plate_sources(flat[[3, 4, 9]], [0.5, 0.3, 0.2])
+        m.setup_mic_samplers(mics, axis, True)
+        m.inject_pressure_nearest([0.061, 0.052, 0.113], 1.0)
+    n = 12 * 10
+    for blk in range(7):
+        d = rng.standard_normal(3).astype(np.float32)
+        for m in (g, o):
+            m.inject_bridge_drive(d, 0.01)
+            if blk % 3 == 1:
+                m.inject_plate_force(flat[[1, 2]], [0.6, 0.4], 0.02)
+        out, rc, done = g.record_mic_samples(5, 4, pa, pb, axis, gain, rho_c)
+        assert rc == 0 and done == 5
+        want = np.zeros((5, 2), np.float32)
+        for j in range(5):
+            o.step(4)
+            sp, vx, vy, vz = o.sample_mics()
+            sig = sp * pa
+            vn = vx * axis[:, 0] + vy * axis[:, 1] + vz * axis[:, 2]
+            sig = np.where(pb != 0, sig + (pb * rho_c) * vn, sig).astype(np.float32)
+            want[j] = sig * gain
+        assert np.array_equal(out.view(np.uint32), want.view(np.uint32)), blk
+        _same(g, o)
+        assert np.array_equal(_bits(g.get_plate_displacement(n)),
+                              _bits(o.get_plate_displacement(n))), blk
+    assert np.abs(o.get_plate_displacement(n)).max() > 0
+
+
+def test_irregular_topology_unstaged_csr_rows_bitwise():
+    """A topology whose 512-cell tiles hold more CSR entries than the cell
+    kernel stages in LDS (1200 cells, 7000 faces: mean degree 11.7), so both
+    the staged and the unstaged row walks run, against the oracle."""
+    from afdtd.amr import AcousticAMR
+    print("amr: irregular topology, unstaged CSR rows", flush=True)
+    t = random_topology(23, nc=1200, nf=7000, walls=True)
+    t["cell_volumes"] = t["cell_volumes"] * 3.0      # keep the order-2 step stable at this degree
+    g, o = AcousticAMR(**t), OracleAMR(**t)
+    assert g.n_csr_entries == o.n_csr_entries == 14000
+    rng = np.random.default_rng(23)
+    p = rng.standard_normal(g.n_cells).astype(np.float32)
+    v = (rng.standard_normal(g.n_faces) * 1e-3).astype(np.float32)
+    for m in (g, o):
+        m.set_state(p, v)
+        m.step(40)
+    _same(g, o)
+    assert np.abs(g.get_pressure()).max() > 0

@@ -292,3 +292,56 @@ def test_fused_unstable_restores_buffers(oracle_mod):
         g.step(401)
     assert g.step_count == o.step_count == 256
     _assert_fields_equal(g, o, "fused unstable")
+
+
+@pytest.mark.parametrize("hetero", [False, True])
+def test_energy_matches_oracle(oracle_mod, hetero):
+    """fdtd_energy (FP64 device reduction) against the oracle's energy terms
+    on identical fields: potential sum p^2/(2K) dV and kinetic sum
+    rho_f v^2/2 dV.  The kinetic terms use the same float inputs (the fields
+    are bitwise equal), so they agree to summation order; the potential term
+    divides by K recovered from kappa = K dt/dx on the device (K itself in
+    the oracle), a float rounding of 1e-7."""
+    Nx, Ny, Nz = 19, 23, 37
+    kw = dict(order=2, courant=0.5)
+    if hetero:
+        rng = np.random.default_rng(12)
+        kw["rho"] = (1.2 * np.exp(0.2 * rng.standard_normal(Nx * Ny * Nz))).astype(np.float32)
+        kw["bulk"] = (1.2 * 343.0 ** 2 * np.exp(0.2 * rng.standard_normal(Nx * Ny * Nz))).astype(np.float32)
+    g, o = _pair(oracle_mod, Nx, Ny, Nz, walls=0.04, n_pml=3, seed=12, **kw)
+    _random_state(o, g, seed=12)
+    g.step(9); o.step(9)
+    _assert_fields_equal(g, o, "energy run")
+    eg = g.energy()
+    eo = o.energy()                 # [P^n P^(n-1)/2K, kinetic, P^2/2K]
+    assert eo[2] > 0 and eo[1] > 0
+    assert abs(eg[0] - eo[2]) <= 1e-6 * eo[2], (eg[0], eo[2])
+    assert abs(eg[1] - eo[1]) <= 1e-12 * eo[1], (eg[1], eo[1])
+
+
+def test_plate_samplers_match_oracle(oracle_mod):
+    """fdtd_sample_plate_weighted and fdtd_sample_plate_displacement_batch on a
+    plate driven through the bridge, against the oracle's restatement, bit
+    for bit (out-of-range weighted indices are skipped by both)."""
+    g, o = _pair(oracle_mod, 20, 24, 16, plate=True, n_pml=2, seed=6, plate_iz=8)
+    Nz, Ny = 16, 24
+    src = np.array([(9 + 1) + Nz * (10 + Ny * 9), (9 + 1) + Nz * (12 + Ny * 11)], np.int32)
+    w = np.array([1.0, 0.5], np.float32)
+    for m in (g, o):
+        m.set_bridge_sources(src, w)
+    for s in range(25):
+        d = np.array([np.sin(0.3 * s), np.cos(0.2 * s)], np.float32) * 2e-3
+        for m in (g, o):
+            m.inject_bridge_drive(d, 1.0)
+            m.step(1)
+    rng = np.random.default_rng(61)
+    idx = rng.integers(-5, 20 * 24 + 5, 60).astype(np.int32)
+    wgt = rng.standard_normal(60).astype(np.float32)
+    a, b = np.float32(g.sample_plate_weighted(idx, wgt)), np.float32(o.sample_plate_weighted(idx, wgt))
+    assert a != 0 and a.view(np.uint32) == b.view(np.uint32), (a, b)
+    idx4 = rng.integers(0, 20 * 24, (33, 4)).astype(np.int32)
+    wgt4 = rng.uniform(0, 1, (33, 4)).astype(np.float32)
+    a = g.sample_plate_displacement_batch(idx4, wgt4)
+    b = o.sample_plate_displacement_batch(idx4, wgt4).astype(np.float32)
+    assert np.abs(b).max() > 0
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))

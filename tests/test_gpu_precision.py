@@ -8,12 +8,10 @@ segment; the exponent is the smallest e with max|v| * 2^-e < 2^15 (from the
 FP32 exponent field of max|v|: e = max(biased - 141, -128)) and the mantissa
 rint(v * 2^-e) clamped to +-32767.
 
-The default crop is 128 x 128 x 64 for 1500 steps; AFDTD_FULL_PRECISION=1
-runs the 256 x 256 x 128 crop for 3000 steps (about a minute of FP64 oracle
-time on the host).
+The FP64 comparison runs on the 256 x 256 x 128 crop for 3000 steps, the
+precision workload's own accuracy set-up (about a minute of FP64 oracle time
+on the host's cores).
 """
-import os
-
 import numpy as np
 import pytest
 
@@ -99,19 +97,23 @@ def test_bfp16_rejects_generic_only_features():
                      plate_active=np.zeros(Nx * Ny, np.uint8), n_pml=0, storage="bfp16")
 
 
-def _crop():
-    if os.environ.get("AFDTD_FULL_PRECISION"):
-        return 256, 256, 128, 3000
-    return 128, 128, 64, 1500
+# Receiver rel-L2 bound against FP64 and final-energy bound per storage on
+# the 256 x 256 x 128 crop after 3000 steps.  The error is set by the stored
+# mantissa: FP32 (24 bits) ~6e-6, bfp16 (15 bits below a shared row
+# exponent) ~7e-4, FP16 (11 bits) ~5e-3, BF16 (8 bits) ~3.4e-2; each bound
+# leaves about 1.5x headroom over the measured value and is below the next
+# wider format's error, so a storage that silently fell back to a narrower
+# one fails.
+BOUNDS = {"f32": (1e-5, 1e-5), "bfp16": (1e-3, 1e-4), "f16": (8e-3, 3e-3),
+          "bf16": (5e-2, 4e-2)}
 
 
 def test_storage_accuracy_vs_fp64(oracle_mod):
     """Receiver traces and final energy of every storage type against the
-    FP64 oracle on the precision crop.  FP32 <= 1e-5 and bfp16 <= 1e-3
-    receiver rel-L2 are asserted; BF16 / FP16 are reported (their 8- and
-    11-bit mantissas leave them above 1e-3 over this many steps)."""
+    FP64 oracle on the precision crop (256 x 256 x 128, 3000 steps), each
+    within the bound its mantissa width allows (BOUNDS)."""
     from afdtd import AcousticFDTD, workloads as W
-    Nx, Ny, Nz, steps = _crop()
+    Nx, Ny, Nz, steps = 256, 256, 128, 3000
     w = W.config4(Nx=Nx, Ny=Ny, Nz=Nz, steps=steps)
     ct = np.zeros(w.cells, np.uint8)
     pa = np.zeros(Nx * Ny, np.uint8)
@@ -141,11 +143,9 @@ def test_storage_accuracy_vs_fp64(oracle_mod):
         g.close()
         print(f"{st:6s} [{errs[st][2]}] {Nx}x{Ny}x{Nz} {steps} steps: receiver rel-L2 vs FP64 "
               f"{errs[st][0]:.3e}, final energy rel. diff {errs[st][1]:.3e}", flush=True)
-    assert errs["f32"][0] <= 1e-5
-    assert errs["bfp16"][0] <= 1e-3
-    assert errs["bfp16"][1] <= 1e-3
-    for st in ("bf16", "f16"):
-        assert np.isfinite(errs[st][0])
+    for st, (bt, be) in BOUNDS.items():
+        assert errs[st][0] <= bt, (st, errs[st])
+        assert errs[st][1] <= be, (st, errs[st])
 
 
 @pytest.mark.parametrize("storage", ["f16", "bf16", "f32"])

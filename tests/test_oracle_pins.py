@@ -393,3 +393,141 @@ def test_reference_dt_rule(oracle_mod):
     want = np.float32(np.float32(0.77) * np.float32(0.01)) / np.float32(np.float32(343.0) * np.float32(1.732050808))
     assert o.dt == float(want)
     assert abs(o.dt * 343.0 / 0.01 - 0.77 / np.sqrt(3)) < 1e-6
+
+
+def _plate_consts(o, mass=7.0, D=0.45, aM=2.0, bK=1e-5, dx=0.01):
+    """The plate constants of acoustic_fdtd.cpp:435-447 from the float
+    parameters of the C ABI, in float64: a = dt^2/m, bh = D dt^2/(m dx^4),
+    bk = bK/dt * bh, vd = 1 - aM dt."""
+    f = lambda v: float(np.float32(v))
+    dt, dx = f(o.dt), f(dx)
+    a = dt * dt / f(mass)
+    bh = f(D) * dt * dt / (f(mass) * dx ** 4)
+    return dict(dt=dt, dx=dx, a=a, bh=bh, bk=f(bK) / dt * bh, vd=1 - f(aM) * dt)
+
+
+def test_plate_air_coupling_one_column_known_answer(oracle_mod):
+    """One active plate node on a 1x1x5 column (cells 0,1 below, plate cell 2,
+    cells 3,4 above), order 2, no PML.  The node has no plate neighbours, so
+    its 13-point biharmonic is (20 - 4*8 + 4*2 + 4) w = 0 and the plate obeys
+    w' = w + vd (w - w_prev) + a ((P_below - P_above) + f) (:1026-1030): the
+    air load is the pressure difference across the plate, pushing it up when
+    the air below is at the higher pressure.  The two faces beside the plate
+    then carry the plate velocity (w' - w)/dt (:1158-1163), which removes
+    volume from the cell below and adds it to the cell above:
+      P0' = p0 + kap vc (p1 - p0)      P1' = p1 - kap (u + vc (p1 - p0))
+      P3' = p3 + kap (u + vc (p4 - p3))  P4' = p4 - kap vc (p4 - p3)
+    with u = a (p1 - p3)/dt.  The second step repeats the plate recurrence
+    with the new pressures."""
+    p = np.array([0.3, 1.7, 0.0, -0.4, 0.9])
+    o = _mk(oracle_mod, 1, 1, 5, n_pml=0, order=2, courant=0.4, plate_iz=2,
+            plate_active=np.ones(1, np.uint8))
+    assert list(o.get_cell_type()) == [0, 0, 2, 0, 0]
+    o.set_fields(P=p)
+    o.step(1)
+    k = _plate_consts(o)
+    dt = k["dt"]
+    vc = np.float64(np.float32(o.dt)) / (np.float64(np.float32(1.2)) * np.float64(np.float32(0.01)))
+    kap = np.float64(np.float32(1.2)) * np.float64(np.float32(343.0)) ** 2 * dt / np.float64(np.float32(0.01))
+    w1 = k["a"] * (p[1] - p[3])
+    np.testing.assert_allclose(o.get_plate_displacement().ravel(), [w1], rtol=1e-12)
+    u = w1 / dt
+    vz = o.get_velocity_z()
+    np.testing.assert_allclose(vz[1:3], [u, u], rtol=1e-12)
+    np.testing.assert_allclose(vz[[0, 3]], [-vc * (p[1] - p[0]), -vc * (p[4] - p[3])], rtol=1e-12)
+    want = [p[0] + kap * vc * (p[1] - p[0]), p[1] - kap * (u + vc * (p[1] - p[0])), 0.0,
+            p[3] + kap * (u + vc * (p[4] - p[3])), p[4] - kap * vc * (p[4] - p[3])]
+    Q = o.get_pressure_field().ravel()
+    np.testing.assert_allclose(Q, want, rtol=1e-11, atol=1e-15)
+    assert Q[1] < p[1] - kap * vc * (p[1] - p[0]) and Q[3] > p[3] + kap * vc * (p[4] - p[3])
+    o.step(1)
+    w2 = w1 + k["vd"] * w1 + k["a"] * (Q[1] - Q[3])
+    np.testing.assert_allclose(o.get_plate_displacement().ravel(), [w2], rtol=1e-11)
+    np.testing.assert_allclose(o.get_velocity_z()[1:3], [(w2 - w1) / dt] * 2, rtol=1e-10)
+
+
+def test_bridge_drive_known_answer(oracle_mod):
+    """fdtd_inject_bridge_drive (:1699-1721): bridge cell c = k + Nz p adds
+    force_scale * drive / dx^2 to the plate node p = c / Nz, whatever k is,
+    and the step consumes it: w1 = a s d / dx^2 at p only (no air load with
+    P = 0), then the node follows the free recurrence (the force buffer is
+    cleared by the step, :1045; walls above and below keep the air out)."""
+    Nx, Ny, Nz, piz = 14, 14, 8, 4
+    ct = np.zeros((Nx, Ny, Nz), np.uint8); ct[:, :, piz - 1] = 1; ct[:, :, piz + 1] = 1
+    pa = np.zeros((Nx, Ny), np.uint8); pa[2:12, 2:12] = 1
+    o = _mk(oracle_mod, Nx, Ny, Nz, n_pml=0, cell_type=ct.ravel(), plate_active=pa.ravel(),
+            plate_iz=piz)
+    p1, p2 = 7 + Ny * 6, 4 + Ny * 9
+    cells = [piz + 1 + Nz * p1, 2 + Nz * p1, piz + 1 + Nz * p2]
+    assert o.set_bridge_sources(cells, [0.5, -2.0, 1.0]) == 0
+    d = np.array([1.5, 0.5, -3.0], np.float32)
+    s_ = np.float32(0.8)
+    o.inject_bridge_drive(d, float(s_))
+    o.step(1)
+    k = _plate_consts(o)
+    f = lambda v: float(np.float32(v))
+    w = o.get_plate_displacement().ravel()
+    np.testing.assert_allclose(w[p1], k["a"] * f(s_) * (f(d[0]) + f(d[1])) / k["dx"] ** 2, rtol=1e-6)
+    np.testing.assert_allclose(w[p2], k["a"] * f(s_) * f(d[2]) / k["dx"] ** 2, rtol=1e-6)
+    assert np.count_nonzero(w) == 2
+    w1 = w.copy()
+    o.step(1)
+    w = o.get_plate_displacement().ravel()
+    s = (k["bh"] + k["bk"]) * w1[p1]
+    np.testing.assert_allclose(w[p1], w1[p1] * (1 + k["vd"]) - 20 * s, rtol=1e-9)
+
+
+@pytest.mark.parametrize("f0, f1", [(0.5, 1.0), (1.0, 0.25)])
+def test_fraction_weighted_divergence_known_answer(oracle_mod, f0, f1):
+    """The 6-cell line of test_edge_delta_one_step_known_answer with open
+    fractions on faces 0 and 1: the velocity update is unchanged (V = vc[1,
+    C2, 0, 0, 0]) and the divergence weighs each face by its fraction
+    (:894-925), so
+    P' = [1 - Cr^2 (C1 f0 + C2^2 f1), Cr^2 C1 (f0 - C2 f1), Cr^2 C2 (C1 f1 + f0), Cr^2 C2^2 f1, 0, 0]."""
+    o = _mk(oracle_mod, 6, 1, 1, n_pml=0)
+    o.set_face_fractions(np.array([f0, f1, 1, 1, 1], np.float32), None, None)
+    P = np.zeros(6); P[0] = 1.0
+    o.set_fields(P=P)
+    o.step(1)
+    cr2 = _cr2(o)
+    want = [1 - cr2 * (C1 * f0 + C2 ** 2 * f1), cr2 * C1 * (f0 - C2 * f1),
+            cr2 * C2 * (C1 * f1 + f0), cr2 * C2 ** 2 * f1, 0, 0]
+    np.testing.assert_allclose(o.get_pressure_field().ravel(), want, rtol=1e-12, atol=1e-15)
+    vc = np.float64(np.float32(o.dt)) / (np.float64(np.float32(1.2)) * np.float64(np.float32(0.01)))
+    np.testing.assert_allclose(o.get_velocity_x(), vc * np.array([1, C2, 0, 0, 0]), rtol=1e-12,
+                               atol=1e-15)
+
+
+def test_plate_samplers_brute_force(oracle_mod):
+    """fdtd_sample_plate_weighted (:1744-1760) and
+    fdtd_sample_plate_displacement_batch (:1332-1353) against numpy sums
+    over a displacement field spread by 12 plate steps, in float32 and the
+    reference's summation order (out-of-range indices skipped)."""
+    Nx, Ny, Nz, piz = 12, 10, 6, 3
+    ct = np.zeros((Nx, Ny, Nz), np.uint8); ct[:, :, piz - 1] = 1; ct[:, :, piz + 1] = 1
+    pa = np.zeros((Nx, Ny), np.uint8); pa[1:11, 1:9] = 1
+    o = _mk(oracle_mod, Nx, Ny, Nz, precision="f32", n_pml=0, cell_type=ct.ravel(),
+            plate_active=pa.ravel(), plate_iz=piz)
+    o.inject_plate_force([5 + Ny * 5, 3 + Ny * 8], [1.0, -0.6], 3.0)
+    o.step(12)
+    w = o.get_plate_displacement().ravel().astype(np.float32)
+    assert np.count_nonzero(w) > 20
+    rng = np.random.default_rng(9)
+    idx = rng.integers(-3, Nx * Ny + 3, 40).astype(np.int32)
+    wgt = rng.standard_normal(40).astype(np.float32)
+    acc = np.float32(0)
+    for p, g in zip(idx, wgt):
+        if 0 <= p < Nx * Ny:
+            acc = np.float32(acc + np.float32(w[p] * g))
+    got = o.sample_plate_weighted(idx, wgt)
+    assert np.float32(got).view(np.uint32) == acc.view(np.uint32)
+    idx4 = rng.integers(0, Nx * Ny, (25, 4)).astype(np.int32)
+    wgt4 = rng.uniform(0, 1, (25, 4)).astype(np.float32)
+    want = np.zeros(25, np.float32)
+    for q in range(25):
+        t = np.float32(wgt4[q, 0] * w[idx4[q, 0]])
+        for m in range(1, 4):
+            t = np.float32(t + np.float32(wgt4[q, m] * w[idx4[q, m]]))
+        want[q] = t
+    got = o.sample_plate_displacement_batch(idx4, wgt4).astype(np.float32)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
