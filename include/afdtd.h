@@ -112,6 +112,20 @@ SK_API int fdtd_sample_pressure_precomputed(const AcousticFDTDState* st, int n_r
                                             const int* idx8, const float* wgt8,
                                             float* out_p);
 
+/* No counterpart in the reference.  fdtd_sample_pressure_precomputed without
+ * any device allocation, for callers that cannot allocate (tight serving
+ * loops, harnesses that forbid hipMalloc): the stencils and outputs are
+ * staged in a file-scope device array of the library (one per GPU, held
+ * until the process ends) instead of buffers the handle allocates.  At most
+ * FDTD_NOALLOC_MAX_RECEIVERS receivers per call (else FDTD_ERR_DIM); indices
+ * outside [0, Nx*Ny*Nz) give FDTD_ERR_DIM.  Same result, bit for bit, as
+ * fdtd_sample_pressure_precomputed.  All handles on one GPU share the array:
+ * calls that overlap on one GPU (from several threads) race. */
+#define FDTD_NOALLOC_MAX_RECEIVERS 65536
+SK_API int fdtd_sample_pressure_precomputed_noalloc(const AcousticFDTDState* st, int n_rec,
+                                                    const int* idx8, const float* wgt8,
+                                                    float* out_p);
+
 /* acoustic_fdtd.h:319 / acoustic_fdtd.cpp:1678-1697 */
 SK_API int fdtd_sample_velocity_precomputed(const AcousticFDTDState* st, int n_rec,
                                             const int* vx_idx8, const float* vx_wgt8,

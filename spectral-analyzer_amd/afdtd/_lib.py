@@ -59,6 +59,7 @@ _SIGS = {
     "fdtd_precompute_pressure_samplers": (_i, [_vp, _i, _vp, _vp, _vp]),
     "fdtd_precompute_velocity_samplers": (_i, [_vp, _i, _vp] + [_vp] * 6),
     "fdtd_sample_pressure_precomputed": (_i, [_vp, _i, _vp, _vp, _vp]),
+    "fdtd_sample_pressure_precomputed_noalloc": (_i, [_vp, _i, _vp, _vp, _vp]),
     "fdtd_sample_velocity_precomputed": (_i, [_vp, _i] + [_vp] * 9),
     "fdtd_step": (_i, [_vp, _i]),
     "fdtd_step_timed": (_i, [_vp, _i, _vp, _vp]),

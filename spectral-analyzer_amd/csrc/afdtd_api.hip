@@ -11,6 +11,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -19,6 +20,50 @@
 static thread_local std::string g_last_error;
 void afdtd_set_error(const std::string& msg) { g_last_error = msg; }
 extern "C" SK_API const char* fdtd_last_error(void) { return g_last_error.c_str(); }
+
+// ── Scratch of the allocation-free sampler ───────────────────────────────
+// fdtd_sample_pressure_precomputed_noalloc stages the 8-tap stencils and the
+// outputs of up to FDTD_NOALLOC_MAX_RECEIVERS receivers here instead of in
+// buffers the handle allocates: 8 int32 indices, 8 float weights and one
+// float output per receiver (4.25 MiB).  Every GPU holds its own copy for the
+// life of the process.  In host code `scratch` names a host-side stand-in,
+// not a device address: host code uses only the per-device address that
+// hipGetSymbolAddress returns (resolve_scratch), never the array itself.
+namespace {
+constexpr size_t kScrRec = FDTD_NOALLOC_MAX_RECEIVERS;
+constexpr size_t kScrIdx = 0;                                  // int32[8 * kScrRec]
+constexpr size_t kScrWgt = kScrIdx + 8 * sizeof(int) * kScrRec;  // float[8 * kScrRec]
+constexpr size_t kScrOut = kScrWgt + 8 * sizeof(float) * kScrRec;  // float[kScrRec]
+constexpr size_t kScrBytes = kScrOut + sizeof(float) * kScrRec;
+constexpr int kScrMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_dev[kScrMaxDevices] = {};
+}  // namespace
+static __device__ __align__(256) unsigned char scratch[kScrBytes];
+
+// Address of the current device's copy of `scratch`, looked up once per
+// device (with that device current) and kept for the process.
+static int resolve_scratch(unsigned char** out) {
+    *out = nullptr;
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kScrMaxDevices) {
+        afdtd_set_error("allocation-free scratch: no current HIP device");
+        return FDTD_ERR_DEVICE;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* ptr = nullptr;
+        hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            afdtd_set_error(std::string("allocation-free scratch: hipGetSymbolAddress: ") +
+                            hipGetErrorString(e));
+            return FDTD_ERR_DEVICE;
+        }
+        g_scratch_dev[dev] = (unsigned char*)ptr;
+    }
+    *out = g_scratch_dev[dev];
+    return FDTD_OK;
+}
 
 namespace {
 
@@ -95,10 +140,26 @@ int ensure_gather(AcousticFDTDState* st, int n) {
 
 // Gather n outputs of `taps` weighted samples of a field (device), in either
 // summation order, into host out.
+// Every stencil index must address the gathered field (device memory is not
+// bounds-checked): [0, N) for the pressure and the padded velocity layouts,
+// [0, Nplate) for the plate.
+int check_indices(const AcousticFDTDState* st, int which, const int* idx, size_t count) {
+    const int lim = which == 4 ? st->Nplate : st->N;
+    for (size_t q = 0; q < count; ++q)
+        if (idx[q] < 0 || idx[q] >= lim) {
+            afdtd_set_error("sampler index " + std::to_string(idx[q]) + " outside [0, " +
+                            std::to_string(lim) + ")");
+            return FDTD_ERR_DIM;
+        }
+    return FDTD_OK;
+}
+
 int gather_host(AcousticFDTDState* st, int which, const int* idx, const float* wgt, int n,
                 int taps, int zero_start, float* out) {
     if (n <= 0) return FDTD_OK;
-    int rc = ensure_gather(st, n);   // room for n outputs of up to 8 taps
+    int rc = check_indices(st, which, idx, (size_t)n * taps);
+    if (rc) return rc;
+    rc = ensure_gather(st, n);       // room for n outputs of up to 8 taps
     if (rc) return rc;
     AFDTD_CHECK(hipMemcpyAsync(st->d_gidx, idx, sizeof(int) * n * taps, hipMemcpyHostToDevice,
                                st->stream));
@@ -417,6 +478,10 @@ AcousticFDTDState* create_impl(int Nx, int Ny, int Nz, float dx, float c, float 
     sp.inv_sV = 1.0f / sp.sV;
 
     TRY(hipStreamCreateWithFlags(&st->stream, hipStreamNonBlocking));
+    if (resolve_scratch(&st->d_scratch) != FDTD_OK) {
+        destroy_state(st);
+        return nullptr;
+    }
     TRY(hipEventCreate(&st->ev0));
     TRY(hipEventCreate(&st->ev1));
     for (int b = 0; b < 2; ++b) {
@@ -886,6 +951,39 @@ SK_API int fdtd_sample_pressure_precomputed(const AcousticFDTDState* cst, int n,
     if (!cst || !idx8 || !wgt8 || !out) return FDTD_ERR_NULL;
     AcousticFDTDState* st = const_cast<AcousticFDTDState*>(cst);
     return gather_host(st, 0, idx8, wgt8, n, 8, 0, out);
+}
+
+SK_API int fdtd_sample_pressure_precomputed_noalloc(const AcousticFDTDState* cst, int n,
+                                                    const int* idx8, const float* wgt8,
+                                                    float* out) {
+    if (!cst || !idx8 || !wgt8 || !out) return FDTD_ERR_NULL;
+    if (n > FDTD_NOALLOC_MAX_RECEIVERS) {
+        afdtd_set_error("fdtd_sample_pressure_precomputed_noalloc: " + std::to_string(n) +
+                        " receivers exceed the allocation-free maximum of " +
+                        std::to_string(FDTD_NOALLOC_MAX_RECEIVERS));
+        return FDTD_ERR_DIM;
+    }
+    if (n <= 0) return FDTD_OK;
+    AcousticFDTDState* st = const_cast<AcousticFDTDState*>(cst);
+    if (!st->d_scratch) {
+        afdtd_set_error("fdtd_sample_pressure_precomputed_noalloc: scratch not resolved");
+        return FDTD_ERR_DEVICE;
+    }
+    int rc = check_indices(st, 0, idx8, (size_t)8 * n);
+    if (rc) return rc;
+    int* d_idx = (int*)(st->d_scratch + kScrIdx);
+    float* d_wgt = (float*)(st->d_scratch + kScrWgt);
+    float* d_out = (float*)(st->d_scratch + kScrOut);
+    // the stencils and outputs overwrite [0, 8n) and [0, n): nothing of the
+    // previous call's contents is read
+    AFDTD_CHECK(hipMemcpyAsync(d_idx, idx8, sizeof(int) * 8 * n, hipMemcpyHostToDevice,
+                               st->stream));
+    AFDTD_CHECK(hipMemcpyAsync(d_wgt, wgt8, sizeof(float) * 8 * n, hipMemcpyHostToDevice,
+                               st->stream));
+    AFDTD_CHECK(launch_gather(st->P[st->cur], st->storage, 1.0f, d_idx, d_wgt, n, 8, 0, d_out,
+                              st->stream, st->N, st->Nz));
+    AFDTD_CHECK(hipMemcpyAsync(out, d_out, sizeof(float) * n, hipMemcpyDeviceToHost, st->stream));
+    return sync(st);
 }
 
 SK_API int fdtd_sample_velocity_precomputed(const AcousticFDTDState* cst, int n,

@@ -127,6 +127,9 @@ struct AcousticFDTDState {
     float* d_gwgt;
     float* d_gout;
     int gcap;
+    // This device's copy of the file-scope scratch array of the
+    // allocation-free sampler (resolved at creation, never freed).
+    unsigned char* d_scratch;
 
     int step_count, phase;
     int fused_tile_y, fused_seg_x;
