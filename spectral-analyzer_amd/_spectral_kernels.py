@@ -12,5 +12,5 @@ FDTD_WALL = 1
 FDTD_PLATE = 2
 FDTD_PML = 3
 
-# The AMR stepper (pybind_kernels.cpp:8877-8920), gradient order 2.
+# The AMR stepper (pybind_kernels.cpp:8877-8920), gradient orders 2 and 8.
 from afdtd.amr import AcousticAMR  # noqa: E402,F401

@@ -48,7 +48,7 @@ def test_adjoint_128cube_2000_steps_vs_central_differences():
         assert tr.shape == (steps, 32)
         loss = ((tr - target) ** 2).sum()
         loss.backward()
-        res[dtype] = dict(s=s, dev=dev, target=target, loss=float(loss),
+        res[dtype] = dict(s=s, dev=dev, target=target, loss=float(loss.detach()),
                           grads=[g.grad.double().cpu().numpy() for g in (mg, lg, eg)])
         print(f"{dtype}: loss {float(loss):.6e}", flush=True)
     r = res["float64"]
