@@ -602,15 +602,16 @@ AcousticFDTDState* create_impl(int Nx, int Ny, int Nz, float dx, float c, float 
     st->f2_dirty = 1;
     if (o->kernel == FDTD_KERNEL_FUSED2) {
         if (!fused2_supported(st)) {
-            afdtd_set_error("two-step sweep needs the order-2 stencil and no walls, plate "
-                            "cells, face fractions or active plate");
+            afdtd_set_error("two-step sweep needs no walls, plate cells, face fractions or "
+                            "active plate (and for the order-4 stencil FP32 storage and a "
+                            "homogeneous medium)");
             destroy_state(st);
             return nullptr;
         }
         st->family = FDTD_KERNEL_FUSED2;
         st->f2_rows = f2_rows_for(o->tile_y);
     } else if (o->kernel == FDTD_KERNEL_AUTO && st->storage == FDTD_STORE_F32 &&
-               fused2_supported(st)) {
+               st->order == 2 && fused2_supported(st)) {
         // FP32 order-2 rooms: two steps per sweep (the single-step sweep is
         // bound by the streaming rate of its byte mix; 16-bit storage halves
         // that mix and keeps the single-step sweep ahead)

@@ -920,7 +920,7 @@ hipError_t launch2(AcousticFDTDState* st, const Pair2Args& a, int nb) {
 }  // namespace
 
 bool fused2_supported(const AcousticFDTDState* st) {
-    return st->order == 2 && fused_supported(st);
+    return (st->order == 2 && fused_supported(st)) || fused4_supported(st);
 }
 
 // Per-tile source and tap list entries of one grid (host side).  Tile
@@ -969,14 +969,18 @@ int f2_tile_rows(int tile_y) { return f2_tyo(f2_rows_for(tile_y)); }
 // Per-tile source and tap tables (uploaded when sources or receivers change).
 static int build_tables(AcousticFDTDState* st) {
     using namespace f2;
-    const int TYO = f2_tyo(st->f2_rows);
+    const int TYO = st->order == 4 ? f4_tile_rows() : f2_tyo(st->f2_rows);
     const int nt = ((st->Ny + TYO - 1) / TYO) * ((st->Nz + TZ - 1) / TZ);
     std::vector<std::vector<int2>> src, tap;
     std::vector<int> sid(st->npsrc), tslot(8 * (size_t)st->nrec);
     for (int s = 0; s < st->npsrc; ++s) sid[s] = s;
     for (size_t q = 0; q < tslot.size(); ++q) tslot[q] = (int)q;
-    f2_tile_lists(st->Nx, st->Ny, st->Nz, TYO, st->npsrc, st->h_pidx.data(), sid.data(),
-                  8 * st->nrec, st->h_ridx.data(), tslot.data(), src, tap);
+    if (st->order == 4)
+        f4_tile_lists(st->Ny, st->Nz, st->npsrc, st->h_pidx.data(), sid.data(), 8 * st->nrec,
+                      st->h_ridx.data(), tslot.data(), src, tap);
+    else
+        f2_tile_lists(st->Nx, st->Ny, st->Nz, TYO, st->npsrc, st->h_pidx.data(), sid.data(),
+                      8 * st->nrec, st->h_ridx.data(), tslot.data(), src, tap);
     auto by_key = [](const int2& x, const int2& y) { return x.x < y.x || (x.x == y.x && x.y < y.y); };
     std::vector<int> ss(nt + 1, 0), ts(nt + 1, 0);
     std::vector<int2> sflat, tflat;
@@ -1023,6 +1027,7 @@ int fused2_prepare(AcousticFDTDState* st) {
 
 hipError_t launch_fused2_pair(AcousticFDTDState* st, int step) {
     using namespace f2;
+    if (st->order == 4) return launch_fused4_pair(st, step);
     Pair2Args a;
     a.p = st->sp;
     a.f.Pin = st->P[st->cur];

@@ -1,0 +1,673 @@
+// afdtd_fused4.hip — two leapfrog steps per sweep for the reference's
+// order-4 stencil (acoustic_fdtd.cpp:1099-1234; coefficients 9/8, -1/24,
+// :889-890; 2nd-order gradient at faces next to the domain edge,
+// :1107-1112; zero-padded divergence, :894-925).
+//
+// The single-step sweep reads and writes every field once per step.  This
+// sweep advances n -> n+2 per pass over HBM.  A workgroup owns a tile of
+// TY = 24 rows x 64 columns (y, z) and marches an x-segment of it; per
+// x-plane q it runs four stages, each trailing the one before:
+//   A faces    (q)   V^{n+1/2} at plane q on the extended tile (the tile plus a
+//                    three-cell ring: a stage-B face 1.5 cells outside the tile
+//                    reads P^{n+1} three cells out) and on the extra face rows
+//                    / columns the ring's divergence reads;
+//   A pressure (q-1) P^{n+1} on the extended tile (+ point sources of step n+1);
+//   B faces    (q-3) V^{n+3/2} of the tile's faces and of the faces up to 1.5
+//                    cells outside it, written to HBM for the tile's own faces;
+//   B pressure (q-4) P^{n+2} of the tile, written to HBM.
+// P^{n+1} and V^{n+1/2} never go to HBM.  P^n is read once per plane (the tile
+// plus a six-cell halo), V^{n-1/2} once (the extended faces).
+//
+// Work mapping (768 threads = 12 wavefronts, one workgroup per CU: 154 KB of
+// LDS): lane = column, wave w owns the tile rows 2w and 2w+1 (slots 0 and 1:
+// both stages); waves 0..8 also own one ring cell (slot 2: waves 0..5 the
+// ring rows -3, -2, -1, 24, 25, 26 at columns 0..63, waves 6..8 the 180
+// cells of columns -3..-1 and 64..66).  A slot keeps its x-direction history
+// in registers (P^n of planes q-1..q+3, its stage-A x-faces of four planes,
+// its stage-B x-faces of four planes), so x-neighbours never go through LDS;
+// y / z neighbours do: P^n of the current plane, the stage-A y / z faces and
+// P^{n+1} of the last four planes, the stage-B y / z faces of two.  The
+// ring's extra stage-A faces (y-face rows -5, -4, 27; z-face columns -5, -4,
+// 67) are one more task for waves 0..5; the P^n halo cells outside the
+// extended tile are loaded by all threads.  One barrier per plane: every LDS
+// value an iteration reads was written before the previous barrier (or by
+// the same lane earlier in the iteration).  The loop body is unrolled four
+// times so that the register queues rotate by phase instead of moving.
+//
+// Arithmetic is the single-step sweep's, in the same order, so the result
+// equals two single steps (and the oracle) bit for bit.  The PML damping of
+// a face or cell is applied only on planes and tiles within reach of the
+// absorbing layer (elsewhere the multiplier is exactly 1).  Receiver taps of
+// step n+1 go to the tap buffer as in the order-2 sweep; the stability check
+// never falls on step n+1 (the host schedules pairs so).
+#include "afdtd_fused_tile.cuh"
+
+#include <string.h>
+
+#include <algorithm>
+#include <type_traits>
+#include <vector>
+
+using fused::FusedPtrs;
+using fused::xcd_remap;
+
+namespace {
+
+namespace f4 {
+constexpr int TZ = 64;                        // own columns (one per lane)
+constexpr int TY = 24;                        // own rows (two per wave)
+constexpr int NW = 12;                        // wavefronts
+constexpr int NT = 64 * NW;
+constexpr int H = 3;                          // P^{n+1} ring (cells)
+constexpr int G = 6;                          // P^n halo (cells)
+constexpr int ER = TY + 2 * H, EC = TZ + 2 * H;   // extended tile 30 x 70 (rows -3..26, cols -3..66)
+constexpr int GR = TY + 2 * G, GC = TZ + 2 * G;   // P^n region 36 x 76 (rows -6..29, cols -6..69)
+constexpr int AYR = TY + 9, AZC = TZ + 9;         // stage-A y-face rows -5..27, z-face cols -5..67
+constexpr int BYR = TY + 3, BZC = TZ + 3;         // stage-B y-face rows -2..24, z-face cols -2..64
+constexpr int PLN = GR * GC, AYN = AYR * EC, AZN = ER * AZC, P1N = ER * EC;
+constexpr int BYN = BYR * TZ, BZN = TY * BZC;
+constexpr int NRG = GR * GC - ER * EC;            // halo cells outside the extended tile (636)
+constexpr int NRGS = (NRG + NT - 1) / NT;         // of them per thread
+constexpr int NCR = 6 * ER;                       // column-ring cells (180)
+constexpr int NYX = 3 * EC, NZX = 3 * ER;         // extra stage-A y faces (210), z faces (90)
+constexpr int MAXS = F2_MAX_SRC;
+constexpr int MAXT = F2_MAX_TAP;
+__device__ __forceinline__ int gix(int r, int c) { return (r + G) * GC + (c + G); }
+__device__ __forceinline__ int p1ix(int e, int c) { return (e + H) * EC + (c + H); }
+__device__ __forceinline__ int ayix(int f, int c) { return (f + 5) * EC + (c + H); }
+__device__ __forceinline__ int azix(int e, int g) { return (e + H) * AZC + (g + 5); }
+__device__ __forceinline__ int byix(int f, int c) { return (f + 2) * TZ + c; }
+__device__ __forceinline__ int bzix(int e, int g) { return e * BZC + (g + 2); }
+// list key: plane << 12 | (extended row + 3) << 7 | (extended column + 3)
+__host__ __device__ __forceinline__ int ekey(int plane, int e, int c) {
+    return (plane << 12) | ((e + H) << 7) | (c + H);
+}
+typedef __amdgpu_buffer_rsrc_t Rsrc;
+__device__ __forceinline__ Rsrc rsrc(const void* base, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes,
+                                             0x00020000);
+}
+__device__ __forceinline__ float bld(Rsrc r, uint32_t off) {
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+__device__ __forceinline__ void bst(Rsrc r, uint32_t off, float v) {
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, off, 0, 2);
+}
+// gradient across a face from its four P taps (2nd order when not inner)
+__device__ __forceinline__ float grad4(float pm, float pl, float pr, float pp, bool inner,
+                                       float C1, float C2) {
+    return inner ? C1 * (pr - pl) + C2 * (pp - pm) : pr - pl;
+}
+}  // namespace f4
+
+struct Pair4Args {
+    StencilParams p;
+    FusedPtrs f;
+    const float* alpha;
+    const float* pdamp;
+    const int* status;
+    int seg, nty, ntz, nseg;
+    const int* src_start;
+    const int2* src;        // (key, source index), sorted by key then source index
+    const int* tap_start;
+    const int2* tap;        // (key, tap slot), sorted by key
+    const float* psig;
+    int plen;
+    int step;               // global index of the first of the two steps
+    float* tapbuf;
+};
+
+template <bool PML>
+__device__ __forceinline__ void fused4_tile(const Pair4Args& a, int ty, int tz, int sg,
+                                            int tile) {
+    using namespace f4;
+    __shared__ float sP[2 * PLN];
+    __shared__ float sAy[4 * AYN];
+    __shared__ float sAz[4 * AZN];
+    __shared__ float sP1[4 * P1N];
+    __shared__ float sBy[2 * BYN];
+    __shared__ float sBz[2 * BZN];
+    __shared__ float sAlpha[65], sPd[65];
+    __shared__ int sSrcKey[MAXS], sTapKey[MAXT], sTapSlot[MAXT];
+    __shared__ float sAmp[MAXS];
+
+    const StencilParams& p = a.p;
+    const int Nx = p.Nx, Ny = p.Ny, Nz = p.Nz, sx = Ny * Nz;
+    const int lane = threadIdx.x, w = __builtin_amdgcn_readfirstlane(threadIdx.y);
+    const int tid = lane + 64 * w;
+    const int j0 = ty * TY, k0 = tz * TZ;
+    const int i0 = sg * a.seg, i1 = min(i0 + a.seg, Nx);
+    const int npml = PML ? p.n_pml : 0;
+    if (PML)
+        for (int d = tid; d <= npml && d < 65; d += NT) { sAlpha[d] = a.alpha[d]; sPd[d] = a.pdamp[d]; }
+    const float C1 = p.C1, C2 = p.C2, vcoef = p.vel_coeff, kap0 = p.bulk_coeff, dt = p.dt;
+    const float* __restrict__ Pin = (const float*)a.f.Pin;
+    float* __restrict__ Pout = (float*)a.f.Pout;
+    const float* __restrict__ Vxi = (const float*)a.f.Vin[0];
+    const float* __restrict__ Vyi = (const float*)a.f.Vin[1];
+    const float* __restrict__ Vzi = (const float*)a.f.Vin[2];
+    float* __restrict__ Vxo = (float*)a.f.Vout[0];
+    float* __restrict__ Vyo = (float*)a.f.Vout[1];
+    float* __restrict__ Vzo = (float*)a.f.Vout[2];
+    const uint32_t pb = (uint32_t)sx * 4u;                   // plane bytes
+    constexpr uint32_t OOB = 0x80000000u;                     // past every plane: load 0, store dropped
+    // descriptor of plane pl of field F (empty outside the grid: loads give 0)
+    auto rP = [&](const float* F, int pl) __attribute__((always_inline)) {
+        return (pl >= 0 && pl < Nx) ? rsrc(F + (size_t)pl * sx, pb) : rsrc(F, 0u);
+    };
+    // PML depth of one axis position; cells outside the grid are clamped to
+    // the table (their values are never used)
+    auto pd = [&](int x, int n) __attribute__((always_inline)) { return min(pml_depth(x, n, npml), npml); };
+    auto damp = [&](int dL, int dR) __attribute__((always_inline)) { return face_damp(sAlpha, dL, dR, dt); };
+
+    // ── slots: 0, 1 own cells (rows 2w, 2w+1), 2 a ring cell ──
+    int se[3], sc[3];
+    bool sact[3];
+    se[0] = 2 * w; sc[0] = lane; sact[0] = true;
+    se[1] = 2 * w + 1; sc[1] = lane; sact[1] = true;
+    const bool ringW = w < 9;           // waves with a ring slot
+    if (w < 6) {
+        se[2] = w < 3 ? w - 3 : TY + w - 3;
+        sc[2] = lane;
+        sact[2] = true;
+    } else {
+        const int idx = (w - 6) * 64 + lane, cs = idx % 6;
+        se[2] = -3 + idx / 6;
+        sc[2] = cs < 3 ? cs - 3 : TZ + cs - 3;
+        sact[2] = ringW && idx < NCR;
+        if (!sact[2]) { se[2] = 0; sc[2] = 0; }
+    }
+    uint32_t off[3];             // byte offset of the slot cell in a plane
+    bool cellv[3], vyv[3], vzv[3], yin[3], zin[3];
+    int dyz0[3], dyzY[3], dyzZ[3];
+#pragma unroll
+    for (int s = 0; s < 3; ++s) {
+        const int j = j0 + se[s], k = k0 + sc[s];
+        cellv[s] = sact[s] && j >= 0 && j < Ny && k >= 0 && k < Nz;
+        off[s] = cellv[s] ? (uint32_t)(j * Nz + k) * 4u : OOB;
+        vyv[s] = cellv[s] && j <= Ny - 2;
+        vzv[s] = cellv[s] && k <= Nz - 2;
+        yin[s] = j > 0 && j < Ny - 2;
+        zin[s] = k > 0 && k < Nz - 2;
+        const int py = pd(j, Ny), pz = pd(k, Nz);
+        dyz0[s] = max(py, pz);
+        dyzY[s] = max(pd(j + 1, Ny), pz);
+        dyzZ[s] = max(py, pd(k + 1, Nz));
+    }
+    // stage-B roles of the ring slot: y-faces of rows -2, -1, TY (waves 1..3),
+    // z-faces of columns -2, -1, 64 on the tile rows (waves 6..8)
+    const bool ringBy = w >= 1 && w <= 3;
+    const bool ringBzW = w >= 6 && w < 9;
+    const bool ringBz = ringBzW && sact[2] && se[2] >= 0 && se[2] < TY &&
+                        (sc[2] == -2 || sc[2] == -1 || sc[2] == TZ);
+    // ── misc task: an extra stage-A face (waves 0..3 y, 4..5 z) ──
+    int mf = 0, mc = 0;          // y: face row f, column c;  z: row e, face column g
+    bool mact = false, mval = false, min_ = false;
+    uint32_t moff = OOB;
+    int mdL = 0, mdR = 0;
+    if (w < 4) {
+        const int idx = w * 64 + lane, t = idx / EC;
+        mact = idx < NYX;
+        mf = t == 0 ? -5 : t == 1 ? -4 : TY + 3;
+        mc = -3 + idx % EC;
+        const int j = j0 + mf, k = k0 + mc;
+        mval = mact && j >= 0 && j <= Ny - 2 && k >= 0 && k < Nz;
+        min_ = j > 0 && j < Ny - 2;
+        if (mval) moff = (uint32_t)(j * Nz + k) * 4u;
+        mdL = max(pd(j, Ny), pd(k, Nz));
+        mdR = max(pd(j + 1, Ny), pd(k, Nz));
+    } else if (w < 6) {
+        const int idx = (w - 4) * 64 + lane, t = idx / ER;
+        mact = idx < NZX;
+        mc = t == 0 ? -5 : t == 1 ? -4 : TZ + 3;
+        mf = -3 + idx % ER;
+        const int j = j0 + mf, k = k0 + mc;
+        mval = mact && j >= 0 && j < Ny && k >= 0 && k <= Nz - 2;
+        min_ = k > 0 && k < Nz - 2;
+        if (mval) moff = (uint32_t)(j * Nz + k) * 4u;
+        mdL = max(pd(j, Ny), pd(k, Nz));
+        mdR = max(pd(j, Ny), pd(k + 1, Nz));
+    }
+    if (!mact) { mf = 0; mc = 0; }
+    // ── P^n halo cells outside the extended tile ──
+    uint32_t roff[NRGS];
+    int rlds[NRGS];
+#pragma unroll
+    for (int m = 0; m < NRGS; ++m) {
+        const int r = tid + NT * m;
+        int row = 0, col = 0;
+        if (r < 3 * GC) { row = -G + r / GC; col = -G + r % GC; }
+        else if (r < 6 * GC) { row = TY + H + (r - 3 * GC) / GC; col = -G + (r - 3 * GC) % GC; }
+        else {
+            const int r2 = r - 6 * GC, cs = r2 % 6;
+            row = -H + r2 / 6;
+            col = cs < 3 ? -G + cs : TZ + H + (cs - 3);
+        }
+        const int j = j0 + row, k = k0 + col;
+        const bool v = r < NRG && j >= 0 && j < Ny && k >= 0 && k < Nz;
+        roff[m] = v ? (uint32_t)(j * Nz + k) * 4u : OOB;
+        rlds[m] = r < NRG ? gix(row, col) : -1;
+    }
+
+    // ── source and tap lists of the tile ──
+    const int s0 = a.src_start[tile], tp0 = a.tap_start[tile];
+    const int ns = a.step < a.plen ? min(a.src_start[tile + 1] - s0, MAXS) : 0;
+    const int nt = min(a.tap_start[tile + 1] - tp0, MAXT);
+    for (int m = tid; m < ns; m += NT) {
+        const int2 en = a.src[s0 + m];
+        sSrcKey[m] = en.x;
+        sAmp[m] = a.psig[(size_t)en.y * a.plen + a.step];
+    }
+    for (int m = tid; m < nt; m += NT) {
+        const int2 en = a.tap[tp0 + m];
+        sTapKey[m] = en.x;
+        sTapSlot[m] = en.y;
+    }
+
+    // ── per-slot pipeline registers ──
+    // At phase P (iteration index mod 4) entry u of a face queue (u = 0
+    // newest) lives in physical slot (P - u) & 3, and P^n of plane p in slot
+    // (p - qf + 1) & 3.  The LDS rings are indexed the same way, by the plane
+    // relative to qf (slot (p - qf) & 3 or & 1), so every ring offset is a
+    // compile-time constant of the phase.
+    float pn[3][4], pnl[3];             // P^n of planes q-1..q+2; q+3 in flight
+    float ax[3][4];                     // stage-A x-faces of planes q, q-1, q-2, q-3
+    float ovx[3], ovy[3], ovz[3], ovm;  // V^{n-1/2} of the next plane's stage-A faces
+    float bx[2][4];                     // own: stage-B x-faces of planes q-3 .. q-6
+    float rg[NRGS];
+#pragma unroll
+    for (int s = 0; s < 3; ++s)
+#pragma unroll
+        for (int u = 0; u < 4; ++u) pn[s][u] = ax[s][u] = 0.0f;
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int u = 0; u < 4; ++u) bx[s][u] = 0.0f;
+    // Damping multipliers of the slot's faces and cell on planes away from
+    // the x-sides of the layer (the x depth is 0 there, so they depend on the
+    // slot alone; exactly 1 for cells outside the layer).
+    float mX[3], mY[3], mZ[3], mP[3], mM = 1.0f;
+#pragma unroll
+    for (int s = 0; s < 3; ++s) {
+        mX[s] = mY[s] = mZ[s] = mP[s] = 1.0f;
+        if (PML) {
+            mX[s] = face_damp(a.alpha, dyz0[s], dyz0[s], dt);
+            mY[s] = face_damp(a.alpha, dyz0[s], dyzY[s], dt);
+            mZ[s] = face_damp(a.alpha, dyz0[s], dyzZ[s], dt);
+            mP[s] = a.pdamp[dyz0[s]];
+        }
+    }
+    if (PML) mM = face_damp(a.alpha, mdL, mdR, dt);
+    // planes whose faces or cells reach the x-sides of the layer
+    auto xpml = [&](int q) __attribute__((always_inline)) {
+        return PML && (q < npml + 1 || q >= Nx - npml - 1);
+    };
+
+    const int qf = i0 - 5;              // first plane of the stage-A faces
+    // prologue: P^n of planes qf-1..qf+2 at the slots, plane qf into LDS,
+    // V^{n-1/2} of plane qf
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const Rsrc r = rP(Pin, qf - 1 + u);
+#pragma unroll
+        for (int s = 0; s < 3; ++s) pn[s][u] = bld(r, off[s]);
+    }
+    {
+        const Rsrc r = rP(Pin, qf);
+        float* Pw = sP;                 // plane qf: ring slot 0
+#pragma unroll
+        for (int m = 0; m < NRGS; ++m) {
+            const float v = bld(r, roff[m]);
+            if (rlds[m] >= 0) Pw[rlds[m]] = v;
+        }
+#pragma unroll
+        for (int s = 0; s < 3; ++s)
+            if (sact[s]) Pw[gix(se[s], sc[s])] = pn[s][1];
+        const Rsrc X = rP(Vxi, qf), Y = rP(Vyi, qf), Z = rP(Vzi, qf);
+#pragma unroll
+        for (int s = 0; s < 3; ++s) {
+            ovx[s] = bld(X, off[s]);
+            ovy[s] = bld(Y, off[s]);
+            ovz[s] = bld(Z, off[s]);
+        }
+        ovm = bld(w < 4 ? Y : Z, moff);
+    }
+    __syncthreads();
+    // cursors into the cached lists and the key each points at
+    int scur = 0, tcur = 0;
+    int skey = ns > 0 ? __builtin_amdgcn_readfirstlane(sSrcKey[0]) : 0x7FFFFFFF;
+    int tkey = nt > 0 ? __builtin_amdgcn_readfirstlane(sTapKey[0]) : 0x7FFFFFFF;
+
+    auto body = [&](const int q, auto phase) __attribute__((always_inline)) {
+        constexpr int P = decltype(phase)::value;
+        constexpr int N0 = P & 3, N1 = (P + 3) & 3, N2 = (P + 2) & 3, N3 = (P + 1) & 3;  // newest .. oldest
+        constexpr int M0 = N0, M1 = (P + 1) & 3, M2 = (P + 2) & 3, M3 = (P + 3) & 3;      // P^n planes q-1 .. q+2
+        const int qa = q - 1, qb = q - 3, qc = q - 4;
+        // halo cells of plane q+1 and the slots' P^n of plane q+3
+        {
+            const Rsrc r1 = rP(Pin, q + 1), r3 = rP(Pin, q + 3);
+#pragma unroll
+            for (int m = 0; m < NRGS; ++m) rg[m] = bld(r1, roff[m]);
+#pragma unroll
+            for (int s = 0; s < 3; ++s) pnl[s] = bld(r3, off[s]);
+        }
+
+        // ════ stage-A faces of plane q ════
+        {
+            const float* Pq = sP + (P & 1) * PLN;
+            float* Ayw = sAy + (P & 3) * AYN;
+            float* Azw = sAz + (P & 3) * AZN;
+            const bool fvx = q >= 0 && q <= Nx - 2, xin = q > 0 && q < Nx - 2;
+            const bool qv = q >= 0 && q < Nx;
+            const bool xd = xpml(q);
+            const int dq = PML ? pd(q, Nx) : 0, dq1 = PML ? pd(q + 1, Nx) : 0;
+#pragma unroll
+            for (int s = 0; s < 3; ++s) {
+                if (s == 2 && !ringW) continue;
+                const int e = se[s], c = sc[s];
+                const float pm = pn[s][M0], pl = pn[s][M1];
+                // x-face between planes q and q+1
+                float nv = ovx[s] - vcoef * grad4(pm, pl, pn[s][M2], pn[s][M3], xin, C1, C2);
+                if (PML) nv = nv * (xd ? damp(max(dq, dyz0[s]), max(dq1, dyz0[s])) : mX[s]);
+                ax[s][N0] = (fvx && cellv[s]) ? nv : 0.0f;
+                // y-face between rows e and e+1
+                nv = ovy[s] - vcoef * grad4(Pq[gix(e - 1, c)], pl, Pq[gix(e + 1, c)],
+                                            Pq[gix(e + 2, c)], yin[s], C1, C2);
+                if (PML) nv = nv * (xd ? damp(max(dq, dyz0[s]), max(dq, dyzY[s])) : mY[s]);
+                const float ny = (qv && vyv[s]) ? nv : 0.0f;
+                // z-face between columns c and c+1
+                nv = ovz[s] - vcoef * grad4(Pq[gix(e, c - 1)], pl, Pq[gix(e, c + 1)],
+                                            Pq[gix(e, c + 2)], zin[s], C1, C2);
+                if (PML) nv = nv * (xd ? damp(max(dq, dyz0[s]), max(dq, dyzZ[s])) : mZ[s]);
+                const float nz = (qv && vzv[s]) ? nv : 0.0f;
+                if (sact[s]) {
+                    Ayw[ayix(e, c)] = ny;
+                    Azw[azix(e, c)] = nz;
+                }
+            }
+            if (w < 4) {            // extra y-face (mf | mf+1) at column mc
+                float nv = ovm - vcoef * grad4(Pq[gix(mf - 1, mc)], Pq[gix(mf, mc)],
+                                               Pq[gix(mf + 1, mc)], Pq[gix(mf + 2, mc)], min_, C1, C2);
+                if (PML) nv = nv * (xd ? damp(max(dq, mdL), max(dq, mdR)) : mM);
+                if (mact) Ayw[ayix(mf, mc)] = (qv && mval) ? nv : 0.0f;
+            } else if (w < 6) {     // extra z-face (mc | mc+1) at row mf
+                float nv = ovm - vcoef * grad4(Pq[gix(mf, mc - 1)], Pq[gix(mf, mc)],
+                                               Pq[gix(mf, mc + 1)], Pq[gix(mf, mc + 2)], min_, C1, C2);
+                if (PML) nv = nv * (xd ? damp(max(dq, mdL), max(dq, mdR)) : mM);
+                if (mact) Azw[azix(mf, mc)] = (qv && mval) ? nv : 0.0f;
+            }
+        }
+        // V^{n-1/2} of plane q+1 (consumed next iteration)
+        {
+            const Rsrc X = rP(Vxi, q + 1), Y = rP(Vyi, q + 1), Z = rP(Vzi, q + 1);
+#pragma unroll
+            for (int s = 0; s < 3; ++s) {
+                if (s == 2 && !ringW) continue;
+                ovx[s] = bld(X, off[s]);
+                ovy[s] = bld(Y, off[s]);
+                ovz[s] = bld(Z, off[s]);
+            }
+            if (w < 6) ovm = bld(w < 4 ? Y : Z, moff);
+        }
+
+        // ════ stage-A pressure of plane qa = q-1 ════
+        if (qa >= i0 - 3) {
+            const float* Ayr = sAy + ((P + 3) & 3) * AYN;
+            const float* Azr = sAz + ((P + 3) & 3) * AZN;
+            float* P1w = sP1 + ((P + 3) & 3) * P1N;
+            const bool qav = qa >= 0 && qa < Nx;
+            const bool xd = xpml(qa);
+            const int dqa = PML ? pd(qa, Nx) : 0;
+            float pnew[3];
+#pragma unroll
+            for (int s = 0; s < 3; ++s) {
+                pnew[s] = 0.0f;
+                if (s == 2 && !ringW) continue;
+                const int e = se[s], c = sc[s];
+                const float xp = ax[s][N1], xm = ax[s][N2], xpp = ax[s][N0], xmm = ax[s][N3];
+                const float yp = Ayr[ayix(e, c)], ym = Ayr[ayix(e - 1, c)];
+                const float ypp = Ayr[ayix(e + 1, c)], ymm = Ayr[ayix(e - 2, c)];
+                const float zp = Azr[azix(e, c)], zm = Azr[azix(e, c - 1)];
+                const float zpp = Azr[azix(e, c + 1)], zmm = Azr[azix(e, c - 2)];
+                const float dv = C1 * ((xp - xm) + (yp - ym) + (zp - zm))
+                               + C2 * ((xpp - xmm) + (ypp - ymm) + (zpp - zmm));
+                float out = pn[s][M0] - kap0 * dv;
+                if (PML) out = out * (xd ? sPd[max(dqa, dyz0[s])] : mP[s]);
+                pnew[s] = (qav && cellv[s]) ? out : 0.0f;
+            }
+            // point sources of step n+1, in source order
+            while ((skey >> 12) <= qa) {
+                if ((skey >> 12) == qa) {
+                    const int e = ((skey >> 7) & 31) - H, c = (skey & 127) - H;
+                    const float amp = sAmp[scur];
+#pragma unroll
+                    for (int s = 0; s < 3; ++s)
+                        if (sact[s] && se[s] == e && sc[s] == c) pnew[s] = pnew[s] + amp;
+                }
+                ++scur;
+                skey = scur < ns ? __builtin_amdgcn_readfirstlane(sSrcKey[scur]) : 0x7FFFFFFF;
+            }
+#pragma unroll
+            for (int s = 0; s < 3; ++s)
+                if (sact[s]) P1w[p1ix(se[s], sc[s])] = pnew[s];
+            // receiver taps of step n+1 (own planes and cells)
+            while ((tkey >> 12) <= qa) {
+                if ((tkey >> 12) == qa) {
+                    const int e = ((tkey >> 7) & 31) - H, c = (tkey & 127) - H;
+                    if (qa >= i0 && qa < i1 && c == lane && (e >> 1) == w)
+                        a.tapbuf[sTapSlot[tcur]] = (e & 1) ? pnew[1] : pnew[0];
+                }
+                ++tcur;
+                tkey = tcur < nt ? __builtin_amdgcn_readfirstlane(sTapKey[tcur]) : 0x7FFFFFFF;
+            }
+        }
+
+        // ════ stage-B faces of plane qb = q-3 ════
+        // P^{n+1} of planes qb-1 .. qb+2 are in LDS slots (q-4)&3 .. (q-1)&3; the
+        // slot of plane q-1 was written earlier in this iteration by the same lane
+        if (qb >= i0 - 2) {
+            const float* P1m = sP1 + (P & 3) * P1N;
+            const float* P1b = sP1 + ((P + 1) & 3) * P1N;
+            const float* P1r = sP1 + ((P + 2) & 3) * P1N;
+            const float* P1p = sP1 + ((P + 3) & 3) * P1N;
+            const bool fvb = qb >= 0 && qb <= Nx - 2, xinb = qb > 0 && qb < Nx - 2;
+            const bool qbv = qb >= 0 && qb < Nx, ownq = qb >= i0 && qb < i1;
+            const bool xd = xpml(qb);
+            const int dqb = PML ? pd(qb, Nx) : 0, dqb1 = PML ? pd(qb + 1, Nx) : 0;
+            const Rsrc X = rP(Vxo, ownq ? qb : -1), Y = rP(Vyo, ownq ? qb : -1),
+                       Z = rP(Vzo, ownq ? qb : -1);
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {   // x-face between planes qb and qb+1
+                const int ix = p1ix(se[s], sc[s]);
+                float nv = ax[s][N3] - vcoef * grad4(P1m[ix], P1b[ix], P1r[ix], P1p[ix], xinb, C1, C2);
+                if (PML) nv = nv * (xd ? damp(max(dqb, dyz0[s]), max(dqb1, dyz0[s])) : mX[s]);
+                nv = (fvb && cellv[s]) ? nv : 0.0f;
+                bx[s][N0] = nv;
+                if (ownq) bst(X, off[s], nv);
+            }
+            if (ownq) {
+                const float* Ayo = sAy + ((P + 1) & 3) * AYN;    // stage-A faces of plane qb
+                const float* Azo = sAz + ((P + 1) & 3) * AZN;
+                float* Byw = sBy + ((P + 1) & 1) * BYN;
+                float* Bzw = sBz + ((P + 1) & 1) * BZN;
+#pragma unroll
+                for (int s = 0; s < 3; ++s) {
+                    if (s == 2 && !ringBy) continue;
+                    const int e = se[s], c = sc[s];
+                    float nv = Ayo[ayix(e, c)] - vcoef * grad4(P1b[p1ix(e - 1, c)], P1b[p1ix(e, c)],
+                                                               P1b[p1ix(e + 1, c)], P1b[p1ix(e + 2, c)],
+                                                               yin[s], C1, C2);
+                    if (PML) nv = nv * (xd ? damp(max(dqb, dyz0[s]), max(dqb, dyzY[s])) : mY[s]);
+                    nv = (qbv && vyv[s]) ? nv : 0.0f;
+                    Byw[byix(e, c)] = nv;
+                    if (s < 2) bst(Y, off[s], nv);
+                }
+#pragma unroll
+                for (int s = 0; s < 3; ++s) {
+                    if (s == 2 && !ringBzW) continue;
+                    // ring lanes without a stage-B z-face read a harmless cell
+                    const int e = (s < 2 || ringBz) ? se[s] : 0, c = (s < 2 || ringBz) ? sc[s] : 0;
+                    float nv = Azo[azix(e, c)] - vcoef * grad4(P1b[p1ix(e, c - 1)], P1b[p1ix(e, c)],
+                                                               P1b[p1ix(e, c + 1)], P1b[p1ix(e, c + 2)],
+                                                               zin[s], C1, C2);
+                    if (PML) nv = nv * (xd ? damp(max(dqb, dyz0[s]), max(dqb, dyzZ[s])) : mZ[s]);
+                    nv = (qbv && vzv[s]) ? nv : 0.0f;
+                    if (s < 2) {
+                        Bzw[bzix(e, c)] = nv;
+                        bst(Z, off[s], nv);
+                    } else if (ringBz) {
+                        Bzw[bzix(e, c)] = nv;
+                    }
+                }
+            }
+        }
+
+        // ════ stage-B pressure of plane qc = q-4 ════
+        if (qc >= i0 && qc < i1) {
+            const float* Byr = sBy + (P & 1) * BYN;
+            const float* Bzr = sBz + (P & 1) * BZN;
+            const float* P1c = sP1 + (P & 3) * P1N;
+            const bool xd = xpml(qc);
+            const int dqc = PML ? pd(qc, Nx) : 0;
+            const Rsrc R = rP(Pout, qc);
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                const int e = se[s], c = sc[s];
+                const float xp = bx[s][N1], xm = bx[s][N2], xpp = bx[s][N0], xmm = bx[s][N3];
+                const float yp = Byr[byix(e, c)], ym = Byr[byix(e - 1, c)];
+                const float ypp = Byr[byix(e + 1, c)], ymm = Byr[byix(e - 2, c)];
+                const float zp = Bzr[bzix(e, c)], zm = Bzr[bzix(e, c - 1)];
+                const float zpp = Bzr[bzix(e, c + 1)], zmm = Bzr[bzix(e, c - 2)];
+                const float dv = C1 * ((xp - xm) + (yp - ym) + (zp - zm))
+                               + C2 * ((xpp - xmm) + (ypp - ymm) + (zpp - zmm));
+                float out = P1c[p1ix(e, c)] - kap0 * dv;
+                if (PML) out = out * (xd ? sPd[max(dqc, dyz0[s])] : mP[s]);
+                bst(R, off[s], out);
+            }
+        }
+
+        // ── P^n plane q+1 into LDS; the slots' P^n of plane q+3 into the queue ──
+        {
+            float* Pw = sP + ((P + 1) & 1) * PLN;
+#pragma unroll
+            for (int m = 0; m < NRGS; ++m)
+                if (rlds[m] >= 0) Pw[rlds[m]] = rg[m];
+#pragma unroll
+            for (int s = 0; s < 3; ++s)
+                if (sact[s]) Pw[gix(se[s], sc[s])] = pn[s][M2];
+        }
+#pragma unroll
+        for (int s = 0; s < 3; ++s) pn[s][M0] = pnl[s];
+        __syncthreads();
+    };
+
+    const int qend = i1 + 3;
+#pragma unroll 1
+    for (int q = qf; q <= qend; q += 4) {
+        body(q, std::integral_constant<int, 0>());
+        if (q + 1 <= qend) body(q + 1, std::integral_constant<int, 1>());
+        if (q + 2 <= qend) body(q + 2, std::integral_constant<int, 2>());
+        if (q + 3 <= qend) body(q + 3, std::integral_constant<int, 3>());
+    }
+}
+
+template <bool PML>
+__global__ void __launch_bounds__(f4::NT) k_fused4(Pair4Args a) {
+    if (*a.status) return;
+    // each XCD takes a contiguous range of work items; within a segment the
+    // items run in bands of up to four z-tiles, z fastest, that sweep y
+    int t = xcd_remap(blockIdx.x, gridDim.x);
+    const int per_sg = a.nty * a.ntz;
+    const int sg = t / per_sg;
+    t -= sg * per_sg;
+    const int band = t / (a.nty * 4);
+    const int gz = min(4, a.ntz - 4 * band);
+    t -= band * a.nty * 4;
+    const int ty = t / gz;
+    const int tz = 4 * band + t % gz;
+    fused4_tile<PML>(a, ty, tz, sg, ty + a.nty * tz);
+}
+
+}  // namespace
+
+bool fused4_supported(const AcousticFDTDState* st) {
+    return st->order == 4 && fused_supported(st) && st->storage == FDTD_STORE_F32 && !st->hetero;
+}
+
+int f4_tile_rows() { return f4::TY; }
+
+// Per-tile source and tap lists: a source goes to every tile whose extended
+// tile (rows -3..TY+2, columns -3..66) holds its cell, a tap to the tile
+// that owns its cell.
+void f4_tile_lists(int Ny, int Nz, int nsrc, const int* src_cell, const int* src_id, int ntap,
+                   const int* tap_cell, const int* tap_slot, std::vector<std::vector<int2>>& src,
+                   std::vector<std::vector<int2>>& tap) {
+    using namespace f4;
+    const int nty = (Ny + TY - 1) / TY, ntz = (Nz + TZ - 1) / TZ;
+    src.assign(nty * ntz, {});
+    tap.assign(nty * ntz, {});
+    for (int s = 0; s < nsrc; ++s) {
+        const int c = src_cell[s];
+        if (c < 0) continue;
+        const int i = c / (Ny * Nz), rem = c % (Ny * Nz);
+        const int j = rem / Nz, kk = rem % Nz;
+        for (int ty = 0; ty < nty; ++ty) {
+            const int e = j - ty * TY;
+            if (e < -H || e >= TY + H) continue;
+            for (int tz = 0; tz < ntz; ++tz) {
+                const int cc = kk - tz * TZ;
+                if (cc < -H || cc >= TZ + H) continue;
+                src[ty + nty * tz].push_back(make_int2(ekey(i, e, cc), src_id[s]));
+            }
+        }
+    }
+    for (int q = 0; q < ntap; ++q) {
+        const int c = tap_cell[q];
+        const int i = c / (Ny * Nz), rem = c % (Ny * Nz);
+        const int j = rem / Nz, kk = rem % Nz;
+        const int ty = j / TY, tz = kk / TZ;
+        tap[ty + nty * tz].push_back(make_int2(ekey(i, j - ty * TY, kk - tz * TZ), tap_slot[q]));
+    }
+}
+
+hipError_t launch_fused4_pair(AcousticFDTDState* st, int step) {
+    using namespace f4;
+    Pair4Args a;
+    memset(&a, 0, sizeof(a));
+    a.p = st->sp;
+    a.f.Pin = st->P[st->cur];
+    a.f.Pout = st->P[st->cur ^ 1];
+    for (int c = 0; c < 3; ++c) {
+        a.f.Vin[c] = st->V[c];
+        a.f.Vout[c] = st->V2[c];
+    }
+    a.alpha = st->d_alpha;
+    a.pdamp = st->d_pdamp;
+    a.status = st->d_status;
+    a.seg = st->fused_seg_x > 0 ? st->fused_seg_x : 64;
+    if (a.seg > st->Nx) a.seg = st->Nx;
+    a.nty = (st->Ny + TY - 1) / TY;
+    a.ntz = (st->Nz + TZ - 1) / TZ;
+    a.nseg = (st->Nx + a.seg - 1) / a.seg;
+    a.src_start = st->d_f2_src_start;
+    a.src = (const int2*)st->d_f2_src;
+    a.tap_start = st->d_f2_tap_start;
+    a.tap = (const int2*)st->d_f2_tap;
+    a.psig = st->d_psig;
+    a.plen = st->npsrc > 0 ? st->plen : 0;
+    a.step = step;
+    a.tapbuf = st->d_f2_tapbuf;
+    const int nb = a.nty * a.ntz * a.nseg;
+    if (st->n_pml > 0)
+        hipLaunchKernelGGL(k_fused4<true>, dim3(nb), dim3(64, NW), 0, st->stream, a);
+    else
+        hipLaunchKernelGGL(k_fused4<false>, dim3(nb), dim3(64, NW), 0, st->stream, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    for (int c = 0; c < 3; ++c) {
+        void* t = st->V[c];
+        st->V[c] = st->V2[c];
+        st->V2[c] = t;
+    }
+    return hipSuccess;
+}

@@ -172,6 +172,14 @@ void f2_tile_lists(int Nx, int Ny, int Nz, int TYO, int nsrc, const int* src_cel
                    std::vector<std::vector<int2>>& src, std::vector<std::vector<int2>>& tap);
 // Own rows of a two-step tile for tile_y 8 or 16.
 int f2_rows_for(int tile_y);   // two-step sweep: rows per thread for a tile height
+// Two-step sweep of the order-4 stencil (afdtd_fused4.hip): FP32, homogeneous
+// medium, rigid walls / PML, no walls, fractions or plate.
+bool fused4_supported(const AcousticFDTDState* st);
+int f4_tile_rows();
+void f4_tile_lists(int Ny, int Nz, int nsrc, const int* src_cell, const int* src_id, int ntap,
+                   const int* tap_cell, const int* tap_slot, std::vector<std::vector<int2>>& src,
+                   std::vector<std::vector<int2>>& tap);
+hipError_t launch_fused4_pair(AcousticFDTDState* st, int step);
 int f2_tile_rows(int tile_y);
 // Longest per-tile source / tap list the two-step sweep caches in LDS.
 constexpr int F2_MAX_SRC = 32, F2_MAX_TAP = 256;
