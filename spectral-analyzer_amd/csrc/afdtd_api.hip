@@ -611,10 +611,11 @@ AcousticFDTDState* create_impl(int Nx, int Ny, int Nz, float dx, float c, float 
         st->family = FDTD_KERNEL_FUSED2;
         st->f2_rows = f2_rows_for(o->tile_y);
     } else if (o->kernel == FDTD_KERNEL_AUTO && st->storage == FDTD_STORE_F32 &&
-               st->order == 2 && fused2_supported(st)) {
-        // FP32 order-2 rooms: two steps per sweep (the single-step sweep is
-        // bound by the streaming rate of its byte mix; 16-bit storage halves
-        // that mix and keeps the single-step sweep ahead)
+               fused2_supported(st)) {
+        // FP32 rooms without walls, fractions or plate, either stencil: two
+        // steps per sweep (the single-step sweep is bound by the streaming
+        // rate of its byte mix; 16-bit storage halves that mix and keeps the
+        // single-step sweep ahead)
         st->family = FDTD_KERNEL_FUSED2;
         st->f2_rows = f2_rows_for(o->tile_y);
     } else if (o->kernel == FDTD_KERNEL_FUSED || o->kernel == FDTD_KERNEL_AUTO) {

@@ -1,7 +1,8 @@
-"""The HIP path (default kernel choice: the two-step sweep for the order-2
-configuration, the single-step sweep for the reference's order-4 scheme)
-reproduces the committed 64^3 regression fixture bit for bit: receiver
-traces of 2000 steps and the SHA-256 of the final pressure field."""
+"""The HIP path reproduces the committed 64^3 regression fixture bit for
+bit: receiver traces of 2000 steps and the SHA-256 of the final pressure
+field, with the default kernel choice (the two-step sweep, for the order-2
+configuration as written and for the reference's order-4 scheme) and with
+the single-step sweep forced."""
 import os
 import sys
 
@@ -14,8 +15,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(HERE, "golden"))
 
 
-@pytest.mark.parametrize("tag,family", [("as_written", "fused2"), ("parity", "fused")])
-def test_hip_reproduces_config1_golden(tag, family):
+@pytest.mark.parametrize("tag,kernel,family", [("as_written", "auto", "fused2"),
+                                               ("parity", "auto", "fused2"),
+                                               ("as_written", "fused", "fused"),
+                                               ("parity", "fused", "fused")])
+def test_hip_reproduces_config1_golden(tag, kernel, family):
     import make_config1_golden as mk
     from afdtd import AcousticFDTD, workloads as W
     gold = np.load(os.path.join(HERE, "golden", "config1_golden.npz"))
@@ -26,7 +30,8 @@ def test_hip_reproduces_config1_golden(tag, family):
     def make(N):
         s = AcousticFDTD(N, N, N, w.dx, c=343.0, rho_air=1.2, cell_type=np.zeros(N ** 3, np.uint8),
                          plate_iz=N // 2, plate_active=np.zeros(N * N, np.uint8), n_pml=0,
-                         order=w.order, courant=w.courant, boundary=w.boundary, refl=w.refl)
+                         order=w.order, courant=w.courant, boundary=w.boundary, refl=w.refl,
+                         kernel=kernel)
         made.append(s)
         return s
     tr, sha = mk.run_case(make, w, 2000)
