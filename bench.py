@@ -7,6 +7,10 @@ room, 512x384x256 cells, order-2 stencil at Courant 0.55, heterogeneous
 density and bulk modulus, locally reacting walls with per-side reflection
 coefficients, 4 sources and 64 receivers (synthetic, seeded).
 
+The default line also carries `forms`: the batched, precision and adjoint
+forms and the reference's own order-4 scheme, each measured in the same run
+at a reduced step count (--no-forms skips them).
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config NAME]
 """
 import argparse
@@ -443,6 +447,13 @@ def bench_amr_audio(a):
     n, dx, n_pml = a.amr_n or 96, 0.005, 8
     grid = AcousticAMRGrid.build_uniform_box(x_min=0, x_max=n * dx, y_min=0, y_max=n * dx,
                                              z_min=0, z_max=n * dx, dx=dx).with_pml_shell(n_pml)
+    plate_kw = None
+    if a.amr_plate:
+        # a plate layer across the box's mid height inside the absorbing shell,
+        # driven through three bridge nodes between blocks
+        act = np.zeros((n, n), np.uint8)
+        act[n_pml + 2:n - n_pml - 2, n_pml + 2:n - n_pml - 2] = 1
+        grid, plate_kw = grid.with_plate_layer(n, n, n, n // 2, act)
     args = (grid.cell_centers, grid.cell_volumes, grid.open_volume_fraction, grid.cell_types,
             grid.face_cell_neg, grid.face_cell_pos, grid.face_area, grid.face_open_fraction,
             grid.face_distance, 343.0, 1.21, dx, 2)
@@ -460,6 +471,13 @@ def bench_amr_audio(a):
         m.set_border_condition(0, 3.0, 0.0, 0.0, n_pml, grid.bounds_min, grid.bounds_max)
         m.inject_pressure_nearest(np.array([0.2, 0.25, 0.22]) * (box / 0.48), 1.0)
         m.setup_mic_samplers(mics, axis, True)
+        if plate_kw is not None:
+            m.setup_plate(**plate_kw)
+            fl = plate_kw["plate_active_flat_idx"]
+            m.set_bridge_plate_sources(fl[[fl.size // 3, fl.size // 2, 2 * fl.size // 3]],
+                                       [0.5, 0.3, 0.2])
+
+    drive = np.float32([1e-3, -2e-3, 1.5e-3])
 
     setup(g)
     # blocks of `block` samples, as a co-evolver would hand them over; the
@@ -470,6 +488,8 @@ def bench_amr_audio(a):
     g.record_mic_samples(block, substeps, pa, pb, axis, gain, rho_c)
     t0 = time.perf_counter()
     for _ in range(samples // block):
+        if plate_kw is not None:
+            g.inject_bridge_drive(drive, 1.0)
         out, rc, done = g.record_mic_samples(block, substeps, pa, pb, axis, gain, rho_c)
         if rc != 0 or not np.all(np.isfinite(out)):
             raise SystemExit(f"amr_record_mic_samples failed rc={rc}")
@@ -479,8 +499,11 @@ def bench_amr_audio(a):
             "warmup": block, "ms_per_step": wall * 1e3 / samples,
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
             "data": "synthetic (uniform box grid, one pulse)",
-            "config": {"workload": f"amr-audio-box-{n}x{n}x{n}-dx5mm-48kHz", "cells": g.n_cells,
-                       "faces": g.n_faces, "substeps": substeps, "mics": 4,
+            "config": {"workload": f"amr-audio-box-{n}x{n}x{n}-dx5mm-48kHz"
+                                   + ("-plate" if plate_kw is not None else ""),
+                       "cells": g.n_cells, "faces": g.n_faces, "substeps": substeps, "mics": 4,
+                       "plate_nodes": 0 if plate_kw is None else
+                       int(plate_kw["plate_active_flat_idx"].size),
                        "samples_per_block": block,
                        "record_layout": g.record_layout},
             "realtime_factor_48k": samples / wall / sr,
@@ -493,6 +516,8 @@ def bench_amr_audio(a):
         t0 = time.perf_counter()
         k = 0
         while time.perf_counter() - t0 < 10.0:
+            if plate_kw is not None and k % block == 0:
+                o.inject_bridge_drive(drive, 1.0)
             o.step(substeps)
             sp, vx, vy, vz = o.sample_mics()
             _ = (sp * pa + (pb * rho_c) * (vx * axis[:, 0] + vy * axis[:, 1] + vz * axis[:, 2])) * gain
@@ -505,6 +530,135 @@ def bench_amr_audio(a):
                                           f"oracle, OpenMP over faces and cells)"}
     print(json.dumps(line), flush=True)
     return 0
+
+
+def _timed_room(w, storage="f32", kernel="auto", steps=40, warmup=4):
+    """Steps of workload w on one handle: (cell-updates/s wall clock, average
+    stencil ms per launch, kernel family)."""
+    s = make_solver(w, storage, kernel)
+    total = warmup + steps
+    s.set_point_sources(w.src, w.signals(total))
+    s.set_receivers(w.rec)
+    s.record_receivers(total)
+    s.step(warmup)
+    t0 = time.perf_counter()
+    _, ms_stencil = s.step_timed(steps)
+    wall = time.perf_counter() - t0
+    fam = s.kernel_family
+    if not np.all(np.isfinite(s.get_receiver_traces())):
+        raise RuntimeError("non-finite receiver traces")
+    del s
+    per_launch = 2 if fam == "fused2" else 1
+    return w.cells * steps / wall, ms_stencil * per_launch, fam
+
+
+def measure_forms(steps=40):
+    """The three forms beside the plain drop-in and the reference's own
+    scheme, each at a reduced step count in the same run as the headline
+    line: batched rooms against per-room launches, 16-bit storage against
+    FP32 on the precision room, the adjoint's reverse rate, and the order-4
+    PML room.  A form that fails reports its error instead of a number."""
+    from afdtd import RoomBatch, AcousticFDTD, workloads as W
+    forms = {}
+
+    def guard(name, fn):
+        try:
+            t0 = time.perf_counter()
+            forms[name] = fn()
+            forms[name]["bench_seconds"] = round(time.perf_counter() - t0, 2)
+        except Exception as e:                       # reported, not fatal
+            forms[name] = {"error": f"{type(e).__name__}: {e}"}
+        print(f"[bench] form {name}: {forms[name]}", file=sys.stderr, flush=True)
+
+    def reference_scheme():
+        w = W.config2_order4_pml(steps=steps + 4)
+        v, ms, fam = _timed_room(w, steps=steps)
+        achieved = 2 * w.cells * 32 / (ms * 1e-3) / 1e9 if fam == "fused2" else \
+            w.cells * 32 / (ms * 1e-3) / 1e9
+        return {"workload": w.name, "value": v, "unit": "cell-updates/s", "kernel": fam,
+                "steps": steps, "roofline_frac": achieved / HBM_PEAK_GBS}
+
+    def batched():
+        c = W.config3(steps=steps)
+        dims = c["dims"]
+        cells = int(np.prod(dims, axis=1).sum())
+        b = RoomBatch(dims, 0.01, c["courant"], c["refl"], c=343.0, rho_air=1.2, order=2)
+        tuned = b.autotune(trial_steps=6)
+        b.set_sources(c["src"], c["signals"])
+        b.set_receivers(c["rec"])
+        b.record(steps + 4)
+        b.step(4)
+        t0 = time.perf_counter()
+        b.step(steps, graph=True)
+        wall_b = time.perf_counter() - t0
+        del b
+        t_rooms = 0.0
+        for r, (Nx, Ny, Nz) in enumerate(dims):
+            Nx, Ny, Nz = int(Nx), int(Ny), int(Nz)
+            h = AcousticFDTD(Nx, Ny, Nz, 0.01, c=343.0, rho_air=1.2,
+                             cell_type=np.zeros(Nx * Ny * Nz, np.uint8), plate_iz=Nz // 2,
+                             plate_active=np.zeros(Nx * Ny, np.uint8), n_pml=0, order=2,
+                             courant=float(c["courant"][r]), boundary="impedance",
+                             refl=c["refl"][r])
+            h.set_point_sources([c["src"][r]], c["signals"][r:r + 1])
+            h.set_receivers(c["rec"][r])
+            h.step(4)
+            t0 = time.perf_counter()
+            h.step(steps)
+            t_rooms += time.perf_counter() - t0
+            del h
+        return {"workload": "config3-256-rooms-48to160-order2", "rooms": int(len(dims)),
+                "cells": cells, "steps": steps, "value": cells * steps / wall_b,
+                "unit": "cell-updates/s", "per_room_value": cells * steps / t_rooms,
+                "speedup_vs_per_room": t_rooms / wall_b, "autotuned": tuned}
+
+    def precision():
+        w = W.config4(steps=steps + 4)
+        out = {"workload": w.name, "steps": steps, "unit": "cell-updates/s"}
+        for st, kern in (("f32", "auto"), ("bf16", "auto"), ("bf16", "fused2"),
+                         ("f16", "auto"), ("bfp16", "auto")):
+            v, _, fam = _timed_room(w, storage=st, kernel=kern, steps=steps)
+            out[f"{st}_{fam}"] = v
+        base = out["f32_fused2"]
+        out["bf16_vs_f32"] = max(out["bf16_fused"], out["bf16_fused2"]) / base
+        out["bfp16_vs_f32"] = out["bfp16_fused"] / base
+        return out
+
+    def adjoint():
+        import torch
+        from afdtd.adjoint import AdjointSolver, simulate_traces
+        c = W.config5(steps=10 * steps)
+        shape, n = c["shape"], c["steps"]
+        N = int(np.prod(shape))
+        sol = AdjointSolver(*shape, c["dtdx"], dtype="float32", checkpoint_every=100)
+        sol.set_sources(c["src"], c["signals"])
+        sol.set_receivers(c["rec"])
+        dev = lambda x: torch.tensor(np.ascontiguousarray(x).ravel(), dtype=torch.float32,
+                                     device="cuda")
+        with torch.no_grad():
+            target = sol.forward(dev(c["m"] + c["pert"][0]), dev(c["l"] + c["pert"][1]),
+                                 dev(c["eta"]), n).clone()
+        mg, lg, eg = (dev(x).requires_grad_() for x in (c["m"], c["l"], c["eta"]))
+        ((simulate_traces(sol, mg, lg, eg, n) - target) ** 2).sum().backward()
+        mg.grad = lg.grad = eg.grad = None
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        loss = ((simulate_traces(sol, mg, lg, eg, n) - target) ** 2).sum()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        loss.backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        return {"workload": "config5-128cube-ckpt100", "steps": n, "cells": N,
+                "value": N * n / (t2 - t1), "unit": "reverse cell-updates/s",
+                "forward_value": N * n / (t1 - t0), "roofline_frac":
+                N * n / (t2 - t1) * 104 / 1e9 / HBM_PEAK_GBS}
+
+    guard("reference_scheme_order4", reference_scheme)
+    guard("batched", batched)
+    guard("precision", precision)
+    guard("adjoint", adjoint)
+    return forms
 
 
 def main():
@@ -523,10 +677,14 @@ def main():
                     help="amr: gradient order of the AMR stepper")
     ap.add_argument("--amr-n", type=int, default=0,
                     help="amr-audio: cells per box edge (default 96)")
+    ap.add_argument("--amr-plate", action="store_true",
+                    help="amr-audio: a plate layer across the box, driven between blocks")
     ap.add_argument("--amr-layout", default="auto", choices=("auto", "direct", "compact"),
                     help="amr: record layout of the step kernels")
     ap.add_argument("--no-fd", action="store_true",
                     help="adjoint: float32 timing only (no float64 pass, no finite differences)")
+    ap.add_argument("--no-forms", action="store_true",
+                    help="room: skip the forms measured beside the headline line")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one MI355X: the multi-GPU (slab-sharded) path is a "
@@ -596,7 +754,8 @@ def main():
                    "courant": w.courant, "boundary": w.boundary, "heterogeneous": w.hetero,
                    "sources": len(w.src), "receivers": int(len(w.rec)),
                    "kernel": s.kernel_family, "storage": a.storage,
-                   "tile_y": a.tile_y or 16,
+                   "tile_y": a.tile_y or (24 if (s.kernel_family == "fused2" and w.order == 4)
+                                          else 16),
                    "seg_x": a.seg_x or (64 if s.kernel_family == "fused2" else 32),
                    "steps_per_launch": steps_per_launch},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
@@ -622,6 +781,8 @@ def main():
     line["roofline"]["frac_of_stream_ceiling"] = achieved / ceiling
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(w)
+    if a.config == "room" and not a.no_forms:
+        line["forms"] = measure_forms()
     print(json.dumps(line), flush=True)
     return 0
 

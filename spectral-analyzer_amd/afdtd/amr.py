@@ -449,6 +449,38 @@ class AcousticAMRGrid:
         return AcousticAMRGrid(**{**self.__dict__, "cell_types": ct})
 
 
+    def with_plate_layer(self, nx: int, ny: int, nz: int, kp: int, active=None,
+                         plate_dx: float = None, rho_h: float = 7.0 * 0.003, D: float = 0.45,
+                         alpha_M: float = 2.0, beta_K: float = 1e-5):
+        """A uniform nx x ny x nz box from build_uniform_box with a Kirchhoff
+        plate in layer kp: plate node (i, j) is cell (i, j, kp), typed PLATE
+        where active; its above face joins it to (i, j, kp+1), its below face
+        joins (i, j, kp-1) to it.  Returns (grid, keyword arguments of
+        AcousticAMR.setup_plate).  Plate nodes are 5 cells apart by default,
+        which keeps the explicit plate update stable at the air time step."""
+        act = np.ones((nx, ny), np.uint8) if active is None else np.asarray(active, np.uint8)
+        ct = self.cell_types.copy().reshape(nx, ny, nz)
+        ct[:, :, kp][act > 0] = 2
+        flat = np.flatnonzero(act.ravel()).astype(np.int32)
+        i, j = flat // ny, flat % ny
+        f0 = (nx - 1) * ny * nz + nx * (ny - 1) * nz          # first z face
+        zf = lambda ii, jj, kk: (f0 + (ii * ny + jj) * (nz - 1) + kk).astype(np.int32)
+        cell = lambda ii, jj, kk: ((ii * ny + jj) * nz + kk).astype(np.int32)
+        n = flat.size
+        pdx = 5 * self.min_dx if plate_dx is None else plate_dx
+        kw = dict(plate_Nx=nx, plate_Ny=ny, plate_dx=pdx,
+                  plate_origin=[float(self.bounds_min[0]), float(self.bounds_min[1]),
+                                float(self.bounds_min[2]) + kp * self.min_dx],
+                  plate_active=act.ravel(), plate_rho_h=rho_h, plate_D=D,
+                  plate_alpha_M=alpha_M, plate_beta_K=beta_K, plate_active_flat_idx=flat,
+                  face_above_starts=np.arange(n + 1, dtype=np.int32),
+                  face_above_idx=zf(i, j, kp), face_above_wgt=np.ones(n, np.float32),
+                  face_below_starts=np.arange(n + 1, dtype=np.int32),
+                  face_below_idx=zf(i, j, kp - 1), face_below_wgt=np.ones(n, np.float32),
+                  cell_above=cell(i, j, kp + 1), cell_below=cell(i, j, kp - 1))
+        return AcousticAMRGrid(**{**self.__dict__, "cell_types": ct.ravel()}), kw
+
+
 class AcousticAMRFDTD:
     """Handle for the GPU AMR stepper (acoustic_amr.py:314-375)."""
 
