@@ -532,6 +532,89 @@ def bench_amr_audio(a):
     return 0
 
 
+def bench_body(a):
+    """A voxelised two-bout body at the bridge factory's spacing (dx = 4 mm,
+    the spacing at which the plate's explicit update with the reference's
+    default plate constants is stable: its stiffness-proportional damping
+    term bK/dt * D dt^2/(m dx^4) grows as dx shrinks), scaled 4x so that the
+    grid is HBM-sized (256-vertex outline 1.44 m x 2 m, rib height 0.4 m,
+    24 cells of padding, 10-cell PML): the soundhole cut, cut-cell face
+    fractions, an active plate and three Gaussian bridge sources
+    (FDTDCoEvolver.from_outline), i.e. the geometry the bridge feeds the
+    solver, which runs the generic kernels.  Timed: the steps after one
+    bridge drive (the stencil), and the bridge's per-sample loop at 44.1 kHz
+    (drive, steps_per_sample steps).  Algorithmic bytes per cell-update: the
+    four fields read and written (32 B), the three face-fraction arrays read
+    (12 B) and the cell type (1 B) = 45 B; the plate (O(surface)) is left out."""
+    from afdtd.bridge import FDTDCoEvolver
+    from afdtd.geometry import two_bout_outline
+    scale, dx, pad = 4.0, 0.004, 24
+    poly = (two_bout_outline(256) * scale).astype(np.float32)
+    body_h = 0.1 * scale
+    bridges = [(-0.08, -0.52), (0.0, -0.52), (0.08, -0.52)]
+    hole = (0.0, 0.24, 0.18)
+    ev = FDTDCoEvolver.from_outline(poly, body_h, bridges, soundhole=hole, dx=dx,
+                                    pad_cells=pad, n_pml=10)
+    info, f = ev.info, ev.fdtd
+    cells = info["Nx"] * info["Ny"] * info["Nz"]
+    steps = a.steps if a.steps != 4000 else 400
+    nb = len(ev.bridge_cells)
+    f.inject_bridge_drive(np.full(nb, 1e-3, np.float32), 1.0)
+    f.step(a.warmup)
+    t0 = time.perf_counter()
+    ms_events, ms_stencil = f.step_timed(steps)
+    wall = time.perf_counter() - t0
+    if not np.all(np.isfinite(f.get_pressure_field())):
+        raise SystemExit("non-finite pressure")
+    bpc = 45
+    achieved = cells * bpc / (ms_stencil * 1e-3) / 1e9
+    # the bridge's per-sample loop: one drive + steps_per_sample steps per sample
+    ns = 64
+    drives = (1e-4 * np.random.default_rng(3).standard_normal((ns, nb))).astype(np.float32)
+    t0 = time.perf_counter()
+    ev.drive_bridge_block(drives)
+    wall_s = time.perf_counter() - t0
+    line = {"metric": "cell-updates/s (3D FDTD, voxelised body, generic kernels, 1 GPU)",
+            "value": cells * steps / wall, "unit": "cell-updates/s", "n_gpus": 1,
+            "steps": steps, "warmup": a.warmup, "ms_per_step": wall * 1e3 / steps,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+            "data": "synthetic (two-bout outline, soundhole, bridge drive)",
+            "config": {"workload": f"body-two-bout-x4-{info['Nx']}x{info['Ny']}x{info['Nz']}-dx4mm",
+                       "cells": cells, "kernel": f.kernel_family, "bridge_cells": nb,
+                       "steps_per_sample": ev.steps_per_sample},
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                         "bytes_per_cell_update": bpc, "cell_updates_per_launch": cells,
+                         "stencil_ms_per_step": ms_stencil, "event_ms_total": ms_events},
+            "bridge_loop": {"samples": ns, "samples_per_s": ns / wall_s,
+                            "realtime_factor_44k1": ns / wall_s / 44100.0}}
+    del ev, f
+    if not a.no_cpu:
+        from afdtd.geometry import apply_soundhole_cutout
+        from oracle import geometry as og
+        from oracle.oracle import OracleFDTD
+        ct, pa, g = og.voxelise(poly, body_h, dx, pad)
+        apply_soundhole_cutout(ct, pa, g, hole)
+        o = OracleFDTD(g["Nx"], g["Ny"], g["Nz"], dx, c=343.0, rho_air=1.21,
+                       cell_type=ct.ravel(), plate_iz=g["plate_iz"], plate_active=pa.ravel(),
+                       n_pml=10, precision="f32")
+        o.set_face_fractions(*og.face_fractions(ct, pa, poly, g))
+        o.step(1)
+        t0 = time.perf_counter()
+        n = 0
+        while time.perf_counter() - t0 < 15.0:
+            o.step(2)
+            n += 2
+        el = time.perf_counter() - t0
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+        line["cpu_baseline"] = {"value": cells * n / el, "unit": "cell-updates/s",
+                                "cores": threads, "kind": "port",
+                                "sample": f"{n} steps of the same body grid ({el:.1f} s, "
+                                          f"float32 oracle, OpenMP over x slabs)"}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def _timed_room(w, storage="f32", kernel="auto", steps=40, warmup=4):
     """Steps of workload w on one handle: (cell-updates/s wall clock, average
     stencil ms per launch, kernel family)."""
@@ -703,6 +786,8 @@ def main():
         return bench_adjoint(a)
     if a.config == "geometry":
         return bench_geometry(a)
+    if a.config == "body":
+        return bench_body(a)
     if a.config == "amr":
         return bench_amr(a)
     if a.config == "amr-audio":

@@ -139,3 +139,44 @@ def compute_cut_cell_fractions(cell_type, plate_active, outline_pts, info: dict)
     if rc != FDTD_OK:
         raise RuntimeError(f"compute_cut_cell_fractions failed (rc={rc}): {last_error()}")
     return vx, vy, vz
+
+
+def apply_soundhole_cutout(cell_type, plate_active, info: dict, soundhole) -> None:
+    """Open a circular soundhole (cx, cy, r) in the plate layer, in place:
+    columns whose centre lies strictly inside the circle lose their plate node
+    and their plate-layer cell becomes air (acoustic_fdtd_bridge.py:411-421,
+    same float64 expressions)."""
+    if soundhole is None:
+        return
+    cx, cy, r = soundhole
+    xs = info["gx_min"] + (np.arange(info["Nx"]) + 0.5) * float(info["dx"])
+    ys = info["gy_min"] + (np.arange(info["Ny"]) + 0.5) * float(info["dx"])
+    X, Y = np.meshgrid(xs, ys, indexing="ij")
+    mask = (X - cx) ** 2 + (Y - cy) ** 2 < r ** 2
+    plate_active[mask] = 0
+    cell_type[:, :, int(info["plate_iz"])][mask] = 0
+
+
+def build_bridge_kernel(bridge_positions_xy, info: dict, sigma_m: float = 0.015):
+    """(cell_indices int32, weights float32) of the bridge saddle sources
+    (acoustic_fdtd_bridge.py:559-599): for each saddle position (x, y), every
+    cell of the layer just below the plate whose column centre gives a
+    Gaussian weight exp(-r^2 / (2 sigma^2)) of at least 1e-4, in the
+    reference's (position, i, j) order and double-precision expression."""
+    Nx, Ny, Nz = int(info["Nx"]), int(info["Ny"]), int(info["Nz"])
+    xs, ys = info["xs"], info["ys"]
+    iz = max(0, int(info["plate_iz"]) - 1)
+    sigma2 = sigma_m * sigma_m
+    idx, wgt = [], []
+    for bx, by in bridge_positions_xy:
+        for i in range(Nx):
+            dxi = (xs[i] - bx) ** 2
+            for j in range(Ny):
+                w = math.exp(-(dxi + (ys[j] - by) ** 2) / (2 * sigma2))
+                if w < 1e-4:
+                    continue
+                idx.append(iz + Nz * (j + Ny * i))
+                wgt.append(float(w))
+    if not idx:
+        raise ValueError("No bridge source cells found — check bridge positions overlap the grid")
+    return np.array(idx, dtype=np.int32), np.array(wgt, dtype=np.float32)

@@ -170,3 +170,41 @@ def test_plate_rule_needs_an_interior_plate_layer():
     bfr = _brute_fractions(poly, ct, pa, g)
     for a, b in zip(fr, bfr):
         assert np.array_equal(a, b)
+
+
+def test_bridge_kernel_and_soundhole_brute_force():
+    """build_bridge_kernel (acoustic_fdtd_bridge.py:559-599) against a
+    per-position, per-column loop; apply_soundhole_cutout (:411-421) against
+    a per-column loop of the circle test."""
+    import math as m
+    from afdtd.geometry import apply_soundhole_cutout, build_bridge_kernel, two_bout_outline
+    from oracle import geometry as og
+    poly = two_bout_outline(64)
+    ct, pa, g = og.voxelise(poly, 0.1, 0.008, 6)
+    bridges = [(0.01, -0.1), (-0.03, -0.12)]
+    idx, wgt = build_bridge_kernel(bridges, g, sigma_m=0.02)
+    ei, ew = [], []
+    iz = g["plate_iz"] - 1
+    for bx, by in bridges:
+        for i in range(g["Nx"]):
+            for j in range(g["Ny"]):
+                x = g["gx_min"] + (i + 0.5) * g["dx"]
+                y = g["gy_min"] + (j + 0.5) * g["dx"]
+                w = m.exp(-((x - bx) ** 2 + (y - by) ** 2) / (2 * 0.02 ** 2))
+                if w >= 1e-4:
+                    ei.append(iz + g["Nz"] * (j + g["Ny"] * i))
+                    ew.append(w)
+    assert np.array_equal(idx, np.array(ei, np.int32))
+    np.testing.assert_allclose(wgt, np.array(ew, np.float32), rtol=1e-6)
+    assert np.all(idx % g["Nz"] == iz)
+    ct2, pa2 = ct.copy(), pa.copy()
+    apply_soundhole_cutout(ct2, pa2, g, (0.0, 0.05, 0.045))
+    for i in range(g["Nx"]):
+        for j in range(g["Ny"]):
+            x = g["gx_min"] + (i + 0.5) * g["dx"]
+            y = g["gy_min"] + (j + 0.5) * g["dx"]
+            hole = (x - 0.0) ** 2 + (y - 0.05) ** 2 < 0.045 ** 2
+            assert pa2[i, j] == (0 if hole else pa[i, j])
+            assert ct2[i, j, g["plate_iz"]] == (0 if hole else ct[i, j, g["plate_iz"]])
+    assert (pa2 != pa).sum() > 10
+    assert np.array_equal(np.delete(ct2, g["plate_iz"], 2), np.delete(ct, g["plate_iz"], 2))
