@@ -243,6 +243,39 @@ inline int packed_to_padded(int ax, int q, int Nx, int Ny, int Nz) {
     return k + Nz * t;
 }
 
+// Entries of a force injection grouped by target plate node p = idx/div
+// (entries out of range or on inactive nodes dropped, as the reference's
+// loops skip them): start[ng+1] | node[ng] | entry[...], each node's entries
+// in call order.  Returns the group count.
+int force_groups(const int* idx, int n, int div, const uint8_t* mask, int nmax,
+                 std::vector<int>& grp) {
+    std::vector<std::pair<int, int>> e;
+    e.reserve(n);
+    for (int i = 0; i < n; ++i) {
+        const int p = idx[i] / div;
+        if (p < 0 || p >= nmax) continue;
+        if (mask && !mask[p]) continue;
+        e.emplace_back(p, i);
+    }
+    std::stable_sort(e.begin(), e.end(),
+                     [](const std::pair<int, int>& a, const std::pair<int, int>& b) {
+                         return a.first < b.first;
+                     });
+    std::vector<int> start, node;
+    for (size_t q = 0; q < e.size(); ++q)
+        if (q == 0 || e[q].first != e[q - 1].first) {
+            start.push_back((int)q);
+            node.push_back(e[q].first);
+        }
+    start.push_back((int)e.size());
+    const int ng = (int)node.size();
+    grp.clear();
+    grp.insert(grp.end(), start.begin(), start.end());
+    grp.insert(grp.end(), node.begin(), node.end());
+    for (const auto& x : e) grp.push_back(x.second);
+    return ng;
+}
+
 int upload_fracs(AcousticFDTDState* st, const float* fx, const float* fy, const float* fz) {
     int Nx = st->Nx, Ny = st->Ny, Nz = st->Nz;
     if (fx && st->Nvx > 0)
@@ -312,7 +345,8 @@ void destroy_state(AcousticFDTDState* st) {
     for (int a = 0; a < 3; ++a) dfree(st->d_w[a]);
     dfree(st->d_ext); dfree(st->d_act); dfree(st->d_above); dfree(st->d_below);
     dfree(st->d_b13); dfree(st->d_vzb); dfree(st->d_vza);
-    dfree(st->d_bidx); dfree(st->d_bdrive); dfree(st->d_fidx);
+    dfree(st->d_bidx); dfree(st->d_bdrive); dfree(st->d_fidx); dfree(st->d_fval);
+    dfree(st->d_bgrp);
     dfree(st->d_pidx); dfree(st->d_psig);
     dfree(st->d_ridx); dfree(st->d_rwgt); dfree(st->d_trace);
     dfree(st->d_gidx); dfree(st->d_gwgt); dfree(st->d_gout);
@@ -836,11 +870,17 @@ SK_API int fdtd_set_bridge_sources(AcousticFDTDState* st, int n, const int* idx,
     st->h_bidx.assign(idx, idx + n);
     st->h_bwgt.resize(n);
     for (int i = 0; i < n; ++i) st->h_bwgt[i] = wgt[i] * wscale;
-    dfree(st->d_bidx); dfree(st->d_bdrive);
-    st->d_bidx = nullptr; st->d_bdrive = nullptr;
+    dfree(st->d_bidx); dfree(st->d_bdrive); dfree(st->d_bgrp);
+    st->d_bidx = nullptr; st->d_bdrive = nullptr; st->d_bgrp = nullptr;
+    st->nbsrc = 0;
+    std::vector<int> grp;
+    const int ng = force_groups(idx, n, st->Nz, nullptr, st->Nplate, grp);
     AFDTD_CHECK(dalloc(&st->d_bidx, n));
     AFDTD_CHECK(dalloc(&st->d_bdrive, n));
+    AFDTD_CHECK(dalloc(&st->d_bgrp, grp.size()));
     AFDTD_CHECK(hipMemcpy(st->d_bidx, idx, sizeof(int) * n, hipMemcpyHostToDevice));
+    AFDTD_CHECK(hipMemcpy(st->d_bgrp, grp.data(), sizeof(int) * grp.size(), hipMemcpyHostToDevice));
+    st->nbgrp = ng;
     st->nbsrc = n;
     return FDTD_OK;
 }
@@ -857,8 +897,8 @@ SK_API int fdtd_inject_bridge_drive(AcousticFDTDState* st, const float* drives, 
     AFDTD_CHECK(hipMemcpyAsync(st->d_bdrive, drives, sizeof(float) * st->nbsrc,
                                hipMemcpyHostToDevice, st->stream));
     float inv_dx2 = 1.0f / (st->dx * st->dx);
-    AFDTD_CHECK(launch_accumulate(st->d_ext, st->d_bidx, st->d_bdrive, st->nbsrc, fs, inv_dx2,
-                                  st->Nz, nullptr, st->Nplate, st->d_status, st->stream));
+    AFDTD_CHECK(launch_accumulate_groups(st->d_ext, st->d_bgrp, st->nbgrp, st->d_bdrive, fs,
+                                         inv_dx2, st->d_status, st->stream));
     return sync(st);
 }
 
@@ -866,25 +906,24 @@ SK_API int fdtd_inject_plate_force(AcousticFDTDState* st, int n, const int* pidx
                                    const float* wgt, float total) {
     if (!st || !pidx || !wgt) return FDTD_ERR_NULL;
     if (n <= 0 || total == 0.0f) return FDTD_OK;
-    if (n > st->fcap) {
-        dfree(st->d_fidx); dfree(st->d_bdrive);
-        st->d_fidx = nullptr; st->d_bdrive = nullptr;
-        int cap = std::max(n, st->nbsrc);
+    std::vector<int> grp;                                               // :1735-1740
+    const int ng = force_groups(pidx, n, 1, st->h_pa.data(), st->Nplate, grp);
+    if (ng == 0) return FDTD_OK;
+    if ((int)grp.size() > st->fcap || n > st->fcap) {
+        dfree(st->d_fidx); dfree(st->d_fval);
+        st->d_fidx = nullptr; st->d_fval = nullptr; st->fcap = 0;
+        const int cap = std::max((int)grp.size(), n);
         AFDTD_CHECK(dalloc(&st->d_fidx, cap));
-        AFDTD_CHECK(dalloc(&st->d_bdrive, cap));
+        AFDTD_CHECK(dalloc(&st->d_fval, cap));
         st->fcap = cap;
     }
-    uint8_t* mask = nullptr;
-    AFDTD_CHECK(dalloc(&mask, st->Nplate));
-    AFDTD_CHECK(hipMemcpyAsync(mask, st->h_pa.data(), st->Nplate, hipMemcpyHostToDevice, st->stream));
-    AFDTD_CHECK(hipMemcpyAsync(st->d_fidx, pidx, sizeof(int) * n, hipMemcpyHostToDevice, st->stream));
-    AFDTD_CHECK(hipMemcpyAsync(st->d_bdrive, wgt, sizeof(float) * n, hipMemcpyHostToDevice, st->stream));
+    AFDTD_CHECK(hipMemcpyAsync(st->d_fidx, grp.data(), sizeof(int) * grp.size(),
+                               hipMemcpyHostToDevice, st->stream));
+    AFDTD_CHECK(hipMemcpyAsync(st->d_fval, wgt, sizeof(float) * n, hipMemcpyHostToDevice, st->stream));
     float inv_dx2 = 1.0f / (st->dx * st->dx);
-    AFDTD_CHECK(launch_accumulate(st->d_ext, st->d_fidx, st->d_bdrive, n, total, inv_dx2, 1, mask,
-                                  st->Nplate, st->d_status, st->stream));
-    int rc = sync(st);
-    dfree(mask);
-    return rc;
+    AFDTD_CHECK(launch_accumulate_groups(st->d_ext, st->d_fidx, ng, st->d_fval, total, inv_dx2,
+                                         st->d_status, st->stream));
+    return sync(st);
 }
 
 SK_API int fdtd_set_face_fractions(AcousticFDTDState* st, const float* fx, const float* fy,

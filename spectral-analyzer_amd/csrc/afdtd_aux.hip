@@ -132,20 +132,23 @@ __global__ void k_fill(float* p, size_t n, float v) {
     if (i < n) p[i] = v;
 }
 
-// Sequential accumulation ext[idx[i] / div] += (scale * val[i]) * (1/dx^2)
-// performed by one thread, in index order, as the reference's loops
-// (acoustic_fdtd.cpp:1716-1719, :1735-1740).  `val` already carries the
-// per-entry factor; `mask` (optional) skips inactive plate nodes.
-__global__ void k_accumulate(float* ext, const int* idx, const float* val, int n, float scale,
-                             float inv_dx2, int div, const uint8_t* mask, int nmax,
-                             const int* status) {
-    if (status[0] || threadIdx.x != 0 || blockIdx.x != 0) return;
-    for (int i = 0; i < n; ++i) {
-        int p = idx[i] / div;
-        if (p < 0 || p >= nmax) continue;
-        if (mask && !mask[p]) continue;
-        ext[p] += scale * val[i] * inv_dx2;
-    }
+// Force injection ext[p] += (scale * val[i]) * (1/dx^2) (acoustic_fdtd.cpp:
+// 1716-1719, :1735-1740), grouped by target node: node grp[ng + 1 + g]
+// receives its entries grp[2 ng + 1 + q], q in [grp[g], grp[g+1]), in call
+// order, so every node's sum is the reference's sequential loop's and the
+// nodes run in parallel.
+__global__ void k_accumulate_groups(float* ext, const int* __restrict__ grp, int ng,
+                                    const float* __restrict__ val, float scale, float inv_dx2,
+                                    const int* status) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (status[0] || g >= ng) return;
+    const int* start = grp;
+    const int* node = grp + ng + 1;
+    const int* ent = grp + 2 * ng + 1;
+    const int p = node[g];
+    float acc = ext[p];
+    for (int q = start[g]; q < start[g + 1]; ++q) acc += scale * val[ent[q]] * inv_dx2;
+    ext[p] = acc;
 }
 
 template <typename T>
@@ -312,11 +315,12 @@ hipError_t launch_fill(float* p, size_t n, float v, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_accumulate(float* ext, const int* idx, const float* val, int n, float scale,
-                              float inv_dx2, int div, const uint8_t* mask, int nmax,
-                              const int* status, hipStream_t s) {
-    hipLaunchKernelGGL(k_accumulate, dim3(1), dim3(64), 0, s, ext, idx, val, n, scale, inv_dx2,
-                       div, mask, nmax, status);
+hipError_t launch_accumulate_groups(float* ext, const int* grp, int ng, const float* val,
+                                    float scale, float inv_dx2, const int* status,
+                                    hipStream_t s) {
+    if (ng <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_accumulate_groups, dim3(blocks_for(ng, 256)), dim3(256), 0, s, ext, grp,
+                       ng, val, scale, inv_dx2, status);
     return hipGetLastError();
 }
 

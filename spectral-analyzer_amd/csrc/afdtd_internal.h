@@ -104,10 +104,13 @@ struct AcousticFDTDState {
     // Bridge sources.
     int nbsrc;
     int* d_bidx;
-    float* d_bdrive;        // staging for drives (nbsrc) / plate forces
+    float* d_bdrive;        // staging for the bridge drives (nbsrc)
+    int* d_bgrp;            // bridge entries grouped by plate node: start | node | entry
+    int nbgrp;
     std::vector<int> h_bidx;
     std::vector<float> h_bwgt;
-    int* d_fidx;            // plate-force staging (indices)
+    int* d_fidx;            // plate-force staging: grouped entries (start | node | entry)
+    float* d_fval;          // plate-force staging: weights
     int fcap;
 
     // Point sources.
@@ -222,9 +225,11 @@ hipError_t launch_plate_inject(AcousticFDTDState* st);
 hipError_t launch_post_step(AcousticFDTDState* st, int step_index, int trace_row);
 hipError_t launch_stability_check(AcousticFDTDState* st, int start, int step_index);
 hipError_t launch_fill(float* p, size_t n, float v, hipStream_t s);
-hipError_t launch_accumulate(float* ext, const int* idx, const float* val, int n,
-                             float scale, float inv_dx2, int div, const uint8_t* mask,
-                             int nmax, const int* status, hipStream_t s);
+// Force injection grouped by target node (grp = start[ng+1] | node[ng] |
+// entry[...]): one thread per node adds its entries in call order.
+hipError_t launch_accumulate_groups(float* ext, const int* grp, int ng, const float* val,
+                                    float scale, float inv_dx2, const int* status,
+                                    hipStream_t s);
 // Ncells / Nz / n locate the per-row exponents of block-floating-point fields
 // (ignored for the other storage types).
 hipError_t launch_gather(const void* F, int storage, float scale, const int* idx,
