@@ -541,11 +541,14 @@ def bench_body(a):
     24 cells of padding, 10-cell PML): the soundhole cut, cut-cell face
     fractions, an active plate and three Gaussian bridge sources
     (FDTDCoEvolver.from_outline), i.e. the geometry the bridge feeds the
-    solver, which runs the generic kernels.  Timed: the steps after one
-    bridge drive (the stencil), and the bridge's per-sample loop at 44.1 kHz
-    (drive, steps_per_sample steps).  Algorithmic bytes per cell-update: the
-    four fields read and written (32 B), the three face-fraction arrays read
-    (12 B) and the cell type (1 B) = 45 B; the plate (O(surface)) is left out."""
+    solver, which runs the single-step sweep with cell geometry (--kernel
+    generic: the two-kernel path).  Timed: the steps after one bridge drive
+    (the stencil), and the bridge's per-sample loop at 44.1 kHz (drive,
+    steps_per_sample steps).  Algorithmic bytes per cell-update: the four
+    fields read and written (32 B) and one geometry code byte (walls, closed,
+    fractional and plate faces) = 33 B; the fractions of cut faces and the
+    plate (O(surface)) are left out.  The reference's own layout would read
+    the cell type and three fraction arrays, 45 B."""
     from afdtd.bridge import FDTDCoEvolver
     from afdtd.geometry import two_bout_outline
     scale, dx, pad = 4.0, 0.004, 24
@@ -554,7 +557,8 @@ def bench_body(a):
     bridges = [(-0.08, -0.52), (0.0, -0.52), (0.08, -0.52)]
     hole = (0.0, 0.24, 0.18)
     ev = FDTDCoEvolver.from_outline(poly, body_h, bridges, soundhole=hole, dx=dx,
-                                    pad_cells=pad, n_pml=10)
+                                    pad_cells=pad, n_pml=10, kernel=a.kernel,
+                                    tile_y=a.tile_y, seg_x=a.seg_x)
     info, f = ev.info, ev.fdtd
     cells = info["Nx"] * info["Ny"] * info["Nz"]
     steps = a.steps if a.steps != 4000 else 400
@@ -566,15 +570,22 @@ def bench_body(a):
     wall = time.perf_counter() - t0
     if not np.all(np.isfinite(f.get_pressure_field())):
         raise SystemExit("non-finite pressure")
-    bpc = 45
+    bpc = 33
     achieved = cells * bpc / (ms_stencil * 1e-3) / 1e9
+    traffic = None
+    if f.kernel_family == "fused" and a.tile_y in (0, 8) and a.seg_x == 0:
+        prof = os.path.join(ROOT, "profiles", "fused_geo_body_x4_395x549x148_order4_pml10.txt")
+        if os.path.exists(prof):
+            for ln in open(prof):
+                if ln.startswith("traffic_bytes_per_launch:"):
+                    traffic = int(ln.split(":")[1])
     # the bridge's per-sample loop: one drive + steps_per_sample steps per sample
     ns = 64
     drives = (1e-4 * np.random.default_rng(3).standard_normal((ns, nb))).astype(np.float32)
     t0 = time.perf_counter()
     ev.drive_bridge_block(drives)
     wall_s = time.perf_counter() - t0
-    line = {"metric": "cell-updates/s (3D FDTD, voxelised body, generic kernels, 1 GPU)",
+    line = {"metric": "cell-updates/s (3D FDTD, voxelised body, 1 GPU)",
             "value": cells * steps / wall, "unit": "cell-updates/s", "n_gpus": 1,
             "steps": steps, "warmup": a.warmup, "ms_per_step": wall * 1e3 / steps,
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
@@ -583,9 +594,10 @@ def bench_body(a):
                        "cells": cells, "kernel": f.kernel_family, "bridge_cells": nb,
                        "steps_per_sample": ev.steps_per_sample},
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                          "bytes_per_cell_update": bpc, "cell_updates_per_launch": cells,
-                         "stencil_ms_per_step": ms_stencil, "event_ms_total": ms_events},
+                         "stencil_ms_per_step": ms_stencil, "event_ms_total": ms_events,
+                         "traffic_GBps": (traffic / (ms_stencil * 1e-3) / 1e9) if traffic else None},
             "bridge_loop": {"samples": ns, "samples_per_s": ns / wall_s,
                             "realtime_factor_44k1": ns / wall_s / 44100.0}}
     del ev, f

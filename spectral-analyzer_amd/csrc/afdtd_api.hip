@@ -340,6 +340,7 @@ void destroy_state(AcousticFDTDState* st) {
     for (int a = 0; a < 3; ++a) { dfree(st->V[a]); dfree(st->V2[a]); }
     dfree(st->stage);
     dfree(st->d_ct); dfree(st->d_fx); dfree(st->d_fy); dfree(st->d_fz);
+    dfree(st->d_geo); dfree(st->d_vpl);
     dfree(st->d_bu); dfree(st->d_kap); dfree(st->d_xi);
     dfree(st->d_alpha); dfree(st->d_pdamp); dfree(st->d_status);
     for (int a = 0; a < 3; ++a) dfree(st->d_w[a]);
@@ -653,8 +654,14 @@ AcousticFDTDState* create_impl(int Nx, int Ny, int Nz, float dx, float c, float 
         st->family = FDTD_KERNEL_FUSED2;
         st->f2_rows = f2_rows_for(o->tile_y);
     } else if (o->kernel == FDTD_KERNEL_FUSED || o->kernel == FDTD_KERNEL_AUTO) {
-        if (fused_supported(st)) st->family = FDTD_KERNEL_FUSED;
-        else if (o->kernel == FDTD_KERNEL_FUSED) {
+        if (fused_supported(st)) {
+            st->family = FDTD_KERNEL_FUSED;
+        } else if (fused_geo_supported(st)) {
+            // walls, plate cells, fractions or an active plate: the single-step
+            // sweep with per-cell geometry codes
+            st->family = FDTD_KERNEL_FUSED;
+            st->fused_geo = 1;
+        } else if (o->kernel == FDTD_KERNEL_FUSED) {
             afdtd_set_error("fused kernel does not support this configuration");
             destroy_state(st);
             return nullptr;
@@ -671,6 +678,7 @@ AcousticFDTDState* create_impl(int Nx, int Ny, int Nz, float dx, float c, float 
             TRY(hipMalloc(&st->V2[a], st->fbytes));
             TRY(hipMemsetAsync(st->V2[a], 0, st->fbytes, st->stream));
         }
+        if (st->fused_geo) TRY(launch_geo_codes(st));
         TRY(hipStreamSynchronize(st->stream));
     }
     return st;
@@ -941,9 +949,22 @@ SK_API int fdtd_set_face_fractions(AcousticFDTDState* st, const float* fx, const
     if (fx || fy || fz) {
         st->has_frac = 1;
         st->sp.has_frac = 1;
-        if ((st->family == FDTD_KERNEL_FUSED || st->family == FDTD_KERNEL_FUSED2) &&
-            !fused_supported(st))
+    }
+    // The plain sweeps carry no geometry: fractions move a fused handle to
+    // the single-step sweep with geometry codes (FP32), else to the generic
+    // kernels.  Both fused families keep the same double-buffered velocities.
+    if ((st->family == FDTD_KERNEL_FUSED || st->family == FDTD_KERNEL_FUSED2) &&
+        !st->fused_geo && !fused_supported(st)) {
+        if (fused_geo_supported(st)) {
+            st->family = FDTD_KERNEL_FUSED;
+            st->fused_geo = 1;
+        } else {
             st->family = FDTD_KERNEL_GENERIC;
+        }
+    }
+    if (st->fused_geo) {
+        AFDTD_CHECK(launch_geo_codes(st));
+        return sync(st);
     }
     return FDTD_OK;
 }

@@ -16,7 +16,7 @@ k_plate_step(int nact, const int* __restrict__ act, const int* __restrict__ b13,
              const float* __restrict__ P, const float* __restrict__ w,
              const float* __restrict__ wp, float* __restrict__ wt,
              const float* __restrict__ ext, float a, float bhc, float vd, float bk,
-             int* status, int step_index) {
+             int* status, int step_index, float* __restrict__ vpl, float inv_dt) {
     if (status[0]) return;
     int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= nact) return;
@@ -39,6 +39,59 @@ k_plate_step(int nact, const int* __restrict__ act, const int* __restrict__ b13,
         return;
     }
     wt[p] = wn;
+    // plate velocity of this step, as the face injection computes it from the
+    // rotated buffers: (w_new - w) / dt
+    if (vpl) vpl[p] = (wn - w[p]) * inv_dt;
+}
+
+// Code byte of every cell for the single-step sweep with cell geometry (bit
+// layout in afdtd_fused_tile.cuh): wall / plate cell, closed faces (fraction
+// == 0, the closed-face rule of the generic sweep) and faces whose
+// divergence tap must be multiplied by the fraction.  The multiply is skipped
+// only where it cannot change a bit: fraction exactly 1, or exactly +0 on a
+// closed face (whose velocity is +0 after the update).
+__global__ void __launch_bounds__(256)
+k_geo_codes(int Nx, int Ny, int Nz, const uint8_t* __restrict__ ct,
+            const float* __restrict__ fx, const float* __restrict__ fy,
+            const float* __restrict__ fz, int has_closed, int has_frac,
+            uint8_t* __restrict__ code) {
+    const size_t N = (size_t)Nx * Ny * Nz;
+    const size_t c = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (c >= N) return;
+    const int k = (int)(c % Nz);
+    const size_t t = c / Nz;
+    const int j = (int)(t % Ny), i = (int)(t / Ny);
+    uint32_t g = 0;
+    if (ct && is_solid(ct[c])) g |= 1u;
+    const bool exists[3] = {i < Nx - 1, j < Ny - 1, k < Nz - 1};
+    const float* F[3] = {fx, fy, fz};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        if (!exists[a] || !F[a]) continue;
+        const float f = F[a][c];
+        const uint32_t b = __float_as_uint(f);
+        const bool closed = has_closed && f == 0.0f;
+        if (closed) g |= 2u << a;
+        if (has_frac && b != 0x3F800000u && !(closed && b == 0u)) g |= 16u << a;
+    }
+    code[c] = (uint8_t)g;
+}
+
+// Plate faces (the Vz faces the plate velocity overrides): plate bit, and the
+// fraction multiply unless the fraction is exactly 1 (the plate velocity
+// replaces the closed face's 0).
+__global__ void __launch_bounds__(256)
+k_geo_plate(int nact, const int* __restrict__ vzb, const int* __restrict__ vza,
+            const float* __restrict__ fz, int has_frac, uint8_t* __restrict__ code) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nact) return;
+    const int f[2] = {vzb[n], vza[n]};
+    for (int s = 0; s < 2; ++s) {
+        if (f[s] < 0) continue;
+        uint32_t g = code[f[s]] | 128u;
+        if (has_frac && __float_as_uint(fz[f[s]]) != 0x3F800000u) g |= 64u;
+        code[f[s]] = (uint8_t)g;
+    }
 }
 
 // Plate velocity as the normal velocity of the two plate-layer Vz faces
@@ -260,7 +313,27 @@ hipError_t launch_plate_step(AcousticFDTDState* st, int step_index) {
                        st->nact, st->d_act, st->d_b13, st->d_above, st->d_below,
                        (const float*)st->P[st->cur], st->d_w[0], st->d_w[1], st->d_w[2],
                        st->d_ext, st->plate_a, st->plate_bh, st->plate_vd, st->plate_bk,
-                       st->d_status, step_index);
+                       st->d_status, step_index, st->fused_geo ? st->d_vpl : nullptr,
+                       1.0f / st->dt);
+    return hipGetLastError();
+}
+
+hipError_t launch_geo_codes(AcousticFDTDState* st) {
+    if (!st->d_geo) {
+        hipError_t e = hipMalloc(&st->d_geo, (size_t)st->N);
+        if (e != hipSuccess) { st->d_geo = nullptr; return e; }
+    }
+    hipLaunchKernelGGL(k_geo_codes, dim3(blocks_for((size_t)st->N, 256)), dim3(256), 0,
+                       st->stream, st->Nx, st->Ny, st->Nz, st->has_solid ? st->d_ct : nullptr,
+                       st->d_fx, st->d_fy, st->d_fz, st->sp.has_closed, st->has_frac, st->d_geo);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || st->nact <= 0) return e;
+    if (!st->d_vpl) {
+        e = hipMalloc(&st->d_vpl, sizeof(float) * st->Nplate);
+        if (e != hipSuccess) { st->d_vpl = nullptr; return e; }
+    }
+    hipLaunchKernelGGL(k_geo_plate, dim3(blocks_for(st->nact, 256)), dim3(256), 0, st->stream,
+                       st->nact, st->d_vzb, st->d_vza, st->d_fz, st->has_frac, st->d_geo);
     return hipGetLastError();
 }
 

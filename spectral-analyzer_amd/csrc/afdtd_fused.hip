@@ -20,8 +20,17 @@
 // The per-face and per-cell arithmetic is the reference's, in the same
 // order (acoustic_fdtd.cpp:1107-1113 gradient/velocity, :894-925 divergence,
 // :1198 pressure), so results equal the generic kernels and the oracle bit
-// for bit in FP32.  Geometry with walls, plate cells, cut-cell fractions or
-// an active plate runs on the generic kernels instead.
+// for bit in FP32.
+//
+// With FP32 storage the sweep also carries the reference's cell geometry
+// (the GEOM build): walls and plate cells keep their pressure, closed faces
+// are set to 0 after their update (acoustic_fdtd.cpp:1146-1149), plate faces
+// take the plate velocity (w - w_prev)/dt (:1151-1158) and the divergence
+// taps are weighted by the face fractions (:894-925).  A code byte per cell
+// says which of these apply, so a cell costs one byte more than in an empty
+// room; only faces whose fraction is not 1 read their fraction, and only
+// plate faces the plate displacements, so the O(surface) geometry adds
+// O(surface) traffic.
 //
 // Workgroup order is XCD-aware: blocks are dealt round-robin to the 8 XCDs,
 // so the block index is remapped to give each XCD a contiguous range of
@@ -39,6 +48,7 @@ struct FusedArgs {
     const float* alpha;   // alpha(d)
     const float* pdamp;   // 1/(1 + alpha(d) dt)
     const int* status;
+    GeoPtrs g;            // cell geometry (GEOM build)
     int seg;              // x-segment length
     int nty, ntz, nseg;   // tile counts
 };
@@ -47,19 +57,23 @@ struct FusedArgs {
 #ifndef AFDTD_FUSED_WPE
 #define AFDTD_FUSED_WPE 1
 #endif
+#ifndef AFDTD_FUSED_GEO_WPE
+#define AFDTD_FUSED_GEO_WPE 1
+#endif
 
-template <int ORDER, typename T, bool HET, int TY, int NW>
-__global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(AFDTD_FUSED_WPE)))
+template <int ORDER, typename T, bool HET, int TY, int NW, bool GEOM>
+__global__ void __launch_bounds__(64 * NW)
+__attribute__((amdgpu_waves_per_eu(GEOM ? AFDTD_FUSED_GEO_WPE : AFDTD_FUSED_WPE)))
 k_fused(FusedArgs a) {
     if (*a.status) return;
     int t = xcd_remap(blockIdx.x, gridDim.x);
     const int ty = t % a.nty; t /= a.nty;
     const int tz = t % a.ntz;
     const int sg = t / a.ntz;
-    fused_tile<ORDER, T, HET, TY, NW>(a.p, a.f, a.alpha, a.pdamp, ty, tz, sg, a.seg);
+    fused_tile<ORDER, T, HET, TY, NW, GEOM>(a.p, a.f, a.alpha, a.pdamp, ty, tz, sg, a.seg, a.g);
 }
 
-template <int ORDER, typename T, bool HET, int TY>
+template <int ORDER, typename T, bool HET, int TY, bool GEOM>
 hipError_t launch_t(AcousticFDTDState* st, const FusedArgs& a0) {
     constexpr int NW = TY / 2 > 4 ? TY / 2 : 4;     // two rows per thread
     FusedArgs a = a0;
@@ -67,16 +81,17 @@ hipError_t launch_t(AcousticFDTDState* st, const FusedArgs& a0) {
     a.ntz = (st->Nz + TZ - 1) / TZ;
     a.nseg = (st->Nx + a.seg - 1) / a.seg;
     const int nb = a.nty * a.ntz * a.nseg;
-    hipLaunchKernelGGL((k_fused<ORDER, T, HET, TY, NW>), dim3(nb), dim3(64, NW), 0, st->stream, a);
+    hipLaunchKernelGGL((k_fused<ORDER, T, HET, TY, NW, GEOM>), dim3(nb), dim3(64, NW), 0,
+                       st->stream, a);
     return hipGetLastError();
 }
 
-template <int ORDER, typename T, bool HET>
+template <int ORDER, typename T, bool HET, bool GEOM = false>
 hipError_t launch_ty(AcousticFDTDState* st, const FusedArgs& a, int ty) {
     switch (ty) {
-        case 8: return launch_t<ORDER, T, HET, 8>(st, a);
-        case 32: return launch_t<ORDER, T, HET, 32>(st, a);
-        default: return launch_t<ORDER, T, HET, 16>(st, a);
+        case 8: return launch_t<ORDER, T, HET, 8, GEOM>(st, a);
+        case 32: return launch_t<ORDER, T, HET, 32, GEOM>(st, a);
+        default: return launch_t<ORDER, T, HET, 16, GEOM>(st, a);
     }
 }
 
@@ -87,6 +102,9 @@ hipError_t launch_het(AcousticFDTDState* st, const FusedArgs& a, int ty) {
 
 template <int ORDER>
 hipError_t launch_store(AcousticFDTDState* st, const FusedArgs& a, int ty) {
+    if (st->fused_geo)
+        return st->hetero ? launch_ty<ORDER, float, true, true>(st, a, ty)
+                          : launch_ty<ORDER, float, false, true>(st, a, ty);
     switch (st->storage) {
         case FDTD_STORE_BF16: return launch_het<ORDER, bf16_t>(st, a, ty);
         case FDTD_STORE_F16: return launch_het<ORDER, f16_t>(st, a, ty);
@@ -99,6 +117,11 @@ hipError_t launch_store(AcousticFDTDState* st, const FusedArgs& a, int ty) {
 
 bool fused_supported(const AcousticFDTDState* st) {
     return !st->has_solid && !st->has_frac && st->nact == 0 && st->n_pml <= 64;
+}
+
+bool fused_geo_supported(const AcousticFDTDState* st) {
+    return st->storage == FDTD_STORE_F32 && st->n_pml <= 64 &&
+           (size_t)st->N * sizeof(float) < 0x80000000u;
 }
 
 hipError_t launch_fused_step(AcousticFDTDState* st) {
@@ -117,7 +140,17 @@ hipError_t launch_fused_step(AcousticFDTDState* st) {
     a.alpha = st->d_alpha;
     a.pdamp = st->d_pdamp;
     a.status = st->d_status;
-    int ty = st->fused_tile_y > 0 ? st->fused_tile_y : 16;
+    a.g = GeoPtrs{};
+    if (st->fused_geo) {
+        if (!st->d_geo) return hipErrorInvalidValue;
+        a.g.code = st->d_geo;
+        a.g.frac[0] = st->d_fx; a.g.frac[1] = st->d_fy; a.g.frac[2] = st->d_fz;
+        a.g.vpl = st->d_vpl;
+        a.g.ncells = (uint32_t)st->N;
+    }
+    // 8-row tiles for the geometry build: its larger register set leaves
+    // three workgroups of 4 waves per CU, against one of 8 waves at 16 rows
+    int ty = st->fused_tile_y > 0 ? st->fused_tile_y : (st->fused_geo ? 8 : 16);
     a.seg = st->fused_seg_x > 0 ? st->fused_seg_x : 32;
     if (a.seg > st->Nx) a.seg = st->Nx;
     hipError_t e = st->order == 4 ? launch_store<4>(st, a, ty) : launch_store<2>(st, a, ty);

@@ -26,6 +26,24 @@ struct FusedPtrs {
     const float* xi;      // per-cell wall sums (hetero + impedance, edge cells)
 };
 
+// Cell geometry of the general sweep (GEOM): one code byte per cell c.
+//   bit 0       c is a wall or plate cell: its pressure is not updated;
+//   bits 1..3   the x / y / z face (c, c + s_axis) is closed: its velocity is
+//               set to 0 after the update (face fraction 0);
+//   bits 4..6   the face's fraction is not 1: its divergence tap is V * frac;
+//   bit 7       the z face is a plate face: its velocity is set to the
+//               plate's velocity (w - w_prev) / dt of this step after the
+//               closed-face rule.
+// Bits are set only for faces inside the room.
+enum : uint32_t { G_SOLID = 1u, G_CLOSED = 2u, G_CUT = 16u, G_PLATE = 128u };
+
+struct GeoPtrs {
+    const uint8_t* code;        // N code bytes (GEOM)
+    const float* frac[3];       // face fractions, cell-indexed
+    const float* vpl;           // plate velocity per plate node (i*Ny + j), this step
+    uint32_t ncells;            // bound of the code array (bytes)
+};
+
 template <int ORDER>
 struct Geo {
     static constexpr int PH = ORDER == 4 ? 3 : 1;   // P halo (cells)
@@ -58,13 +76,15 @@ __device__ __forceinline__ float face_new(float vold, float pm, float pl, float 
     return vold - coef * g;
 }
 
-template <int ORDER, typename T, bool HET, int TY, int NW>
+template <int ORDER, typename T, bool HET, int TY, int NW, bool GEOM = false>
 __device__ __forceinline__ void fused_tile(const StencilParams& p, const FusedPtrs& fp,
                                            const float* __restrict__ alpha_tab,
                                            const float* __restrict__ pdamp_tab,
-                                           int ty, int tz, int sg, int seg) {
+                                           int ty, int tz, int sg, int seg,
+                                           const GeoPtrs& gp = GeoPtrs{}) {
     typedef Geo<ORDER> G;
     typedef Store<T> S;
+    static_assert(!GEOM || S::exact, "cell geometry runs with FP32 storage");
     constexpr int PH = G::PH, FLO = G::FLO, FHI = G::FHI, NS = G::NS, LA = G::LA, NQ = G::NQ;
     constexpr int R = TY / NW;                      // rows per thread
     constexpr int PR = TY + 2 * PH, PC = TZ + 2 * PH;
@@ -222,6 +242,24 @@ __device__ __forceinline__ void fused_tile(const StencilParams& p, const FusedPt
         RawT x[R], y[R], z[R], a, b;
         float k[HET ? R : 1], xi[HET ? R : 1];
         int ex[R], ey[R], ez[R], ea, eb;        // row exponents (block storage only)
+        // geometry codes (GEOM): of the own Vx face's cell (plane f), of the
+        // own cells (plane q), of the halo faces A and B; 0 outside the room
+        uint32_t gx[GEOM ? R : 1], gq[GEOM ? R : 1], ga, gb;
+    };
+    // Fractions and plate displacements of the faces of one plane whose codes
+    // ask for them, loaded at the start of the plane (GEOM).
+    struct Lazy {
+        float fx[R], fy[R], fz[R], fa, fb;
+        float vp[R], vpb;
+    };
+    const __amdgpu_buffer_rsrc_t grs = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<uint8_t*>(gp.code), (short)0, (int)gp.ncells, 0x00020000);
+    // code byte at cell offset off, or 0 when !v (the offset lies beyond the
+    // descriptor's bound, so the load returns 0)
+    auto ldg = [&](bool v, int off) -> uint32_t {
+        if constexpr (GEOM)
+            return __builtin_amdgcn_raw_buffer_load_b8(grs, v ? (uint32_t)off : 0x80000000u, 0, 0);
+        else return 0u;
     };
     Old oE, oO;
     float vq[NQ][R];
@@ -342,6 +380,39 @@ __device__ __forceinline__ void fused_tile(const StencilParams& p, const FusedPt
             o.eb = lde(Vzi, q, qv && vB, eB);
         }
     };
+    // Geometry codes of plane q and of the Vx face f (GEOM).  Issued before
+    // the plane prefetch, so they have arrived when the next plane starts
+    // and the fraction and plate loads they ask for can be issued at once.
+    auto load_codes = [&](int q, int f, Old& o) {
+        if constexpr (GEOM) {
+            const bool fv = f >= 0 && f <= Nx - 2;
+            const bool qv = q >= i0 && q < i1;
+            const int bf = f * sx, bq = q * sx;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                o.gx[r] = ldg(fv && ownV[r], bf + ownOff[r]);
+                o.gq[r] = ldg(qv && ownV[r], bq + ownOff[r]);
+            }
+            o.ga = ldg(qv && vA, bq + offA);
+            o.gb = ldg(qv && vB, bq + offB);
+        }
+    };
+    // The fractions and plate displacements the codes of plane q ask for.
+    auto lazy_load = [&](int q, const Old& o, Lazy& z) {
+        if constexpr (GEOM) {
+            const int bf = (q + FHI) * sx, bq = q * sx, pq = q * Ny;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                if (o.gx[r] & G_CUT) z.fx[r] = gp.frac[0][bf + ownOff[r]];
+                if (o.gq[r] & (G_CUT << 1)) z.fy[r] = gp.frac[1][bq + ownOff[r]];
+                if (o.gq[r] & (G_CUT << 2)) z.fz[r] = gp.frac[2][bq + ownOff[r]];
+                if (o.gq[r] & G_PLATE) z.vp[r] = gp.vpl[pq + j0 + w * R + r];
+            }
+            if (o.ga & (G_CUT << 1)) z.fa = gp.frac[1][bq + offA];
+            if (o.gb & (G_CUT << 2)) z.fb = gp.frac[2][bq + offB];
+            if (o.gb & G_PLATE) z.vpb = gp.vpl[pq + jB];
+        }
+    };
     auto slot_of = [](int pl) { return ((pl % NS) + NS) % NS; };
     auto slot_add = [](int s0, int d) { const int t = s0 + d; return t >= NS ? t - NS : t; };
     auto damp = [&](int dL, int dR) { return face_damp(sAlpha, dL, dR, dt); };
@@ -353,6 +424,9 @@ __device__ __forceinline__ void fused_tile(const StencilParams& p, const FusedPt
         // and the face tiles at plane q-1 before the previous plane's last
         // barrier, so no barrier is needed before refilling them.
         store_plane(slot_add(sq0, LA));
+        Lazy z;
+        lazy_load(q, o, z);
+        load_codes(q + 1, q + 1 + FHI, n);
         // Issued on every plane, also the last one (whose prefetch is never
         // used: plane q+1 >= i1 is invalid, so its loads read element 0 or a
         // face inside the room).  A conditional prefetch would make the
@@ -392,10 +466,15 @@ __device__ __forceinline__ void fused_tile(const StencilParams& p, const FusedPt
                                            ORDER == 4 ? s2[a] : 0.0f, inner, coef, C1, C2);
                 if (npml > 0) nv = nv * damp(imax3(di0, dJ[r], dK), imax3(di1, dJ[r], dK));
                 nv = val ? nv : 0.0f;
+                float tap = nv;                         // the face's divergence tap
+                if constexpr (GEOM) {
+                    if (o.gx[r] & G_CLOSED) nv = 0.0f;
+                    tap = (o.gx[r] & G_CUT) ? nv * z.fx[r] : nv;
+                }
                 if (own) put_row(Vxo, f, r, nv, val, true, BLK ? o.ex[r] : 0);
 #pragma unroll
                 for (int s = 0; s < NQ - 1; ++s) vq[s][r] = vq[s + 1][r];
-                vq[NQ - 1][r] = nv;
+                vq[NQ - 1][r] = tap;
             }
         }
 
@@ -413,8 +492,13 @@ __device__ __forceinline__ void fused_tile(const StencilParams& p, const FusedPt
                                            vyIn[r], coef, C1, C2);
                 if (npml > 0) nv = nv * damp(imax3(dI, dJ[r], dK), imax3(dI, dJ1[r], dK));
                 nv = vyV[r] ? nv : 0.0f;
+                float tap = nv;
+                if constexpr (GEOM) {
+                    if (o.gq[r] & (G_CLOSED << 1)) nv = 0.0f;
+                    tap = (o.gq[r] & (G_CUT << 1)) ? nv * z.fy[r] : nv;
+                }
                 put_row(Vyo, q, r, nv, vyV[r], true, BLK ? o.ey[r] : 0);
-                sVy[(FLO + w * R + r) * TZ + lane] = nv;
+                sVy[(FLO + w * R + r) * TZ + lane] = tap;
             }
             // ── halo Vy face row (task A) ──
             if (tA) {
@@ -424,7 +508,12 @@ __device__ __forceinline__ void fused_tile(const StencilParams& p, const FusedPt
                                            sq[a + PC], ORDER == 4 ? sq[a + 2 * PC] : 0.0f, inA,
                                            coef, C1, C2);
                 if (npml > 0) nv = nv * damp(imax3(dI, dJA, dK), imax3(dI, dJA1, dK));
-                sVy[frA * TZ + lane] = vA ? nv : 0.0f;
+                nv = vA ? nv : 0.0f;
+                if constexpr (GEOM) {
+                    if (o.ga & (G_CLOSED << 1)) nv = 0.0f;
+                    if (o.ga & (G_CUT << 1)) nv = nv * z.fa;
+                }
+                sVy[frA * TZ + lane] = nv;
             }
             // ── Vz faces of own rows (face k between cols k and k+1) ──
 #pragma unroll
@@ -436,8 +525,14 @@ __device__ __forceinline__ void fused_tile(const StencilParams& p, const FusedPt
                                            C1, C2);
                 if (npml > 0) nv = nv * damp(imax3(dI, dJ[r], dK), imax3(dI, dJ[r], dK1));
                 nv = vzV[r] ? nv : 0.0f;
+                float tap = nv;
+                if constexpr (GEOM) {
+                    if (o.gq[r] & (G_CLOSED << 2)) nv = 0.0f;
+                    if (o.gq[r] & G_PLATE) nv = z.vp[r];
+                    tap = (o.gq[r] & (G_CUT << 2)) ? nv * z.fz[r] : nv;
+                }
                 put_row(Vzo, q, r, nv, vzV[r], true, BLK ? o.ez[r] : 0);
-                sVz[(w * R + r) * FC + FLO + lane] = nv;
+                sVz[(w * R + r) * FC + FLO + lane] = tap;
             }
             // ── halo Vz face (task B) ──
             if (tB) {
@@ -446,7 +541,13 @@ __device__ __forceinline__ void fused_tile(const StencilParams& p, const FusedPt
                 float nv = face_new<ORDER>(vel(o.b, BLK ? o.eb : 0), ORDER == 4 ? sq[a - 1] : 0.0f, sq[a], sq[a + 1],
                                            ORDER == 4 ? sq[a + 2] : 0.0f, inB, coef, C1, C2);
                 if (npml > 0) nv = nv * damp(imax3(dI, dJB, dKB), imax3(dI, dJB, dKB1));
-                sVz[lyB * FC + fcB] = vB ? nv : 0.0f;
+                nv = vB ? nv : 0.0f;
+                if constexpr (GEOM) {
+                    if (o.gb & (G_CLOSED << 2)) nv = 0.0f;
+                    if (o.gb & G_PLATE) nv = z.vpb;
+                    if (o.gb & (G_CUT << 2)) nv = nv * z.fb;
+                }
+                sVz[lyB * FC + fcB] = nv;
             }
             __syncthreads();                               // faces visible
             // ── pressure of own cells at plane q ──
@@ -459,7 +560,7 @@ __device__ __forceinline__ void fused_tile(const StencilParams& p, const FusedPt
                 const float yp = sVy[fy], ym = sVy[fy - TZ];
                 const float zp = sVz[fz], zm = sVz[fz - 1];
                 float dv;
-                if (ORDER == 2) {
+                if constexpr (ORDER == 2) {
                     dv = (xp - xm) + (yp - ym) + (zp - zm);
                 } else {
                     const float xpp = vq[FLO + 1][r], xmm = vq[FLO - 2][r];
@@ -484,7 +585,9 @@ __device__ __forceinline__ void fused_tile(const StencilParams& p, const FusedPt
                     out = pc - kap * dv;
                 }
                 if (npml > 0) out = out * sPd[imax3(dI, dJ[r], dK)];
-                put_row(Pout, q, r, out, ownV[r], false, -1000);
+                bool pst = ownV[r];
+                if constexpr (GEOM) pst = pst && !(o.gq[r] & G_SOLID);   // walls keep P
+                put_row(Pout, q, r, out, pst, false, -1000);
             }
         }
     };
@@ -494,6 +597,7 @@ __device__ __forceinline__ void fused_tile(const StencilParams& p, const FusedPt
         load_plane(pl);
         store_plane(slot_of(pl));
     }
+    load_codes(q0, q0 + FHI, oE);
     load_plane(q0 + LA);
     load_old(q0, q0 + FHI, oE);
     __syncthreads();

@@ -136,6 +136,11 @@ struct AcousticFDTDState {
 
     int step_count, phase;
     int fused_tile_y, fused_seg_x;
+    // Single-step sweep with cell geometry: per-cell code bytes (walls,
+    // closed / fractional / plate faces), rebuilt when the fractions change.
+    int fused_geo;
+    uint8_t* d_geo;
+    float* d_vpl;           // plate velocity of the current step (nact > 0)
 
     // Two-step sweep: host copies of source cells and receiver taps, and the
     // per-tile tables built from them (rebuilt when f2_dirty).
@@ -156,6 +161,10 @@ hipError_t launch_velocity_generic(AcousticFDTDState* st);
 hipError_t launch_pressure_generic(AcousticFDTDState* st);
 hipError_t launch_fused_step(AcousticFDTDState* st);
 bool fused_supported(const AcousticFDTDState* st);
+// The single-step sweep with cell geometry (FP32 storage).
+bool fused_geo_supported(const AcousticFDTDState* st);
+// Build st->d_geo from the cell types, face fractions and plate faces.
+hipError_t launch_geo_codes(AcousticFDTDState* st);
 // Two steps (global indices step, step+1) in one sweep; the receiver taps of
 // the first step go to a tap buffer that launch_fused2_taps turns into a
 // trace row.

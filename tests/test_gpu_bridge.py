@@ -4,8 +4,9 @@ two-bout body at the factory's defaults (dx 4 mm, 12 cells of padding, PML
 10), cut a soundhole, build the solver, set the cut-cell face fractions and
 the Gaussian bridge sources, then per audio sample drive the bridge cells
 and step steps_per_sample steps; read the pressure field, the volume slices,
-the soundboard plane, the plate and world-space receivers, and reset.  This
-is the engine's generic path (walls, face fractions and an active plate)."""
+the soundboard plane, the plate and world-space receivers, and reset.  Run
+on the generic kernels and on the default choice, the single-step sweep with
+cell geometry (walls, face fractions and an active plate)."""
 import numpy as np
 import pytest
 
@@ -16,7 +17,8 @@ def _bits(a):
     return np.ascontiguousarray(a, np.float32).view(np.uint32)
 
 
-def test_fdtd_coevolver_call_sequence_matches_oracle(oracle_mod):
+@pytest.mark.parametrize("kernel", ["generic", "auto"])
+def test_fdtd_coevolver_call_sequence_matches_oracle(oracle_mod, kernel):
     from afdtd.bridge import FDTDCoEvolver
     from afdtd.geometry import apply_soundhole_cutout, two_bout_outline
     from oracle import geometry as og
@@ -25,12 +27,12 @@ def test_fdtd_coevolver_call_sequence_matches_oracle(oracle_mod):
     bridges = [(-0.02, -0.13), (0.0, -0.13), (0.02, -0.13)]
     hole = (0.0, 0.06, 0.04)
     ev = FDTDCoEvolver.from_outline(poly, body_h, bridges, soundhole=hole, dx=dx,
-                                    pad_cells=pad, n_pml=n_pml)
+                                    pad_cells=pad, n_pml=n_pml, kernel=kernel)
     info = ev.info
     Nx, Ny, Nz = info["Nx"], info["Ny"], info["Nz"]
     print(f"bridge sequence: {Nx}x{Ny}x{Nz}, {len(ev.bridge_cells)} bridge cells, "
           f"{ev.steps_per_sample} steps per sample, kernel {ev.fdtd.kernel_family}", flush=True)
-    assert ev.fdtd.kernel_family == "generic"
+    assert ev.fdtd.kernel_family == ("generic" if kernel == "generic" else "fused")
     # the same sequence on the oracle
     ct, pa, g = og.voxelise(poly, body_h, dx, pad)
     apply_soundhole_cutout(ct, pa, g, hole)
