@@ -710,13 +710,14 @@ def measure_forms(steps=40):
     def precision():
         w = W.config4(steps=steps + 4)
         out = {"workload": w.name, "steps": steps, "unit": "cell-updates/s"}
-        for st, kern in (("f32", "auto"), ("bf16", "auto"), ("bf16", "fused2"),
-                         ("f16", "auto"), ("bfp16", "auto")):
+        for st, kern in (("f32", "auto"), ("f32", "fused"), ("bf16", "auto"), ("bf16", "fused2"),
+                         ("f16", "auto"), ("f16", "fused2"), ("bfp16", "auto"), ("f16s", "auto")):
             v, _, fam = _timed_room(w, storage=st, kernel=kern, steps=steps)
             out[f"{st}_{fam}"] = v
-        base = out["f32_fused2"]
+        base = out["f32_fused2"]          # the FP32 path a handle gets by default
         out["bf16_vs_f32"] = max(out["bf16_fused"], out["bf16_fused2"]) / base
         out["bfp16_vs_f32"] = out["bfp16_fused"] / base
+        out["f16s_vs_f32"] = out["f16s_fused"] / base
         return out
 
     def adjoint():
@@ -845,7 +846,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": {"f32": "fp32", "bf16": "bf16-storage/fp32", "f16": "fp16-storage/fp32",
-                  "bfp16": "int16-block-float-storage/fp32"}[a.storage],
+                  "bfp16": "int16-block-float-storage/fp32",
+                  "f16s": "fp16-row-scaled-storage/fp32"}[a.storage],
         "data": "synthetic (seeded media, sources and receivers)",
         "config": {"workload": w.name, "cells": w.cells, "stencil_order": w.order,
                    "courant": w.courant, "boundary": w.boundary, "heterogeneous": w.hetero,

@@ -202,6 +202,11 @@ SK_API int fdtd_get_surface_emission(const AcousticFDTDState* st, int n_surf,
  * magnitude).  Fused sweep only: no walls, plate cells, face fractions or
  * active plate. */
 #define FDTD_STORE_BFP16 3
+/* FP16 with a per-row power-of-two scale: FP16 values plus one exponent per
+ * 64-cell z-row segment (the layout and exponent rule of BFP16; relative
+ * error <= 2^-11 per value down to 2^-28 of the segment's largest
+ * magnitude).  Fused sweep only, as BFP16. */
+#define FDTD_STORE_F16S 4
 
 /* Kernel family selection. */
 #define FDTD_KERNEL_AUTO    0

@@ -17,6 +17,7 @@ FDTD_OK, FDTD_ERR_NULL, FDTD_ERR_DIM, FDTD_ERR_UNSTABLE = 0, -1, -2, -3
 FDTD_ERR_DEVICE, FDTD_ERR_ARG = -4, -5
 FDTD_BOUNDARY_RIGID_PML, FDTD_BOUNDARY_IMPEDANCE = 0, 1
 FDTD_STORE_F32, FDTD_STORE_BF16, FDTD_STORE_F16, FDTD_STORE_BFP16 = 0, 1, 2, 3
+FDTD_STORE_F16S = 4
 FDTD_KERNEL_AUTO, FDTD_KERNEL_GENERIC, FDTD_KERNEL_FUSED, FDTD_KERNEL_FUSED2 = 0, 1, 2, 3
 
 # The 26 functions of the reference ABI (csrc/include/acoustic_fdtd.h:137-506).

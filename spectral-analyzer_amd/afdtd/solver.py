@@ -10,8 +10,9 @@ Extension keywords of the constructor (no reference counterpart):
 ``order`` (4 reference, 2 Yee), ``courant`` (None = reference dt rule),
 ``boundary`` ('rigid' = rigid edges + PML, 'impedance'), ``refl`` (six
 reflection coefficients), ``rho`` / ``bulk`` (per-cell media), ``storage``
-('f32', 'bf16', 'f16', 'bfp16': int16 block floating point) and ``kernel`` ('auto', 'generic', 'fused', 'fused2': two
-steps per sweep, order-2 stencil).
+('f32', 'bf16', 'f16', 'bfp16': int16 block floating point, 'f16s': FP16 with a
+power-of-two scale per 64-cell z-row) and ``kernel`` ('auto', 'generic', 'fused',
+'fused2': two steps per sweep).
 """
 from __future__ import annotations
 
@@ -24,7 +25,7 @@ from . import _lib
 from ._lib import lib
 
 _STORAGE = {"f32": _lib.FDTD_STORE_F32, "bf16": _lib.FDTD_STORE_BF16, "f16": _lib.FDTD_STORE_F16,
-            "bfp16": _lib.FDTD_STORE_BFP16}
+            "bfp16": _lib.FDTD_STORE_BFP16, "f16s": _lib.FDTD_STORE_F16S}
 _KERNEL = {"auto": _lib.FDTD_KERNEL_AUTO, "generic": _lib.FDTD_KERNEL_GENERIC,
            "fused": _lib.FDTD_KERNEL_FUSED, "fused2": _lib.FDTD_KERNEL_FUSED2}
 _BOUNDARY = {"rigid": _lib.FDTD_BOUNDARY_RIGID_PML, "pml": _lib.FDTD_BOUNDARY_RIGID_PML,

@@ -98,7 +98,7 @@ int storage_bytes(int storage) { return storage == FDTD_STORE_F32 ? 4 : 2; }
 size_t field_bytes(int storage, int Nx, int Ny, int Nz) {
     const size_t N = (size_t)Nx * Ny * Nz;
     size_t b = N * storage_bytes(storage);
-    if (storage == FDTD_STORE_BFP16)
+    if (storage == FDTD_STORE_BFP16 || storage == FDTD_STORE_F16S)
         b = ((N * 2 + 3) & ~(size_t)3) + sizeof(int) * (size_t)Nx * Ny * ((Nz + 63) / 64);
     return b;
 }
@@ -391,7 +391,7 @@ AcousticFDTDState* create_impl(int Nx, int Ny, int Nz, float dx, float c, float 
         afdtd_set_error("impedance boundary requires stencil_order 2");
         return nullptr;
     }
-    if (o->storage < 0 || o->storage > 3) {
+    if (o->storage < 0 || o->storage > 4) {
         afdtd_set_error("unknown storage type");
         return nullptr;
     }
@@ -667,9 +667,10 @@ AcousticFDTDState* create_impl(int Nx, int Ny, int Nz, float dx, float c, float 
             return nullptr;
         }
     }
-    if (st->storage == FDTD_STORE_BFP16 && st->family != FDTD_KERNEL_FUSED) {
-        afdtd_set_error("block floating-point storage runs on the single-step fused sweep "
-                        "only (no walls, plate cells, face fractions or active plate)");
+    if ((st->storage == FDTD_STORE_BFP16 || st->storage == FDTD_STORE_F16S) &&
+        st->family != FDTD_KERNEL_FUSED) {
+        afdtd_set_error("block-scaled storage (bfp16, f16s) runs on the single-step fused "
+                        "sweep only (no walls, plate cells, face fractions or active plate)");
         destroy_state(st);
         return nullptr;
     }
@@ -937,9 +938,9 @@ SK_API int fdtd_inject_plate_force(AcousticFDTDState* st, int n, const int* pidx
 SK_API int fdtd_set_face_fractions(AcousticFDTDState* st, const float* fx, const float* fy,
                                    const float* fz) {
     if (!st) return FDTD_ERR_NULL;
-    if (st->storage == FDTD_STORE_BFP16 && (fx || fy || fz)) {
-        afdtd_set_error("face fractions need the generic sweep, which has no block "
-                        "floating-point storage");
+    if ((st->storage == FDTD_STORE_BFP16 || st->storage == FDTD_STORE_F16S) && (fx || fy || fz)) {
+        afdtd_set_error("face fractions need FP32, BF16 or FP16 storage (no block-scaled "
+                        "storage: bfp16, f16s)");
         return FDTD_ERR_ARG;
     }
     int rc = alloc_fracs(st);

@@ -127,7 +127,8 @@ k_post_step(T* __restrict__ P, int npsrc, const int* __restrict__ pidx,
         // each source cell after the addition, one source after another.
         if (threadIdx.x < 64 && step_index < plen) {
             const int lane = threadIdx.x;
-            int16_t* q = (int16_t*)P;
+            typedef typename S::Raw Raw;
+            Raw* q = (Raw*)P;
             int* E = bfp_exps((void*)P, g.N);
             for (int s = 0; s < npsrc; ++s) {
                 const int c = pidx[s];
@@ -138,7 +139,7 @@ k_post_step(T* __restrict__ P, int npsrc, const int* __restrict__ pidx,
                 float v = valid ? bfp_dec(q[cell], E[row]) : 0.0f;
                 if (k == kk) v = v + psig[(size_t)s * plen + step_index];
                 const int e = bfp_exp(wave_absmax(v));
-                if (valid) q[cell] = bfp_enc(v, e);
+                if (valid) q[cell] = blk_enc<Raw>(v, e);
                 if (lane == 0) E[row] = e;
                 __threadfence();      // the next source may re-read this row
             }
@@ -233,8 +234,10 @@ __global__ void k_to_f32(const T* __restrict__ src, float* __restrict__ dst, siz
 }
 
 // FP32 field -> block storage, one wavefront per 64-cell row segment.
+template <typename T>
 __global__ void __launch_bounds__(256)
-k_from_f32_bfp(const float* __restrict__ src, bfp16_t* __restrict__ dst, int nrows, FieldGeo g) {
+k_from_f32_bfp(const float* __restrict__ src, T* __restrict__ dst, int nrows, FieldGeo g) {
+    typedef typename Store<T>::Raw Raw;
     const int lane = threadIdx.x & 63;
     const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (row >= nrows) return;                   // whole wavefront leaves together
@@ -244,7 +247,7 @@ k_from_f32_bfp(const float* __restrict__ src, bfp16_t* __restrict__ dst, int nro
     const int cell = line * g.Nz + k;
     const float v = valid ? src[cell] : 0.0f;
     const int e = bfp_exp(wave_absmax(v));
-    if (valid) ((int16_t*)dst)[cell] = bfp_enc(v, e);
+    if (valid) ((Raw*)dst)[cell] = blk_enc<Raw>(v, e);
     if (lane == 0) bfp_exps((void*)dst, g.N)[row] = e;
 }
 
@@ -358,6 +361,7 @@ hipError_t launch_post_step(AcousticFDTDState* st, int step_index, int trace_row
         case FDTD_STORE_BF16: POST(bf16_t); break;
         case FDTD_STORE_F16: POST(f16_t); break;
         case FDTD_STORE_BFP16: POST(bfp16_t); break;
+        case FDTD_STORE_F16S: POST(f16s_t); break;
         default: POST(float);
     }
 #undef POST
@@ -376,6 +380,7 @@ hipError_t launch_stability_check(AcousticFDTDState* st, int start, int step_ind
         case FDTD_STORE_BF16: CHK(bf16_t); break;
         case FDTD_STORE_F16: CHK(f16_t); break;
         case FDTD_STORE_BFP16: CHK(bfp16_t); break;
+        case FDTD_STORE_F16S: CHK(f16s_t); break;
         default: CHK(float);
     }
 #undef CHK
@@ -409,6 +414,7 @@ hipError_t launch_gather(const void* F, int storage, float scale, const int* idx
         case FDTD_STORE_BF16: G(bf16_t); break;
         case FDTD_STORE_F16: G(f16_t); break;
         case FDTD_STORE_BFP16: G(bfp16_t); break;
+        case FDTD_STORE_F16S: G(f16s_t); break;
         default: G(float);
     }
 #undef G
@@ -426,6 +432,7 @@ hipError_t launch_to_f32(const void* src, int storage, float scale, float* dst, 
         case FDTD_STORE_BF16: C(bf16_t); break;
         case FDTD_STORE_F16: C(f16_t); break;
         case FDTD_STORE_BFP16: C(bfp16_t); break;
+        case FDTD_STORE_F16S: C(f16s_t); break;
         default: C(float);
     }
 #undef C
@@ -435,11 +442,15 @@ hipError_t launch_to_f32(const void* src, int storage, float scale, float* dst, 
 hipError_t launch_from_f32(const float* src, void* dst, int storage, float scale, size_t n,
                              hipStream_t s, int Nz) {
     if (n == 0) return hipSuccess;
-    if (storage == FDTD_STORE_BFP16) {
+    if (storage == FDTD_STORE_BFP16 || storage == FDTD_STORE_F16S) {
         const FieldGeo g = field_geo((int)n, Nz);
         const int nrows = (int)(n / Nz) * g.nrz;
-        hipLaunchKernelGGL(k_from_f32_bfp, dim3((nrows + 3) / 4), dim3(256), 0, s, src,
-                           (bfp16_t*)dst, nrows, g);
+        if (storage == FDTD_STORE_BFP16)
+            hipLaunchKernelGGL(k_from_f32_bfp<bfp16_t>, dim3((nrows + 3) / 4), dim3(256), 0, s,
+                               src, (bfp16_t*)dst, nrows, g);
+        else
+            hipLaunchKernelGGL(k_from_f32_bfp<f16s_t>, dim3((nrows + 3) / 4), dim3(256), 0, s,
+                               src, (f16s_t*)dst, nrows, g);
         return hipGetLastError();
     }
 #define C(T)                                                                                \
@@ -471,6 +482,7 @@ hipError_t launch_energy(AcousticFDTDState* st, double* d_out) {
         case FDTD_STORE_BF16: E(bf16_t); break;
         case FDTD_STORE_F16: E(f16_t); break;
         case FDTD_STORE_BFP16: E(bfp16_t); break;
+        case FDTD_STORE_F16S: E(f16s_t); break;
         default: E(float);
     }
 #undef E

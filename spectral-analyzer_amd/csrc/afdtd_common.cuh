@@ -76,8 +76,25 @@ template <> struct Store<bfp16_t> {
     }
     static constexpr bool exact = false;
 };
+// FP16 with a per-row power-of-two scale (f16s): the layout and exponent
+// rule of bfp16 with FP16 values in place of the int16 mantissas; a value is
+// h * 2^e, h = v * 2^-e rounded to nearest even in FP16.  The exponent puts
+// the segment's largest magnitude in [2^14, 2^15), below FP16's largest
+// finite value, and values down to 2^-28 of it stay normal (relative error
+// <= 2^-11 each).
+struct f16s_t {
+    uint16_t h;
+};
+template <> struct Store<f16s_t> {
+    typedef f16_t Raw;
+    static __device__ __forceinline__ Raw ldr(const f16s_t* p, int i) {
+        return ((const f16_t*)p)[i];
+    }
+    static constexpr bool exact = false;
+};
 template <typename T> struct IsBlock { static constexpr bool value = false; };
 template <> struct IsBlock<bfp16_t> { static constexpr bool value = true; };
+template <> struct IsBlock<f16s_t> { static constexpr bool value = true; };
 
 __host__ __device__ __forceinline__ size_t bfp_exp_offset(size_t N) { return (N * 2 + 3) & ~(size_t)3; }
 __device__ __forceinline__ float bfp_dec(int16_t q, int e) { return ldexpf((float)q, e); }
@@ -91,6 +108,14 @@ __device__ __forceinline__ int16_t bfp_enc(float v, int e) {
     const float x = rintf(ldexpf(v, -e));
     return (int16_t)fminf(fmaxf(x, -32767.0f), 32767.0f);
 }
+// the same pair for f16s (NaN and infinities pass through, so the stability
+// check sees them)
+__device__ __forceinline__ float bfp_dec(f16_t h, int e) { return ldexpf((float)h, e); }
+__device__ __forceinline__ f16_t f16s_enc(float v, int e) { return (f16_t)ldexpf(v, -e); }
+// encode for the block type whose stored word is Raw
+template <typename Raw> __device__ __forceinline__ Raw blk_enc(float v, int e);
+template <> __device__ __forceinline__ int16_t blk_enc<int16_t>(float v, int e) { return bfp_enc(v, e); }
+template <> __device__ __forceinline__ f16_t blk_enc<f16_t>(float v, int e) { return f16s_enc(v, e); }
 // Largest |v| over the 64 lanes of the wavefront (all lanes must be active;
 // the result is wave-uniform).  DPP row shifts 1, 2, 4, 8 leave the maximum
 // of each 16-lane row in its lane 15 (lanes without a source keep 0, the
@@ -123,10 +148,12 @@ __device__ __forceinline__ const int* bfp_exps(const void* F, int N) {
 __device__ __forceinline__ int* bfp_exps(void* F, int N) {
     return (int*)((char*)F + bfp_exp_offset((size_t)N));
 }
-// element c of a bfp16 field (aux kernels; rows are found from c)
+// element c of a block field (aux kernels; rows are found from c)
+template <typename T>
 __device__ __forceinline__ float bfp_ld(const void* F, int c, int N, int Nz, int nrz) {
+    typedef typename Store<T>::Raw Raw;
     const int row = (c / Nz) * nrz + (c % Nz) / 64;
-    return bfp_dec(((const int16_t*)F)[c], bfp_exps(F, N)[row]);
+    return bfp_dec(((const Raw*)F)[c], bfp_exps(F, N)[row]);
 }
 
 // Cell geometry for element access to block-floating-point fields.
@@ -139,7 +166,11 @@ template <typename T>
 __device__ __forceinline__ float fld(const T* F, int c, FieldGeo) { return Store<T>::ld(F, c); }
 template <>
 __device__ __forceinline__ float fld<bfp16_t>(const bfp16_t* F, int c, FieldGeo g) {
-    return bfp_ld(F, c, g.N, g.Nz, g.nrz);
+    return bfp_ld<bfp16_t>(F, c, g.N, g.Nz, g.nrz);
+}
+template <>
+__device__ __forceinline__ float fld<f16s_t>(const f16s_t* F, int c, FieldGeo g) {
+    return bfp_ld<f16s_t>(F, c, g.N, g.Nz, g.nrz);
 }
 
 __device__ __forceinline__ int is_solid(uint8_t c) { return c == 1 || c == 2; }

@@ -109,6 +109,7 @@ hipError_t launch_store(AcousticFDTDState* st, const FusedArgs& a, int ty) {
         case FDTD_STORE_BF16: return launch_het<ORDER, bf16_t>(st, a, ty);
         case FDTD_STORE_F16: return launch_het<ORDER, f16_t>(st, a, ty);
         case FDTD_STORE_BFP16: return launch_het<ORDER, bfp16_t>(st, a, ty);
+        case FDTD_STORE_F16S: return launch_het<ORDER, f16s_t>(st, a, ty);
         default: return launch_het<ORDER, float>(st, a, ty);
     }
 }

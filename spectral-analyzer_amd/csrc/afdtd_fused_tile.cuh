@@ -300,7 +300,9 @@ __device__ __forceinline__ void fused_tile(const StencilParams& p, const FusedPt
                        int e_old) {
         if constexpr (BLK) {
             const int e = bfp_row_exp(v, valid, e_old);
-            if (valid) __builtin_nontemporal_store(bfp_enc(v, e), (int16_t*)F + pl * sx + ownOff[r]);
+            if (valid)
+                __builtin_nontemporal_store(__builtin_bit_cast(uint16_t, blk_enc<RawT>(v, e)),
+                                            (uint16_t*)F + pl * sx + ownOff[r]);
             if (lane == 0 && j0 + w * R + r < Ny)
                 bfp_exps((void*)F, NC)[pl * esx + ownE[r]] = e;
         } else {
