@@ -627,6 +627,98 @@ def bench_body(a):
     return 0
 
 
+def coevolver_workload(dx=0.004, pad=10, n_pml=10, n_strings=6, n_segs=240):
+    """The instrument the reference's scene factory builds
+    (acoustic_fdtd_bridge.py:1105-1302) on a uniform body: a two-bout
+    outline (rib 0.1 m) voxelised at dx with pad cells and a PML, three
+    bridge saddles, six strings of n_segs segments with the shared neck
+    mode, one bridge single-coil, a cardioid 75 mm above the top plus a ring
+    of eight (44.1 kHz, modal stride 16)."""
+    from afdtd.geometry import two_bout_outline, voxelise_guitar_body
+    from afdtd.instrument import body_dict, guitar_defs
+    poly = two_bout_outline(96)
+    body_h = 0.1
+    bridges = [(-0.02, -0.13), (0.0, -0.13), (0.02, -0.13)]
+    ct, pa, info = voxelise_guitar_body(poly, body_h, dx=dx, pad_cells=pad, n_pml=n_pml)
+    S, P, M = guitar_defs(body_h, bridges, n_strings=n_strings, n_segs=n_segs)
+    B = body_dict(ct, pa, info, bridges, body_h, n_pml=n_pml)
+    return S, P, M, B
+
+
+def bench_coevolver(a):
+    """The co-evolver's per-sample loop on the device (coevolver_step): per
+    audio sample, K body sub-steps, each one string / bridge / neck kernel
+    and the body FDTD step (single-step sweep with cell geometry), then the
+    pickups and mics.  Metric: audio samples per second (1 GPU); the body
+    sweep's roofline is reported at 33 B per cell-update (four fields read
+    and written, one geometry code byte).  CPU baseline: the co-evolver
+    oracle (float32, OpenMP body FDTD) on the same instrument."""
+    from afdtd.coevolver import AcousticCoEvolver
+    S, P, M, B = coevolver_workload()
+    co = AcousticCoEvolver(S, P, M, B, 44100.0, modal_stride=16)
+    cells = B["Nx"] * B["Ny"] * B["Nz"]
+    K = co.substeps
+    samples = a.steps if a.steps != 4000 else 2048
+    for si in range(len(S)):
+        co.schedule_pluck(8 * si, si, 0.2 + 0.1 * si, 0.002, 0)
+    co.step(max(a.warmup, 1))
+    t0 = time.perf_counter()
+    ms_dev = co.step_timed(samples)
+    wall = time.perf_counter() - t0
+    mic = co.get_mic_output(0, min(samples, 4096))
+    if not np.all(np.isfinite(mic)) or not np.any(mic):
+        raise SystemExit("co-evolver produced no finite mic output")
+    # the body sweep alone, on the co-evolver's own FDTD handle
+    from afdtd import lib
+    import ctypes as C
+    h = co.fdtd_handle()
+    mt, msw = C.c_float(), C.c_float()
+    nst = 200
+    rc = lib().fdtd_step_timed(h, nst, C.byref(mt), C.byref(msw))
+    if rc:
+        raise SystemExit(f"fdtd_step_timed: {rc}")
+    bpc = 33
+    achieved = cells * bpc / (msw.value * 1e-3) / 1e9
+    line = {"metric": "audio samples/s (instrument co-evolver, uniform body, 1 GPU)",
+            "value": samples / wall, "unit": "samples/s", "n_gpus": 1, "steps": samples,
+            "warmup": a.warmup, "ms_per_step": wall * 1e3 / samples, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+            "data": "synthetic (two-bout body, six plucked strings)",
+            "config": {"workload": f"coevolver-6str-240seg-body-{B['Nx']}x{B['Ny']}x{B['Nz']}-dx4mm",
+                       "cells": cells, "substeps_per_sample": K, "strings": len(S),
+                       "string_nodes": sum(np.asarray(d["path_xyz"]).size // 3 for d in S),
+                       "pickups": len(P), "mics": len(M), "sample_rate": 44100},
+            "realtime_factor_44k1": samples / wall / 44100.0,
+            "device_ms_per_sample": ms_dev / samples,
+            "device_us_per_substep": ms_dev * 1e3 / (samples * K),
+            "roofline": {"bound": "hbm", "kernel": "body sweep (fused, cell geometry)",
+                         "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                         "bytes_per_cell_update": bpc, "cell_updates_per_launch": cells,
+                         "stencil_ms_per_step": msw.value,
+                         "step_share_of_substep": msw.value / (ms_dev / (samples * K))}}
+    del co
+    if not a.no_cpu:
+        from oracle import coevo
+        o = coevo.OracleCoEvolver(S, P, M, B, 44100.0, modal_stride=16)
+        for si in range(len(S)):
+            o.schedule_pluck(8 * si, si, 0.2 + 0.1 * si, 0.002, 0)
+        o.step(1)
+        t0 = time.perf_counter()
+        n = 0
+        while time.perf_counter() - t0 < 15.0 and n < samples:
+            o.step(8)
+            n += 8
+        el = time.perf_counter() - t0
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+        line["cpu_baseline"] = {"value": n / el, "unit": "samples/s", "cores": threads,
+                                "kind": "port",
+                                "sample": f"{n} samples of the same instrument ({el:.1f} s, "
+                                          f"float32 co-evolver oracle, OpenMP body FDTD)"}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def _timed_room(w, storage="f32", kernel="auto", steps=40, warmup=4):
     """Steps of workload w on one handle: (cell-updates/s wall clock, average
     stencil ms per launch, kernel family)."""
@@ -801,6 +893,8 @@ def main():
         return bench_geometry(a)
     if a.config == "body":
         return bench_body(a)
+    if a.config == "coevolver":
+        return bench_coevolver(a)
     if a.config == "amr":
         return bench_amr(a)
     if a.config == "amr-audio":

@@ -1,0 +1,307 @@
+"""ctypes front end of the co-evolver oracle (test infrastructure only).
+
+Only tests/, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+bench.py import this module, as the checker.  It wraps libcoevo_oracle.so,
+a plain-C restatement of the reference's uniform co-evolver
+(csrc/kernels/acoustic_coevolver.cpp) built on the float32 FDTD oracle.
+The constructor takes the same string / pickup / mic / body dicts as the
+reference's pybind class (pybind_kernels.cpp:5756-5875); the descriptor
+structs are built here, independently of the product's own marshalling.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libcoevo_oracle.so")
+_lib = None
+
+
+class _Str(C.Structure):
+    _fields_ = [("n_segs", C.c_int), ("path_xyz", C.c_void_p), ("tension_N", C.c_float),
+                ("linear_mass_kgm", C.c_float), ("damping", C.c_float),
+                ("stiffness_EI", C.c_float), ("axial_stiffness_N", C.c_float),
+                ("neck_freq_hz", C.c_float), ("neck_mass_kg", C.c_float), ("neck_Q", C.c_float)]
+
+
+class _Pk(C.Structure):
+    _fields_ = [("type", C.c_int), ("pos", C.c_float * 3), ("axis", C.c_float * 3),
+                ("pole_sigma", C.c_float), ("coil_spacing", C.c_float),
+                ("sensitivity", C.c_float), ("string_mask", C.c_uint32)]
+
+
+class _Mic(C.Structure):
+    _fields_ = [("pos", C.c_float * 3), ("axis", C.c_float * 3), ("polar_a", C.c_float),
+                ("polar_b", C.c_float), ("gain", C.c_float)]
+
+
+class _Body(C.Structure):
+    _fields_ = [("Nx", C.c_int), ("Ny", C.c_int), ("Nz", C.c_int), ("dx", C.c_float),
+                ("origin", C.c_float * 3), ("c", C.c_float), ("rho_air", C.c_float),
+                ("cell_type", C.c_void_p), ("plate_iz", C.c_int), ("plate_active", C.c_void_p),
+                ("plate_mass_density", C.c_float), ("plate_stiffness_D", C.c_float),
+                ("plate_alpha_M", C.c_float), ("plate_beta_K", C.c_float), ("n_pml", C.c_int),
+                ("n_bridge_src", C.c_int), ("bridge_src_xyz", C.c_void_p),
+                ("bridge_sigma", C.c_float)]
+
+
+def build(force: bool = False) -> str:
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-s", "-B" if force else "all"], cwd=_HERE, check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        build()
+        L = C.CDLL(_LIB_PATH)
+        vp, i, f = C.c_void_p, C.c_int, C.c_float
+        sig = {
+            "orc_coevo_create": (vp, [i, vp, i, vp, i, vp, vp, f, i, f]),
+            "orc_coevo_destroy": (None, [vp]),
+            "orc_coevo_step": (i, [vp, i]),
+            "orc_coevo_step_worker": (i, [vp, i]),
+            "orc_coevo_step_block_with_drive": (i, [vp, vp, i, i, f, vp]),
+            "orc_coevo_pluck": (i, [vp, i, f, f, i]),
+            "orc_coevo_inject_force": (i, [vp, i, i, vp, f]),
+            "orc_coevo_schedule_pluck": (i, [vp, i, i, f, f, i]),
+            "orc_coevo_clear_schedule": (None, [vp]),
+            "orc_coevo_reset": (i, [vp]),
+            "orc_coevo_set_damping_scale": (None, [vp, f]),
+            "orc_coevo_set_face_fractions": (i, [vp, vp, vp, vp]),
+            "orc_coevo_get_pickup_output": (i, [vp, i, vp, i]),
+            "orc_coevo_get_mic_output": (i, [vp, i, vp, i]),
+            "orc_coevo_fdtd": (vp, [vp]),
+            "orc_coevo_substeps": (i, [vp]),
+            "orc_coevo_written": (i, [vp]),
+            "orc_coevo_dt_fdtd": (f, [vp]),
+            "orc_coevo_n_bridge": (i, [vp]),
+            "orc_coevo_n_neck": (i, [vp]),
+            "orc_coevo_string_state": (i, [vp, i, vp, i]),
+            "orc_coevo_neck_state": (None, [vp, vp]),
+            "orc_coevo_modal": (i, [vp, i, vp, vp, i]),
+            "orc_coevo_string_frame": (i, [vp, i, vp, i]),
+            "orc_get_fields_f32": (i, [vp, vp, vp, vp, vp]),
+            "orc_get_plate_f32": (i, [vp, vp]),
+            "orc_get_step_count_f32": (i, [vp]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _v(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _f3(x):
+    a = np.asarray(x, dtype=np.float32).ravel()
+    return (C.c_float * 3)(float(a[0]), float(a[1]), float(a[2]))
+
+
+class OracleCoEvolver:
+    """The co-evolver oracle with the reference constructor's dict inputs."""
+
+    def __init__(self, string_defs, pickup_defs, mic_defs, body, sample_rate,
+                 modal_stride=16, force_scale=1.0):
+        L = self._L = lib()
+        self._h = None
+        self._keep = []
+        ns, npk, nm = len(string_defs), len(pickup_defs), len(mic_defs)
+        sd = (_Str * max(ns, 1))()
+        self.n_nodes = []
+        for q, d in enumerate(string_defs):
+            p = np.ascontiguousarray(np.asarray(d["path_xyz"], np.float32).ravel())
+            self._keep.append(p)
+            sd[q].n_segs = p.size // 3 - 1
+            self.n_nodes.append(p.size // 3)
+            sd[q].path_xyz = _v(p)
+            sd[q].tension_N = d["tension_N"]
+            sd[q].linear_mass_kgm = d["linear_mass_kgm"]
+            sd[q].damping = d["damping"]
+            sd[q].stiffness_EI = d.get("stiffness_EI", 0.0)
+            sd[q].axial_stiffness_N = d.get("axial_stiffness_N", 0.0)
+            sd[q].neck_freq_hz = d.get("neck_freq_hz", 0.0)
+            sd[q].neck_mass_kg = d.get("neck_mass_kg", 0.0)
+            sd[q].neck_Q = d.get("neck_Q", 0.0)
+        pd = (_Pk * max(npk, 1))()
+        for q, d in enumerate(pickup_defs):
+            pd[q].type = d["type"]
+            pd[q].pos = _f3(d["pos"])
+            pd[q].axis = _f3(d["axis"])
+            pd[q].pole_sigma = d["pole_sigma"]
+            pd[q].coil_spacing = d.get("coil_spacing", 0.018)
+            pd[q].sensitivity = d.get("sensitivity", 1.0)
+            pd[q].string_mask = int(d.get("string_mask", -1)) & 0xFFFFFFFF
+        md = (_Mic * max(nm, 1))()
+        for q, d in enumerate(mic_defs):
+            md[q].pos = _f3(d["pos"])
+            md[q].axis = _f3(d.get("axis", (0.0, 0.0, 1.0)))
+            md[q].polar_a = d.get("polar_a", 1.0)
+            md[q].polar_b = d.get("polar_b", 0.0)
+            md[q].gain = d.get("gain", 1.0)
+        ct = np.ascontiguousarray(np.asarray(body["cell_type"], np.uint8).ravel())
+        pa = np.ascontiguousarray(np.asarray(body["plate_active"], np.uint8).ravel())
+        bs = np.ascontiguousarray(np.asarray(body["bridge_src_xyz"], np.float32).ravel())
+        self._keep += [ct, pa, bs, sd, pd, md]
+        b = _Body()
+        b.Nx, b.Ny, b.Nz = body["Nx"], body["Ny"], body["Nz"]
+        b.dx = body["dx"]
+        b.origin = _f3((body.get("gx_min", 0.0), body.get("gy_min", 0.0), body.get("gz_min", 0.0)))
+        b.c = body.get("c", 343.0)
+        b.rho_air = body.get("rho_air", 1.21)
+        b.cell_type = _v(ct)
+        b.plate_iz = body["plate_iz"]
+        b.plate_active = _v(pa)
+        b.plate_mass_density = body.get("plate_mass_density", 7.0)
+        b.plate_stiffness_D = body.get("plate_stiffness_D", 0.45)
+        b.plate_alpha_M = body.get("plate_alpha_M", 2.0)
+        b.plate_beta_K = body.get("plate_beta_K", 1e-5)
+        b.n_pml = body.get("n_pml", 10)
+        b.n_bridge_src = bs.size // 3
+        b.bridge_src_xyz = _v(bs)
+        b.bridge_sigma = body.get("bridge_sigma", 0.01)
+        self._keep.append(b)
+        self.Nx, self.Ny, self.Nz = b.Nx, b.Ny, b.Nz
+        self.n_strings, self.n_pickups, self.n_mics = ns, npk, nm
+        h = L.orc_coevo_create(ns, C.cast(sd, C.c_void_p) if ns else None, npk,
+                               C.cast(pd, C.c_void_p) if npk else None, nm,
+                               C.cast(md, C.c_void_p) if nm else None,
+                               C.cast(C.pointer(b), C.c_void_p), float(sample_rate),
+                               int(modal_stride), float(force_scale))
+        if not h:
+            raise RuntimeError("orc_coevo_create failed")
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            self._L.orc_coevo_destroy(self._h)
+            self._h = None
+
+    def step(self, n):
+        return self._L.orc_coevo_step(self._h, int(n))
+
+    def step_worker(self, n):
+        return self._L.orc_coevo_step_worker(self._h, int(n))
+
+    def step_block_with_drive(self, blocks, pos, n):
+        arrs = [] if blocks is None else [np.ascontiguousarray(np.asarray(x, np.float32))
+                                          for x in blocks]
+        ptrs = (C.c_void_p * max(len(arrs), 1))(*[_v(a) for a in arrs])
+        out = np.zeros(int(n), np.float32)
+        rc = self._L.orc_coevo_step_block_with_drive(
+            self._h, C.cast(ptrs, C.c_void_p) if arrs else None,
+            len(arrs) if arrs else self.n_strings, int(n), float(pos), _v(out))
+        return rc, out
+
+    def pluck_string(self, si, pos, amp, ndur=0):
+        return self._L.orc_coevo_pluck(self._h, si, pos, amp, ndur)
+
+    def inject_string_force(self, si, f, pos):
+        f = np.ascontiguousarray(np.asarray(f, np.float32))
+        return self._L.orc_coevo_inject_force(self._h, si, f.size, _v(f), pos)
+
+    def schedule_pluck(self, onset, si, pos, amp, ndur=0):
+        return self._L.orc_coevo_schedule_pluck(self._h, onset, si, pos, amp, ndur)
+
+    def clear_pluck_schedule(self):
+        self._L.orc_coevo_clear_schedule(self._h)
+
+    def reset(self):
+        return self._L.orc_coevo_reset(self._h)
+
+    def set_damping_scale(self, s):
+        self._L.orc_coevo_set_damping_scale(self._h, s)
+
+    def set_face_fractions(self, fx, fy, fz):
+        a = [np.ascontiguousarray(np.asarray(x, np.float32)) for x in (fx, fy, fz)]
+        return self._L.orc_coevo_set_face_fractions(self._h, _v(a[0]), _v(a[1]), _v(a[2]))
+
+    def get_pickup_output(self, p, n):
+        out = np.zeros(int(n), np.float32)
+        rc = self._L.orc_coevo_get_pickup_output(self._h, p, _v(out), int(n))
+        assert rc == 0
+        return out
+
+    def get_mic_output(self, m, n):
+        out = np.zeros(int(n), np.float32)
+        rc = self._L.orc_coevo_get_mic_output(self._h, m, _v(out), int(n))
+        assert rc == 0
+        return out
+
+    @property
+    def substeps(self):
+        return self._L.orc_coevo_substeps(self._h)
+
+    @property
+    def samples_written(self):
+        return self._L.orc_coevo_written(self._h)
+
+    @property
+    def dt_fdtd(self):
+        return self._L.orc_coevo_dt_fdtd(self._h)
+
+    @property
+    def n_bridge(self):
+        return self._L.orc_coevo_n_bridge(self._h)
+
+    @property
+    def n_neck(self):
+        return self._L.orc_coevo_n_neck(self._h)
+
+    def fields(self):
+        f = self._L.orc_coevo_fdtd(self._h)
+        Nx, Ny, Nz = self.Nx, self.Ny, self.Nz
+        P = np.zeros(Nx * Ny * Nz, np.float32)
+        vx = np.zeros((Nx - 1) * Ny * Nz, np.float32)
+        vy = np.zeros(Nx * (Ny - 1) * Nz, np.float32)
+        vz = np.zeros(Nx * Ny * (Nz - 1), np.float32)
+        self._L.orc_get_fields_f32(f, _v(P), _v(vx), _v(vy), _v(vz))
+        return P, vx, vy, vz
+
+    def plate(self):
+        f = self._L.orc_coevo_fdtd(self._h)
+        w = np.zeros(self.Nx * self.Ny, np.float32)
+        self._L.orc_get_plate_f32(f, _v(w))
+        return w
+
+    def fdtd_step_count(self):
+        return self._L.orc_get_step_count_f32(self._L.orc_coevo_fdtd(self._h))
+
+    def string_state(self, si):
+        N = self.n_nodes[si]
+        out = np.zeros(4 * N + 6, np.float32)
+        rc = self._L.orc_coevo_string_state(self._h, si, _v(out), out.size)
+        assert rc == 0
+        return {"u0": out[:N], "u1": out[N:2 * N], "up0": out[2 * N:3 * N],
+                "up1": out[3 * N:4 * N], "T_eff": out[4 * N], "saddle_w": out[4 * N + 1],
+                "saddle_w_prev": out[4 * N + 2], "bridge_w": out[4 * N + 3],
+                "bridge_w_prev": out[4 * N + 4], "n_sub": int(out[4 * N + 5])}
+
+    def neck_state(self):
+        out = np.zeros(3, np.float32)
+        self._L.orc_coevo_neck_state(self._h, _v(out))
+        return out
+
+    def modal(self, si, n=32):
+        re = np.zeros(n, np.float32)
+        im = np.zeros(n, np.float32)
+        self._L.orc_coevo_modal(self._h, si, _v(re), _v(im), n)
+        return re, im
+
+    def string_frame(self, si):
+        S = self.n_nodes[si] - 1
+        out = np.zeros(9 * S + 4, np.float32)
+        rc = self._L.orc_coevo_string_frame(self._h, si, _v(out), out.size)
+        assert rc == 0
+        return {"tan": out[:3 * S].reshape(S, 3), "nrm": out[3 * S:6 * S].reshape(S, 3),
+                "bin": out[6 * S:9 * S].reshape(S, 3), "ds": out[9 * S], "L": out[9 * S + 1],
+                "dt_s": out[9 * S + 2], "biharm": out[9 * S + 3]}

@@ -114,7 +114,67 @@ _SIGS = {
                                 + [_i, _i, _vp, _vp, _vp]),
     "fdtd_cut_cell_fractions_device": (_i, [_vp, _vp, _vp, _i, _i, _i, _i] + [C.c_double] * 4
                                        + [_i, _i, _vp, _vp, _vp, _vp, _vp]),
+    # instrument co-evolver (include/afdtd_coevolver.h)
+    "coevolver_create": (_vp, [_i, _vp, _i, _vp, _i, _vp, _vp, _f, _i, _f]),
+    "coevolver_destroy": (None, [_vp]),
+    "coevolver_pluck_string": (_i, [_vp, _i, _f, _f, _i]),
+    "coevolver_inject_string_force": (_i, [_vp, _i, _i, _vp, _f]),
+    "coevolver_schedule_pluck": (_i, [_vp, _i, _i, _f, _f, _i]),
+    "coevolver_clear_pluck_schedule": (None, [_vp]),
+    "coevolver_step": (_i, [_vp, _i]),
+    "coevolver_step_timed": (_i, [_vp, _i, _vp]),
+    "coevolver_step_block_with_drive": (_i, [_vp, _vp, _i, _i, _f, _vp]),
+    "coevolver_step_async": (_i, [_vp, _i]),
+    "coevolver_get_progress": (None, [_vp, _vp]),
+    "coevolver_wait": (_i, [_vp]),
+    "coevolver_cancel": (_i, [_vp]),
+    "coevolver_is_running": (_i, [_vp]),
+    "coevolver_get_pickup_output": (_i, [_vp, _i, _vp, _i]),
+    "coevolver_get_mic_output": (_i, [_vp, _i, _vp, _i]),
+    "coevolver_get_pressure_field": (_i, [_vp, _vp, _i]),
+    "coevolver_get_pressure_field_uniform": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _i]),
+    "coevolver_get_amr_cell_centers": (_i, [_vp, _vp, _i]),
+    "coevolver_get_plate_displacement": (_i, [_vp, _vp, _i]),
+    "coevolver_get_plate_dims": (_i, [_vp, _vp, _vp, _vp]),
+    "coevolver_get_modal_amplitudes": (_i, [_vp, _i, _vp, _vp, _i]),
+    "coevolver_sample_pressure": (_i, [_vp, _i, _vp, _vp]),
+    "coevolver_get_string_velocity": (_i, [_vp, _i, _vp, _i]),
+    "coevolver_get_string_displacement": (_i, [_vp, _i, _vp, _i]),
+    "coevolver_get_string_position": (_i, [_vp, _i, _vp, _i]),
+    "coevolver_get_n_strings": (_i, [_vp]),
+    "coevolver_get_n_pickups": (_i, [_vp]),
+    "coevolver_get_n_mics": (_i, [_vp]),
+    "coevolver_get_dt_audio": (_f, [_vp]),
+    "coevolver_get_dt_fdtd": (_f, [_vp]),
+    "coevolver_set_damping_scale": (None, [_vp, _f]),
+    "coevolver_reset": (_i, [_vp]),
+    "coevolver_set_face_fractions": (_i, [_vp, _vp, _vp, _vp]),
+    "coevolver_get_surface_emission": (_i, [_vp, _i, _vp, _vp, _vp, _vp]),
+    "coevolver_sample_velocity": (_i, [_vp, _i, _vp, _vp, _vp, _vp]),
+    "coevolver_get_fdtd": (_vp, [_vp]),
+    "coevolver_get_substeps": (_i, [_vp]),
+    "coevolver_get_string_substeps": (_i, [_vp, _i]),
+    "coevolver_get_samples_written": (_i, [_vp]),
+    "coevolver_get_string_state": (_i, [_vp, _i, _vp, _i]),
+    "coevolver_get_neck_state": (_i, [_vp, _vp]),
 }
+
+# The reference's uniform co-evolver ABI (csrc/include/acoustic_coevolver.h:306-897).
+COEVOLVER_SYMBOLS = (
+    "coevolver_create", "coevolver_destroy", "coevolver_pluck_string",
+    "coevolver_inject_string_force", "coevolver_schedule_pluck",
+    "coevolver_clear_pluck_schedule", "coevolver_step", "coevolver_step_block_with_drive",
+    "coevolver_step_async", "coevolver_get_progress", "coevolver_wait", "coevolver_cancel",
+    "coevolver_is_running", "coevolver_get_pickup_output", "coevolver_get_mic_output",
+    "coevolver_get_pressure_field", "coevolver_get_pressure_field_uniform",
+    "coevolver_get_amr_cell_centers", "coevolver_get_plate_displacement",
+    "coevolver_get_plate_dims", "coevolver_get_modal_amplitudes", "coevolver_sample_pressure",
+    "coevolver_get_string_velocity", "coevolver_get_string_displacement",
+    "coevolver_get_string_position", "coevolver_get_n_strings", "coevolver_get_n_pickups",
+    "coevolver_get_n_mics", "coevolver_get_dt_audio", "coevolver_get_dt_fdtd",
+    "coevolver_set_damping_scale", "coevolver_reset", "coevolver_set_face_fractions",
+    "coevolver_get_surface_emission", "coevolver_sample_velocity",
+)
 
 
 def _init_torch_first():

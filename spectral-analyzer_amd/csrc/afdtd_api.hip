@@ -691,9 +691,14 @@ AcousticFDTDState* create_impl(int Nx, int Ny, int Nz, float dx, float c, float 
 // the first q+1 steps of this call.
 int apply_failure(AcousticFDTDState* st, int start_step, int n_steps, int start_cur,
                   float* const w0[3], void* const v0[3], void* const v20[3], int start_trace,
-                  int start_phase, const std::vector<uint8_t>& par) {
+                  int start_phase, const std::vector<uint8_t>& par, int* status_out) {
     int status[2] = {0, 0};
     AFDTD_CHECK(hipMemcpy(status, st->d_status, sizeof(status), hipMemcpyDeviceToHost));
+    if (status_out) { status_out[0] = status[0]; status_out[1] = status[1]; }
+    if (status[0] == 3 && st->hook_clears_ext && st->nact > 0)
+        // the failing step completed its plate update; a hooked run left the
+        // accumulator for the next kernel, which no longer runs
+        AFDTD_CHECK(launch_fill(st->d_ext, st->Nplate, 0.0f, st->stream));
     if (status[0] == 0) {
         st->step_count = start_step + n_steps;
         return FDTD_OK;
@@ -724,7 +729,7 @@ int apply_failure(AcousticFDTDState* st, int start_step, int n_steps, int start_
     if (status[0] == 3)
         fprintf(stderr, "[fdtd_pressure] UNSTABLE step=%d (sampled stability check)\n",
                 st->step_count);
-    else
+    else if (status[0] != FDTD_STATUS_HOOK)
         fprintf(stderr, "[fdtd_plate] UNSTABLE step=%d (plate displacement guard)\n",
                 st->step_count);
     fflush(stderr);
@@ -733,11 +738,13 @@ int apply_failure(AcousticFDTDState* st, int start_step, int n_steps, int start_
     return FDTD_ERR_UNSTABLE;
 }
 
-// Enqueue one full step with global index s.
-int enqueue_step(AcousticFDTDState* st, int s, hipEvent_t ea, hipEvent_t eb) {
+// Enqueue one full step with global index s.  keep_ext: the plate force
+// accumulator is left for the caller's next hooked kernel to clear.
+int enqueue_step(AcousticFDTDState* st, int s, hipEvent_t ea, hipEvent_t eb,
+                 bool keep_ext = false) {
     if (st->nact > 0) {
         AFDTD_CHECK(launch_plate_step(st, s));
-        AFDTD_CHECK(launch_fill(st->d_ext, st->Nplate, 0.0f, st->stream));
+        if (!keep_ext) AFDTD_CHECK(launch_fill(st->d_ext, st->Nplate, 0.0f, st->stream));
         float* t = st->d_w[2]; st->d_w[2] = st->d_w[1]; st->d_w[1] = st->d_w[0]; st->d_w[0] = t;
     }
     if (ea) AFDTD_CHECK(hipEventRecord(ea, st->stream));
@@ -779,7 +786,8 @@ int enqueue_pair(AcousticFDTDState* st, int s, hipEvent_t ea, hipEvent_t eb) {
     return FDTD_OK;
 }
 
-int step_impl(AcousticFDTDState* st, int n_steps, float* ms_total, float* ms_stencil) {
+int step_impl(AcousticFDTDState* st, int n_steps, float* ms_total, float* ms_stencil,
+              const StepHook* hook = nullptr, int* status_out = nullptr) {
     const int start_step = st->step_count, start_cur = st->cur;
     const int start_trace = st->trace_n, start_phase = st->phase;
     float* w0[3] = {st->d_w[0], st->d_w[1], st->d_w[2]};
@@ -801,14 +809,23 @@ int step_impl(AcousticFDTDState* st, int n_steps, float* ms_total, float* ms_ste
         const int s = start_step + q;
         hipEvent_t ea = ms_stencil ? evs[2 * launches] : nullptr;
         hipEvent_t eb = ms_stencil ? evs[2 * launches + 1] : nullptr;
-        const bool pair = st->family == FDTD_KERNEL_FUSED2 && st->f2_ok && q + 1 < n_steps &&
-                          ((s + 1) & 0xFF) != 0 && st->nact == 0;
-        int rc = pair ? enqueue_pair(st, s, ea, eb) : enqueue_step(st, s, ea, eb);
+        if (hook) {
+            int rc = (*hook)(q, s);
+            if (rc) return rc;
+        }
+        const bool pair = !hook && st->family == FDTD_KERNEL_FUSED2 && st->f2_ok &&
+                          q + 1 < n_steps && ((s + 1) & 0xFF) != 0 && st->nact == 0;
+        const bool keep_ext = hook && st->hook_clears_ext && q + 1 < n_steps;
+        int rc = pair ? enqueue_pair(st, s, ea, eb) : enqueue_step(st, s, ea, eb, keep_ext);
         if (rc) return rc;
         ++launches;
         const int adv = pair ? 2 : 1;
         for (int u = 0; u < adv; ++u) par[q + u] = (uint8_t)(launches & 1);
         q += adv;
+    }
+    if (hook) {
+        int rc = (*hook)(n_steps, start_step + n_steps);
+        if (rc) return rc;
     }
     if (ms_total) AFDTD_CHECK(hipEventRecord(st->ev1, st->stream));
     AFDTD_CHECK(hipStreamSynchronize(st->stream));
@@ -825,10 +842,18 @@ int step_impl(AcousticFDTDState* st, int n_steps, float* ms_total, float* ms_ste
         for (auto& e : evs) (void)hipEventDestroy(e);
     }
     return apply_failure(st, start_step, n_steps, start_cur, w0, v0, v20, start_trace,
-                         start_phase, par);
+                         start_phase, par, status_out);
 }
 
 }  // namespace
+
+int afdtd_step_hooked(AcousticFDTDState* st, int n_steps, const StepHook& hook, float* ms_total,
+                      int* status_out) {
+    if (!st) return FDTD_ERR_NULL;
+    if (status_out) { status_out[0] = 0; status_out[1] = 0; }
+    if (n_steps <= 0) return FDTD_OK;
+    return step_impl(st, n_steps, ms_total, nullptr, &hook, status_out);
+}
 
 extern "C" {
 

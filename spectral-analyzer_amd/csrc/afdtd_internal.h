@@ -135,6 +135,9 @@ struct AcousticFDTDState {
     unsigned char* d_scratch;
 
     int step_count, phase;
+    // Hooked steps (afdtd_step_hooked): the hook's kernel clears the plate
+    // force accumulator, so every step but the call's last skips its fill.
+    int hook_clears_ext;
     int fused_tile_y, fused_seg_x;
     // Single-step sweep with cell geometry: per-cell code bytes (walls,
     // closed / fractional / plate faces), rebuilt when the fractions change.
@@ -155,6 +158,19 @@ struct AcousticFDTDState {
     int f2_ok;              // tables fit the per-tile caches (else single steps)
     int f2_rows;            // extended rows per thread: 2 (16-row tiles) or 1 (8-row)
 };
+
+// Steps with a per-step hook: hook(q, s) runs before step q (global index s)
+// is enqueued and once more with q == n_steps after the last one, and
+// enqueues its own work on st->stream; every step is a single-step launch.
+// Step bookkeeping and failure handling are those of fdtd_step.  A hooked
+// kernel that finds its own failure sets d_status = {FDTD_STATUS_HOOK, s}
+// (s = the step that must not run); status_out receives the raw status word
+// read at the end of the call ({0, 0} when nothing failed).
+#include <functional>
+using StepHook = std::function<int(int q, int s)>;
+constexpr int FDTD_STATUS_HOOK = 16;
+int afdtd_step_hooked(AcousticFDTDState* st, int n_steps, const StepHook& hook, float* ms_total,
+                      int* status_out);
 
 // Launchers (all enqueue on st->stream; return hipError_t of the launch).
 hipError_t launch_velocity_generic(AcousticFDTDState* st);
