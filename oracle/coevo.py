@@ -89,6 +89,8 @@ def lib():
             "orc_get_fields_f32": (i, [vp, vp, vp, vp, vp]),
             "orc_get_plate_f32": (i, [vp, vp]),
             "orc_get_step_count_f32": (i, [vp]),
+            "orc_coevo_set_string": (i, [vp, i, vp, vp, vp, vp]),
+            "orc_coevo_string_params": (None, [vp, i, vp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -305,3 +307,14 @@ class OracleCoEvolver:
         return {"tan": out[:3 * S].reshape(S, 3), "nrm": out[3 * S:6 * S].reshape(S, 3),
                 "bin": out[6 * S:9 * S].reshape(S, 3), "ds": out[9 * S], "L": out[9 * S + 1],
                 "dt_s": out[9 * S + 2], "biharm": out[9 * S + 3]}
+
+    def set_string(self, si, u0=None, u1=None, up0=None, up1=None):
+        arrs = [None if x is None else np.ascontiguousarray(np.asarray(x, np.float32))
+                for x in (u0, u1, up0, up1)]
+        rc = self._L.orc_coevo_set_string(self._h, si, *[None if x is None else _v(x) for x in arrs])
+        assert rc == 0
+
+    def string_params(self, si):
+        out = np.zeros(4, np.float32)
+        self._L.orc_coevo_string_params(self._h, si, _v(out))
+        return {"n_sub": int(out[0]), "ds": float(out[1]), "dt_s": float(out[2]), "c2": float(out[3])}

@@ -1044,3 +1044,27 @@ SK_EXPORT int orc_coevo_string_frame(const OrcCoevo* st, int si, float* out, int
     out[9 * S + 3] = ss->biharm;
     return 0;
 }
+
+/* Test hook: overwrite one string's displacement fields (u[0], u[1],
+ * u_prev[0], u_prev[1]; NULL keeps a field). */
+SK_EXPORT int orc_coevo_set_string(OrcCoevo* st, int si, const float* u0, const float* u1,
+                                   const float* up0, const float* up1)
+{
+    if (si < 0 || si >= st->ns) return CE_ERR_STRING_IDX;
+    OString* ss = &st->s[si];
+    size_t b = ss->n_nodes * sizeof(float);
+    if (u0) memcpy(ss->u[0], u0, b);
+    if (u1) memcpy(ss->u[1], u1, b);
+    if (up0) memcpy(ss->up[0], up0, b);
+    if (up1) memcpy(ss->up[1], up1, b);
+    return 0;
+}
+
+/* Sub-steps per body step of one string and the string's ds, dt_s, c2 at T0. */
+SK_EXPORT void orc_coevo_string_params(const OrcCoevo* st, int si, float* out4)
+{
+    const OString* ss = &st->s[si];
+    out4[0] = (float)ss->n_sub; out4[1] = ss->ds; out4[2] = ss->dt_s;
+    float cfl = sqrtf(ss->T0 / ss->mu) * ss->dt_s / ss->ds;
+    out4[3] = cfl * cfl;
+}
