@@ -91,6 +91,8 @@ def lib():
             "orc_get_step_count_f32": (i, [vp]),
             "orc_coevo_set_string": (i, [vp, i, vp, vp, vp, vp]),
             "orc_coevo_string_params": (None, [vp, i, vp]),
+            "orc_coevo_tables": (None, [vp] * 8),
+            "orc_get_cell_type_f32": (i, [vp, vp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -318,3 +320,19 @@ class OracleCoEvolver:
         out = np.zeros(4, np.float32)
         self._L.orc_coevo_string_params(self._h, si, _v(out))
         return {"n_sub": int(out[0]), "ds": float(out[1]), "dt_s": float(out[2]), "c2": float(out[3])}
+
+    def tables(self):
+        nb, nn, ns = self.n_bridge, self.n_neck, self.n_strings
+        bi, bw = np.zeros(max(nb, 1), np.int32), np.zeros(max(nb, 1), np.float32)
+        sbk = np.zeros(max(nb * ns, 1), np.float32)
+        ni, nw = np.zeros(max(nn, 1), np.int32), np.zeros(max(nn, 1), np.float32)
+        si, sw = np.zeros(max(4 * ns, 1), np.int32), np.zeros(max(4 * ns, 1), np.float32)
+        self._L.orc_coevo_tables(self._h, _v(bi), _v(bw), _v(sbk), _v(ni), _v(nw), _v(si), _v(sw))
+        return {"bidx": bi[:nb], "bwgt": bw[:nb], "sbk": sbk[:nb * ns].reshape(ns, nb),
+                "nidx": ni[:nn], "nwgt": nw[:nn], "sidx": si[:4 * ns].reshape(ns, 4),
+                "swgt": sw[:4 * ns].reshape(ns, 4)}
+
+    def cell_type(self):
+        out = np.zeros(self.Nx * self.Ny * self.Nz, np.uint8)
+        self._L.orc_get_cell_type_f32(self._L.orc_coevo_fdtd(self._h), _v(out))
+        return out

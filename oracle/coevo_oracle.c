@@ -1068,3 +1068,14 @@ SK_EXPORT void orc_coevo_string_params(const OrcCoevo* st, int si, float* out4)
     float cfl = sqrtf(ss->T0 / ss->mu) * ss->dt_s / ss->ds;
     out4[3] = cfl * cfl;
 }
+
+/* Set-up tables: bridge cells and weights (nb), per-string bridge kernels
+ * (ns x nb), neck cells and weights (nn), saddle taps (ns x 4). */
+SK_EXPORT void orc_coevo_tables(const OrcCoevo* st, int* bidx, float* bwgt, float* sbk,
+                                int* nidx, float* nwgt, int* sidx, float* swgt)
+{
+    if (st->nb) { memcpy(bidx, st->bidx, st->nb * sizeof(int)); memcpy(bwgt, st->bwgt, st->nb * sizeof(float)); }
+    if (st->nb && st->ns && st->sbk) memcpy(sbk, st->sbk, (size_t)st->ns * st->nb * sizeof(float));
+    if (st->nn) { memcpy(nidx, st->nidx, st->nn * sizeof(int)); memcpy(nwgt, st->nwgt, st->nn * sizeof(float)); }
+    if (st->ns) { memcpy(sidx, st->sad_idx, st->ns * 4 * sizeof(int)); memcpy(swgt, st->sad_wgt, st->ns * 4 * sizeof(float)); }
+}

@@ -323,3 +323,71 @@ def test_bridge_and_neck_modes_obey_their_odes(co):
     assert worst_b < 1e-3 and worst_n < 4e-3, (worst_b, worst_n)
     assert max(abs(float(h[0][0]["bridge_w"])) for h in hist) > 0
     assert max(abs(h[1][0]) for h in hist) > 0
+
+
+def test_setup_tables_brute_force(co):
+    """Bridge cells (Gaussian disc of radius 3 sigma under the plate, weights
+    >= 1e-4, merged, normalised; :799-861), per-string bridge kernels (:1322-
+    1375), the neck-joint kernel (centre 12 mm below the highest active plate
+    row, :868-940), the saddle bilinear taps (:1160-1183) and the canonical
+    soundhole carved from the plate (:946-981), recomputed in float64 from
+    those rules."""
+    import sys
+    sys.path.insert(0, __import__("os").path.dirname(__file__))
+    from test_gpu_coevolver import _instrument
+    S, P, M, B = _instrument(dx=0.008, pad=4, n_segs=40, n_strings=3, n_pml=0)
+    o = co.OracleCoEvolver(S, P, M, B, 44100.0, 16)
+    t = o.tables()
+    Nx, Ny, Nz, dx = B["Nx"], B["Ny"], B["Nz"], B["dx"]
+    org = np.array([B["gx_min"], B["gy_min"]])
+    sig = B["bridge_sigma"]
+    rc = int(math.ceil(3 * sig / dx)) + 1
+    cells = {}
+    for sx, sy, _ in np.asarray(B["bridge_src_xyz"], np.float64):
+        c0 = np.rint((np.array([sx, sy]) - org) / dx).astype(int)
+        for di in range(-rc, rc + 1):
+            for dj in range(-rc, rc + 1):
+                i, j = c0[0] + di, c0[1] + dj
+                if not (0 <= i < Nx and 0 <= j < Ny):
+                    continue
+                w = math.exp(-((i * dx + org[0] - sx) ** 2 + (j * dx + org[1] - sy) ** 2) /
+                             (2 * sig * sig))
+                if w < 1e-4:
+                    continue
+                flat = (B["plate_iz"] - 1) + Nz * (j + Ny * i)
+                cells[flat] = cells.get(flat, 0.0) + w
+    assert list(t["bidx"]) == list(cells)                  # first-seen order
+    tot = sum(cells.values())
+    assert np.allclose(t["bwgt"], np.array(list(cells.values())) / tot, rtol=1e-5, atol=0)
+    ij = np.array([((f // Nz) // Ny, (f // Nz) % Ny) for f in cells], np.float64)
+    xy = ij * dx + org
+    for si, d in enumerate(S):
+        sx, sy = np.asarray(d["path_xyz"], np.float64)[-1, :2]
+        k = np.exp(-((xy[:, 0] - sx) ** 2 + (xy[:, 1] - sy) ** 2) / (2 * sig * sig))
+        assert np.allclose(t["sbk"][si], k / k.sum(), rtol=1e-5, atol=1e-9)
+        g = np.clip((np.array([sx, sy]) - org) / dx, 0, [Nx - 1, Ny - 1])
+        i0, j0 = int(g[0]), int(g[1])
+        i1, j1 = min(i0 + 1, Nx - 1), min(j0 + 1, Ny - 1)
+        fi, fj = g[0] - i0, g[1] - j0
+        assert list(t["sidx"][si]) == [j0 + Ny * i0, j0 + Ny * i1, j1 + Ny * i0, j1 + Ny * i1]
+        assert np.allclose(t["swgt"][si], [(1 - fi) * (1 - fj), fi * (1 - fj), (1 - fi) * fj,
+                                           fi * fj], atol=1e-6)
+    pa = np.asarray(B["plate_active"]).reshape(Nx, Ny).astype(bool)
+    ys = np.arange(Ny) * dx + org[1]
+    xs = np.arange(Nx) * dx + org[0]
+    ymax = ys[np.where(pa.any(axis=0))[0].max()]
+    cy, s2 = ymax - 0.012, 0.030
+    rcn = int(math.ceil(3 * s2 / dx)) + 1
+    ci, cj = int(math.floor((0.0 - org[0]) / dx)), int(math.floor((cy - org[1]) / dx))
+    nk = [(j + Ny * i, math.exp(-(xs[i] ** 2 + (ys[j] - cy) ** 2) / (2 * s2 * s2)))
+          for i in range(max(ci - rcn, 0), min(ci + rcn + 1, Nx))
+          for j in range(max(cj - rcn, 0), min(cj + rcn + 1, Ny)) if pa[i, j]]
+    assert list(t["nidx"]) == [p for p, _ in nk]
+    wn = np.array([w for _, w in nk])
+    assert np.allclose(t["nwgt"], wn / wn.sum(), rtol=1e-5, atol=0)
+    # soundhole: plate cells of the body FDTD = active minus the disc
+    X, Y = np.meshgrid(xs, ys, indexing="ij")
+    hole = X ** 2 + (Y - 0.050) ** 2 < 0.028 ** 2
+    plate = o.cell_type().reshape(Nx, Ny, Nz)[:, :, B["plate_iz"]] == 2
+    assert hole.any() and (pa & hole).any()
+    assert np.array_equal(plate, pa & ~hole)
