@@ -162,9 +162,40 @@ __device__ bool string_pol(const float* __restrict__ u, const float* __restrict_
     return wave_any(bad);
 }
 
+// Geometric tension of a string from its current displacements (u0, u1):
+// arc length in index order, strain, T0 + EA strain and the CFL check
+// (acoustic_coevolver.cpp:563-593).  Lane 0 stores T_eff; returns true
+// (wave-uniform) on failure.
+__device__ bool tension_of(const float* u0, const float* u1, float* scr, int N, const CoStr& s,
+                           int lane, float* teff_out) {
+    const float ds2 = s.ds * s.ds;
+    bool bad = false;
+    for (int i = lane; i < N - 1; i += 64) {
+        const float d0 = u0[i + 1] - u0[i];
+        const float d1 = u1[i + 1] - u1[i];
+        if (!__builtin_isfinite(d0) || !__builtin_isfinite(d1)) bad = true;
+        scr[i] = sqrtf(ds2 + d0 * d0 + d1 * d1);
+    }
+    wave_sync();
+    if (wave_any(bad)) return true;
+    int tbad = 0;
+    if (lane == 0) {
+        const float arc = seq_sum(scr, N - 1, 0.0f);
+        float strain = (arc - s.L) / s.L;
+        if (!__builtin_isfinite(strain)) tbad = 1;
+        if (strain < 0.0f) strain = 0.0f;
+        const float teff = s.T0 + s.EA * strain;
+        if (!__builtin_isfinite(teff) || teff <= 0.0f) tbad = 1;
+        const float cfl = sqrtf(teff / s.mu) * s.dt_s / s.ds;
+        if (!__builtin_isfinite(cfl) || cfl * cfl > 0.9025f) tbad = 1;
+        *teff_out = teff;
+    }
+    return __shfl(tbad, 0) != 0;
+}
+
 // LDS: 8 words per string node (six polarisation buffers, the pol-0 force,
 // a sum scratch), then neck products [nn], fpl[64], drive[nb], flags[8],
-// per-string T_eff and u0[1] [2 * ns].  The post phase reuses the string
+// per-string T_eff and u0[1] [2 * ns], the current sub-step's T_eff [ns].  The post phase reuses the string
 // words as scratch (modal staging 2T, pickup velocities and products 4S).
 __global__ __launch_bounds__(kThreads) void k_coevo(CoConst c, CoLaunch a) {
     if (c.status[0]) return;
@@ -176,6 +207,7 @@ __global__ __launch_bounds__(kThreads) void k_coevo(CoConst c, CoLaunch a) {
     int* flag = (int*)(drive + c.nb);          // [0] failure, [1..2] neck_w / neck force, [3..4] sums
     float* fsT = (float*)(flag + 8);
     float* fsU = fsT + c.ns;
+    float* teff0 = fsU + c.ns;                 // T_eff of each string's current sub-step
     if (tid == 0) flag[0] = 0;
 
     // ── finish the previous sub-step (:2105-2125, and :2129-2223 at a sample's end) ──
@@ -293,7 +325,9 @@ __global__ __launch_bounds__(kThreads) void k_coevo(CoConst c, CoLaunch a) {
     if (a.post && c.clear_ext)
         for (int p = tid; p < c.Nplate; p += kThreads) a.ext[p] = 0.0f;
 
-    // ── neck mode (:1934-1983) ──
+    // ── neck mode (:1934-1983) on the last wavefront, beside the strings'
+    // loads and first tension (the two do not depend on each other) ──
+    constexpr int SW = kThreads / 64 - 1;      // string wavefronts
     if (c.neck_on) {
         // out-of-range plate indices are skipped by the reference; adding +0
         // to the running sum (which starts at +0, so never holds -0) is the same
@@ -306,7 +340,9 @@ __global__ __launch_bounds__(kThreads) void k_coevo(CoConst c, CoLaunch a) {
             fsU[si] = c.u[c.str[si].node_off + 1];
         }
         __syncthreads();
-        if (tid == 0) {
+    }
+    if (wave == SW) {
+        if (c.neck_on && lane == 0) {
             const float pw = seq_sum(nprod, c.nn, 0.0f);
             const float nw0 = c.neck[0], nwp = c.neck[1], npw = c.neck[2];
             float fs = 0.0f;
@@ -335,13 +371,37 @@ __global__ __launch_bounds__(kThreads) void k_coevo(CoConst c, CoLaunch a) {
             ((float*)flag)[1] = nw;
             ((float*)flag)[2] = force;
         }
-        __syncthreads();
-        if (flag[0]) return;
+    } else {
+        // string loads, the direct drive and the tension of the first sub-step
+        for (int si = wave; si < c.ns; si += SW) {
+            const CoStr s = c.str[si];
+            const int N = s.n_nodes;
+            float* base = lds + 8 * s.node_off;
+            float* fe0 = base + 6 * N;
+            float* scr = base + 7 * N;
+            for (int i = lane; i < N; i += 64) {
+                const int n = s.node_off + i;
+                base[i] = c.u[n];
+                base[N + i] = c.u[2 * c.tot_nodes + n];
+                base[3 * N + i] = c.u[c.tot_nodes + n];
+                base[4 * N + i] = c.u[3 * c.tot_nodes + n];
+                fe0[i] = 0.0f;
+            }
+            wave_sync();
+            if (lane == 0 && a.sub == 0 && c.blk_dval) {          // direct drive (:2300-2309)
+                const float d = c.blk_dval[a.sample * c.ns + si];
+                if (d != 0.0f) fe0[c.blk_dnode[si]] += d;
+            }
+            if (tension_of(base, base + 3 * N, scr, N, s, lane, &teff0[si]))
+                if (lane == 0) fail_here(c, a, &flag[0]);
+        }
     }
+    __syncthreads();
+    if (flag[0]) return;
     const float nut = c.neck_on ? ((float*)flag)[1] : 0.0f;
 
     // ── strings (:1987-2003): one wavefront per string ──
-    for (int si = wave; si < c.ns; si += kThreads / 64) {
+    for (int si = wave; si < c.ns && wave < SW; si += SW) {
         const CoStr s = c.str[si];
         const int N = s.n_nodes;
         float* base = lds + 8 * s.node_off;
@@ -349,53 +409,21 @@ __global__ __launch_bounds__(kThreads) void k_coevo(CoConst c, CoLaunch a) {
         for (int b = 0; b < 6; ++b) buf[b] = base + b * N;
         float* fe0 = base + 6 * N;
         float* scr = base + 7 * N;
-        for (int i = lane; i < N; i += 64) {
-            const int n = s.node_off + i;
-            buf[0][i] = c.u[n];
-            buf[1][i] = c.u[2 * c.tot_nodes + n];
-            buf[3][i] = c.u[c.tot_nodes + n];
-            buf[4][i] = c.u[3 * c.tot_nodes + n];
-            fe0[i] = 0.0f;
-        }
-        wave_sync();
-        if (lane == 0 && a.sub == 0 && c.blk_dval) {          // direct drive (:2300-2309)
-            const float d = c.blk_dval[a.sample * c.ns + si];
-            if (d != 0.0f) fe0[c.blk_dnode[si]] += d;
-        }
         int r0u = 0, r0p = 1, r0t = 2, r1u = 3, r1p = 4, r1t = 5;
-        float Teff = c.sw[si * kSw + kTeff];
+        float Teff = teff0[si];
         const float bridge_w = c.sw[si * kSw + kBr];
         const float per_sub = c.blk_sub[a.sample * c.ns + si];
         const int fnode = c.blk_node[a.sample * c.ns + si];
-        const float ds2 = s.ds * s.ds;
         bool failed = false;
         for (int q = 0; q < s.n_sub && !failed; ++q) {
-            // geometric tension (:563-593)
-            bool bad = false;
-            for (int i = lane; i < N - 1; i += 64) {
-                const float d0 = buf[r0u][i + 1] - buf[r0u][i];
-                const float d1 = buf[r1u][i + 1] - buf[r1u][i];
-                if (!__builtin_isfinite(d0) || !__builtin_isfinite(d1)) bad = true;
-                scr[i] = sqrtf(ds2 + d0 * d0 + d1 * d1);
+            // geometric tension (:563-593); the first sub-step's was taken above
+            if (q > 0 && tension_of(buf[r0u], buf[r1u], scr, N, s, lane, &teff0[si])) {
+                failed = true;
+                break;
             }
             wave_sync();
-            if (wave_any(bad)) { failed = true; break; }
-            float teff_new = 0.0f;
-            int tbad = 0;
-            if (lane == 0) {
-                const float arc = seq_sum(scr, N - 1, 0.0f);
-                float strain = (arc - s.L) / s.L;
-                if (!__builtin_isfinite(strain)) tbad = 1;
-                if (strain < 0.0f) strain = 0.0f;
-                teff_new = s.T0 + s.EA * strain;
-                if (!__builtin_isfinite(teff_new) || teff_new <= 0.0f) tbad = 1;
-                const float cfl = sqrtf(teff_new / s.mu) * s.dt_s / s.ds;
-                if (!__builtin_isfinite(cfl) || cfl * cfl > 0.9025f) tbad = 1;
-                if (per_sub != 0.0f) fe0[fnode] += per_sub;
-            }
-            tbad = __shfl(tbad, 0);
-            if (tbad) { failed = true; break; }
-            Teff = __shfl(teff_new, 0);
+            Teff = teff0[si];
+            if (lane == 0 && per_sub != 0.0f) fe0[fnode] += per_sub;
             wave_sync();
             const float cfl = sqrtf(Teff / s.mu) * s.dt_s / s.ds;
             const float c2 = cfl * cfl;
@@ -412,7 +440,7 @@ __global__ __launch_bounds__(kThreads) void k_coevo(CoConst c, CoLaunch a) {
             { const int t = r1p; r1p = r1u; r1u = r1t; r1t = t; }
             wave_sync();
             // shape validity (:538-561)
-            bad = false;
+            bool bad = false;
             for (int i = lane; i < N - 1; i += 64) {
                 const float d0 = buf[r0u][i + 1] - buf[r0u][i];
                 const float d1 = buf[r1u][i + 1] - buf[r1u][i];
@@ -1479,7 +1507,7 @@ SK_API AcousticCoEvolverState* coevolver_create(int n_strings, const CoEvolverSt
     // LDS: 8 words per string node, then neck products, fpl[64], drive[nb], flags
     // (the post phase's 2T modal staging and 4S pickup rows fit in the 8T string words)
     st->lds_bytes = sizeof(float) * ((size_t)8 * st->tot_nodes + st->nn + kMaxStrings + st->nb +
-                                     8 + 2 * (size_t)std::max(n_strings, 1));
+                                     8 + 3 * (size_t)std::max(n_strings, 1));
     if (st->lds_bytes > (size_t)kLdsLimit) {
         afdtd_set_error("coevolver_create: strings, neck and bridge kernels exceed the LDS of one "
                         "workgroup");

@@ -154,10 +154,12 @@ hipError_t launch_fused_step(AcousticFDTDState* st) {
     int ty = st->fused_tile_y > 0 ? st->fused_tile_y : (st->fused_geo ? 8 : 16);
     a.seg = st->fused_seg_x > 0 ? st->fused_seg_x : 32;
     if (st->fused_seg_x <= 0) {
-        // small grids: shorter x-segments until the launch has two workgroups
-        // per CU (each cell's arithmetic does not depend on the segment)
+        // small grids: shorter x-segments (down to 4 planes) until the launch
+        // has two workgroups per CU; each cell's arithmetic does not depend on
+        // the segment (a 718 K-cell body: 32 planes 98 us, 8 planes 36 us,
+        // 4 planes 27 us, 2 planes 32 us per step)
         const long tiles = (long)((st->Ny + ty - 1) / ty) * ((st->Nz + TZ - 1) / TZ);
-        while (a.seg > 8 && tiles * ((st->Nx + a.seg - 1) / a.seg) < 512) a.seg /= 2;
+        while (a.seg > 4 && tiles * ((st->Nx + a.seg - 1) / a.seg) < 512) a.seg /= 2;
     }
     if (a.seg > st->Nx) a.seg = st->Nx;
     hipError_t e = st->order == 4 ? launch_store<4>(st, a, ty) : launch_store<2>(st, a, ty);
