@@ -1,7 +1,9 @@
 /*
  * afdtd_coevolver.h — C ABI of the instrument co-evolver in libafdtd.so:
  * strings, bridge and neck impedances, pickups and microphones coupled to
- * the uniform-grid body FDTD, advanced per audio sample.
+ * a body (the uniform-grid FDTD of afdtd.h, or the AMR stepper of
+ * afdtd_amr.h through the pressure-backend descriptor), advanced per audio
+ * sample.
  *
  * Each declaration has the name, signature, return codes and ownership of
  * the reference's co-evolver ABI (csrc/include/acoustic_coevolver.h:line)
@@ -129,6 +131,75 @@ typedef struct {
     float          bridge_sigma;
 } CoEvolverBodyDef;
 
+#ifndef AFDTD_HAVE_AMR_COEVOLVER_DESCRIPTOR
+#define AFDTD_HAVE_AMR_COEVOLVER_DESCRIPTOR
+/* AMR body descriptor — reference acoustic_coevolver.h:325-397: field order
+ * and types of the reference's struct.  The arrays are copied by the
+ * constructor and need only outlive the call. */
+typedef struct AMRCoevolverDescriptor {
+    int            n_cells;
+    const double*  cell_centers;
+    const double*  cell_volumes;
+    const double*  open_volume_frac;
+    const uint8_t* cell_types;
+    const int16_t* cell_levels;
+
+    int            n_faces;
+    const int32_t* face_cell_neg;
+    const int32_t* face_cell_pos;
+    const double*  face_area;
+    const double*  face_open_frac;
+    const double*  face_distance;
+
+    double c;
+    double rho_air;
+    double min_dx;
+
+    double bounds_min[3];
+    double bounds_max[3];
+
+    int            plate_Nx;
+    int            plate_Ny;
+    float          plate_dx;
+    float          plate_origin[3];
+    const uint8_t* plate_active;
+    float          plate_mass_density;
+    float          plate_stiffness_D;
+    float          plate_alpha_M;
+    float          plate_beta_K;
+
+    int            n_active_plate;
+    const int32_t* plate_active_flat_idx;
+    const int32_t* plate_face_above_starts;
+    const int32_t* plate_face_above_idx;
+    const float*   plate_face_above_wgt;
+    const int32_t* plate_face_below_starts;
+    const int32_t* plate_face_below_idx;
+    const float*   plate_face_below_wgt;
+    const int32_t* plate_cell_above;
+    const int32_t* plate_cell_below;
+
+    int            n_bridge_plate;
+    const int32_t* bridge_plate_idx;
+    const float*   bridge_plate_wgt;
+
+    int            n_neck_plate;
+    const int32_t* neck_plate_idx;
+    const float*   neck_plate_wgt;
+
+    double soundhole_cx;
+    double soundhole_cy;
+    double soundhole_radius;
+
+    int   border_mode;
+    float border_sigma_order;
+    float border_R_reflection;
+    float border_Z_match;
+    int   n_pml;
+    int   gradient_order;
+} AMRCoevolverDescriptor;
+#endif
+
 typedef struct AcousticCoEvolverState AcousticCoEvolverState;
 
 /* Construction — reference acoustic_coevolver.h:306-318, definition
@@ -141,6 +212,33 @@ SK_API AcousticCoEvolverState* coevolver_create(
     int n_pickups, const CoEvolverPickupDef* pickup_defs,
     int n_mics, const CoEvolverMicDef* mic_defs,
     const CoEvolverBodyDef* body, float sample_rate, int modal_stride, float force_scale);
+
+/* AMR-backed construction — reference acoustic_coevolver.h:419-429,
+ * definition acoustic_coevolver.cpp:1387-1746: the body is the AMR
+ * pressure backend built from the descriptor (amr_pressure_backend_create:
+ * CSR stepper, plate, bridge and optional neck plate sources, border
+ * condition); the plate grid stands in for the uniform grid (saddle taps,
+ * bridge kernels with sigma 2 * plate_dx over the bridge plate nodes); mics
+ * sample through the backend's stencils; the neck mode is off without neck
+ * plate nodes.  NULL on failure, with the stage label in
+ * coevolver_create_amr_last_error(). */
+SK_API AcousticCoEvolverState* coevolver_create_amr(
+    int n_strings, const CoEvolverStringDef* string_defs,
+    int n_pickups, const CoEvolverPickupDef* pickup_defs,
+    int n_mics, const CoEvolverMicDef* mic_defs,
+    const AMRCoevolverDescriptor* amr_desc,
+    float sample_rate, int modal_stride);
+
+/* Stage label of the last coevolver_create_amr failure on this thread, ""
+ * after a success — reference acoustic_coevolver.h:431-432,
+ * acoustic_coevolver.cpp:1749-1752. */
+SK_API const char* coevolver_create_amr_last_error(void);
+
+/* Extensions (no reference counterpart as C symbols): the AMR pressure
+ * backend's error code name and message of the last failed
+ * coevolver_create_amr on this thread, for callers without C++ linkage. */
+SK_API const char* coevolver_amr_backend_error_code(void);
+SK_API const char* coevolver_amr_backend_error_message(void);
 
 /* reference acoustic_coevolver.h:320, acoustic_coevolver.cpp:1797-1890 */
 SK_API void coevolver_destroy(AcousticCoEvolverState* st);
@@ -177,7 +275,14 @@ SK_API int coevolver_wait(AcousticCoEvolverState* st);
 SK_API int coevolver_cancel(AcousticCoEvolverState* st);
 SK_API int coevolver_is_running(const AcousticCoEvolverState* st);
 
-/* Outputs — reference acoustic_coevolver.h:628-818, acoustic_coevolver.cpp:2511-2696 */
+/* Outputs — reference acoustic_coevolver.h:628-818, acoustic_coevolver.cpp:2511-2696.
+ * On an AMR body, as there: the pressure field is the flat cell array, the
+ * uniform variant scatters it onto a box grid, the plate getters read the
+ * plate grid, coevolver_get_amr_cell_centers returns the cell centres
+ * (CE_ERR_PARAM on a uniform body), coevolver_sample_pressure returns
+ * CE_ERR_PARAM, and the calls that need a uniform grid
+ * (coevolver_set_face_fractions, coevolver_get_surface_emission,
+ * coevolver_sample_velocity) return CE_ERR_NULL. */
 SK_API int coevolver_get_pickup_output(const AcousticCoEvolverState* st, int pickup_idx,
                                        float* out, int n_samples);
 SK_API int coevolver_get_mic_output(const AcousticCoEvolverState* st, int mic_idx, float* out,
@@ -209,7 +314,10 @@ SK_API int coevolver_get_n_mics(const AcousticCoEvolverState* st);
 SK_API float coevolver_get_dt_audio(const AcousticCoEvolverState* st);
 SK_API float coevolver_get_dt_fdtd(const AcousticCoEvolverState* st);
 
-/* reference acoustic_coevolver.h:830-847, acoustic_coevolver.cpp:2698-2758 */
+/* reference acoustic_coevolver.h:830-847, acoustic_coevolver.cpp:2698-2758.
+ * Reset clears strings, neck, modal state, rings and counters and zeroes the
+ * uniform body's fields; an AMR body is left as it is (the reference resets
+ * only a uniform body, :2702). */
 SK_API void coevolver_set_damping_scale(AcousticCoEvolverState* st, float scale);
 SK_API int coevolver_reset(AcousticCoEvolverState* st);
 SK_API int coevolver_set_face_fractions(AcousticCoEvolverState* st, const float* vx_frac,

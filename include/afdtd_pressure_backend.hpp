@@ -20,76 +20,12 @@
 
 #include "afdtd.h"
 #include "afdtd_amr.h"
+#include "afdtd_coevolver.h"
 
 #define AFDTD_BACKEND_API __attribute__((visibility("default")))
 
-#ifndef AFDTD_HAVE_AMR_COEVOLVER_DESCRIPTOR
-#define AFDTD_HAVE_AMR_COEVOLVER_DESCRIPTOR
-/* Field order and types of the reference's descriptor
- * (acoustic_coevolver.h:331-397). */
-typedef struct AMRCoevolverDescriptor {
-    int            n_cells;
-    const double*  cell_centers;
-    const double*  cell_volumes;
-    const double*  open_volume_frac;
-    const uint8_t* cell_types;
-    const int16_t* cell_levels;
-
-    int            n_faces;
-    const int32_t* face_cell_neg;
-    const int32_t* face_cell_pos;
-    const double*  face_area;
-    const double*  face_open_frac;
-    const double*  face_distance;
-
-    double c;
-    double rho_air;
-    double min_dx;
-
-    double bounds_min[3];
-    double bounds_max[3];
-
-    int            plate_Nx;
-    int            plate_Ny;
-    float          plate_dx;
-    float          plate_origin[3];
-    const uint8_t* plate_active;
-    float          plate_mass_density;
-    float          plate_stiffness_D;
-    float          plate_alpha_M;
-    float          plate_beta_K;
-
-    int            n_active_plate;
-    const int32_t* plate_active_flat_idx;
-    const int32_t* plate_face_above_starts;
-    const int32_t* plate_face_above_idx;
-    const float*   plate_face_above_wgt;
-    const int32_t* plate_face_below_starts;
-    const int32_t* plate_face_below_idx;
-    const float*   plate_face_below_wgt;
-    const int32_t* plate_cell_above;
-    const int32_t* plate_cell_below;
-
-    int            n_bridge_plate;
-    const int32_t* bridge_plate_idx;
-    const float*   bridge_plate_wgt;
-
-    int            n_neck_plate;
-    const int32_t* neck_plate_idx;
-    const float*   neck_plate_wgt;
-
-    double soundhole_cx;
-    double soundhole_cy;
-    double soundhole_radius;
-
-    int   border_mode;
-    float border_sigma_order;
-    float border_R_reflection;
-    float border_Z_match;
-    int   n_pml;
-    int   gradient_order;
-} AMRCoevolverDescriptor;
-#endif
+/* AMRCoevolverDescriptor: include/afdtd_coevolver.h (reference
+ * acoustic_coevolver.h:331-397) */
 
 /* acoustic_pressure_backend.h:36-201 */
 struct AFDTD_BACKEND_API IPressureBackend {

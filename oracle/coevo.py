@@ -49,6 +49,68 @@ class _Body(C.Structure):
                 ("bridge_sigma", C.c_float)]
 
 
+# the AMR body descriptor, field for field (acoustic_coevolver.h:331-397)
+_AMR_FIELDS = [
+    ("n_cells", "i"), ("cell_centers", "p"), ("cell_volumes", "p"), ("open_volume_frac", "p"),
+    ("cell_types", "p"), ("cell_levels", "p"), ("n_faces", "i"), ("face_cell_neg", "p"),
+    ("face_cell_pos", "p"), ("face_area", "p"), ("face_open_frac", "p"), ("face_distance", "p"),
+    ("c", "d"), ("rho_air", "d"), ("min_dx", "d"), ("bounds_min", "d3"), ("bounds_max", "d3"),
+    ("plate_Nx", "i"), ("plate_Ny", "i"), ("plate_dx", "f"), ("plate_origin", "f3"),
+    ("plate_active", "p"), ("plate_mass_density", "f"), ("plate_stiffness_D", "f"),
+    ("plate_alpha_M", "f"), ("plate_beta_K", "f"), ("n_active_plate", "i"),
+    ("plate_active_flat_idx", "p"), ("plate_face_above_starts", "p"),
+    ("plate_face_above_idx", "p"), ("plate_face_above_wgt", "p"),
+    ("plate_face_below_starts", "p"), ("plate_face_below_idx", "p"),
+    ("plate_face_below_wgt", "p"), ("plate_cell_above", "p"), ("plate_cell_below", "p"),
+    ("n_bridge_plate", "i"), ("bridge_plate_idx", "p"), ("bridge_plate_wgt", "p"),
+    ("n_neck_plate", "i"), ("neck_plate_idx", "p"), ("neck_plate_wgt", "p"),
+    ("soundhole_cx", "d"), ("soundhole_cy", "d"), ("soundhole_radius", "d"),
+    ("border_mode", "i"), ("border_sigma_order", "f"), ("border_R_reflection", "f"),
+    ("border_Z_match", "f"), ("n_pml", "i"), ("gradient_order", "i")]
+_CT = {"i": C.c_int, "p": C.c_void_p, "d": C.c_double, "f": C.c_float,
+       "d3": C.c_double * 3, "f3": C.c_float * 3}
+# array fields and their dtypes (pybind_kernels.cpp:5961-5988)
+_AMR_ARRAYS = {
+    "cell_centers": np.float64, "cell_volumes": np.float64, "open_volume_frac": np.float64,
+    "cell_types": np.uint8, "cell_levels": np.int16, "face_cell_neg": np.int32,
+    "face_cell_pos": np.int32, "face_area": np.float64, "face_open_frac": np.float64,
+    "face_distance": np.float64, "plate_active": np.uint8, "plate_active_flat_idx": np.int32,
+    "plate_face_above_starts": np.int32, "plate_face_above_idx": np.int32,
+    "plate_face_above_wgt": np.float32, "plate_face_below_starts": np.int32,
+    "plate_face_below_idx": np.int32, "plate_face_below_wgt": np.float32,
+    "plate_cell_above": np.int32, "plate_cell_below": np.int32, "bridge_plate_idx": np.int32,
+    "bridge_plate_wgt": np.float32, "neck_plate_idx": np.int32, "neck_plate_wgt": np.float32}
+
+
+class _AmrDesc(C.Structure):
+    _fields_ = [(n, _CT[t]) for n, t in _AMR_FIELDS]
+
+
+def _amr_desc(d, keep):
+    """The descriptor struct from the reference's amr_dict keys."""
+    s = _AmrDesc()
+    for n, t in _AMR_FIELDS:
+        if t == "p":
+            a = np.ascontiguousarray(np.asarray(d[n], _AMR_ARRAYS[n]).ravel())
+            keep.append(a)
+            setattr(s, n, _v(a))
+        elif t in ("d3", "f3"):
+            a = np.asarray(d[n], np.float64).ravel()
+            setattr(s, n, _CT[t](*[float(x) for x in a[:3]]))
+        elif n in ("border_mode", "n_pml"):
+            setattr(s, n, int(d.get(n, 0)))
+        elif n in ("border_sigma_order",):
+            setattr(s, n, float(d.get(n, 3.0)))
+        elif n in ("border_R_reflection", "border_Z_match"):
+            setattr(s, n, float(d.get(n, 0.0)))
+        elif t == "i":
+            setattr(s, n, int(d[n]))
+        else:
+            setattr(s, n, float(d[n]))
+    keep.append(s)
+    return s
+
+
 def build(force: bool = False) -> str:
     if force or not os.path.exists(_LIB_PATH):
         subprocess.run(["make", "-s", "-B" if force else "all"], cwd=_HERE, check=True)
@@ -93,6 +155,12 @@ def lib():
             "orc_coevo_string_params": (None, [vp, i, vp]),
             "orc_coevo_tables": (None, [vp] * 8),
             "orc_get_cell_type_f32": (i, [vp, vp]),
+            "orc_coevo_create_amr": (vp, [i, vp, i, vp, i, vp, vp, f, i, C.POINTER(C.c_char_p)]),
+            "orc_coevo_amr": (vp, [vp]),
+            "amo_get_pressure": (i, [vp, vp, i]),
+            "amo_get_velocity": (i, [vp, vp, i]),
+            "amo_get_plate": (i, [vp, vp, i]),
+            "amo_get_step_count": (i, [vp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -118,7 +186,44 @@ class OracleCoEvolver:
                  modal_stride=16, force_scale=1.0):
         L = self._L = lib()
         self._h = None
+        self._amr_desc = None
         self._keep = []
+        sd, pd, md = self._defs(string_defs, pickup_defs, mic_defs)
+        ns, npk, nm = len(string_defs), len(pickup_defs), len(mic_defs)
+        ct = np.ascontiguousarray(np.asarray(body["cell_type"], np.uint8).ravel())
+        pa = np.ascontiguousarray(np.asarray(body["plate_active"], np.uint8).ravel())
+        bs = np.ascontiguousarray(np.asarray(body["bridge_src_xyz"], np.float32).ravel())
+        self._keep += [ct, pa, bs]
+        b = _Body()
+        b.Nx, b.Ny, b.Nz = body["Nx"], body["Ny"], body["Nz"]
+        b.dx = body["dx"]
+        b.origin = _f3((body.get("gx_min", 0.0), body.get("gy_min", 0.0), body.get("gz_min", 0.0)))
+        b.c = body.get("c", 343.0)
+        b.rho_air = body.get("rho_air", 1.21)
+        b.cell_type = _v(ct)
+        b.plate_iz = body["plate_iz"]
+        b.plate_active = _v(pa)
+        b.plate_mass_density = body.get("plate_mass_density", 7.0)
+        b.plate_stiffness_D = body.get("plate_stiffness_D", 0.45)
+        b.plate_alpha_M = body.get("plate_alpha_M", 2.0)
+        b.plate_beta_K = body.get("plate_beta_K", 1e-5)
+        b.n_pml = body.get("n_pml", 10)
+        b.n_bridge_src = bs.size // 3
+        b.bridge_src_xyz = _v(bs)
+        b.bridge_sigma = body.get("bridge_sigma", 0.01)
+        self._keep.append(b)
+        self.Nx, self.Ny, self.Nz = b.Nx, b.Ny, b.Nz
+        self.n_strings, self.n_pickups, self.n_mics = ns, npk, nm
+        h = L.orc_coevo_create(ns, C.cast(sd, C.c_void_p) if ns else None, npk,
+                               C.cast(pd, C.c_void_p) if npk else None, nm,
+                               C.cast(md, C.c_void_p) if nm else None,
+                               C.cast(C.pointer(b), C.c_void_p), float(sample_rate),
+                               int(modal_stride), float(force_scale))
+        if not h:
+            raise RuntimeError("orc_coevo_create failed")
+        self._h = h
+
+    def _defs(self, string_defs, pickup_defs, mic_defs):
         ns, npk, nm = len(string_defs), len(pickup_defs), len(mic_defs)
         sd = (_Str * max(ns, 1))()
         self.n_nodes = []
@@ -152,38 +257,35 @@ class OracleCoEvolver:
             md[q].polar_a = d.get("polar_a", 1.0)
             md[q].polar_b = d.get("polar_b", 0.0)
             md[q].gain = d.get("gain", 1.0)
-        ct = np.ascontiguousarray(np.asarray(body["cell_type"], np.uint8).ravel())
-        pa = np.ascontiguousarray(np.asarray(body["plate_active"], np.uint8).ravel())
-        bs = np.ascontiguousarray(np.asarray(body["bridge_src_xyz"], np.float32).ravel())
-        self._keep += [ct, pa, bs, sd, pd, md]
-        b = _Body()
-        b.Nx, b.Ny, b.Nz = body["Nx"], body["Ny"], body["Nz"]
-        b.dx = body["dx"]
-        b.origin = _f3((body.get("gx_min", 0.0), body.get("gy_min", 0.0), body.get("gz_min", 0.0)))
-        b.c = body.get("c", 343.0)
-        b.rho_air = body.get("rho_air", 1.21)
-        b.cell_type = _v(ct)
-        b.plate_iz = body["plate_iz"]
-        b.plate_active = _v(pa)
-        b.plate_mass_density = body.get("plate_mass_density", 7.0)
-        b.plate_stiffness_D = body.get("plate_stiffness_D", 0.45)
-        b.plate_alpha_M = body.get("plate_alpha_M", 2.0)
-        b.plate_beta_K = body.get("plate_beta_K", 1e-5)
-        b.n_pml = body.get("n_pml", 10)
-        b.n_bridge_src = bs.size // 3
-        b.bridge_src_xyz = _v(bs)
-        b.bridge_sigma = body.get("bridge_sigma", 0.01)
-        self._keep.append(b)
-        self.Nx, self.Ny, self.Nz = b.Nx, b.Ny, b.Nz
+        self._keep += [sd, pd, md]
+        return sd, pd, md
+
+    @classmethod
+    def from_amr(cls, amr_desc, string_defs, pickup_defs, mic_defs, sample_rate,
+                 modal_stride=16):
+        """coevolver_create_amr (acoustic_coevolver.cpp:1387-1746) with the
+        reference's amr_dict keys (pybind_kernels.cpp:5887-6060)."""
+        self = cls.__new__(cls)
+        L = self._L = lib()
+        self._h = None
+        self._keep = []
+        sd, pd, md = self._defs(string_defs, pickup_defs, mic_defs)
+        ns, npk, nm = len(string_defs), len(pickup_defs), len(mic_defs)
+        d = _amr_desc(amr_desc, self._keep)
+        self._amr_desc = amr_desc
+        self.Nx, self.Ny, self.Nz = int(amr_desc["plate_Nx"]), int(amr_desc["plate_Ny"]), 1
+        self.n_cells, self.n_faces = int(amr_desc["n_cells"]), int(amr_desc["n_faces"])
         self.n_strings, self.n_pickups, self.n_mics = ns, npk, nm
-        h = L.orc_coevo_create(ns, C.cast(sd, C.c_void_p) if ns else None, npk,
-                               C.cast(pd, C.c_void_p) if npk else None, nm,
-                               C.cast(md, C.c_void_p) if nm else None,
-                               C.cast(C.pointer(b), C.c_void_p), float(sample_rate),
-                               int(modal_stride), float(force_scale))
+        stage = C.c_char_p()
+        h = L.orc_coevo_create_amr(ns, C.cast(sd, C.c_void_p) if ns else None, npk,
+                                   C.cast(pd, C.c_void_p) if npk else None, nm,
+                                   C.cast(md, C.c_void_p) if nm else None,
+                                   C.cast(C.pointer(d), C.c_void_p), float(sample_rate),
+                                   int(modal_stride), C.byref(stage))
         if not h:
-            raise RuntimeError("orc_coevo_create failed")
+            raise RuntimeError(f"orc_coevo_create_amr failed at {stage.value.decode()}")
         self._h = h
+        return self
 
     def __del__(self):
         if getattr(self, "_h", None):
@@ -262,6 +364,13 @@ class OracleCoEvolver:
         return self._L.orc_coevo_n_neck(self._h)
 
     def fields(self):
+        if self._amr_desc is not None:                   # AMR body: cells, faces
+            a = self._L.orc_coevo_amr(self._h)
+            P = np.zeros(self.n_cells, np.float32)
+            V = np.zeros(self.n_faces, np.float32)
+            self._L.amo_get_pressure(a, _v(P), P.size)
+            self._L.amo_get_velocity(a, _v(V), V.size)
+            return P, V
         f = self._L.orc_coevo_fdtd(self._h)
         Nx, Ny, Nz = self.Nx, self.Ny, self.Nz
         P = np.zeros(Nx * Ny * Nz, np.float32)
@@ -272,6 +381,10 @@ class OracleCoEvolver:
         return P, vx, vy, vz
 
     def plate(self):
+        if self._amr_desc is not None:
+            w = np.zeros(self.Nx * self.Ny, np.float32)
+            self._L.amo_get_plate(self._L.orc_coevo_amr(self._h), _v(w), w.size)
+            return w
         f = self._L.orc_coevo_fdtd(self._h)
         w = np.zeros(self.Nx * self.Ny, np.float32)
         self._L.orc_get_plate_f32(f, _v(w))

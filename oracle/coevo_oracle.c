@@ -25,6 +25,7 @@
  * Compiled with -ffp-contract=off (no FMA contraction), as the reference.
  */
 #include "fdtd_oracle.c"
+#include "amr_oracle.c"
 
 #include <float.h>
 #include <stdio.h>
@@ -72,7 +73,8 @@ typedef struct OrcCoevo {
     OString* s;
     OPickup* pk;
     OMic* mic;
-    OrcState_f32* f;
+    OrcState_f32* f;            /* uniform body, or */
+    Amo* amr;                   /* AMR body */
     int Nx, Ny, Nz, plate_iz;
     float dx, origin[3], rho_air, c;
     int* bidx; float* bwgt; int nb;
@@ -355,6 +357,7 @@ SK_EXPORT void orc_coevo_destroy(OrcCoevo* st)
 {
     if (!st) return;
     if (st->f) orc_destroy_f32(st->f);
+    if (st->amr) amo_destroy(st->amr);
     if (st->s) for (int i = 0; i < st->ns; ++i) co_string_free(&st->s[i]);
     if (st->pk) for (int p = 0; p < st->npk; ++p) {
         if (st->pk[p].Bn) for (int i = 0; i < st->ns; ++i) free(st->pk[p].Bn[i]);
@@ -373,44 +376,16 @@ SK_EXPORT void orc_coevo_destroy(OrcCoevo* st)
     free(st);
 }
 
-/* coevolver_create — :987-1381 (the FDTD of fdtd_create: order 4, reference dt) */
-SK_EXPORT OrcCoevo* orc_coevo_create(int ns, const CoEvolverStringDef* sd, int npk,
-                                     const CoEvolverPickupDef* pd, int nmic,
-                                     const CoEvolverMicDef* md, const CoEvolverBodyDef* b,
-                                     float sample_rate, int modal_stride, float force_scale)
+/* Set-up shared by both bodies (:1063-1381, :1480-1745); the body, grid
+ * fields, dt_fdtd, K and the bridge (and AMR neck) plate lists are in place. */
+static int co_setup(OrcCoevo* st, int ns, const CoEvolverStringDef* sd, int npk,
+                    const CoEvolverPickupDef* pd, int nmic, const CoEvolverMicDef* md,
+                    const CoEvolverBodyDef* b, float bridge_sigma)
 {
-    (void)force_scale;
-    if (!sd || !pd || !md || !b) return NULL;
-    if (ns < 0 || npk < 0 || nmic < 0) return NULL;
-    if (sample_rate <= 0 || modal_stride <= 0) return NULL;
-    OrcCoevo* st = (OrcCoevo*)calloc(1, sizeof(OrcCoevo));
-    uint8_t* pa = co_soundhole(b);
-    OrcConfig cf;
-    memset(&cf, 0, sizeof(cf));
-    cf.Nx = b->Nx; cf.Ny = b->Ny; cf.Nz = b->Nz; cf.dx = b->dx; cf.c = b->c;
-    cf.rho_air = b->rho_air; cf.plate_iz = b->plate_iz; cf.plate_mass = b->plate_mass_density;
-    cf.plate_D = b->plate_stiffness_D; cf.plate_alpha_M = b->plate_alpha_M;
-    cf.plate_beta_K = b->plate_beta_K; cf.n_pml = b->n_pml; cf.order = 4; cf.courant = 0.0f;
-    cf.boundary = 0;
-    for (int q = 0; q < 6; ++q) cf.refl[q] = 1.0f;
-    st->f = orc_create_f32(&cf, b->cell_type, pa, NULL, NULL);
-    free(pa);
-    if (!st->f) goto fail;
-    st->Nx = b->Nx; st->Ny = b->Ny; st->Nz = b->Nz; st->dx = b->dx;
-    st->rho_air = b->rho_air; st->c = b->c; st->plate_iz = b->plate_iz;
-    for (int q = 0; q < 3; ++q) st->origin[q] = b->origin[q];
-    st->sr = sample_rate; st->dt_audio = 1.0f / sample_rate; st->dt_fdtd = orc_get_dt_f32(st->f);
-    st->gamma_scale = 1.0f; st->modal_stride = modal_stride;
-    st->K = (int)ceilf(st->dt_audio / st->dt_fdtd);
-    if (st->K < 1) st->K = 1;
-    if (st->K > CO_MAX_SUBSTEPS) st->K = CO_MAX_SUBSTEPS;
-    if (b->n_bridge_src > 0 && b->bridge_src_xyz)
-        co_bridge_kernel(st, b->n_bridge_src, b->bridge_src_xyz, b->bridge_sigma);
-    if (st->nb > 0) orc_set_bridge_sources_f32(st->f, st->nb, st->bidx, st->bwgt);
     st->ns = ns;
     st->s = (OString*)calloc(ns > 0 ? ns : 1, sizeof(OString));
     for (int i = 0; i < ns; ++i)
-        if (co_string_init(&st->s[i], &sd[i], st->dt_fdtd)) goto fail;
+        if (co_string_init(&st->s[i], &sd[i], st->dt_fdtd)) return -1;
     for (int i = 0; i < ns; ++i) {
         if (sd[i].neck_freq_hz > 0.0f && sd[i].neck_mass_kg > 0.0f) {
             st->n_m = sd[i].neck_mass_kg;
@@ -424,7 +399,11 @@ SK_EXPORT OrcCoevo* orc_coevo_create(int ns, const CoEvolverStringDef* sd, int n
             break;
         }
     }
-    if (st->n_m > 0.0f && co_neck_kernel(st, b->plate_active, 0.030f)) goto fail;
+    if (st->amr) {
+        if (st->n_m > 0.0f && st->nn == 0) st->n_m = 0.0f;                 /* :1510-1512 */
+    } else if (st->n_m > 0.0f && co_neck_kernel(st, b->plate_active, 0.030f)) {
+        return -1;
+    }
     st->npk = npk;
     st->pk = (OPickup*)calloc(npk > 0 ? npk : 1, sizeof(OPickup));
     for (int p = 0; p < npk; ++p) {
@@ -476,7 +455,15 @@ SK_EXPORT OrcCoevo* orc_coevo_create(int ns, const CoEvolverStringDef* sd, int n
             st->sad_wgt[i * 4 + 3] = fi * fj;
         }
     }
-    if (nmic > 0) {                                                      /* :1197-1246 */
+    if (nmic > 0 && st->amr) {                                          /* :1613-1646 */
+        float* pos = (float*)calloc(nmic * 3, sizeof(float));
+        float* ax = (float*)calloc(nmic * 3, sizeof(float));
+        for (int m = 0; m < nmic; ++m)
+            for (int q = 0; q < 3; ++q) { pos[m * 3 + q] = md[m].pos[q]; ax[m * 3 + q] = md[m].axis[q]; }
+        int rc = amo_setup_mics(st->amr, nmic, pos, ax, st->any_vel);
+        free(pos); free(ax);
+        if (rc != AMO_OK) return -1;
+    } else if (nmic > 0) {                                               /* :1197-1246 */
         float* g = (float*)calloc(nmic * 3, sizeof(float));
         for (int m = 0; m < nmic; ++m) for (int q = 0; q < 3; ++q) g[m * 3 + q] = st->mic[m].g[q];
         st->mP_i = (int*)calloc(nmic * 8, sizeof(int));
@@ -522,7 +509,7 @@ SK_EXPORT OrcCoevo* orc_coevo_create(int ns, const CoEvolverStringDef* sd, int n
         int nb = st->nb;
         st->sbk = (float*)calloc((size_t)ns * nb, sizeof(float));
         st->bdrive = (float*)calloc(nb, sizeof(float));
-        float sigma = (b->bridge_sigma > 0.0f) ? b->bridge_sigma : st->dx;
+        float sigma = bridge_sigma;
         float i2s2 = 0.5f / (sigma * sigma);
         for (int i = 0; i < ns; ++i) {
             const OString* ss = &st->s[i];
@@ -543,6 +530,123 @@ SK_EXPORT OrcCoevo* orc_coevo_create(int ns, const CoEvolverStringDef* sd, int n
             if (ws > 1e-12f) for (int c = 0; c < nb; ++c) row[c] /= ws;
         }
     }
+    return 0;
+}
+
+/* coevolver_create — :987-1381 (the FDTD of fdtd_create: order 4, reference dt) */
+SK_EXPORT OrcCoevo* orc_coevo_create(int ns, const CoEvolverStringDef* sd, int npk,
+                                     const CoEvolverPickupDef* pd, int nmic,
+                                     const CoEvolverMicDef* md, const CoEvolverBodyDef* b,
+                                     float sample_rate, int modal_stride, float force_scale)
+{
+    (void)force_scale;
+    if (!sd || !pd || !md || !b) return NULL;
+    if (ns < 0 || npk < 0 || nmic < 0) return NULL;
+    if (sample_rate <= 0 || modal_stride <= 0) return NULL;
+    OrcCoevo* st = (OrcCoevo*)calloc(1, sizeof(OrcCoevo));
+    uint8_t* pa = co_soundhole(b);
+    OrcConfig cf;
+    memset(&cf, 0, sizeof(cf));
+    cf.Nx = b->Nx; cf.Ny = b->Ny; cf.Nz = b->Nz; cf.dx = b->dx; cf.c = b->c;
+    cf.rho_air = b->rho_air; cf.plate_iz = b->plate_iz; cf.plate_mass = b->plate_mass_density;
+    cf.plate_D = b->plate_stiffness_D; cf.plate_alpha_M = b->plate_alpha_M;
+    cf.plate_beta_K = b->plate_beta_K; cf.n_pml = b->n_pml; cf.order = 4; cf.courant = 0.0f;
+    cf.boundary = 0;
+    for (int q = 0; q < 6; ++q) cf.refl[q] = 1.0f;
+    st->f = orc_create_f32(&cf, b->cell_type, pa, NULL, NULL);
+    free(pa);
+    if (!st->f) goto fail;
+    st->Nx = b->Nx; st->Ny = b->Ny; st->Nz = b->Nz; st->dx = b->dx;
+    st->rho_air = b->rho_air; st->c = b->c; st->plate_iz = b->plate_iz;
+    for (int q = 0; q < 3; ++q) st->origin[q] = b->origin[q];
+    st->sr = sample_rate; st->dt_audio = 1.0f / sample_rate; st->dt_fdtd = orc_get_dt_f32(st->f);
+    st->gamma_scale = 1.0f; st->modal_stride = modal_stride;
+    st->K = (int)ceilf(st->dt_audio / st->dt_fdtd);
+    if (st->K < 1) st->K = 1;
+    if (st->K > CO_MAX_SUBSTEPS) st->K = CO_MAX_SUBSTEPS;
+    if (b->n_bridge_src > 0 && b->bridge_src_xyz)
+        co_bridge_kernel(st, b->n_bridge_src, b->bridge_src_xyz, b->bridge_sigma);
+    if (st->nb > 0) orc_set_bridge_sources_f32(st->f, st->nb, st->bidx, st->bwgt);
+    if (co_setup(st, ns, sd, npk, pd, nmic, md, b,
+                 (b->bridge_sigma > 0.0f) ? b->bridge_sigma : st->dx)) goto fail;
+    return st;
+fail:
+    orc_coevo_destroy(st);
+    return NULL;
+}
+
+/* coevolver_create_amr — :1387-1746: the AMR pressure backend of
+ * amr_pressure_backend_create (acoustic_pressure_backend.cpp:273-569:
+ * create, plate, bridge and optional neck sources, border condition) and
+ * the plate grid in place of the uniform grid.  *stage gets the failure
+ * label. */
+SK_EXPORT OrcCoevo* orc_coevo_create_amr(int ns, const CoEvolverStringDef* sd, int npk,
+                                         const CoEvolverPickupDef* pd, int nmic,
+                                         const CoEvolverMicDef* md,
+                                         const AMRCoevolverDescriptor* d, float sample_rate,
+                                         int modal_stride, const char** stage)
+{
+    *stage = "";
+    if (ns < 0 || npk < 0 || nmic < 0) { *stage = "invalid_counts"; return NULL; }
+    if ((ns > 0 && !sd) || (npk > 0 && !pd) || (nmic > 0 && !md) || !d) {
+        *stage = "invalid_null_args"; return NULL;
+    }
+    if (sample_rate <= 0 || modal_stride <= 0) { *stage = "invalid_timing_args"; return NULL; }
+    OrcCoevo* st = (OrcCoevo*)calloc(1, sizeof(OrcCoevo));
+    *stage = "pressure_backend_create";
+    if (d->n_cells <= 0 || d->n_faces <= 0 || d->n_bridge_plate <= 0 || !d->bridge_plate_idx ||
+        !d->bridge_plate_wgt || d->plate_Nx <= 0 || d->plate_Ny <= 0 || d->plate_dx <= 0.0f ||
+        d->plate_mass_density <= 0.0f || d->plate_stiffness_D <= 0.0f) goto fail;
+    st->amr = amo_create(d->n_cells, d->cell_centers, d->cell_volumes, d->open_volume_frac,
+                         d->cell_types, d->n_faces, d->face_cell_neg, d->face_cell_pos,
+                         d->face_area, d->face_open_frac, d->face_distance, d->c, d->rho_air,
+                         d->min_dx, d->gradient_order);
+    if (!st->amr) goto fail;
+    if (amo_setup_plate(st->amr, d->plate_Nx, d->plate_Ny, d->plate_dx, d->plate_origin,
+                        d->plate_active, d->plate_mass_density, d->plate_stiffness_D,
+                        d->plate_alpha_M, d->plate_beta_K, d->n_active_plate,
+                        d->plate_active_flat_idx, d->plate_face_above_starts,
+                        d->plate_face_above_idx, d->plate_face_above_wgt,
+                        d->plate_face_below_starts, d->plate_face_below_idx,
+                        d->plate_face_below_wgt, d->plate_cell_above, d->plate_cell_below) != AMO_OK)
+        goto fail;
+    if (amo_set_bridge_sources(st->amr, d->n_bridge_plate, d->bridge_plate_idx,
+                               d->bridge_plate_wgt) != AMO_OK) goto fail;
+    if (d->n_neck_plate > 0 && d->neck_plate_idx && d->neck_plate_wgt &&
+        amo_set_neck_sources(st->amr, d->n_neck_plate, d->neck_plate_idx,
+                             d->neck_plate_wgt) != AMO_OK) goto fail;
+    if (d->n_pml > 0 &&
+        amo_set_border_condition(st->amr, d->border_mode,
+                                 d->border_sigma_order > 0.0f ? d->border_sigma_order : 3.0f,
+                                 d->border_R_reflection, d->border_Z_match, d->n_pml,
+                                 d->bounds_min, d->bounds_max) != AMO_OK) goto fail;
+    st->Nx = d->plate_Nx; st->Ny = d->plate_Ny; st->Nz = 1; st->dx = d->plate_dx;
+    for (int q = 0; q < 3; ++q) st->origin[q] = d->plate_origin[q];
+    st->rho_air = (float)d->rho_air; st->c = (float)d->c;
+    st->sr = sample_rate; st->dt_audio = 1.0f / sample_rate;
+    st->dt_fdtd = (float)amo_get_dt(st->amr);
+    *stage = "dt_fdtd_invalid";
+    if (st->dt_fdtd <= 0.0f) goto fail;
+    st->gamma_scale = 1.0f; st->modal_stride = modal_stride;
+    st->K = (int)ceilf(st->dt_audio / st->dt_fdtd);
+    if (st->K < 1) st->K = 1;
+    if (st->K > CO_MAX_SUBSTEPS) st->K = CO_MAX_SUBSTEPS;
+    st->nb = d->n_bridge_plate;
+    st->bidx = (int*)malloc(st->nb * sizeof(int));
+    st->bwgt = (float*)malloc(st->nb * sizeof(float));
+    memcpy(st->bidx, d->bridge_plate_idx, st->nb * sizeof(int));
+    memcpy(st->bwgt, d->bridge_plate_wgt, st->nb * sizeof(float));
+    if (d->n_neck_plate > 0 && d->neck_plate_idx && d->neck_plate_wgt) {
+        st->nn = d->n_neck_plate;
+        st->nidx = (int*)malloc(st->nn * sizeof(int));
+        st->nwgt = (float*)malloc(st->nn * sizeof(float));
+        memcpy(st->nidx, d->neck_plate_idx, st->nn * sizeof(int));
+        memcpy(st->nwgt, d->neck_plate_wgt, st->nn * sizeof(float));
+    }
+    *stage = "setup";
+    if (co_setup(st, ns, sd, npk, pd, nmic, md, NULL,
+                 (d->plate_dx > 0.0f) ? 2.0f * d->plate_dx : 0.005f)) goto fail;
+    *stage = "";
     return st;
 fail:
     orc_coevo_destroy(st);
@@ -649,7 +753,8 @@ static int co_sample(OrcCoevo* st, int sample_abs)
         float neck_force = 0.0f;
         if (st->n_m > 0.0f && st->nn > 0) {                              /* :1934-1983 */
             float pw;
-            orc_sample_plate_weighted_f32(st->f, st->nn, st->nidx, st->nwgt, &pw);
+            if (st->amr) amo_sample_plate_weighted(st->amr, st->nn, st->nidx, st->nwgt, &pw);
+            else orc_sample_plate_weighted_f32(st->f, st->nn, st->nidx, st->nwgt, &pw);
             float fs = 0.0f;
             for (int i = 0; i < st->ns; ++i) {
                 OString* ss = &st->s[i];
@@ -714,12 +819,20 @@ static int co_sample(OrcCoevo* st, int sample_abs)
                 st->bdrive[c] = a;
             }
         }
-        if (nb > 0 && st->bdrive) orc_inject_bridge_drive_f32(st->f, st->bdrive, 1.0f);
-        if (neck_force != 0.0f && st->nn > 0)
-            orc_inject_plate_force_f32(st->f, st->nn, st->nidx, st->nwgt, neck_force);
-        int rc = orc_step_f32(st->f, 1);
-        if (rc) return -3;
-        orc_sample_plate_batch_f32(st->f, st->ns, st->sad_idx, st->sad_wgt, st->sad_w);
+        if (st->amr) {                                                    /* :2064-2119 */
+            if (nb > 0 && st->bdrive) amo_inject_bridge_drive(st->amr, st->bdrive, nb, 1.0f);
+            if (neck_force != 0.0f && st->nn > 0)
+                amo_inject_plate_force(st->amr, st->nn, st->nidx, st->nwgt, neck_force);
+            if (amo_step(st->amr, 1)) return -3;
+            amo_sample_plate_batch(st->amr, st->ns, st->sad_idx, st->sad_wgt, st->sad_w);
+        } else {
+            if (nb > 0 && st->bdrive) orc_inject_bridge_drive_f32(st->f, st->bdrive, 1.0f);
+            if (neck_force != 0.0f && st->nn > 0)
+                orc_inject_plate_force_f32(st->f, st->nn, st->nidx, st->nwgt, neck_force);
+            int rc = orc_step_f32(st->f, 1);
+            if (rc) return -3;
+            orc_sample_plate_batch_f32(st->f, st->ns, st->sad_idx, st->sad_wgt, st->sad_w);
+        }
         for (int i = 0; i < st->ns; ++i) st->s[i].saddle_w = st->sad_w[i];
         ++st->modal_counter;
         if (st->modal_counter >= st->modal_stride) {
@@ -764,7 +877,21 @@ static int co_sample(OrcCoevo* st, int sample_abs)
         st->pk_acc[p] = sig;
     }
     int op = st->out_pos;
-    if (st->nmic > 0) {                                                  /* :2183-2219 */
+    if (st->nmic > 0 && st->amr) {                                       /* :2185-2219 */
+        float* mv = (float*)calloc((size_t)4 * st->nmic, sizeof(float));
+        amo_sample_mics(st->amr, st->nmic, mv, mv + st->nmic, mv + 2 * st->nmic, mv + 3 * st->nmic);
+        for (int m = 0; m < st->nmic; ++m) {
+            const OMic* ms = &st->mic[m];
+            float sig = mv[m] * ms->pa;
+            if (ms->pb != 0.0f) {
+                float vn = mv[st->nmic + m] * ms->axis[0] + mv[2 * st->nmic + m] * ms->axis[1]
+                         + mv[3 * st->nmic + m] * ms->axis[2];
+                sig += ms->pb * (st->rho_air * st->c) * vn;
+            }
+            st->mic_out[(size_t)m * CO_OUTPUT_BUF + op] = sig * ms->gain;
+        }
+        free(mv);
+    } else if (st->nmic > 0) {                                           /* :2183-2219 */
         const OrcState_f32* f = st->f;
         for (int m = 0; m < st->nmic; ++m) {
             const int* id = st->mP_i + 8 * m;
@@ -934,7 +1061,7 @@ SK_EXPORT int orc_coevo_step_block_with_drive(OrcCoevo* st, const float* const* 
 /* coevolver_reset — :2698-2738 */
 SK_EXPORT int orc_coevo_reset(OrcCoevo* st)
 {
-    orc_reset_f32(st->f);
+    if (st->f) orc_reset_f32(st->f);          /* an AMR body is not reset (:2702) */
     for (int i = 0; i < st->ns; ++i) {
         OString* ss = &st->s[i];
         for (int p = 0; p < 2; ++p) {
@@ -989,6 +1116,7 @@ SK_EXPORT int orc_coevo_get_mic_output(const OrcCoevo* st, int m, float* out, in
 }
 
 SK_EXPORT OrcState_f32* orc_coevo_fdtd(OrcCoevo* st) { return st->f; }
+SK_EXPORT Amo* orc_coevo_amr(OrcCoevo* st) { return st->amr; }
 SK_EXPORT int orc_coevo_substeps(const OrcCoevo* st) { return st->K; }
 SK_EXPORT int orc_coevo_written(const OrcCoevo* st) { return st->written; }
 SK_EXPORT float orc_coevo_dt_fdtd(const OrcCoevo* st) { return st->dt_fdtd; }

@@ -157,6 +157,10 @@ _SIGS = {
     "coevolver_get_samples_written": (_i, [_vp]),
     "coevolver_get_string_state": (_i, [_vp, _i, _vp, _i]),
     "coevolver_get_neck_state": (_i, [_vp, _vp]),
+    "coevolver_create_amr": (_vp, [_i, _vp, _i, _vp, _i, _vp, _vp, _f, _i]),
+    "coevolver_create_amr_last_error": (C.c_char_p, []),
+    "coevolver_amr_backend_error_code": (C.c_char_p, []),
+    "coevolver_amr_backend_error_message": (C.c_char_p, []),
 }
 
 # The reference's uniform co-evolver ABI (csrc/include/acoustic_coevolver.h:306-897).
@@ -173,7 +177,8 @@ COEVOLVER_SYMBOLS = (
     "coevolver_get_string_position", "coevolver_get_n_strings", "coevolver_get_n_pickups",
     "coevolver_get_n_mics", "coevolver_get_dt_audio", "coevolver_get_dt_fdtd",
     "coevolver_set_damping_scale", "coevolver_reset", "coevolver_set_face_fractions",
-    "coevolver_get_surface_emission", "coevolver_sample_velocity",
+    "coevolver_get_surface_emission", "coevolver_sample_velocity", "coevolver_create_amr",
+    "coevolver_create_amr_last_error",
 )
 
 

@@ -8,8 +8,9 @@ body dict, with the same keys and defaults; methods, properties and
 ``coevolver_*`` C ABI of libafdtd.so (include/afdtd_coevolver.h), whose
 per-sample loop runs on the device.
 
-Only the uniform-grid body is provided (``coevolver_create``); the AMR
-factory ``from_amr_desc`` raises.
+Both bodies are provided: the uniform grid (the constructor,
+``coevolver_create``) and the AMR stepper (``from_amr_desc``,
+``coevolver_create_amr``, with the reference's descriptor dict).
 """
 from __future__ import annotations
 
@@ -158,6 +159,90 @@ def build_defs(string_defs, pickup_defs, mic_defs, body):
     return (ns, npk, nm), ptrs, C.pointer(b), keep + [sd, pd, md, b]
 
 
+class AMRCoevolverDescriptor(C.Structure):      # acoustic_coevolver.h:331-397
+    _fields_ = [
+        ("n_cells", C.c_int), ("cell_centers", C.c_void_p), ("cell_volumes", C.c_void_p),
+        ("open_volume_frac", C.c_void_p), ("cell_types", C.c_void_p),
+        ("cell_levels", C.c_void_p), ("n_faces", C.c_int), ("face_cell_neg", C.c_void_p),
+        ("face_cell_pos", C.c_void_p), ("face_area", C.c_void_p),
+        ("face_open_frac", C.c_void_p), ("face_distance", C.c_void_p), ("c", C.c_double),
+        ("rho_air", C.c_double), ("min_dx", C.c_double), ("bounds_min", C.c_double * 3),
+        ("bounds_max", C.c_double * 3), ("plate_Nx", C.c_int), ("plate_Ny", C.c_int),
+        ("plate_dx", C.c_float), ("plate_origin", C.c_float * 3), ("plate_active", C.c_void_p),
+        ("plate_mass_density", C.c_float), ("plate_stiffness_D", C.c_float),
+        ("plate_alpha_M", C.c_float), ("plate_beta_K", C.c_float), ("n_active_plate", C.c_int),
+        ("plate_active_flat_idx", C.c_void_p), ("plate_face_above_starts", C.c_void_p),
+        ("plate_face_above_idx", C.c_void_p), ("plate_face_above_wgt", C.c_void_p),
+        ("plate_face_below_starts", C.c_void_p), ("plate_face_below_idx", C.c_void_p),
+        ("plate_face_below_wgt", C.c_void_p), ("plate_cell_above", C.c_void_p),
+        ("plate_cell_below", C.c_void_p), ("n_bridge_plate", C.c_int),
+        ("bridge_plate_idx", C.c_void_p), ("bridge_plate_wgt", C.c_void_p),
+        ("n_neck_plate", C.c_int), ("neck_plate_idx", C.c_void_p), ("neck_plate_wgt", C.c_void_p),
+        ("soundhole_cx", C.c_double), ("soundhole_cy", C.c_double),
+        ("soundhole_radius", C.c_double), ("border_mode", C.c_int),
+        ("border_sigma_order", C.c_float), ("border_R_reflection", C.c_float),
+        ("border_Z_match", C.c_float), ("n_pml", C.c_int), ("gradient_order", C.c_int)]
+
+
+# (key, dtype) of the descriptor's arrays as the pybind factory reads them
+# (pybind_kernels.cpp:5961-5988)
+_AMR_ARRAYS = (
+    ("cell_centers", np.float64), ("cell_volumes", np.float64),
+    ("open_volume_frac", np.float64), ("cell_types", np.uint8), ("cell_levels", np.int16),
+    ("face_cell_neg", np.int32), ("face_cell_pos", np.int32), ("face_area", np.float64),
+    ("face_open_frac", np.float64), ("face_distance", np.float64),
+    ("plate_active", np.uint8), ("plate_active_flat_idx", np.int32),
+    ("plate_face_above_starts", np.int32), ("plate_face_above_idx", np.int32),
+    ("plate_face_above_wgt", np.float32), ("plate_face_below_starts", np.int32),
+    ("plate_face_below_idx", np.int32), ("plate_face_below_wgt", np.float32),
+    ("plate_cell_above", np.int32), ("plate_cell_below", np.int32),
+    ("bridge_plate_idx", np.int32), ("bridge_plate_wgt", np.float32),
+    ("neck_plate_idx", np.int32), ("neck_plate_wgt", np.float32))
+
+
+def build_amr_desc(amr_dict):
+    """The AMRCoevolverDescriptor the pybind factory fills from the dict of
+    ``build_amr_coevolver_descriptor()`` (pybind_kernels.cpp:5959-6037): the
+    same keys, dtypes and optional-key defaults.  Returns (struct, keep)."""
+    keep = []
+    d = AMRCoevolverDescriptor()
+    for key, dt in _AMR_ARRAYS:
+        a = np.ascontiguousarray(np.asarray(amr_dict[key], dtype=dt).ravel())
+        keep.append(a)
+        setattr(d, key, _v(a))
+    d.n_cells = int(amr_dict["n_cells"])
+    d.n_faces = int(amr_dict["n_faces"])
+    d.c = float(amr_dict["c"])
+    d.rho_air = float(amr_dict["rho_air"])
+    d.min_dx = float(amr_dict["min_dx"])
+    bmin = np.asarray(amr_dict["bounds_min"], np.float64).ravel()
+    bmax = np.asarray(amr_dict["bounds_max"], np.float64).ravel()
+    d.bounds_min = (C.c_double * 3)(*[float(x) for x in bmin[:3]])
+    d.bounds_max = (C.c_double * 3)(*[float(x) for x in bmax[:3]])
+    d.plate_Nx = int(amr_dict["plate_Nx"])
+    d.plate_Ny = int(amr_dict["plate_Ny"])
+    d.plate_dx = float(amr_dict["plate_dx"])
+    d.plate_origin = _f3(amr_dict["plate_origin"])
+    d.plate_mass_density = float(amr_dict["plate_mass_density"])
+    d.plate_stiffness_D = float(amr_dict["plate_stiffness_D"])
+    d.plate_alpha_M = float(amr_dict["plate_alpha_M"])
+    d.plate_beta_K = float(amr_dict["plate_beta_K"])
+    d.n_active_plate = int(amr_dict["n_active_plate"])
+    d.n_bridge_plate = int(amr_dict["n_bridge_plate"])
+    d.n_neck_plate = int(amr_dict["n_neck_plate"])
+    d.soundhole_cx = float(amr_dict["soundhole_cx"])
+    d.soundhole_cy = float(amr_dict["soundhole_cy"])
+    d.soundhole_radius = float(amr_dict["soundhole_radius"])
+    d.border_mode = int(amr_dict.get("border_mode", 0))
+    d.border_sigma_order = float(amr_dict.get("border_sigma_order", 3.0))
+    d.border_R_reflection = float(amr_dict.get("border_R_reflection", 0.0))
+    d.border_Z_match = float(amr_dict.get("border_Z_match", 0.0))
+    d.n_pml = int(amr_dict.get("n_pml", 0))
+    d.gradient_order = int(amr_dict["gradient_order"])
+    keep.append(d)
+    return d, keep
+
+
 class AcousticCoEvolver:
     """Strings, bridge / neck impedances, pickups and microphones coupled to the
     body FDTD, advanced per audio sample on the device."""
@@ -171,6 +256,7 @@ class AcousticCoEvolver:
         self._n_strings, self._n_pickups, self._n_mics = ns, npk, nm
         self._n_nodes = [np.asarray(d["path_xyz"]).size // 3 for d in string_defs]
         self._Nx, self._Ny, self._Nz = int(body["Nx"]), int(body["Ny"]), int(body["Nz"])
+        self._n_cells_amr = 0
         h = self._L.coevolver_create(ns, sp, npk, pp, nm, mp, C.cast(bp, C.c_void_p),
                                      float(sample_rate), int(modal_stride), float(force_scale))
         if not h:
@@ -178,9 +264,46 @@ class AcousticCoEvolver:
         self._h = h
 
     @classmethod
-    def from_amr_desc(cls, *args, **kwargs):
-        raise RuntimeError("AcousticCoEvolver.from_amr_desc: the AMR-backed co-evolver is not "
-                           "provided by this engine (uniform body only)")
+    def from_amr_desc(cls, amr_desc, string_defs, pickup_defs, mic_defs, sample_rate: float,
+                      modal_stride: int = 16):
+        """AMR-backed co-evolver (pybind_kernels.cpp:5887-6098): the body is the
+        AMR stepper built from ``amr_desc``; get_pressure_field scatters the
+        cell pressures onto a box grid of spacing max(4 min_dx, 5 mm)."""
+        self = cls.__new__(cls)
+        self._L = lib()
+        self._h = None
+        dummy = {"cell_type": np.zeros(1, np.uint8), "plate_active": np.zeros(1, np.uint8),
+                 "bridge_src_xyz": np.zeros(0, np.float32), "Nx": 1, "Ny": 1, "Nz": 1,
+                 "dx": 1.0, "plate_iz": 0}
+        (ns, npk, nm), (sp, pp, mp), _bp, keep = build_defs(
+            list(string_defs), list(pickup_defs), list(mic_defs), dummy)
+        d, dkeep = build_amr_desc(amr_desc)
+        self._keep = keep + dkeep
+        self._n_strings, self._n_pickups, self._n_mics = ns, npk, nm
+        self._n_nodes = [np.asarray(x["path_xyz"]).size // 3 for x in string_defs]
+        h = self._L.coevolver_create_amr(ns, sp, npk, pp, nm, mp, C.cast(C.pointer(d), C.c_void_p),
+                                         float(sample_rate), int(modal_stride))
+        if not h:
+            code = (self._L.coevolver_amr_backend_error_code() or b"").decode()
+            msg = (self._L.coevolver_amr_backend_error_message() or b"").decode()
+            stage = (self._L.coevolver_create_amr_last_error() or b"").decode()
+            detail = "coevolver_create_amr failed"
+            if code:
+                detail += f" [{code}]"
+            if msg:
+                detail += f": {msg}"
+            if stage:
+                detail += f" (stage: {stage})"
+            raise RuntimeError(detail)
+        self._h = h
+        viz = max(float(d.min_dx) * 4.0, 0.005)
+        n = [max(1, min(int(round((d.bounds_max[q] - d.bounds_min[q]) / viz)) + 1, 256))
+             for q in range(3)]
+        self._Nx, self._Ny, self._Nz = n
+        self._n_cells_amr = int(d.n_cells)
+        self._bmin = (C.c_double * 3)(*d.bounds_min)
+        self._bmax = (C.c_double * 3)(*d.bounds_max)
+        return self
 
     def __del__(self):
         try:
@@ -287,13 +410,33 @@ class AcousticCoEvolver:
 
     def get_pressure_field(self) -> np.ndarray:
         out = np.zeros((self._Nx, self._Ny, self._Nz), np.float32)
-        rc = self._L.coevolver_get_pressure_field(self._h, _v(out), out.size)
+        if self._n_cells_amr > 0:                    # pybind_kernels.cpp:6227-6236
+            rc = self._L.coevolver_get_pressure_field_uniform(
+                self._h, self._Nx, self._Ny, self._Nz, C.cast(self._bmin, C.c_void_p),
+                C.cast(self._bmax, C.c_void_p), _v(out), out.size)
+        else:
+            rc = self._L.coevolver_get_pressure_field(self._h, _v(out), out.size)
         if rc != CE_OK:
             raise RuntimeError(f"get_pressure_field error: {rc}")
         return out
 
     def get_amr_cell_centers(self):
-        raise RuntimeError("get_amr_cell_centers: not an AMR coevolver")
+        if self._n_cells_amr <= 0:
+            raise RuntimeError("get_amr_cell_centers: not an AMR coevolver")
+        out = np.zeros((self._n_cells_amr, 3), np.float32)
+        rc = self._L.coevolver_get_amr_cell_centers(self._h, _v(out), out.size)
+        if rc != CE_OK:
+            raise RuntimeError(f"get_amr_cell_centers error: {rc}")
+        return out
+
+    def get_amr_pressure(self) -> np.ndarray:
+        """Extension: the AMR body's flat cell pressures (the C ABI's
+        coevolver_get_pressure_field on an AMR handle)."""
+        out = np.zeros(self._n_cells_amr, np.float32)
+        rc = self._L.coevolver_get_pressure_field(self._h, _v(out), out.size)
+        if rc != CE_OK:
+            raise RuntimeError(f"get_pressure_field error: {rc}")
+        return out
 
     def get_plate_dims(self):
         a, b, c = C.c_int(), C.c_int(), C.c_int()

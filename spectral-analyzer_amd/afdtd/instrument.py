@@ -104,3 +104,74 @@ def body_dict(cell_type, plate_active, info: dict, bridge_positions, body_h: flo
          "n_pml": n_pml, "bridge_src_xyz": bxyz, "bridge_sigma": bridge_sigma}
     d.update(plate)
     return d
+
+
+def _plate_patch(positions, origin, pdx, nx, ny, radius):
+    """Plate nodes within `radius` plate cells of each (x, y), with Gaussian
+    weights (sigma one plate cell) normalised per position; entries of
+    different positions may name the same node."""
+    idx, wgt = [], []
+    for x, y in positions:
+        gi, gj = (x - origin[0]) / pdx, (y - origin[1]) / pdx
+        ii, jj = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
+        d2 = (ii - gi) ** 2 + (jj - gj) ** 2
+        sel = np.flatnonzero((d2 <= radius * radius).ravel())
+        if sel.size == 0:
+            sel = np.array([int(np.argmin(d2.ravel()))])
+        w = np.exp(-0.5 * d2.ravel()[sel])
+        idx.append(sel.astype(np.int32))
+        wgt.append((w / w.sum()).astype(np.float32))
+    if not idx:
+        return np.zeros(0, np.int32), np.zeros(0, np.float32)
+    return np.concatenate(idx), np.concatenate(wgt)
+
+
+def amr_box_desc(nx: int, ny: int, nz: int, kp: int, dx: float, origin, bridge_positions,
+                 neck_positions=(), plate_dx: float = None, n_pml: int = 0, border_mode: int = 0,
+                 border_R: float = 0.0, gradient_order: int = 2, bridge_radius: float = 1.5,
+                 neck_radius: float = 1.5, c: float = 343.0, rho_air: float = 1.21):
+    """An AMR body descriptor dict with the keys of the reference's
+    ``build_amr_coevolver_descriptor()`` (acoustic_amr.py; read by
+    pybind_kernels.cpp:5959-6037): a uniform AMR box of nx x ny x nz cells of
+    size dx from `origin`, an optional PML shell, and a Kirchhoff plate in
+    layer kp (``AcousticAMRGrid.with_plate_layer``: plate node (i, j) is cell
+    (i, j, kp), nodes plate_dx apart, 5 dx by default), with bridge and neck
+    plate patches around the given (x, y) positions."""
+    from .amr import AcousticAMRGrid
+    x0, y0, z0 = (float(v) for v in origin)
+    g = AcousticAMRGrid.build_uniform_box(x_min=x0, x_max=x0 + nx * dx, y_min=y0,
+                                          y_max=y0 + ny * dx, z_min=z0, z_max=z0 + nz * dx,
+                                          dx=dx)
+    if n_pml:
+        g = g.with_pml_shell(n_pml)
+    g, kw = g.with_plate_layer(nx, ny, nz, kp, plate_dx=plate_dx)
+    pdx, porg = float(kw["plate_dx"]), [float(v) for v in kw["plate_origin"]]
+    bidx, bwgt = _plate_patch(bridge_positions, porg, pdx, nx, ny, bridge_radius)
+    nidx, nwgt = _plate_patch(neck_positions, porg, pdx, nx, ny, neck_radius)
+    flat = np.asarray(kw["plate_active_flat_idx"], np.int32)
+    return {
+        "n_cells": g.n_cells, "cell_centers": g.cell_centers, "cell_volumes": g.cell_volumes,
+        "open_volume_frac": g.open_volume_fraction, "cell_types": g.cell_types,
+        "cell_levels": g.cell_levels, "n_faces": g.n_faces, "face_cell_neg": g.face_cell_neg,
+        "face_cell_pos": g.face_cell_pos, "face_area": g.face_area,
+        "face_open_frac": g.face_open_fraction, "face_distance": g.face_distance,
+        "c": float(c), "rho_air": float(rho_air), "min_dx": float(g.min_dx),
+        "bounds_min": g.bounds_min, "bounds_max": g.bounds_max,
+        "plate_Nx": nx, "plate_Ny": ny, "plate_dx": pdx,
+        "plate_origin": np.array(porg, np.float32), "plate_active": kw["plate_active"],
+        "plate_mass_density": kw["plate_rho_h"], "plate_stiffness_D": kw["plate_D"],
+        "plate_alpha_M": kw["plate_alpha_M"], "plate_beta_K": kw["plate_beta_K"],
+        "n_active_plate": int(flat.size), "plate_active_flat_idx": flat,
+        "plate_face_above_starts": kw["face_above_starts"],
+        "plate_face_above_idx": kw["face_above_idx"],
+        "plate_face_above_wgt": kw["face_above_wgt"],
+        "plate_face_below_starts": kw["face_below_starts"],
+        "plate_face_below_idx": kw["face_below_idx"],
+        "plate_face_below_wgt": kw["face_below_wgt"],
+        "plate_cell_above": kw["cell_above"], "plate_cell_below": kw["cell_below"],
+        "n_bridge_plate": int(bidx.size), "bridge_plate_idx": bidx, "bridge_plate_wgt": bwgt,
+        "n_neck_plate": int(nidx.size), "neck_plate_idx": nidx, "neck_plate_wgt": nwgt,
+        "soundhole_cx": 0.0, "soundhole_cy": 0.0, "soundhole_radius": 0.0,
+        "border_mode": int(border_mode), "border_sigma_order": 3.0,
+        "border_R_reflection": float(border_R), "border_Z_match": 0.0, "n_pml": int(n_pml),
+        "gradient_order": int(gradient_order)}

@@ -38,6 +38,7 @@
 #include <vector>
 
 #include "../../include/afdtd_amr.h"
+#include "afdtd_internal.h"
 
 static thread_local std::string g_amr_stage = "ok";
 // stencil-build progress of the last order-8 create (global, as the reference's)
@@ -1051,6 +1052,65 @@ void report_diverged(AcousticAMRState* st) {
             n_bad);
 }
 
+// Enqueue the kernels of step s of a call (s counts from the call's first
+// step; each kernel skips when an earlier step of the call failed): faces,
+// plate-driven faces, cells, plate.  `mid` (optional) is recorded between
+// the face and the cell kernels.
+void enqueue_amr_step(AcousticAMRState* st, int s, float dt, float bulk, float dt_over_rho,
+                      hipEvent_t mid) {
+    if (st->order == 8 && st->compact)
+        k_amr_faces8_c<<<(st->nf + kFaceTile8C - 1) / kFaceTile8C, 256, 0, st->stream>>>(
+            st->d_fc, st->d_fcls, st->d_v, st->d_p, st->nf, dt_over_rho, st->d_status, s);
+    else if (st->order == 8)
+        k_amr_faces8<<<blocks(st->nf), 256, 0, st->stream>>>(
+            st->d_sc, st->d_sw, st->d_face, st->d_v, st->d_p, st->nf, dt_over_rho,
+            st->d_status, s);
+    else if (st->compact)
+        k_amr_faces_c<<<(st->nf + kFaceTileC - 1) / kFaceTileC, 256, 0, st->stream>>>(
+            st->d_fc, st->d_fcls, st->d_v, st->d_p, st->nf, dt_over_rho, st->d_status, s);
+    else
+        k_amr_faces<<<(st->nf + kFaceTileD - 1) / kFaceTileD, 256, 0, st->stream>>>(
+            st->d_face, st->d_v, st->d_p, st->nf, dt_over_rho, st->d_status, s);
+    if (st->plate_on && st->n_bc > 0) {                      // :1157
+        const float* w = st->d_pw[(st->prot + s + 1) % 3];
+        const float* wp = st->d_pw[(st->prot + s) % 3];
+        k_amr_plate_bc<<<blocks(st->n_bc), 256, 0, st->stream>>>(
+            st->d_pint + st->o_bcf, st->d_pint + st->o_bcn, st->d_pint + st->o_bcs,
+            st->n_bc, w, wp, dt, st->d_v, st->d_status, s);
+    }
+    if (mid) (void)hipEventRecord(mid, st->stream);
+    const unsigned cell_blocks = (unsigned)((st->nc + kCellTile - 1) / kCellTile);
+    if (st->compact)
+        k_amr_cells<true><<<cell_blocks, 256, 0, st->stream>>>(
+            st->d_ccls, st->d_cc, st->d_start, st->d_csr, st->d_w, st->d_v, st->d_p, st->nc,
+            bulk, st->d_status, s);
+    else
+        k_amr_cells<false><<<cell_blocks, 256, 0, st->stream>>>(
+            st->d_cell, nullptr, st->d_start, st->d_csr, st->d_w, st->d_v, st->d_p, st->nc,
+            bulk, st->d_status, s);
+    if (st->plate_on && st->n_active > 0) {                  // :1179
+        PlateConsts k;
+        k.Nx = st->pNx;
+        k.Ny = st->pNy;
+        k.n_active = st->n_active;
+        k.dx = st->p_dx;
+        k.dx4 = 1.0f / (st->p_dx * st->p_dx * st->p_dx * st->p_dx);
+        k.dt2 = dt * dt;
+        k.inv_rh = 1.0f / st->p_rh;
+        k.damp_fwd = 1.0f + 0.5f * st->p_aM * dt;
+        k.damp_bwd = 1.0f - 0.5f * st->p_aM * dt;
+        k.cD0 = st->p_D * (1.0f + st->p_bK / dt);
+        k.cDp = st->p_D * (st->p_bK / dt);
+        const int* I = st->d_pint;
+        const float* F = st->d_pflt;
+        k_amr_plate_step<<<blocks(st->n_active), 256, 0, st->stream>>>(
+            k, I + st->o_active, I + st->o_ab, I + st->o_abs, F + st->o_abw, I + st->o_bl,
+            I + st->o_bls, F + st->o_blw, I + st->o_ca, I + st->o_cb, st->d_p,
+            st->d_pw[(st->prot + s + 1) % 3], st->d_pw[(st->prot + s) % 3],
+            st->d_pw[(st->prot + s + 2) % 3], st->d_ext, st->d_status, s);
+    }
+}
+
 int step_impl(AcousticAMRState* st, int n, float* ms_total, float* ms_faces, float* ms_cells,
               const MicMix* mix = nullptr) {
     if (!st) return SK_ERR_NULL_STATE;
@@ -1093,57 +1153,7 @@ int step_impl(AcousticAMRState* st, int n, float* ms_total, float* ms_faces, flo
     k_amr_status_init<<<1, 1, 0, st->stream>>>(st->d_status, st->d_best);
     for (int s = 0; s < n; ++s) {
         if (ms_total) (void)hipEventRecord(ev[2 * (size_t)s], st->stream);
-        if (st->order == 8 && st->compact)
-            k_amr_faces8_c<<<(st->nf + kFaceTile8C - 1) / kFaceTile8C, 256, 0, st->stream>>>(
-                st->d_fc, st->d_fcls, st->d_v, st->d_p, st->nf, dt_over_rho, st->d_status, s);
-        else if (st->order == 8)
-            k_amr_faces8<<<blocks(st->nf), 256, 0, st->stream>>>(
-                st->d_sc, st->d_sw, st->d_face, st->d_v, st->d_p, st->nf, dt_over_rho,
-                st->d_status, s);
-        else if (st->compact)
-            k_amr_faces_c<<<(st->nf + kFaceTileC - 1) / kFaceTileC, 256, 0, st->stream>>>(
-                st->d_fc, st->d_fcls, st->d_v, st->d_p, st->nf, dt_over_rho, st->d_status, s);
-        else
-            k_amr_faces<<<(st->nf + kFaceTileD - 1) / kFaceTileD, 256, 0, st->stream>>>(
-                st->d_face, st->d_v, st->d_p, st->nf, dt_over_rho, st->d_status, s);
-        if (st->plate_on && st->n_bc > 0) {                      // :1157
-            const float* w = st->d_pw[(st->prot + s + 1) % 3];
-            const float* wp = st->d_pw[(st->prot + s) % 3];
-            k_amr_plate_bc<<<blocks(st->n_bc), 256, 0, st->stream>>>(
-                st->d_pint + st->o_bcf, st->d_pint + st->o_bcn, st->d_pint + st->o_bcs,
-                st->n_bc, w, wp, dt, st->d_v, st->d_status, s);
-        }
-        if (ms_total) (void)hipEventRecord(ev[2 * (size_t)s + 1], st->stream);
-        const unsigned cell_blocks = (unsigned)((st->nc + kCellTile - 1) / kCellTile);
-        if (st->compact)
-            k_amr_cells<true><<<cell_blocks, 256, 0, st->stream>>>(
-                st->d_ccls, st->d_cc, st->d_start, st->d_csr, st->d_w, st->d_v, st->d_p, st->nc,
-                bulk, st->d_status, s);
-        else
-            k_amr_cells<false><<<cell_blocks, 256, 0, st->stream>>>(
-                st->d_cell, nullptr, st->d_start, st->d_csr, st->d_w, st->d_v, st->d_p, st->nc,
-                bulk, st->d_status, s);
-        if (st->plate_on && st->n_active > 0) {                  // :1179
-            PlateConsts k;
-            k.Nx = st->pNx;
-            k.Ny = st->pNy;
-            k.n_active = st->n_active;
-            k.dx = st->p_dx;
-            k.dx4 = 1.0f / (st->p_dx * st->p_dx * st->p_dx * st->p_dx);
-            k.dt2 = dt * dt;
-            k.inv_rh = 1.0f / st->p_rh;
-            k.damp_fwd = 1.0f + 0.5f * st->p_aM * dt;
-            k.damp_bwd = 1.0f - 0.5f * st->p_aM * dt;
-            k.cD0 = st->p_D * (1.0f + st->p_bK / dt);
-            k.cDp = st->p_D * (st->p_bK / dt);
-            const int* I = st->d_pint;
-            const float* F = st->d_pflt;
-            k_amr_plate_step<<<blocks(st->n_active), 256, 0, st->stream>>>(
-                k, I + st->o_active, I + st->o_ab, I + st->o_abs, F + st->o_abw, I + st->o_bl,
-                I + st->o_bls, F + st->o_blw, I + st->o_ca, I + st->o_cb, st->d_p,
-                st->d_pw[(st->prot + s + 1) % 3], st->d_pw[(st->prot + s) % 3],
-                st->d_pw[(st->prot + s + 2) % 3], st->d_ext, st->d_status, s);
-        }
+        enqueue_amr_step(st, s, dt, bulk, dt_over_rho, ms_total ? ev[2 * (size_t)s + 1] : nullptr);
         if (mix && (s + 1) % mix->substeps == 0) {
             const int* I = st->d_mint;
             const float* F = st->d_mflt;
@@ -1201,6 +1211,100 @@ int step_impl(AcousticAMRState* st, int n, float* ms_total, float* ms_faces, flo
 }
 
 }  // namespace
+
+__global__ void k_amr_hook_init(int* status) { status[3] = INT_MAX; }
+
+// Device step count after a hooked call: the steps before a failed hook, or
+// before a diverged step, or all n.
+__global__ void k_amr_step_count_hooked(int* status, int n) {
+    status[2] += status[3] != INT_MAX ? status[3] : (status[0] < n ? status[0] : n);
+}
+
+int amr_step_hooked(AcousticAMRState* st, int n, const AmrStepHook& hook, float* ms_total,
+                    int* done, int* hook_failed) {
+    *done = 0;
+    *hook_failed = 0;
+    if (ms_total) *ms_total = 0.0f;
+    if (!st) return SK_ERR_NULL_STATE;
+    if (n < 0) return SK_ERR_DIM_MISMATCH;
+    const float dt = (float)st->dt;
+    const float bulk = (float)(st->rho * st->c * st->c * st->dt);
+    const float dt_over_rho = dt / (float)st->rho;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (ms_total && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+        return AMR_ERR_DEVICE;
+    auto args = [&](int q) {
+        AmrHookArgs a;
+        a.q = q;
+        a.w = st->plate_on ? st->d_pw[(st->prot + q + 1) % 3] : nullptr;
+        a.ext = st->d_ext;
+        a.p = st->d_p;
+        a.v = st->d_v;
+        return a;
+    };
+    int rc = SK_OK;
+    k_amr_status_init<<<1, 1, 0, st->stream>>>(st->d_status, st->d_best);
+    k_amr_hook_init<<<1, 1, 0, st->stream>>>(st->d_status);
+    if (e0) (void)hipEventRecord(e0, st->stream);
+    for (int s = 0; s < n && rc == SK_OK; ++s) {
+        rc = hook(args(s));
+        if (rc == SK_OK) enqueue_amr_step(st, s, dt, bulk, dt_over_rho, nullptr);
+    }
+    if (rc == SK_OK) rc = hook(args(n));
+    k_amr_step_count_hooked<<<1, 1, 0, st->stream>>>(st->d_status, n);
+    if (e1) (void)hipEventRecord(e1, st->stream);
+    int status[4] = {INT_MAX, INT_MAX, 0, INT_MAX};
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(status, st->d_status, sizeof(status), hipMemcpyDeviceToHost, st->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(st->stream);
+    if (e == hipSuccess && ms_total) (void)hipEventElapsedTime(ms_total, e0, e1);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (e != hipSuccess || rc != SK_OK) return AMR_ERR_DEVICE;
+    const bool rot = st->plate_on && st->n_active > 0;
+    if (status[3] != INT_MAX) {                      // the hook failed before step status[3]
+        *done = status[3];
+        *hook_failed = 1;
+        st->step_count += status[3];
+        if (rot) st->prot = (st->prot + status[3]) % 3;
+        return SK_ERR_DIVERGED;
+    }
+    if (status[0] < n) {                             // step status[0] diverged
+        *done = status[0];
+        st->step_count += status[0];
+        if (rot) st->prot = (st->prot + status[0] + 1) % 3;
+        report_diverged(st);
+        return SK_ERR_DIVERGED;
+    }
+    *done = n;
+    st->step_count += n;
+    if (rot) st->prot = (st->prot + n) % 3;
+    return SK_OK;
+}
+
+int amr_co_view(AcousticAMRState* st, AmrCoView* v) {
+    if (!st || !v) return SK_ERR_NULL_STATE;
+    v->stream = st->stream;
+    v->status = st->d_status;
+    v->n_cells = st->nc;
+    v->n_pnodes = st->n_pnodes;
+    v->pNx = st->pNx;
+    v->pNy = st->pNy;
+    v->n_mics = st->n_mics;
+    const int* I = st->d_mint;
+    const float* F = st->d_mflt;
+    const bool m = st->n_mics > 0 && I && F;
+    v->mp_start = m ? I : nullptr;
+    v->mp_cell = m ? I + st->o_mpc : nullptr;
+    v->mp_w = m ? F : nullptr;
+    v->mv_start = m ? I + st->o_mvs : nullptr;
+    v->mv_face = m ? I + st->o_mvf : nullptr;
+    v->mv_w[0] = m ? F + st->o_mwx : nullptr;
+    v->mv_w[1] = m ? F + st->o_mwy : nullptr;
+    v->mv_w[2] = m ? F + st->o_mwz : nullptr;
+    return SK_OK;
+}
 
 extern "C" {
 

@@ -8,10 +8,12 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <cstddef>
 #include <new>
 #include <string>
 
 #include "../../include/afdtd_pressure_backend.hpp"
+#include "afdtd_internal.h"
 
 // ── UniformPressureBackend (acoustic_pressure_backend.cpp:64-258) ────────
 
@@ -167,8 +169,8 @@ const char* code_name(AmrPressureBackendCreateError c) {
     return "UNKNOWN";
 }
 
-IPressureBackend* fail(AmrPressureBackendCreateError c, const std::string& m,
-                       AcousticAMRState* amr = nullptr) {
+std::nullptr_t fail(AmrPressureBackendCreateError c, const std::string& m,
+                    AcousticAMRState* amr = nullptr) {
     g_code = c;
     g_msg = m;
     amr_destroy(amr);
@@ -236,7 +238,7 @@ private:
 const char* amr_pressure_backend_last_error_code(void) { return code_name(g_code); }
 const char* amr_pressure_backend_last_error_message(void) { return g_msg.c_str(); }
 
-IPressureBackend* amr_pressure_backend_create(const AMRCoevolverDescriptor* d) {
+AcousticAMRState* amr_state_from_descriptor(const AMRCoevolverDescriptor* d) {
     g_code = AMR_BACKEND_CREATE_OK;
     g_msg.clear();
     if (!d) return fail(AMR_BACKEND_DESC_NULL, "AMRCoevolverDescriptor is null");
@@ -301,6 +303,12 @@ IPressureBackend* amr_pressure_backend_create(const AMRCoevolverDescriptor* d) {
                                  d->border_R_reflection, d->border_Z_match, d->n_pml,
                                  d->bounds_min, d->bounds_max) != SK_OK)
         return fail(AMR_BACKEND_BORDER_SETUP_FAILED, "amr_set_border_condition failed", amr);
+    return amr;
+}
+
+IPressureBackend* amr_pressure_backend_create(const AMRCoevolverDescriptor* d) {
+    AcousticAMRState* amr = amr_state_from_descriptor(d);
+    if (!amr) return nullptr;
     auto* b = new (std::nothrow) AMRPressureBackend(amr, d->n_cells, d->plate_Nx, d->plate_Ny);
     if (!b) return fail(AMR_BACKEND_NEW_FAILED, "AMRPressureBackend allocation failed", amr);
     return b;

@@ -30,15 +30,20 @@
 #include <algorithm>
 #include <atomic>
 #include <cfloat>
+#include <climits>
 #include <functional>
 #include <string>
 #include <thread>
 #include <vector>
 
 #include "../../include/afdtd_coevolver.h"
+#include "../../include/afdtd_pressure_backend.hpp"
 #include "afdtd_internal.h"
 
 namespace {
+
+// stage label of the last coevolver_create_amr failure (acoustic_coevolver.cpp:31-34)
+thread_local std::string g_amr_err;
 
 constexpr int kModes = 32;          // acoustic_coevolver.cpp:40
 constexpr int kMaxSub = 16;         // :41
@@ -65,6 +70,9 @@ enum { kTeff = 0, kSad = 1, kSadP = 2, kBr = 3, kBrP = 4, kSw = 8 };
 struct CoConst {
     int ns, npk, nmic, nmag, tot_seg, tot_nodes, nb, nbgrp, nn, nngrp, neck_on, any_vel;
     int Nplate, clear_ext;
+    int amr;                                // body: 0 uniform FDTD, 1 AMR stepper
+    int st_ok;                              // status[0] when nothing failed (0 / INT_MAX)
+    int lds_str;                            // LDS words before the neck products
     float dt_b, inv_dx2, rhoc;
     float n_m, n_c, n_k, nb_k, nb_c;      // neck mass, total damping / stiffness, joint terms
     const CoStr* str;
@@ -97,7 +105,15 @@ struct CoConst {
     const int* blk_node;                    // [samples][ns] its node
     const float* blk_dval;                  // [samples][ns] direct drive (block-drive form)
     const int* blk_dnode;                   // [ns]
-    int* status;                            // body FDTD status word
+    int* status;                            // body status words
+    // AMR body: bridge plate weights and the mic stencils (CSR over mics)
+    const float* bwgt;
+    const int* amp_start;
+    const int* amp_cell;
+    const float* amp_w;
+    const int* amv_start;
+    const int* amv_face;
+    const float* amv_w[3];
 };
 
 struct CoLaunch {
@@ -117,9 +133,17 @@ __device__ __forceinline__ void wave_sync() {
 
 __device__ __forceinline__ bool wave_any(bool b) { return __ballot(b) != 0ull; }
 
+// A failure found by this kernel before body step a.step: the uniform
+// body's status becomes {FDTD_STATUS_HOOK, step}; the AMR body's hook word
+// (status[3]) takes the step and its failing-step word drops below it, so
+// that the step and everything after it is skipped.
 __device__ __forceinline__ void fail_here(const CoConst& c, const CoLaunch& a, int* sflag) {
     *sflag = 1;
-    if (atomicCAS(&c.status[0], 0, FDTD_STATUS_HOOK) == 0) c.status[1] = a.step;
+    if (c.amr) {
+        if (atomicCAS(&c.status[3], INT_MAX, a.step) == INT_MAX) atomicMin(&c.status[0], a.step - 1);
+    } else if (atomicCAS(&c.status[0], 0, FDTD_STATUS_HOOK) == 0) {
+        c.status[1] = a.step;
+    }
 }
 
 // acc + x[0] + x[1] + ... + x[n-1], one lane, in index order; the loads run
@@ -198,10 +222,10 @@ __device__ bool tension_of(const float* u0, const float* u1, float* scr, int N, 
 // per-string T_eff and u0[1] [2 * ns], the current sub-step's T_eff [ns].  The post phase reuses the string
 // words as scratch (modal staging 2T, pickup velocities and products 4S).
 __global__ __launch_bounds__(kThreads) void k_coevo(CoConst c, CoLaunch a) {
-    if (c.status[0]) return;
+    if (c.status[0] != c.st_ok) return;
     extern __shared__ float lds[];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    float* nprod = lds + 8 * c.tot_nodes;
+    float* nprod = lds + c.lds_str;
     float* fpl = nprod + c.nn;
     float* drive = fpl + kMaxStrings;
     int* flag = (int*)(drive + c.nb);          // [0] failure, [1..2] neck_w / neck force, [3..4] sums
@@ -215,8 +239,9 @@ __global__ __launch_bounds__(kThreads) void k_coevo(CoConst c, CoLaunch a) {
         if (tid < c.ns) {
             const int* id = c.sad_idx + 4 * tid;
             const float* g = c.sad_wgt + 4 * tid;
-            c.sw[tid * kSw + kSad] = g[0] * a.w[id[0]] + g[1] * a.w[id[1]] + g[2] * a.w[id[2]] +
-                                     g[3] * a.w[id[3]];
+            // the AMR sampler's sum starts from +0 (acoustic_amr.cpp:1600-1616)
+            const float t0 = c.amr ? 0.0f + g[0] * a.w[id[0]] : g[0] * a.w[id[0]];
+            c.sw[tid * kSw + kSad] = t0 + g[1] * a.w[id[1]] + g[2] * a.w[id[2]] + g[3] * a.w[id[3]];
         }
         if (a.modal) {
             for (int i = tid; i < c.tot_nodes; i += kThreads) {
@@ -269,7 +294,8 @@ __global__ __launch_bounds__(kThreads) void k_coevo(CoConst c, CoLaunch a) {
                     __syncthreads();
                 }
             }
-            for (int t = tid; t < c.npk + c.nmic; t += kThreads) {
+            const int nout = c.amr ? c.npk : c.npk + c.nmic;
+            for (int t = tid; t < nout; t += kThreads) {
                 if (t < c.npk) {
                     const int p = t;
                     if (c.pk_type[p] == PICKUP_PIEZO) {         // :1892-1904, :2174-2181
@@ -310,6 +336,62 @@ __global__ __launch_bounds__(kThreads) void k_coevo(CoConst c, CoLaunch a) {
                     float sig = P * mp[0];
                     if (mp[1] != 0.0f) {
                         const float vn = V[0] * mp[3] + V[1] * mp[4] + V[2] * mp[5];
+                        sig += mp[1] * c.rhoc * vn;
+                    }
+                    c.mic_out[(size_t)m * kRing + a.out_pos] = sig * mp[2];
+                }
+            }
+            if (c.amr && c.nmic > 0) {
+                // AMR mic stencils (acoustic_amr.cpp:1762-1792): the products of
+                // a chunk of entries (all mics' entries are contiguous) are formed
+                // in parallel, then each mic's running sums take its entries of
+                // the chunk in stencil order, from +0
+                float* pp = lds;                           // [kThreads]
+                float* pv = lds + kThreads;                // [3][kThreads]
+                float* acc = lds + 4 * kThreads;           // [4][nmic]: p, vx, vy, vz
+                __syncthreads();
+                for (int t = tid; t < 4 * c.nmic; t += kThreads) acc[t] = 0.0f;
+                const int p0 = c.amp_start[0], p1 = c.amp_start[c.nmic];
+                for (int base = p0; base < p1; base += kThreads) {
+                    const int k = base + tid;
+                    if (k < p1) pp[tid] = c.amp_w[k] * a.P[c.amp_cell[k]];
+                    __syncthreads();
+                    for (int m = tid; m < c.nmic; m += kThreads) {
+                        const int lo = max(c.amp_start[m], base);
+                        const int hi = min(c.amp_start[m + 1], base + kThreads);
+                        float r = acc[m];
+                        for (int j = lo; j < hi; ++j) r += pp[j - base];
+                        acc[m] = r;
+                    }
+                    __syncthreads();
+                }
+                const int v0 = c.amv_start[0], v1 = c.amv_start[c.nmic];
+                for (int base = v0; base < v1; base += kThreads) {
+                    const int k = base + tid;
+                    if (k < v1) {
+                        const float vf = a.V[0][c.amv_face[k]];
+                        pv[tid] = c.amv_w[0][k] * vf;
+                        pv[kThreads + tid] = c.amv_w[1][k] * vf;
+                        pv[2 * kThreads + tid] = c.amv_w[2][k] * vf;
+                    }
+                    __syncthreads();
+                    for (int t = tid; t < 3 * c.nmic; t += kThreads) {
+                        const int ax = t / c.nmic, m = t - ax * c.nmic;
+                        const int lo = max(c.amv_start[m], base);
+                        const int hi = min(c.amv_start[m + 1], base + kThreads);
+                        float r = acc[(1 + ax) * c.nmic + m];
+                        for (int j = lo; j < hi; ++j) r += pv[ax * kThreads + j - base];
+                        acc[(1 + ax) * c.nmic + m] = r;
+                    }
+                    __syncthreads();
+                }
+                __syncthreads();
+                for (int m = tid; m < c.nmic; m += kThreads) {     // :2208-2219
+                    const float* mp = c.mic_par + 6 * m;
+                    float sig = acc[m] * mp[0];
+                    if (mp[1] != 0.0f) {
+                        const float vn = acc[c.nmic + m] * mp[3] + acc[2 * c.nmic + m] * mp[4] +
+                                         acc[3 * c.nmic + m] * mp[5];
                         sig += mp[1] * c.rhoc * vn;
                     }
                     c.mic_out[(size_t)m * kRing + a.out_pos] = sig * mp[2];
@@ -518,14 +600,19 @@ __global__ __launch_bounds__(kThreads) void k_coevo(CoConst c, CoLaunch a) {
             drive[cix] = acc;
         }
         __syncthreads();
-        // fdtd_inject_bridge_drive(drive, 1): ext[p] += 1 * d * inv_dx2 per entry, in order
+        // uniform fdtd_inject_bridge_drive(drive, 1): ext[p] += 1 * d * inv_dx2 per
+        // entry; AMR amr_inject_bridge_drive: ext[p] += d * w * 1; in order
         for (int g = tid; g < c.nbgrp; g += kThreads) {
             const int* start = c.bgrp;
             const int* node = c.bgrp + c.nbgrp + 1;
             const int* ent = c.bgrp + 2 * c.nbgrp + 1;
             const int p = node[g];
             float acc = a.ext[p];
-            for (int q = start[g]; q < start[g + 1]; ++q) acc += 1.0f * drive[ent[q]] * c.inv_dx2;
+            if (c.amr)
+                for (int q = start[g]; q < start[g + 1]; ++q)
+                    acc += drive[ent[q]] * c.bwgt[ent[q]] * 1.0f;
+            else
+                for (int q = start[g]; q < start[g + 1]; ++q) acc += 1.0f * drive[ent[q]] * c.inv_dx2;
             a.ext[p] = acc;
         }
         __syncthreads();
@@ -540,8 +627,12 @@ __global__ __launch_bounds__(kThreads) void k_coevo(CoConst c, CoLaunch a) {
                 const int* ent = c.ngrp + 2 * c.nngrp + 1;
                 const int p = node[g];
                 float acc = a.ext[p];
-                for (int q = start[g]; q < start[g + 1]; ++q)
-                    acc += total * c.nwgt[ent[q]] * c.inv_dx2;
+                // uniform: total * w / dx^2; AMR amr_inject_plate_force: w * total
+                if (c.amr)
+                    for (int q = start[g]; q < start[g + 1]; ++q) acc += total * c.nwgt[ent[q]];
+                else
+                    for (int q = start[g]; q < start[g + 1]; ++q)
+                        acc += total * c.nwgt[ent[q]] * c.inv_dx2;
                 a.ext[p] = acc;
             }
         }
@@ -612,7 +703,8 @@ int group_entries(const int* idx, int n, int div, const uint8_t* mask, int nmax,
 }  // namespace
 
 struct AcousticCoEvolverState {
-    AcousticFDTDState* fdtd = nullptr;
+    AcousticFDTDState* fdtd = nullptr;     // uniform body, or
+    AcousticAMRState* amr = nullptr;       // AMR body (coevolver_create_amr)
     int ns = 0, npk = 0, nmic = 0, Nx = 0, Ny = 0, Nz = 0, plate_iz = 0, K = 1;
     float dx = 0, origin[3] = {0, 0, 0}, rho_air = 0, c = 0, sr = 0, dt_audio = 0, dt_fdtd = 0;
     float gamma_scale = 1.0f, gamma_dev = -1.0f;
@@ -636,6 +728,8 @@ struct AcousticCoEvolverState {
     float *d_sw = nullptr, *d_u = nullptr, *d_neck = nullptr, *d_nwgt = nullptr;
     int *d_nidx = nullptr, *d_ngrp = nullptr;
     float* d_sbk = nullptr;
+    float* d_bwgt = nullptr;               // AMR body: bridge plate weights
+    int* d_bgrp = nullptr;                 // AMR body: bridge force groups
     int* d_sad_idx = nullptr;
     float* d_sad_wgt = nullptr;
     float *d_basis = nullptr, *d_modal = nullptr, *d_K0 = nullptr, *d_K1 = nullptr;
@@ -665,11 +759,13 @@ void co_free(AcousticCoEvolverState* st) {
                     st->d_modal, st->d_K0, st->d_K1, st->d_magmap, st->d_pk_type,
                     st->d_pk_mask, st->d_pk_sens, st->d_mPi, st->d_mVi, st->d_mPw, st->d_mVw,
                     st->d_mic_par, st->d_pk_out, st->d_mic_out, st->d_blk_sub, st->d_blk_dval,
-                    st->d_blk_node, st->d_blk_dnode};
+                    st->d_blk_node, st->d_blk_dnode, st->d_bwgt, st->d_bgrp};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (st->fdtd) fdtd_destroy(st->fdtd);
     st->fdtd = nullptr;
+    if (st->amr) amr_destroy(st->amr);
+    st->amr = nullptr;
 }
 
 // Frame (parallel transport from a world-Y or world-Z bootstrap) —
@@ -1048,6 +1144,12 @@ bool co_pointers_set(const CoConst& k) {
     return k.nb == 0 || k.bgrp != nullptr;
 }
 
+hipStream_t co_stream(const AcousticCoEvolverState* st) {
+    if (st->fdtd) return st->fdtd->stream;
+    AmrCoView v;
+    return (st->amr && amr_co_view(st->amr, &v) == SK_OK) ? v.stream : nullptr;
+}
+
 int ensure_blk(AcousticCoEvolverState* st, int n) {
     const size_t need = (size_t)n * std::max(st->ns, 1);
     if ((int)need <= st->blk_cap) return CE_OK;
@@ -1074,18 +1176,19 @@ int run_block(AcousticCoEvolverState* st, const BlockPlan& bp, int n, bool drive
     int rc = ensure_blk(st, n);
     if (rc) return rc;
     AcousticFDTDState* f = st->fdtd;
+    const hipStream_t sm = co_stream(st);
     const size_t cnt = (size_t)n * st->ns;
     if (cnt) {
         if (hipMemcpyAsync(st->d_blk_sub, bp.sub.data(), cnt * sizeof(float),
-                           hipMemcpyHostToDevice, f->stream) != hipSuccess ||
+                           hipMemcpyHostToDevice, sm) != hipSuccess ||
             hipMemcpyAsync(st->d_blk_node, bp.node.data(), cnt * sizeof(int),
-                           hipMemcpyHostToDevice, f->stream) != hipSuccess)
+                           hipMemcpyHostToDevice, sm) != hipSuccess)
             return CE_ERR_FDTD_STEP;
         if (drive &&
             (hipMemcpyAsync(st->d_blk_dval, bp.dval.data(), cnt * sizeof(float),
-                            hipMemcpyHostToDevice, f->stream) != hipSuccess ||
+                            hipMemcpyHostToDevice, sm) != hipSuccess ||
              hipMemcpyAsync(st->d_blk_dnode, bp.dnode.data(), st->ns * sizeof(int),
-                            hipMemcpyHostToDevice, f->stream) != hipSuccess))
+                            hipMemcpyHostToDevice, sm) != hipSuccess))
             return CE_ERR_FDTD_STEP;
     }
     // damping scale of this block (:1924)
@@ -1117,39 +1220,61 @@ int run_block(AcousticCoEvolverState* st, const BlockPlan& bp, int n, bool drive
     }
     const int out0 = st->out_pos;
     const size_t lds = st->lds_bytes;
-    StepHook hook = [&](int q, int s) -> int {
+    auto launch = [&](int q, int step, const float* w, float* ext, const float* P,
+                      const float* const* V) -> bool {
         CoLaunch a;
         a.sample = q / K;
         a.sub = q % K;
-        a.step = s;
+        a.step = step;
         a.post = q > 0;
         a.modal = q > 0 ? modal[q - 1] : 0;
         a.sample_end = q > 0 && ((q - 1) % K) == K - 1;
         a.out_pos = q > 0 ? (out0 + (q - 1) / K) % kRing : 0;
         a.run = q < nsteps;
-        a.w = f->d_w[0];
-        a.ext = f->d_ext;
-        a.P = (const float*)f->P[f->cur];
-        for (int ax = 0; ax < 3; ++ax) a.V[ax] = (const float*)f->V[ax];
-        if (!a.post && !a.run) return FDTD_OK;
-        hipLaunchKernelGGL(k_coevo, dim3(1), dim3(kThreads), lds, f->stream, kc, a);
-        AFDTD_CHECK(hipGetLastError());
-        return FDTD_OK;
+        a.w = w;
+        a.ext = ext;
+        a.P = P;
+        for (int ax = 0; ax < 3; ++ax) a.V[ax] = V[ax];
+        if (!a.post && !a.run) return true;
+        hipLaunchKernelGGL(k_coevo, dim3(1), dim3(kThreads), lds, sm, kc, a);
+        return hipGetLastError() == hipSuccess;
     };
-    int status[2] = {0, 0};
-    const int start_step = f->step_count;
-    int frc = afdtd_step_hooked(f, nsteps, hook, ms_total, status);
     int done_steps = nsteps;
-    if (frc == FDTD_ERR_UNSTABLE) done_steps = status[1] - start_step;
-    else if (frc != FDTD_OK) return CE_ERR_FDTD_STEP;
+    bool ok = true;
+    if (st->amr) {
+        // AMR body: the hook sees relative step indices; a failed hook or a
+        // diverged step ends the call with the steps before it completed
+        AmrStepHook hook = [&](const AmrHookArgs& h) -> int {
+            const float* V[3] = {h.v, nullptr, nullptr};
+            return launch(h.q, h.q, h.w, h.ext, h.p, V) ? SK_OK : AMR_ERR_DEVICE;
+        };
+        int hook_failed = 0;
+        const int arc = amr_step_hooked(st->amr, nsteps, hook, ms_total, &done_steps, &hook_failed);
+        if (arc != SK_OK && arc != SK_ERR_DIVERGED) return CE_ERR_FDTD_STEP;
+        ok = arc == SK_OK;
+    } else {
+        StepHook hook = [&](int q, int s) -> int {
+            const float* V[3] = {(const float*)f->V[0], (const float*)f->V[1],
+                                 (const float*)f->V[2]};
+            if (!launch(q, s, f->d_w[0], f->d_ext, (const float*)f->P[f->cur], V))
+                return FDTD_ERR_NULL;
+            return FDTD_OK;
+        };
+        int status[2] = {0, 0};
+        const int start_step = f->step_count;
+        const int frc = afdtd_step_hooked(f, nsteps, hook, ms_total, status);
+        if (frc == FDTD_ERR_UNSTABLE) done_steps = status[1] - start_step;
+        else if (frc != FDTD_OK) return CE_ERR_FDTD_STEP;
+        ok = frc == FDTD_OK;
+    }
     // sub-steps that finished (saddle gathered, modal counter advanced)
     for (int q = 0; q < done_steps; ++q)
         if (++st->modal_counter >= st->modal_stride) st->modal_counter = 0;
-    const int samples = frc == FDTD_OK ? n : done_steps / K;
+    const int samples = ok ? n : done_steps / K;
     st->out_pos = (out0 + samples) % kRing;
     st->written += samples;
     *done = samples;
-    return frc == FDTD_OK ? CE_OK : CE_ERR_UNSTABLE;
+    return ok ? CE_OK : CE_ERR_UNSTABLE;
 }
 
 // n samples in chunks; mode as plan_block.  mic0 (block-drive form) gets
@@ -1242,6 +1367,311 @@ int read_strings(const AcousticCoEvolverState* st, std::vector<float>& u) {
 
 }  // namespace
 
+namespace {
+
+// Set-up shared by both bodies (acoustic_coevolver.cpp:1063-1381 and
+// :1480-1745): strings, neck constants, pickups and their GEMV rows, mics,
+// saddle taps, per-string bridge kernels, modal basis, force groups, device
+// state and the kernel's constants.  The body, the grid fields (Nx, Ny, Nz,
+// dx, origin; the plate grid with Nz = 1 on an AMR body), dt_fdtd, K and the
+// bridge (and, on an AMR body, neck) plate lists are in place.  Returns
+// nullptr, or the failing stage.
+const char* co_setup(AcousticCoEvolverState* st, int n_strings, const CoEvolverStringDef* sd,
+                     int n_pickups, const CoEvolverPickupDef* pd, int n_mics,
+                     const CoEvolverMicDef* md, const CoEvolverBodyDef* body, float bridge_sigma) {
+    AcousticFDTDState* f = st->fdtd;
+    const int nplate = f ? f->Nplate : st->Nx * st->Ny;
+    st->ns = n_strings;
+    st->s.resize(n_strings);
+    for (int i = 0; i < n_strings; ++i)
+        if (!string_init(st->s[i], sd[i], st->dt_fdtd)) return "string_create";
+    for (int i = 0; i < n_strings; ++i) {                                  // :1079-1101
+        if (sd[i].neck_freq_hz > 0.0f && sd[i].neck_mass_kg > 0.0f) {
+            st->n_m = sd[i].neck_mass_kg;
+            const float om = 2.0f * kPi * sd[i].neck_freq_hz;
+            st->n_k = st->n_m * om * om;
+            const float Q = (sd[i].neck_Q > 0.0f) ? sd[i].neck_Q : 30.0f;
+            st->n_c = st->n_m * om / Q;
+            const float oj = 2.0f * kPi * 95.0f;
+            st->nb_k = st->n_m * oj * oj;
+            st->nb_c = st->n_m * oj / 2.0f;
+            break;
+        }
+    }
+    if (st->amr) {
+        if (st->n_m > 0.0f && st->nn == 0) st->n_m = 0.0f;          // :1510-1512
+    } else if (st->n_m > 0.0f && neck_cells(st, body->plate_active, 0.030f) != CE_OK) {
+        return "neck_cells";
+    }
+    // node / segment offsets
+    int no = 0, so = 0;
+    for (auto& s : st->s) {
+        s.node_off = no;
+        s.seg_off = so;
+        no += s.n_nodes;
+        so += s.n_segs;
+    }
+    st->tot_nodes = no;
+    st->tot_seg = so;
+    if (no > kMaxNodes) {
+        afdtd_set_error("coevolver_create: more than 4096 string nodes in total");
+        return "string_nodes";
+    }
+    // pickups (:1107-1132, GEMV rows :1248-1306)
+    st->npk = n_pickups;
+    std::vector<float> K0, K1;
+    std::vector<int> magmap;
+    for (int p = 0; p < n_pickups; ++p) {
+        st->pk_type.push_back(pd[p].type);
+        st->pk_mask.push_back(pd[p].string_mask);
+        st->pk_sens.push_back(pd[p].sensitivity);
+    }
+    for (int p = 0; p < n_pickups; ++p)
+        if (pd[p].type != PICKUP_PIEZO) ++st->nmag;
+    if (st->nmag > 0 && st->tot_seg > 0) {
+        K0.assign((size_t)st->nmag * st->tot_seg, 0.0f);
+        K1.assign((size_t)st->nmag * st->tot_seg, 0.0f);
+        int r = 0;
+        for (int p = 0; p < n_pickups; ++p) {
+            if (pd[p].type == PICKUP_PIEZO) continue;
+            magmap.push_back(p);
+            for (int i = 0; i < n_strings; ++i) {
+                if (!(pd[p].string_mask & (1u << (unsigned)i))) continue;
+                std::vector<float> Bn, Bb;
+                bkernels(pd[p], st->s[i], Bn, Bb);
+                for (int q = 0; q < st->s[i].n_segs; ++q) {
+                    K0[(size_t)r * st->tot_seg + st->s[i].seg_off + q] = Bn[q] * st->s[i].ds;
+                    K1[(size_t)r * st->tot_seg + st->s[i].seg_off + q] = Bb[q] * st->s[i].ds;
+                }
+            }
+            ++r;
+        }
+    } else {
+        st->nmag = std::min(st->nmag, 0);
+    }
+    // mics (:1135-1246)
+    st->nmic = n_mics;
+    std::vector<float> mic_par((size_t)6 * n_mics);
+    st->mic_g.resize((size_t)3 * n_mics);
+    for (int m = 0; m < n_mics; ++m) {
+        for (int q = 0; q < 3; ++q)
+            st->mic_g[3 * m + q] = (md[m].pos[q] - st->origin[q]) / st->dx;
+        mic_par[6 * m + 0] = md[m].polar_a;
+        mic_par[6 * m + 1] = md[m].polar_b;
+        mic_par[6 * m + 2] = md[m].gain;
+        for (int q = 0; q < 3; ++q) mic_par[6 * m + 3 + q] = md[m].axis[q];
+        if (md[m].polar_b != 0.0f) st->any_vel = 1;
+    }
+    std::vector<int> mPi((size_t)8 * n_mics), mVi;
+    std::vector<float> mPw((size_t)8 * n_mics), mVw;
+    if (n_mics > 0 && st->amr) {                                          // :1613-1646
+        std::vector<float> pos((size_t)3 * n_mics), axis((size_t)3 * n_mics);
+        for (int m = 0; m < n_mics; ++m)
+            for (int q = 0; q < 3; ++q) {
+                pos[3 * m + q] = md[m].pos[q];
+                axis[3 * m + q] = md[m].axis[q];
+            }
+        if (amr_setup_mic_samplers(st->amr, n_mics, pos.data(), axis.data(), st->any_vel) != SK_OK)
+            return "setup_mic_samplers";
+    } else if (n_mics > 0) {
+        if (fdtd_precompute_pressure_samplers(f, n_mics, st->mic_g.data(), mPi.data(),
+                                              mPw.data()) != FDTD_OK)
+            return "setup_mic_samplers";
+        mVi.assign((size_t)24 * n_mics, 0);
+        mVw.assign((size_t)24 * n_mics, 0.0f);
+        if (st->any_vel) {
+            const size_t o = (size_t)8 * n_mics;
+            if (fdtd_precompute_velocity_samplers(f, n_mics, st->mic_g.data(), mVi.data(),
+                                                  mVw.data(), mVi.data() + o, mVw.data() + o,
+                                                  mVi.data() + 2 * o,
+                                                  mVw.data() + 2 * o) != FDTD_OK)
+                return "setup_mic_samplers";
+            for (int ax = 0; ax < 3; ++ax)
+                for (size_t q = 0; q < o; ++q)
+                    mVi[ax * o + q] = packed_to_padded_v(ax, mVi[ax * o + q], st->Nx, st->Ny,
+                                                         st->Nz);
+        }
+    }
+    // saddle bilinear taps on the plate (:1160-1183)
+    std::vector<int> sad_idx((size_t)4 * n_strings);
+    std::vector<float> sad_wgt((size_t)4 * n_strings);
+    for (int i = 0; i < n_strings; ++i) {
+        const HString& s = st->s[i];
+        const int Nn = s.n_nodes;
+        float gi = (s.xyz[(Nn - 1) * 3 + 0] - st->origin[0]) / st->dx;
+        float gj = (s.xyz[(Nn - 1) * 3 + 1] - st->origin[1]) / st->dx;
+        if (gi < 0.0f) gi = 0.0f;
+        if (gi > (float)(st->Nx - 1)) gi = (float)(st->Nx - 1);
+        if (gj < 0.0f) gj = 0.0f;
+        if (gj > (float)(st->Ny - 1)) gj = (float)(st->Ny - 1);
+        const int i0 = (int)gi, i1 = (i0 < st->Nx - 1) ? i0 + 1 : i0;
+        const int j0 = (int)gj, j1 = (j0 < st->Ny - 1) ? j0 + 1 : j0;
+        const float fi = gi - (float)i0, fj = gj - (float)j0;
+        sad_idx[i * 4 + 0] = j0 + st->Ny * i0;
+        sad_idx[i * 4 + 1] = j0 + st->Ny * i1;
+        sad_idx[i * 4 + 2] = j1 + st->Ny * i0;
+        sad_idx[i * 4 + 3] = j1 + st->Ny * i1;
+        sad_wgt[i * 4 + 0] = (1.0f - fi) * (1.0f - fj);
+        sad_wgt[i * 4 + 1] = fi * (1.0f - fj);
+        sad_wgt[i * 4 + 2] = (1.0f - fi) * fj;
+        sad_wgt[i * 4 + 3] = fi * fj;
+    }
+    // per-string bridge kernels (:1322-1375)
+    std::vector<float> sbk;
+    if (n_strings > 0 && st->nb > 0) {
+        const int nb = st->nb;
+        sbk.assign((size_t)n_strings * nb, 0.0f);
+        const float sigma = bridge_sigma;
+        const float i2s2 = 0.5f / (sigma * sigma);
+        for (int i = 0; i < n_strings; ++i) {
+            const HString& s = st->s[i];
+            const int Nn = s.n_nodes;
+            const float sx = s.xyz[(Nn - 1) * 3 + 0], sy = s.xyz[(Nn - 1) * 3 + 1];
+            float* row = &sbk[(size_t)i * nb];
+            float ws = 0.0f;
+            for (int cix = 0; cix < nb; ++cix) {
+                const int flat = st->bidx[cix];
+                const int t = flat / st->Nz;
+                const int j = t % st->Ny, ii = t / st->Ny;
+                const float cx = (float)ii * st->dx + st->origin[0];
+                const float cy = (float)j * st->dx + st->origin[1];
+                const float d2x = sx - cx, d2y = sy - cy;
+                row[cix] = expf(-(d2x * d2x + d2y * d2y) * i2s2);
+                ws += row[cix];
+            }
+            if (ws > 1e-12f)
+                for (int cix = 0; cix < nb; ++cix) row[cix] /= ws;
+        }
+    }
+    // modal basis, node-major for coalesced mode reads (:494-505)
+    std::vector<float> basis((size_t)st->tot_nodes * kModes);
+    for (const auto& s : st->s) {
+        const int Nn = s.n_nodes;
+        const float inv = 1.0f / (float)(Nn > 1 ? Nn - 1 : 1);
+        for (int k = 0; k < kModes; ++k) {
+            const float kpi = (float)(k + 1) * kPi;
+            for (int i = 0; i < Nn; ++i)
+                basis[(size_t)(s.node_off + i) * kModes + k] = sinf(kpi * (float)i * inv);
+        }
+    }
+    // neck force groups over the carved plate mask the body FDTD holds
+    std::vector<int> ngrp;
+    int nngrp = 0;
+    if (st->nn > 0)
+        nngrp = group_entries(st->nidx.data(), st->nn, 1, st->amr ? nullptr : f->h_pa.data(),
+                              nplate, ngrp);
+    // AMR body: the bridge plate entries grouped by node (the uniform body's
+    // groups live in its FDTD state)
+    std::vector<int> bgrp;
+    int nbgrp = f ? f->nbgrp : 0;
+    if (st->amr && st->nb > 0)
+        nbgrp = group_entries(st->bidx.data(), st->nb, 1, nullptr, nplate, bgrp);
+    // device state
+    std::vector<CoStr> hs(n_strings);
+    std::vector<float> sw((size_t)kSw * std::max(n_strings, 1), 0.0f);
+    for (int i = 0; i < n_strings; ++i) {
+        const HString& s = st->s[i];
+        CoStr& d = hs[i];
+        d.n_nodes = s.n_nodes; d.node_off = s.node_off; d.seg_off = s.seg_off; d.n_sub = s.n_sub;
+        d.ds = s.ds; d.L = s.L; d.T0 = s.T0; d.EA = s.EA; d.mu = s.mu; d.dt_s = s.dt_s;
+        d.biharm = s.biharm; d.gamma_eff = s.gamma;
+        d.b_m = s.b_m; d.b_k = s.b_k; d.b_c = s.b_c; d.inv_dt_s = 1.0f / s.dt_s;
+        sw[(size_t)i * kSw + kTeff] = s.T0;
+    }
+    st->gamma_dev = 1.0f;
+    std::vector<float> zeros_u((size_t)4 * std::max(st->tot_nodes, 1), 0.0f);
+    std::vector<float> neck(4, 0.0f);
+#define UP(p, v) if (dupload(&st->p, v) != hipSuccess) return "device_alloc"
+    UP(d_str, hs);
+    UP(d_sw, sw);
+    UP(d_u, zeros_u);
+    UP(d_neck, neck);
+    UP(d_nidx, st->nidx);
+    UP(d_nwgt, st->nwgt);
+    UP(d_ngrp, ngrp);
+    UP(d_sbk, sbk);
+    UP(d_sad_idx, sad_idx);
+    UP(d_sad_wgt, sad_wgt);
+    UP(d_basis, basis);
+    UP(d_K0, K0);
+    UP(d_K1, K1);
+    UP(d_magmap, magmap);
+    UP(d_pk_type, st->pk_type);
+    UP(d_pk_mask, st->pk_mask);
+    UP(d_pk_sens, st->pk_sens);
+    UP(d_mPi, mPi);
+    UP(d_mPw, mPw);
+    UP(d_mVi, mVi);
+    UP(d_mVw, mVw);
+    UP(d_mic_par, mic_par);
+    UP(d_bwgt, st->bwgt);
+    UP(d_bgrp, bgrp);
+#undef UP
+    {
+        std::vector<float> modal0((size_t)2 * kModes * std::max(n_strings, 1), 0.0f);
+        std::vector<float> pk0((size_t)kRing * std::max(n_pickups, 1), 0.0f);
+        std::vector<float> mic0((size_t)kRing * std::max(n_mics, 1), 0.0f);
+        std::vector<int> dn(std::max(n_strings, 1), 1);
+        if (dupload(&st->d_modal, modal0) != hipSuccess || dupload(&st->d_pk_out, pk0) != hipSuccess ||
+            dupload(&st->d_mic_out, mic0) != hipSuccess || dupload(&st->d_blk_dnode, dn) != hipSuccess)
+            return "device_alloc";
+    }
+    // LDS: 8 words per string node, then neck products, fpl[64], drive[nb], flags
+    // (the post phase's 2T modal staging and 4S pickup rows fit in the 8T string words)
+    // (an AMR body's mic stencil sums take 4 * kThreads + 4 * n_mics words of it)
+    size_t lds_str = (size_t)8 * st->tot_nodes;
+    if (st->amr && n_mics > 0) lds_str = std::max(lds_str, (size_t)4 * kThreads + 4 * (size_t)n_mics);
+    st->lds_bytes = sizeof(float) * (lds_str + st->nn + kMaxStrings + st->nb + 8 +
+                                     3 * (size_t)std::max(n_strings, 1));
+    if (st->lds_bytes > (size_t)kLdsLimit) {
+        afdtd_set_error("coevolver_create: strings, neck and bridge kernels exceed the LDS of one "
+                        "workgroup");
+        return "lds_limit";
+    }
+    if (hipFuncSetAttribute((const void*)k_coevo, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)st->lds_bytes) != hipSuccess)
+        return "device_alloc";
+    CoConst& k = st->k;
+    k.ns = n_strings; k.npk = n_pickups; k.nmic = n_mics; k.nmag = st->nmag;
+    k.tot_seg = st->tot_seg; k.tot_nodes = st->tot_nodes; k.nb = st->nb; k.nbgrp = nbgrp;
+    k.nn = st->nn; k.nngrp = nngrp; k.neck_on = (st->n_m > 0.0f && st->nn > 0) ? 1 : 0;
+    k.any_vel = st->any_vel; k.Nplate = nplate; k.clear_ext = st->amr ? 0 : 1;
+    k.amr = st->amr ? 1 : 0;
+    k.st_ok = st->amr ? INT_MAX : 0;
+    k.lds_str = (int)lds_str;
+    float dt_b = st->dt_fdtd;
+    if (dt_b <= 0.0f) dt_b = st->dt_audio;
+    k.dt_b = dt_b;
+    k.inv_dx2 = f ? 1.0f / (f->dx * f->dx) : 0.0f;
+    k.rhoc = st->rho_air * st->c;
+    k.n_m = st->n_m; k.n_c = st->n_c + st->nb_c; k.n_k = st->n_k + st->nb_k;
+    k.nb_k = st->nb_k; k.nb_c = st->nb_c;
+    k.str = st->d_str; k.sw = st->d_sw; k.u = st->d_u; k.neck = st->d_neck;
+    k.nidx = st->d_nidx; k.nwgt = st->d_nwgt; k.ngrp = st->d_ngrp; k.sbk = st->d_sbk;
+    k.bgrp = st->amr ? st->d_bgrp : f->d_bgrp; k.sad_idx = st->d_sad_idx; k.sad_wgt = st->d_sad_wgt;
+    k.basis = st->d_basis; k.modal = st->d_modal; k.K0 = st->d_K0; k.K1 = st->d_K1;
+    k.magmap = st->d_magmap; k.pk_type = st->d_pk_type; k.pk_mask = st->d_pk_mask;
+    k.pk_sens = st->d_pk_sens; k.mPi = st->d_mPi; k.mPw = st->d_mPw; k.mVi = st->d_mVi;
+    k.mVw = st->d_mVw; k.mic_par = st->d_mic_par; k.pk_out = st->d_pk_out;
+    k.mic_out = st->d_mic_out; k.blk_dnode = st->d_blk_dnode;
+    k.bwgt = st->d_bwgt;
+    if (st->amr) {
+        AmrCoView v;
+        if (amr_co_view(st->amr, &v) != SK_OK) return "device_alloc";
+        k.status = v.status;
+        k.amp_start = v.mp_start; k.amp_cell = v.mp_cell; k.amp_w = v.mp_w;
+        k.amv_start = v.mv_start; k.amv_face = v.mv_face;
+        for (int ax = 0; ax < 3; ++ax) k.amv_w[ax] = v.mv_w[ax];
+    } else {
+        k.status = f->d_status;
+        f->hook_clears_ext = 1;
+    }
+    if (ensure_blk(st, 64) != CE_OK) return "device_alloc";
+    return nullptr;
+}
+
+}  // namespace
+
 extern "C" {
 
 SK_API AcousticCoEvolverState* coevolver_create(int n_strings, const CoEvolverStringDef* sd,
@@ -1291,254 +1721,81 @@ SK_API AcousticCoEvolverState* coevolver_create(int n_strings, const CoEvolverSt
     if (st->nb > 0 &&
         fdtd_set_bridge_sources(f, st->nb, st->bidx.data(), st->bwgt.data()) != FDTD_OK)
         return fail();
-    st->ns = n_strings;
-    st->s.resize(n_strings);
-    for (int i = 0; i < n_strings; ++i)
-        if (!string_init(st->s[i], sd[i], st->dt_fdtd)) return fail();
-    for (int i = 0; i < n_strings; ++i) {                                  // :1079-1101
-        if (sd[i].neck_freq_hz > 0.0f && sd[i].neck_mass_kg > 0.0f) {
-            st->n_m = sd[i].neck_mass_kg;
-            const float om = 2.0f * kPi * sd[i].neck_freq_hz;
-            st->n_k = st->n_m * om * om;
-            const float Q = (sd[i].neck_Q > 0.0f) ? sd[i].neck_Q : 30.0f;
-            st->n_c = st->n_m * om / Q;
-            const float oj = 2.0f * kPi * 95.0f;
-            st->nb_k = st->n_m * oj * oj;
-            st->nb_c = st->n_m * oj / 2.0f;
-            break;
-        }
-    }
-    if (st->n_m > 0.0f && neck_cells(st, body->plate_active, 0.030f) != CE_OK) return fail();
-    // node / segment offsets
-    int no = 0, so = 0;
-    for (auto& s : st->s) {
-        s.node_off = no;
-        s.seg_off = so;
-        no += s.n_nodes;
-        so += s.n_segs;
-    }
-    st->tot_nodes = no;
-    st->tot_seg = so;
-    if (no > kMaxNodes) {
-        afdtd_set_error("coevolver_create: more than 4096 string nodes in total");
-        return fail();
-    }
-    // pickups (:1107-1132, GEMV rows :1248-1306)
-    st->npk = n_pickups;
-    std::vector<float> K0, K1;
-    std::vector<int> magmap;
-    for (int p = 0; p < n_pickups; ++p) {
-        st->pk_type.push_back(pd[p].type);
-        st->pk_mask.push_back(pd[p].string_mask);
-        st->pk_sens.push_back(pd[p].sensitivity);
-    }
-    for (int p = 0; p < n_pickups; ++p)
-        if (pd[p].type != PICKUP_PIEZO) ++st->nmag;
-    if (st->nmag > 0 && st->tot_seg > 0) {
-        K0.assign((size_t)st->nmag * st->tot_seg, 0.0f);
-        K1.assign((size_t)st->nmag * st->tot_seg, 0.0f);
-        int r = 0;
-        for (int p = 0; p < n_pickups; ++p) {
-            if (pd[p].type == PICKUP_PIEZO) continue;
-            magmap.push_back(p);
-            for (int i = 0; i < n_strings; ++i) {
-                if (!(pd[p].string_mask & (1u << (unsigned)i))) continue;
-                std::vector<float> Bn, Bb;
-                bkernels(pd[p], st->s[i], Bn, Bb);
-                for (int q = 0; q < st->s[i].n_segs; ++q) {
-                    K0[(size_t)r * st->tot_seg + st->s[i].seg_off + q] = Bn[q] * st->s[i].ds;
-                    K1[(size_t)r * st->tot_seg + st->s[i].seg_off + q] = Bb[q] * st->s[i].ds;
-                }
-            }
-            ++r;
-        }
-    } else {
-        st->nmag = std::min(st->nmag, 0);
-    }
-    // mics (:1135-1246)
-    st->nmic = n_mics;
-    std::vector<float> mic_par((size_t)6 * n_mics);
-    st->mic_g.resize((size_t)3 * n_mics);
-    for (int m = 0; m < n_mics; ++m) {
-        for (int q = 0; q < 3; ++q)
-            st->mic_g[3 * m + q] = (md[m].pos[q] - st->origin[q]) / st->dx;
-        mic_par[6 * m + 0] = md[m].polar_a;
-        mic_par[6 * m + 1] = md[m].polar_b;
-        mic_par[6 * m + 2] = md[m].gain;
-        for (int q = 0; q < 3; ++q) mic_par[6 * m + 3 + q] = md[m].axis[q];
-        if (md[m].polar_b != 0.0f) st->any_vel = 1;
-    }
-    std::vector<int> mPi((size_t)8 * n_mics), mVi;
-    std::vector<float> mPw((size_t)8 * n_mics), mVw;
-    if (n_mics > 0) {
-        if (fdtd_precompute_pressure_samplers(f, n_mics, st->mic_g.data(), mPi.data(),
-                                              mPw.data()) != FDTD_OK)
-            return fail();
-        mVi.assign((size_t)24 * n_mics, 0);
-        mVw.assign((size_t)24 * n_mics, 0.0f);
-        if (st->any_vel) {
-            const size_t o = (size_t)8 * n_mics;
-            if (fdtd_precompute_velocity_samplers(f, n_mics, st->mic_g.data(), mVi.data(),
-                                                  mVw.data(), mVi.data() + o, mVw.data() + o,
-                                                  mVi.data() + 2 * o,
-                                                  mVw.data() + 2 * o) != FDTD_OK)
-                return fail();
-            for (int ax = 0; ax < 3; ++ax)
-                for (size_t q = 0; q < o; ++q)
-                    mVi[ax * o + q] = packed_to_padded_v(ax, mVi[ax * o + q], st->Nx, st->Ny,
-                                                         st->Nz);
-        }
-    }
-    // saddle bilinear taps on the plate (:1160-1183)
-    std::vector<int> sad_idx((size_t)4 * n_strings);
-    std::vector<float> sad_wgt((size_t)4 * n_strings);
-    for (int i = 0; i < n_strings; ++i) {
-        const HString& s = st->s[i];
-        const int Nn = s.n_nodes;
-        float gi = (s.xyz[(Nn - 1) * 3 + 0] - st->origin[0]) / st->dx;
-        float gj = (s.xyz[(Nn - 1) * 3 + 1] - st->origin[1]) / st->dx;
-        if (gi < 0.0f) gi = 0.0f;
-        if (gi > (float)(st->Nx - 1)) gi = (float)(st->Nx - 1);
-        if (gj < 0.0f) gj = 0.0f;
-        if (gj > (float)(st->Ny - 1)) gj = (float)(st->Ny - 1);
-        const int i0 = (int)gi, i1 = (i0 < st->Nx - 1) ? i0 + 1 : i0;
-        const int j0 = (int)gj, j1 = (j0 < st->Ny - 1) ? j0 + 1 : j0;
-        const float fi = gi - (float)i0, fj = gj - (float)j0;
-        sad_idx[i * 4 + 0] = j0 + st->Ny * i0;
-        sad_idx[i * 4 + 1] = j0 + st->Ny * i1;
-        sad_idx[i * 4 + 2] = j1 + st->Ny * i0;
-        sad_idx[i * 4 + 3] = j1 + st->Ny * i1;
-        sad_wgt[i * 4 + 0] = (1.0f - fi) * (1.0f - fj);
-        sad_wgt[i * 4 + 1] = fi * (1.0f - fj);
-        sad_wgt[i * 4 + 2] = (1.0f - fi) * fj;
-        sad_wgt[i * 4 + 3] = fi * fj;
-    }
-    // per-string bridge kernels (:1322-1375)
-    std::vector<float> sbk;
-    if (n_strings > 0 && st->nb > 0) {
-        const int nb = st->nb;
-        sbk.assign((size_t)n_strings * nb, 0.0f);
-        const float sigma = (body->bridge_sigma > 0.0f) ? body->bridge_sigma : st->dx;
-        const float i2s2 = 0.5f / (sigma * sigma);
-        for (int i = 0; i < n_strings; ++i) {
-            const HString& s = st->s[i];
-            const int Nn = s.n_nodes;
-            const float sx = s.xyz[(Nn - 1) * 3 + 0], sy = s.xyz[(Nn - 1) * 3 + 1];
-            float* row = &sbk[(size_t)i * nb];
-            float ws = 0.0f;
-            for (int cix = 0; cix < nb; ++cix) {
-                const int flat = st->bidx[cix];
-                const int t = flat / st->Nz;
-                const int j = t % st->Ny, ii = t / st->Ny;
-                const float cx = (float)ii * st->dx + st->origin[0];
-                const float cy = (float)j * st->dx + st->origin[1];
-                const float d2x = sx - cx, d2y = sy - cy;
-                row[cix] = expf(-(d2x * d2x + d2y * d2y) * i2s2);
-                ws += row[cix];
-            }
-            if (ws > 1e-12f)
-                for (int cix = 0; cix < nb; ++cix) row[cix] /= ws;
-        }
-    }
-    // modal basis, node-major for coalesced mode reads (:494-505)
-    std::vector<float> basis((size_t)st->tot_nodes * kModes);
-    for (const auto& s : st->s) {
-        const int Nn = s.n_nodes;
-        const float inv = 1.0f / (float)(Nn > 1 ? Nn - 1 : 1);
-        for (int k = 0; k < kModes; ++k) {
-            const float kpi = (float)(k + 1) * kPi;
-            for (int i = 0; i < Nn; ++i)
-                basis[(size_t)(s.node_off + i) * kModes + k] = sinf(kpi * (float)i * inv);
-        }
-    }
-    // neck force groups over the carved plate mask the body FDTD holds
-    std::vector<int> ngrp;
-    int nngrp = 0;
-    if (st->nn > 0)
-        nngrp = group_entries(st->nidx.data(), st->nn, 1, f->h_pa.data(), f->Nplate, ngrp);
-    // device state
-    std::vector<CoStr> hs(n_strings);
-    std::vector<float> sw((size_t)kSw * std::max(n_strings, 1), 0.0f);
-    for (int i = 0; i < n_strings; ++i) {
-        const HString& s = st->s[i];
-        CoStr& d = hs[i];
-        d.n_nodes = s.n_nodes; d.node_off = s.node_off; d.seg_off = s.seg_off; d.n_sub = s.n_sub;
-        d.ds = s.ds; d.L = s.L; d.T0 = s.T0; d.EA = s.EA; d.mu = s.mu; d.dt_s = s.dt_s;
-        d.biharm = s.biharm; d.gamma_eff = s.gamma;
-        d.b_m = s.b_m; d.b_k = s.b_k; d.b_c = s.b_c; d.inv_dt_s = 1.0f / s.dt_s;
-        sw[(size_t)i * kSw + kTeff] = s.T0;
-    }
-    st->gamma_dev = 1.0f;
-    std::vector<float> zeros_u((size_t)4 * std::max(st->tot_nodes, 1), 0.0f);
-    std::vector<float> neck(4, 0.0f);
-#define UP(p, v) if (dupload(&st->p, v) != hipSuccess) return fail()
-    UP(d_str, hs);
-    UP(d_sw, sw);
-    UP(d_u, zeros_u);
-    UP(d_neck, neck);
-    UP(d_nidx, st->nidx);
-    UP(d_nwgt, st->nwgt);
-    UP(d_ngrp, ngrp);
-    UP(d_sbk, sbk);
-    UP(d_sad_idx, sad_idx);
-    UP(d_sad_wgt, sad_wgt);
-    UP(d_basis, basis);
-    UP(d_K0, K0);
-    UP(d_K1, K1);
-    UP(d_magmap, magmap);
-    UP(d_pk_type, st->pk_type);
-    UP(d_pk_mask, st->pk_mask);
-    UP(d_pk_sens, st->pk_sens);
-    UP(d_mPi, mPi);
-    UP(d_mPw, mPw);
-    UP(d_mVi, mVi);
-    UP(d_mVw, mVw);
-    UP(d_mic_par, mic_par);
-#undef UP
-    {
-        std::vector<float> modal0((size_t)2 * kModes * std::max(n_strings, 1), 0.0f);
-        std::vector<float> pk0((size_t)kRing * std::max(n_pickups, 1), 0.0f);
-        std::vector<float> mic0((size_t)kRing * std::max(n_mics, 1), 0.0f);
-        std::vector<int> dn(std::max(n_strings, 1), 1);
-        if (dupload(&st->d_modal, modal0) != hipSuccess || dupload(&st->d_pk_out, pk0) != hipSuccess ||
-            dupload(&st->d_mic_out, mic0) != hipSuccess || dupload(&st->d_blk_dnode, dn) != hipSuccess)
-            return fail();
-    }
-    // LDS: 8 words per string node, then neck products, fpl[64], drive[nb], flags
-    // (the post phase's 2T modal staging and 4S pickup rows fit in the 8T string words)
-    st->lds_bytes = sizeof(float) * ((size_t)8 * st->tot_nodes + st->nn + kMaxStrings + st->nb +
-                                     8 + 3 * (size_t)std::max(n_strings, 1));
-    if (st->lds_bytes > (size_t)kLdsLimit) {
-        afdtd_set_error("coevolver_create: strings, neck and bridge kernels exceed the LDS of one "
-                        "workgroup");
-        return fail();
-    }
-    if (hipFuncSetAttribute((const void*)k_coevo, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)st->lds_bytes) != hipSuccess)
-        return fail();
-    CoConst& k = st->k;
-    k.ns = n_strings; k.npk = n_pickups; k.nmic = n_mics; k.nmag = st->nmag;
-    k.tot_seg = st->tot_seg; k.tot_nodes = st->tot_nodes; k.nb = st->nb; k.nbgrp = f->nbgrp;
-    k.nn = st->nn; k.nngrp = nngrp; k.neck_on = (st->n_m > 0.0f && st->nn > 0) ? 1 : 0;
-    k.any_vel = st->any_vel; k.Nplate = f->Nplate; k.clear_ext = 1;
-    float dt_b = st->dt_fdtd;
-    if (dt_b <= 0.0f) dt_b = st->dt_audio;
-    k.dt_b = dt_b;
-    k.inv_dx2 = 1.0f / (f->dx * f->dx);
-    k.rhoc = st->rho_air * st->c;
-    k.n_m = st->n_m; k.n_c = st->n_c + st->nb_c; k.n_k = st->n_k + st->nb_k;
-    k.nb_k = st->nb_k; k.nb_c = st->nb_c;
-    k.str = st->d_str; k.sw = st->d_sw; k.u = st->d_u; k.neck = st->d_neck;
-    k.nidx = st->d_nidx; k.nwgt = st->d_nwgt; k.ngrp = st->d_ngrp; k.sbk = st->d_sbk;
-    k.bgrp = f->d_bgrp; k.sad_idx = st->d_sad_idx; k.sad_wgt = st->d_sad_wgt;
-    k.basis = st->d_basis; k.modal = st->d_modal; k.K0 = st->d_K0; k.K1 = st->d_K1;
-    k.magmap = st->d_magmap; k.pk_type = st->d_pk_type; k.pk_mask = st->d_pk_mask;
-    k.pk_sens = st->d_pk_sens; k.mPi = st->d_mPi; k.mPw = st->d_mPw; k.mVi = st->d_mVi;
-    k.mVw = st->d_mVw; k.mic_par = st->d_mic_par; k.pk_out = st->d_pk_out;
-    k.mic_out = st->d_mic_out; k.blk_dnode = st->d_blk_dnode; k.status = f->d_status;
-    if (ensure_blk(st, 64) != CE_OK) return fail();
-    f->hook_clears_ext = 1;
+    const float sigma = (body->bridge_sigma > 0.0f) ? body->bridge_sigma : st->dx;   // :1322-1375
+    if (co_setup(st, n_strings, sd, n_pickups, pd, n_mics, md, body, sigma)) return fail();
     return st;
+}
+
+SK_API AcousticCoEvolverState* coevolver_create_amr(int n_strings,
+                                                    const CoEvolverStringDef* sd, int n_pickups,
+                                                    const CoEvolverPickupDef* pd, int n_mics,
+                                                    const CoEvolverMicDef* md,
+                                                    const AMRCoevolverDescriptor* d,
+                                                    float sample_rate, int modal_stride) {
+    g_amr_err = "";                                                      // :1398-1413
+    if (n_strings < 0 || n_pickups < 0 || n_mics < 0) {
+        g_amr_err = "invalid_counts";
+        return nullptr;
+    }
+    if ((n_strings > 0 && !sd) || (n_pickups > 0 && !pd) || (n_mics > 0 && !md) || !d) {
+        g_amr_err = "invalid_null_args";
+        return nullptr;
+    }
+    if (sample_rate <= 0 || modal_stride <= 0) {
+        g_amr_err = "invalid_timing_args";
+        return nullptr;
+    }
+    if (n_strings > kMaxStrings) {
+        afdtd_set_error("coevolver_create_amr: at most 64 strings");
+        g_amr_err = "strings_alloc";
+        return nullptr;
+    }
+    AcousticCoEvolverState* st = new (std::nothrow) AcousticCoEvolverState();
+    if (!st) return nullptr;
+    auto fail = [&](const char* stage) -> AcousticCoEvolverState* {
+        g_amr_err = stage;
+        co_free(st);
+        delete st;
+        return nullptr;
+    };
+    st->amr = amr_state_from_descriptor(d);                             // :1422-1424
+    if (!st->amr) return fail("pressure_backend_create");
+    st->Nx = d->plate_Nx; st->Ny = d->plate_Ny; st->Nz = 1; st->dx = d->plate_dx;   // :1429-1437
+    for (int q = 0; q < 3; ++q) st->origin[q] = d->plate_origin[q];
+    st->rho_air = (float)d->rho_air;
+    st->c = (float)d->c;
+    st->sr = sample_rate;
+    st->dt_audio = 1.0f / sample_rate;
+    st->dt_fdtd = (float)amr_get_dt(st->amr);
+    if (st->dt_fdtd <= 0.0f) return fail("dt_fdtd_invalid");
+    st->modal_stride = modal_stride;
+    st->K = (int)ceilf(st->dt_audio / st->dt_fdtd);
+    if (st->K < 1) st->K = 1;
+    if (st->K > kMaxSub) st->K = kMaxSub;
+    st->nb = d->n_bridge_plate;                                          // :1459-1477
+    st->bidx.assign(d->bridge_plate_idx, d->bridge_plate_idx + st->nb);
+    st->bwgt.assign(d->bridge_plate_wgt, d->bridge_plate_wgt + st->nb);
+    if (d->n_neck_plate > 0 && d->neck_plate_idx && d->neck_plate_wgt) {
+        st->nn = d->n_neck_plate;
+        st->nidx.assign(d->neck_plate_idx, d->neck_plate_idx + st->nn);
+        st->nwgt.assign(d->neck_plate_wgt, d->neck_plate_wgt + st->nn);
+    }
+    const float sigma = (d->plate_dx > 0.0f) ? 2.0f * d->plate_dx : 0.005f;   // :1711-1712
+    if (const char* stage = co_setup(st, n_strings, sd, n_pickups, pd, n_mics, md, nullptr, sigma))
+        return fail(stage);
+    return st;
+}
+
+SK_API const char* coevolver_create_amr_last_error(void) { return g_amr_err.c_str(); }
+
+// Extension: the pressure backend's error code name and message of the last
+// failed coevolver_create_amr on this thread (the reference's pybind layer
+// reads them through C++ calls, pybind_kernels.cpp:6060-6080).
+SK_API const char* coevolver_amr_backend_error_code(void) {
+    return amr_pressure_backend_last_error_code();
+}
+SK_API const char* coevolver_amr_backend_error_message(void) {
+    return amr_pressure_backend_last_error_message();
 }
 
 SK_API void coevolver_destroy(AcousticCoEvolverState* st) {
@@ -1713,6 +1970,7 @@ SK_API int coevolver_get_mic_output(const AcousticCoEvolverState* cst, int mic_i
 SK_API int coevolver_get_pressure_field(const AcousticCoEvolverState* st, float* out,
                                         int out_len) {
     if (!st) return CE_ERR_NULL;
+    if (st->amr) return amr_get_pressure(st->amr, out, out_len);        // flat cell array
     return fdtd_get_pressure_field(st->fdtd, out, out_len);
 }
 
@@ -1722,19 +1980,21 @@ SK_API int coevolver_get_pressure_field_uniform(const AcousticCoEvolverState* st
                                                 int out_len) {
     if (!st || !out || !bmin || !bmax) return CE_ERR_NULL;
     if (out_len != Nx * Ny * Nz) return CE_ERR_DIM;
+    if (st->amr) return amr_scatter_pressure_uniform(st->amr, Nx, Ny, Nz, bmin, bmax, out);
     return fdtd_get_pressure_field(st->fdtd, out, out_len);
 }
 
 SK_API int coevolver_get_amr_cell_centers(const AcousticCoEvolverState* st, float* out_xyz,
                                           int n_cells_3) {
-    (void)n_cells_3;
     if (!st || !out_xyz) return CE_ERR_NULL;
-    return CE_ERR_PARAM;                                                 // uniform body
+    if (!st->amr) return CE_ERR_PARAM;                                   // uniform body
+    return amr_get_cell_centers(st->amr, out_xyz, n_cells_3);
 }
 
 SK_API int coevolver_get_plate_displacement(const AcousticCoEvolverState* st, float* out,
                                             int out_len) {
     if (!st) return CE_ERR_NULL;
+    if (st->amr) return amr_get_plate_displacement(st->amr, out, out_len);
     return fdtd_get_plate_displacement(st->fdtd, out, out_len);
 }
 
@@ -1766,6 +2026,7 @@ SK_API int coevolver_get_modal_amplitudes(const AcousticCoEvolverState* st, int 
 SK_API int coevolver_sample_pressure(const AcousticCoEvolverState* st, int n_rec,
                                      const float* rec_xyz, float* out_p) {   // :2608-2625
     if (!st || !rec_xyz || !out_p) return CE_ERR_NULL;
+    if (st->amr) return CE_ERR_PARAM;                                    // :2613-2616
     std::vector<float> g((size_t)3 * std::max(n_rec, 0));
     for (int r = 0; r < n_rec; ++r)
         for (int q = 0; q < 3; ++q) g[3 * r + q] = (rec_xyz[3 * r + q] - st->origin[q]) / st->dx;
@@ -1847,8 +2108,10 @@ SK_API float coevolver_get_dt_fdtd(const AcousticCoEvolverState* st) {
 SK_API int coevolver_reset(AcousticCoEvolverState* st) {                  // :2698-2738
     if (!st) return CE_ERR_NULL;
     if (busy(st)) return CE_ERR_BUSY;
-    if (fdtd_reset(st->fdtd) != FDTD_OK) return CE_ERR_NULL;
-    hipStream_t sm = st->fdtd->stream;
+    // the body: a uniform FDTD is zeroed; an AMR body is left as it is (the
+    // reference calls fdtd_reset on its null uniform handle, :2702)
+    if (st->fdtd && fdtd_reset(st->fdtd) != FDTD_OK) return CE_ERR_NULL;
+    hipStream_t sm = co_stream(st);
     std::vector<float> sw((size_t)kSw * std::max(st->ns, 1));
     if (st->ns && hipMemcpy(sw.data(), st->d_sw, sw.size() * sizeof(float),
                             hipMemcpyDeviceToHost) != hipSuccess)
@@ -1892,6 +2155,7 @@ SK_API int coevolver_get_surface_emission(const AcousticCoEvolverState* st, int 
                                           const float* surf_xyz, const float* surf_normals,
                                           float* out_P, float* out_vn) {    // :2760-2779
     if (!st || !surf_xyz || !surf_normals || !out_P || !out_vn) return CE_ERR_NULL;
+    if (!st->fdtd) return CE_ERR_NULL;                                   // no uniform grid
     std::vector<float> g((size_t)3 * std::max(n_surf, 0));
     for (int r = 0; r < n_surf; ++r)
         for (int q = 0; q < 3; ++q) g[3 * r + q] = (surf_xyz[3 * r + q] - st->origin[q]) / st->dx;
@@ -1904,6 +2168,7 @@ SK_API int coevolver_sample_velocity(const AcousticCoEvolverState* st, int n_rec
                                      const float* rec_xyz, float* out_vx, float* out_vy,
                                      float* out_vz) {                        // :2781-2797
     if (!st || !rec_xyz || !out_vx || !out_vy || !out_vz) return CE_ERR_NULL;
+    if (!st->fdtd) return CE_ERR_NULL;                                   // no uniform grid
     std::vector<float> g((size_t)3 * std::max(n_rec, 0));
     for (int r = 0; r < n_rec; ++r)
         for (int q = 0; q < 3; ++q) g[3 * r + q] = (rec_xyz[3 * r + q] - st->origin[q]) / st->dx;

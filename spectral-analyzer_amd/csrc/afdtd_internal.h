@@ -172,6 +172,45 @@ constexpr int FDTD_STATUS_HOOK = 16;
 int afdtd_step_hooked(AcousticFDTDState* st, int n_steps, const StepHook& hook, float* ms_total,
                       int* status_out);
 
+// The same for the AMR stepper (afdtd_amr.h).  hook(a) runs before step
+// a.q of the call (a.q < n) and once more with a.q == n; it sees the plate
+// displacement current at that point (the buffer rotation advances by one
+// per step) and the device buffers the co-evolver's kernel reads and writes.
+// A hooked kernel that finds its own failure before step q sets status[3]
+// = q and lowers status[0] to q - 1, so that step q and everything after it
+// is skipped.  Returns SK_OK, SK_ERR_DIVERGED (a step diverged, or the hook
+// failed: *hook_failed) or AMR_ERR_DEVICE; *done = the steps that completed.
+#include "../../include/afdtd_amr.h"
+#include "../../include/afdtd_coevolver.h"
+struct AmrHookArgs {
+    int q;
+    const float* w;         // current plate displacement
+    float* ext;             // plate force accumulator
+    const float* p;         // cell pressures
+    const float* v;         // face velocities
+};
+using AmrStepHook = std::function<int(const AmrHookArgs&)>;
+int amr_step_hooked(AcousticAMRState* st, int n, const AmrStepHook& hook, float* ms_total,
+                    int* done, int* hook_failed);
+// What the co-evolver's kernel needs of an AMR state: its stream, status
+// words, plate size and the device mic stencils (CSR: per-mic starts, then
+// cells / faces and weights).
+struct AmrCoView {
+    hipStream_t stream;
+    int* status;
+    int n_cells, n_pnodes, pNx, pNy, n_mics;
+    const int* mp_start;
+    const int* mp_cell;
+    const float* mp_w;
+    const int* mv_start;
+    const int* mv_face;
+    const float* mv_w[3];
+};
+int amr_co_view(AcousticAMRState* st, AmrCoView* v);
+// The AMR state of amr_pressure_backend_create (the same checks, calls,
+// error codes and messages), without the interface object around it.
+AcousticAMRState* amr_state_from_descriptor(const AMRCoevolverDescriptor* d);
+
 // Launchers (all enqueue on st->stream; return hipError_t of the launch).
 hipError_t launch_velocity_generic(AcousticFDTDState* st);
 hipError_t launch_pressure_generic(AcousticFDTDState* st);
