@@ -719,6 +719,109 @@ def bench_coevolver(a):
     return 0
 
 
+def coevolver_amr_workload(nx=96, ny=128, nz=56, kp=20, dx=0.004, n_pml=8):
+    """The same instrument on an AMR body (coevolver_create_amr): a uniform
+    AMR box of nx x ny x nz cells at dx (688 K cells, about the uniform
+    body's 718 K) with an 8-cell anechoic shell, the plate in layer kp
+    (nodes 5 dx apart), bridge and neck plate patches; six 240-segment
+    strings, one bridge single-coil, a cardioid plus a ring of eight."""
+    from afdtd.instrument import amr_box_desc, guitar_defs
+    org = (-0.192, -0.300, 0.0)
+    body_h = org[2] + kp * dx
+    bridges = [(-0.02, -0.13), (0.0, -0.13), (0.02, -0.13)]
+    D = amr_box_desc(nx, ny, nz, kp, dx, org, bridges, neck_positions=[(0.0, 0.2)],
+                     n_pml=n_pml, border_mode=0)
+    S, P, M = guitar_defs(body_h, bridges)
+    return S, P, M, D
+
+
+def bench_coevolver_amr(a):
+    """The co-evolver's per-sample loop on an AMR body (coevolver_step on a
+    coevolver_create_amr handle): per audio sample, K sub-steps of one
+    string / bridge / neck kernel and the AMR step (faces, plate faces,
+    cells, plate), then pickups and the mics' stencil sums.  Metric: audio
+    samples per second (1 GPU).  Roofline: the AMR step's dominant kernel on
+    a plain AMR handle of the same grid, with the AMR stepper's byte
+    accounting (order 2, the handle's record layout).  CPU baseline: the co-evolver
+    oracle on the AMR oracle, same instrument."""
+    from afdtd.coevolver import AcousticCoEvolver
+    S, P, M, D = coevolver_amr_workload()
+    co = AcousticCoEvolver.from_amr_desc(D, S, P, M, 44100.0, modal_stride=16)
+    K = co.substeps
+    samples = a.steps if a.steps != 4000 else 2048
+    for si in range(len(S)):
+        co.schedule_pluck(8 * si, si, 0.2 + 0.1 * si, 0.002, 0)
+    co.step(max(a.warmup, 1))
+    t0 = time.perf_counter()
+    ms_dev = co.step_timed(samples)
+    wall = time.perf_counter() - t0
+    mic = co.get_mic_output(0, min(samples, 4096))
+    if not np.all(np.isfinite(mic)) or not np.any(mic):
+        raise SystemExit("AMR co-evolver produced no finite mic output")
+    del co
+    # the AMR step alone on a plain handle of the same grid
+    from afdtd.amr import AcousticAMR
+    g = AcousticAMR(D["cell_centers"], D["cell_volumes"], D["open_volume_frac"],
+                    D["cell_types"], D["face_cell_neg"], D["face_cell_pos"], D["face_area"],
+                    D["face_open_frac"], D["face_distance"], D["c"], D["rho_air"], D["min_dx"],
+                    D["gradient_order"])
+    g.set_border_condition(0, 3.0, 0.0, 0.0, D["n_pml"], D["bounds_min"], D["bounds_max"])
+    layout = g.record_layout                 # the layout the co-evolver's body picked too
+    nst = 200
+    rc, ms_t, ms_f, ms_c = g.step_timed(nst)
+    if rc:
+        raise SystemExit(f"amr_step_timed failed rc={rc}")
+    C_, F, E = g.n_cells, D["n_faces"], g.n_csr_entries
+    rec_f, rec_c = (8, 4) if layout == "compact" else (16, 16)
+    b_faces = (8 + rec_f) * F + 4 * C_
+    b_cells = (12 + rec_c) * C_ + 4 * E + 8 * F
+    dom = "cells" if ms_c >= ms_f else "faces"
+    ach = (b_cells / (ms_c / nst * 1e-3) if dom == "cells" else
+           b_faces / (ms_f / nst * 1e-3)) / 1e9
+    del g
+    line = {"metric": "audio samples/s (instrument co-evolver, AMR body, 1 GPU)",
+            "value": samples / wall, "unit": "samples/s", "n_gpus": 1, "steps": samples,
+            "warmup": a.warmup, "ms_per_step": wall * 1e3 / samples, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+            "data": "synthetic (AMR box body with plate, six plucked strings)",
+            "config": {"workload": f"coevolver-amr-6str-240seg-box-{D['plate_Nx']}x"
+                                   f"{D['plate_Ny']}x{D['n_cells'] // (D['plate_Nx'] * D['plate_Ny'])}-dx4mm",
+                       "cells": D["n_cells"], "faces": D["n_faces"], "substeps_per_sample": K,
+                       "strings": len(S),
+                       "string_nodes": sum(np.asarray(d["path_xyz"]).size // 3 for d in S),
+                       "pickups": len(P), "mics": len(M), "sample_rate": 44100},
+            "realtime_factor_44k1": samples / wall / 44100.0,
+            "device_ms_per_sample": ms_dev / samples,
+            "device_us_per_substep": ms_dev * 1e3 / (samples * K),
+            "roofline": {"bound": "hbm", "kernel": f"k_amr_{dom} (plain handle, same grid, "
+                                                   f"{layout} records)",
+                         "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                         "bytes_per_launch": b_cells if dom == "cells" else b_faces,
+                         "amr_step_ms": (ms_f + ms_c) / nst,
+                         "step_share_of_substep": ((ms_f + ms_c) / nst)
+                                                  / (ms_dev / (samples * K))}}
+    if not a.no_cpu:
+        from oracle import coevo
+        o = coevo.OracleCoEvolver.from_amr(D, S, P, M, 44100.0, modal_stride=16)
+        for si in range(len(S)):
+            o.schedule_pluck(8 * si, si, 0.2 + 0.1 * si, 0.002, 0)
+        o.step(1)
+        t0 = time.perf_counter()
+        n = 0
+        while time.perf_counter() - t0 < 15.0 and n < samples:
+            o.step(8)
+            n += 8
+        el = time.perf_counter() - t0
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+        line["cpu_baseline"] = {"value": n / el, "unit": "samples/s", "cores": threads,
+                                "kind": "port",
+                                "sample": f"{n} samples of the same instrument ({el:.1f} s, "
+                                          f"float32 co-evolver oracle on the AMR oracle, OpenMP)"}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def _timed_room(w, storage="f32", kernel="auto", steps=40, warmup=4):
     """Steps of workload w on one handle: (cell-updates/s wall clock, average
     stencil ms per launch, kernel family)."""
@@ -895,6 +998,8 @@ def main():
         return bench_body(a)
     if a.config == "coevolver":
         return bench_coevolver(a)
+    if a.config == "coevolver-amr":
+        return bench_coevolver_amr(a)
     if a.config == "amr":
         return bench_amr(a)
     if a.config == "amr-audio":
