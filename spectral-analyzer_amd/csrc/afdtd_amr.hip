@@ -1295,6 +1295,8 @@ int amr_co_view(AcousticAMRState* st, AmrCoView* v) {
     const int* I = st->d_mint;
     const float* F = st->d_mflt;
     const bool m = st->n_mics > 0 && I && F;
+    v->n_mp = m ? st->o_mvf - st->o_mpc : 0;
+    v->n_mv = m ? st->o_mwy - st->o_mwx : 0;
     v->mp_start = m ? I : nullptr;
     v->mp_cell = m ? I + st->o_mpc : nullptr;
     v->mp_w = m ? F : nullptr;

@@ -749,6 +749,7 @@ int enqueue_step(AcousticFDTDState* st, int s, hipEvent_t ea, hipEvent_t eb,
     }
     if (ea) AFDTD_CHECK(hipEventRecord(ea, st->stream));
     if (st->family == FDTD_KERNEL_FUSED || st->family == FDTD_KERNEL_FUSED2) {
+        st->side_step = s;
         AFDTD_CHECK(launch_fused_step(st));
     } else {
         AFDTD_CHECK(launch_velocity_generic(st));

@@ -64,13 +64,18 @@ struct CoStr {
     float b_m, b_k, b_c, inv_dt_s;
 };
 
-// Per-string state words (device): T_eff, saddle_w, saddle_w_prev, bridge_w, bridge_w_prev.
-enum { kTeff = 0, kSad = 1, kSadP = 2, kBr = 3, kBrP = 4, kSw = 8 };
+// Per-string state words (device): T_eff, saddle_w, saddle_w_prev, bridge_w, bridge_w_prev,
+// and the arc length of the current shape with its valid flag (the last
+// shape check's sum, which is the next tension's sum: same shape, same terms).
+enum { kTeff = 0, kSad = 1, kSadP = 2, kBr = 3, kBrP = 4, kArc = 5, kArcOk = 6, kSw = 8 };
 
 struct CoConst {
     int ns, npk, nmic, nmag, tot_seg, tot_nodes, nb, nbgrp, nn, nngrp, neck_on, any_vel;
     int Nplate, clear_ext;
     int amr;                                // body: 0 uniform FDTD, 1 AMR stepper
+    int defer;                              // output sums run on idle wavefronts (see k_coevo)
+    const float* npre;                      // neck plate sum formed by the body sweep: sum, tag
+    int anp, anv;                           // AMR mic stencil entries: pressure, velocity
     int st_ok;                              // status[0] when nothing failed (0 / INT_MAX)
     int lds_str;                            // LDS words before the neck products
     float dt_b, inv_dx2, rhoc;
@@ -118,6 +123,7 @@ struct CoConst {
 
 struct CoLaunch {
     int sample, sub, step;
+    int npre;                               // the previous body step formed the neck plate sum
     int post, modal, sample_end, out_pos, run;
     const float* w;                         // current plate displacement
     float* ext;                             // plate force accumulator
@@ -146,18 +152,45 @@ __device__ __forceinline__ void fail_here(const CoConst& c, const CoLaunch& a, i
     }
 }
 
-// acc + x[0] + x[1] + ... + x[n-1], one lane, in index order; the loads run
-// sixteen ahead of the adds so the chain is bound by the adds, not by LDS.
+// acc + x[0] + x[1] + ... + x[n-1], one lane, in index order.  The terms
+// come from LDS in blocks of sixteen, two blocks in flight: the next block
+// is loaded while the adds of the current one run, so the dependent adds
+// are not held up by the load latency.
+__device__ __forceinline__ void load16(float* r, const float* x) {
+#pragma unroll
+    for (int q = 0; q < 16; ++q) r[q] = x[q];
+}
+__device__ __forceinline__ float add16(float acc, const float* r) {
+#pragma unroll
+    for (int q = 0; q < 16; ++q) acc += r[q];
+    return acc;
+}
 __device__ __forceinline__ float seq_sum(const float* x, int n, float acc) {
-    int i = 0;
-    for (; i + 16 <= n; i += 16) {
-        float v[16];
-#pragma unroll
-        for (int q = 0; q < 16; ++q) v[q] = x[i + q];
-#pragma unroll
-        for (int q = 0; q < 16; ++q) acc += v[q];
+    const int nb = n >> 4;                      // full blocks of sixteen
+    float a[16], b[16];
+    if (nb > 0) {
+        load16(a, x);
+        int k = 1;
+        // (the scheduling barriers keep each block's loads ahead of the
+        // previous block's adds; the compiler would otherwise sink them)
+        for (; k + 1 < nb; k += 2) {            // blocks k (into b) and k + 1 (into a)
+            load16(b, x + 16 * k);
+            __builtin_amdgcn_sched_barrier(0);
+            acc = add16(acc, a);
+            load16(a, x + 16 * (k + 1));
+            __builtin_amdgcn_sched_barrier(0);
+            acc = add16(acc, b);
+        }
+        if (k < nb) {                           // block k is the last one
+            load16(b, x + 16 * k);
+            __builtin_amdgcn_sched_barrier(0);
+            acc = add16(acc, a);
+            acc = add16(acc, b);
+        } else {
+            acc = add16(acc, a);
+        }
     }
-    for (; i < n; ++i) acc += x[i];
+    for (int i = nb * 16; i < n; ++i) acc += x[i];
     return acc;
 }
 
@@ -186,10 +219,24 @@ __device__ bool string_pol(const float* __restrict__ u, const float* __restrict_
     return wave_any(bad);
 }
 
+// Tension from the arc length (acoustic_coevolver.cpp:576-592): strain,
+// T0 + EA strain and the CFL check.  One lane; returns 1 on failure.
+__device__ __forceinline__ int tension_arc(const CoStr& s, float arc, float* teff_out) {
+    int tbad = 0;
+    float strain = (arc - s.L) / s.L;
+    if (!__builtin_isfinite(strain)) tbad = 1;
+    if (strain < 0.0f) strain = 0.0f;
+    const float teff = s.T0 + s.EA * strain;
+    if (!__builtin_isfinite(teff) || teff <= 0.0f) tbad = 1;
+    const float cfl = sqrtf(teff / s.mu) * s.dt_s / s.ds;
+    if (!__builtin_isfinite(cfl) || cfl * cfl > 0.9025f) tbad = 1;
+    *teff_out = teff;
+    return tbad;
+}
+
 // Geometric tension of a string from its current displacements (u0, u1):
-// arc length in index order, strain, T0 + EA strain and the CFL check
-// (acoustic_coevolver.cpp:563-593).  Lane 0 stores T_eff; returns true
-// (wave-uniform) on failure.
+// arc length in index order, then tension_arc (acoustic_coevolver.cpp:
+// 563-593).  Lane 0 stores T_eff; returns true (wave-uniform) on failure.
 __device__ bool tension_of(const float* u0, const float* u1, float* scr, int N, const CoStr& s,
                            int lane, float* teff_out) {
     const float ds2 = s.ds * s.ds;
@@ -203,17 +250,7 @@ __device__ bool tension_of(const float* u0, const float* u1, float* scr, int N, 
     wave_sync();
     if (wave_any(bad)) return true;
     int tbad = 0;
-    if (lane == 0) {
-        const float arc = seq_sum(scr, N - 1, 0.0f);
-        float strain = (arc - s.L) / s.L;
-        if (!__builtin_isfinite(strain)) tbad = 1;
-        if (strain < 0.0f) strain = 0.0f;
-        const float teff = s.T0 + s.EA * strain;
-        if (!__builtin_isfinite(teff) || teff <= 0.0f) tbad = 1;
-        const float cfl = sqrtf(teff / s.mu) * s.dt_s / s.ds;
-        if (!__builtin_isfinite(cfl) || cfl * cfl > 0.9025f) tbad = 1;
-        *teff_out = teff;
-    }
+    if (lane == 0) tbad = tension_arc(s, seq_sum(scr, N - 1, 0.0f), teff_out);
     return __shfl(tbad, 0) != 0;
 }
 
@@ -232,7 +269,13 @@ __global__ __launch_bounds__(kThreads) void k_coevo(CoConst c, CoLaunch a) {
     float* fsT = (float*)(flag + 8);
     float* fsU = fsT + c.ns;
     float* teff0 = fsU + c.ns;                 // T_eff of each string's current sub-step
+    float* dprod = teff0 + c.ns;               // deferred output products (c.defer)
     if (tid == 0) flag[0] = 0;
+    // At a sample's end the products of the pickup GEMV rows (and of the AMR
+    // mic stencils) are formed here, but their one-lane sums run in the run
+    // phase on two wavefronts that carry no string, beside the neck mode and
+    // the strings' loads, instead of holding up every wavefront here.
+    const bool dfr = c.defer && a.run;
 
     // ── finish the previous sub-step (:2105-2125, and :2129-2223 at a sample's end) ──
     if (a.post) {
@@ -277,7 +320,16 @@ __global__ __launch_bounds__(kThreads) void k_coevo(CoConst c, CoLaunch a) {
                     }
                 }
                 __syncthreads();
-                for (int r = 0; r < c.nmag; ++r) {         // :2152-2171
+                for (int r = 0; dfr && r < c.nmag; ++r) {  // products only, summed later
+                    const float* k0 = c.K0 + (size_t)r * S;
+                    const float* k1 = c.K1 + (size_t)r * S;
+                    float* d = dprod + (size_t)r * 2 * S;
+                    for (int j = tid; j < S; j += kThreads) {
+                        d[j] = k0[j] * v0[j];
+                        d[S + j] = k1[j] * v1[j];
+                    }
+                }
+                for (int r = 0; !dfr && r < c.nmag; ++r) {  // :2152-2171
                     const float* k0 = c.K0 + (size_t)r * S;
                     const float* k1 = c.K1 + (size_t)r * S;
                     for (int j = tid; j < S; j += kThreads) {
@@ -341,7 +393,20 @@ __global__ __launch_bounds__(kThreads) void k_coevo(CoConst c, CoLaunch a) {
                     c.mic_out[(size_t)m * kRing + a.out_pos] = sig * mp[2];
                 }
             }
-            if (c.amr && c.nmic > 0) {
+            if (c.amr && c.nmic > 0 && dfr) {
+                // products of every mic's stencil entries, summed later
+                float* pp = dprod + (size_t)2 * S * c.nmag;
+                float* pv = pp + c.anp;
+                const int p0 = c.amp_start[0], v0i = c.amv_start[0];
+                for (int k = tid; k < c.anp; k += kThreads)
+                    pp[k] = c.amp_w[p0 + k] * a.P[c.amp_cell[p0 + k]];
+                for (int k = tid; k < c.anv; k += kThreads) {
+                    const float vf = a.V[0][c.amv_face[v0i + k]];
+                    pv[k] = c.amv_w[0][v0i + k] * vf;
+                    pv[c.anv + k] = c.amv_w[1][v0i + k] * vf;
+                    pv[2 * c.anv + k] = c.amv_w[2][v0i + k] * vf;
+                }
+            } else if (c.amr && c.nmic > 0) {
                 // AMR mic stencils (acoustic_amr.cpp:1762-1792): the products of
                 // a chunk of entries (all mics' entries are contiguous) are formed
                 // in parallel, then each mic's running sums take its entries of
@@ -410,10 +475,14 @@ __global__ __launch_bounds__(kThreads) void k_coevo(CoConst c, CoLaunch a) {
     // ── neck mode (:1934-1983) on the last wavefront, beside the strings'
     // loads and first tension (the two do not depend on each other) ──
     constexpr int SW = kThreads / 64 - 1;      // string wavefronts
+    // The neck plate sum over the current plate: formed by the previous body
+    // step's sweep (same products, same order) when it ran that side job and
+    // its tag names that step, else here.
+    const bool pre = c.neck_on && a.npre && ((const int*)c.npre)[1] == a.step - 1;
     if (c.neck_on) {
         // out-of-range plate indices are skipped by the reference; adding +0
         // to the running sum (which starts at +0, so never holds -0) is the same
-        for (int i = tid; i < c.nn; i += kThreads) {
+        for (int i = tid; i < c.nn && !pre; i += kThreads) {
             const int p = c.nidx[i];
             nprod[i] = (p < 0 || p >= c.Nplate) ? 0.0f : a.w[p] * c.nwgt[i];
         }
@@ -425,7 +494,7 @@ __global__ __launch_bounds__(kThreads) void k_coevo(CoConst c, CoLaunch a) {
     }
     if (wave == SW) {
         if (c.neck_on && lane == 0) {
-            const float pw = seq_sum(nprod, c.nn, 0.0f);
+            const float pw = pre ? c.npre[0] : seq_sum(nprod, c.nn, 0.0f);
             const float nw0 = c.neck[0], nwp = c.neck[1], npw = c.neck[2];
             float fs = 0.0f;
             for (int si = 0; si < c.ns; ++si) {
@@ -474,8 +543,60 @@ __global__ __launch_bounds__(kThreads) void k_coevo(CoConst c, CoLaunch a) {
                 const float d = c.blk_dval[a.sample * c.ns + si];
                 if (d != 0.0f) fe0[c.blk_dnode[si]] += d;
             }
-            if (tension_of(base, base + 3 * N, scr, N, s, lane, &teff0[si]))
-                if (lane == 0) fail_here(c, a, &flag[0]);
+            // the shape's arc length is the last shape check's (kArcOk), or is
+            // summed here (first sub-step after creation or reset)
+            bool tb;
+            if (c.sw[si * kSw + kArcOk] != 0.0f) {
+                int b = 0;
+                if (lane == 0) b = tension_arc(s, c.sw[si * kSw + kArc], &teff0[si]);
+                tb = __shfl(b, 0) != 0;
+            } else {
+                tb = tension_of(base, base + 3 * N, scr, N, s, lane, &teff0[si]);
+            }
+            if (tb && lane == 0) fail_here(c, a, &flag[0]);
+        }
+        if (a.sample_end && c.defer && wave == SW - 1 && c.nmag > 0) {
+            // pickup rows (:2152-2171): lane 2r sums row r's K0 products, lane
+            // 2r + 1 its K1 products, each in index order
+            const int S = c.tot_seg;
+            float sum = 0.0f;
+            if (lane < 2 * c.nmag)
+                sum = seq_sum(dprod + (size_t)(lane >> 1) * 2 * S + (lane & 1) * S, S, 0.0f);
+            const float other = __shfl_xor(sum, 1);
+            if (lane < 2 * c.nmag && (lane & 1) == 0)
+                c.pk_out[(size_t)c.magmap[lane >> 1] * kRing + a.out_pos] = sum + other;
+        }
+        if (a.sample_end && c.defer && wave == SW - 2 && c.amr && c.nmic > 0) {
+            // AMR mic stencil sums (acoustic_amr.cpp:1762-1792), one lane per
+            // mic and component, then the polar mix (:2208-2219)
+            const int S = c.tot_seg, M = c.nmic;
+            const float* pp = dprod + (size_t)2 * S * c.nmag;
+            const float* pv = pp + c.anp;
+            float* dres = (float*)pv + 3 * c.anv;
+            if (lane < 4 * M) {
+                const int kind = lane / M, m = lane - kind * M;
+                float r;
+                if (kind == 0) {
+                    const int lo = c.amp_start[m] - c.amp_start[0];
+                    r = seq_sum(pp + lo, c.amp_start[m + 1] - c.amp_start[m], 0.0f);
+                } else {
+                    const int lo = c.amv_start[m] - c.amv_start[0];
+                    r = seq_sum(pv + (size_t)(kind - 1) * c.anv + lo,
+                                c.amv_start[m + 1] - c.amv_start[m], 0.0f);
+                }
+                dres[lane] = r;
+            }
+            wave_sync();
+            if (lane < M) {
+                const float* mp = c.mic_par + 6 * lane;
+                float sig = dres[lane] * mp[0];
+                if (mp[1] != 0.0f) {
+                    const float vn = dres[M + lane] * mp[3] + dres[2 * M + lane] * mp[4] +
+                                     dres[3 * M + lane] * mp[5];
+                    sig += mp[1] * c.rhoc * vn;
+                }
+                c.mic_out[(size_t)lane * kRing + a.out_pos] = sig * mp[2];
+            }
         }
     }
     __syncthreads();
@@ -497,11 +618,17 @@ __global__ __launch_bounds__(kThreads) void k_coevo(CoConst c, CoLaunch a) {
         const float per_sub = c.blk_sub[a.sample * c.ns + si];
         const int fnode = c.blk_node[a.sample * c.ns + si];
         bool failed = false;
+        float arc = 0.0f;                       // lane 0: arc length of the checked shape
         for (int q = 0; q < s.n_sub && !failed; ++q) {
-            // geometric tension (:563-593); the first sub-step's was taken above
-            if (q > 0 && tension_of(buf[r0u], buf[r1u], scr, N, s, lane, &teff0[si])) {
-                failed = true;
-                break;
+            // geometric tension (:563-593); the first sub-step's was taken above,
+            // later ones take the previous shape check's arc length
+            if (q > 0) {
+                int b = 0;
+                if (lane == 0) b = tension_arc(s, arc, &teff0[si]);
+                if (__shfl(b, 0)) {
+                    failed = true;
+                    break;
+                }
             }
             wave_sync();
             Teff = teff0[si];
@@ -535,7 +662,7 @@ __global__ __launch_bounds__(kThreads) void k_coevo(CoConst c, CoLaunch a) {
             if (wave_any(bad)) { failed = true; break; }
             int sbad = 0;
             if (lane == 0) {
-                const float arc = seq_sum(scr, N - 1, 0.0f);
+                arc = seq_sum(scr, N - 1, 0.0f);
                 const float fr = (arc - s.L) / s.L;
                 if (!__builtin_isfinite(fr) || fr > kMaxStretch) sbad = 1;
             }
@@ -555,6 +682,8 @@ __global__ __launch_bounds__(kThreads) void k_coevo(CoConst c, CoLaunch a) {
         }
         if (lane == 0) {
             c.sw[si * kSw + kTeff] = Teff;
+            c.sw[si * kSw + kArc] = arc;
+            c.sw[si * kSw + kArcOk] = 1.0f;
             fsU[si] = buf[r0u][N - 2];                 // for the bridge below
             fsT[si] = Teff;
         }
@@ -729,6 +858,7 @@ struct AcousticCoEvolverState {
     int *d_nidx = nullptr, *d_ngrp = nullptr;
     float* d_sbk = nullptr;
     float* d_bwgt = nullptr;               // AMR body: bridge plate weights
+    float* d_npre = nullptr;               // uniform body: neck plate sum from the sweep
     int* d_bgrp = nullptr;                 // AMR body: bridge force groups
     int* d_sad_idx = nullptr;
     float* d_sad_wgt = nullptr;
@@ -759,7 +889,7 @@ void co_free(AcousticCoEvolverState* st) {
                     st->d_modal, st->d_K0, st->d_K1, st->d_magmap, st->d_pk_type,
                     st->d_pk_mask, st->d_pk_sens, st->d_mPi, st->d_mVi, st->d_mPw, st->d_mVw,
                     st->d_mic_par, st->d_pk_out, st->d_mic_out, st->d_blk_sub, st->d_blk_dval,
-                    st->d_blk_node, st->d_blk_dnode, st->d_bwgt, st->d_bgrp};
+                    st->d_blk_node, st->d_blk_dnode, st->d_bwgt, st->d_bgrp, st->d_npre};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (st->fdtd) fdtd_destroy(st->fdtd);
@@ -1220,9 +1350,13 @@ int run_block(AcousticCoEvolverState* st, const BlockPlan& bp, int n, bool drive
     }
     const int out0 = st->out_pos;
     const size_t lds = st->lds_bytes;
+    // uniform body: the sweep of each step forms the next sub-step's neck sum
+    const bool side = !st->amr && kc.neck_on && st->d_npre && f->fused_geo &&
+                      (f->family == FDTD_KERNEL_FUSED || f->family == FDTD_KERNEL_FUSED2);
     auto launch = [&](int q, int step, const float* w, float* ext, const float* P,
                       const float* const* V) -> bool {
         CoLaunch a;
+        a.npre = side && q > 0;
         a.sample = q / K;
         a.sub = q % K;
         a.step = step;
@@ -1262,7 +1396,15 @@ int run_block(AcousticCoEvolverState* st, const BlockPlan& bp, int n, bool drive
         };
         int status[2] = {0, 0};
         const int start_step = f->step_count;
+        if (side) {
+            f->side_idx = st->d_nidx;
+            f->side_wgt = st->d_nwgt;
+            f->side_n = st->nn;
+            f->side_out = st->d_npre;
+        }
         const int frc = afdtd_step_hooked(f, nsteps, hook, ms_total, status);
+        f->side_n = 0;
+        f->side_out = nullptr;
         if (frc == FDTD_ERR_UNSTABLE) done_steps = status[1] - start_step;
         else if (frc != FDTD_OK) return CE_ERR_FDTD_STEP;
         ok = frc == FDTD_OK;
@@ -1606,6 +1748,12 @@ const char* co_setup(AcousticCoEvolverState* st, int n_strings, const CoEvolverS
     UP(d_mic_par, mic_par);
     UP(d_bwgt, st->bwgt);
     UP(d_bgrp, bgrp);
+    if (!st->amr) {                       // neck sum slot: +0 and a tag no step has
+        std::vector<float> np0(2, 0.0f);
+        const int none = INT_MIN;
+        memcpy(&np0[1], &none, sizeof(int));
+        UP(d_npre, np0);
+    }
 #undef UP
     {
         std::vector<float> modal0((size_t)2 * kModes * std::max(n_strings, 1), 0.0f);
@@ -1623,6 +1771,17 @@ const char* co_setup(AcousticCoEvolverState* st, int n_strings, const CoEvolverS
     if (st->amr && n_mics > 0) lds_str = std::max(lds_str, (size_t)4 * kThreads + 4 * (size_t)n_mics);
     st->lds_bytes = sizeof(float) * (lds_str + st->nn + kMaxStrings + st->nb + 8 +
                                      3 * (size_t)std::max(n_strings, 1));
+    // deferred output sums: two wavefronts without a string (n_strings <= 13),
+    // one lane per sum (at most 32 magnetic pickups, 16 AMR mics), and LDS
+    // for the products
+    AmrCoView av{};
+    if (st->amr && amr_co_view(st->amr, &av) != SK_OK) return "device_alloc";
+    size_t ndef = (size_t)2 * st->tot_seg * st->nmag;
+    if (st->amr && n_mics > 0) ndef += (size_t)av.n_mp + 3 * (size_t)av.n_mv + 4 * (size_t)n_mics;
+    const bool defer = ndef > 0 && n_strings <= kThreads / 64 - 3 && st->nmag <= 32 &&
+                       !(st->amr && n_mics > 16) &&
+                       st->lds_bytes + sizeof(float) * ndef <= (size_t)kLdsLimit;
+    if (defer) st->lds_bytes += sizeof(float) * ndef;
     if (st->lds_bytes > (size_t)kLdsLimit) {
         afdtd_set_error("coevolver_create: strings, neck and bridge kernels exceed the LDS of one "
                         "workgroup");
@@ -1637,6 +1796,9 @@ const char* co_setup(AcousticCoEvolverState* st, int n_strings, const CoEvolverS
     k.nn = st->nn; k.nngrp = nngrp; k.neck_on = (st->n_m > 0.0f && st->nn > 0) ? 1 : 0;
     k.any_vel = st->any_vel; k.Nplate = nplate; k.clear_ext = st->amr ? 0 : 1;
     k.amr = st->amr ? 1 : 0;
+    k.defer = defer ? 1 : 0;
+    k.anp = av.n_mp;
+    k.anv = av.n_mv;
     k.st_ok = st->amr ? INT_MAX : 0;
     k.lds_str = (int)lds_str;
     float dt_b = st->dt_fdtd;
@@ -1655,9 +1817,9 @@ const char* co_setup(AcousticCoEvolverState* st, int n_strings, const CoEvolverS
     k.mVw = st->d_mVw; k.mic_par = st->d_mic_par; k.pk_out = st->d_pk_out;
     k.mic_out = st->d_mic_out; k.blk_dnode = st->d_blk_dnode;
     k.bwgt = st->d_bwgt;
+    k.npre = st->d_npre;
     if (st->amr) {
-        AmrCoView v;
-        if (amr_co_view(st->amr, &v) != SK_OK) return "device_alloc";
+        const AmrCoView& v = av;
         k.status = v.status;
         k.amp_start = v.mp_start; k.amp_cell = v.mp_cell; k.amp_w = v.mp_w;
         k.amv_start = v.mv_start; k.amv_face = v.mv_face;

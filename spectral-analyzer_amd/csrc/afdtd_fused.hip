@@ -51,7 +51,48 @@ struct FusedArgs {
     GeoPtrs g;            // cell geometry (GEOM build)
     int seg;              // x-segment length
     int nty, ntz, nseg;   // tile counts
+    // side job (GEOM build, side_n > 0): the last workgroup forms a weighted
+    // plate sum instead of a tile
+    const float* side_w;
+    const int* side_idx;
+    const float* side_wgt;
+    int side_n, side_nplate, side_tag;
+    float* side_out;
 };
+
+// The side job: products w[idx[i]] * wgt[i] (0 for an index outside the
+// plate) formed by the workgroup in chunks, summed by one thread in index
+// order from +0; the sum and the tag go to side_out.
+__device__ void side_sum(const FusedArgs& a) {
+    constexpr int CH = 2048;
+    __shared__ float sb[CH];
+    const int tid = threadIdx.x + 64 * threadIdx.y, nt = 64 * blockDim.y;
+    float acc = 0.0f;
+    for (int base = 0; base < a.side_n; base += CH) {
+        const int m = min(CH, a.side_n - base);
+        for (int i = tid; i < m; i += nt) {
+            const int p = a.side_idx[base + i];
+            sb[i] = (p < 0 || p >= a.side_nplate) ? 0.0f : a.side_w[p] * a.side_wgt[base + i];
+        }
+        __syncthreads();
+        if (tid == 0) {
+            int i = 0;
+            for (; i + 8 <= m; i += 8) {
+                float v[8];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) v[q] = sb[i + q];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) acc += v[q];
+            }
+            for (; i < m; ++i) acc += sb[i];
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        a.side_out[0] = acc;
+        ((int*)a.side_out)[1] = a.side_tag;
+    }
+}
 
 // Minimum waves per SIMD the register allocation must allow (occupancy).
 #ifndef AFDTD_FUSED_WPE
@@ -66,7 +107,17 @@ __global__ void __launch_bounds__(64 * NW)
 __attribute__((amdgpu_waves_per_eu(GEOM ? AFDTD_FUSED_GEO_WPE : AFDTD_FUSED_WPE)))
 k_fused(FusedArgs a) {
     if (*a.status) return;
-    int t = xcd_remap(blockIdx.x, gridDim.x);
+    int ntile = gridDim.x;
+    if constexpr (GEOM) {
+        if (a.side_n > 0) {
+            if (blockIdx.x == gridDim.x - 1) {
+                side_sum(a);
+                return;
+            }
+            ntile -= 1;
+        }
+    }
+    int t = xcd_remap(blockIdx.x, ntile);
     const int ty = t % a.nty; t /= a.nty;
     const int tz = t % a.ntz;
     const int sg = t / a.ntz;
@@ -80,7 +131,7 @@ hipError_t launch_t(AcousticFDTDState* st, const FusedArgs& a0) {
     a.nty = (st->Ny + TY - 1) / TY;
     a.ntz = (st->Nz + TZ - 1) / TZ;
     a.nseg = (st->Nx + a.seg - 1) / a.seg;
-    const int nb = a.nty * a.ntz * a.nseg;
+    const int nb = a.nty * a.ntz * a.nseg + (GEOM && a.side_n > 0 ? 1 : 0);
     hipLaunchKernelGGL((k_fused<ORDER, T, HET, TY, NW, GEOM>), dim3(nb), dim3(64, NW), 0,
                        st->stream, a);
     return hipGetLastError();
@@ -148,6 +199,16 @@ hipError_t launch_fused_step(AcousticFDTDState* st) {
         a.g.frac[0] = st->d_fx; a.g.frac[1] = st->d_fy; a.g.frac[2] = st->d_fz;
         a.g.vpl = st->d_vpl;
         a.g.ncells = (uint32_t)st->N;
+    }
+    a.side_n = 0;
+    if (st->fused_geo && st->side_n > 0 && st->side_out) {
+        a.side_w = st->d_w[0];                 // the displacement this step ends with
+        a.side_idx = st->side_idx;
+        a.side_wgt = st->side_wgt;
+        a.side_n = st->side_n;
+        a.side_nplate = st->Nplate;
+        a.side_tag = st->side_step;
+        a.side_out = st->side_out;
     }
     // 8-row tiles for the geometry build: its larger register set leaves
     // three workgroups of 4 waves per CU, against one of 8 waves at 16 rows

@@ -144,6 +144,15 @@ struct AcousticFDTDState {
     int fused_geo;
     uint8_t* d_geo;
     float* d_vpl;           // plate velocity of the current step (nact > 0)
+    // Side job of the geometry sweep (set by a hooked caller, the co-evolver):
+    // one extra workgroup forms sum_i w[idx[i]] * wgt[i] over the plate
+    // displacement the sweep's step ends with, in index order from +0, and
+    // stores it with the step index as tag.
+    const int* side_idx;
+    const float* side_wgt;
+    int side_n;
+    float* side_out;        // device: [0] the sum, [1] the step index (int bits)
+    int side_step;          // step index of the sweep being enqueued
 
     // Two-step sweep: host copies of source cells and receiver taps, and the
     // per-tile tables built from them (rebuilt when f2_dirty).
@@ -199,6 +208,7 @@ struct AmrCoView {
     hipStream_t stream;
     int* status;
     int n_cells, n_pnodes, pNx, pNy, n_mics;
+    int n_mp, n_mv;                         // stencil entries: pressure cells, velocity faces
     const int* mp_start;
     const int* mp_cell;
     const float* mp_w;
