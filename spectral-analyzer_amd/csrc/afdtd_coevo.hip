@@ -75,6 +75,8 @@ struct CoConst {
     int amr;                                // body: 0 uniform FDTD, 1 AMR stepper
     int defer;                              // output sums run on idle wavefronts (see k_coevo)
     const float* npre;                      // neck plate sum formed by the body sweep: sum, tag
+    int sad_early;                          // saddle gather in the run phase (n_strings <= 12)
+    int sbk_lds;                            // per-string bridge kernels staged in LDS
     int anp, anv;                           // AMR mic stencil entries: pressure, velocity
     int st_ok;                              // status[0] when nothing failed (0 / INT_MAX)
     int lds_str;                            // LDS words before the neck products
@@ -269,7 +271,11 @@ __global__ __launch_bounds__(kThreads) void k_coevo(CoConst c, CoLaunch a) {
     float* fsT = (float*)(flag + 8);
     float* fsU = fsT + c.ns;
     float* teff0 = fsU + c.ns;                 // T_eff of each string's current sub-step
-    float* dprod = teff0 + c.ns;               // deferred output products (c.defer)
+    float* swl = teff0 + c.ns;                 // [ns][kSw] state words for this launch
+    float* psub = swl + c.ns * kSw;            // [ns] this sample's per-sub-step force
+    int* pnode = (int*)(psub + c.ns);          // [ns] its node
+    float* sbkl = (float*)(pnode + c.ns);      // [ns][nb] bridge kernels (c.sbk_lds)
+    float* dprod = sbkl + (c.sbk_lds ? c.ns * c.nb : 0);   // deferred output products (c.defer)
     if (tid == 0) flag[0] = 0;
     // At a sample's end the products of the pickup GEMV rows (and of the AMR
     // mic stencils) are formed here, but their one-lane sums run in the run
@@ -279,7 +285,7 @@ __global__ __launch_bounds__(kThreads) void k_coevo(CoConst c, CoLaunch a) {
 
     // ── finish the previous sub-step (:2105-2125, and :2129-2223 at a sample's end) ──
     if (a.post) {
-        if (tid < c.ns) {
+        if (tid < c.ns && !(c.sad_early && a.run)) {
             const int* id = c.sad_idx + 4 * tid;
             const float* g = c.sad_wgt + 4 * tid;
             // the AMR sampler's sum starts from +0 (acoustic_amr.cpp:1600-1616)
@@ -527,6 +533,12 @@ __global__ __launch_bounds__(kThreads) void k_coevo(CoConst c, CoLaunch a) {
         for (int si = wave; si < c.ns; si += SW) {
             const CoStr s = c.str[si];
             const int N = s.n_nodes;
+            // this launch's copies of the string's state words and force
+            // (the saddle word comes from the early gather below when it runs)
+            if (lane < kSw && !(lane == kSad && c.sad_early && a.post))
+                swl[si * kSw + lane] = c.sw[si * kSw + lane];
+            if (lane == kSw) psub[si] = c.blk_sub[a.sample * c.ns + si];
+            if (lane == kSw + 1) pnode[si] = c.blk_node[a.sample * c.ns + si];
             float* base = lds + 8 * s.node_off;
             float* fe0 = base + 6 * N;
             float* scr = base + 7 * N;
@@ -554,6 +566,22 @@ __global__ __launch_bounds__(kThreads) void k_coevo(CoConst c, CoLaunch a) {
                 tb = tension_of(base, base + 3 * N, scr, N, s, lane, &teff0[si]);
             }
             if (tb && lane == 0) fail_here(c, a, &flag[0]);
+        }
+        // wavefronts without a string: the bridge kernels into LDS, and (n_strings
+        // <= 12) the previous sub-step's saddle gather (:2105-2125), which only
+        // the bridge phase reads
+        if (c.sbk_lds && wave >= c.ns)
+            for (int i = (wave - c.ns) * 64 + lane; i < c.ns * c.nb; i += (SW - c.ns) * 64)
+                sbkl[i] = c.sbk[i];
+        if (c.sad_early && a.post && wave == SW - 3) {
+            for (int si = lane; si < c.ns; si += 64) {
+                const int* id = c.sad_idx + 4 * si;
+                const float* g = c.sad_wgt + 4 * si;
+                const float t0 = c.amr ? 0.0f + g[0] * a.w[id[0]] : g[0] * a.w[id[0]];
+                const float sad = t0 + g[1] * a.w[id[1]] + g[2] * a.w[id[2]] + g[3] * a.w[id[3]];
+                c.sw[si * kSw + kSad] = sad;
+                swl[si * kSw + kSad] = sad;
+            }
         }
         if (a.sample_end && c.defer && wave == SW - 1 && c.nmag > 0) {
             // pickup rows (:2152-2171): lane 2r sums row r's K0 products, lane
@@ -614,9 +642,9 @@ __global__ __launch_bounds__(kThreads) void k_coevo(CoConst c, CoLaunch a) {
         float* scr = base + 7 * N;
         int r0u = 0, r0p = 1, r0t = 2, r1u = 3, r1p = 4, r1t = 5;
         float Teff = teff0[si];
-        const float bridge_w = c.sw[si * kSw + kBr];
-        const float per_sub = c.blk_sub[a.sample * c.ns + si];
-        const int fnode = c.blk_node[a.sample * c.ns + si];
+        const float bridge_w = swl[si * kSw + kBr];
+        const float per_sub = psub[si];
+        const int fnode = pnode[si];
         bool failed = false;
         float arc = 0.0f;                       // lane 0: arc length of the checked shape
         for (int q = 0; q < s.n_sub && !failed; ++q) {
@@ -696,7 +724,8 @@ __global__ __launch_bounds__(kThreads) void k_coevo(CoConst c, CoLaunch a) {
         if (tid < c.ns) {
             const CoStr s = c.str[tid];
             float* w = c.sw + tid * kSw;
-            const float br = w[kBr], brp = w[kBrP], sad = w[kSad], sadp = w[kSadP];
+            const float* wl = swl + tid * kSw;
+            const float br = wl[kBr], brp = wl[kBrP], sad = wl[kSad], sadp = wl[kSadP];
             const float sl = (fsU[tid] - br) / s.ds;
             bool bad = !__builtin_isfinite(sl) || fabsf(sl) > 1.0f;
             if (!bad) {
@@ -725,7 +754,8 @@ __global__ __launch_bounds__(kThreads) void k_coevo(CoConst c, CoLaunch a) {
         if (flag[0]) return;
         for (int cix = tid; cix < c.nb; cix += kThreads) {
             float acc = 0.0f;
-            for (int si = 0; si < c.ns; ++si) acc += c.sbk[(size_t)si * c.nb + cix] * fpl[si];
+            const float* kb = c.sbk_lds ? sbkl : c.sbk;
+            for (int si = 0; si < c.ns; ++si) acc += kb[(size_t)si * c.nb + cix] * fpl[si];
             drive[cix] = acc;
         }
         __syncthreads();
@@ -1770,7 +1800,11 @@ const char* co_setup(AcousticCoEvolverState* st, int n_strings, const CoEvolverS
     size_t lds_str = (size_t)8 * st->tot_nodes;
     if (st->amr && n_mics > 0) lds_str = std::max(lds_str, (size_t)4 * kThreads + 4 * (size_t)n_mics);
     st->lds_bytes = sizeof(float) * (lds_str + st->nn + kMaxStrings + st->nb + 8 +
-                                     3 * (size_t)std::max(n_strings, 1));
+                                     (3 + kSw + 2) * (size_t)std::max(n_strings, 1));
+    // the per-string bridge kernels in LDS when they fit beside the rest
+    const size_t nsbk = (size_t)n_strings * st->nb;
+    const bool sbk_lds = nsbk > 0 && st->lds_bytes + sizeof(float) * nsbk + 16384 <= (size_t)kLdsLimit;
+    if (sbk_lds) st->lds_bytes += sizeof(float) * nsbk;
     // deferred output sums: two wavefronts without a string (n_strings <= 13),
     // one lane per sum (at most 32 magnetic pickups, 16 AMR mics), and LDS
     // for the products
@@ -1797,6 +1831,8 @@ const char* co_setup(AcousticCoEvolverState* st, int n_strings, const CoEvolverS
     k.any_vel = st->any_vel; k.Nplate = nplate; k.clear_ext = st->amr ? 0 : 1;
     k.amr = st->amr ? 1 : 0;
     k.defer = defer ? 1 : 0;
+    k.sad_early = n_strings <= kThreads / 64 - 4 ? 1 : 0;
+    k.sbk_lds = sbk_lds ? 1 : 0;
     k.anp = av.n_mp;
     k.anv = av.n_mv;
     k.st_ok = st->amr ? INT_MAX : 0;
