@@ -268,6 +268,10 @@ SK_API int fdtd_set_state(AcousticFDTDState* st, const float* P, const float* vx
 
 /* Introspection: which kernel family runs, dt, stencil order. */
 SK_API int fdtd_get_kernel_family(const AcousticFDTDState* st);
+/* Extension (no counterpart in the reference): steps one sweep advances, 2
+ * for the two-step sweeps (rooms, and order-4 FP32 grids with walls, face
+ * fractions or an active plate), else 1. */
+SK_API int fdtd_get_sweep_steps(const AcousticFDTDState* st);
 SK_API const char* fdtd_last_error(void);
 
 /* Timing helper: run n_steps with HIP events around the step kernels and

@@ -67,6 +67,7 @@ _SIGS = {
     "fdtd_get_step_count": (_i, [_vp]),
     "fdtd_get_dt": (_f, [_vp]),
     "fdtd_get_kernel_family": (_i, [_vp]),
+    "fdtd_get_sweep_steps": (_i, [_vp]),
     "fdtd_reset": (_i, [_vp]),
     "fdtd_get_pressure_field": (_i, [_vp, _vp, _i]),
     "fdtd_get_plate_displacement": (_i, [_vp, _vp, _i]),

@@ -265,6 +265,12 @@ class AcousticFDTD:
         return {1: "generic", 2: "fused", 3: "fused2"}.get(
             self._L.fdtd_get_kernel_family(self._h), "?")
 
+    @property
+    def steps_per_sweep(self) -> int:
+        """Steps one sweep advances: 2 on the two-step sweeps (a room, or an
+        order-4 FP32 grid with walls, face fractions or an active plate)."""
+        return int(self._L.fdtd_get_sweep_steps(self._h))
+
     def set_point_sources(self, cell_idx, signals):
         idx = np.ascontiguousarray(np.asarray(cell_idx, np.int32).ravel())
         sig = np.ascontiguousarray(np.asarray(signals, np.float32).reshape(idx.size, -1))

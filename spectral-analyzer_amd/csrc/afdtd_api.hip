@@ -341,6 +341,7 @@ void destroy_state(AcousticFDTDState* st) {
     dfree(st->stage);
     dfree(st->d_ct); dfree(st->d_fx); dfree(st->d_fy); dfree(st->d_fz);
     dfree(st->d_geo); dfree(st->d_vpl);
+    dfree(st->d_g2); dfree(st->d_tw); dfree(st->d_ppre); dfree(st->d_pab);
     dfree(st->d_bu); dfree(st->d_kap); dfree(st->d_xi);
     dfree(st->d_alpha); dfree(st->d_pdamp); dfree(st->d_status);
     for (int a = 0; a < 3; ++a) dfree(st->d_w[a]);
@@ -635,7 +636,14 @@ AcousticFDTDState* create_impl(int Nx, int Ny, int Nz, float dx, float c, float 
     st->fused_seg_x = o->seg_x;
     st->family = FDTD_KERNEL_GENERIC;
     st->f2_dirty = 1;
-    if (o->kernel == FDTD_KERNEL_FUSED2) {
+    st->kernel_opt = o->kernel;
+    if (o->kernel == FDTD_KERNEL_FUSED2 && !fused2_supported(st) && fused4_geo_supported(st)) {
+        // walls, fractions or an active plate on the order-4 FP32 grid: the
+        // two-step sweep with cell geometry
+        st->family = FDTD_KERNEL_FUSED;
+        st->fused_geo = 1;
+        st->geo_pair = 1;
+    } else if (o->kernel == FDTD_KERNEL_FUSED2) {
         if (!fused2_supported(st)) {
             afdtd_set_error("two-step sweep needs no walls, plate cells, face fractions or "
                             "active plate (and for the order-4 stencil FP32 storage and a "
@@ -657,10 +665,12 @@ AcousticFDTDState* create_impl(int Nx, int Ny, int Nz, float dx, float c, float 
         if (fused_supported(st)) {
             st->family = FDTD_KERNEL_FUSED;
         } else if (fused_geo_supported(st)) {
-            // walls, plate cells, fractions or an active plate: the single-step
-            // sweep with per-cell geometry codes
+            // walls, plate cells, fractions or an active plate: the sweep with
+            // per-cell geometry codes, two steps per pass for the order-4
+            // stencil unless the single-step sweep is asked for
             st->family = FDTD_KERNEL_FUSED;
             st->fused_geo = 1;
+            st->geo_pair = o->kernel == FDTD_KERNEL_AUTO && fused4_geo_supported(st);
         } else if (o->kernel == FDTD_KERNEL_FUSED) {
             afdtd_set_error("fused kernel does not support this configuration");
             destroy_state(st);
@@ -680,6 +690,7 @@ AcousticFDTDState* create_impl(int Nx, int Ny, int Nz, float dx, float c, float 
             TRY(hipMemsetAsync(st->V2[a], 0, st->fbytes, st->stream));
         }
         if (st->fused_geo) TRY(launch_geo_codes(st));
+        if (st->geo_pair) TRY(launch_geo_pair_codes(st));
         TRY(hipStreamSynchronize(st->stream));
     }
     return st;
@@ -787,6 +798,31 @@ int enqueue_pair(AcousticFDTDState* st, int s, hipEvent_t ea, hipEvent_t eb) {
     return FDTD_OK;
 }
 
+// Enqueue steps s and s+1 as one two-step sweep with cell geometry (the
+// caller schedules pairs away from stability-check steps, hooks and side
+// jobs).  Step s's plate kernel runs first, as in enqueue_step; step s+1's
+// plate update runs inside the pair launch.
+int enqueue_geo_pair(AcousticFDTDState* st, int s, hipEvent_t ea, hipEvent_t eb) {
+    if (st->nact > 0) {
+        AFDTD_CHECK(launch_plate_step(st, s));
+        AFDTD_CHECK(launch_fill(st->d_ext, st->Nplate, 0.0f, st->stream));
+        float* t = st->d_w[2]; st->d_w[2] = st->d_w[1]; st->d_w[1] = st->d_w[0]; st->d_w[0] = t;
+    }
+    int row0 = -1, row1 = -1;
+    if (st->d_trace && st->trace_n < st->trace_cap) row0 = st->trace_n++;
+    if (st->d_trace && st->trace_n < st->trace_cap) row1 = st->trace_n++;
+    if (ea) AFDTD_CHECK(hipEventRecord(ea, st->stream));
+    AFDTD_CHECK(launch_fused4_geo_pair(st, s, row0));
+    if (eb) AFDTD_CHECK(hipEventRecord(eb, st->stream));
+    st->cur ^= 1;
+    AFDTD_CHECK(launch_post_step(st, s + 1, row1));
+    if (((s + 2) & 0xFF) == 0) {
+        AFDTD_CHECK(launch_stability_check(st, st->phase, s + 1));
+        st->phase = (st->phase + 1) % 1013;
+    }
+    return FDTD_OK;
+}
+
 int step_impl(AcousticFDTDState* st, int n_steps, float* ms_total, float* ms_stencil,
               const StepHook* hook = nullptr, int* status_out = nullptr) {
     const int start_step = st->step_count, start_cur = st->cur;
@@ -802,7 +838,7 @@ int step_impl(AcousticFDTDState* st, int n_steps, float* ms_total, float* ms_ste
     if (ms_total) AFDTD_CHECK(hipEventRecord(st->ev0, st->stream));
     std::vector<uint8_t> par((size_t)n_steps, 0);
     int launches = 0;
-    if (st->family == FDTD_KERNEL_FUSED2) {
+    if (st->family == FDTD_KERNEL_FUSED2 || (st->geo_pair && !hook)) {
         int rc = fused2_prepare(st);
         if (rc) return rc;
     }
@@ -814,13 +850,16 @@ int step_impl(AcousticFDTDState* st, int n_steps, float* ms_total, float* ms_ste
             int rc = (*hook)(q, s);
             if (rc) return rc;
         }
-        const bool pair = !hook && st->family == FDTD_KERNEL_FUSED2 && st->f2_ok &&
-                          q + 1 < n_steps && ((s + 1) & 0xFF) != 0 && st->nact == 0;
+        const bool pairable = !hook && st->f2_ok && q + 1 < n_steps && ((s + 1) & 0xFF) != 0;
+        const bool pair = pairable && st->family == FDTD_KERNEL_FUSED2 && st->nact == 0;
+        const bool gpair = pairable && st->geo_pair && st->side_n == 0;
         const bool keep_ext = hook && st->hook_clears_ext && q + 1 < n_steps;
-        int rc = pair ? enqueue_pair(st, s, ea, eb) : enqueue_step(st, s, ea, eb, keep_ext);
+        int rc = gpair ? enqueue_geo_pair(st, s, ea, eb)
+                 : pair ? enqueue_pair(st, s, ea, eb)
+                        : enqueue_step(st, s, ea, eb, keep_ext);
         if (rc) return rc;
         ++launches;
-        const int adv = pair ? 2 : 1;
+        const int adv = (pair || gpair) ? 2 : 1;
         for (int u = 0; u < adv; ++u) par[q + u] = (uint8_t)(launches & 1);
         q += adv;
     }
@@ -985,12 +1024,14 @@ SK_API int fdtd_set_face_fractions(AcousticFDTDState* st, const float* fx, const
         if (fused_geo_supported(st)) {
             st->family = FDTD_KERNEL_FUSED;
             st->fused_geo = 1;
+            st->geo_pair = st->kernel_opt != FDTD_KERNEL_FUSED && fused4_geo_supported(st);
         } else {
             st->family = FDTD_KERNEL_GENERIC;
         }
     }
     if (st->fused_geo) {
         AFDTD_CHECK(launch_geo_codes(st));
+        if (st->geo_pair) AFDTD_CHECK(launch_geo_pair_codes(st));
         return sync(st);
     }
     return FDTD_OK;
@@ -1111,6 +1152,11 @@ SK_API int fdtd_step_timed(AcousticFDTDState* st, int n_steps, float* ms_total,
 SK_API int fdtd_get_step_count(const AcousticFDTDState* st) { return st ? st->step_count : 0; }
 SK_API float fdtd_get_dt(const AcousticFDTDState* st) { return st ? st->dt : 0.0f; }
 SK_API int fdtd_get_kernel_family(const AcousticFDTDState* st) { return st ? st->family : 0; }
+
+SK_API int fdtd_get_sweep_steps(const AcousticFDTDState* st) {
+    if (!st) return 0;
+    return (st->family == FDTD_KERNEL_FUSED2 || st->geo_pair) ? 2 : 1;
+}
 
 SK_API int fdtd_reset(AcousticFDTDState* st) {
     if (!st) return FDTD_ERR_NULL;                                      // :1297-1310

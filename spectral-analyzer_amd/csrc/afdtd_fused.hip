@@ -58,6 +58,7 @@ struct FusedArgs {
     const float* side_wgt;
     int side_n, side_nplate, side_tag;
     float* side_out;
+    int run_on;           // the sweep runs while the status word equals this
 };
 
 // The side job: products w[idx[i]] * wgt[i] (0 for an index outside the
@@ -106,7 +107,7 @@ template <int ORDER, typename T, bool HET, int TY, int NW, bool GEOM>
 __global__ void __launch_bounds__(64 * NW)
 __attribute__((amdgpu_waves_per_eu(GEOM ? AFDTD_FUSED_GEO_WPE : AFDTD_FUSED_WPE)))
 k_fused(FusedArgs a) {
-    if (*a.status) return;
+    if (*a.status != a.run_on) return;
     int ntile = gridDim.x;
     if constexpr (GEOM) {
         if (a.side_n > 0) {
@@ -176,9 +177,12 @@ bool fused_geo_supported(const AcousticFDTDState* st) {
            (size_t)st->N * sizeof(float) < 0x80000000u;
 }
 
-hipError_t launch_fused_step(AcousticFDTDState* st) {
+hipError_t launch_fused_step(AcousticFDTDState* st) { return launch_fused_step_if(st, 0); }
+
+hipError_t launch_fused_step_if(AcousticFDTDState* st, int run_on) {
     if (!st->V2[0]) return hipErrorInvalidValue;
     FusedArgs a;
+    a.run_on = run_on;
     a.p = st->sp;
     a.f.Pin = st->P[st->cur];
     a.f.Pout = st->P[st->cur ^ 1];

@@ -98,6 +98,15 @@ __device__ __forceinline__ float grad4(float pm, float pl, float pr, float pp, b
                                        float C1, float C2) {
     return inner ? C1 * (pr - pl) + C2 * (pp - pm) : pr - pl;
 }
+__device__ __forceinline__ uint32_t bld8(Rsrc r, uint32_t off) {
+    return (uint32_t)__builtin_amdgcn_raw_buffer_load_b8(r, off, 0, 0);
+}
+// Pair code of a cell (GEOM build): bit 0 wall / plate cell (pressure kept),
+// bits 1..3 closed x / y / z face, bit 7 plate z face (as the single-step
+// code), bit 4 some divergence tap of the cell is fraction-weighted, bit 5
+// the cell is above or below an active plate node.
+constexpr uint32_t G2_TAPCUT = 16u, G2_PLNB = 32u;
+constexpr int PLS = TY * 5;                   // plate ring: 5 terms per tile row and plane
 }  // namespace f4
 
 struct Pair4Args {
@@ -115,12 +124,27 @@ struct Pair4Args {
     int plen;
     int step;               // global index of the first of the two steps
     float* tapbuf;
+    // cell geometry (GEOM build)
+    const uint8_t* g2;      // per-cell pair code (G2_* bits)
+    const float* tw;        // [3][N] divergence-tap weights: the face fraction where the
+                            // tap is weighted, else exactly 1
+    const float* vpl;       // plate velocity of step n per plate node (i*Ny + j)
+    const float* pre;       // [5][Nplate] step n+1's plate terms: T1, c3, c4, fb, w
+    float* pab;             // [Nplate][2] P^{n+1} above / below each active node
+    int Nplate, piz, plate_on;
+    float pa, inv_dt;       // plate coefficient a, 1/dt
 };
 
-template <bool PML>
+template <bool PML, bool GEOM>
 __device__ __forceinline__ void fused4_tile(const Pair4Args& a, int ty, int tz, int sg,
                                             int tile) {
     using namespace f4;
+    using fused::G_SOLID;
+    using fused::G_CLOSED;
+    using fused::G_CUT;
+    using fused::G_PLATE;
+    __shared__ float sPl[GEOM ? 8 * PLS : 1];   // step n+1's plate terms, planes mod 8
+    __shared__ float sVp[GEOM ? 2 * ER : 1];    // step n's plate velocity, rows -3..26, planes mod 2
     __shared__ float sP[2 * PLN];
     __shared__ float sAy[4 * AYN];
     __shared__ float sAz[4 * AZN];
@@ -303,6 +327,55 @@ __device__ __forceinline__ void fused4_tile(const Pair4Args& a, int ty, int tz, 
         return PML && (q < npml + 1 || q >= Nx - npml - 1);
     };
 
+    // ── cell geometry (GEOM) ──
+    // gh[s]: pair codes of the slot cell, byte d = plane q-d at iteration q;
+    // gn[s] / gmn: the codes of plane q+1 in flight.
+    uint64_t gh[3] = {0, 0, 0};
+    uint32_t gn[3] = {0, 0, 0}, gmc = 0, gmn = 0;
+    float plv = 0.0f, vpv = 0.0f;
+    // Plate loader lanes, for tiles whose z faces reach the plate's two face
+    // columns: lanes 0..9 of wave w fetch step n+1's five terms of the rows
+    // 2w, 2w+1 (stage B), lanes 10..12 of waves 0..9 step n's plate velocity
+    // of the rows -3+3w .. -1+3w (stage A), for the next plane.
+    const bool plTile = GEOM && a.plate_on && a.piz >= k0 - 5 && a.piz - 1 <= k0 + TZ + 3;
+    const int plRow = 2 * w + lane / 5, plComp = lane % 5;
+    const bool plLane = plTile && lane < 10 && j0 + plRow < Ny;
+    const int vpRow = -H + 3 * w + (lane - 10);
+    const bool vpLane = plTile && w < 10 && lane >= 10 && lane < 13 && j0 + vpRow >= 0 &&
+                        j0 + vpRow < Ny;
+    // tap weight of face (axis, cell); 0 outside the grid, where the tap is
+    // +0 and +0 * 0 = +0 leaves it as it is
+    const uint32_t twb = GEOM ? (uint32_t)Nx * (uint32_t)sx * 4u : 0u;
+    const Rsrc TW0 = rsrc(a.tw, twb), TW1 = rsrc(a.tw + (GEOM ? (size_t)Nx * sx : 0), twb),
+               TW2 = rsrc(a.tw + (GEOM ? 2 * (size_t)Nx * sx : 0), twb);
+    auto twt = [&](int ax, int i, int j, int k) __attribute__((always_inline)) {
+        const uint32_t o = (i >= 0 && i < Nx && j >= 0 && j < Ny && k >= 0 && k < Nz)
+                               ? (uint32_t)(i * sx + j * Nz + k) * 4u : OOB;
+        return bld(ax == 0 ? TW0 : ax == 1 ? TW1 : TW2, o);
+    };
+    // a tap is weighted where its weight is not exactly 1 (the single-step
+    // sweep's rule: the fraction bit is clear exactly where the weight is 1)
+    auto wtap = [](float v, float wgt) __attribute__((always_inline)) {
+        return __float_as_uint(wgt) != 0x3F800000u ? v * wgt : v;
+    };
+    // divergence of cell (i, j, k) whose taps include weighted faces (cold
+    // path); taps x+, x-, x++, x--, then y, then z
+    auto cut_dv = [&](int i, int j, int k, float xp, float xm, float xpp, float xmm, float yp,
+                      float ym, float ypp, float ymm, float zp, float zm, float zpp,
+                      float zmm) __attribute__((always_inline)) {
+        const float dx1 = wtap(xp, twt(0, i, j, k)) - wtap(xm, twt(0, i - 1, j, k));
+        const float dx2 = wtap(xpp, twt(0, i + 1, j, k)) - wtap(xmm, twt(0, i - 2, j, k));
+        const float dy1 = wtap(yp, twt(1, i, j, k)) - wtap(ym, twt(1, i, j - 1, k));
+        const float dy2 = wtap(ypp, twt(1, i, j + 1, k)) - wtap(ymm, twt(1, i, j - 2, k));
+        const float dz1 = wtap(zp, twt(2, i, j, k)) - wtap(zm, twt(2, i, j, k - 1));
+        const float dz2 = wtap(zpp, twt(2, i, j, k + 1)) - wtap(zmm, twt(2, i, j, k - 2));
+        return C1 * ((dx1 + dy1) + dz1) + C2 * ((dx2 + dy2) + dz2);
+    };
+    auto rG = [&](int pl) __attribute__((always_inline)) {
+        return (pl >= 0 && pl < Nx) ? rsrc(a.g2 + (size_t)pl * sx, (uint32_t)sx) : rsrc(a.g2, 0u);
+    };
+
+
     const int qf = i0 - 5;              // first plane of the stage-A faces
     // prologue: P^n of planes qf-1..qf+2 at the slots, plane qf into LDS,
     // V^{n-1/2} of plane qf
@@ -331,6 +404,14 @@ __device__ __forceinline__ void fused4_tile(const Pair4Args& a, int ty, int tz, 
             ovz[s] = bld(Z, off[s]);
         }
         ovm = bld(w < 4 ? Y : Z, moff);
+        if constexpr (GEOM) {
+            const Rsrc g = rG(qf);
+#pragma unroll
+            for (int s = 0; s < 3; ++s) gh[s] = bld8(g, off[s] >> 2);
+            gmc = bld8(g, moff >> 2);
+            if (vpLane && qf >= 0 && qf < Nx)
+                sVp[(qf & 1) * ER + vpRow + H] = a.vpl[(size_t)qf * Ny + j0 + vpRow];
+        }
     }
     __syncthreads();
     // cursors into the cached lists and the key each points at
@@ -351,6 +432,17 @@ __device__ __forceinline__ void fused4_tile(const Pair4Args& a, int ty, int tz, 
 #pragma unroll
             for (int s = 0; s < 3; ++s) pnl[s] = bld(r3, off[s]);
         }
+        if constexpr (GEOM) {
+            // pair codes of plane q+1; step n+1's plate terms of plane q+1
+            const Rsrc g = rG(q + 1);
+#pragma unroll
+            for (int s = 0; s < 3; ++s) gn[s] = bld8(g, off[s] >> 2);
+            gmn = bld8(g, moff >> 2);
+            if (plLane && q + 1 >= 0 && q + 1 < Nx)
+                plv = a.pre[(size_t)plComp * a.Nplate + (size_t)(q + 1) * Ny + j0 + plRow];
+            if (vpLane && q + 1 >= 0 && q + 1 < Nx)
+                vpv = a.vpl[(size_t)(q + 1) * Ny + j0 + vpRow];
+        }
 
         // ════ stage-A faces of plane q ════
         {
@@ -369,17 +461,25 @@ __device__ __forceinline__ void fused4_tile(const Pair4Args& a, int ty, int tz, 
                 // x-face between planes q and q+1
                 float nv = ovx[s] - vcoef * grad4(pm, pl, pn[s][M2], pn[s][M3], xin, C1, C2);
                 if (PML) nv = nv * (xd ? damp(max(dq, dyz0[s]), max(dq1, dyz0[s])) : mX[s]);
-                ax[s][N0] = (fvx && cellv[s]) ? nv : 0.0f;
+                const uint32_t g0 = GEOM ? (uint32_t)gh[s] & 0xFFu : 0u;   // plane q
+                float nxv = (fvx && cellv[s]) ? nv : 0.0f;
+                if (GEOM && (g0 & G_CLOSED)) nxv = 0.0f;
+                ax[s][N0] = nxv;
                 // y-face between rows e and e+1
                 nv = ovy[s] - vcoef * grad4(Pq[gix(e - 1, c)], pl, Pq[gix(e + 1, c)],
                                             Pq[gix(e + 2, c)], yin[s], C1, C2);
                 if (PML) nv = nv * (xd ? damp(max(dq, dyz0[s]), max(dq, dyzY[s])) : mY[s]);
-                const float ny = (qv && vyv[s]) ? nv : 0.0f;
+                float ny = (qv && vyv[s]) ? nv : 0.0f;
+                if (GEOM && (g0 & (G_CLOSED << 1))) ny = 0.0f;
                 // z-face between columns c and c+1
                 nv = ovz[s] - vcoef * grad4(Pq[gix(e, c - 1)], pl, Pq[gix(e, c + 1)],
                                             Pq[gix(e, c + 2)], zin[s], C1, C2);
                 if (PML) nv = nv * (xd ? damp(max(dq, dyz0[s]), max(dq, dyzZ[s])) : mZ[s]);
-                const float nz = (qv && vzv[s]) ? nv : 0.0f;
+                float nz = (qv && vzv[s]) ? nv : 0.0f;
+                if constexpr (GEOM) {
+                    if (g0 & (G_CLOSED << 2)) nz = 0.0f;
+                    if (g0 & G_PLATE) nz = sVp[(q & 1) * ER + e + H];
+                }
                 if (sact[s]) {
                     Ayw[ayix(e, c)] = ny;
                     Azw[azix(e, c)] = nz;
@@ -389,16 +489,24 @@ __device__ __forceinline__ void fused4_tile(const Pair4Args& a, int ty, int tz, 
                 float nv = ovm - vcoef * grad4(Pq[gix(mf - 1, mc)], Pq[gix(mf, mc)],
                                                Pq[gix(mf + 1, mc)], Pq[gix(mf + 2, mc)], min_, C1, C2);
                 if (PML) nv = nv * (xd ? damp(max(dq, mdL), max(dq, mdR)) : mM);
-                if (mact) Ayw[ayix(mf, mc)] = (qv && mval) ? nv : 0.0f;
+                nv = (qv && mval) ? nv : 0.0f;
+                if (GEOM && (gmc & (G_CLOSED << 1))) nv = 0.0f;
+                if (mact) Ayw[ayix(mf, mc)] = nv;
             } else if (w < 6) {     // extra z-face (mc | mc+1) at row mf
                 float nv = ovm - vcoef * grad4(Pq[gix(mf, mc - 1)], Pq[gix(mf, mc)],
                                                Pq[gix(mf, mc + 1)], Pq[gix(mf, mc + 2)], min_, C1, C2);
                 if (PML) nv = nv * (xd ? damp(max(dq, mdL), max(dq, mdR)) : mM);
-                if (mact) Azw[azix(mf, mc)] = (qv && mval) ? nv : 0.0f;
+                nv = (qv && mval) ? nv : 0.0f;
+                if constexpr (GEOM) {
+                    if (gmc & (G_CLOSED << 2)) nv = 0.0f;
+                    if (gmc & G_PLATE) nv = sVp[(q & 1) * ER + mf + H];
+                }
+                if (mact) Azw[azix(mf, mc)] = nv;
             }
         }
-        // V^{n-1/2} of plane q+1 (consumed next iteration)
-        {
+        // V^{n-1/2} of plane q+1 (consumed next iteration; with geometry after
+        // the stage-A pressure, whose weighted-tap path needs the registers)
+        auto load_old_v = [&]() __attribute__((always_inline)) {
             const Rsrc X = rP(Vxi, q + 1), Y = rP(Vyi, q + 1), Z = rP(Vzi, q + 1);
 #pragma unroll
             for (int s = 0; s < 3; ++s) {
@@ -408,7 +516,8 @@ __device__ __forceinline__ void fused4_tile(const Pair4Args& a, int ty, int tz, 
                 ovz[s] = bld(Z, off[s]);
             }
             if (w < 6) ovm = bld(w < 4 ? Y : Z, moff);
-        }
+        };
+        if constexpr (!GEOM) load_old_v();
 
         // ════ stage-A pressure of plane qa = q-1 ════
         if (qa >= i0 - 3) {
@@ -429,10 +538,14 @@ __device__ __forceinline__ void fused4_tile(const Pair4Args& a, int ty, int tz, 
                 const float ypp = Ayr[ayix(e + 1, c)], ymm = Ayr[ayix(e - 2, c)];
                 const float zp = Azr[azix(e, c)], zm = Azr[azix(e, c - 1)];
                 const float zpp = Azr[azix(e, c + 1)], zmm = Azr[azix(e, c - 2)];
-                const float dv = C1 * ((xp - xm) + (yp - ym) + (zp - zm))
-                               + C2 * ((xpp - xmm) + (ypp - ymm) + (zpp - zmm));
+                float dv = C1 * ((xp - xm) + (yp - ym) + (zp - zm))
+                         + C2 * ((xpp - xmm) + (ypp - ymm) + (zpp - zmm));
+                const uint32_t g1 = GEOM ? (uint32_t)(gh[s] >> 8) & 0xFFu : 0u;   // plane qa
+                if (GEOM && (g1 & G2_TAPCUT))
+                    dv = cut_dv(qa, j0 + e, k0 + c, xp, xm, xpp, xmm, yp, ym, ypp, ymm, zp, zm, zpp, zmm);
                 float out = pn[s][M0] - kap0 * dv;
                 if (PML) out = out * (xd ? sPd[max(dqa, dyz0[s])] : mP[s]);
+                if (GEOM && (g1 & G_SOLID)) out = pn[s][M0];          // walls keep P
                 pnew[s] = (qav && cellv[s]) ? out : 0.0f;
             }
             // point sources of step n+1, in source order
@@ -450,6 +563,18 @@ __device__ __forceinline__ void fused4_tile(const Pair4Args& a, int ty, int tz, 
 #pragma unroll
             for (int s = 0; s < 3; ++s)
                 if (sact[s]) P1w[p1ix(se[s], sc[s])] = pnew[s];
+            // P^{n+1} beside the plate, for the host-side plate update of step n+1
+            if constexpr (GEOM) {
+                if (qa >= i0 && qa < i1) {
+#pragma unroll
+                    for (int s = 0; s < 2; ++s) {
+                        if (((uint32_t)(gh[s] >> 8) & G2_PLNB) && cellv[s]) {
+                            const size_t pn_ = (size_t)qa * Ny + j0 + se[s];
+                            a.pab[2 * pn_ + (k0 + sc[s] == a.piz + 1 ? 0 : 1)] = pnew[s];
+                        }
+                    }
+                }
+            }
             // receiver taps of step n+1 (own planes and cells)
             while ((tkey >> 12) <= qa) {
                 if ((tkey >> 12) == qa) {
@@ -461,6 +586,8 @@ __device__ __forceinline__ void fused4_tile(const Pair4Args& a, int ty, int tz, 
                 tkey = tcur < nt ? __builtin_amdgcn_readfirstlane(sTapKey[tcur]) : 0x7FFFFFFF;
             }
         }
+
+        if constexpr (GEOM) load_old_v();
 
         // ════ stage-B faces of plane qb = q-3 ════
         // P^{n+1} of planes qb-1 .. qb+2 are in LDS slots (q-4)&3 .. (q-1)&3; the
@@ -482,6 +609,7 @@ __device__ __forceinline__ void fused4_tile(const Pair4Args& a, int ty, int tz, 
                 float nv = ax[s][N3] - vcoef * grad4(P1m[ix], P1b[ix], P1r[ix], P1p[ix], xinb, C1, C2);
                 if (PML) nv = nv * (xd ? damp(max(dqb, dyz0[s]), max(dqb1, dyz0[s])) : mX[s]);
                 nv = (fvb && cellv[s]) ? nv : 0.0f;
+                if (GEOM && ((uint32_t)(gh[s] >> 24) & G_CLOSED)) nv = 0.0f;
                 bx[s][N0] = nv;
                 if (ownq) bst(X, off[s], nv);
             }
@@ -499,6 +627,7 @@ __device__ __forceinline__ void fused4_tile(const Pair4Args& a, int ty, int tz, 
                                                                yin[s], C1, C2);
                     if (PML) nv = nv * (xd ? damp(max(dqb, dyz0[s]), max(dqb, dyzY[s])) : mY[s]);
                     nv = (qbv && vyv[s]) ? nv : 0.0f;
+                    if (GEOM && ((uint32_t)(gh[s] >> 24) & (G_CLOSED << 1))) nv = 0.0f;
                     Byw[byix(e, c)] = nv;
                     if (s < 2) bst(Y, off[s], nv);
                 }
@@ -512,6 +641,19 @@ __device__ __forceinline__ void fused4_tile(const Pair4Args& a, int ty, int tz, 
                                                                zin[s], C1, C2);
                     if (PML) nv = nv * (xd ? damp(max(dqb, dyz0[s]), max(dqb, dyzZ[s])) : mZ[s]);
                     nv = (qbv && vzv[s]) ? nv : 0.0f;
+                    if constexpr (GEOM) {
+                        const uint32_t g3 = (s < 2 || ringBz) ? (uint32_t)(gh[s] >> 24) & 0xFFu : 0u;
+                        if (g3 & (G_CLOSED << 2)) nv = 0.0f;
+                        if (g3 & G_PLATE) {
+                            // step n+1's plate update of node (qb, j0+e), as the
+                            // plate kernel evaluates it, from P^{n+1} beside the plate
+                            const float* pl = sPl + (qb & 7) * PLS + e * 5;
+                            const float Pa = a.piz + 1 < Nz ? P1b[p1ix(e, a.piz + 1 - k0)] : 0.0f;
+                            const float Pb = a.piz - 1 >= 0 ? P1b[p1ix(e, a.piz - 1 - k0)] : 0.0f;
+                            const float wn = pl[0] + a.pa * ((Pb - Pa) + pl[3]) - pl[1] - pl[2];
+                            nv = (wn - pl[4]) * a.inv_dt;
+                        }
+                    }
                     if (s < 2) {
                         Bzw[bzix(e, c)] = nv;
                         bst(Z, off[s], nv);
@@ -538,11 +680,14 @@ __device__ __forceinline__ void fused4_tile(const Pair4Args& a, int ty, int tz, 
                 const float ypp = Byr[byix(e + 1, c)], ymm = Byr[byix(e - 2, c)];
                 const float zp = Bzr[bzix(e, c)], zm = Bzr[bzix(e, c - 1)];
                 const float zpp = Bzr[bzix(e, c + 1)], zmm = Bzr[bzix(e, c - 2)];
-                const float dv = C1 * ((xp - xm) + (yp - ym) + (zp - zm))
-                               + C2 * ((xpp - xmm) + (ypp - ymm) + (zpp - zmm));
+                float dv = C1 * ((xp - xm) + (yp - ym) + (zp - zm))
+                         + C2 * ((xpp - xmm) + (ypp - ymm) + (zpp - zmm));
+                const uint32_t g4 = GEOM ? (uint32_t)(gh[s] >> 32) & 0xFFu : 0u;   // plane qc
+                if (GEOM && (g4 & G2_TAPCUT))
+                    dv = cut_dv(qc, j0 + e, k0 + c, xp, xm, xpp, xmm, yp, ym, ypp, ymm, zp, zm, zpp, zmm);
                 float out = P1c[p1ix(e, c)] - kap0 * dv;
                 if (PML) out = out * (xd ? sPd[max(dqc, dyz0[s])] : mP[s]);
-                bst(R, off[s], out);
+                if (!GEOM || !(g4 & G_SOLID)) bst(R, off[s], out);   // walls keep P
             }
         }
 
@@ -558,6 +703,15 @@ __device__ __forceinline__ void fused4_tile(const Pair4Args& a, int ty, int tz, 
         }
 #pragma unroll
         for (int s = 0; s < 3; ++s) pn[s][M0] = pnl[s];
+        if constexpr (GEOM) {
+            // codes of plane q+1 into the history; the plate rows of plane q+1
+            // into the LDS rings
+#pragma unroll
+            for (int s = 0; s < 3; ++s) gh[s] = (gh[s] << 8) | gn[s];
+            gmc = gmn;
+            if (plLane) sPl[((q + 1) & 7) * PLS + plRow * 5 + plComp] = plv;
+            if (vpLane) sVp[((q + 1) & 1) * ER + vpRow + H] = vpv;
+        }
         __syncthreads();
     };
 
@@ -571,7 +725,7 @@ __device__ __forceinline__ void fused4_tile(const Pair4Args& a, int ty, int tz, 
     }
 }
 
-template <bool PML>
+template <bool PML, bool GEOM>
 __global__ void __launch_bounds__(f4::NT) k_fused4(Pair4Args a) {
     if (*a.status) return;
     // each XCD takes a contiguous range of work items; within a segment the
@@ -585,10 +739,168 @@ __global__ void __launch_bounds__(f4::NT) k_fused4(Pair4Args a) {
     t -= band * a.nty * 4;
     const int ty = t / gz;
     const int tz = 4 * band + t % gz;
-    fused4_tile<PML>(a, ty, tz, sg, ty + a.nty * tz);
+    fused4_tile<PML, GEOM>(a, ty, tz, sg, ty + a.nty * tz);
+}
+
+// Pair codes (bit layout at G2_TAPCUT) from the single-step codes: the
+// wall, closed and plate bits of the cell, and whether any of its twelve
+// divergence taps (x faces i, i-1, i+1, i-2 of the cell's line, likewise y
+// and z) carries the fraction bit of its axis.
+__global__ void __launch_bounds__(256)
+k_geo2_codes(int Nx, int Ny, int Nz, const uint8_t* __restrict__ code, uint8_t* __restrict__ g2) {
+    const size_t N = (size_t)Nx * Ny * Nz;
+    const size_t c = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (c >= N) return;
+    const int k = (int)(c % Nz);
+    const size_t t = c / Nz;
+    const int j = (int)(t % Ny), i = (int)(t / Ny);
+    const size_t sx = (size_t)Ny * Nz;
+    using namespace fused;
+    uint32_t out = code[c] & (G_SOLID | (G_CLOSED * 7u) | G_PLATE);
+    bool cut = false;
+    const int d[4] = {0, -1, 1, -2};
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        if (i + d[u] >= 0 && i + d[u] < Nx) cut |= (code[c + d[u] * (long)sx] & G_CUT) != 0;
+        if (j + d[u] >= 0 && j + d[u] < Ny) cut |= (code[c + d[u] * (long)Nz] & (G_CUT << 1)) != 0;
+        if (k + d[u] >= 0 && k + d[u] < Nz) cut |= (code[c + d[u]] & (G_CUT << 2)) != 0;
+    }
+    if (cut) out |= f4::G2_TAPCUT;
+    g2[c] = (uint8_t)out;
+}
+
+// Plate-neighbour bit on the cells above and below each active node.
+__global__ void __launch_bounds__(256)
+k_geo2_plate(int nact, const int* __restrict__ above, const int* __restrict__ below,
+             uint8_t* __restrict__ g2) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nact) return;
+    if (above[n] >= 0) g2[above[n]] = (uint8_t)(g2[above[n]] | f4::G2_PLNB);
+    if (below[n] >= 0) g2[below[n]] = (uint8_t)(g2[below[n]] | f4::G2_PLNB);
+}
+
+// Divergence-tap weights: the face fraction where the single-step code
+// weights the tap, exactly 1 elsewhere (the fraction bit is set only where
+// the fraction is not exactly 1).
+__global__ void __launch_bounds__(256)
+k_tap_weights(size_t N, const uint8_t* __restrict__ code, const float* __restrict__ fx,
+              const float* __restrict__ fy, const float* __restrict__ fz, float* __restrict__ tw) {
+    const size_t c = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (c >= N) return;
+    const uint32_t g = code[c];
+    tw[c] = (g & fused::G_CUT) ? fx[c] : 1.0f;
+    tw[N + c] = (g & (fused::G_CUT << 1)) ? fy[c] : 1.0f;
+    tw[2 * N + c] = (g & (fused::G_CUT << 2)) ? fz[c] : 1.0f;
+}
+
+// Step n+1's plate update split at the pressure term: per active node p,
+// with w = w^{n+1}, wp = w^n, the terms T1 = w + vd (w - wp), c3 = bhc bh,
+// c4 = bk (bh - bhp), fb and w itself, so that the update is
+// ((T1 + a ((Pb - Pa) + fb)) - c3) - c4, the plate kernel's expression
+// (acoustic_fdtd.cpp:984-1053) with its 13-tap sums in tap order.
+__global__ void __launch_bounds__(256)
+k_plate_terms(int nact, const int* __restrict__ act, const int* __restrict__ b13,
+              const float* __restrict__ w, const float* __restrict__ wp,
+              const float* __restrict__ ext, float vd, float bhc, float bk, int Np,
+              float* __restrict__ pre, const int* status) {
+    if (status[0]) return;
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nact) return;
+    const float W13[13] = {20.f, -8.f, -8.f, -8.f, -8.f, 2.f, 2.f, 2.f, 2.f, 1.f, 1.f, 1.f, 1.f};
+    const int p = act[n];
+    const int* nb = b13 + 13 * n;
+    float bh = 0.0f, bhp = 0.0f;
+    for (int t = 0; t < 13; ++t) bh += W13[t] * w[nb[t]];
+    for (int t = 0; t < 13; ++t) bhp += W13[t] * wp[nb[t]];
+    const float vp = w[p] - wp[p];
+    pre[p] = w[p] + vd * vp;
+    pre[(size_t)Np + p] = bhc * bh;
+    pre[2 * (size_t)Np + p] = bk * (bh - bhp);
+    pre[3 * (size_t)Np + p] = ext[p];
+    pre[4 * (size_t)Np + p] = w[p];
+}
+
+constexpr int kPairFail = 9;    // status word while a paired step's plate guard is handled
+
+// Step n+1's plate update from the terms and the sweep's P^{n+1} beside the
+// plate; the guard (acoustic_fdtd.cpp:1032-1040) marks the pair for the
+// single-step redo of step n.
+__global__ void __launch_bounds__(256)
+k_plate_pair(int nact, const int* __restrict__ act, const int* __restrict__ above,
+             const int* __restrict__ below, const float* __restrict__ pre, int Np,
+             const float* __restrict__ pab, float a, float* __restrict__ wt, int* status,
+             int step_index) {
+    if (status[0]) return;
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nact) return;
+    const int p = act[n];
+    const float Pa = above[n] >= 0 ? pab[2 * (size_t)p] : 0.0f;
+    const float Pb = below[n] >= 0 ? pab[2 * (size_t)p + 1] : 0.0f;
+    const float wn = pre[p] + a * ((Pb - Pa) + pre[3 * (size_t)Np + p]) - pre[(size_t)Np + p]
+                   - pre[2 * (size_t)Np + p];
+    if (!isfinite(wn) || fabsf(wn) > 0.02f) {
+        if (atomicCAS(&status[0], 0, kPairFail) == 0) status[1] = step_index;
+        return;
+    }
+    wt[p] = wn;
+}
+
+// After the redo of step n: its point sources (in source order, as the
+// post-step kernel adds them), then the plate failure status of step n+1.
+__global__ void k_pair_redo_end(float* __restrict__ P, int npsrc, const int* __restrict__ pidx,
+                                const float* __restrict__ psig, int plen, int step_index,
+                                int* status) {
+    if (status[0] != kPairFail) return;
+    if (step_index < plen)
+        for (int s = 0; s < npsrc; ++s) P[pidx[s]] = P[pidx[s]] + psig[(size_t)s * plen + step_index];
+    status[0] = 2;
 }
 
 }  // namespace
+
+bool fused4_geo_supported(const AcousticFDTDState* st) {
+    return st->order == 4 && st->storage == FDTD_STORE_F32 && !st->hetero && st->n_pml <= 64 &&
+           fused_geo_supported(st);
+}
+
+hipError_t launch_geo_pair_codes(AcousticFDTDState* st) {
+    if (!st->d_g2) {
+        hipError_t e = hipMalloc(&st->d_g2, (size_t)st->N);
+        if (e != hipSuccess) { st->d_g2 = nullptr; return e; }
+    }
+    hipLaunchKernelGGL(k_geo2_codes, dim3((unsigned)(((size_t)st->N + 255) / 256)), dim3(256), 0,
+                       st->stream, st->Nx, st->Ny, st->Nz, st->d_geo, st->d_g2);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if (st->has_frac) {
+        if (!st->d_tw) {
+            e = hipMalloc(&st->d_tw, 3 * sizeof(float) * (size_t)st->N);
+            if (e != hipSuccess) { st->d_tw = nullptr; return e; }
+        }
+        hipLaunchKernelGGL(k_tap_weights, dim3((unsigned)(((size_t)st->N + 255) / 256)), dim3(256),
+                           0, st->stream, (size_t)st->N, st->d_geo, st->d_fx, st->d_fy, st->d_fz,
+                           st->d_tw);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    if (st->nact <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_geo2_plate, dim3((st->nact + 255) / 256), dim3(256), 0, st->stream,
+                       st->nact, st->d_above, st->d_below, st->d_g2);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if (!st->d_ppre) {
+        e = hipMalloc(&st->d_ppre, 5 * sizeof(float) * st->Nplate);
+        if (e != hipSuccess) { st->d_ppre = nullptr; return e; }
+        e = hipMemsetAsync(st->d_ppre, 0, 5 * sizeof(float) * st->Nplate, st->stream);
+        if (e != hipSuccess) return e;
+    }
+    if (!st->d_pab) {
+        e = hipMalloc(&st->d_pab, 2 * sizeof(float) * st->Nplate);
+        if (e != hipSuccess) { st->d_pab = nullptr; return e; }
+        e = hipMemsetAsync(st->d_pab, 0, 2 * sizeof(float) * st->Nplate, st->stream);
+    }
+    return e;
+}
 
 bool fused4_supported(const AcousticFDTDState* st) {
     return st->order == 4 && fused_supported(st) && st->storage == FDTD_STORE_F32 && !st->hetero;
@@ -630,7 +942,7 @@ void f4_tile_lists(int Ny, int Nz, int nsrc, const int* src_cell, const int* src
     }
 }
 
-hipError_t launch_fused4_pair(AcousticFDTDState* st, int step) {
+static Pair4Args pair4_args(AcousticFDTDState* st, int step) {
     using namespace f4;
     Pair4Args a;
     memset(&a, 0, sizeof(a));
@@ -657,17 +969,83 @@ hipError_t launch_fused4_pair(AcousticFDTDState* st, int step) {
     a.plen = st->npsrc > 0 ? st->plen : 0;
     a.step = step;
     a.tapbuf = st->d_f2_tapbuf;
-    const int nb = a.nty * a.ntz * a.nseg;
-    if (st->n_pml > 0)
-        hipLaunchKernelGGL(k_fused4<true>, dim3(nb), dim3(64, NW), 0, st->stream, a);
-    else
-        hipLaunchKernelGGL(k_fused4<false>, dim3(nb), dim3(64, NW), 0, st->stream, a);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
+    return a;
+}
+
+#define AFDTD_RET(expr)                         \
+    do {                                        \
+        hipError_t _e = (expr);                 \
+        if (_e != hipSuccess) return _e;        \
+    } while (0)
+
+static void swap_velocities(AcousticFDTDState* st) {
     for (int c = 0; c < 3; ++c) {
         void* t = st->V[c];
         st->V[c] = st->V2[c];
         st->V2[c] = t;
     }
+}
+
+hipError_t launch_fused4_pair(AcousticFDTDState* st, int step) {
+    using namespace f4;
+    const Pair4Args a = pair4_args(st, step);
+    const int nb = a.nty * a.ntz * a.nseg;
+    if (st->n_pml > 0)
+        hipLaunchKernelGGL((k_fused4<true, false>), dim3(nb), dim3(64, NW), 0, st->stream, a);
+    else
+        hipLaunchKernelGGL((k_fused4<false, false>), dim3(nb), dim3(64, NW), 0, st->stream, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    swap_velocities(st);
+    return hipSuccess;
+}
+
+hipError_t launch_fused4_geo_pair(AcousticFDTDState* st, int step, int trace_row0) {
+    using namespace f4;
+    if (!st->d_geo || !st->d_g2 || (st->nact > 0 && (!st->d_ppre || !st->d_pab || !st->d_vpl)))
+        return hipErrorInvalidValue;
+    const int nact = st->nact;
+    const unsigned pb = (unsigned)((nact + 255) / 256);
+    if (nact > 0) {
+        hipLaunchKernelGGL(k_plate_terms, dim3(pb), dim3(256), 0, st->stream, nact, st->d_act,
+                           st->d_b13, st->d_w[0], st->d_w[1], st->d_ext, st->plate_vd,
+                           st->plate_bh, st->plate_bk, st->Nplate, st->d_ppre, st->d_status);
+        AFDTD_RET(hipGetLastError());
+    }
+    Pair4Args a = pair4_args(st, step);
+    a.g2 = st->d_g2;
+    a.tw = st->d_tw;
+    a.vpl = st->d_vpl;
+    a.pre = st->d_ppre;
+    a.pab = st->d_pab;
+    a.Nplate = st->Nplate;
+    a.piz = st->plate_iz;
+    a.plate_on = nact > 0;
+    a.pa = st->plate_a;
+    a.inv_dt = 1.0f / st->dt;
+    const int nb = a.nty * a.ntz * a.nseg;
+    if (st->n_pml > 0)
+        hipLaunchKernelGGL((k_fused4<true, true>), dim3(nb), dim3(64, NW), 0, st->stream, a);
+    else
+        hipLaunchKernelGGL((k_fused4<false, true>), dim3(nb), dim3(64, NW), 0, st->stream, a);
+    AFDTD_RET(hipGetLastError());
+    AFDTD_RET(launch_fused2_taps(st, trace_row0));
+    if (nact > 0) {
+        hipLaunchKernelGGL(k_plate_pair, dim3(pb), dim3(256), 0, st->stream, nact, st->d_act,
+                           st->d_above, st->d_below, st->d_ppre, st->Nplate, st->d_pab,
+                           st->plate_a, st->d_w[2], st->d_status, step + 1);
+        AFDTD_RET(hipGetLastError());
+        float* t = st->d_w[2]; st->d_w[2] = st->d_w[1]; st->d_w[1] = st->d_w[0]; st->d_w[0] = t;
+        // redo of step n over the sweep's output when step n+1's guard tripped
+        // (inputs intact: the sweep writes the other buffers)
+        // (the single-step launcher swaps the velocity buffers: swap back)
+        AFDTD_RET(launch_fused_step_if(st, kPairFail));
+        swap_velocities(st);
+        hipLaunchKernelGGL(k_pair_redo_end, dim3(1), dim3(1), 0, st->stream,
+                           (float*)st->P[st->cur ^ 1], st->npsrc, st->d_pidx, st->d_psig, st->plen,
+                           step, st->d_status);
+        AFDTD_RET(hipGetLastError());
+    }
+    swap_velocities(st);
     return hipSuccess;
 }

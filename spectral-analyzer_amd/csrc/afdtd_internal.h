@@ -144,6 +144,15 @@ struct AcousticFDTDState {
     int fused_geo;
     uint8_t* d_geo;
     float* d_vpl;           // plate velocity of the current step (nact > 0)
+    // Two-step sweep with cell geometry (order 4, FP32, homogeneous): per-cell
+    // pair codes, step n+1's plate terms [5][Nplate] and P^{n+1} beside the
+    // plate [Nplate][2].
+    int geo_pair;
+    int kernel_opt;         // the kernel family the handle was created with
+    uint8_t* d_g2;
+    float* d_tw;            // [3][N] divergence-tap weights (fractions present)
+    float* d_ppre;
+    float* d_pab;
     // Side job of the geometry sweep (set by a hooked caller, the co-evolver):
     // one extra workgroup forms sum_i w[idx[i]] * wgt[i] over the plate
     // displacement the sweep's step ends with, in index order from +0, and
@@ -257,6 +266,20 @@ void f4_tile_lists(int Ny, int Nz, int nsrc, const int* src_cell, const int* src
                    const int* tap_cell, const int* tap_slot, std::vector<std::vector<int2>>& src,
                    std::vector<std::vector<int2>>& tap);
 hipError_t launch_fused4_pair(AcousticFDTDState* st, int step);
+// The two-step sweep with cell geometry (walls, closed / fractional faces,
+// active plate): order 4, FP32, homogeneous medium.
+bool fused4_geo_supported(const AcousticFDTDState* st);
+// Build st->d_g2 from st->d_geo and the plate (after launch_geo_codes).
+hipError_t launch_geo_pair_codes(AcousticFDTDState* st);
+// Steps s, s+1 with cell geometry: the step-s plate kernel (and its force
+// fill) has run and the plate buffers are rotated; this enqueues step s+1's
+// plate terms, the sweep, the step-s receiver taps (trace_row0), step s+1's
+// plate update (force fill and rotation included) and, if that update trips
+// the plate guard, step s again as one single step over the sweep's output,
+// leaving status {2, s+1}.
+hipError_t launch_fused4_geo_pair(AcousticFDTDState* st, int step, int trace_row0);
+// Single-step sweep that runs only while the status word equals run_on.
+hipError_t launch_fused_step_if(AcousticFDTDState* st, int run_on);
 int f2_tile_rows(int tile_y);
 // Longest per-tile source / tap list the two-step sweep caches in LDS.
 constexpr int F2_MAX_SRC = 32, F2_MAX_TAP = 256;
