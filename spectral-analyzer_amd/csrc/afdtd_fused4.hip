@@ -124,6 +124,7 @@ struct Pair4Args {
     int plen;
     int step;               // global index of the first of the two steps
     float* tapbuf;
+    const int* items;       // work-item list (nullptr: item = block)
     // cell geometry (GEOM build)
     const uint8_t* g2;      // per-cell pair code (G2_* bits)
     const float* tw;        // [3][N] divergence-tap weights: the face fraction where the
@@ -135,7 +136,7 @@ struct Pair4Args {
     float pa, inv_dt;       // plate coefficient a, 1/dt
 };
 
-template <bool PML, bool GEOM>
+template <bool PML, bool GEOM, bool CUT>
 __device__ __forceinline__ void fused4_tile(const Pair4Args& a, int ty, int tz, int sg,
                                             int tile) {
     using namespace f4;
@@ -203,7 +204,7 @@ __device__ __forceinline__ void fused4_tile(const Pair4Args& a, int ty, int tz, 
     }
     uint32_t off[3];             // byte offset of the slot cell in a plane
     bool cellv[3], vyv[3], vzv[3], yin[3], zin[3];
-    int dyz0[3], dyzY[3], dyzZ[3];
+    int dz3[3];                  // PML depths of the slot cell | y face << 8 | z face << 16
 #pragma unroll
     for (int s = 0; s < 3; ++s) {
         const int j = j0 + se[s], k = k0 + sc[s];
@@ -214,10 +215,11 @@ __device__ __forceinline__ void fused4_tile(const Pair4Args& a, int ty, int tz, 
         yin[s] = j > 0 && j < Ny - 2;
         zin[s] = k > 0 && k < Nz - 2;
         const int py = pd(j, Ny), pz = pd(k, Nz);
-        dyz0[s] = max(py, pz);
-        dyzY[s] = max(pd(j + 1, Ny), pz);
-        dyzZ[s] = max(py, pd(k + 1, Nz));
+        dz3[s] = max(py, pz) | (max(pd(j + 1, Ny), pz) << 8) | (max(py, pd(k + 1, Nz)) << 16);
     }
+    auto D0 = [&](int s) __attribute__((always_inline)) { return dz3[s] & 0xFF; };
+    auto DY = [&](int s) __attribute__((always_inline)) { return (dz3[s] >> 8) & 0xFF; };
+    auto DZ = [&](int s) __attribute__((always_inline)) { return dz3[s] >> 16; };
     // stage-B roles of the ring slot: y-faces of rows -2, -1, TY (waves 1..3),
     // z-faces of columns -2, -1, 64 on the tile rows (waves 6..8)
     const bool ringBy = w >= 1 && w <= 3;
@@ -315,10 +317,10 @@ __device__ __forceinline__ void fused4_tile(const Pair4Args& a, int ty, int tz, 
     for (int s = 0; s < 3; ++s) {
         mX[s] = mY[s] = mZ[s] = mP[s] = 1.0f;
         if (PML) {
-            mX[s] = face_damp(a.alpha, dyz0[s], dyz0[s], dt);
-            mY[s] = face_damp(a.alpha, dyz0[s], dyzY[s], dt);
-            mZ[s] = face_damp(a.alpha, dyz0[s], dyzZ[s], dt);
-            mP[s] = a.pdamp[dyz0[s]];
+            mX[s] = face_damp(a.alpha, D0(s), D0(s), dt);
+            mY[s] = face_damp(a.alpha, D0(s), DY(s), dt);
+            mZ[s] = face_damp(a.alpha, D0(s), DZ(s), dt);
+            mP[s] = a.pdamp[D0(s)];
         }
     }
     if (PML) mM = face_damp(a.alpha, mdL, mdR, dt);
@@ -460,7 +462,7 @@ __device__ __forceinline__ void fused4_tile(const Pair4Args& a, int ty, int tz, 
                 const float pm = pn[s][M0], pl = pn[s][M1];
                 // x-face between planes q and q+1
                 float nv = ovx[s] - vcoef * grad4(pm, pl, pn[s][M2], pn[s][M3], xin, C1, C2);
-                if (PML) nv = nv * (xd ? damp(max(dq, dyz0[s]), max(dq1, dyz0[s])) : mX[s]);
+                if (PML) nv = nv * (xd ? damp(max(dq, D0(s)), max(dq1, D0(s))) : mX[s]);
                 const uint32_t g0 = GEOM ? (uint32_t)gh[s] & 0xFFu : 0u;   // plane q
                 float nxv = (fvx && cellv[s]) ? nv : 0.0f;
                 if (GEOM && (g0 & G_CLOSED)) nxv = 0.0f;
@@ -468,13 +470,13 @@ __device__ __forceinline__ void fused4_tile(const Pair4Args& a, int ty, int tz, 
                 // y-face between rows e and e+1
                 nv = ovy[s] - vcoef * grad4(Pq[gix(e - 1, c)], pl, Pq[gix(e + 1, c)],
                                             Pq[gix(e + 2, c)], yin[s], C1, C2);
-                if (PML) nv = nv * (xd ? damp(max(dq, dyz0[s]), max(dq, dyzY[s])) : mY[s]);
+                if (PML) nv = nv * (xd ? damp(max(dq, D0(s)), max(dq, DY(s))) : mY[s]);
                 float ny = (qv && vyv[s]) ? nv : 0.0f;
                 if (GEOM && (g0 & (G_CLOSED << 1))) ny = 0.0f;
                 // z-face between columns c and c+1
                 nv = ovz[s] - vcoef * grad4(Pq[gix(e, c - 1)], pl, Pq[gix(e, c + 1)],
                                             Pq[gix(e, c + 2)], zin[s], C1, C2);
-                if (PML) nv = nv * (xd ? damp(max(dq, dyz0[s]), max(dq, dyzZ[s])) : mZ[s]);
+                if (PML) nv = nv * (xd ? damp(max(dq, D0(s)), max(dq, DZ(s))) : mZ[s]);
                 float nz = (qv && vzv[s]) ? nv : 0.0f;
                 if constexpr (GEOM) {
                     if (g0 & (G_CLOSED << 2)) nz = 0.0f;
@@ -541,10 +543,10 @@ __device__ __forceinline__ void fused4_tile(const Pair4Args& a, int ty, int tz, 
                 float dv = C1 * ((xp - xm) + (yp - ym) + (zp - zm))
                          + C2 * ((xpp - xmm) + (ypp - ymm) + (zpp - zmm));
                 const uint32_t g1 = GEOM ? (uint32_t)(gh[s] >> 8) & 0xFFu : 0u;   // plane qa
-                if (GEOM && (g1 & G2_TAPCUT))
+                if (CUT && (g1 & G2_TAPCUT))
                     dv = cut_dv(qa, j0 + e, k0 + c, xp, xm, xpp, xmm, yp, ym, ypp, ymm, zp, zm, zpp, zmm);
                 float out = pn[s][M0] - kap0 * dv;
-                if (PML) out = out * (xd ? sPd[max(dqa, dyz0[s])] : mP[s]);
+                if (PML) out = out * (xd ? sPd[max(dqa, D0(s))] : mP[s]);
                 if (GEOM && (g1 & G_SOLID)) out = pn[s][M0];          // walls keep P
                 pnew[s] = (qav && cellv[s]) ? out : 0.0f;
             }
@@ -607,7 +609,7 @@ __device__ __forceinline__ void fused4_tile(const Pair4Args& a, int ty, int tz, 
             for (int s = 0; s < 2; ++s) {   // x-face between planes qb and qb+1
                 const int ix = p1ix(se[s], sc[s]);
                 float nv = ax[s][N3] - vcoef * grad4(P1m[ix], P1b[ix], P1r[ix], P1p[ix], xinb, C1, C2);
-                if (PML) nv = nv * (xd ? damp(max(dqb, dyz0[s]), max(dqb1, dyz0[s])) : mX[s]);
+                if (PML) nv = nv * (xd ? damp(max(dqb, D0(s)), max(dqb1, D0(s))) : mX[s]);
                 nv = (fvb && cellv[s]) ? nv : 0.0f;
                 if (GEOM && ((uint32_t)(gh[s] >> 24) & G_CLOSED)) nv = 0.0f;
                 bx[s][N0] = nv;
@@ -625,7 +627,7 @@ __device__ __forceinline__ void fused4_tile(const Pair4Args& a, int ty, int tz, 
                     float nv = Ayo[ayix(e, c)] - vcoef * grad4(P1b[p1ix(e - 1, c)], P1b[p1ix(e, c)],
                                                                P1b[p1ix(e + 1, c)], P1b[p1ix(e + 2, c)],
                                                                yin[s], C1, C2);
-                    if (PML) nv = nv * (xd ? damp(max(dqb, dyz0[s]), max(dqb, dyzY[s])) : mY[s]);
+                    if (PML) nv = nv * (xd ? damp(max(dqb, D0(s)), max(dqb, DY(s))) : mY[s]);
                     nv = (qbv && vyv[s]) ? nv : 0.0f;
                     if (GEOM && ((uint32_t)(gh[s] >> 24) & (G_CLOSED << 1))) nv = 0.0f;
                     Byw[byix(e, c)] = nv;
@@ -639,7 +641,7 @@ __device__ __forceinline__ void fused4_tile(const Pair4Args& a, int ty, int tz, 
                     float nv = Azo[azix(e, c)] - vcoef * grad4(P1b[p1ix(e, c - 1)], P1b[p1ix(e, c)],
                                                                P1b[p1ix(e, c + 1)], P1b[p1ix(e, c + 2)],
                                                                zin[s], C1, C2);
-                    if (PML) nv = nv * (xd ? damp(max(dqb, dyz0[s]), max(dqb, dyzZ[s])) : mZ[s]);
+                    if (PML) nv = nv * (xd ? damp(max(dqb, D0(s)), max(dqb, DZ(s))) : mZ[s]);
                     nv = (qbv && vzv[s]) ? nv : 0.0f;
                     if constexpr (GEOM) {
                         const uint32_t g3 = (s < 2 || ringBz) ? (uint32_t)(gh[s] >> 24) & 0xFFu : 0u;
@@ -683,10 +685,10 @@ __device__ __forceinline__ void fused4_tile(const Pair4Args& a, int ty, int tz, 
                 float dv = C1 * ((xp - xm) + (yp - ym) + (zp - zm))
                          + C2 * ((xpp - xmm) + (ypp - ymm) + (zpp - zmm));
                 const uint32_t g4 = GEOM ? (uint32_t)(gh[s] >> 32) & 0xFFu : 0u;   // plane qc
-                if (GEOM && (g4 & G2_TAPCUT))
+                if (CUT && (g4 & G2_TAPCUT))
                     dv = cut_dv(qc, j0 + e, k0 + c, xp, xm, xpp, xmm, yp, ym, ypp, ymm, zp, zm, zpp, zmm);
                 float out = P1c[p1ix(e, c)] - kap0 * dv;
-                if (PML) out = out * (xd ? sPd[max(dqc, dyz0[s])] : mP[s]);
+                if (PML) out = out * (xd ? sPd[max(dqc, D0(s))] : mP[s]);
                 if (!GEOM || !(g4 & G_SOLID)) bst(R, off[s], out);   // walls keep P
             }
         }
@@ -725,12 +727,15 @@ __device__ __forceinline__ void fused4_tile(const Pair4Args& a, int ty, int tz, 
     }
 }
 
-template <bool PML, bool GEOM>
+// CUT: the tiles whose cells have fraction-weighted divergence taps (the
+// weighted path costs registers, so the other tiles run without it)
+template <bool PML, bool GEOM, bool CUT>
 __global__ void __launch_bounds__(f4::NT) k_fused4(Pair4Args a) {
     if (*a.status) return;
     // each XCD takes a contiguous range of work items; within a segment the
     // items run in bands of up to four z-tiles, z fastest, that sweep y
     int t = xcd_remap(blockIdx.x, gridDim.x);
+    if (a.items) t = a.items[t];
     const int per_sg = a.nty * a.ntz;
     const int sg = t / per_sg;
     t -= sg * per_sg;
@@ -739,7 +744,7 @@ __global__ void __launch_bounds__(f4::NT) k_fused4(Pair4Args a) {
     t -= band * a.nty * 4;
     const int ty = t / gz;
     const int tz = 4 * band + t % gz;
-    fused4_tile<PML, GEOM>(a, ty, tz, sg, ty + a.nty * tz);
+    fused4_tile<PML, GEOM, CUT>(a, ty, tz, sg, ty + a.nty * tz);
 }
 
 // Pair codes (bit layout at G2_TAPCUT) from the single-step codes: the
@@ -864,6 +869,7 @@ bool fused4_geo_supported(const AcousticFDTDState* st) {
 }
 
 hipError_t launch_geo_pair_codes(AcousticFDTDState* st) {
+    ++st->g2_gen;                   // the item lists follow the codes
     if (!st->d_g2) {
         hipError_t e = hipMalloc(&st->d_g2, (size_t)st->N);
         if (e != hipSuccess) { st->d_g2 = nullptr; return e; }
@@ -991,12 +997,79 @@ hipError_t launch_fused4_pair(AcousticFDTDState* st, int step) {
     const Pair4Args a = pair4_args(st, step);
     const int nb = a.nty * a.ntz * a.nseg;
     if (st->n_pml > 0)
-        hipLaunchKernelGGL((k_fused4<true, false>), dim3(nb), dim3(64, NW), 0, st->stream, a);
+        hipLaunchKernelGGL((k_fused4<true, false, false>), dim3(nb), dim3(64, NW), 0, st->stream, a);
     else
-        hipLaunchKernelGGL((k_fused4<false, false>), dim3(nb), dim3(64, NW), 0, st->stream, a);
+        hipLaunchKernelGGL((k_fused4<false, false, false>), dim3(nb), dim3(64, NW), 0, st->stream, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     swap_velocities(st);
+    return hipSuccess;
+}
+
+// Per work item (x-segment sg of tile (ty, tz), numbered as k_fused4 numbers
+// them): does any cell the item updates (stage A: the extended tile over
+// planes i0-3 .. i1+2) carry the weighted-tap bit?
+__global__ void __launch_bounds__(256)
+k_item_cut(const uint8_t* __restrict__ g2, int Nx, int Ny, int Nz, int seg, int nty, int ntz,
+           int* __restrict__ flag) {
+    using namespace f4;
+    int t = blockIdx.x;
+    const int per_sg = nty * ntz;
+    const int sg = t / per_sg;
+    t -= sg * per_sg;
+    const int band = t / (nty * 4);
+    const int gz = min(4, ntz - 4 * band);
+    t -= band * nty * 4;
+    const int ty = t / gz, tz = 4 * band + t % gz;
+    const int i0 = max(sg * seg - 3, 0), i1 = min(sg * seg + seg + 3, Nx);
+    const int ja = max(ty * TY - H, 0), jb = min(ty * TY + TY + H, Ny);
+    const int ka = max(tz * TZ - H, 0), kb = min(tz * TZ + TZ + H, Nz);
+    const int nk = kb - ka, nj = jb - ja;
+    int any = 0;
+    for (int i = i0; i < i1 && !any; ++i)
+        for (int r = threadIdx.x; r < nj * nk; r += 256) {
+            const int j = ja + r / nk, k = ka + r % nk;
+            if (g2[((size_t)i * Ny + j) * Nz + k] & G2_TAPCUT) { any = 1; break; }
+        }
+    any = __syncthreads_or(any);
+    if (threadIdx.x == 0) flag[blockIdx.x] = any;
+}
+
+// Work-item lists of the geometry sweep (rebuilt when the codes or the
+// segment length change): items without weighted taps first, then those
+// with them, each in item order.
+static hipError_t geo_item_lists(AcousticFDTDState* st, const Pair4Args& a) {
+    if (st->d_g2_items && st->g2_items_seg == a.seg && st->g2_items_gen == st->g2_gen)
+        return hipSuccess;
+    const int nb = a.nty * a.ntz * a.nseg;
+    std::vector<int> flag(nb, 0), list;
+    if (st->has_frac) {
+        int* d_flag = nullptr;
+        AFDTD_RET(hipMalloc(&d_flag, sizeof(int) * nb));
+        hipLaunchKernelGGL(k_item_cut, dim3(nb), dim3(256), 0, st->stream, st->d_g2, st->Nx,
+                           st->Ny, st->Nz, a.seg, a.nty, a.ntz, d_flag);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(flag.data(), d_flag, sizeof(int) * nb, hipMemcpyDeviceToHost,
+                               st->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(st->stream);
+        (void)hipFree(d_flag);
+        AFDTD_RET(e);
+    }
+    for (int pass = 0; pass < 2; ++pass)
+        for (int t = 0; t < nb; ++t)
+            if ((flag[t] != 0) == (pass == 1)) list.push_back(t);
+    st->g2_nslow = 0;
+    for (int t = 0; t < nb; ++t) st->g2_nslow += flag[t] != 0;
+    st->g2_nfast = nb - st->g2_nslow;
+    if (st->d_g2_items) AFDTD_RET(hipFree(st->d_g2_items));
+    st->d_g2_items = nullptr;
+    AFDTD_RET(hipMalloc(&st->d_g2_items, sizeof(int) * nb));
+    AFDTD_RET(hipMemcpyAsync(st->d_g2_items, list.data(), sizeof(int) * nb, hipMemcpyHostToDevice,
+                             st->stream));
+    AFDTD_RET(hipStreamSynchronize(st->stream));
+    st->g2_items_seg = a.seg;
+    st->g2_items_gen = st->g2_gen;
     return hipSuccess;
 }
 
@@ -1023,12 +1096,28 @@ hipError_t launch_fused4_geo_pair(AcousticFDTDState* st, int step, int trace_row
     a.plate_on = nact > 0;
     a.pa = st->plate_a;
     a.inv_dt = 1.0f / st->dt;
-    const int nb = a.nty * a.ntz * a.nseg;
-    if (st->n_pml > 0)
-        hipLaunchKernelGGL((k_fused4<true, true>), dim3(nb), dim3(64, NW), 0, st->stream, a);
-    else
-        hipLaunchKernelGGL((k_fused4<false, true>), dim3(nb), dim3(64, NW), 0, st->stream, a);
-    AFDTD_RET(hipGetLastError());
+    AFDTD_RET(geo_item_lists(st, a));
+    // tiles without weighted taps, then the tiles with them (disjoint outputs)
+    a.items = st->d_g2_items;
+    if (st->g2_nfast > 0) {
+        if (st->n_pml > 0)
+            hipLaunchKernelGGL((k_fused4<true, true, false>), dim3(st->g2_nfast), dim3(64, NW), 0,
+                               st->stream, a);
+        else
+            hipLaunchKernelGGL((k_fused4<false, true, false>), dim3(st->g2_nfast), dim3(64, NW), 0,
+                               st->stream, a);
+        AFDTD_RET(hipGetLastError());
+    }
+    a.items = st->d_g2_items + st->g2_nfast;
+    if (st->g2_nslow > 0) {
+        if (st->n_pml > 0)
+            hipLaunchKernelGGL((k_fused4<true, true, true>), dim3(st->g2_nslow), dim3(64, NW), 0,
+                               st->stream, a);
+        else
+            hipLaunchKernelGGL((k_fused4<false, true, true>), dim3(st->g2_nslow), dim3(64, NW), 0,
+                               st->stream, a);
+        AFDTD_RET(hipGetLastError());
+    }
     AFDTD_RET(launch_fused2_taps(st, trace_row0));
     if (nact > 0) {
         hipLaunchKernelGGL(k_plate_pair, dim3(pb), dim3(256), 0, st->stream, nact, st->d_act,
