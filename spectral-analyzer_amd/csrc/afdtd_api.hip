@@ -343,6 +343,12 @@ void destroy_state(AcousticFDTDState* st) {
     dfree(st->d_geo); dfree(st->d_vpl);
     dfree(st->d_g2); dfree(st->d_tw); dfree(st->d_ppre); dfree(st->d_pab);
     dfree(st->d_g2_items);
+    if (st->g2_stream) {
+        (void)hipStreamSynchronize(st->g2_stream);
+        (void)hipStreamDestroy(st->g2_stream);
+        (void)hipEventDestroy(st->g2_ev[0]);
+        (void)hipEventDestroy(st->g2_ev[1]);
+    }
     dfree(st->d_bu); dfree(st->d_kap); dfree(st->d_xi);
     dfree(st->d_alpha); dfree(st->d_pdamp); dfree(st->d_status);
     for (int a = 0; a < 3; ++a) dfree(st->d_w[a]);
@@ -666,12 +672,12 @@ AcousticFDTDState* create_impl(int Nx, int Ny, int Nz, float dx, float c, float 
         if (fused_supported(st)) {
             st->family = FDTD_KERNEL_FUSED;
         } else if (fused_geo_supported(st)) {
-            // walls, plate cells, fractions or an active plate: the single-step
-            // sweep with per-cell geometry codes (the two-step sweep with
-            // geometry runs when it is asked for)
+            // walls, plate cells, fractions or an active plate: the sweep with
+            // per-cell geometry codes, two steps per pass for the order-4
+            // stencil unless the single-step sweep is asked for
             st->family = FDTD_KERNEL_FUSED;
             st->fused_geo = 1;
-            st->geo_pair = 0;
+            st->geo_pair = o->kernel == FDTD_KERNEL_AUTO && fused4_geo_supported(st);
         } else if (o->kernel == FDTD_KERNEL_FUSED) {
             afdtd_set_error("fused kernel does not support this configuration");
             destroy_state(st);
@@ -1025,7 +1031,7 @@ SK_API int fdtd_set_face_fractions(AcousticFDTDState* st, const float* fx, const
         if (fused_geo_supported(st)) {
             st->family = FDTD_KERNEL_FUSED;
             st->fused_geo = 1;
-            st->geo_pair = st->kernel_opt == FDTD_KERNEL_FUSED2 && fused4_geo_supported(st);
+            st->geo_pair = st->kernel_opt != FDTD_KERNEL_FUSED && fused4_geo_supported(st);
         } else {
             st->family = FDTD_KERNEL_GENERIC;
         }

@@ -153,6 +153,8 @@ struct AcousticFDTDState {
     float* d_tw;            // [3][N] divergence-tap weights (fractions present)
     int* d_g2_items;        // work items: nfast without weighted taps, then nslow with
     int g2_nfast, g2_nslow, g2_items_seg, g2_gen, g2_items_gen;
+    hipStream_t g2_stream;  // the weighted-tap tiles' stream (default priority)
+    hipEvent_t g2_ev[2];
     float* d_ppre;
     float* d_pab;
     // Side job of the geometry sweep (set by a hooked caller, the co-evolver):
