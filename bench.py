@@ -541,8 +541,9 @@ def bench_body(a):
     24 cells of padding, 10-cell PML): the soundhole cut, cut-cell face
     fractions, an active plate and three Gaussian bridge sources
     (FDTDCoEvolver.from_outline), i.e. the geometry the bridge feeds the
-    solver, which runs the single-step sweep with cell geometry (--kernel
-    generic: the two-kernel path).  Timed: the steps after one bridge drive
+    solver, which runs the two-step sweep with cell geometry (--kernel fused:
+    the single-step sweep with cell geometry; --kernel generic: the
+    two-kernel path).  Timed: the steps after one bridge drive
     (the stencil), and the bridge's per-sample loop at 44.1 kHz (drive,
     steps_per_sample steps).  Algorithmic bytes per cell-update: the four
     fields read and written (32 B) and one geometry code byte (walls, closed,
@@ -573,8 +574,11 @@ def bench_body(a):
     bpc = 33
     achieved = cells * bpc / (ms_stencil * 1e-3) / 1e9
     traffic = None
+    pair = f.steps_per_sweep == 2
     if f.kernel_family == "fused" and a.tile_y in (0, 8) and a.seg_x == 0:
-        prof = os.path.join(ROOT, "profiles", "fused_geo_body_x4_395x549x148_order4_pml10.txt")
+        prof = os.path.join(ROOT, "profiles",
+                            ("fused4_geo" if pair else "fused_geo") +
+                            "_body_x4_395x549x148_order4_pml10.txt")
         if os.path.exists(prof):
             for ln in open(prof):
                 if ln.startswith("traffic_bytes_per_launch:"):
@@ -591,13 +595,15 @@ def bench_body(a):
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
             "data": "synthetic (two-bout outline, soundhole, bridge drive)",
             "config": {"workload": f"body-two-bout-x4-{info['Nx']}x{info['Ny']}x{info['Nz']}-dx4mm",
-                       "cells": cells, "kernel": f.kernel_family, "bridge_cells": nb,
-                       "steps_per_sample": ev.steps_per_sample},
+                       "cells": cells, "kernel": f.kernel_family + ("-two-step" if pair else ""),
+                       "bridge_cells": nb, "steps_per_sample": ev.steps_per_sample},
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                         "bytes_per_cell_update": bpc, "cell_updates_per_launch": cells,
+                         "bytes_per_cell_update": bpc,
+                         "cell_updates_per_launch": cells * f.steps_per_sweep,
                          "stencil_ms_per_step": ms_stencil, "event_ms_total": ms_events,
-                         "traffic_GBps": (traffic / (ms_stencil * 1e-3) / 1e9) if traffic else None},
+                         "traffic_GBps": (traffic / (f.steps_per_sweep * ms_stencil * 1e-3) / 1e9)
+                         if traffic else None},
             "bridge_loop": {"samples": ns, "samples_per_s": ns / wall_s,
                             "realtime_factor_44k1": ns / wall_s / 44100.0}}
     del ev, f
