@@ -1,6 +1,6 @@
 """Topology-driven (AMR) acoustic stepper on the GPU: the Python side.
 
-Mirrors the reference's AMR Python surface for gradient order 2:
+Mirrors the reference's AMR Python surface (gradient orders 2 and 8):
 
 * ``AcousticAMR`` — the pybind class ``_spectral_kernels.AcousticAMR``
   (csrc/bindings/pybind_kernels.cpp:6506-6603, :8877-8920): same constructor
@@ -81,7 +81,7 @@ def _p(a):
 
 
 class AcousticAMR:
-    """GPU drop-in for ``_spectral_kernels.AcousticAMR`` (gradient order 2)."""
+    """GPU drop-in for ``_spectral_kernels.AcousticAMR`` (gradient orders 2 and 8)."""
 
     def __init__(self, cell_centers, cell_volumes, open_volume_fraction, cell_types,
                  face_cell_neg, face_cell_pos, face_area, face_open_fraction, face_distance,
