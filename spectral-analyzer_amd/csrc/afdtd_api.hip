@@ -342,7 +342,7 @@ void destroy_state(AcousticFDTDState* st) {
     dfree(st->d_ct); dfree(st->d_fx); dfree(st->d_fy); dfree(st->d_fz);
     dfree(st->d_geo); dfree(st->d_vpl);
     dfree(st->d_g2); dfree(st->d_tw); dfree(st->d_ppre); dfree(st->d_pab);
-    dfree(st->d_g2_items);
+    dfree(st->d_g2_items); dfree(st->d_g2p);
     if (st->g2_stream) {
         (void)hipStreamSynchronize(st->g2_stream);
         (void)hipStreamDestroy(st->g2_stream);

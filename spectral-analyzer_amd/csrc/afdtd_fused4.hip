@@ -127,6 +127,7 @@ struct Pair4Args {
     const int* items;       // work-item list (nullptr: item = block)
     // cell geometry (GEOM build)
     const uint8_t* g2;      // per-cell pair code (G2_* bits)
+    const uint32_t* g2p;    // the pair codes of planes 4g .. 4g+3 of a cell in word g*sx + cell
     const float* tw;        // [3][N] divergence-tap weights: the face fraction where the
                             // tap is weighted, else exactly 1
     const float* vpl;       // plate velocity of step n per plate node (i*Ny + j)
@@ -334,6 +335,7 @@ __device__ __forceinline__ void fused4_tile(const Pair4Args& a, int ty, int tz, 
     // gn[s] / gmn: the codes of plane q+1 in flight.
     uint64_t gh[3] = {0, 0, 0};
     uint32_t gn[3] = {0, 0, 0}, gmc = 0, gmn = 0;
+    uint32_t gw[3] = {0, 0, 0}, gwm = 0;     // packed codes of planes q+1 .. q+4 (phase 0)
     float plv = 0.0f, vpv = 0.0f;
     // Plate loader lanes, for tiles whose z faces reach the plate's two face
     // columns: lanes 0..9 of wave w fetch step n+1's five terms of the rows
@@ -435,11 +437,21 @@ __device__ __forceinline__ void fused4_tile(const Pair4Args& a, int ty, int tz, 
             for (int s = 0; s < 3; ++s) pnl[s] = bld(r3, off[s]);
         }
         if constexpr (GEOM) {
-            // pair codes of plane q+1; step n+1's plate terms of plane q+1
-            const Rsrc g = rG(q + 1);
+            // pair codes of plane q+1 (one load per four planes: the segment
+            // start is a multiple of 4, so phase 0 has q+1 = 4g); step n+1's
+            // plate terms of plane q+1
+            if constexpr (P == 0) {
+                const int grp = (q + 1) >> 2;
+                const Rsrc g = (grp >= 0 && 4 * grp < Nx)
+                                   ? rsrc(a.g2p + (size_t)grp * sx, (uint32_t)sx * 4u)
+                                   : rsrc(a.g2p, 0u);
 #pragma unroll
-            for (int s = 0; s < 3; ++s) gn[s] = bld8(g, off[s] >> 2);
-            gmn = bld8(g, moff >> 2);
+                for (int s = 0; s < 3; ++s) gw[s] = __builtin_amdgcn_raw_buffer_load_b32(g, off[s], 0, 0);
+                gwm = __builtin_amdgcn_raw_buffer_load_b32(g, moff, 0, 0);
+            }
+#pragma unroll
+            for (int s = 0; s < 3; ++s) gn[s] = (gw[s] >> (8 * P)) & 0xFFu;
+            gmn = (gwm >> (8 * P)) & 0xFFu;
             if (plLane && q + 1 >= 0 && q + 1 < Nx)
                 plv = a.pre[(size_t)plComp * a.Nplate + (size_t)(q + 1) * Ny + j0 + plRow];
             if (vpLane && q + 1 >= 0 && q + 1 < Nx)
@@ -784,6 +796,22 @@ k_geo2_plate(int nact, const int* __restrict__ above, const int* __restrict__ be
     if (below[n] >= 0) g2[below[n]] = (uint8_t)(g2[below[n]] | f4::G2_PLNB);
 }
 
+// Pair codes packed four planes to a word: word g*sx + c holds the codes
+// of cell c in planes 4g .. 4g+3 (byte p - 4g; 0 past the grid).
+__global__ void __launch_bounds__(256)
+k_geo2_pack(int Nx, size_t sx, const uint8_t* __restrict__ g2, uint32_t* __restrict__ g2p) {
+    const size_t ng = (size_t)((Nx + 3) / 4);
+    const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= ng * sx) return;
+    const size_t g = t / sx, c = t % sx;
+    uint32_t w = 0;
+    for (int b = 0; b < 4; ++b) {
+        const size_t pl = 4 * g + b;
+        if (pl < (size_t)Nx) w |= (uint32_t)g2[pl * sx + c] << (8 * b);
+    }
+    g2p[t] = w;
+}
+
 // Divergence-tap weights: the face fraction where the single-step code
 // weights the tap, exactly 1 elsewhere (the fraction bit is set only where
 // the fraction is not exactly 1).
@@ -889,11 +917,21 @@ hipError_t launch_geo_pair_codes(AcousticFDTDState* st) {
         e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
-    if (st->nact <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_geo2_plate, dim3((st->nact + 255) / 256), dim3(256), 0, st->stream,
-                       st->nact, st->d_above, st->d_below, st->d_g2);
+    if (st->nact > 0) {
+        hipLaunchKernelGGL(k_geo2_plate, dim3((st->nact + 255) / 256), dim3(256), 0, st->stream,
+                           st->nact, st->d_above, st->d_below, st->d_g2);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    const size_t sx = (size_t)st->Ny * st->Nz, npk = (size_t)((st->Nx + 3) / 4) * sx;
+    if (!st->d_g2p) {
+        e = hipMalloc(&st->d_g2p, sizeof(uint32_t) * npk);
+        if (e != hipSuccess) { st->d_g2p = nullptr; return e; }
+    }
+    hipLaunchKernelGGL(k_geo2_pack, dim3((unsigned)((npk + 255) / 256)), dim3(256), 0, st->stream,
+                       st->Nx, sx, st->d_g2, st->d_g2p);
     e = hipGetLastError();
-    if (e != hipSuccess) return e;
+    if (e != hipSuccess || st->nact <= 0) return e;
     if (!st->d_ppre) {
         e = hipMalloc(&st->d_ppre, 5 * sizeof(float) * st->Nplate);
         if (e != hipSuccess) { st->d_ppre = nullptr; return e; }
@@ -1087,6 +1125,7 @@ hipError_t launch_fused4_geo_pair(AcousticFDTDState* st, int step, int trace_row
     }
     Pair4Args a = pair4_args(st, step);
     a.g2 = st->d_g2;
+    a.g2p = st->d_g2p;
     a.tw = st->d_tw;
     a.vpl = st->d_vpl;
     a.pre = st->d_ppre;
@@ -1096,12 +1135,13 @@ hipError_t launch_fused4_geo_pair(AcousticFDTDState* st, int step, int trace_row
     a.plate_on = nact > 0;
     a.pa = st->plate_a;
     a.inv_dt = 1.0f / st->dt;
-    if (st->fused_seg_x <= 0) {
-        // 32-plane segments: more work items per launch (395 x 549 x 148
-        // body: 0.46 ms per step against 0.62 at 64 and 0.49 at 16)
-        a.seg = std::min(32, st->Nx);
-        a.nseg = (st->Nx + a.seg - 1) / a.seg;
-    }
+    // 32-plane segments by default: more work items per launch (395 x 549 x
+    // 148 body: 0.46 ms per step against 0.62 at 64 and 0.49 at 16); a
+    // segment is a whole number of 4-plane code words
+    a.seg = st->fused_seg_x > 0 ? st->fused_seg_x : 32;
+    a.seg = (a.seg + 3) & ~3;
+    if (a.seg >= st->Nx) a.seg = st->Nx;
+    a.nseg = (st->Nx + a.seg - 1) / a.seg;
     AFDTD_RET(geo_item_lists(st, a));
     // The tiles with weighted taps run on a second stream beside the others
     // (disjoint outputs, shared read-only inputs), so that neither launch

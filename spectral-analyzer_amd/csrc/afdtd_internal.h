@@ -150,6 +150,7 @@ struct AcousticFDTDState {
     int geo_pair;
     int kernel_opt;         // the kernel family the handle was created with
     uint8_t* d_g2;
+    uint32_t* d_g2p;        // the pair codes, four planes per word
     float* d_tw;            // [3][N] divergence-tap weights (fractions present)
     int* d_g2_items;        // work items: nfast without weighted taps, then nslow with
     int g2_nfast, g2_nslow, g2_items_seg, g2_gen, g2_items_gen;
