@@ -2,8 +2,8 @@
 build: walls, closed and fractional faces, an active plate whose second
 update runs inside the sweep) against the float32 oracle and the single-step
 geometry sweep, bit for bit.  Cases: ragged grids, PML 0 to 10, the plate's
-two face columns inside a tile, on a tile seam (z = 63 | 64) and one column
-past it, fractions 0, -0, 1 and in (0, 1) on open and plate faces, point
+two face columns inside a tile, on a tile seam (z = 63 | 64), one column
+past it and on either z side of the grid, fractions 0, -0, 1 and in (0, 1) on open and plate faces, point
 sources and receivers beside walls, and a plate guard that trips on the
 second step of a pair (the pair's first step is kept)."""
 import numpy as np
@@ -26,7 +26,10 @@ def _make(oracle_mod, shape, n_pml, seed, walls, plate, fracs, kernel, piz=None)
     if fr is not None:
         if piz != piz0:        # the scene's plate-face fractions sit at piz0: move them
             fz = fr[2].reshape(Nx, Ny, Nz - 1)
-            fz[3, 4, piz - 1], fz[4, 5, piz], fz[5, 6, piz - 1], fz[6, 6, piz] = 0.0, -0.0, 0.5, 0.0
+            for (i, j, k), v in (((3, 4, piz - 1), 0.0), ((4, 5, piz), -0.0),
+                                 ((5, 6, piz - 1), 0.5), ((6, 6, piz), 0.0)):
+                if 0 <= k < Nz - 1:
+                    fz[i, j, k] = v
         if g is not None:
             g.set_face_fractions(*fr)
         o.set_face_fractions(*fr)
@@ -52,6 +55,8 @@ CASES = [
     ((19, 30, 128), 10, 0.02, True, True, 64),     # plate faces z = 63 | 64: a tile seam
     ((18, 27, 130), 4, 0.03, True, True, 65),      # faces 64, 65: next tile's first columns
     ((16, 50, 148), 10, 0.0, True, True, None),    # 148 = 64 + 64 + 20 columns
+    ((14, 26, 40), 2, 0.02, True, True, 0),        # plate on the z = 0 side: no face below
+    ((14, 26, 40), 0, 0.02, True, True, 39),       # on the far side: no face above
 ]
 
 
