@@ -1135,10 +1135,11 @@ hipError_t launch_fused4_geo_pair(AcousticFDTDState* st, int step, int trace_row
     a.plate_on = nact > 0;
     a.pa = st->plate_a;
     a.inv_dt = 1.0f / st->dt;
-    // 32-plane segments by default: more work items per launch (395 x 549 x
-    // 148 body: 0.46 ms per step against 0.62 at 64 and 0.49 at 16); a
-    // segment is a whole number of 4-plane code words
-    a.seg = st->fused_seg_x > 0 ? st->fused_seg_x : 32;
+    // 24-plane segments by default: more work items per launch than the
+    // plain pair's 64 (395 x 549 x 148 body, ms per step at segments of
+    // 16 / 24 / 32 / 40 / 48 / 64 planes: 0.432 / 0.418 / 0.431 / 0.424 /
+    // 0.448 / 0.472); a segment is a whole number of 4-plane code words
+    a.seg = st->fused_seg_x > 0 ? st->fused_seg_x : 24;
     a.seg = (a.seg + 3) & ~3;
     if (a.seg >= st->Nx) a.seg = st->Nx;
     a.nseg = (st->Nx + a.seg - 1) / a.seg;
