@@ -8,8 +8,8 @@ density and bulk modulus, locally reacting walls with per-side reflection
 coefficients, 4 sources and 64 receivers (synthetic, seeded).
 
 The default line also carries `forms`: the batched, precision and adjoint
-forms and the reference's own order-4 scheme, each measured in the same run
-at a reduced step count (--no-forms skips them).
+forms, the reference's own order-4 scheme and the voxelised body, each
+measured in the same run at a reduced step count (--no-forms skips them).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config NAME]
 """
@@ -852,8 +852,10 @@ def measure_forms(steps=40):
     """The three forms beside the plain drop-in and the reference's own
     scheme, each at a reduced step count in the same run as the headline
     line: batched rooms against per-room launches, 16-bit storage against
-    FP32 on the precision room, the adjoint's reverse rate, and the order-4
-    PML room.  A form that fails reports its error instead of a number."""
+    FP32 on the precision room, the adjoint's reverse rate, the order-4
+    PML room, and the voxelised body on the two-step and the single-step
+    sweeps with geometry.  A form that fails reports its error instead of a
+    number."""
     from afdtd import RoomBatch, AcousticFDTD, workloads as W
     forms = {}
 
@@ -951,10 +953,35 @@ def measure_forms(steps=40):
                 "forward_value": N * n / (t1 - t0), "roofline_frac":
                 N * n / (t2 - t1) * 104 / 1e9 / HBM_PEAK_GBS}
 
+    def voxelised_body():
+        from afdtd.bridge import FDTDCoEvolver
+        from afdtd.geometry import two_bout_outline
+        poly = (two_bout_outline(256) * 4.0).astype(np.float32)
+        out = {"workload": "body-two-bout-x4-395x549x148-dx4mm", "steps": steps,
+               "unit": "cell-updates/s"}
+        for kern in ("auto", "fused"):
+            ev = FDTDCoEvolver.from_outline(poly, 0.4, [(-0.08, -0.52), (0.0, -0.52), (0.08, -0.52)],
+                                            soundhole=(0.0, 0.24, 0.18), dx=0.004, pad_cells=24,
+                                            n_pml=10, kernel=kern)
+            f = ev.fdtd
+            cells = ev.info["Nx"] * ev.info["Ny"] * ev.info["Nz"]
+            f.inject_bridge_drive(np.full(len(ev.bridge_cells), 1e-3, np.float32), 1.0)
+            f.step(4)
+            t0 = time.perf_counter()
+            _, ms = f.step_timed(steps)
+            wall = time.perf_counter() - t0
+            name = "two_step" if f.steps_per_sweep == 2 else "single_step"
+            out[name] = cells * steps / wall
+            out[name + "_roofline_frac"] = cells * 33 / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS
+            del ev, f
+        out["two_step_vs_single_step"] = out["two_step"] / out["single_step"]
+        return out
+
     guard("reference_scheme_order4", reference_scheme)
     guard("batched", batched)
     guard("precision", precision)
     guard("adjoint", adjoint)
+    guard("voxelised_body", voxelised_body)
     return forms
 
 
