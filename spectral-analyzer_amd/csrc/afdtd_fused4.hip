@@ -312,7 +312,10 @@ __device__ __forceinline__ void fused4_tile(const Pair4Args& a, int ty, int tz, 
         for (int u = 0; u < 4; ++u) bx[s][u] = 0.0f;
     // Damping multipliers of the slot's faces and cell on planes away from
     // the x-sides of the layer (the x depth is 0 there, so they depend on the
-    // slot alone; exactly 1 for cells outside the layer).
+    // slot alone; exactly 1 for cells outside the layer).  The CUT build
+    // evaluates them per plane from the LDS tables instead (the same values:
+    // with x depth 0 the per-plane expression is this one), which frees
+    // twelve registers for its weighted-tap path.
     float mX[3], mY[3], mZ[3], mP[3], mM = 1.0f;
 #pragma unroll
     for (int s = 0; s < 3; ++s) {
@@ -474,7 +477,7 @@ __device__ __forceinline__ void fused4_tile(const Pair4Args& a, int ty, int tz, 
                 const float pm = pn[s][M0], pl = pn[s][M1];
                 // x-face between planes q and q+1
                 float nv = ovx[s] - vcoef * grad4(pm, pl, pn[s][M2], pn[s][M3], xin, C1, C2);
-                if (PML) nv = nv * (xd ? damp(max(dq, D0(s)), max(dq1, D0(s))) : mX[s]);
+                if (PML) nv = nv * ((CUT || xd) ? damp(max(dq, D0(s)), max(dq1, D0(s))) : mX[s]);
                 const uint32_t g0 = GEOM ? (uint32_t)gh[s] & 0xFFu : 0u;   // plane q
                 float nxv = (fvx && cellv[s]) ? nv : 0.0f;
                 if (GEOM && (g0 & G_CLOSED)) nxv = 0.0f;
@@ -482,13 +485,13 @@ __device__ __forceinline__ void fused4_tile(const Pair4Args& a, int ty, int tz, 
                 // y-face between rows e and e+1
                 nv = ovy[s] - vcoef * grad4(Pq[gix(e - 1, c)], pl, Pq[gix(e + 1, c)],
                                             Pq[gix(e + 2, c)], yin[s], C1, C2);
-                if (PML) nv = nv * (xd ? damp(max(dq, D0(s)), max(dq, DY(s))) : mY[s]);
+                if (PML) nv = nv * ((CUT || xd) ? damp(max(dq, D0(s)), max(dq, DY(s))) : mY[s]);
                 float ny = (qv && vyv[s]) ? nv : 0.0f;
                 if (GEOM && (g0 & (G_CLOSED << 1))) ny = 0.0f;
                 // z-face between columns c and c+1
                 nv = ovz[s] - vcoef * grad4(Pq[gix(e, c - 1)], pl, Pq[gix(e, c + 1)],
                                             Pq[gix(e, c + 2)], zin[s], C1, C2);
-                if (PML) nv = nv * (xd ? damp(max(dq, D0(s)), max(dq, DZ(s))) : mZ[s]);
+                if (PML) nv = nv * ((CUT || xd) ? damp(max(dq, D0(s)), max(dq, DZ(s))) : mZ[s]);
                 float nz = (qv && vzv[s]) ? nv : 0.0f;
                 if constexpr (GEOM) {
                     if (g0 & (G_CLOSED << 2)) nz = 0.0f;
@@ -558,7 +561,7 @@ __device__ __forceinline__ void fused4_tile(const Pair4Args& a, int ty, int tz, 
                 if (CUT && (g1 & G2_TAPCUT))
                     dv = cut_dv(qa, j0 + e, k0 + c, xp, xm, xpp, xmm, yp, ym, ypp, ymm, zp, zm, zpp, zmm);
                 float out = pn[s][M0] - kap0 * dv;
-                if (PML) out = out * (xd ? sPd[max(dqa, D0(s))] : mP[s]);
+                if (PML) out = out * ((CUT || xd) ? sPd[max(dqa, D0(s))] : mP[s]);
                 if (GEOM && (g1 & G_SOLID)) out = pn[s][M0];          // walls keep P
                 pnew[s] = (qav && cellv[s]) ? out : 0.0f;
             }
@@ -621,7 +624,7 @@ __device__ __forceinline__ void fused4_tile(const Pair4Args& a, int ty, int tz, 
             for (int s = 0; s < 2; ++s) {   // x-face between planes qb and qb+1
                 const int ix = p1ix(se[s], sc[s]);
                 float nv = ax[s][N3] - vcoef * grad4(P1m[ix], P1b[ix], P1r[ix], P1p[ix], xinb, C1, C2);
-                if (PML) nv = nv * (xd ? damp(max(dqb, D0(s)), max(dqb1, D0(s))) : mX[s]);
+                if (PML) nv = nv * ((CUT || xd) ? damp(max(dqb, D0(s)), max(dqb1, D0(s))) : mX[s]);
                 nv = (fvb && cellv[s]) ? nv : 0.0f;
                 if (GEOM && ((uint32_t)(gh[s] >> 24) & G_CLOSED)) nv = 0.0f;
                 bx[s][N0] = nv;
@@ -639,7 +642,7 @@ __device__ __forceinline__ void fused4_tile(const Pair4Args& a, int ty, int tz, 
                     float nv = Ayo[ayix(e, c)] - vcoef * grad4(P1b[p1ix(e - 1, c)], P1b[p1ix(e, c)],
                                                                P1b[p1ix(e + 1, c)], P1b[p1ix(e + 2, c)],
                                                                yin[s], C1, C2);
-                    if (PML) nv = nv * (xd ? damp(max(dqb, D0(s)), max(dqb, DY(s))) : mY[s]);
+                    if (PML) nv = nv * ((CUT || xd) ? damp(max(dqb, D0(s)), max(dqb, DY(s))) : mY[s]);
                     nv = (qbv && vyv[s]) ? nv : 0.0f;
                     if (GEOM && ((uint32_t)(gh[s] >> 24) & (G_CLOSED << 1))) nv = 0.0f;
                     Byw[byix(e, c)] = nv;
@@ -653,7 +656,7 @@ __device__ __forceinline__ void fused4_tile(const Pair4Args& a, int ty, int tz, 
                     float nv = Azo[azix(e, c)] - vcoef * grad4(P1b[p1ix(e, c - 1)], P1b[p1ix(e, c)],
                                                                P1b[p1ix(e, c + 1)], P1b[p1ix(e, c + 2)],
                                                                zin[s], C1, C2);
-                    if (PML) nv = nv * (xd ? damp(max(dqb, D0(s)), max(dqb, DZ(s))) : mZ[s]);
+                    if (PML) nv = nv * ((CUT || xd) ? damp(max(dqb, D0(s)), max(dqb, DZ(s))) : mZ[s]);
                     nv = (qbv && vzv[s]) ? nv : 0.0f;
                     if constexpr (GEOM) {
                         const uint32_t g3 = (s < 2 || ringBz) ? (uint32_t)(gh[s] >> 24) & 0xFFu : 0u;
@@ -700,7 +703,7 @@ __device__ __forceinline__ void fused4_tile(const Pair4Args& a, int ty, int tz, 
                 if (CUT && (g4 & G2_TAPCUT))
                     dv = cut_dv(qc, j0 + e, k0 + c, xp, xm, xpp, xmm, yp, ym, ypp, ymm, zp, zm, zpp, zmm);
                 float out = P1c[p1ix(e, c)] - kap0 * dv;
-                if (PML) out = out * (xd ? sPd[max(dqc, D0(s))] : mP[s]);
+                if (PML) out = out * ((CUT || xd) ? sPd[max(dqc, D0(s))] : mP[s]);
                 if (!GEOM || !(g4 & G_SOLID)) bst(R, off[s], out);   // walls keep P
             }
         }
